@@ -1,0 +1,198 @@
+/*
+ * suetes_sisl.h -- C ABI of the MI355X-native SISL dynamical-core kernels.
+ *
+ * Every entry point takes plain device pointers, sizes and a HIP stream
+ * (passed as void*), allocates nothing (all buffers are caller-owned) and
+ * returns 0 or a negative SislStatus.  Suffix _f64 / _f32 is the element
+ * type of every array argument; Krylov inner products, norms and the
+ * Hessenberg least squares are FP64 in both variants.
+ *
+ * Array layout ("internal layout"): a field of reference shape (NX, NY, NZ)
+ * is stored with x fastest, i.e. element (i, j, k) at ((k*NY)+j)*NX+i.  The
+ * reference stores z fastest (suetes/regional3d/geometry.py:221-224); the
+ * host layer permutes once at the trajectory boundary.  Staggered fields:
+ * u (nx+1, ny, nz), v (nx, ny+1, nz), w / eta_dot (nx, ny, nz+1),
+ * mass points (nx, ny, nz).  A "Krylov vector" is the packed concatenation
+ * u | v | w | pi | eta_dot in that order, each in the internal layout, in
+ * the solver's scaled units (pi x 1000, eta_dot x dz_w_full; reference
+ * suetes/regional3d/steppers.py:413-424,445-461).
+ */
+#ifndef SUETES_SISL_H
+#define SUETES_SISL_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum SislStatus {
+  SISL_OK = 0,
+  SISL_ERR_ARG = -1,      /* null pointer / bad size / bad option          */
+  SISL_ERR_LAUNCH = -2,   /* a HIP launch or runtime call failed           */
+  SISL_ERR_NOT_FINITE = -3/* GMRES produced a non-finite residual          */
+};
+
+/* Grid descriptor shared by every entry point. */
+typedef struct SislGrid {
+  int32_t nx, ny, nz;          /* mass-point counts                           */
+  int32_t periodic_x, periodic_y;
+} SislGrid;
+
+/*
+ * Coefficients of the linear operator A (euler.py:374-439 through
+ * steppers.py:439-461) and of the column preconditioner M
+ * (steppers.py:153-253), precomputed once per stepper by the host layer.
+ * Pointers are device pointers of the entry point's element type.
+ * Shapes: [u] (nx+1,ny,nz) [v] (nx,ny+1,nz) [w] (nx,ny,nz+1) [m] (nx,ny,nz);
+ * 2-D: [u2] (nx+1,ny) [v2] (nx,ny+1) [m2] (nx,ny).
+ */
+typedef struct SislOperator {
+  SislGrid g;
+  double alpha;        /* semi-implicit off-centring                         */
+  /* A: pressure-gradient, Coriolis, divergence, kinematic couplings */
+  const void *cua;     /* [u]  alpha dt cp th_v_u / 1000                     */
+  const void *zxu;     /* [u]  dz/dxi at u (includes m_u)                    */
+  const void *cx;      /* [u]  rho_u th_v_u dz_u / m_u                       */
+  const void *cva;     /* [v]  alpha dt cp th_v_v / 1000                     */
+  const void *zev;     /* [v]  dz/deta at v (includes m_v)                   */
+  const void *cy;      /* [v]  rho_v th_v_v dz_v / m_v                       */
+  const void *dw;      /* [w]  1 + dt tau                                    */
+  const void *cw;      /* [w]  alpha dt cp th_v_w / dz_w / 1000              */
+  const void *cz;      /* [w]  rho_w th_v_w                                  */
+  const void *kx;      /* [w]  m_w z_xi_w                                    */
+  const void *ky;      /* [w]  m_w z_eta_w                                   */
+  const void *rdzw;    /* [w]  1 / dz_w_full                                 */
+  const void *cpm;     /* [m]  1000 alpha dt C_pi / dz_m                     */
+  const void *mudx;    /* [u2] m_u / dx                                      */
+  const void *mvdy;    /* [v2] m_v / dy                                      */
+  const void *afu;     /* [u2] alpha dt f_u                                  */
+  const void *afv;     /* [v2] alpha dt f_v                                  */
+  const void *mmx;     /* [m2] m_m^2 / dx                                    */
+  const void *mmy;     /* [m2] m_m^2 / dy                                    */
+  /* M: column Helmholtz solve */
+  const void *idw;     /* [w]  1 / (1 + dt tau)                              */
+  const void *gw;      /* [w]  alpha dt cp th_v_w / ((1 + dt tau) dz_w)      */
+  const void *lpi;     /* [m]  alpha dt C_pi / dz_m                          */
+  const void *lower;   /* [m]  sub-diagonal of the column matrix             */
+  const void *invden;  /* [m]  1 / Thomas pivot                              */
+  const void *cprime;  /* [m]  Thomas super-diagonal ratio c'                */
+  const void *dzw0;    /* [m2] dz_w_full at the surface w level              */
+  const void *dzwn;    /* [m2] dz_w_full at the top w level                  */
+} SislOperator;
+
+/* Number of elements in a packed Krylov vector for grid g. */
+int64_t sisl_krylov_size(const SislGrid *g);
+
+/* ---- (a) semi-Lagrangian interpolation ------------------------------- */
+
+/* tensor_product_interp_3d (suetes/regional3d/operators.py:146-209):
+ * Catmull-Rom tricubic of `field` (dims fnx,fny,fnz) at index-space points
+ * coords[0..2][n] (x, y, z components, each n = fnx*fny*fnz long, one per
+ * output point in the internal layout), optional 8-corner clip. */
+int sisl_tricubic_f64(int32_t fnx, int32_t fny, int32_t fnz, const double *field,
+                      const double *coords, double *out, int32_t use_limiter, void *stream);
+int sisl_tricubic_f32(int32_t fnx, int32_t fny, int32_t fnz, const float *field,
+                      const float *coords, float *out, int32_t use_limiter, void *stream);
+
+/* Workspace of the deterministic tricubic adjoint, int32 device arrays:
+ * key[n], perm[n], start[n+1], count[n], block_sums[sisl_scan_blocks(n)]. */
+int32_t sisl_scan_blocks(int64_t n);
+/* Build the destination-sorted segment map of one coordinate set
+ * (reusable by every adjoint call that shares the coordinates). */
+int sisl_tricubic_adj_plan_f64(int32_t fnx, int32_t fny, int32_t fnz, const double *coords,
+                               int32_t *key, int32_t *perm, int32_t *start, int32_t *count,
+                               int32_t *block_sums, void *stream);
+int sisl_tricubic_adj_plan_f32(int32_t fnx, int32_t fny, int32_t fnz, const float *coords,
+                               int32_t *key, int32_t *perm, int32_t *start, int32_t *count,
+                               int32_t *block_sums, void *stream);
+/* Transpose of sisl_tricubic (no limiter): gfield = T^T gout, accumulated
+ * per destination in a fixed order (bit-reproducible). gfield is overwritten. */
+int sisl_tricubic_adj_f64(int32_t fnx, int32_t fny, int32_t fnz, const double *coords,
+                          const int32_t *perm, const int32_t *start, const double *gout,
+                          double *gfield, void *stream);
+int sisl_tricubic_adj_f32(int32_t fnx, int32_t fny, int32_t fnz, const float *coords,
+                          const int32_t *perm, const int32_t *start, const float *gout,
+                          float *gfield, void *stream);
+
+/* _get_index_velocities + compute_departure_indices
+ * (suetes/regional3d/steppers.py:276-360): departure coordinates at
+ * staggering loc (0=m, 1=u, 2=v, 3=w) from trajectory winds u, v, eta_dot.
+ * m2 is the 2-D map factor at loc; coords_out is 3 x (loc size). */
+int sisl_departure_f64(const SislGrid *g, int32_t loc, const double *u, const double *v,
+                       const double *eta, const double *m2, double dx, double dy, double dt,
+                       int32_t iterations, double *scratch3, double *coords_out, void *stream);
+int sisl_departure_f32(const SislGrid *g, int32_t loc, const float *u, const float *v,
+                       const float *eta, const float *m2, double dx, double dy, double dt,
+                       int32_t iterations, float *scratch3, float *coords_out, void *stream);
+
+/* ---- (b) operator apply ---------------------------------------------- */
+
+/* y = A x on packed Krylov vectors (Euler3D.linear_operator in the solver's
+ * scaled units, suetes/regional3d/steppers.py:439-461). */
+int sisl_helmholtz_apply_f64(const SislOperator *op, const double *x, double *y, void *stream);
+int sisl_helmholtz_apply_f32(const SislOperator *op, const float *x, float *y, void *stream);
+/* y = A^T x (the transpose used by the GMRES adjoint solve,
+ * steppers.py:473-481). scratch: 3 mass-point arrays. */
+int sisl_helmholtz_apply_T_f64(const SislOperator *op, const double *x, double *y,
+                               double *scratch, void *stream);
+int sisl_helmholtz_apply_T_f32(const SislOperator *op, const float *x, float *y,
+                               float *scratch, void *stream);
+
+/* ---- (c) column preconditioner ----------------------------------------- */
+
+/* z = M r (VerticalPreconditioner.__call__, steppers.py:187-253).
+ * In place (z == r) is allowed. */
+int sisl_precond_apply_f64(const SislOperator *op, const double *r, double *z, void *stream);
+int sisl_precond_apply_f32(const SislOperator *op, const float *r, float *z, void *stream);
+/* z = M^T r (jax.linear_transpose(M_fn), steppers.py:474). scratch: 2 mass
+ * arrays. z must not alias r. */
+int sisl_precond_apply_T_f64(const SislOperator *op, const double *r, double *z,
+                             double *scratch, void *stream);
+int sisl_precond_apply_T_f32(const SislOperator *op, const float *r, float *z,
+                             float *scratch, void *stream);
+
+/* ---- GMRES (jax.scipy.sparse.linalg.gmres, batched, steppers.py:465-481) */
+
+typedef struct SislGmresParams {
+  double tol;              /* relative tolerance                              */
+  double atol;             /* absolute tolerance                              */
+  int32_t restart;         /* Krylov dimension per cycle (<= 64)              */
+  int32_t maxiter;         /* maximum restart cycles                          */
+  int32_t transpose;       /* 0: solve A x = b with M;  1: A^T x = b with M^T */
+  int32_t early_exit;      /* 0: every cycle runs `restart` Arnoldi steps
+                              (reference semantics); 1: stop a cycle once the
+                              Givens residual estimate meets the target       */
+  double inner_rtol;       /* mixed solver: inner FP32 cycle target (relative)*/
+  int32_t max_outer;       /* mixed solver: refinement iterations             */
+} SislGmresParams;
+
+typedef struct SislGmresStats {
+  int32_t cycles;          /* restart cycles (outer iterations for mixed)     */
+  int32_t arnoldi_steps;   /* total Arnoldi steps executed                    */
+  double residual;         /* final ||M(b - A x)||                            */
+  double ptol;             /* stopping threshold                              */
+  double b_norm;
+} SislGmresStats;
+
+/* Bytes of device workspace the solvers need (caller allocates, 256-B aligned). */
+int64_t sisl_gmres_workspace_bytes_f64(const SislGrid *g, int32_t restart);
+int64_t sisl_gmres_workspace_bytes_mixed(const SislGrid *g, int32_t restart);
+
+/* FP64 restarted left-preconditioned GMRES. x: in = x0, out = solution. */
+int sisl_gmres_f64(const SislOperator *op64, const SislGmresParams *p, const double *b,
+                   double *x, void *workspace, SislGmresStats *stats, void *stream);
+/* Mixed precision: FP64 outer residual refinement (A, M, norms in FP64),
+ * FP32 inner GMRES (A, M, basis in FP32; inner products in FP64). */
+int sisl_gmres_mixed(const SislOperator *op64, const SislOperator *op32,
+                     const SislGmresParams *p, const double *b, double *x, void *workspace,
+                     SislGmresStats *stats, void *stream);
+
+/* Packed-vector helpers used by the host layer. out = sum over d of x*y (FP64). */
+int sisl_dot_f64(const double *x, const double *y, int64_t n, double *out_dev, double *partials,
+                 void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SUETES_SISL_H */

@@ -1,0 +1,116 @@
+"""ctypes binding of libsuetes_sisl.so (the C ABI declared in include/suetes_sisl.h).
+
+The library is loaded lazily on first use; a missing library or a call on a
+non-HIP tensor raises immediately.  There is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import torch
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libsuetes_sisl.so")
+
+ERRORS = {-1: "bad argument", -2: "HIP launch failed", -3: "non-finite residual"}
+
+EXPORTED = [
+    "sisl_krylov_size", "sisl_tricubic_f64", "sisl_tricubic_f32", "sisl_scan_blocks",
+    "sisl_tricubic_adj_plan_f64", "sisl_tricubic_adj_plan_f32", "sisl_tricubic_adj_f64",
+    "sisl_tricubic_adj_f32", "sisl_departure_f64", "sisl_departure_f32",
+    "sisl_helmholtz_apply_f64", "sisl_helmholtz_apply_f32", "sisl_helmholtz_apply_T_f64",
+    "sisl_helmholtz_apply_T_f32", "sisl_precond_apply_f64", "sisl_precond_apply_f32",
+    "sisl_precond_apply_T_f64", "sisl_precond_apply_T_f32", "sisl_gmres_workspace_bytes_f64",
+    "sisl_gmres_workspace_bytes_mixed", "sisl_gmres_f64", "sisl_gmres_mixed", "sisl_dot_f64",
+]
+
+
+class SislGrid(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+                ("periodic_x", C.c_int32), ("periodic_y", C.c_int32)]
+
+
+OPERATOR_FIELDS = ["cua", "zxu", "cx", "cva", "zev", "cy", "dw", "cw", "cz", "kx", "ky", "rdzw",
+                   "cpm", "mudx", "mvdy", "afu", "afv", "mmx", "mmy", "idw", "gw", "lpi", "lower",
+                   "invden", "cprime", "dzw0", "dzwn"]
+
+
+class SislOperator(C.Structure):
+    _fields_ = [("g", SislGrid), ("alpha", C.c_double)] + [(f, C.c_void_p) for f in OPERATOR_FIELDS]
+
+
+class SislGmresParams(C.Structure):
+    _fields_ = [("tol", C.c_double), ("atol", C.c_double), ("restart", C.c_int32),
+                ("maxiter", C.c_int32), ("transpose", C.c_int32), ("early_exit", C.c_int32),
+                ("inner_rtol", C.c_double), ("max_outer", C.c_int32)]
+
+
+class SislGmresStats(C.Structure):
+    _fields_ = [("cycles", C.c_int32), ("arnoldi_steps", C.c_int32), ("residual", C.c_double),
+                ("ptol", C.c_double), ("b_norm", C.c_double)]
+
+
+_lib = None
+
+
+def lib():
+    """Load the library (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build() "
+                               "(python -m suetes_amd._build) first")
+        L = C.CDLL(LIB_PATH)
+        vp, i32, i64, d = C.c_void_p, C.c_int32, C.c_int64, C.c_double
+        sig = {
+            "sisl_krylov_size": (i64, [C.POINTER(SislGrid)]),
+            "sisl_scan_blocks": (i32, [i64]),
+            "sisl_gmres_workspace_bytes_f64": (i64, [C.POINTER(SislGrid), i32]),
+            "sisl_gmres_workspace_bytes_mixed": (i64, [C.POINTER(SislGrid), i32]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype, fn.argtypes = res, args
+        for p in ("f64", "f32"):
+            getattr(L, f"sisl_tricubic_{p}").argtypes = [i32, i32, i32, vp, vp, vp, i32, vp]
+            getattr(L, f"sisl_tricubic_adj_plan_{p}").argtypes = [i32, i32, i32, vp, vp, vp, vp, vp, vp, vp]
+            getattr(L, f"sisl_tricubic_adj_{p}").argtypes = [i32, i32, i32, vp, vp, vp, vp, vp, vp]
+            getattr(L, f"sisl_departure_{p}").argtypes = [C.POINTER(SislGrid), i32, vp, vp, vp, vp,
+                                                          d, d, d, i32, vp, vp, vp]
+            getattr(L, f"sisl_helmholtz_apply_{p}").argtypes = [C.POINTER(SislOperator), vp, vp, vp]
+            getattr(L, f"sisl_helmholtz_apply_T_{p}").argtypes = [C.POINTER(SislOperator), vp, vp, vp, vp]
+            getattr(L, f"sisl_precond_apply_{p}").argtypes = [C.POINTER(SislOperator), vp, vp, vp]
+            getattr(L, f"sisl_precond_apply_T_{p}").argtypes = [C.POINTER(SislOperator), vp, vp, vp, vp]
+        L.sisl_gmres_f64.argtypes = [C.POINTER(SislOperator), C.POINTER(SislGmresParams), vp, vp, vp,
+                                     C.POINTER(SislGmresStats), vp]
+        L.sisl_gmres_mixed.argtypes = [C.POINTER(SislOperator), C.POINTER(SislOperator),
+                                       C.POINTER(SislGmresParams), vp, vp, vp,
+                                       C.POINTER(SislGmresStats), vp]
+        L.sisl_dot_f64.argtypes = [vp, vp, i64, vp, vp, vp]
+        _lib = L
+    return _lib
+
+
+def check(rc: int, what: str):
+    if rc != 0:
+        raise RuntimeError(f"{what} failed: {ERRORS.get(rc, rc)}")
+
+
+def stream_ptr(device=None) -> int:
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+def require_device(*tensors):
+    """The kernels run only on HIP device memory."""
+    for t in tensors:
+        if t is None:
+            continue
+        if not t.is_cuda:
+            raise RuntimeError("suetes_amd kernels need tensors on a HIP device (cuda:N); "
+                               "there is no CPU path")
+        if not t.is_contiguous():
+            raise RuntimeError("suetes_amd kernels need contiguous tensors in the internal layout")
+
+
+def ptr(t) -> int:
+    return t.data_ptr() if t is not None else 0

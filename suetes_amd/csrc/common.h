@@ -1,0 +1,105 @@
+// Shared device helpers for the SISL kernels (gfx950, wave64).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "suetes_sisl.h"
+
+#define SISL_CHECK_LAUNCH()                                        \
+  do {                                                             \
+    hipError_t e_ = hipGetLastError();                             \
+    if (e_ != hipSuccess) return SISL_ERR_LAUNCH;                  \
+  } while (0)
+
+namespace sisl {
+
+constexpr int kBlock = 256;
+
+inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+inline unsigned grid_for(int64_t n, int block = kBlock) {
+  int64_t b = (n + block - 1) / block;
+  return (unsigned)(b < 1 ? 1 : b);
+}
+
+// Field extents in the internal (x-fastest) layout.
+struct Dims {
+  int nx, ny, nz;
+  __host__ __device__ int64_t n() const { return (int64_t)nx * ny * nz; }
+  __host__ __device__ int64_t at(int i, int j, int k) const {
+    return ((int64_t)k * ny + j) * nx + i;
+  }
+};
+
+// Staggered extents of the C grid.
+struct Stag {
+  Dims u, v, w, m;
+  int64_t ou, ov, ow, om, oe, n;  // offsets of the packed Krylov vector
+  int nx, ny, nz, px, py;
+  __host__ __device__ static Stag make(const SislGrid &g) {
+    Stag s;
+    s.nx = g.nx; s.ny = g.ny; s.nz = g.nz; s.px = g.periodic_x; s.py = g.periodic_y;
+    s.u = Dims{g.nx + 1, g.ny, g.nz};
+    s.v = Dims{g.nx, g.ny + 1, g.nz};
+    s.w = Dims{g.nx, g.ny, g.nz + 1};
+    s.m = Dims{g.nx, g.ny, g.nz};
+    s.ou = 0;
+    s.ov = s.ou + s.u.n();
+    s.ow = s.ov + s.v.n();
+    s.om = s.ow + s.w.n();
+    s.oe = s.om + s.m.n();
+    s.n = s.oe + s.w.n();
+    return s;
+  }
+  __host__ __device__ int64_t u2(int i, int j) const { return (int64_t)j * (nx + 1) + i; }
+  __host__ __device__ int64_t v2(int i, int j) const { return (int64_t)j * nx + i; }
+  __host__ __device__ int64_t m2(int i, int j) const { return (int64_t)j * nx + i; }
+};
+
+template <typename T>
+struct OpT {
+  Stag s;
+  T alpha;
+  const T *cua, *zxu, *cx, *cva, *zev, *cy, *dw, *cw, *cz, *kx, *ky, *rdzw, *cpm;
+  const T *mudx, *mvdy, *afu, *afv, *mmx, *mmy;
+  const T *idw, *gw, *lpi, *lower, *invden, *cprime, *dzw0, *dzwn;
+  static OpT make(const SislOperator &o) {
+    OpT r;
+    r.s = Stag::make(o.g);
+    r.alpha = (T)o.alpha;
+#define SISL_P(name) r.name = static_cast<const T *>(o.name)
+    SISL_P(cua); SISL_P(zxu); SISL_P(cx); SISL_P(cva); SISL_P(zev); SISL_P(cy);
+    SISL_P(dw); SISL_P(cw); SISL_P(cz); SISL_P(kx); SISL_P(ky); SISL_P(rdzw); SISL_P(cpm);
+    SISL_P(mudx); SISL_P(mvdy); SISL_P(afu); SISL_P(afv); SISL_P(mmx); SISL_P(mmy);
+    SISL_P(idw); SISL_P(gw); SISL_P(lpi); SISL_P(lower); SISL_P(invden); SISL_P(cprime);
+    SISL_P(dzw0); SISL_P(dzwn);
+#undef SISL_P
+    return r;
+  }
+  bool complete() const {
+    const void *p[] = {cua, zxu, cx, cva, zev, cy, dw, cw, cz, kx, ky, rdzw, cpm, mudx, mvdy,
+                       afu, afv, mmx, mmy, idw, gw, lpi, lower, invden, cprime, dzw0, dzwn};
+    for (auto q : p)
+      if (!q) return false;
+    return s.nx >= 2 && s.ny >= 1 && s.nz >= 2;
+  }
+};
+
+// Wave64 sum reduction through DPP/shuffles.
+template <typename T>
+__device__ inline T wave_sum(T v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+}  // namespace sisl
+
+namespace sisl {
+// Launchers shared between translation units (explicitly instantiated for
+// float and double next to their kernels).
+template <typename T> int launch_A(const OpT<T> &op, const T *x, T *y, hipStream_t s);
+template <typename T> int launch_AT(const OpT<T> &op, const T *x, T *y, T *scratch, hipStream_t s);
+template <typename T> int launch_M(const OpT<T> &op, const T *r, T *z, hipStream_t s);
+template <typename T> int launch_MT(const OpT<T> &op, const T *r, T *z, T *scratch, hipStream_t s);
+}  // namespace sisl

@@ -1,0 +1,465 @@
+// (a) Semi-Lagrangian interpolation: tricubic gather, its deterministic
+// transpose, index velocities and departure points.
+//
+// Reference: suetes/regional3d/operators.py:134-209 (tricubic),
+//            suetes/regional3d/steppers.py:276-367 (trajectories).
+#include "common.h"
+
+namespace sisl {
+
+// Catmull-Rom weights of the reference's cubic_weight (operators.py:134-144)
+// written per sample: f(t) = w0 p0 + w1 p1 + w2 p2 + w3 p3.
+template <typename T>
+__device__ __forceinline__ void cr_weights(T t, T w[4]) {
+  const T t2 = t * t, t3 = t2 * t;
+  w[0] = (T)-0.5 * t3 + t2 - (T)0.5 * t;
+  w[1] = (T)1.5 * t3 - (T)2.5 * t2 + (T)1.0;
+  w[2] = (T)-1.5 * t3 + (T)2.0 * t2 + (T)0.5 * t;
+  w[3] = (T)0.5 * t3 - (T)0.5 * t2;
+}
+
+// Clamp to [0, n-1], split into base index and fraction (operators.py:169-174).
+template <typename T>
+__device__ __forceinline__ int clamp_split(T c, int n, T &t) {
+  T cc = c < (T)0 ? (T)0 : c;
+  const T hi = (T)(n - 1);
+  cc = cc > hi ? hi : cc;
+  const T f = floor(cc);
+  t = cc - f;
+  return (int)f;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) tricubic_kernel(Dims d, const T *__restrict__ f,
+                                                       const T *__restrict__ co,
+                                                       T *__restrict__ out, int lim) {
+  const int64_t n = d.n();
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  T tx, ty, tz;
+  const int x0 = clamp_split(co[p], d.nx, tx);
+  const int y0 = clamp_split(co[n + p], d.ny, ty);
+  const int z0 = clamp_split(co[2 * n + p], d.nz, tz);
+  int ix[4] = {max(x0 - 1, 0), x0, min(x0 + 1, d.nx - 1), min(x0 + 2, d.nx - 1)};
+  int iy[4] = {max(y0 - 1, 0), y0, min(y0 + 1, d.ny - 1), min(y0 + 2, d.ny - 1)};
+  int iz[4] = {max(z0 - 1, 0), z0, min(z0 + 1, d.nz - 1), min(z0 + 2, d.nz - 1)};
+  T wx[4], wy[4], wz[4];
+  cr_weights(tx, wx);
+  cr_weights(ty, wy);
+  cr_weights(tz, wz);
+  // Cascade z -> y -> x as the reference does.
+  T acc_x = 0;
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    T acc_y = 0;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      T acc_z = 0;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) acc_z += wz[c] * f[d.at(ix[a], iy[b], iz[c])];
+      acc_y += wy[b] * acc_z;
+    }
+    acc_x += wx[a] * acc_y;
+  }
+  if (lim) {
+    const int x1 = min(x0 + 1, d.nx - 1), y1 = min(y0 + 1, d.ny - 1), z1 = min(z0 + 1, d.nz - 1);
+    T lo = f[d.at(x0, y0, z0)], hi = lo;
+    const int cx[7] = {x1, x0, x1, x0, x1, x0, x1};
+    const int cy[7] = {y0, y1, y1, y0, y0, y1, y1};
+    const int cz[7] = {z0, z0, z0, z1, z1, z1, z1};
+#pragma unroll
+    for (int q = 0; q < 7; ++q) {
+      const T nb = f[d.at(cx[q], cy[q], cz[q])];
+      lo = nb < lo ? nb : lo;
+      hi = nb > hi ? nb : hi;
+    }
+    acc_x = acc_x < lo ? lo : acc_x;
+    acc_x = acc_x > hi ? hi : acc_x;
+  }
+  out[p] = acc_x;
+}
+
+// ---- deterministic transpose ------------------------------------------------
+// Plan: sort output points by the base cell of their stencil (counting sort),
+// then every destination cell gathers from the <= 7x7x7 base cells whose
+// clamped taps can reach it, visiting points in increasing index order.
+
+template <typename T>
+__global__ void adj_key_kernel(Dims d, const T *__restrict__ co, int32_t *__restrict__ key,
+                               int32_t *__restrict__ count) {
+  const int64_t n = d.n();
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  T t;
+  const int x0 = clamp_split(co[p], d.nx, t);
+  const int y0 = clamp_split(co[n + p], d.ny, t);
+  const int z0 = clamp_split(co[2 * n + p], d.nz, t);
+  const int32_t k = (int32_t)d.at(x0, y0, z0);
+  key[p] = k;
+  atomicAdd(&count[k], 1);
+}
+
+constexpr int kScanItems = 16;
+constexpr int kScanTile = kBlock * kScanItems;
+
+// Exclusive block-local scan of count -> start, block totals -> sums.
+__global__ void __launch_bounds__(256) scan_tile_kernel(const int32_t *__restrict__ in,
+                                                        int32_t *__restrict__ out,
+                                                        int32_t *__restrict__ sums, int64_t n) {
+  __shared__ int32_t wsum[kBlock / 64];
+  const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+  int32_t v[kScanItems];
+  int32_t run = 0;
+#pragma unroll
+  for (int q = 0; q < kScanItems; ++q) {
+    const int64_t i = base + q;
+    const int32_t x = i < n ? in[i] : 0;
+    v[q] = run;
+    run += x;
+  }
+  // inclusive scan of per-thread totals across the block
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  int32_t incl = run;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int32_t y = __shfl_up(incl, off, 64);
+    if (lane >= off) incl += y;
+  }
+  if (lane == 63) wsum[wid] = incl;
+  __syncthreads();
+  int32_t wofs = 0;
+  for (int q = 0; q < wid; ++q) wofs += wsum[q];
+  const int32_t excl = incl - run + wofs;
+#pragma unroll
+  for (int q = 0; q < kScanItems; ++q) {
+    const int64_t i = base + q;
+    if (i < n) out[i] = v[q] + excl;
+  }
+  if (threadIdx.x == kBlock - 1) sums[blockIdx.x] = excl + run;
+}
+
+// Single-block exclusive scan of the block totals (nb <= 256*64).
+__global__ void __launch_bounds__(256) scan_sums_kernel(int32_t *__restrict__ sums, int nb,
+                                                        int32_t *__restrict__ total) {
+  __shared__ int32_t wsum[kBlock / 64];
+  constexpr int kI = 64;
+  const int base = threadIdx.x * kI;
+  int32_t run = 0;
+  for (int q = 0; q < kI; ++q) {
+    const int i = base + q;
+    if (i < nb) run += sums[i];
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  int32_t incl = run;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int32_t y = __shfl_up(incl, off, 64);
+    if (lane >= off) incl += y;
+  }
+  if (lane == 63) wsum[wid] = incl;
+  __syncthreads();
+  int32_t wofs = 0;
+  for (int q = 0; q < wid; ++q) wofs += wsum[q];
+  int32_t acc = incl - run + wofs;
+  for (int q = 0; q < kI; ++q) {
+    const int i = base + q;
+    if (i < nb) {
+      const int32_t x = sums[i];
+      sums[i] = acc;
+      acc += x;
+    }
+  }
+  if (threadIdx.x == kBlock - 1) *total = acc;
+}
+
+__global__ void scan_add_kernel(int32_t *__restrict__ out, const int32_t *__restrict__ sums,
+                                int64_t n, int32_t *__restrict__ count) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  out[i] += sums[i / kScanTile];
+  count[i] = 0;  // becomes the scatter cursor
+}
+
+__global__ void adj_scatter_kernel(int64_t n, const int32_t *__restrict__ key,
+                                   const int32_t *__restrict__ start, int32_t *__restrict__ cur,
+                                   int32_t *__restrict__ perm) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int32_t k = key[p];
+  const int32_t pos = atomicAdd(&cur[k], 1);
+  perm[start[k] + pos] = (int32_t)p;
+}
+
+// Sort each (small) segment by point index so accumulation order is fixed.
+__global__ void adj_segsort_kernel(int64_t n, const int32_t *__restrict__ start,
+                                   int32_t *__restrict__ perm) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int32_t s = start[k], e = start[k + 1];
+  for (int32_t a = s + 1; a < e; ++a) {
+    const int32_t x = perm[a];
+    int32_t b = a - 1;
+    while (b >= s && perm[b] > x) {
+      perm[b + 1] = perm[b];
+      --b;
+    }
+    perm[b + 1] = x;
+  }
+}
+
+// (base, tap) pairs whose clamped tap index equals i on an axis of length n.
+__device__ __forceinline__ int tap_candidates(int i, int n, int *b, int *t) {
+  int c = 0;
+  if (i + 1 <= n - 1) { b[c] = i + 1; t[c] = 0; ++c; }
+  if (i == 0) { b[c] = 0; t[c] = 0; ++c; }
+  b[c] = i; t[c] = 1; ++c;
+  if (i - 1 >= 0) { b[c] = i - 1; t[c] = 2; ++c; }
+  if (i == n - 1) { b[c] = n - 1; t[c] = 2; ++c; }
+  if (i - 2 >= 0) { b[c] = i - 2; t[c] = 3; ++c; }
+  if (i == n - 1) {
+    if (n - 2 >= 0 && n - 2 != i - 2) { b[c] = n - 2; t[c] = 3; ++c; }
+    if (n - 1 != i - 2) { b[c] = n - 1; t[c] = 3; ++c; }
+  }
+  return c;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) tricubic_adj_kernel(Dims d, const T *__restrict__ co,
+                                                           const int32_t *__restrict__ perm,
+                                                           const int32_t *__restrict__ start,
+                                                           const T *__restrict__ g,
+                                                           T *__restrict__ gf) {
+  const int64_t n = d.n();
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const int i = (int)(q % d.nx);
+  const int j = (int)((q / d.nx) % d.ny);
+  const int k = (int)(q / ((int64_t)d.nx * d.ny));
+  int bx[8], txa[8], by[8], tya[8], bz[8], tza[8];
+  const int nxc = tap_candidates(i, d.nx, bx, txa);
+  const int nyc = tap_candidates(j, d.ny, by, tya);
+  const int nzc = tap_candidates(k, d.nz, bz, tza);
+  T acc = 0;
+  for (int a = 0; a < nxc; ++a)
+    for (int b = 0; b < nyc; ++b)
+      for (int c = 0; c < nzc; ++c) {
+        const int64_t key = d.at(bx[a], by[b], bz[c]);
+        const int32_t s = start[key], e = start[key + 1];
+        for (int32_t r = s; r < e; ++r) {
+          const int64_t p = perm[r];
+          T tx, ty, tz;
+          clamp_split(co[p], d.nx, tx);
+          clamp_split(co[n + p], d.ny, ty);
+          clamp_split(co[2 * n + p], d.nz, tz);
+          T wx[4], wy[4], wz[4];
+          cr_weights(tx, wx);
+          cr_weights(ty, wy);
+          cr_weights(tz, wz);
+          acc += wx[txa[a]] * (wy[tya[b]] * (wz[tza[c]] * g[p]));
+        }
+      }
+  gf[q] = acc;
+}
+
+// ---- trajectories ---------------------------------------------------------
+
+// Index-space velocities at staggering `loc` (steppers.py:276-316).
+template <typename T>
+__global__ void index_velocity_kernel(Stag s, int loc, const T *__restrict__ u,
+                                      const T *__restrict__ v, const T *__restrict__ e,
+                                      const T *__restrict__ m2, T rdx, T rdy,
+                                      T *__restrict__ out, Dims L) {
+  const int64_t n = L.n();
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int i = (int)(p % L.nx);
+  const int j = (int)((p / L.nx) % L.ny);
+  const int k = (int)(p / ((int64_t)L.nx * L.ny));
+  const int nx = s.nx, ny = s.ny, nz = s.nz;
+  // column averages to mass points
+  auto um = [&](int ii, int jj, int kk) { return (T)0.5 * (u[s.u.at(ii, jj, kk)] + u[s.u.at(ii + 1, jj, kk)]); };
+  auto vm = [&](int ii, int jj, int kk) { return (T)0.5 * (v[s.v.at(ii, jj, kk)] + v[s.v.at(ii, jj + 1, kk)]); };
+  auto em = [&](int ii, int jj, int kk) { return (T)0.5 * (e[s.w.at(ii, jj, kk)] + e[s.w.at(ii, jj, kk + 1)]); };
+  T ul, vl, wl, mf;
+  if (loc == 0) {
+    ul = um(i, j, k); vl = vm(i, j, k); wl = em(i, j, k); mf = m2[s.m2(i, j)];
+  } else if (loc == 1) {
+    ul = u[s.u.at(i, j, k)];
+    T va, vb, ea, eb;
+    if (i == 0 || i == nx) {
+      if (s.px) { va = vm(0, j, k); vb = vm(nx - 1, j, k); ea = em(0, j, k); eb = em(nx - 1, j, k);
+                  vl = (T)0.5 * (va + vb); wl = (T)0.5 * (ea + eb); }
+      else { const int c = i == 0 ? 0 : nx - 1; vl = vm(c, j, k); wl = em(c, j, k); }
+    } else {
+      vl = (T)0.5 * (vm(i - 1, j, k) + vm(i, j, k));
+      wl = (T)0.5 * (em(i - 1, j, k) + em(i, j, k));
+    }
+    mf = m2[s.u2(i, j)];
+  } else if (loc == 2) {
+    vl = v[s.v.at(i, j, k)];
+    if (j == 0 || j == ny) {
+      if (s.py) { ul = (T)0.5 * (um(i, 0, k) + um(i, ny - 1, k)); wl = (T)0.5 * (em(i, 0, k) + em(i, ny - 1, k)); }
+      else { const int c = j == 0 ? 0 : ny - 1; ul = um(i, c, k); wl = em(i, c, k); }
+    } else {
+      ul = (T)0.5 * (um(i, j - 1, k) + um(i, j, k));
+      wl = (T)0.5 * (em(i, j - 1, k) + em(i, j, k));
+    }
+    mf = m2[s.v2(i, j)];
+  } else {
+    wl = e[s.w.at(i, j, k)];
+    if (k == 0) { ul = um(i, j, 0); vl = vm(i, j, 0); }
+    else if (k == nz) { ul = um(i, j, nz - 1); vl = vm(i, j, nz - 1); }
+    else { ul = (T)0.5 * (um(i, j, k - 1) + um(i, j, k)); vl = (T)0.5 * (vm(i, j, k - 1) + vm(i, j, k)); }
+    mf = m2[s.m2(i, j)];
+  }
+  out[p] = (ul * mf) * rdx;
+  out[n + p] = (vl * mf) * rdy;
+  out[2 * n + p] = wl;
+}
+
+// map_coordinates(order=1, mode='nearest') of one field (steppers.py:367).
+template <typename T>
+__device__ __forceinline__ T trilinear_nearest(const T *__restrict__ f, Dims d, T x, T y, T z) {
+  const T fx = floor(x), fy = floor(y), fz = floor(z);
+  const T wx1 = x - fx, wy1 = y - fy, wz1 = z - fz;
+  const T wx0 = (T)1 - wx1, wy0 = (T)1 - wy1, wz0 = (T)1 - wz1;
+  const int ix = (int)fx, iy = (int)fy, iz = (int)fz;
+  const int x0 = min(max(ix, 0), d.nx - 1), x1 = min(max(ix + 1, 0), d.nx - 1);
+  const int y0 = min(max(iy, 0), d.ny - 1), y1 = min(max(iy + 1, 0), d.ny - 1);
+  const int z0 = min(max(iz, 0), d.nz - 1), z1 = min(max(iz + 1, 0), d.nz - 1);
+  T r = 0;
+  r += f[d.at(x0, y0, z0)] * (wx0 * wy0 * wz0);
+  r += f[d.at(x0, y0, z1)] * (wx0 * wy0 * wz1);
+  r += f[d.at(x0, y1, z0)] * (wx0 * wy1 * wz0);
+  r += f[d.at(x0, y1, z1)] * (wx0 * wy1 * wz1);
+  r += f[d.at(x1, y0, z0)] * (wx1 * wy0 * wz0);
+  r += f[d.at(x1, y0, z1)] * (wx1 * wy0 * wz1);
+  r += f[d.at(x1, y1, z0)] * (wx1 * wy1 * wz0);
+  r += f[d.at(x1, y1, z1)] * (wx1 * wy1 * wz1);
+  return r;
+}
+
+// Midpoint trajectory iteration (steppers.py:318-360).
+template <typename T>
+__global__ void departure_kernel(Dims L, const T *__restrict__ vel, T dt, int iters,
+                                 T *__restrict__ out) {
+  const int64_t n = L.n();
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const T X = (T)(int)(p % L.nx);
+  const T Y = (T)(int)((p / L.nx) % L.ny);
+  const T Z = (T)(int)(p / ((int64_t)L.nx * L.ny));
+  T ax = dt * vel[p], ay = dt * vel[n + p], az = dt * vel[2 * n + p];
+  for (int it = 0; it < iters; ++it) {
+    const T mx = X - (T)0.5 * ax, my = Y - (T)0.5 * ay, mz = Z - (T)0.5 * az;
+    const T um = trilinear_nearest(vel, L, mx, my, mz);
+    const T vm = trilinear_nearest(vel + n, L, mx, my, mz);
+    const T wm = trilinear_nearest(vel + 2 * n, L, mx, my, mz);
+    ax = dt * um; ay = dt * vm; az = dt * wm;
+  }
+  out[p] = X - ax;
+  out[n + p] = Y - ay;
+  out[2 * n + p] = Z - az;
+}
+
+template <typename T>
+int tricubic(int fnx, int fny, int fnz, const T *f, const T *co, T *out, int lim, void *st) {
+  if (!f || !co || !out || fnx < 1 || fny < 1 || fnz < 1) return SISL_ERR_ARG;
+  Dims d{fnx, fny, fnz};
+  hipLaunchKernelGGL(tricubic_kernel<T>, dim3(grid_for(d.n())), dim3(kBlock), 0, as_stream(st),
+                     d, f, co, out, lim);
+  SISL_CHECK_LAUNCH();
+  return SISL_OK;
+}
+
+template <typename T>
+int tricubic_plan(int fnx, int fny, int fnz, const T *co, int32_t *key, int32_t *perm,
+                  int32_t *start, int32_t *count, int32_t *sums, void *st) {
+  if (!co || !key || !perm || !start || !count || !sums) return SISL_ERR_ARG;
+  Dims d{fnx, fny, fnz};
+  const int64_t n = d.n();
+  const int nb = (int)((n + kScanTile - 1) / kScanTile);
+  if (nb > kBlock * 64 || n >= (int64_t)1 << 31) return SISL_ERR_ARG;
+  hipStream_t s = as_stream(st);
+  if (hipMemsetAsync(count, 0, sizeof(int32_t) * n, s) != hipSuccess) return SISL_ERR_LAUNCH;
+  hipLaunchKernelGGL(adj_key_kernel<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, d, co, key, count);
+  hipLaunchKernelGGL(scan_tile_kernel, dim3(nb), dim3(kBlock), 0, s, count, start, sums, n);
+  hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(kBlock), 0, s, sums, nb, start + n);
+  hipLaunchKernelGGL(scan_add_kernel, dim3(grid_for(n)), dim3(kBlock), 0, s, start, sums, n, count);
+  hipLaunchKernelGGL(adj_scatter_kernel, dim3(grid_for(n)), dim3(kBlock), 0, s, n, key, start,
+                     count, perm);
+  hipLaunchKernelGGL(adj_segsort_kernel, dim3(grid_for(n)), dim3(kBlock), 0, s, n, start, perm);
+  SISL_CHECK_LAUNCH();
+  return SISL_OK;
+}
+
+template <typename T>
+int tricubic_adj(int fnx, int fny, int fnz, const T *co, const int32_t *perm, const int32_t *start,
+                 const T *g, T *gf, void *st) {
+  if (!co || !perm || !start || !g || !gf) return SISL_ERR_ARG;
+  Dims d{fnx, fny, fnz};
+  hipLaunchKernelGGL(tricubic_adj_kernel<T>, dim3(grid_for(d.n())), dim3(kBlock), 0,
+                     as_stream(st), d, co, perm, start, g, gf);
+  SISL_CHECK_LAUNCH();
+  return SISL_OK;
+}
+
+template <typename T>
+int departure(const SislGrid *g, int loc, const T *u, const T *v, const T *e, const T *m2,
+              double dx, double dy, double dt, int iters, T *scratch3, T *out, void *st) {
+  if (!g || !u || !v || !e || !m2 || !scratch3 || !out || loc < 0 || loc > 3) return SISL_ERR_ARG;
+  Stag s = Stag::make(*g);
+  Dims L = loc == 0 ? s.m : loc == 1 ? s.u : loc == 2 ? s.v : s.w;
+  hipStream_t hs = as_stream(st);
+  hipLaunchKernelGGL(index_velocity_kernel<T>, dim3(grid_for(L.n())), dim3(kBlock), 0, hs, s, loc,
+                     u, v, e, m2, (T)(1.0 / dx), (T)(1.0 / dy), scratch3, L);
+  hipLaunchKernelGGL(departure_kernel<T>, dim3(grid_for(L.n())), dim3(kBlock), 0, hs, L,
+                     (const T *)scratch3, (T)dt, iters, out);
+  SISL_CHECK_LAUNCH();
+  return SISL_OK;
+}
+
+}  // namespace sisl
+
+extern "C" {
+
+int32_t sisl_scan_blocks(int64_t n) { return (int32_t)((n + sisl::kScanTile - 1) / sisl::kScanTile); }
+
+int sisl_tricubic_f64(int32_t a, int32_t b, int32_t c, const double *f, const double *co,
+                      double *out, int32_t lim, void *s) {
+  return sisl::tricubic<double>(a, b, c, f, co, out, lim, s);
+}
+int sisl_tricubic_f32(int32_t a, int32_t b, int32_t c, const float *f, const float *co,
+                      float *out, int32_t lim, void *s) {
+  return sisl::tricubic<float>(a, b, c, f, co, out, lim, s);
+}
+int sisl_tricubic_adj_plan_f64(int32_t a, int32_t b, int32_t c, const double *co, int32_t *key,
+                               int32_t *perm, int32_t *start, int32_t *count, int32_t *sums,
+                               void *s) {
+  return sisl::tricubic_plan<double>(a, b, c, co, key, perm, start, count, sums, s);
+}
+int sisl_tricubic_adj_plan_f32(int32_t a, int32_t b, int32_t c, const float *co, int32_t *key,
+                               int32_t *perm, int32_t *start, int32_t *count, int32_t *sums,
+                               void *s) {
+  return sisl::tricubic_plan<float>(a, b, c, co, key, perm, start, count, sums, s);
+}
+int sisl_tricubic_adj_f64(int32_t a, int32_t b, int32_t c, const double *co, const int32_t *perm,
+                          const int32_t *start, const double *g, double *gf, void *s) {
+  return sisl::tricubic_adj<double>(a, b, c, co, perm, start, g, gf, s);
+}
+int sisl_tricubic_adj_f32(int32_t a, int32_t b, int32_t c, const float *co, const int32_t *perm,
+                          const int32_t *start, const float *g, float *gf, void *s) {
+  return sisl::tricubic_adj<float>(a, b, c, co, perm, start, g, gf, s);
+}
+int sisl_departure_f64(const SislGrid *g, int32_t loc, const double *u, const double *v,
+                       const double *e, const double *m2, double dx, double dy, double dt,
+                       int32_t it, double *scr, double *out, void *s) {
+  return sisl::departure<double>(g, loc, u, v, e, m2, dx, dy, dt, it, scr, out, s);
+}
+int sisl_departure_f32(const SislGrid *g, int32_t loc, const float *u, const float *v,
+                       const float *e, const float *m2, double dx, double dy, double dt,
+                       int32_t it, float *scr, float *out, void *s) {
+  return sisl::departure<float>(g, loc, u, v, e, m2, dx, dy, dt, it, scr, out, s);
+}
+
+}  // extern "C"

@@ -1,0 +1,168 @@
+"""Stagger-aware C-grid operators and the tricubic semi-Lagrangian interpolator.
+
+``CGridOperator3D`` mirrors suetes/regional3d/operators.py:12-132 on internal
+(Z, Y, X) tensors: reference axis 0 (x) is tensor dim 2, axis 1 (y) dim 1,
+axis 2 (z) dim 0.  ``tensor_product_interp_3d`` (operators.py:146-209) runs
+the HIP gather; its backward is the deterministic segmented transpose.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import torch
+
+from . import _lib
+
+DIM = {0: 2, 1: 1, 2: 0}
+
+
+class CGridOperator3D:
+    """diff / avg between staggerings with the reference's edge closures."""
+
+    def __init__(self, grid, periodic_axes=()):
+        if set(periodic_axes) - {0, 1}:
+            raise ValueError("Only horizontal axes 0 (x) and 1 (y) may be periodic")
+        self.grid = grid
+        self.periodic_axes = frozenset(periodic_axes)
+        self.use_stop_grad = True
+
+    def m3(self, loc):
+        return self.grid.i_m_factors[loc][None]
+
+    def _apply_padding(self, f, axis, from_loc, to_loc):
+        # operators.py:37-63: duplicate the edge values (detached when use_stop_grad)
+        if from_loc == "m" and to_loc in ("u", "v", "w"):
+            d = DIM[axis]
+            n = f.shape[d]
+            lo, hi = f.narrow(d, 0, 1), f.narrow(d, n - 1, 1)
+            if self.use_stop_grad:
+                lo, hi = lo.detach(), hi.detach()
+            return torch.cat([lo, f, hi], dim=d)
+        return f
+
+    def diff(self, f, axis, from_loc, to_loc):
+        # operators.py:65-103
+        d = DIM[axis]
+        delta = self.grid.delta[axis]
+        n = f.shape[d]
+        der = (f.narrow(d, 1, n - 1) - f.narrow(d, 0, n - 1)) / delta
+        if axis in self.periodic_axes and from_loc == "m" and to_loc in ("u", "v"):
+            b = (f.narrow(d, 0, 1) - f.narrow(d, n - 1, 1)) / delta
+            der = torch.cat([b, der, b], dim=d)
+        else:
+            der = self._apply_padding(der, axis, from_loc, to_loc)
+        if axis in (0, 1):
+            der = der * self.m3(to_loc)
+        return der
+
+    def avg(self, f, axis, from_loc, to_loc):
+        # operators.py:105-132
+        d = DIM[axis]
+        n = f.shape[d]
+        a = 0.5 * (f.narrow(d, 0, n - 1) + f.narrow(d, 1, n - 1))
+        if axis in self.periodic_axes and from_loc == "m" and to_loc in ("u", "v"):
+            b = 0.5 * (f.narrow(d, 0, 1) + f.narrow(d, n - 1, 1))
+            return torch.cat([b, a, b], dim=d)
+        return self._apply_padding(a, axis, from_loc, to_loc)
+
+
+# ---------------------------------------------------------------------------
+# tricubic interpolation (HIP)
+# ---------------------------------------------------------------------------
+class TricubicPlan:
+    """Destination-sorted segment map of one coordinate set, built once and
+    shared by every adjoint that uses the same departure points."""
+
+    def __init__(self, coords: torch.Tensor, dims):
+        L = _lib.lib()
+        fz, fy, fx = dims
+        n = fx * fy * fz
+        dev = coords.device
+        i32 = dict(dtype=torch.int32, device=dev)
+        self.key = torch.empty(n, **i32)
+        self.perm = torch.empty(n, **i32)
+        self.start = torch.empty(n + 1, **i32)
+        count = torch.empty(n, **i32)
+        sums = torch.empty(max(1, L.sisl_scan_blocks(n)), **i32)
+        fn = L.sisl_tricubic_adj_plan_f64 if coords.dtype == torch.float64 else L.sisl_tricubic_adj_plan_f32
+        _lib.check(fn(fx, fy, fz, coords.data_ptr(), self.key.data_ptr(), self.perm.data_ptr(),
+                      self.start.data_ptr(), count.data_ptr(), sums.data_ptr(), _lib.stream_ptr(dev)),
+                   "tricubic adjoint plan")
+
+
+class Coords:
+    """Departure coordinates of one staggering: tensor (3, Z, Y, X) in index
+    space (component 0 = x index) plus a lazily built adjoint plan."""
+
+    def __init__(self, data: torch.Tensor):
+        self.data = data
+        self._plan = None
+
+    def plan(self):
+        if self._plan is None:
+            self._plan = TricubicPlan(self.data, tuple(self.data.shape[1:]))
+        return self._plan
+
+
+def _tricubic_fwd(field, coords, use_limiter):
+    _lib.require_device(field, coords)
+    if coords.dtype != field.dtype or tuple(coords.shape) != (3,) + tuple(field.shape):
+        raise ValueError("coords must be (3,)+field.shape with the field's dtype")
+    fz, fy, fx = field.shape
+    out = torch.empty_like(field)
+    L = _lib.lib()
+    fn = L.sisl_tricubic_f64 if field.dtype == torch.float64 else L.sisl_tricubic_f32
+    _lib.check(fn(fx, fy, fz, field.data_ptr(), coords.data_ptr(), out.data_ptr(),
+                  int(bool(use_limiter)), _lib.stream_ptr(field.device)), "tricubic")
+    return out
+
+
+class _Tricubic(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, field, coords_obj, use_limiter):
+        ctx.coords = coords_obj
+        ctx.use_limiter = use_limiter
+        return _tricubic_fwd(field.contiguous(), coords_obj.data, use_limiter)
+
+    @staticmethod
+    def backward(ctx, g):
+        if ctx.use_limiter:
+            raise NotImplementedError("the adjoint of the limited tricubic is not implemented")
+        co = ctx.coords.data
+        plan = ctx.coords.plan()
+        g = g.contiguous()
+        gf = torch.empty_like(g)
+        fz, fy, fx = g.shape
+        L = _lib.lib()
+        fn = L.sisl_tricubic_adj_f64 if g.dtype == torch.float64 else L.sisl_tricubic_adj_f32
+        _lib.check(fn(fx, fy, fz, co.data_ptr(), plan.perm.data_ptr(), plan.start.data_ptr(),
+                      g.data_ptr(), gf.data_ptr(), _lib.stream_ptr(g.device)), "tricubic adjoint")
+        return gf, None, None
+
+
+def tensor_product_interp_3d(field, coords, use_limiter=False):
+    """Tricubic Catmull-Rom gather at departure points (operators.py:146-209).
+
+    ``field``: internal (Z, Y, X) tensor; ``coords``: a :class:`Coords` (or a
+    (3, Z, Y, X) tensor) of index-space departure points.  When coords are
+    FP32 and the field FP64 the interpolation runs in FP32 and the result is
+    returned in FP64 (mixed precision)."""
+    if not isinstance(coords, Coords):
+        coords = Coords(coords)
+    if coords.data.dtype == torch.float32 and field.dtype == torch.float64:
+        return _Tricubic.apply(field.float(), coords, use_limiter).double()
+    return _Tricubic.apply(field, coords, use_limiter)
+
+
+def tricubic_transpose(g, coords):
+    """Explicit T^T g (deterministic), for tests and custom adjoints."""
+    if not isinstance(coords, Coords):
+        coords = Coords(coords)
+    plan = coords.plan()
+    gf = torch.empty_like(g)
+    fz, fy, fx = g.shape
+    L = _lib.lib()
+    fn = L.sisl_tricubic_adj_f64 if g.dtype == torch.float64 else L.sisl_tricubic_adj_f32
+    _lib.check(fn(fx, fy, fz, coords.data.data_ptr(), plan.perm.data_ptr(), plan.start.data_ptr(),
+                  g.data_ptr(), gf.data_ptr(), _lib.stream_ptr(g.device)), "tricubic adjoint")
+    return gf

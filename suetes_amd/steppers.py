@@ -1,0 +1,594 @@
+"""The SISL dynamical-core step on MI355X (drop-in for the reference's step call).
+
+Mirrors ``build_dynamical_core`` / ``SISLStepper3D`` and their helpers
+(suetes/regional3d/steppers.py:12-122, 125-527, 530-647, 650-1073).  The hot
+kernels (departure points, tricubic interpolation, the implicit operator,
+the column preconditioner and the whole GMRES solve) run in HIP through the
+C ABI; the remaining per-step glue is torch ops on device tensors, so torch
+autograd composes the reverse mode from the hand-written kernel adjoints.
+
+State dicts at the public API have the reference's keys and shapes
+(u (nx+1,ny,nz), ...).  Internally every field is a contiguous (Z, Y, X)
+tensor; the public tensors returned by ``step`` are views of it, so a
+trajectory stays in the internal layout with no copies.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import torch
+
+from . import _lib
+from .euler import Euler3D
+from .geometry import ref_view, to_internal
+from .operators import Coords, tensor_product_interp_3d
+
+KEYS = ("u", "v", "w", "pi", "eta_dot")
+LOC_ID = {"m": 0, "u": 1, "v": 2, "w": 3}
+_FLOAT_KEYS_SCALAR = ("is_first_step",)
+
+
+def _is_field(v):
+    return torch.is_tensor(v) and v.dim() == 3
+
+
+def state_to_internal(state):
+    """Reference-shaped state dict -> internal dict (fields become (Z,Y,X))."""
+    return {k: (to_internal(v) if _is_field(v) else v) for k, v in state.items()}
+
+
+def state_to_reference(istate):
+    return {k: (ref_view(v) if _is_field(v) else v) for k, v in istate.items()}
+
+
+def _set_slice(t, dim, index, value):
+    """Functional ``t.at[...].set(value)`` along one dim (autograd-safe)."""
+    t = t.clone()
+    t.narrow(dim, index % t.shape[dim], 1).copy_(value)
+    return t
+
+
+def _zero_ends(t, dim=0):
+    t = t.clone()
+    t.narrow(dim, 0, 1).zero_()
+    t.narrow(dim, t.shape[dim] - 1, 1).zero_()
+    return t
+
+
+# ---------------------------------------------------------------------------
+def build_dynamical_core(core_type, grid, operators, constants, initial_state, physics_suite=None,
+                         interior_mask=None, **kwargs):
+    """Factory (steppers.py:12-122): returns ``(stepper, dt)`` and adds the
+    history keys to ``initial_state`` in place.
+
+    Extra keyword arguments of this implementation: ``precision`` ('fp64' or
+    'mixed'), ``inner_rtol`` (mixed solver), ``solver_early_exit``."""
+    core = core_type.lower()
+    if core == "split-explicit":
+        raise NotImplementedError("the split-explicit core is not part of this build")
+    if core != "sisl":
+        raise ValueError(f"Unknown core_type: '{core_type}'. Use 'sisl' or 'split-explicit'.")
+    p = {"dt": 120.0, "damp_height": 9000.0, "max_damp": 3.0, "nu_div_factor": 0.1,
+         "nu_h_factor": 0.1, "N_bv": 0.01, "alpha": 0.55, "use_limiter": False,
+         "use_checkpointing": False, "solver_tol": 1e-12, "solver_maxiter": 100,
+         "solver_restart": 100, "background_state": None, "precision": "fp64",
+         "inner_rtol": 1e-4, "solver_early_exit": False}
+    p.update(kwargs)
+    ref = p["background_state"] if p["background_state"] is not None else initial_state
+    physics = Euler3D(grid, operators, constants, dt=p["dt"], initial_era5_state=ref,
+                      damp_height=p["damp_height"], max_damp=p["max_damp"],
+                      nu_div_factor=p["nu_div_factor"], nu_h_factor=p["nu_h_factor"], N_bv=p["N_bv"],
+                      physics_suite=physics_suite, interior_mask=interior_mask)
+    if "u_prev" not in initial_state:
+        initial_state["u_prev"] = initial_state["u"]
+        initial_state["v_prev"] = initial_state["v"]
+        initial_state["w_prev"] = initial_state["w"]
+        initial_state["eta_dot_prev"] = initial_state.get("eta_dot", torch.zeros_like(initial_state["w"]))
+        initial_state["tend_th_v_prev"] = torch.zeros_like(initial_state["th_v"])
+        initial_state["is_first_step"] = 1.0
+        for key in (physics_suite.tracer_keys if physics_suite is not None else []):
+            if key in initial_state:
+                initial_state[f"tend_{key}_prev"] = torch.zeros_like(initial_state[key])
+    stepper = SISLStepper3D(physics, p["dt"], alpha=p["alpha"], use_limiter=p["use_limiter"],
+                            use_checkpointing=p["use_checkpointing"], solver_tol=p["solver_tol"],
+                            solver_maxiter=p["solver_maxiter"], solver_restart=p["solver_restart"],
+                            precision=p["precision"], inner_rtol=p["inner_rtol"],
+                            solver_early_exit=p["solver_early_exit"])
+    return stepper, p["dt"]
+
+
+# ---------------------------------------------------------------------------
+class SemiLagrangianAdvector3D:
+    """Trajectories and interpolation (steppers.py:256-374), HIP kernels."""
+
+    def __init__(self, grid, physics, dt, coord_dtype=torch.float64):
+        self.grid, self.physics, self.dt, self.op = grid, physics, dt, physics.op
+        self.coord_dtype = coord_dtype
+        g = grid
+        self._cgrid = _lib.SislGrid(g.nx, g.ny, g.nz, int(0 in self.op.periodic_axes),
+                                    int(1 in self.op.periodic_axes))
+
+    def compute_departure_indices(self, state, loc="m", iterations=2):
+        """Departure coordinates (3, Z, Y, X) of staggering ``loc`` in index
+        space, wrapped in a :class:`Coords` (stop-gradient, as the reference)."""
+        with torch.no_grad():
+            dt_ = self.coord_dtype
+            u = state["u"].detach().to(dt_).contiguous()
+            v = state["v"].detach().to(dt_).contiguous()
+            e = state["eta_dot"].detach().to(dt_).contiguous()
+            m2 = self.grid.i_m_factors[loc].detach().to(dt_).contiguous()
+            _lib.require_device(u, v, e, m2)
+            g = self.grid
+            shape = {"m": (g.nz, g.ny, g.nx), "u": (g.nz, g.ny, g.nx + 1),
+                     "v": (g.nz, g.ny + 1, g.nx), "w": (g.nz + 1, g.ny, g.nx)}[loc]
+            out = torch.empty((3,) + shape, dtype=dt_, device=u.device)
+            scr = torch.empty_like(out)
+            L = _lib.lib()
+            fn = L.sisl_departure_f64 if dt_ == torch.float64 else L.sisl_departure_f32
+            _lib.check(fn(C.byref(self._cgrid), LOC_ID[loc], u.data_ptr(), v.data_ptr(), e.data_ptr(),
+                          m2.data_ptr(), g.dx, g.dy, float(self.dt), int(iterations), scr.data_ptr(),
+                          out.data_ptr(), _lib.stream_ptr(u.device)), "departure points")
+        return Coords(out)
+
+    def advect_cubic(self, field, coords, use_limiter=False):
+        return tensor_product_interp_3d(field, coords, use_limiter)
+
+
+# ---------------------------------------------------------------------------
+class FluxFormAdvector:
+    """Split flux-form semi-Lagrangian transport (steppers.py:530-647).
+
+    Torch ops on device tensors; autograd gives the transpose, including
+    the Courant-number (velocity) sensitivity the reference keeps."""
+
+    def __init__(self, grid, dt, periodic_axes=()):
+        self.grid, self.dt = grid, dt
+        self.periodic_axes = frozenset(periodic_axes)
+
+    @staticmethod
+    def advect_1d(mass, cfl, dim, periodic=False):
+        N = mass.shape[dim]
+        shape = [1] * mass.dim()
+        shape[dim] = N + 1
+        ar = torch.arange(N + 1, dtype=mass.dtype, device=mass.device).reshape(shape)
+        zero = torch.zeros_like(mass.narrow(dim, 0, 1))
+        if periodic:
+            ext = torch.cat([mass, mass, mass], dim=dim)
+            M = torch.cat([zero, torch.cumsum(ext, dim=dim)], dim=dim)
+            idx = ar + N - cfl
+        else:
+            M = torch.cat([zero, torch.cumsum(mass, dim=dim)], dim=dim)
+            idx = torch.minimum(torch.maximum(ar - cfl, torch.zeros_like(cfl)),
+                                torch.full_like(cfl, float(N)))
+        n = M.shape[dim]
+        lo = torch.floor(idx)
+        t = idx - lo
+        i = lo.long()
+        a = torch.gather(M, dim, torch.clamp(i, 0, n - 1))
+        b = torch.gather(M, dim, torch.clamp(i + 1, 0, n - 1))
+        Md = (1.0 - t) * a + t * b
+        return Md.narrow(dim, 1, N) - Md.narrow(dim, 0, N)
+
+    def advect_3d_split(self, field, state, bg):
+        g = self.grid
+        cx = (state["u"] * g.i_m_factors["u"][None] * self.dt) / g.dx
+        cy = (state["v"] * g.i_m_factors["v"][None] * self.dt) / g.dy
+        cz = state["eta_dot"] * self.dt
+        vol = (g.dx * g.dy / g.i_m_factors["m"][None] ** 2) * bg["dz_m_full"]
+        m = field * vol
+        m = self.advect_1d(m, cx, 2, 0 in self.periodic_axes)
+        m = self.advect_1d(m, cy, 1, 1 in self.periodic_axes)
+        m = self.advect_1d(m, cz, 0, False)
+        return m / vol
+
+
+# ---------------------------------------------------------------------------
+def pack(d):
+    return torch.cat([d[k].reshape(-1) for k in KEYS])
+
+
+def unpack(x, like):
+    out, o = {}, 0
+    for k in KEYS:
+        n = like[k].numel()
+        out[k] = x[o:o + n].view(like[k].shape)
+        o += n
+    return out
+
+
+class _OperatorSet:
+    """Device coefficient tensors plus the SislOperator struct pointing at them."""
+
+    def __init__(self, grid, periodic_axes, coefs, alpha, dtype):
+        from ._lib import OPERATOR_FIELDS, SislGrid, SislOperator
+        self.tensors = {k: coefs[k].detach().to(dtype).contiguous() for k in OPERATOR_FIELDS}
+        _lib.require_device(*self.tensors.values())
+        g = SislGrid(grid.nx, grid.ny, grid.nz, int(0 in periodic_axes), int(1 in periodic_axes))
+        self.struct = SislOperator(g, float(alpha), *[self.tensors[k].data_ptr() for k in OPERATOR_FIELDS])
+        self.dtype = dtype
+
+
+class VerticalPreconditioner:
+    """Column Helmholtz preconditioner (steppers.py:125-253) on the HIP kernel."""
+
+    def __init__(self, physics, dt, alpha=0.55, pi_scale=1000.0):
+        self.physics, self.dt, self.alpha, self.pi_scale = physics, dt, alpha, pi_scale
+        self._op = None
+
+    def precompute_banded(self, bg):
+        co = self.physics.operator_coefficients(bg, self.dt, self.alpha)
+        self._op = _OperatorSet(self.physics.grid, self.physics.op.periodic_axes, co, self.alpha,
+                                torch.float64)
+        self.lower, self.invden, self.cprime = co["lower"], co["invden"], co["cprime"]
+
+    def __call__(self, rhs_scaled):
+        x = pack(rhs_scaled).contiguous()
+        _lib.require_device(x)
+        z = torch.empty_like(x)
+        _lib.check(_lib.lib().sisl_precond_apply_f64(C.byref(self._op.struct), x.data_ptr(), z.data_ptr(),
+                                                     _lib.stream_ptr(x.device)), "preconditioner")
+        return unpack(z, rhs_scaled)
+
+
+class _GmresSolve(torch.autograd.Function):
+    """x = A^{-1} b with the custom_linear_solve adjoint (steppers.py:473-485):
+    the cotangent of b is the transposed solve with the transposed
+    preconditioner; x0 receives none."""
+
+    @staticmethod
+    def forward(ctx, solver, b, x0):
+        ctx.solver = solver
+        return solver.solve_flat(b.detach(), x0.detach(), transpose=False)
+
+    @staticmethod
+    def backward(ctx, gx):
+        y = ctx.solver.solve_flat(gx.contiguous(), torch.zeros_like(gx), transpose=True)
+        return None, y, None
+
+
+class SemiImplicitSolver3D:
+    """GMRES solve of the acoustic/gravity system (steppers.py:377-527).
+
+    ``precision='fp64'`` runs the reference algorithm (every restart cycle
+    runs ``solver_restart`` Arnoldi steps); ``'mixed'`` runs FP32 operator,
+    preconditioner and Krylov basis inside an FP64 residual-refinement loop
+    with the same stopping test."""
+
+    def __init__(self, physics, dt, alpha=0.55, solver_tol=1e-4, solver_maxiter=20, solver_restart=20,
+                 precision="fp64", inner_rtol=1e-4, early_exit=False):
+        self.physics, self.dt, self.alpha = physics, dt, alpha
+        self.pi_scale = 1000.0
+        self.solver_tol, self.solver_maxiter, self.solver_restart = solver_tol, solver_maxiter, solver_restart
+        if precision not in ("fp64", "mixed"):
+            raise ValueError("precision must be 'fp64' or 'mixed'")
+        self.precision = precision
+        self.inner_rtol = inner_rtol
+        self.early_exit = early_exit
+        self._bg_id = None
+        self.last_stats = []
+        self.record_stats = True
+
+    # -- setup ------------------------------------------------------------------
+    def prepare(self, bg):
+        key = id(bg["th_v"])
+        if self._bg_id == key:
+            return
+        co = self.physics.operator_coefficients(bg, self.dt, self.alpha)
+        g, per = self.physics.grid, self.physics.op.periodic_axes
+        self.op64 = _OperatorSet(g, per, co, self.alpha, torch.float64)
+        self.op32 = _OperatorSet(g, per, co, self.alpha, torch.float32) if self.precision == "mixed" else None
+        L = _lib.lib()
+        R = min(self.solver_restart, 64)
+        if self.solver_restart > 64:
+            raise ValueError("solver_restart above 64 is not supported by the HIP GMRES")
+        nbytes = (L.sisl_gmres_workspace_bytes_mixed if self.precision == "mixed"
+                  else L.sisl_gmres_workspace_bytes_f64)(C.byref(self.op64.struct.g), R)
+        self.workspace = torch.empty(int(nbytes), dtype=torch.uint8, device=co["cua"].device)
+        self._bg_id = key
+        self._bg = bg
+
+    def _params(self, transpose):
+        return _lib.SislGmresParams(float(self.solver_tol), 0.0, int(self.solver_restart),
+                                    int(self.solver_maxiter), int(transpose), int(bool(self.early_exit)),
+                                    float(self.inner_rtol), 0)
+
+    def solve_flat(self, b, x0, transpose=False):
+        """Solve on packed scaled vectors; returns a new tensor."""
+        _lib.require_device(b)
+        x = x0.clone().contiguous()
+        b = b.contiguous()
+        st = _lib.SislGmresStats()
+        p = self._params(transpose)
+        L = _lib.lib()
+        s = _lib.stream_ptr(b.device)
+        if self.precision == "mixed":
+            rc = L.sisl_gmres_mixed(C.byref(self.op64.struct), C.byref(self.op32.struct), C.byref(p),
+                                    b.data_ptr(), x.data_ptr(), self.workspace.data_ptr(), C.byref(st), s)
+        else:
+            rc = L.sisl_gmres_f64(C.byref(self.op64.struct), C.byref(p), b.data_ptr(), x.data_ptr(),
+                                  self.workspace.data_ptr(), C.byref(st), s)
+        _lib.check(rc, "GMRES")
+        if self.record_stats:
+            self.last_stats.append({"transpose": bool(transpose), "cycles": st.cycles,
+                                    "arnoldi_steps": st.arnoldi_steps, "residual": st.residual,
+                                    "ptol": st.ptol})
+        return x
+
+    def solve(self, rhs_prime, bg_precomputed, x0=None):
+        """Internal dicts in, internal dict out (steppers.py:401-496)."""
+        self.prepare(bg_precomputed)
+        dzw = bg_precomputed["dz_w_full"]
+        rs = {k: (rhs_prime[k] * self.pi_scale if k == "pi" else rhs_prime[k]) for k in KEYS}
+        if x0 is not None:
+            x0s = {"u": x0["u"], "v": x0["v"], "w": x0["w"], "pi": x0["pi"] * self.pi_scale,
+                   "eta_dot": x0["eta_dot"] * dzw}
+        else:
+            x0s = rs
+        xs = _GmresSolve.apply(self, pack(rs), pack(x0s))
+        x = unpack(xs, rs)
+        return {"u": x["u"], "v": x["v"], "w": x["w"], "pi": x["pi"] / self.pi_scale,
+                "eta_dot": x["eta_dot"] / dzw}
+
+    def apply_A(self, x_flat, transpose=False, dtype=torch.float64):
+        """y = A x (or A^T x) on packed scaled vectors with the HIP kernel."""
+        ops = self.op64 if dtype == torch.float64 else self.op32
+        x = x_flat.to(dtype).contiguous()
+        y = torch.empty_like(x)
+        L = _lib.lib()
+        s = _lib.stream_ptr(x.device)
+        sfx = "f64" if dtype == torch.float64 else "f32"
+        if transpose:
+            g = self.physics.grid
+            scr = torch.empty(3 * g.nx * g.ny * g.nz, dtype=dtype, device=x.device)
+            rc = getattr(L, f"sisl_helmholtz_apply_T_{sfx}")(C.byref(ops.struct), x.data_ptr(), y.data_ptr(),
+                                                             scr.data_ptr(), s)
+        else:
+            rc = getattr(L, f"sisl_helmholtz_apply_{sfx}")(C.byref(ops.struct), x.data_ptr(), y.data_ptr(), s)
+        _lib.check(rc, "operator apply")
+        return y
+
+    def apply_M(self, x_flat, transpose=False, dtype=torch.float64):
+        ops = self.op64 if dtype == torch.float64 else self.op32
+        x = x_flat.to(dtype).contiguous()
+        y = torch.empty_like(x)
+        L = _lib.lib()
+        s = _lib.stream_ptr(x.device)
+        sfx = "f64" if dtype == torch.float64 else "f32"
+        if transpose:
+            g = self.physics.grid
+            scr = torch.empty(3 * g.nx * g.ny * g.nz, dtype=dtype, device=x.device)
+            rc = getattr(L, f"sisl_precond_apply_T_{sfx}")(C.byref(ops.struct), x.data_ptr(), y.data_ptr(),
+                                                           scr.data_ptr(), s)
+        else:
+            rc = getattr(L, f"sisl_precond_apply_{sfx}")(C.byref(ops.struct), x.data_ptr(), y.data_ptr(), s)
+        _lib.check(rc, "preconditioner")
+        return y
+
+    def relative_residual(self, solution, rhs_prime, bg_precomputed):
+        """||A x - b|| / ||b|| in the scaled norm (steppers.py:498-527)."""
+        Ax = self.physics.linear_operator(solution, bg_precomputed, self.dt, alpha=self.alpha)
+        num = den = 0.0
+        for k in KEYS:
+            s = self.pi_scale if k == "pi" else 1.0
+            r = (Ax[k] - rhs_prime[k]) * s
+            num += float((r * r).sum())
+            den += float(((rhs_prime[k] * s) ** 2).sum())
+        return (num / max(den, torch.finfo(torch.float64).tiny)) ** 0.5
+
+
+# ---------------------------------------------------------------------------
+class SISLStepper3D:
+    """Semi-implicit semi-Lagrangian step (steppers.py:650-1073)."""
+
+    def __init__(self, physics, dt, alpha=0.55, use_limiter=False, use_checkpointing=False,
+                 solver_tol=1e-4, solver_maxiter=20, solver_restart=20, precision="fp64",
+                 inner_rtol=1e-4, solver_early_exit=False):
+        self.physics, self.dt, self.alpha = physics, dt, alpha
+        self.use_limiter = use_limiter
+        self.precision = precision
+        cdt = torch.float32 if precision == "mixed" else torch.float64
+        self.advector = SemiLagrangianAdvector3D(physics.grid, physics, dt, coord_dtype=cdt)
+        self.ffsl_advector = FluxFormAdvector(physics.grid, dt, periodic_axes=physics.op.periodic_axes)
+        self.implicit_solver = SemiImplicitSolver3D(physics, dt, alpha=alpha, solver_tol=solver_tol,
+                                                    solver_maxiter=solver_maxiter,
+                                                    solver_restart=solver_restart, precision=precision,
+                                                    inner_rtol=inner_rtol, early_exit=solver_early_exit)
+        self.use_checkpointing = use_checkpointing
+        self.tracer_keys = physics.physics_suite.tracer_keys if physics.physics_suite is not None else []
+        self._bg = None
+        self._ext_cache = None
+        self.record = None
+
+    # -- public API (reference shapes) -----------------------------------------
+    def step(self, state, t, forcing, bc_fn, ml_params=None):
+        if "eta_dot" not in state:
+            state["eta_dot"] = torch.zeros_like(state["w"])
+        out = self.step_internal(state_to_internal(state), t, forcing, bc_fn, ml_params)
+        return state_to_reference(out)
+
+    def integrate(self, state, t_start, num_steps, forcing, bc_fn):
+        s = state_to_internal(state)
+        if "eta_dot" not in s:
+            s["eta_dot"] = torch.zeros_like(s["w"])
+        for i in range(int(num_steps)):
+            s = self.step_internal(s, t_start + i * self.dt, forcing, bc_fn)
+        return state_to_reference(s)
+
+    def bg_precomputed(self):
+        if self._bg is None:
+            ph = self.physics
+            self._bg = ph.precompute_bg(ph.bg_reference())
+        return self._bg
+
+    # -- internal step ------------------------------------------------------------
+    def _call_bc(self, bc_fn, st, forcing):
+        owner = getattr(bc_fn, "__self__", None)
+        if owner is not None and getattr(bc_fn, "__name__", "") == "blend" and hasattr(owner, "blend_internal"):
+            # a sponge's blend: run it on internal tensors (no layout round trip)
+            if self._ext_cache is None or self._ext_cache[0] is not forcing:
+                self._ext_cache = (forcing, state_to_internal(forcing))
+            return owner.blend_internal(st, self._ext_cache[1])
+        if getattr(bc_fn, "suetes_internal", False):
+            return bc_fn(st, forcing)
+        out = bc_fn(state_to_reference(st), forcing)
+        return state_to_internal(out)
+
+    def _explicit(self, sp, bg, ml_params):
+        ph = self.physics
+        if ph.physics_suite is None:
+            return ph.get_tendencies(sp, bg, True, ml_params)
+        # physics suites see reference-shaped dicts
+        orig = ph.physics_suite
+
+        class _Adapt:
+            tracer_keys = getattr(orig, "tracer_keys", [])
+
+            def get_explicit_tendencies(self, s, b, interior_mask=None, ml_params=None):
+                r = orig.get_explicit_tendencies(state_to_reference(s), state_to_reference(b),
+                                                 interior_mask=interior_mask, ml_params=ml_params)
+                return state_to_internal(r)
+        ph.physics_suite = _Adapt()
+        try:
+            return ph.get_tendencies(sp, bg, True, ml_params)
+        finally:
+            ph.physics_suite = orig
+
+    def step_internal(self, state, t, forcing, bc_fn, ml_params=None):
+        ph, op, g, c = self.physics, self.physics.op, self.physics.grid, self.physics.c
+        a, dt = self.alpha, self.dt
+        if "eta_dot" not in state:
+            state["eta_dot"] = torch.zeros_like(state["w"])
+        isf = state.get("is_first_step", 1.0)
+        first = float(isf) == 1.0
+        if "u_prev" in state:
+            def ex(k):
+                return state[k] if first else 1.5 * state[k] - 0.5 * state[k + "_prev"]
+            traj = {"u": ex("u"), "v": ex("v"), "w": ex("w"), "eta_dot": ex("eta_dot")}
+        else:
+            traj = state
+        adv = self.advector
+        co = {loc: adv.compute_departure_indices(traj, loc) for loc in ("u", "v", "w", "m")}
+        bg = self.bg_precomputed()
+        th_bg, pi_bg = ph.i_theta_bg, ph.i_pi_bg
+        thp_n = state["th_v"] - th_bg
+        sp_n = {"u": state["u"], "v": state["v"], "w": state["w"], "th_v": state["th_v"],
+                "pi": state["pi"] - pi_bg, "eta_dot": state["eta_dot"],
+                "th_v_prime_u": op.avg(thp_n, 0, "m", "u"),
+                "th_v_prime_v": op.avg(thp_n, 1, "m", "v"),
+                "th_v_prime_w": op.avg(thp_n, 2, "m", "w")}
+        for k in ("theta_surf", "target_th_v"):
+            if k in state:
+                sp_n[k] = state[k]
+        if self.use_checkpointing and torch.is_grad_enabled():
+            from torch.utils.checkpoint import checkpoint
+            tn = checkpoint(self._explicit, sp_n, bg, ml_params, use_reentrant=False)
+        else:
+            tn = self._explicit(sp_n, bg, ml_params)
+        u_in = state["u"] + dt * ((1.0 - a) * tn["u"] + a * tn.get("phys_diff_u", 0.0))
+        v_in = state["v"] + dt * ((1.0 - a) * tn["v"] + a * tn.get("phys_diff_v", 0.0))
+        w_in = state["w"] + dt * ((1.0 - a) * tn["w"] + a * tn.get("phys_diff_w", 0.0))
+        nl_n = ph.nonlinear_pi_tendency(sp_n, thp_n, bg)
+        pi_in = sp_n["pi"] + (1.0 - a) * dt * (tn["pi"] + nl_n)
+        th_in = thp_n + (1.0 - a) * dt * tn["th_v"]
+        tracers_in = {}
+        for key in self.tracer_keys:
+            if key in state:
+                te = tn.get(key, 0.0)
+                if "u_prev" in state and not first:
+                    te = 1.5 * tn.get(key, 0.0) - 0.5 * state.get(f"tend_{key}_prev", 0.0)
+                tracers_in[key] = state[key] + dt * te
+        uw = op.avg(op.avg(state["u"], 0, "u", "m"), 2, "m", "w")
+        vw = op.avg(op.avg(state["v"], 1, "v", "m"), 2, "m", "w")
+        res_n = bg["dz_w_full"] * state["eta_dot"] + uw * g.i_z_xi_w + vw * g.i_z_eta_w - state["w"]
+        lim = self.use_limiter
+        cub = adv.advect_cubic
+        R_eta = -(1.0 - a) * cub(res_n, co["w"], lim)
+        rhs_u = cub(u_in, co["u"], lim)
+        rhs_v = cub(v_in, co["v"], lim)
+        if 0 in op.periodic_axes:
+            rhs_u = _set_slice(rhs_u, 2, -1, rhs_u[:, :, :1])
+        else:
+            rhs_u = _set_slice(_set_slice(rhs_u, 2, 0, state["u"][:, :, :1]), 2, -1, state["u"][:, :, -1:])
+        if 1 in op.periodic_axes:
+            rhs_v = _set_slice(rhs_v, 1, -1, rhs_v[:, :1])
+        else:
+            rhs_v = _set_slice(_set_slice(rhs_v, 1, 0, state["v"][:, :1]), 1, -1, state["v"][:, -1:])
+        rhs_w = cub(w_in, co["w"], lim)
+        rhs_pi = cub(pi_in, co["m"], lim)
+        th_adv = cub(th_in, co["m"], lim)
+        th_next = th_adv + a * dt * tn["th_v"] + th_bg
+        rho_next = self.ffsl_advector.advect_3d_split(state["rho"], state, bg)
+        tracers_next = {}
+        for key in self.tracer_keys:
+            if key in state:
+                rq = self.ffsl_advector.advect_3d_split(state["rho"] * tracers_in[key], state, bg)
+                tracers_next[key] = rq / (rho_next + c.get("eps", 1e-15))
+        R_eta_adv = _zero_ends(R_eta, 0)
+        dthw = op.diff(bg["th_v"], 2, "m", "w") * (g.dz / bg["dz_w_full"])
+        dthm = op.avg(dthw, 2, "w", "m")
+        w_m = op.avg(state["w"], 2, "w", "m")
+        mw0 = g.i_m_factors["w"]
+        w_m_next = w_m
+        guess = sp_n
+        ug, vg = rhs_u, rhs_v
+        zxu = op.diff(g.i_Z_m, 0, "m", "u")
+        zev = op.diff(g.i_Z_m, 1, "m", "v")
+        cad = c["cp"] * a * dt
+        solver = self.implicit_solver
+        for _ in range(2):
+            tth1 = tn["th_v"] + (w_m - w_m_next) * dthm
+            th_next = th_adv + a * dt * tth1 + th_bg
+            thp1 = th_next - th_bg
+            thu1 = op.avg(thp1, 0, "m", "u")
+            thv1 = op.avg(thp1, 1, "m", "v")
+            thw1 = op.avg(thp1, 2, "m", "w")
+            buoy = a * dt * (c["g"] * (thw1 / bg["th_v_w"]))
+            pg = guess["pi"]
+            gx = op.diff(pg, 0, "m", "u")
+            gy = op.diff(pg, 1, "m", "v")
+            gzw = op.diff(pg, 2, "m", "w") * (g.dz / bg["dz_w_full"])
+            gzm = op.avg(gzw, 2, "w", "m")
+            gzu = op.avg(gzm, 0, "m", "u")
+            gzv = op.avg(gzm, 1, "m", "v")
+            nlu = -cad * thu1 * (gx - zxu * gzu)
+            nlv = -cad * thv1 * (gy - zev * gzv)
+            nlw = -cad * thw1 * gzw
+            nld = ph.nonlinear_pi_tendency(guess, thp1, bg)
+            mun, mvn = ph.metric_curvature_tendencies(ug, vg)
+            rhs_u_k = rhs_u + nlu + a * dt * mun
+            rhs_v_k = rhs_v + nlv + a * dt * mvn
+            rhs_w_k = rhs_w + buoy + nlw
+            rhs_pi_k = rhs_pi + a * dt * nld
+            uwg = op.avg(op.avg(ug, 0, "u", "m"), 2, "m", "w")
+            vwg = op.avg(op.avg(vg, 1, "v", "m"), 2, "m", "w")
+            kb = mw0 * (uwg[0] * g.i_z_xi_w[0] + vwg[0] * g.i_z_eta_w[0])
+            rhs_w_k = _set_slice(_set_slice(rhs_w_k, 0, 0, kb[None]), 0, -1, torch.zeros_like(kb[None]))
+            rk = {"u": rhs_u_k, "v": rhs_v_k, "w": rhs_w_k, "pi": rhs_pi_k, "eta_dot": R_eta_adv}
+            nxt = solver.solve(rk, bg, x0=guess)
+            if self.record is not None:
+                self.record.setdefault("solves", []).append((rk, guess, nxt))
+            guess = nxt
+            w_m_next = op.avg(nxt["w"], 2, "w", "m")
+            ug, vg = nxt["u"], nxt["v"]
+        out = {"u": guess["u"], "v": guess["v"], "w": guess["w"], "pi": guess["pi"] + pi_bg,
+               "rho": rho_next, "th_v": th_next, "eta_dot": guess["eta_dot"]}
+        for key in self.tracer_keys:
+            if key in tracers_next:
+                out[key] = tracers_next[key]
+        div = op.diff(out["u"], 0, "u", "m") + op.diff(out["v"], 1, "v", "m")
+        out["u"] = out["u"] + ph.nu_div * dt * op.diff(div, 0, "m", "u")
+        out["v"] = out["v"] + ph.nu_div * dt * op.diff(div, 1, "m", "v")
+        if ph.physics_suite is not None:
+            out = state_to_internal(ph.physics_suite.apply_state_updates(state_to_reference(out)))
+        out = self._call_bc(bc_fn, out, forcing)
+        tmass = {k: out["rho"] * out[k] for k in self.tracer_keys if k in out}
+        out["rho"] = c["p0"] / (c["Rd"] * out["th_v"]) * (out["pi"] ** (c["cvd"] / c["Rd"]))
+        for k, rq in tmass.items():
+            out[k] = rq / (out["rho"] + c.get("eps", 1e-15))
+        out["u_prev"], out["v_prev"], out["w_prev"] = state["u"], state["v"], state["w"]
+        out["eta_dot_prev"] = state["eta_dot"]
+        out["tend_th_v_prev"] = tn["th_v"]
+        for key in self.tracer_keys:
+            out[f"tend_{key}_prev"] = tn.get(key, torch.zeros_like(out[key]))
+        out["is_first_step"] = 0.0
+        return out

@@ -1,0 +1,210 @@
+"""GPU parity of each HIP kernel against the FP64 oracle, through the C ABI.
+
+Tolerances: FP64 kernels agree with the oracle to rel-L2 <= 1e-12 (rounding
+order differs); FP32 kernels to <= 2e-6; transposes pass dot-product tests to
+1e-12 (FP64); the tricubic adjoint is bit-reproducible run to run.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import sisl as O
+from tests.helpers import oracle_case, rel_l2, small_bubble, small_lam
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda"
+
+
+def cases():
+    from suetes_amd.cases import build_case
+    return build_case
+
+
+def to_int(a, dtype=torch.float64):
+    return torch.from_numpy(np.ascontiguousarray(np.asarray(a).transpose(2, 1, 0))).to(DEV, dtype)
+
+
+def from_int(t):
+    return t.detach().cpu().numpy().transpose(2, 1, 0)
+
+
+def pack_ref(d):
+    return torch.cat([to_int(d[k]).reshape(-1) for k in O.KEYS])
+
+
+def unpack_to_ref(x, like_ref):
+    out, o = {}, 0
+    for k in O.KEYS:
+        shp = like_ref[k].shape
+        n = int(np.prod(shp))
+        out[k] = x[o:o + n].view(shp[2], shp[1], shp[0]).cpu().numpy().transpose(2, 1, 0)
+        o += n
+    return out
+
+
+@pytest.mark.parametrize("shape", [(9, 7, 6), (10, 7, 6), (9, 8, 6), (9, 7, 7), (1, 5, 4)])
+@pytest.mark.parametrize("lim", [False, True])
+def test_tricubic_matches_oracle(shape, lim):
+    from suetes_amd.operators import tensor_product_interp_3d
+    rng = np.random.default_rng(hash(shape) % 1000 + lim)
+    f = rng.standard_normal(shape)
+    pts = np.stack([rng.uniform(-2, s + 1, size=shape) for s in shape])
+    pts[:, 0, 0, 0] = [0.0, 0.0, 0.0]
+    pts[:, -1, -1, -1] = [s - 1.0 for s in shape]
+    ref = O.tricubic(f, pts, lim)
+    co = torch.stack([to_int(pts[a]) for a in range(3)])
+    out = tensor_product_interp_3d(to_int(f), co, lim)
+    assert rel_l2(from_int(out), ref) < 1e-13
+    out32 = tensor_product_interp_3d(to_int(f, torch.float32), co.float(), lim)
+    assert rel_l2(from_int(out32), ref) < 2e-6
+
+
+def test_tricubic_adjoint_dot_and_determinism():
+    from suetes_amd.operators import Coords, tensor_product_interp_3d, tricubic_transpose
+    rng = np.random.default_rng(11)
+    shape = (13, 11, 9)
+    pts = np.stack([rng.uniform(-1, s, size=shape) for s in shape])
+    pts[:, :2] = np.round(pts[:, :2])  # clamped edges and integer points
+    co = Coords(torch.stack([to_int(pts[a]) for a in range(3)]))
+    f = to_int(rng.standard_normal(shape))
+    g = to_int(rng.standard_normal(shape))
+    Tf = tensor_product_interp_3d(f, co)
+    Ttg = tricubic_transpose(g, co)
+    lhs = float((Tf * g).sum())
+    rhs = float((f * Ttg).sum())
+    assert abs(lhs - rhs) <= 1e-12 * max(abs(lhs), 1.0)
+    again = tricubic_transpose(g, Coords(co.data.clone()))
+    assert torch.equal(Ttg, again)
+    # dense check on a tiny grid: column q of T^T is the response to e_q
+    small = (4, 3, 5)
+    p2 = np.stack([rng.uniform(-1, s, size=small) for s in small])
+    c2 = Coords(torch.stack([to_int(p2[a]) for a in range(3)]))
+    n = int(np.prod(small))
+    Tm = np.zeros((n, n))
+    for q in range(n):
+        e = np.zeros(n)
+        e[q] = 1.0
+        Tm[:, q] = O.tricubic(e.reshape(small), p2).ravel()
+    gg = rng.standard_normal(small)
+    ref = (Tm.T @ gg.ravel()).reshape(small)
+    assert rel_l2(from_int(tricubic_transpose(to_int(gg), c2)), ref) < 1e-13
+
+
+@pytest.mark.parametrize("cfgf", [small_lam, small_bubble])
+def test_departure_points_match_oracle(cfgf):
+    cfg = cfgf()
+    case = cases()(cfg, device=DEV)
+    go, opo, sto, st_o, bco, ext = oracle_case(cfg)
+    rng = np.random.default_rng(5)
+    s = {"u": rng.uniform(-20, 20, go.Z_u.shape), "v": rng.uniform(-20, 20, go.Z_v.shape),
+         "eta_dot": rng.uniform(-0.05, 0.05, go.Z_w.shape), "w": np.zeros(go.Z_w.shape)}
+    adv_p = case.stepper.advector
+    for loc in ("u", "v", "w", "m"):
+        ref = sto.advector.departure(s, loc)
+        got = adv_p.compute_departure_indices({k: to_int(v) for k, v in s.items()}, loc).data
+        got = np.stack([from_int(got[a]) for a in range(3)])
+        assert rel_l2(got, ref) < 1e-13, loc
+
+
+def _solver_pair(cfg):
+    case = cases()(cfg, device=DEV)
+    go, opo, sto, st_o, bco, ext = oracle_case(cfg)
+    bgo = sto.physics.precompute_bg(sto.physics.bg_reference())
+    A_o, M_o = sto.solver.operators(bgo)
+    sol = case.stepper.implicit_solver
+    sol.prepare(case.stepper.bg_precomputed())
+    return case, go, A_o, M_o, sol
+
+
+def _rand_krylov(go, rng):
+    return {"u": rng.standard_normal(go.Z_u.shape), "v": rng.standard_normal(go.Z_v.shape),
+            "w": rng.standard_normal(go.Z_w.shape), "pi": rng.standard_normal(go.Z_m.shape),
+            "eta_dot": rng.standard_normal(go.Z_w.shape)}
+
+
+@pytest.mark.parametrize("cfgf", [small_lam, small_bubble])
+def test_operator_and_preconditioner_match_oracle(cfgf):
+    case, go, A_o, M_o, sol = _solver_pair(cfgf())
+    rng = np.random.default_rng(3)
+    x = _rand_krylov(go, rng)
+    xp = pack_ref(x)
+    Ax = unpack_to_ref(sol.apply_A(xp), x)
+    ref = A_o(x)
+    for k in O.KEYS:
+        assert rel_l2(Ax[k], ref[k]) < 1e-12, ("A", k)
+    Mx = unpack_to_ref(sol.apply_M(xp), x)
+    ref = M_o(x)
+    for k in O.KEYS:
+        assert rel_l2(Mx[k], ref[k]) < 1e-12, ("M", k)
+    # in-place M (used inside the Arnoldi step)
+    y = xp.clone()
+    from suetes_amd import _lib
+    import ctypes as C
+    _lib.check(_lib.lib().sisl_precond_apply_f64(C.byref(sol.op64.struct), y.data_ptr(), y.data_ptr(),
+                                                 _lib.stream_ptr()), "M")
+    assert torch.equal(y, sol.apply_M(xp))
+    # FP32 kernels
+    sol32 = case.stepper.implicit_solver
+    from suetes_amd.steppers import _OperatorSet
+    co = sol32.physics.operator_coefficients(case.stepper.bg_precomputed(), sol32.dt, sol32.alpha)
+    sol32.op32 = _OperatorSet(sol32.physics.grid, sol32.physics.op.periodic_axes, co, sol32.alpha, torch.float32)
+    Ax32 = unpack_to_ref(sol32.apply_A(xp, dtype=torch.float32).double(), x)
+    for k in O.KEYS:
+        assert rel_l2(Ax32[k], A_o(x)[k]) < 2e-6, ("A32", k)
+    Mx32 = unpack_to_ref(sol32.apply_M(xp, dtype=torch.float32).double(), x)
+    for k in O.KEYS:
+        assert rel_l2(Mx32[k], M_o(x)[k]) < 2e-5, ("M32", k)
+
+
+@pytest.mark.parametrize("cfgf", [small_lam, small_bubble])
+def test_transposes_pass_dot_tests(cfgf):
+    case, go, A_o, M_o, sol = _solver_pair(cfgf())
+    rng = np.random.default_rng(9)
+    x = pack_ref(_rand_krylov(go, rng))
+    y = pack_ref(_rand_krylov(go, rng))
+    for f, name in ((sol.apply_A, "A"), (sol.apply_M, "M")):
+        lhs = float(torch.dot(f(x), y))
+        rhs = float(torch.dot(x, f(y, transpose=True)))
+        assert abs(lhs - rhs) <= 1e-12 * (abs(lhs) + 1e-300), name
+
+
+def test_gmres_solution_matches_oracle():
+    cfg = small_lam()
+    case, go, A_o, M_o, sol = _solver_pair(cfg)
+    rng = np.random.default_rng(21)
+    known = _rand_krylov(go, rng)
+    b = A_o(known)
+    P = O.Packer(b)
+    # reference-semantics solve at a moderate tolerance: same cycles, same iterates up to rounding
+    for tol, maxiter in ((1e-6, 3), (1e-13, 40)):
+        xo = O.gmres(lambda v: P.pack(A_o(P.unpack(v))), P.pack(b), tol=tol, restart=10, maxiter=maxiter,
+                     M=lambda v: P.pack(M_o(P.unpack(v))))
+        sol.solver_tol, sol.solver_restart, sol.solver_maxiter = tol, 10, maxiter
+        sol._bg_id = None
+        sol.prepare(case.stepper.bg_precomputed())
+        xg = sol.solve_flat(pack_ref(b), torch.zeros_like(pack_ref(b)))
+        got = unpack_to_ref(xg, b)
+        ref = P.unpack(xo)
+        for k in O.KEYS:
+            assert rel_l2(got[k], ref[k]) < (1e-11 if tol < 1e-10 else 1e-9), (tol, k)
+    # transposed solve: A^T y = c
+    c = pack_ref(_rand_krylov(go, rng))
+    yT = sol.solve_flat(c, torch.zeros_like(c), transpose=True)
+    r = sol.apply_A(yT, transpose=True) - c
+    assert float(r.norm() / c.norm()) < 1e-9
+
+
+def test_mixed_solver_reaches_tolerance():
+    cfg = small_lam()
+    case, go, A_o, M_o, sol = _solver_pair(cfg)
+    rng = np.random.default_rng(22)
+    b = pack_ref(A_o(_rand_krylov(go, rng)))
+    from suetes_amd.steppers import SemiImplicitSolver3D
+    ms = SemiImplicitSolver3D(sol.physics, sol.dt, sol.alpha, solver_tol=1e-10, solver_maxiter=30,
+                              solver_restart=20, precision="mixed")
+    ms.prepare(case.stepper.bg_precomputed())
+    x = ms.solve_flat(b, torch.zeros_like(b))
+    st = ms.last_stats[-1]
+    assert st["residual"] <= st["ptol"]
+    x64 = sol.solve_flat(b, torch.zeros_like(b))
+    assert float((x - x64).norm() / x64.norm()) < 1e-8

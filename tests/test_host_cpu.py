@@ -1,0 +1,130 @@
+"""Host-side logic of the product on the CPU: the torch glue (grid, background,
+tendencies, torch linear operator, FFSL, sponge) against the oracle, and the
+C-ABI library exports.  No kernel is launched here."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+import torch
+
+import suetes_amd as S
+from suetes_amd import _lib
+from suetes_amd import workloads as W
+from suetes_amd.cases import make_transform
+from oracle import sisl as O
+from tests.helpers import C, oracle_transform, small_lam, rel_l2
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def both(cfg):
+    gk = cfg["grid"]
+    terr = W.make_terrain(cfg["terrain"], gk)
+    go = O.Grid(gk["nx"], gk["ny"], gk["nz"], gk["dx"], gk["dy"], gk["dz"], gk["lat_center"], gk["lon_center"],
+                h_func=terr, transform=oracle_transform(cfg["transform"]))
+    gp = S.RegionalGrid3D(gk["nx"], gk["ny"], gk["nz"], gk["dx"], gk["dy"], gk["dz"], gk["lat_center"],
+                          gk["lon_center"], h_func=terr, transform=make_transform(cfg["transform"]))
+    return go, gp
+
+
+def tnp(t):
+    return t.detach().cpu().numpy()
+
+
+def test_grid_matches_oracle():
+    go, gp = both(small_lam())
+    for name in ("Z_m", "Z_u", "Z_v", "Z_w", "dz_m_full", "dz_w_full", "z_xi_w", "z_eta_w", "f_u", "f_v"):
+        a, b = tnp(getattr(gp, name)), getattr(go, name)
+        np.testing.assert_allclose(a, b, rtol=1e-13, atol=1e-9, err_msg=name)
+    for loc in "muvw":
+        np.testing.assert_allclose(tnp(gp.m_factors[loc]), go.m_factors[loc], rtol=1e-15)
+
+
+def _phys(cfg):
+    go, gp = both(cfg)
+    st = W.initial_state(cfg, go.Z_m, go.x_m, go.y_m)
+    opo, opp = O.Ops(go, cfg["periodic_axes"]), S.CGridOperator3D(gp, cfg["periodic_axes"])
+    core = cfg["core"]
+    kw = dict(dt=core["dt"], N_bv=core["N_bv"], damp_height=5000.0, max_damp=3.0, nu_div_factor=0.1,
+              nu_h_factor=0.1)
+    pho = O.Euler(go, opo, C, initial_era5_state=st, **kw)
+    sti = {k: torch.from_numpy(np.ascontiguousarray(v.transpose(2, 1, 0))) for k, v in st.items()}
+    php = S.Euler3D(gp, opp, C, initial_era5_state={"i_th_v": sti["th_v"], "i_pi": sti["pi"]}, **kw)
+    return go, gp, pho, php, st
+
+
+def _int(a):
+    return torch.from_numpy(np.ascontiguousarray(np.asarray(a).transpose(2, 1, 0)))
+
+
+def test_background_and_tendencies_match_oracle():
+    go, gp, pho, php, st = _phys(small_lam())
+    np.testing.assert_allclose(tnp(php.theta_bg), pho.theta_bg, rtol=1e-14)
+    np.testing.assert_allclose(tnp(php.pi_bg), pho.pi_bg, rtol=1e-14)
+    np.testing.assert_allclose(tnp(php.tau_damp), pho.tau_damp, rtol=1e-12, atol=1e-15)
+    bgo = pho.precompute_bg(pho.bg_reference())
+    bgp = php.precompute_bg(php.bg_reference())
+    rng = np.random.default_rng(0)
+    sp = {"u": rng.standard_normal(go.Z_u.shape), "v": rng.standard_normal(go.Z_v.shape),
+          "w": rng.standard_normal(go.Z_w.shape) * 0.1, "pi": rng.standard_normal(go.Z_m.shape) * 1e-4,
+          "eta_dot": rng.standard_normal(go.Z_w.shape) * 1e-4}
+    thp = rng.standard_normal(go.Z_m.shape)
+    sp["th_v"] = pho.theta_bg + thp
+    sp["th_v_prime_u"] = O.Ops(go).avg(thp, 0, "m", "u")
+    sp["th_v_prime_v"] = O.Ops(go).avg(thp, 1, "m", "v")
+    sp["th_v_prime_w"] = O.Ops(go).avg(thp, 2, "m", "w")
+    spp = {k: _int(v) for k, v in sp.items()}
+    to = pho.get_tendencies(sp, bgo, True)
+    tp = php.get_tendencies(spp, bgp, True)
+    for k in ("u", "v", "w", "pi", "th_v"):
+        assert rel_l2(tnp(tp[k]).transpose(2, 1, 0), to[k]) < 1e-12, k
+    Lo = pho.linear_operator(sp, bgo, 10.0)
+    Lp = php.linear_operator(spp, bgp, 10.0)
+    for k in Lo:
+        assert rel_l2(tnp(Lp[k]).transpose(2, 1, 0), Lo[k]) < 1e-13, k
+    nlo = pho.nonlinear_pi_tendency(sp, thp, bgo)
+    nlp = php.nonlinear_pi_tendency(spp, _int(thp), bgp)
+    assert rel_l2(tnp(nlp).transpose(2, 1, 0), nlo) < 1e-12
+
+
+def test_ffsl_and_sponge_match_oracle():
+    cfg = small_lam()
+    go, gp, pho, php, st = _phys(cfg)
+    bgo = pho.precompute_bg(pho.bg_reference())
+    bgp = php.precompute_bg(php.bg_reference())
+    rng = np.random.default_rng(1)
+    s = {"u": rng.uniform(-30, 30, go.Z_u.shape), "v": rng.uniform(-30, 30, go.Z_v.shape),
+         "eta_dot": rng.uniform(-0.02, 0.02, go.Z_w.shape)}
+    rho = st["rho"] * (1 + 0.01 * rng.standard_normal(go.Z_m.shape))
+    for per in ((), (0, 1)):
+        fo = O.FFSL(go, 10.0, per).advect_3d_split(rho, s, bgo)
+        fp = S.FluxFormAdvector(gp, 10.0, per).advect_3d_split(_int(rho), {k: _int(v) for k, v in s.items()}, bgp)
+        assert rel_l2(tnp(fp).transpose(2, 1, 0), fo) < 1e-13
+    opo, opp = O.Ops(go), S.CGridOperator3D(gp)
+    spo = O.DaviesSponge(go, opo, sponge_depth=4, dt=10.0)
+    spp = S.DaviesSponge(gp, opp, sponge_depth=4, dt=10.0)
+    state = {"u": s["u"], "v": s["v"], "th_v": st["th_v"] + 1.0, "pi": st["pi"] * 1.001, "w": st["w"]}
+    ext = {k: np.zeros_like(v) + 0.5 for k, v in state.items()}
+    bo = spo.blend(state, ext)
+    bp = spp.blend({k: torch.from_numpy(v.copy()) for k, v in state.items()},
+                   {k: torch.from_numpy(v.copy()) for k, v in ext.items()})
+    for k in bo:
+        np.testing.assert_allclose(tnp(bp[k]), bo[k], rtol=1e-15, atol=0)
+
+
+def test_library_exports_every_declared_symbol():
+    hdr = open(os.path.join(ROOT, "include", "suetes_sisl.h")).read()
+    names = sorted(set(re.findall(r"^(?:int|int32_t|int64_t)\s+(sisl_[A-Za-z0-9_]+)\(", hdr, re.M)))
+    assert len(names) >= 20
+    L = _lib.lib()
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    assert set(names) == set(_lib.EXPORTED)
+
+
+def test_kernels_refuse_cpu_tensors():
+    f = torch.zeros(4, 4, 4, dtype=torch.float64)
+    with pytest.raises(RuntimeError, match="HIP device"):
+        S.tensor_product_interp_3d(f, torch.zeros(3, 4, 4, 4, dtype=torch.float64))
