@@ -316,6 +316,18 @@ class HyperFilter:
 # ---------------------------------------------------------------------------
 # Euler3D  (suetes/regional3d/euler.py:13-439)
 # ---------------------------------------------------------------------------
+def exact_mean(a):
+    """Exactly rounded mean of all elements (math.fsum)."""
+    import math
+    a = np.asarray(a, dtype=np.float64).ravel()
+    return math.fsum(a.tolist()) / a.size
+
+
+def exact_mean_levels(a):
+    """Exactly rounded horizontal mean per z level of an (x, y, z) array."""
+    return np.array([exact_mean(a[:, :, k]) for k in range(a.shape[2])])
+
+
 class Euler:
     def __init__(self, grid, op, c, dt, initial_era5_state=None, N_bv=0.01,
                  damp_height=20000.0, max_damp=0.5, nu_div_factor=0.8,
@@ -328,10 +340,12 @@ class Euler:
         self.interior_mask = interior_mask
         Z_m = grid.Z_m
         if initial_era5_state is not None:
-            z1 = np.mean(Z_m, axis=(0, 1))
-            th1 = np.mean(np.asarray(initial_era5_state['th_v']), axis=(0, 1))
+            # horizontal means (jnp.mean) taken exactly rounded (math.fsum) so
+            # the background does not depend on a summation order
+            z1 = exact_mean_levels(Z_m)
+            th1 = exact_mean_levels(np.asarray(initial_era5_state['th_v']))
             self.theta_bg = np.interp(Z_m, z1, th1)
-            pi_top = np.mean(np.asarray(initial_era5_state['pi'])[:, :, -1])
+            pi_top = exact_mean(np.asarray(initial_era5_state['pi'])[:, :, -1])
         else:
             self.theta_0 = 300.0
             if N_bv > 0.0:

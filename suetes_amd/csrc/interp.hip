@@ -285,31 +285,33 @@ __global__ void index_velocity_kernel(Stag s, int loc, const T *__restrict__ u,
     ul = um(i, j, k); vl = vm(i, j, k); wl = em(i, j, k); mf = m2[s.m2(i, j)];
   } else if (loc == 1) {
     ul = u[s.u.at(i, j, k)];
-    T va, vb, ea, eb;
-    if (i == 0 || i == nx) {
-      if (s.px) { va = vm(0, j, k); vb = vm(nx - 1, j, k); ea = em(0, j, k); eb = em(nx - 1, j, k);
-                  vl = (T)0.5 * (va + vb); wl = (T)0.5 * (ea + eb); }
-      else { const int c = i == 0 ? 0 : nx - 1; vl = vm(c, j, k); wl = em(c, j, k); }
+    // op.avg(m->u) pads with the averaged edge values (operators.py:57-63,
+    // 121-132): faces 0 and nx repeat faces 1 and nx-1 unless x is periodic.
+    if (s.px && (i == 0 || i == nx)) {
+      vl = (T)0.5 * (vm(0, j, k) + vm(nx - 1, j, k));
+      wl = (T)0.5 * (em(0, j, k) + em(nx - 1, j, k));
     } else {
-      vl = (T)0.5 * (vm(i - 1, j, k) + vm(i, j, k));
-      wl = (T)0.5 * (em(i - 1, j, k) + em(i, j, k));
+      const int f = min(max(i, 1), nx - 1);
+      vl = (T)0.5 * (vm(f - 1, j, k) + vm(f, j, k));
+      wl = (T)0.5 * (em(f - 1, j, k) + em(f, j, k));
     }
     mf = m2[s.u2(i, j)];
   } else if (loc == 2) {
     vl = v[s.v.at(i, j, k)];
-    if (j == 0 || j == ny) {
-      if (s.py) { ul = (T)0.5 * (um(i, 0, k) + um(i, ny - 1, k)); wl = (T)0.5 * (em(i, 0, k) + em(i, ny - 1, k)); }
-      else { const int c = j == 0 ? 0 : ny - 1; ul = um(i, c, k); wl = em(i, c, k); }
+    if (s.py && (j == 0 || j == ny)) {
+      ul = (T)0.5 * (um(i, 0, k) + um(i, ny - 1, k));
+      wl = (T)0.5 * (em(i, 0, k) + em(i, ny - 1, k));
     } else {
-      ul = (T)0.5 * (um(i, j - 1, k) + um(i, j, k));
-      wl = (T)0.5 * (em(i, j - 1, k) + em(i, j, k));
+      const int f = min(max(j, 1), ny - 1);
+      ul = (T)0.5 * (um(i, f - 1, k) + um(i, f, k));
+      wl = (T)0.5 * (em(i, f - 1, k) + em(i, f, k));
     }
     mf = m2[s.v2(i, j)];
   } else {
     wl = e[s.w.at(i, j, k)];
-    if (k == 0) { ul = um(i, j, 0); vl = vm(i, j, 0); }
-    else if (k == nz) { ul = um(i, j, nz - 1); vl = vm(i, j, nz - 1); }
-    else { ul = (T)0.5 * (um(i, j, k - 1) + um(i, j, k)); vl = (T)0.5 * (vm(i, j, k - 1) + vm(i, j, k)); }
+    const int f = min(max(k, 1), nz - 1);
+    ul = (T)0.5 * (um(i, j, f - 1) + um(i, j, f));
+    vl = (T)0.5 * (vm(i, j, f - 1) + vm(i, j, f));
     mf = m2[s.m2(i, j)];
   }
   out[p] = (ul * mf) * rdx;

@@ -14,16 +14,32 @@ import torch
 
 
 def interp1d(x, xp, fp):
-    """numpy/jax ``interp`` semantics: linear, clamped to the end values."""
+    """``interp`` with NumPy's arithmetic: for xp[j] <= x < xp[j+1],
+    slope*(x - xp[j]) + fp[j]; exact node hits return fp[j]; clamped to the
+    end values outside [xp[0], xp[-1]]."""
     n = xp.shape[0]
-    i = torch.clamp(torch.searchsorted(xp.contiguous(), x.contiguous().reshape(-1), right=True), 1, n - 1)
-    i = i.reshape(x.shape)
-    x0, x1 = xp[i - 1], xp[i]
-    f0, f1 = fp[i - 1], fp[i]
-    t = (x - x0) / (x1 - x0)
-    f = f0 + t * (f1 - f0)
-    f = torch.where(x < xp[0], fp[0].expand_as(f), f)
-    return torch.where(x > xp[-1], fp[-1].expand_as(f), f)
+    j = torch.searchsorted(xp.contiguous(), x.contiguous().reshape(-1), right=True).reshape(x.shape) - 1
+    jc = torch.clamp(j, 0, n - 2)
+    x0, x1, f0, f1 = xp[jc], xp[jc + 1], fp[jc], fp[jc + 1]
+    f = ((f1 - f0) / (x1 - x0)) * (x - x0) + f0
+    f = torch.where(x == x0, f0, f)
+    f = torch.where(j < 0, fp[0].expand_as(f), f)
+    return torch.where(j >= n - 1, fp[-1].expand_as(f), f)
+
+
+def exact_mean(t, dims):
+    """Mean over ``dims`` whose value is exactly rounded (math.fsum on the
+    host, independent of summation order) and whose gradient is the mean's."""
+    import math
+    m = t.mean(dim=dims)
+    a = t.detach().double().cpu().numpy()
+    axes = tuple(d % t.dim() for d in dims)
+    keep = [d for d in range(t.dim()) if d not in axes]
+    a = a.transpose(keep + list(axes)).reshape([t.shape[d] for d in keep] + [-1])
+    flat = a.reshape(-1, a.shape[-1])
+    vals = [math.fsum(r.tolist()) / r.size for r in flat]
+    ex = torch.tensor(vals, dtype=t.dtype).reshape(m.shape).to(t.device)
+    return m + (ex - m).detach()
 
 
 class HyperFilter:
@@ -109,10 +125,10 @@ class Euler3D:
                 initial_era5_state["pi"].permute(2, 1, 0)
             th = th.to(Zm)
             pi = pi.to(Zm)
-            z1 = Zm.mean(dim=(1, 2))
-            th1 = th.mean(dim=(1, 2))
+            z1 = exact_mean(Zm, (1, 2))
+            th1 = exact_mean(th, (1, 2))
             self.i_theta_bg = interp1d(Zm, z1, th1)
-            pi_top = pi[-1].mean()
+            pi_top = exact_mean(pi[-1], (0, 1))
         else:
             self.theta_0 = 300.0
             if N_bv > 0.0:
@@ -128,7 +144,9 @@ class Euler3D:
         dzz = Zm[1:] - Zm[:-1]
         thw = 0.5 * (self.i_theta_bg[1:] + self.i_theta_bg[:-1])
         dpi = -(c["g"] * dzz) / (c["cp"] * thw)
-        csr = torch.flip(torch.cumsum(torch.flip(dpi, (0,)), dim=0), (0,))
+        # hydrostatic integration from the lid, summed sequentially on the host
+        dev = dpi.device
+        csr = torch.flip(torch.cumsum(torch.flip(dpi.cpu(), (0,)), dim=0), (0,)).to(dev)
         top = pi_top * torch.ones_like(Zm[:1])
         self.i_pi_bg = torch.cat([top - csr, top], dim=0)
         Zw = grid.i_Z_w

@@ -398,6 +398,7 @@ class SISLStepper3D:
         self._bg = None
         self._ext_cache = None
         self.record = None
+        self.record_coords = None
 
     # -- public API (reference shapes) -----------------------------------------
     def step(self, state, t, forcing, bc_fn, ml_params=None):
@@ -406,12 +407,22 @@ class SISLStepper3D:
         out = self.step_internal(state_to_internal(state), t, forcing, bc_fn, ml_params)
         return state_to_reference(out)
 
-    def integrate(self, state, t_start, num_steps, forcing, bc_fn):
+    def integrate(self, state, t_start, num_steps, forcing, bc_fn, checkpoint=False):
+        """``num_steps`` steps (steppers.py:692-714).  With ``checkpoint=True``
+        each step is a torch checkpoint whose recomputation in the backward
+        pass replays the stored departure points (memory O(one step) plus
+        the per-step state and departure points)."""
         s = state_to_internal(state)
         if "eta_dot" not in s:
             s["eta_dot"] = torch.zeros_like(s["w"])
         for i in range(int(num_steps)):
-            s = self.step_internal(s, t_start + i * self.dt, forcing, bc_fn)
+            t = t_start + i * self.dt
+            if checkpoint and torch.is_grad_enabled():
+                from torch.utils.checkpoint import checkpoint as ckpt
+                co = self.departure_points(s)
+                s = ckpt(self.step_internal, s, t, forcing, bc_fn, None, co, use_reentrant=False)
+            else:
+                s = self.step_internal(s, t, forcing, bc_fn)
         return state_to_reference(s)
 
     def bg_precomputed(self):
@@ -453,11 +464,10 @@ class SISLStepper3D:
         finally:
             ph.physics_suite = orig
 
-    def step_internal(self, state, t, forcing, bc_fn, ml_params=None):
-        ph, op, g, c = self.physics, self.physics.op, self.physics.grid, self.physics.c
-        a, dt = self.alpha, self.dt
-        if "eta_dot" not in state:
-            state["eta_dot"] = torch.zeros_like(state["w"])
+    def departure_points(self, state):
+        """Departure points of the four staggerings for the step that starts
+        from ``state`` (internal dict): trajectory winds extrapolated to n+1/2
+        (steppers.py:748-762), then two midpoint iterations (`:764-767`)."""
         isf = state.get("is_first_step", 1.0)
         first = float(isf) == 1.0
         if "u_prev" in state:
@@ -466,8 +476,21 @@ class SISLStepper3D:
             traj = {"u": ex("u"), "v": ex("v"), "w": ex("w"), "eta_dot": ex("eta_dot")}
         else:
             traj = state
+        return {loc: self.advector.compute_departure_indices(traj, loc) for loc in ("u", "v", "w", "m")}
+
+    def step_internal(self, state, t, forcing, bc_fn, ml_params=None, coords=None):
+        """One step on an internal dict.  ``coords`` (from
+        :meth:`departure_points`) replays stored departure points, e.g. when a
+        checkpointed step is recomputed in the backward pass."""
+        ph, op, g, c = self.physics, self.physics.op, self.physics.grid, self.physics.c
+        a, dt = self.alpha, self.dt
+        if "eta_dot" not in state:
+            state["eta_dot"] = torch.zeros_like(state["w"])
+        first = float(state.get("is_first_step", 1.0)) == 1.0
         adv = self.advector
-        co = {loc: adv.compute_departure_indices(traj, loc) for loc in ("u", "v", "w", "m")}
+        co = coords if coords is not None else self.departure_points(state)
+        if self.record_coords is not None:
+            self.record_coords.append(co)
         bg = self.bg_precomputed()
         th_bg, pi_bg = ph.i_theta_bg, ph.i_pi_bg
         thp_n = state["th_v"] - th_bg
