@@ -87,9 +87,12 @@ int64_t sisl_krylov_size(const SislGrid *g);
 /* ---- (a) semi-Lagrangian interpolation ------------------------------- */
 
 /* tensor_product_interp_3d (suetes/regional3d/operators.py:146-209):
- * Catmull-Rom tricubic of `field` (dims fnx,fny,fnz) at index-space points
- * coords[0..2][n] (x, y, z components, each n = fnx*fny*fnz long, one per
- * output point in the internal layout), optional 8-corner clip. */
+ * Catmull-Rom tricubic of `field` (dims fnx,fny,fnz) at the departure points
+ * of every output point, optional 8-corner clip.  `coords` holds departure
+ * OFFSETS in index space, d = x_d - x_a (x, y, z components, each
+ * n = fnx*fny*fnz long, in the internal layout): the point of output (i,j,k)
+ * is (i + d_x, j + d_y, k + d_z), clamped to the field like the reference's
+ * coordinates.  Offsets keep full precision in FP32. */
 int sisl_tricubic_f64(int32_t fnx, int32_t fny, int32_t fnz, const double *field,
                       const double *coords, double *out, int32_t use_limiter, void *stream);
 int sisl_tricubic_f32(int32_t fnx, int32_t fny, int32_t fnz, const float *field,
@@ -116,9 +119,10 @@ int sisl_tricubic_adj_f32(int32_t fnx, int32_t fny, int32_t fnz, const float *co
                           float *gfield, void *stream);
 
 /* _get_index_velocities + compute_departure_indices
- * (suetes/regional3d/steppers.py:276-360): departure coordinates at
- * staggering loc (0=m, 1=u, 2=v, 3=w) from trajectory winds u, v, eta_dot.
- * m2 is the 2-D map factor at loc; coords_out is 3 x (loc size). */
+ * (suetes/regional3d/steppers.py:276-360): departure points at staggering
+ * loc (0=m, 1=u, 2=v, 3=w) from trajectory winds u, v, eta_dot, written as
+ * offsets x_d - x_a (3 x loc size).  m2 is the 2-D map factor at loc;
+ * scratch3 holds 3 x loc size elements. */
 int sisl_departure_f64(const SislGrid *g, int32_t loc, const double *u, const double *v,
                        const double *eta, const double *m2, double dx, double dy, double dt,
                        int32_t iterations, double *scratch3, double *coords_out, void *stream);

@@ -98,8 +98,14 @@ __device__ inline T wave_sum(T v) {
 namespace sisl {
 // Launchers shared between translation units (explicitly instantiated for
 // float and double next to their kernels).
-template <typename T> int launch_A(const OpT<T> &op, const T *x, T *y, hipStream_t s);
-template <typename T> int launch_AT(const OpT<T> &op, const T *x, T *y, T *scratch, hipStream_t s);
-template <typename T> int launch_M(const OpT<T> &op, const T *r, T *z, hipStream_t s);
-template <typename T> int launch_MT(const OpT<T> &op, const T *r, T *z, T *scratch, hipStream_t s);
+// `stop` (device int, optional): the kernels return at once when *stop != 0
+// (used to retire the remaining Arnoldi steps of a converged cycle).
+template <typename T>
+int launch_A(const OpT<T> &op, const T *x, T *y, hipStream_t s, const int *stop = nullptr);
+template <typename T>
+int launch_AT(const OpT<T> &op, const T *x, T *y, T *scratch, hipStream_t s, const int *stop = nullptr);
+template <typename T>
+int launch_M(const OpT<T> &op, const T *r, T *z, hipStream_t s, const int *stop = nullptr);
+template <typename T>
+int launch_MT(const OpT<T> &op, const T *r, T *z, T *scratch, hipStream_t s, const int *stop = nullptr);
 }  // namespace sisl

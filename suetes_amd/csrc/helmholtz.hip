@@ -41,7 +41,9 @@ struct AView {
 
 template <typename T>
 __global__ void __launch_bounds__(256) helmholtz_kernel(OpT<T> o, const T *__restrict__ x,
-                                                        T *__restrict__ y) {
+                                                        T *__restrict__ y,
+                                                        const int *__restrict__ stop) {
+  if (stop && *stop) return;
   const Stag &s = o.s;
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= s.n) return;
@@ -146,7 +148,9 @@ template <typename T>
 __global__ void __launch_bounds__(256) helmholtz_T1_kernel(OpT<T> o, const T *__restrict__ y,
                                                            T *__restrict__ gzm_b,
                                                            T *__restrict__ um_b,
-                                                           T *__restrict__ vm_b) {
+                                                           T *__restrict__ vm_b,
+                                                           const int *__restrict__ stop) {
+  if (stop && *stop) return;
   const Stag &s = o.s;
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= s.m.n()) return;
@@ -179,7 +183,9 @@ __global__ void __launch_bounds__(256) helmholtz_T2_kernel(OpT<T> o, const T *__
                                                            const T *__restrict__ gzm_b,
                                                            const T *__restrict__ um_b,
                                                            const T *__restrict__ vm_b,
-                                                           T *__restrict__ xo) {
+                                                           T *__restrict__ xo,
+                                                           const int *__restrict__ stop) {
+  if (stop && *stop) return;
   const Stag &s = o.s;
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= s.n) return;
@@ -262,27 +268,30 @@ __global__ void __launch_bounds__(256) helmholtz_T2_kernel(OpT<T> o, const T *__
 }
 
 template <typename T>
-int launch_A(const OpT<T> &o, const T *x, T *y, hipStream_t st) {
-  hipLaunchKernelGGL(helmholtz_kernel<T>, dim3(grid_for(o.s.n)), dim3(kBlock), 0, st, o, x, y);
+int launch_A(const OpT<T> &o, const T *x, T *y, hipStream_t st, const int *stop) {
+  hipLaunchKernelGGL(helmholtz_kernel<T>, dim3(grid_for(o.s.n)), dim3(kBlock), 0, st, o, x, y, stop);
   SISL_CHECK_LAUNCH();
   return SISL_OK;
 }
 
 template <typename T>
-int launch_AT(const OpT<T> &o, const T *y, T *x, T *scratch, hipStream_t st) {
+int launch_AT(const OpT<T> &o, const T *y, T *x, T *scratch, hipStream_t st, const int *stop) {
   const int64_t nm = o.s.m.n();
   T *g = scratch, *um = scratch + nm, *vm = scratch + 2 * nm;
-  hipLaunchKernelGGL(helmholtz_T1_kernel<T>, dim3(grid_for(nm)), dim3(kBlock), 0, st, o, y, g, um, vm);
+  hipLaunchKernelGGL(helmholtz_T1_kernel<T>, dim3(grid_for(nm)), dim3(kBlock), 0, st, o, y, g, um, vm,
+                     stop);
   hipLaunchKernelGGL(helmholtz_T2_kernel<T>, dim3(grid_for(o.s.n)), dim3(kBlock), 0, st, o, y,
-                     (const T *)g, (const T *)um, (const T *)vm, x);
+                     (const T *)g, (const T *)um, (const T *)vm, x, stop);
   SISL_CHECK_LAUNCH();
   return SISL_OK;
 }
 
-template int launch_A<double>(const OpT<double> &, const double *, double *, hipStream_t);
-template int launch_A<float>(const OpT<float> &, const float *, float *, hipStream_t);
-template int launch_AT<double>(const OpT<double> &, const double *, double *, double *, hipStream_t);
-template int launch_AT<float>(const OpT<float> &, const float *, float *, float *, hipStream_t);
+template int launch_A<double>(const OpT<double> &, const double *, double *, hipStream_t, const int *);
+template int launch_A<float>(const OpT<float> &, const float *, float *, hipStream_t, const int *);
+template int launch_AT<double>(const OpT<double> &, const double *, double *, double *, hipStream_t,
+                               const int *);
+template int launch_AT<float>(const OpT<float> &, const float *, float *, float *, hipStream_t,
+                              const int *);
 
 }  // namespace sisl
 

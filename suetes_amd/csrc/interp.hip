@@ -18,15 +18,24 @@ __device__ __forceinline__ void cr_weights(T t, T w[4]) {
   w[3] = (T)0.5 * t3 - (T)0.5 * t2;
 }
 
-// Clamp to [0, n-1], split into base index and fraction (operators.py:169-174).
+// Departure points are stored as offsets d = x_d - x_a from the arrival
+// index X (exact in FP32 too: |d| is a few cells, while absolute indices up
+// to 256 would keep only ~1.5e-5 cells in FP32).  Split X + d into base
+// index and fraction, clamped to [0, n-1] like the reference's coordinates
+// (operators.py:169-174).
 template <typename T>
-__device__ __forceinline__ int clamp_split(T c, int n, T &t) {
-  T cc = c < (T)0 ? (T)0 : c;
-  const T hi = (T)(n - 1);
-  cc = cc > hi ? hi : cc;
-  const T f = floor(cc);
-  t = cc - f;
-  return (int)f;
+__device__ __forceinline__ int offset_split(int X, T d, int n, T &t) {
+  const T fl = floor(d);
+  int b = X + (int)fl;
+  t = d - fl;
+  if (b < 0) {
+    b = 0;
+    t = (T)0;
+  } else if (b > n - 1 || (b == n - 1 && t > (T)0)) {
+    b = n - 1;
+    t = (T)0;
+  }
+  return b;
 }
 
 template <typename T>
@@ -36,10 +45,11 @@ __global__ void __launch_bounds__(256) tricubic_kernel(Dims d, const T *__restri
   const int64_t n = d.n();
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
+  const int X = (int)(p % d.nx), Y = (int)((p / d.nx) % d.ny), Z = (int)(p / ((int64_t)d.nx * d.ny));
   T tx, ty, tz;
-  const int x0 = clamp_split(co[p], d.nx, tx);
-  const int y0 = clamp_split(co[n + p], d.ny, ty);
-  const int z0 = clamp_split(co[2 * n + p], d.nz, tz);
+  const int x0 = offset_split(X, co[p], d.nx, tx);
+  const int y0 = offset_split(Y, co[n + p], d.ny, ty);
+  const int z0 = offset_split(Z, co[2 * n + p], d.nz, tz);
   int ix[4] = {max(x0 - 1, 0), x0, min(x0 + 1, d.nx - 1), min(x0 + 2, d.nx - 1)};
   int iy[4] = {max(y0 - 1, 0), y0, min(y0 + 1, d.ny - 1), min(y0 + 2, d.ny - 1)};
   int iz[4] = {max(z0 - 1, 0), z0, min(z0 + 1, d.nz - 1), min(z0 + 2, d.nz - 1)};
@@ -90,10 +100,11 @@ __global__ void adj_key_kernel(Dims d, const T *__restrict__ co, int32_t *__rest
   const int64_t n = d.n();
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
+  const int X = (int)(p % d.nx), Y = (int)((p / d.nx) % d.ny), Z = (int)(p / ((int64_t)d.nx * d.ny));
   T t;
-  const int x0 = clamp_split(co[p], d.nx, t);
-  const int y0 = clamp_split(co[n + p], d.ny, t);
-  const int z0 = clamp_split(co[2 * n + p], d.nz, t);
+  const int x0 = offset_split(X, co[p], d.nx, t);
+  const int y0 = offset_split(Y, co[n + p], d.ny, t);
+  const int z0 = offset_split(Z, co[2 * n + p], d.nz, t);
   const int32_t k = (int32_t)d.at(x0, y0, z0);
   key[p] = k;
   atomicAdd(&count[k], 1);
@@ -247,10 +258,12 @@ __global__ void __launch_bounds__(256) tricubic_adj_kernel(Dims d, const T *__re
         const int32_t s = start[key], e = start[key + 1];
         for (int32_t r = s; r < e; ++r) {
           const int64_t p = perm[r];
+          const int X = (int)(p % d.nx), Y = (int)((p / d.nx) % d.ny);
+          const int Z = (int)(p / ((int64_t)d.nx * d.ny));
           T tx, ty, tz;
-          clamp_split(co[p], d.nx, tx);
-          clamp_split(co[n + p], d.ny, ty);
-          clamp_split(co[2 * n + p], d.nz, tz);
+          offset_split(X, co[p], d.nx, tx);
+          offset_split(Y, co[n + p], d.ny, ty);
+          offset_split(Z, co[2 * n + p], d.nz, tz);
           T wx[4], wy[4], wz[4];
           cr_weights(tx, wx);
           cr_weights(ty, wy);
@@ -319,16 +332,26 @@ __global__ void index_velocity_kernel(Stag s, int loc, const T *__restrict__ u,
   out[2 * n + p] = wl;
 }
 
-// map_coordinates(order=1, mode='nearest') of one field (steppers.py:367).
+// map_coordinates(order=1, mode='nearest') of one field at X + d
+// (steppers.py:367): per axis i = floor(x), t = x - i, indices clamped.
 template <typename T>
-__device__ __forceinline__ T trilinear_nearest(const T *__restrict__ f, Dims d, T x, T y, T z) {
-  const T fx = floor(x), fy = floor(y), fz = floor(z);
-  const T wx1 = x - fx, wy1 = y - fy, wz1 = z - fz;
+__device__ __forceinline__ void lin_split(int X, T d, int n, int &i0, int &i1, T &w1) {
+  const T fl = floor(d);
+  const int i = X + (int)fl;
+  w1 = d - fl;
+  i0 = min(max(i, 0), n - 1);
+  i1 = min(max(i + 1, 0), n - 1);
+}
+
+template <typename T>
+__device__ __forceinline__ T trilinear_nearest(const T *__restrict__ f, Dims d, int X, int Y, int Z,
+                                               T dx, T dy, T dz) {
+  int x0, x1, y0, y1, z0, z1;
+  T wx1, wy1, wz1;
+  lin_split(X, dx, d.nx, x0, x1, wx1);
+  lin_split(Y, dy, d.ny, y0, y1, wy1);
+  lin_split(Z, dz, d.nz, z0, z1, wz1);
   const T wx0 = (T)1 - wx1, wy0 = (T)1 - wy1, wz0 = (T)1 - wz1;
-  const int ix = (int)fx, iy = (int)fy, iz = (int)fz;
-  const int x0 = min(max(ix, 0), d.nx - 1), x1 = min(max(ix + 1, 0), d.nx - 1);
-  const int y0 = min(max(iy, 0), d.ny - 1), y1 = min(max(iy + 1, 0), d.ny - 1);
-  const int z0 = min(max(iz, 0), d.nz - 1), z1 = min(max(iz + 1, 0), d.nz - 1);
   T r = 0;
   r += f[d.at(x0, y0, z0)] * (wx0 * wy0 * wz0);
   r += f[d.at(x0, y0, z1)] * (wx0 * wy0 * wz1);
@@ -341,27 +364,26 @@ __device__ __forceinline__ T trilinear_nearest(const T *__restrict__ f, Dims d, 
   return r;
 }
 
-// Midpoint trajectory iteration (steppers.py:318-360).
+// Midpoint trajectory iteration (steppers.py:318-360); writes the departure
+// offsets -alpha = x_d - x_a.
 template <typename T>
 __global__ void departure_kernel(Dims L, const T *__restrict__ vel, T dt, int iters,
                                  T *__restrict__ out) {
   const int64_t n = L.n();
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
-  const T X = (T)(int)(p % L.nx);
-  const T Y = (T)(int)((p / L.nx) % L.ny);
-  const T Z = (T)(int)(p / ((int64_t)L.nx * L.ny));
+  const int X = (int)(p % L.nx), Y = (int)((p / L.nx) % L.ny), Z = (int)(p / ((int64_t)L.nx * L.ny));
   T ax = dt * vel[p], ay = dt * vel[n + p], az = dt * vel[2 * n + p];
   for (int it = 0; it < iters; ++it) {
-    const T mx = X - (T)0.5 * ax, my = Y - (T)0.5 * ay, mz = Z - (T)0.5 * az;
-    const T um = trilinear_nearest(vel, L, mx, my, mz);
-    const T vm = trilinear_nearest(vel + n, L, mx, my, mz);
-    const T wm = trilinear_nearest(vel + 2 * n, L, mx, my, mz);
+    const T mx = (T)-0.5 * ax, my = (T)-0.5 * ay, mz = (T)-0.5 * az;
+    const T um = trilinear_nearest(vel, L, X, Y, Z, mx, my, mz);
+    const T vm = trilinear_nearest(vel + n, L, X, Y, Z, mx, my, mz);
+    const T wm = trilinear_nearest(vel + 2 * n, L, X, Y, Z, mx, my, mz);
     ax = dt * um; ay = dt * vm; az = dt * wm;
   }
-  out[p] = X - ax;
-  out[n + p] = Y - ay;
-  out[2 * n + p] = Z - az;
+  out[p] = -ax;
+  out[n + p] = -ay;
+  out[2 * n + p] = -az;
 }
 
 template <typename T>

@@ -27,39 +27,61 @@ constexpr int kPStride = kMaxRestart + 2;  // partial sums per block
 // Device-side scalar state of one solve.
 struct Small {
   double H[kMaxRestart * (kMaxRestart + 1)];  // row j = Arnoldi step j (JAX layout)
-  double h1[kMaxRestart + 2];
-  double h2[kMaxRestart + 2];
-  double nn0, nn1, inv, target;
+  double h1[kMaxRestart + 2];                 // first projection (+ w.w in slot nv)
+  double h2[kMaxRestart + 2];                 // re-orthogonalisation projection
+  double nn1a[1], nn1b[1];                    // ||w||^2 after pass 1 / pass 2
+  double inv, target;
   double g[kMaxRestart + 1], cs[kMaxRestart], sn[kMaxRestart];
   double y[kMaxRestart];
   double scal[4];
-  int broken, steps, pad0, pad1;
+  int broken;   // cycle stopped (breakdown or early exit): later kernels skip
+  int steps;    // Arnoldi steps completed in this cycle
+  int gate2;    // second Gram-Schmidt pass required
+  int pad;
 };
 
 // ---- block reduction helpers -------------------------------------------------
-template <int NV>
-__device__ __forceinline__ void block_store_partials(double (&acc)[NV], int nv, double *partials) {
-  __shared__ double red[kBlock / 64][NV];
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+// One value per thread -> partials[slot][block] (fixed-order, deterministic).
+__device__ __forceinline__ void block_store_slot(double v, int slot, double *partials) {
+  __shared__ double red1[kBlock / 64];
+  const double s = wave_sum(v);
+  if ((threadIdx.x & 63) == 0) red1[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0;
 #pragma unroll
-  for (int v = 0; v < NV; ++v) {
-    if (v < nv) {
-      const double s = wave_sum(acc[v]);
-      if (lane == 0) red[wid][v] = s;
-    }
+    for (int q = 0; q < kBlock / 64; ++q) t += red1[q];
+    partials[(int64_t)slot * kRedGrid + blockIdx.x] = t;
   }
   __syncthreads();
-  for (int v = threadIdx.x; v < nv; v += blockDim.x) {
-    double s = 0;
+}
+
+// kGrp values per thread -> partials[v0 + r][block] for r < cnt.
+template <int G>
+__device__ __forceinline__ void block_store_group(double (&acc)[G], int v0, int cnt, double *partials) {
+  __shared__ double red[kBlock / 64][G];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
-    for (int q = 0; q < kBlock / 64; ++q) s += red[q][v];
-    partials[(int64_t)v * kRedGrid + blockIdx.x] = s;
+  for (int r = 0; r < G; ++r) {
+    const double s = wave_sum(acc[r]);
+    if (lane == 0) red[wid][r] = s;
   }
+  __syncthreads();
+  if (threadIdx.x < cnt) {
+    double t = 0;
+#pragma unroll
+    for (int q = 0; q < kBlock / 64; ++q) t += red[q][threadIdx.x];
+    partials[(int64_t)(v0 + threadIdx.x) * kRedGrid + blockIdx.x] = t;
+  }
+  __syncthreads();
 }
 
 // out[v] = sum over blocks of partials[v][b], fixed order.  grid = nv blocks.
 __global__ void __launch_bounds__(256) reduce_partials_kernel(const double *__restrict__ partials,
-                                                              double *__restrict__ out) {
+                                                              double *__restrict__ out,
+                                                              const int *__restrict__ stop,
+                                                              const int *__restrict__ gate) {
+  if ((stop && *stop) || (gate && !*gate)) return;
   __shared__ double red[kBlock / 64];
   const int v = blockIdx.x;
   double s = 0;
@@ -74,70 +96,160 @@ __global__ void __launch_bounds__(256) reduce_partials_kernel(const double *__re
   }
 }
 
-// acc[0..nv-1] = V[v] . w ; acc[nv] = w . w
-template <typename TV, int MAXV>
-__global__ void __launch_bounds__(256) dots_kernel(const TV *__restrict__ V, int64_t ldv, int nv,
-                                                   const TV *__restrict__ w, int64_t n,
-                                                   double *__restrict__ partials) {
-  double acc[MAXV + 1];
+// 16-byte vector access: 2 doubles or 4 floats per lane per load.
+template <typename TV> struct Vec;
+template <> struct Vec<double> { static constexpr int N = 2; using T = double2; };
+template <> struct Vec<float> { static constexpr int N = 4; using T = float4; };
+
+template <typename TV>
+__device__ __forceinline__ void vload(const TV *__restrict__ p, int64_t q, TV (&a)[Vec<TV>::N]) {
+  union { typename Vec<TV>::T v; TV e[Vec<TV>::N]; } u;
+  u.v = reinterpret_cast<const typename Vec<TV>::T *>(p)[q];
 #pragma unroll
-  for (int v = 0; v <= MAXV; ++v) acc[v] = 0;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += stride) {
-    const double we = (double)w[e];
+  for (int k = 0; k < Vec<TV>::N; ++k) a[k] = u.e[k];
+}
+template <typename TV>
+__device__ __forceinline__ void vstore(TV *__restrict__ p, int64_t q, const TV (&a)[Vec<TV>::N]) {
+  union { typename Vec<TV>::T v; TV e[Vec<TV>::N]; } u;
 #pragma unroll
-    for (int v = 0; v < MAXV; ++v)
-      if (v < nv) acc[v] += (double)V[(int64_t)v * ldv + e] * we;
-    acc[MAXV] += we * we;
-  }
-  const double ww = acc[MAXV];  // w.w goes to slot nv
-#pragma unroll
-  for (int v = 0; v <= MAXV; ++v)
-    if (v == nv) acc[v] = ww;
-  block_store_partials<MAXV + 1>(acc, nv + 1, partials);
+  for (int k = 0; k < Vec<TV>::N; ++k) u.e[k] = a[k];
+  reinterpret_cast<typename Vec<TV>::T *>(p)[q] = u.v;
 }
 
-// w <- w - sum_v h[v] V[v]; then acc[v] = V[v] . w (if reproj), acc[last] = w . w
-template <typename TV, int MAXV>
+// Vectors are visited in groups of 8 whose loads carry no branches (indices
+// past nv are clamped to the last vector and their contribution masked), so
+// each wave keeps 8 x 1 KB in flight per group.  Group loops are runtime
+// loops, which keeps register use (and so occupancy) independent of nv.
+constexpr int kGrp = 8;
+
+// Uniform early exit of a pass: skip when *stop != 0, or when a gate exists
+// and *gate == 0 (the second Gram-Schmidt pass).
+__device__ __forceinline__ bool pass_skipped(const int *stop, const int *gate) {
+  return (stop && *stop) || (gate && !*gate);
+}
+
+// partial[v][block] = sum over the block's elements of V[v] . w for every v in
+// [v0, v0+8) of every group, plus (slot nv) w . w.  FP64 accumulation.
+template <typename TV>
+__global__ void __launch_bounds__(256) dots_kernel(const TV *__restrict__ V, int64_t ldv, int nv,
+                                                   const TV *__restrict__ w, int64_t n,
+                                                   double *__restrict__ partials,
+                                                   const int *__restrict__ stop,
+                                                   const int *__restrict__ gate) {
+  if (pass_skipped(stop, gate)) return;
+  constexpr int N = Vec<TV>::N;
+  const int64_t nq = n / N;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t tail = n - nq * N;
+  {  // w . w
+    double acc[1] = {0.0};
+    for (int64_t q = t0; q < nq; q += stride) {
+      TV we[N];
+      vload(w, q, we);
+#pragma unroll
+      for (int k = 0; k < N; ++k) acc[0] += (double)we[k] * (double)we[k];
+    }
+    if (t0 < tail) acc[0] += (double)w[nq * N + t0] * (double)w[nq * N + t0];
+    block_store_slot(acc[0], nv, partials);
+  }
+#pragma unroll 1
+  for (int g = 0; g * kGrp < nv; ++g) {
+    double acc[kGrp];
+#pragma unroll
+    for (int r = 0; r < kGrp; ++r) acc[r] = 0;
+    const TV *base[kGrp];
+#pragma unroll
+    for (int r = 0; r < kGrp; ++r) base[r] = V + (int64_t)min(g * kGrp + r, nv - 1) * ldv;
+    for (int64_t q = t0; q < nq; q += stride) {
+      TV we[N], x[kGrp][N];
+      vload(w, q, we);
+#pragma unroll
+      for (int r = 0; r < kGrp; ++r) vload(base[r], q, x[r]);
+#pragma unroll
+      for (int r = 0; r < kGrp; ++r)
+#pragma unroll
+        for (int k = 0; k < N; ++k) acc[r] += (double)x[r][k] * (double)we[k];
+    }
+    if (t0 < tail) {
+      const int64_t e = nq * N + t0;
+#pragma unroll
+      for (int r = 0; r < kGrp; ++r) acc[r] += (double)base[r][e] * (double)w[e];
+    }
+    const int cnt = min(kGrp, nv - g * kGrp);
+    block_store_group(acc, g * kGrp, cnt, partials);
+  }
+}
+
+// w <- w - sum_v h[v] V[v] (h read from device memory), slot 0 <- w . w
+template <typename TV>
 __global__ void __launch_bounds__(256) update_kernel(const TV *__restrict__ V, int64_t ldv, int nv,
                                                      const double *__restrict__ h,
-                                                     TV *__restrict__ w, int64_t n, int reproj,
-                                                     double *__restrict__ partials) {
-  __shared__ double hv[MAXV];
-  for (int v = threadIdx.x; v < MAXV; v += blockDim.x) hv[v] = v < nv ? h[v] : 0.0;
+                                                     TV *__restrict__ w, int64_t n,
+                                                     double *__restrict__ partials,
+                                                     const int *__restrict__ stop,
+                                                     const int *__restrict__ gate) {
+  if (pass_skipped(stop, gate)) return;
+  constexpr int N = Vec<TV>::N;
+  __shared__ double hs[kMaxRestart + 8];
+  const int nvr = ((nv + kGrp - 1) / kGrp) * kGrp;
+  for (int v = threadIdx.x; v < nvr; v += blockDim.x) hs[v] = v < nv ? h[v] : 0.0;
   __syncthreads();
-  double acc[MAXV + 1];
-#pragma unroll
-  for (int v = 0; v <= MAXV; ++v) acc[v] = 0;
+  const int64_t nq = n / N;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += stride) {
-    double vals[MAXV];
-    double s = (double)w[e];
+  const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double ww = 0;
+  for (int64_t q = t0; q < nq; q += stride) {
+    TV we[N];
+    vload(w, q, we);
+    double s[N];
 #pragma unroll
-    for (int v = 0; v < MAXV; ++v) {
-      if (v < nv) {
-        vals[v] = (double)V[(int64_t)v * ldv + e];
-        s -= hv[v] * vals[v];
-      } else {
-        vals[v] = 0;
+    for (int k = 0; k < N; ++k) s[k] = (double)we[k];
+#pragma unroll 1
+    for (int g = 0; g * kGrp < nv; ++g) {
+      TV x[kGrp][N];
+#pragma unroll
+      for (int r = 0; r < kGrp; ++r) vload(V + (int64_t)min(g * kGrp + r, nv - 1) * ldv, q, x[r]);
+#pragma unroll
+      for (int r = 0; r < kGrp; ++r) {
+        const double hv = hs[g * kGrp + r];  // zero past nv
+#pragma unroll
+        for (int k = 0; k < N; ++k) s[k] -= hv * (double)x[r][k];
       }
     }
-    const TV sn = (TV)s;
-    w[e] = sn;
-    const double sd = (double)sn;
-    if (reproj) {
+    TV sn[N];
 #pragma unroll
-      for (int v = 0; v < MAXV; ++v)
-        if (v < nv) acc[v] += vals[v] * sd;
-    }
-    acc[MAXV] += sd * sd;
+    for (int k = 0; k < N; ++k) sn[k] = (TV)s[k];
+    vstore(w, q, sn);
+#pragma unroll
+    for (int k = 0; k < N; ++k) ww += (double)sn[k] * (double)sn[k];
   }
-  const int nout = reproj ? nv : 0;
-  const double ww = acc[MAXV];
-#pragma unroll
-  for (int v = 0; v <= MAXV; ++v)
-    if (v == nout) acc[v] = ww;
-  block_store_partials<MAXV + 1>(acc, nout + 1, partials);
+  if (t0 < n - nq * N) {  // scalar tail
+    const int64_t e = nq * N + t0;
+    double sd = (double)w[e];
+    for (int v = 0; v < nv; ++v) sd -= hs[v] * (double)V[(int64_t)v * ldv + e];
+    const TV sn = (TV)sd;
+    w[e] = sn;
+    ww += (double)sn * (double)sn;
+  }
+  block_store_slot(ww, 0, partials);
+}
+
+// JAX _iterative_classical_gram_schmidt loop condition: a second pass runs
+// when ||r|| < ||q||/sqrt(2) after the first (norms below eps count as 0).
+__global__ void decide_kernel(Small *sm, int nv) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (sm->broken) {
+    sm->gate2 = 0;
+    return;
+  }
+  double r2 = 0;
+  for (int v = 0; v < nv; ++v) r2 += sm->h1[v] * sm->h1[v];
+  double rn = sqrt(r2), qn = sqrt(sm->nn1a[0]);
+  const double eps = 2.220446049250313e-16;
+  if (!(rn > eps)) rn = 0.0;
+  if (!(qn > eps)) qn = 0.0;
+  sm->gate2 = rn < qn / sqrt(2.0) ? 1 : 0;
 }
 
 // Arnoldi bookkeeping for step j (one thread): JAX's H row, breakdown test
@@ -148,10 +260,11 @@ __global__ void finalize_kernel(Small *sm, int j, int R, int early_exit) {
     sm->inv = 0.0;
     return;
   }
+  const int two = sm->gate2;
   double h[kMaxRestart + 2];
-  for (int v = 0; v <= j; ++v) h[v] = sm->h1[v] + sm->h2[v];
-  const double vn0 = sqrt(sm->nn0);
-  double vn1 = sqrt(sm->nn1);
+  for (int v = 0; v <= j; ++v) h[v] = sm->h1[v] + (two ? sm->h2[v] : 0.0);
+  const double vn0 = sqrt(sm->h1[j + 1]);  // ||w||^2 before projection sits in slot nv
+  double vn1 = sqrt(two ? sm->nn1b[0] : sm->nn1a[0]);
   const double thresh = 2.220446049250313e-16 * vn0;
   if (!(vn1 > thresh)) vn1 = 0.0;
   h[j + 1] = vn1;
@@ -176,17 +289,57 @@ __global__ void finalize_kernel(Small *sm, int j, int R, int early_exit) {
   if (early_exit && fabs(sm->g[j + 1]) <= sm->target) sm->broken = 1;
 }
 
-// V[j+1] = w * inv (exact zero once the cycle stopped)
+// V[j+1] = w * inv; nothing once the cycle has stopped (those columns are
+// outside the least squares).
 template <typename TV>
 __global__ void normalize_kernel(const TV *__restrict__ w, TV *__restrict__ out, int64_t n,
-                                 const Small *__restrict__ sm) {
+                                 const Small *__restrict__ sm, int stopped_before) {
+  if (stopped_before) return;
   const double inv = sm->inv;
+  constexpr int N = Vec<TV>::N;
+  const int64_t nq = n / N;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += stride)
+  const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (int64_t q = t0; q < nq; q += stride) {
+    TV a[N];
+    vload(w, q, a);
+#pragma unroll
+    for (int k = 0; k < N; ++k) a[k] = inv != 0.0 ? (TV)((double)a[k] * inv) : (TV)0;
+    vstore(out, q, a);
+  }
+  if (t0 < n - nq * N) {
+    const int64_t e = nq * N + t0;
     out[e] = inv != 0.0 ? (TV)((double)w[e] * inv) : (TV)0;
+  }
 }
 
-// out = src * scal[idx] safe-normalised (JAX _safe_normalize, thresh eps)
+// V[j+1] = w * inv, gated on the device stop flag (used inside a cycle)
+template <typename TV>
+__global__ void normalize_gated_kernel(const TV *__restrict__ w, TV *__restrict__ out, int64_t n,
+                                       const Small *__restrict__ sm, const int *__restrict__ stop_before) {
+  if (*stop_before) return;
+  const double inv = sm->inv;
+  constexpr int N = Vec<TV>::N;
+  const int64_t nq = n / N;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (int64_t q = t0; q < nq; q += stride) {
+    TV a[N];
+    vload(w, q, a);
+#pragma unroll
+    for (int k = 0; k < N; ++k) a[k] = inv != 0.0 ? (TV)((double)a[k] * inv) : (TV)0;
+    vstore(out, q, a);
+  }
+  if (t0 < n - nq * N) {
+    const int64_t e = nq * N + t0;
+    out[e] = inv != 0.0 ? (TV)((double)w[e] * inv) : (TV)0;
+  }
+}
+
+// snapshot of the stop flag taken before a step's finalize
+__global__ void snapshot_kernel(const Small *sm, int *dst) { *dst = sm->broken; }
+
+// out = src / ||src|| safe-normalised (JAX _safe_normalize, thresh eps)
 template <typename TS, typename TD>
 __global__ void unit_kernel(const TS *__restrict__ src, TD *__restrict__ out, int64_t n,
                             const double *__restrict__ norm2) {
@@ -205,21 +358,41 @@ __global__ void residual_kernel(const T *__restrict__ b, T *__restrict__ t, int6
     t[e] = b[e] - t[e];
 }
 
-// x += scale * sum_v y[v] V[v]
-template <typename TV, int MAXV>
+// x += sum_v y[v] V[v]   (x is FP64)
+template <typename TV>
 __global__ void __launch_bounds__(256) combine_kernel(const TV *__restrict__ V, int64_t ldv, int nv,
                                                       const double *__restrict__ y,
                                                       double *__restrict__ x, int64_t n) {
-  double yv[MAXV];
-#pragma unroll
-  for (int v = 0; v < MAXV; ++v) yv[v] = v < nv ? y[v] : 0.0;
+  constexpr int N = Vec<TV>::N;
+  __shared__ double ys[kMaxRestart + 8];
+  const int nvr = ((nv + kGrp - 1) / kGrp) * kGrp;
+  for (int v = threadIdx.x; v < nvr; v += blockDim.x) ys[v] = v < nv ? y[v] : 0.0;
+  __syncthreads();
+  const int64_t nq = n / N;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += stride) {
-    double s = 0;
+  const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (int64_t q = t0; q < nq; q += stride) {
+    double s[N];
 #pragma unroll
-    for (int v = 0; v < MAXV; ++v)
-      if (v < nv) s += yv[v] * (double)V[(int64_t)v * ldv + e];
-    x[e] += s;
+    for (int k = 0; k < N; ++k) s[k] = 0;
+#pragma unroll 1
+    for (int g = 0; g * kGrp < nv; ++g) {
+      TV xv[kGrp][N];
+#pragma unroll
+      for (int r = 0; r < kGrp; ++r) vload(V + (int64_t)min(g * kGrp + r, nv - 1) * ldv, q, xv[r]);
+#pragma unroll
+      for (int r = 0; r < kGrp; ++r)
+#pragma unroll
+        for (int k = 0; k < N; ++k) s[k] += ys[g * kGrp + r] * (double)xv[r][k];
+    }
+#pragma unroll
+    for (int k = 0; k < N; ++k) x[q * N + k] += s[k];
+  }
+  if (t0 < n - nq * N) {
+    const int64_t e = nq * N + t0;
+    double sd = 0;
+    for (int v = 0; v < nv; ++v) sd += ys[v] * (double)V[(int64_t)v * ldv + e];
+    x[e] += sd;
   }
 }
 
@@ -232,6 +405,7 @@ __global__ void reset_cycle_kernel(Small *sm, int R, double g0, double target) {
   if (t == 0) {
     sm->broken = 0;
     sm->steps = 0;
+    sm->gate2 = 0;
     sm->inv = 0.0;
     for (int v = 0; v <= R; ++v) sm->g[v] = 0.0;
     sm->g[0] = g0;
@@ -241,58 +415,36 @@ __global__ void reset_cycle_kernel(Small *sm, int R, double g0, double target) {
 
 // ---- host-side dispatch -------------------------------------------------------
 
-template <typename TV, int MAXV>
-static void dots_launch(const TV *V, int64_t ldv, int nv, const TV *w, int64_t n, double *partials,
-                        hipStream_t s) {
-  hipLaunchKernelGGL((dots_kernel<TV, MAXV>), dim3(kRedGrid), dim3(kBlock), 0, s, V, ldv, nv, w, n,
-                     partials);
-}
 template <typename TV>
 static void dots(const TV *V, int64_t ldv, int nv, const TV *w, int64_t n, double *partials,
-                 hipStream_t s) {
-  if (nv <= 8) dots_launch<TV, 8>(V, ldv, nv, w, n, partials, s);
-  else if (nv <= 16) dots_launch<TV, 16>(V, ldv, nv, w, n, partials, s);
-  else if (nv <= 24) dots_launch<TV, 24>(V, ldv, nv, w, n, partials, s);
-  else if (nv <= 32) dots_launch<TV, 32>(V, ldv, nv, w, n, partials, s);
-  else if (nv <= 48) dots_launch<TV, 48>(V, ldv, nv, w, n, partials, s);
-  else dots_launch<TV, 64>(V, ldv, nv, w, n, partials, s);
-}
-template <typename TV, int MAXV>
-static void update_launch(const TV *V, int64_t ldv, int nv, const double *h, TV *w, int64_t n,
-                          int reproj, double *partials, hipStream_t s) {
-  hipLaunchKernelGGL((update_kernel<TV, MAXV>), dim3(kRedGrid), dim3(kBlock), 0, s, V, ldv, nv, h, w,
-                     n, reproj, partials);
+                 hipStream_t s, const int *stop = nullptr, const int *gate = nullptr) {
+  hipLaunchKernelGGL(dots_kernel<TV>, dim3(kRedGrid), dim3(kBlock), 0, s, V, ldv, nv, w, n, partials,
+                     stop, gate);
 }
 template <typename TV>
-static void update(const TV *V, int64_t ldv, int nv, const double *h, TV *w, int64_t n, int reproj,
-                   double *partials, hipStream_t s) {
-  if (nv <= 8) update_launch<TV, 8>(V, ldv, nv, h, w, n, reproj, partials, s);
-  else if (nv <= 16) update_launch<TV, 16>(V, ldv, nv, h, w, n, reproj, partials, s);
-  else if (nv <= 24) update_launch<TV, 24>(V, ldv, nv, h, w, n, reproj, partials, s);
-  else if (nv <= 32) update_launch<TV, 32>(V, ldv, nv, h, w, n, reproj, partials, s);
-  else if (nv <= 48) update_launch<TV, 48>(V, ldv, nv, h, w, n, reproj, partials, s);
-  else update_launch<TV, 64>(V, ldv, nv, h, w, n, reproj, partials, s);
-}
-template <typename TV, int MAXV>
-static void combine_launch(const TV *V, int64_t ldv, int nv, const double *y, double *x, int64_t n,
-                           hipStream_t s) {
-  hipLaunchKernelGGL((combine_kernel<TV, MAXV>), dim3(kRedGrid), dim3(kBlock), 0, s, V, ldv, nv, y, x,
-                     n);
+static void update(const TV *V, int64_t ldv, int nv, const double *h, TV *w, int64_t n,
+                   double *partials, hipStream_t s, const int *stop = nullptr,
+                   const int *gate = nullptr) {
+  hipLaunchKernelGGL(update_kernel<TV>, dim3(kRedGrid), dim3(kBlock), 0, s, V, ldv, nv, h, w, n,
+                     partials, stop, gate);
 }
 template <typename TV>
 static void combine(const TV *V, int64_t ldv, int nv, const double *y, double *x, int64_t n,
                     hipStream_t s) {
-  if (nv <= 8) combine_launch<TV, 8>(V, ldv, nv, y, x, n, s);
-  else if (nv <= 16) combine_launch<TV, 16>(V, ldv, nv, y, x, n, s);
-  else if (nv <= 32) combine_launch<TV, 32>(V, ldv, nv, y, x, n, s);
-  else combine_launch<TV, 64>(V, ldv, nv, y, x, n, s);
+  hipLaunchKernelGGL(combine_kernel<TV>, dim3(kRedGrid), dim3(kBlock), 0, s, V, ldv, nv, y, x, n);
+}
+static void reduce(const double *partials, int nslots, double *out, hipStream_t s,
+                   const int *stop = nullptr, const int *gate = nullptr) {
+  hipLaunchKernelGGL(reduce_partials_kernel, dim3(nslots), dim3(kBlock), 0, s, partials, out, stop, gate);
 }
 
 static inline int64_t align256(int64_t b) { return (b + 255) & ~(int64_t)255; }
+// leading dimension of the basis: rows start 256-byte aligned for vector loads
+static inline int64_t basis_ld(int64_t n) { return (n + 63) & ~(int64_t)63; }
 
 // Workspace carve-up (byte offsets).
 struct Layout {
-  int64_t V, w, t, scr, partials, small, V32, w32, t32, scr32, total;
+  int64_t V, w, t, scr, partials, small, flags, V32, w32, t32, scr32, total;
 };
 
 static Layout layout(const Stag &s, int R, bool mixed) {
@@ -301,16 +453,17 @@ static Layout layout(const Stag &s, int R, bool mixed) {
   const int64_t n = s.n, nm = s.m.n();
   auto take = [&](int64_t bytes) { const int64_t r = o; o += align256(bytes); return r; };
   L.small = take(sizeof(Small));
+  L.flags = take(sizeof(int) * (kMaxRestart + 2));
   L.partials = take(sizeof(double) * kPStride * kRedGrid);
-  L.w = take(sizeof(double) * n);
-  L.t = take(sizeof(double) * n);
+  L.w = take(sizeof(double) * basis_ld(n));
+  L.t = take(sizeof(double) * basis_ld(n));
   L.scr = take(sizeof(double) * 3 * nm);
   if (!mixed) {
-    L.V = take(sizeof(double) * n * (R + 1));
+    L.V = take(sizeof(double) * basis_ld(n) * (R + 1));
   } else {
-    L.V32 = take(sizeof(float) * n * (R + 1));
-    L.w32 = take(sizeof(float) * n);
-    L.t32 = take(sizeof(float) * n);
+    L.V32 = take(sizeof(float) * basis_ld(n) * (R + 1));
+    L.w32 = take(sizeof(float) * basis_ld(n));
+    L.t32 = take(sizeof(float) * basis_ld(n));
     L.scr32 = take(sizeof(float) * 3 * nm);
   }
   L.total = o;
@@ -328,9 +481,7 @@ static void host_lstsq(const double *H, int R, int s, double beta, double *y) {
     }
     r[a] = H[a * (R + 1) + 0] * beta;
   }
-  // Gaussian elimination with partial pivoting
-  std::vector<int> piv(s);
-  for (int c = 0; c < s; ++c) {
+  for (int c = 0; c < s; ++c) {  // Gaussian elimination, partial pivoting
     int p = c;
     for (int q = c + 1; q < s; ++q)
       if (std::fabs(G[(size_t)q * s + c]) > std::fabs(G[(size_t)p * s + c])) p = q;
@@ -354,12 +505,6 @@ static void host_lstsq(const double *H, int R, int s, double beta, double *y) {
   }
 }
 
-__global__ void copy_nn0_kernel(Small *sm, int nv) {
-  sm->nn0 = sm->h1[nv];
-  sm->h1[nv] = 0.0;
-  sm->h2[nv] = 0.0;
-}
-
 template <typename TV>
 struct Krylov {
   const OpT<TV> &op;
@@ -367,48 +512,53 @@ struct Krylov {
   TV *V, *w, *t, *scr;
   double *partials;
   Small *sm;
+  int *flags;   // flags[j]: stop flag snapshot before step j
   int64_t n;
   hipStream_t s;
+  int64_t ldv() const { return basis_ld(n); }
 
-  // w <- (M A) V[j]  (or M^T A^T)
-  int apply(const TV *in, TV *out) {
+  // out <- (M A) in  (or M^T A^T); kernels skip when *stop != 0
+  int apply(const TV *in, TV *out, const int *stop) {
     int rc;
     if (!transpose) {
-      if ((rc = launch_A<TV>(op, in, out, s))) return rc;
-      return launch_M<TV>(op, out, out, s);
+      if ((rc = launch_A<TV>(op, in, out, s, stop))) return rc;
+      return launch_M<TV>(op, out, out, s, stop);
     }
-    if ((rc = launch_AT<TV>(op, in, t, scr, s))) return rc;
-    return launch_MT<TV>(op, t, out, scr, s);
+    if ((rc = launch_AT<TV>(op, in, t, scr, s, stop))) return rc;
+    return launch_MT<TV>(op, t, out, scr, s, stop);
   }
 
-  // One Arnoldi step with CGS2 (JAX _kth_arnoldi_iteration).
+  // One Arnoldi step (JAX _kth_arnoldi_iteration) with JAX's iterated CGS.
   int arnoldi(int j, int R, int early_exit) {
     int rc;
-    TV *Vj = V + (int64_t)j * n;
-    if ((rc = apply(Vj, w))) return rc;
+    const int64_t ld = ldv();
+    int *stop = flags + j;
+    hipLaunchKernelGGL(snapshot_kernel, dim3(1), dim3(1), 0, s, (const Small *)sm, stop);
+    if ((rc = apply(V + (int64_t)j * ld, w, stop))) return rc;
     const int nv = j + 1;
-    dots<TV>(V, n, nv, w, n, partials, s);
-    hipLaunchKernelGGL(reduce_partials_kernel, dim3(nv + 1), dim3(kBlock), 0, s, partials, sm->h1);
-    // nn0 sits in h1[nv]; copy it into place with the finalize (reads h1[nv])
-    update<TV>(V, n, nv, sm->h1, w, n, 1, partials, s);
-    hipLaunchKernelGGL(reduce_partials_kernel, dim3(nv + 1), dim3(kBlock), 0, s, partials, sm->h2);
-    update<TV>(V, n, nv, sm->h2, w, n, 0, partials, s);
-    hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(kBlock), 0, s, partials, &sm->nn1);
-    hipLaunchKernelGGL(copy_nn0_kernel, dim3(1), dim3(1), 0, s, sm, nv);
+    const int *gate = &sm->gate2;
+    dots<TV>(V, ld, nv, w, n, partials, s, stop);
+    reduce(partials, nv + 1, sm->h1, s, stop);             // h1[0..nv-1], ||w||^2 in h1[nv]
+    update<TV>(V, ld, nv, sm->h1, w, n, partials, s, stop);
+    reduce(partials, 1, sm->nn1a, s, stop);
+    hipLaunchKernelGGL(decide_kernel, dim3(1), dim3(1), 0, s, sm, nv);
+    dots<TV>(V, ld, nv, w, n, partials, s, stop, gate);
+    reduce(partials, nv, sm->h2, s, stop, gate);
+    update<TV>(V, ld, nv, sm->h2, w, n, partials, s, stop, gate);
+    reduce(partials, 1, sm->nn1b, s, stop, gate);
     hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(64), 0, s, sm, j, R, early_exit);
-    hipLaunchKernelGGL(normalize_kernel<TV>, dim3(kRedGrid), dim3(kBlock), 0, s, (const TV *)w,
-                       V + (int64_t)(j + 1) * n, n, (const Small *)sm);
+    hipLaunchKernelGGL(normalize_gated_kernel<TV>, dim3(kRedGrid), dim3(kBlock), 0, s, (const TV *)w,
+                       V + (int64_t)(j + 1) * ld, n, (const Small *)sm, (const int *)stop);
     SISL_CHECK_LAUNCH();
     return SISL_OK;
   }
-
 };
 
-// squared norm of a vector into *out (device), via the dots pass with nv = 0
+// squared norm of a vector into *out (device)
 template <typename TV>
 static void norm2(const TV *x, int64_t n, double *partials, double *out, hipStream_t s) {
   dots<TV>(x, n, 0, x, n, partials, s);
-  hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(kBlock), 0, s, partials, out);
+  reduce(partials, 1, out, s);
 }
 
 static int sync_copy(void *dst, const void *src, size_t bytes, hipStream_t s) {
@@ -424,7 +574,7 @@ static int gmres64(const OpT<double> &op, const SislGmresParams &p, const double
   const Layout L = layout(op.s, R, false);
   Krylov<double> K{op, p.transpose, (double *)(ws + L.V), (double *)(ws + L.w), (double *)(ws + L.t),
                    (double *)(ws + L.scr), (double *)(ws + L.partials), (Small *)(ws + L.small),
-                   op.s.n, s};
+                   (int *)(ws + L.flags), op.s.n, s};
   const int64_t n = op.s.n;
   Small *sm = K.sm;
   double hs[4];
@@ -439,7 +589,7 @@ static int gmres64(const OpT<double> &op, const SislGmresParams &p, const double
     if ((rc = launch_MT<double>(op, K.w, K.t, K.scr, s))) return rc;
     norm2<double>(K.t, n, K.partials, &sm->scal[1], s);
   }
-  // initial residual M (b - A x0)
+  // residual M (b - A x), unit vector into V[0]
   auto residual = [&](double *scal_slot) -> int {
     int r2;
     if (!p.transpose) {
@@ -480,7 +630,7 @@ static int gmres64(const OpT<double> &op, const SislGmresParams &p, const double
     host_lstsq(H.data(), R, steps, rnorm, y);
     if (hipMemcpyAsync(sm->y, y, sizeof(double) * steps, hipMemcpyHostToDevice, s) != hipSuccess)
       return SISL_ERR_LAUNCH;
-    combine<double>(K.V, n, steps, sm->y, x, n, s);
+    combine<double>(K.V, K.ldv(), steps, sm->y, x, n, s);
     if ((rc = residual(&sm->scal[2]))) return rc;
     if ((rc = sync_copy(hs, sm->scal, sizeof(double) * 3, s))) return rc;
     rnorm = std::sqrt(hs[2]);
@@ -507,7 +657,8 @@ static int gmres_mixed(const OpT<double> &op64, const OpT<float> &op32, const Si
   double *partials = (double *)(ws + L.partials);
   double *w64 = (double *)(ws + L.w), *t64 = (double *)(ws + L.t), *scr64 = (double *)(ws + L.scr);
   Krylov<float> K{op32, p.transpose, (float *)(ws + L.V32), (float *)(ws + L.w32),
-                  (float *)(ws + L.t32), (float *)(ws + L.scr32), partials, sm, n, s};
+                  (float *)(ws + L.t32), (float *)(ws + L.scr32), partials, sm,
+                  (int *)(ws + L.flags), n, s};
   int rc;
   double hs[4];
   auto M64 = [&](double *in, double *out) -> int {
@@ -555,7 +706,7 @@ static int gmres_mixed(const OpT<double> &op64, const OpT<float> &op32, const Si
     host_lstsq(H.data(), R, steps, rnorm, y);
     if (hipMemcpyAsync(sm->y, y, sizeof(double) * steps, hipMemcpyHostToDevice, s) != hipSuccess)
       return SISL_ERR_LAUNCH;
-    combine<float>(K.V, n, steps, sm->y, x, n, s);
+    combine<float>(K.V, K.ldv(), steps, sm->y, x, n, s);
     ++outer;
   }
   if (st) {
@@ -607,7 +758,7 @@ int sisl_dot_f64(const double *x, const double *y, int64_t n, double *out, doubl
   if (!x || !y || !out || !partials || n < 0) return SISL_ERR_ARG;
   hipStream_t s = sisl::as_stream(stream);
   sisl::dots<double>(x, n, 1, y, n, partials, s);
-  hipLaunchKernelGGL(sisl::reduce_partials_kernel, dim3(1), dim3(sisl::kBlock), 0, s, partials, out);
+  sisl::reduce(partials, 1, out, s);
   SISL_CHECK_LAUNCH();
   return SISL_OK;
 }

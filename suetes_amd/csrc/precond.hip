@@ -12,7 +12,9 @@ namespace sisl {
 
 template <typename T>
 __global__ void __launch_bounds__(256) precond_kernel(OpT<T> o, const T *__restrict__ r,
-                                                      T *__restrict__ z) {
+                                                      T *__restrict__ z,
+                                                      const int *__restrict__ stop) {
+  if (stop && *stop) return;
   const Stag &s = o.s;
   const int nx = s.nx, ny = s.ny, nz = s.nz;
   const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -75,7 +77,9 @@ template <typename T>
 __global__ void __launch_bounds__(256) precond_T_kernel(OpT<T> o, const T *__restrict__ zb,
                                                         T *__restrict__ rb,
                                                         T *__restrict__ umb,
-                                                        T *__restrict__ vmb) {
+                                                        T *__restrict__ vmb,
+                                                        const int *__restrict__ stop) {
+  if (stop && *stop) return;
   const Stag &s = o.s;
   const int nx = s.nx, ny = s.ny, nz = s.nz;
   const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -139,7 +143,9 @@ template <typename T>
 __global__ void __launch_bounds__(256) precond_T_faces_kernel(OpT<T> o, const T *__restrict__ zb,
                                                               const T *__restrict__ umb,
                                                               const T *__restrict__ vmb,
-                                                              T *__restrict__ rb) {
+                                                              T *__restrict__ rb,
+                                                              const int *__restrict__ stop) {
+  if (stop && *stop) return;
   const Stag &s = o.s;
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= s.ow) return;
@@ -164,33 +170,35 @@ __global__ void __launch_bounds__(256) precond_T_faces_kernel(OpT<T> o, const T 
 }
 
 template <typename T>
-int launch_M(const OpT<T> &o, const T *r, T *z, hipStream_t st) {
+int launch_M(const OpT<T> &o, const T *r, T *z, hipStream_t st, const int *stop) {
   if (r != z) {
     if (hipMemcpyAsync(z, r, sizeof(T) * o.s.ow, hipMemcpyDeviceToDevice, st) != hipSuccess)
       return SISL_ERR_LAUNCH;
   }
   const int64_t cols = (int64_t)o.s.nx * o.s.ny;
-  hipLaunchKernelGGL(precond_kernel<T>, dim3(grid_for(cols)), dim3(kBlock), 0, st, o, r, z);
+  hipLaunchKernelGGL(precond_kernel<T>, dim3(grid_for(cols)), dim3(kBlock), 0, st, o, r, z, stop);
   SISL_CHECK_LAUNCH();
   return SISL_OK;
 }
 
 template <typename T>
-int launch_MT(const OpT<T> &o, const T *zb, T *rb, T *scratch, hipStream_t st) {
+int launch_MT(const OpT<T> &o, const T *zb, T *rb, T *scratch, hipStream_t st, const int *stop) {
   const int64_t cols = (int64_t)o.s.nx * o.s.ny;
   const int64_t nm = o.s.m.n();
   hipLaunchKernelGGL(precond_T_kernel<T>, dim3(grid_for(cols)), dim3(kBlock), 0, st, o, zb, rb,
-                     scratch, scratch + nm);
+                     scratch, scratch + nm, stop);
   hipLaunchKernelGGL(precond_T_faces_kernel<T>, dim3(grid_for(o.s.ow)), dim3(kBlock), 0, st, o, zb,
-                     (const T *)scratch, (const T *)(scratch + nm), rb);
+                     (const T *)scratch, (const T *)(scratch + nm), rb, stop);
   SISL_CHECK_LAUNCH();
   return SISL_OK;
 }
 
-template int launch_M<double>(const OpT<double> &, const double *, double *, hipStream_t);
-template int launch_M<float>(const OpT<float> &, const float *, float *, hipStream_t);
-template int launch_MT<double>(const OpT<double> &, const double *, double *, double *, hipStream_t);
-template int launch_MT<float>(const OpT<float> &, const float *, float *, float *, hipStream_t);
+template int launch_M<double>(const OpT<double> &, const double *, double *, hipStream_t, const int *);
+template int launch_M<float>(const OpT<float> &, const float *, float *, hipStream_t, const int *);
+template int launch_MT<double>(const OpT<double> &, const double *, double *, double *, hipStream_t,
+                               const int *);
+template int launch_MT<float>(const OpT<float> &, const float *, float *, float *, hipStream_t,
+                              const int *);
 
 }  // namespace sisl
 

@@ -91,12 +91,21 @@ class TricubicPlan:
 
 
 class Coords:
-    """Departure coordinates of one staggering: tensor (3, Z, Y, X) in index
-    space (component 0 = x index) plus a lazily built adjoint plan."""
+    """Departure points of one staggering: tensor (3, Z, Y, X) of index-space
+    offsets x_d - x_a (component 0 along x) plus a lazily built adjoint plan.
+    ``absolute()`` returns the reference's coordinates (steppers.py:360)."""
 
     def __init__(self, data: torch.Tensor):
         self.data = data
         self._plan = None
+
+    def absolute(self):
+        _, nz, ny, nx = self.data.shape
+        dev, dt = self.data.device, self.data.dtype
+        idx = [torch.arange(nx, device=dev, dtype=dt).reshape(1, 1, -1),
+               torch.arange(ny, device=dev, dtype=dt).reshape(1, -1, 1),
+               torch.arange(nz, device=dev, dtype=dt).reshape(-1, 1, 1)]
+        return torch.stack([self.data[a] + idx[a] for a in range(3)])
 
     def plan(self):
         if self._plan is None:
@@ -144,7 +153,7 @@ def tensor_product_interp_3d(field, coords, use_limiter=False):
     """Tricubic Catmull-Rom gather at departure points (operators.py:146-209).
 
     ``field``: internal (Z, Y, X) tensor; ``coords``: a :class:`Coords` (or a
-    (3, Z, Y, X) tensor) of index-space departure points.  When coords are
+    (3, Z, Y, X) tensor) of departure offsets x_d - x_a in index space.  When coords are
     FP32 and the field FP64 the interpolation runs in FP32 and the result is
     returned in FP64 (mixed precision)."""
     if not isinstance(coords, Coords):

@@ -72,7 +72,7 @@ def build_dynamical_core(core_type, grid, operators, constants, initial_state, p
          "nu_h_factor": 0.1, "N_bv": 0.01, "alpha": 0.55, "use_limiter": False,
          "use_checkpointing": False, "solver_tol": 1e-12, "solver_maxiter": 100,
          "solver_restart": 100, "background_state": None, "precision": "fp64",
-         "inner_rtol": 1e-4, "solver_early_exit": False}
+         "inner_rtol": 1e-4, "inner_restart": None, "solver_early_exit": False}
     p.update(kwargs)
     ref = p["background_state"] if p["background_state"] is not None else initial_state
     physics = Euler3D(grid, operators, constants, dt=p["dt"], initial_era5_state=ref,
@@ -93,7 +93,7 @@ def build_dynamical_core(core_type, grid, operators, constants, initial_state, p
                             use_checkpointing=p["use_checkpointing"], solver_tol=p["solver_tol"],
                             solver_maxiter=p["solver_maxiter"], solver_restart=p["solver_restart"],
                             precision=p["precision"], inner_rtol=p["inner_rtol"],
-                            solver_early_exit=p["solver_early_exit"])
+                            inner_restart=p["inner_restart"], solver_early_exit=p["solver_early_exit"])
     return stepper, p["dt"]
 
 
@@ -255,7 +255,7 @@ class SemiImplicitSolver3D:
     with the same stopping test."""
 
     def __init__(self, physics, dt, alpha=0.55, solver_tol=1e-4, solver_maxiter=20, solver_restart=20,
-                 precision="fp64", inner_rtol=1e-4, early_exit=False):
+                 precision="fp64", inner_rtol=1e-4, early_exit=False, inner_restart=None):
         self.physics, self.dt, self.alpha = physics, dt, alpha
         self.pi_scale = 1000.0
         self.solver_tol, self.solver_maxiter, self.solver_restart = solver_tol, solver_maxiter, solver_restart
@@ -263,6 +263,7 @@ class SemiImplicitSolver3D:
             raise ValueError("precision must be 'fp64' or 'mixed'")
         self.precision = precision
         self.inner_rtol = inner_rtol
+        self.inner_restart = inner_restart
         self.early_exit = early_exit
         self._bg_id = None
         self.last_stats = []
@@ -278,17 +279,22 @@ class SemiImplicitSolver3D:
         self.op64 = _OperatorSet(g, per, co, self.alpha, torch.float64)
         self.op32 = _OperatorSet(g, per, co, self.alpha, torch.float32) if self.precision == "mixed" else None
         L = _lib.lib()
-        R = min(self.solver_restart, 64)
-        if self.solver_restart > 64:
-            raise ValueError("solver_restart above 64 is not supported by the HIP GMRES")
+        R = self._restart()
+        if R > 64:
+            raise ValueError("restart above 64 is not supported by the HIP GMRES")
         nbytes = (L.sisl_gmres_workspace_bytes_mixed if self.precision == "mixed"
                   else L.sisl_gmres_workspace_bytes_f64)(C.byref(self.op64.struct.g), R)
         self.workspace = torch.empty(int(nbytes), dtype=torch.uint8, device=co["cua"].device)
         self._bg_id = key
         self._bg = bg
 
+    def _restart(self):
+        if self.precision == "mixed" and self.inner_restart:
+            return int(self.inner_restart)
+        return int(self.solver_restart)
+
     def _params(self, transpose):
-        return _lib.SislGmresParams(float(self.solver_tol), 0.0, int(self.solver_restart),
+        return _lib.SislGmresParams(float(self.solver_tol), 0.0, self._restart(),
                                     int(self.solver_maxiter), int(transpose), int(bool(self.early_exit)),
                                     float(self.inner_rtol), 0)
 
@@ -382,7 +388,7 @@ class SISLStepper3D:
 
     def __init__(self, physics, dt, alpha=0.55, use_limiter=False, use_checkpointing=False,
                  solver_tol=1e-4, solver_maxiter=20, solver_restart=20, precision="fp64",
-                 inner_rtol=1e-4, solver_early_exit=False):
+                 inner_rtol=1e-4, solver_early_exit=False, inner_restart=None):
         self.physics, self.dt, self.alpha = physics, dt, alpha
         self.use_limiter = use_limiter
         self.precision = precision
@@ -392,7 +398,8 @@ class SISLStepper3D:
         self.implicit_solver = SemiImplicitSolver3D(physics, dt, alpha=alpha, solver_tol=solver_tol,
                                                     solver_maxiter=solver_maxiter,
                                                     solver_restart=solver_restart, precision=precision,
-                                                    inner_rtol=inner_rtol, early_exit=solver_early_exit)
+                                                    inner_rtol=inner_rtol, early_exit=solver_early_exit,
+                                                    inner_restart=inner_restart)
         self.use_checkpointing = use_checkpointing
         self.tracer_keys = physics.physics_suite.tracer_keys if physics.physics_suite is not None else []
         self._bg = None

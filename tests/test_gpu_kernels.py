@@ -28,6 +28,17 @@ def from_int(t):
     return t.detach().cpu().numpy().transpose(2, 1, 0)
 
 
+def index_grid(shape):
+    return np.stack(np.meshgrid(*[np.arange(n, dtype=np.float64) for n in shape], indexing="ij"))
+
+
+def offsets_int(pts, dtype=torch.float64):
+    """Absolute index-space points (3, x, y, z) -> offsets x_d - x_a as a
+    (3, Z, Y, X) device tensor (the kernels' departure-point format)."""
+    off = pts - index_grid(pts.shape[1:])
+    return torch.stack([to_int(off[a], dtype) for a in range(3)])
+
+
 def pack_ref(d):
     return torch.cat([to_int(d[k]).reshape(-1) for k in O.KEYS])
 
@@ -52,7 +63,7 @@ def test_tricubic_matches_oracle(shape, lim):
     pts[:, 0, 0, 0] = [0.0, 0.0, 0.0]
     pts[:, -1, -1, -1] = [s - 1.0 for s in shape]
     ref = O.tricubic(f, pts, lim)
-    co = torch.stack([to_int(pts[a]) for a in range(3)])
+    co = offsets_int(pts)
     out = tensor_product_interp_3d(to_int(f), co, lim)
     assert rel_l2(from_int(out), ref) < 1e-13
     out32 = tensor_product_interp_3d(to_int(f, torch.float32), co.float(), lim)
@@ -65,7 +76,7 @@ def test_tricubic_adjoint_dot_and_determinism():
     shape = (13, 11, 9)
     pts = np.stack([rng.uniform(-1, s, size=shape) for s in shape])
     pts[:, :2] = np.round(pts[:, :2])  # clamped edges and integer points
-    co = Coords(torch.stack([to_int(pts[a]) for a in range(3)]))
+    co = Coords(offsets_int(pts))
     f = to_int(rng.standard_normal(shape))
     g = to_int(rng.standard_normal(shape))
     Tf = tensor_product_interp_3d(f, co)
@@ -78,7 +89,7 @@ def test_tricubic_adjoint_dot_and_determinism():
     # dense check on a tiny grid: column q of T^T is the response to e_q
     small = (4, 3, 5)
     p2 = np.stack([rng.uniform(-1, s, size=small) for s in small])
-    c2 = Coords(torch.stack([to_int(p2[a]) for a in range(3)]))
+    c2 = Coords(offsets_int(p2))
     n = int(np.prod(small))
     Tm = np.zeros((n, n))
     for q in range(n):
@@ -102,7 +113,7 @@ def test_departure_points_match_oracle(cfgf):
     for loc in ("u", "v", "w", "m"):
         ref = sto.advector.departure(s, loc)
         got = adv_p.compute_departure_indices({k: to_int(v) for k, v in s.items()}, loc).data
-        got = np.stack([from_int(got[a]) for a in range(3)])
+        got = np.stack([from_int(got[a]) for a in range(3)]) + index_grid(ref.shape[1:])
         assert rel_l2(got, ref) < 1e-13, loc
 
 
