@@ -94,6 +94,18 @@ def kernel_roofline(case, reps=20):
             "algorithmic_bytes": nbytes, "avg_launch_us": dt * 1e6}
 
 
+def copy_reference(nbytes=1 << 29, reps=5):
+    """A plain device copy of known size (reads and writes `nbytes` each):
+    the scale for the FETCH_SIZE / WRITE_SIZE hardware counters."""
+    import torch
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    return nbytes
+
+
 def adjoint_cost(case, nsteps=1):
     """(t_value_and_grad - t_forward) / t_forward over `nsteps` steps
     (core_ad_scaling/run.py:255-258 definition), on the mixed path."""
@@ -165,6 +177,9 @@ def main():
     ap.add_argument("--nz", type=int, default=80)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="only launch the operator kernel and a plain copy of known size "
+                         "(the run that hardware-counter passes profile)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1",
@@ -176,6 +191,12 @@ def main():
 
     cfg = W.lam(nx=args.nx, ny=args.nx, nz=args.nz)
     log(f"building workload lam {args.nx}x{args.nx}x{args.nz}")
+    if args.kernels_only:
+        cmx = build_case(cfg, device="cuda", precision="mixed")
+        roof = kernel_roofline(cmx)
+        nbytes = copy_reference()
+        print(json.dumps({"roofline": roof, "copy_bytes": nbytes}), flush=True)
+        return
     c64 = build_case(cfg, device="cuda", precision="fp64")
     cmx = build_case(cfg, device="cuda", precision="mixed")
     log("timing mixed")

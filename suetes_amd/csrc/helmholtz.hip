@@ -39,20 +39,24 @@ struct AView {
   }
 };
 
+// Launch geometry of the point-wise kernels: block (64, 4) threads over
+// (i, j); blockIdx.z enumerates (segment, k).  No integer division per thread.
+constexpr int kTX = 64, kTY = 4;
+
 template <typename T>
 __global__ void __launch_bounds__(256) helmholtz_kernel(OpT<T> o, const T *__restrict__ x,
                                                         T *__restrict__ y,
                                                         const int *__restrict__ stop) {
   if (stop && *stop) return;
   const Stag &s = o.s;
-  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= s.n) return;
-  AView<T> a(o, x);
   const int nx = s.nx, ny = s.ny, nz = s.nz;
-  T out;
-  if (q < s.ov) {  // ---- L_u (euler.py:398, 411-415) ----
-    const int64_t p = q - s.ou;
-    const int i = (int)(p % (nx + 1)), j = (int)((p / (nx + 1)) % ny), k = (int)(p / ((int64_t)(nx + 1) * ny));
+  const int seg = blockIdx.z / (nz + 1), k = blockIdx.z - seg * (nz + 1);
+  const int i = blockIdx.x * kTX + threadIdx.x, j = blockIdx.y * kTY + threadIdx.y;
+  AView<T> a(o, x);
+  if (seg == 0) {  // ---- L_u (euler.py:398, 411-415) ----
+    if (i > nx || j >= ny || k >= nz) return;
+    const int64_t p = s.u.at(i, j, k);
+    T out;
     if (!s.px && (i == 0 || i == nx)) {
       out = a.U[p];
     } else {
@@ -64,9 +68,11 @@ __global__ void __launch_bounds__(256) helmholtz_kernel(OpT<T> o, const T *__res
       const T vau = (T)0.5 * (a.vm(im, j, k) + a.vm(f, j, k));
       out = a.U[pf] + o.cua[pf] * (o.mudx[s.u2(f, j)] * dpi - o.zxu[pf] * gzu) - o.afu[s.u2(f, j)] * vau;
     }
-  } else if (q < s.ow) {  // ---- L_v (euler.py:399, 416-420) ----
-    const int64_t p = q - s.ov;
-    const int i = (int)(p % nx), j = (int)((p / nx) % (ny + 1)), k = (int)(p / ((int64_t)nx * (ny + 1)));
+    y[s.ou + p] = out;
+  } else if (seg == 1) {  // ---- L_v (euler.py:399, 416-420) ----
+    if (i >= nx || j > ny || k >= nz) return;
+    const int64_t p = s.v.at(i, j, k);
+    T out;
     if (!s.py && (j == 0 || j == ny)) {
       out = a.V[p];
     } else {
@@ -78,26 +84,30 @@ __global__ void __launch_bounds__(256) helmholtz_kernel(OpT<T> o, const T *__res
       const T uav = (T)0.5 * (a.um(i, jm, k) + a.um(i, f, k));
       out = a.V[pf] + o.cva[pf] * (o.mvdy[s.v2(i, f)] * dpi - o.zev[pf] * gzv) + o.afv[s.v2(i, f)] * uav;
     }
-  } else if (q < s.om) {  // ---- L_w (euler.py:400, 426-427) ----
-    const int64_t p = q - s.ow;
-    const int i = (int)(p % nx), j = (int)((p / nx) % ny), k = (int)(p / ((int64_t)nx * ny));
+    y[s.ov + p] = out;
+  } else if (seg == 2) {  // ---- L_w (euler.py:400, 426-427) ----
+    if (i >= nx || j >= ny) return;
+    const int64_t p = s.w.at(i, j, k);
+    T out;
     if (k == 0 || k == nz) {
       out = a.W[p];
     } else {
       out = o.dw[p] * a.W[p] + o.cw[p] * (a.P[s.m.at(i, j, k)] - a.P[s.m.at(i, j, k - 1)]);
     }
-  } else if (q < s.oe) {  // ---- L_pi (euler.py:298-318, 401) ----
-    const int64_t p = q - s.om;
-    const int i = (int)(p % nx), j = (int)((p / nx) % ny), k = (int)(p / ((int64_t)nx * ny));
+    y[s.ow + p] = out;
+  } else if (seg == 3) {  // ---- L_pi (euler.py:298-318, 401) ----
+    if (i >= nx || j >= ny || k >= nz) return;
+    const int64_t p = s.m.at(i, j, k);
     const T fx = a.U[s.u.at(i + 1, j, k)] * o.cx[s.u.at(i + 1, j, k)] - a.U[s.u.at(i, j, k)] * o.cx[s.u.at(i, j, k)];
     const T fy = a.V[s.v.at(i, j + 1, k)] * o.cy[s.v.at(i, j + 1, k)] - a.V[s.v.at(i, j, k)] * o.cy[s.v.at(i, j, k)];
     const T ftop = (k + 1 == nz) ? (T)0 : a.E[s.w.at(i, j, k + 1)] * o.cz[s.w.at(i, j, k + 1)];
     const T fbot = (k == 0) ? (T)0 : a.E[s.w.at(i, j, k)] * o.cz[s.w.at(i, j, k)];
     const int64_t q2 = s.m2(i, j);
-    out = a.P[p] + o.cpm[p] * (o.mmx[q2] * fx + o.mmy[q2] * fy + (ftop - fbot));
+    y[s.om + p] = a.P[p] + o.cpm[p] * (o.mmx[q2] * fx + o.mmy[q2] * fy + (ftop - fbot));
   } else {  // ---- L_eta (euler.py:429-437) ----
-    const int64_t p = q - s.oe;
-    const int i = (int)(p % nx), j = (int)((p / nx) % ny), k = (int)(p / ((int64_t)nx * ny));
+    if (i >= nx || j >= ny) return;
+    const int64_t p = s.w.at(i, j, k);
+    T out;
     if (k == 0 || k == nz) {
       out = a.E[p] * o.rdzw[p];
     } else {
@@ -105,8 +115,13 @@ __global__ void __launch_bounds__(256) helmholtz_kernel(OpT<T> o, const T *__res
       const T vw = (T)0.5 * (a.vm(i, j, k - 1) + a.vm(i, j, k));
       out = o.alpha * (a.E[p] + o.kx[p] * uw + o.ky[p] * vw - a.W[p]);
     }
+    y[s.oe + p] = out;
   }
-  y[q] = out;
+}
+
+inline dim3 point_grid(const Stag &s, int segments) {
+  return dim3((unsigned)((s.nx + 1 + kTX - 1) / kTX), (unsigned)((s.ny + 1 + kTY - 1) / kTY),
+              (unsigned)(segments * (s.nz + 1)));
 }
 
 // ---- transpose ---------------------------------------------------------------
@@ -152,10 +167,10 @@ __global__ void __launch_bounds__(256) helmholtz_T1_kernel(OpT<T> o, const T *__
                                                            const int *__restrict__ stop) {
   if (stop && *stop) return;
   const Stag &s = o.s;
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= s.m.n()) return;
   const int nx = s.nx, ny = s.ny, nz = s.nz;
-  const int i = (int)(p % nx), j = (int)((p / nx) % ny), k = (int)(p / ((int64_t)nx * ny));
+  const int i = blockIdx.x * kTX + threadIdx.x, j = blockIdx.y * kTY + threadIdx.y, k = blockIdx.z;
+  if (i >= nx || j >= ny || k >= nz) return;
+  const int64_t p = s.m.at(i, j, k);
   ATView<T> t(o, y);
   const int fl = i, fr = t.ufold(i + 1);
   const T ul = t.ut(fl, j, k), ur = t.ut(i + 1, j, k);
@@ -187,14 +202,16 @@ __global__ void __launch_bounds__(256) helmholtz_T2_kernel(OpT<T> o, const T *__
                                                            const int *__restrict__ stop) {
   if (stop && *stop) return;
   const Stag &s = o.s;
-  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= s.n) return;
-  ATView<T> t(o, y);
   const int nx = s.nx, ny = s.ny, nz = s.nz;
+  const int seg = blockIdx.z / (nz + 1), k = blockIdx.z - seg * (nz + 1);
+  const int i = blockIdx.x * kTX + threadIdx.x, j = blockIdx.y * kTY + threadIdx.y;
+  ATView<T> t(o, y);
   T out;
-  if (q < s.ov) {  // ---- U bar ----
-    const int64_t p = q - s.ou;
-    const int i = (int)(p % (nx + 1)), j = (int)((p / (nx + 1)) % ny), k = (int)(p / ((int64_t)(nx + 1) * ny));
+  int64_t q;
+  if (seg == 0) {  // ---- U bar ----
+    if (i > nx || j >= ny || k >= nz) return;
+    const int64_t p = s.u.at(i, j, k);
+    q = s.ou + p;
     if (s.px) out = (i == 0) ? t.U[p] + t.U[s.u.at(nx, j, k)] : (i == nx ? (T)0 : t.U[p]);
     else out = t.U[p];
     T fl = 0;
@@ -205,9 +222,10 @@ __global__ void __launch_bounds__(256) helmholtz_T2_kernel(OpT<T> o, const T *__
     if (i - 1 >= 0) ub += um_b[s.m.at(i - 1, j, k)];
     if (i <= nx - 1) ub += um_b[s.m.at(i, j, k)];
     out += (T)0.5 * ub;
-  } else if (q < s.ow) {  // ---- V bar ----
-    const int64_t p = q - s.ov;
-    const int i = (int)(p % nx), j = (int)((p / nx) % (ny + 1)), k = (int)(p / ((int64_t)nx * (ny + 1)));
+  } else if (seg == 1) {  // ---- V bar ----
+    if (i >= nx || j > ny || k >= nz) return;
+    const int64_t p = s.v.at(i, j, k);
+    q = s.ov + p;
     if (s.py) out = (j == 0) ? t.V[p] + t.V[s.v.at(i, ny, k)] : (j == ny ? (T)0 : t.V[p]);
     else out = t.V[p];
     T fl = 0;
@@ -218,14 +236,16 @@ __global__ void __launch_bounds__(256) helmholtz_T2_kernel(OpT<T> o, const T *__
     if (j - 1 >= 0) vb += vm_b[s.m.at(i, j - 1, k)];
     if (j <= ny - 1) vb += vm_b[s.m.at(i, j, k)];
     out += (T)0.5 * vb;
-  } else if (q < s.om) {  // ---- W bar ----
-    const int64_t p = q - s.ow;
-    const int k = (int)(p / ((int64_t)nx * ny));
+  } else if (seg == 2) {  // ---- W bar ----
+    if (i >= nx || j >= ny) return;
+    const int64_t p = s.w.at(i, j, k);
+    q = s.ow + p;
     if (k == 0 || k == nz) out = t.W[p];
     else out = o.dw[p] * t.W[p] - o.alpha * t.E[p];
-  } else if (q < s.oe) {  // ---- P bar ----
-    const int64_t p = q - s.om;
-    const int i = (int)(p % nx), j = (int)((p / nx) % ny), k = (int)(p / ((int64_t)nx * ny));
+  } else if (seg == 3) {  // ---- P bar ----
+    if (i >= nx || j >= ny || k >= nz) return;
+    const int64_t p = s.m.at(i, j, k);
+    q = s.om + p;
     out = t.P[p];
     {  // x faces i (P is the upper cell) and i+1 (P is the lower cell)
       const int64_t a = s.u.at(i, j, k), b = s.u.at(t.ufold(i + 1), j, k);
@@ -255,8 +275,9 @@ __global__ void __launch_bounds__(256) helmholtz_T2_kernel(OpT<T> o, const T *__
     if (k == 0) out -= gzwb(0);
     if (k == nz - 2) out -= gzwb(nz);
   } else {  // ---- E bar ----
-    const int64_t p = q - s.oe;
-    const int i = (int)(p % nx), j = (int)((p / nx) % ny), k = (int)(p / ((int64_t)nx * ny));
+    if (i >= nx || j >= ny) return;
+    const int64_t p = s.w.at(i, j, k);
+    q = s.oe + p;
     if (k == 0 || k == nz) {
       out = t.E[p] * o.rdzw[p];
     } else {
@@ -269,7 +290,7 @@ __global__ void __launch_bounds__(256) helmholtz_T2_kernel(OpT<T> o, const T *__
 
 template <typename T>
 int launch_A(const OpT<T> &o, const T *x, T *y, hipStream_t st, const int *stop) {
-  hipLaunchKernelGGL(helmholtz_kernel<T>, dim3(grid_for(o.s.n)), dim3(kBlock), 0, st, o, x, y, stop);
+  hipLaunchKernelGGL(helmholtz_kernel<T>, point_grid(o.s, 5), dim3(kTX, kTY), 0, st, o, x, y, stop);
   SISL_CHECK_LAUNCH();
   return SISL_OK;
 }
@@ -278,9 +299,10 @@ template <typename T>
 int launch_AT(const OpT<T> &o, const T *y, T *x, T *scratch, hipStream_t st, const int *stop) {
   const int64_t nm = o.s.m.n();
   T *g = scratch, *um = scratch + nm, *vm = scratch + 2 * nm;
-  hipLaunchKernelGGL(helmholtz_T1_kernel<T>, dim3(grid_for(nm)), dim3(kBlock), 0, st, o, y, g, um, vm,
-                     stop);
-  hipLaunchKernelGGL(helmholtz_T2_kernel<T>, dim3(grid_for(o.s.n)), dim3(kBlock), 0, st, o, y,
+  const dim3 g1((unsigned)((o.s.nx + kTX - 1) / kTX), (unsigned)((o.s.ny + kTY - 1) / kTY),
+                (unsigned)o.s.nz);
+  hipLaunchKernelGGL(helmholtz_T1_kernel<T>, g1, dim3(kTX, kTY), 0, st, o, y, g, um, vm, stop);
+  hipLaunchKernelGGL(helmholtz_T2_kernel<T>, point_grid(o.s, 5), dim3(kTX, kTY), 0, st, o, y,
                      (const T *)g, (const T *)um, (const T *)vm, x, stop);
   SISL_CHECK_LAUNCH();
   return SISL_OK;
