@@ -85,6 +85,28 @@ struct OpT {
   }
 };
 
+// The hardware hands consecutive workgroups of a launch to the 8 XCDs in
+// turn.  Renumber them so that each XCD receives one contiguous range of
+// logical tiles: neighbouring tiles then share an L2 and their halos hit it.
+constexpr int kXcds = 8;
+__device__ inline int xcd_tile(int b, int nblocks) {
+  if (nblocks % kXcds) return b;
+  return (b % kXcds) * (nblocks / kXcds) + b / kXcds;
+}
+
+// Load/store through a wave-uniform base pointer and a 32-bit element index.
+// The byte offset is formed in 32 bits so that the access compiles to the
+// scalar-base + 32-bit-vector-offset form (no 64-bit address per array in
+// vector registers).  Callers guarantee every array stays below 4 GiB.
+template <typename T>
+__device__ __forceinline__ T ldg(const T *base, uint32_t idx) {
+  return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + idx * (uint32_t)sizeof(T));
+}
+template <typename T>
+__device__ __forceinline__ void stg(T *base, uint32_t idx, T v) {
+  *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + idx * (uint32_t)sizeof(T)) = v;
+}
+
 // Wave64 sum reduction through DPP/shuffles.
 template <typename T>
 __device__ inline T wave_sum(T v) {

@@ -2,10 +2,9 @@
 //
 // A is Euler3D.linear_operator (suetes/regional3d/euler.py:374-439) seen
 // through the solver's scaling (steppers.py:439-461): packed Krylov vectors
-// u | v | w | pi*1000 | eta_dot*dz_w.  One thread per output element; all
-// staggered neighbours come through the cache hierarchy.  Every coefficient
-// that the reference recomputes per call (background averages, metrics,
-// map factors) is precomputed once per stepper by the host layer.
+// u | v | w | pi*1000 | eta_dot*dz_w.  Every coefficient that the reference
+// recomputes per call (background averages, metrics, map factors) is
+// precomputed once per stepper by the host layer.
 #include "common.h"
 
 namespace sisl {
@@ -39,83 +38,149 @@ struct AView {
   }
 };
 
-// Launch geometry of the point-wise kernels: block (64, 4) threads over
-// (i, j); blockIdx.z enumerates (segment, k).  No integer division per thread.
+// Launch geometry.  Forward operator: one thread per (i, j) column of a
+// 64 x 4 tile, marching a chunk of `kchunk` levels upwards and carrying the
+// vertical neighbours in registers; it writes all five packed outputs of its
+// column, so every input array is read once per launch (plus tile halos, which
+// the XCD-aware tile order keeps in L2).  Transpose kernels: 3-D grids,
+// blockIdx.z enumerating (segment, level).
 constexpr int kTX = 64, kTY = 4;
 
 template <typename T>
 __global__ void __launch_bounds__(256) helmholtz_kernel(OpT<T> o, const T *__restrict__ x,
-                                                        T *__restrict__ y,
+                                                        T *__restrict__ y, int kchunk,
                                                         const int *__restrict__ stop) {
   if (stop && *stop) return;
   const Stag &s = o.s;
   const int nx = s.nx, ny = s.ny, nz = s.nz;
-  const int seg = blockIdx.z / (nz + 1), k = blockIdx.z - seg * (nz + 1);
-  const int i = blockIdx.x * kTX + threadIdx.x, j = blockIdx.y * kTY + threadIdx.y;
-  AView<T> a(o, x);
-  if (seg == 0) {  // ---- L_u (euler.py:398, 411-415) ----
-    if (i > nx || j >= ny || k >= nz) return;
-    const int64_t p = s.u.at(i, j, k);
-    T out;
-    if (!s.px && (i == 0 || i == nx)) {
-      out = a.U[p];
-    } else {
-      const int f = (i == nx) ? 0 : i;           // periodic: face nx is face 0
-      const int im = f == 0 ? nx - 1 : f - 1;
-      const int64_t pf = s.u.at(f, j, k);
-      const T dpi = a.P[s.m.at(f, j, k)] - a.P[s.m.at(im, j, k)];
-      const T gzu = (T)0.5 * (a.gzm(im, j, k) + a.gzm(f, j, k));
-      const T vau = (T)0.5 * (a.vm(im, j, k) + a.vm(f, j, k));
-      out = a.U[pf] + o.cua[pf] * (o.mudx[s.u2(f, j)] * dpi - o.zxu[pf] * gzu) - o.afu[s.u2(f, j)] * vau;
+  const int nbx = (nx + kTX - 1) / kTX, nby = (ny + kTY - 1) / kTY;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int bx = tile % nbx, by = (tile / nbx) % nby, bz = tile / (nbx * nby);
+  const int i = bx * kTX + threadIdx.x, j = by * kTY + threadIdx.y;
+  if (i >= nx || j >= ny) return;
+  const int k0 = bz * kchunk, k1 = min(k0 + kchunk, nz + 1);
+  const int il = i == 0 ? nx - 1 : i - 1, jd = j == 0 ? ny - 1 : j - 1;
+  const T *U = x + s.ou, *V = x + s.ov, *W = x + s.ow, *P = x + s.om, *E = x + s.oe;
+  T *YU = y + s.ou, *YV = y + s.ov, *YW = y + s.ow, *YP = y + s.om, *YE = y + s.oe;
+  const bool uform = s.px || i > 0, vform = s.py || j > 0;  // else face 0 is the identity
+  const T mudx = ldg(o.mudx, (uint32_t)s.u2(i, j)), afu = ldg(o.afu, (uint32_t)s.u2(i, j));
+  const T mvdy = ldg(o.mvdy, (uint32_t)s.v2(i, j)), afv = ldg(o.afv, (uint32_t)s.v2(i, j));
+  const T mmx = ldg(o.mmx, (uint32_t)s.m2(i, j)), mmy = ldg(o.mmy, (uint32_t)s.m2(i, j));
+  const T half = (T)0.5, alpha = o.alpha;
+  // 32-bit element offsets at level 0 and the plane strides: own column
+  // (i, j), left column (il, j), lower column (i, jd).  Mass and w points
+  // share the (i, j) plane layout.
+  const uint32_t pm = (uint32_t)nx * ny, pu = (uint32_t)(nx + 1) * ny, pv = (uint32_t)nx * (ny + 1);
+  const uint32_t mc = (uint32_t)j * nx + i, ml = (uint32_t)j * nx + il, md = (uint32_t)jd * nx + i;
+  const uint32_t uc = (uint32_t)j * (nx + 1) + i, ud = (uint32_t)jd * (nx + 1) + i;
+  const uint32_t vc = (uint32_t)j * nx + i, vl = (uint32_t)j * nx + il;
+  const uint32_t ux = (uint32_t)j * (nx + 1) + nx;  // east face of row j
+  const uint32_t vy = (uint32_t)ny * nx + i;         // north face of column i
+  // carried state at level k: P at k and k-1, gz_w at k (three columns),
+  // u_m / v_m of the own column at k-1, eta_dot and its flux at w level k
+  T pc0 = 0, pl0 = 0, pd0 = 0, pcm = 0, plm = 0, pdm = 0, gc0 = 0, gl0 = 0, gd0 = 0;
+  T umc_p = 0, vmc_p = 0;
+  if (k0 < nz) {
+    const uint32_t q = (uint32_t)k0 * pm;
+    pc0 = ldg(P, mc + q); pl0 = ldg(P, ml + q); pd0 = ldg(P, md + q);
+  }
+  if (k0 >= 1) {
+    const uint32_t q = (uint32_t)(k0 - 1) * pm;
+    pcm = ldg(P, mc + q); plm = ldg(P, ml + q); pdm = ldg(P, md + q);
+    const uint32_t qu = (uint32_t)(k0 - 1) * pu, qv = (uint32_t)(k0 - 1) * pv;
+    umc_p = half * (ldg(U, uc + qu) + ldg(U, uc + qu + 1));
+    vmc_p = half * (ldg(V, vc + qv) + ldg(V, vc + qv + nx));
+  }
+  if (k0 == 0) {
+    gc0 = (ldg(P, mc + pm) - pc0) * ldg(o.rdzw, mc);
+    gl0 = (ldg(P, ml + pm) - pl0) * ldg(o.rdzw, ml);
+    gd0 = (ldg(P, md + pm) - pd0) * ldg(o.rdzw, md);
+  } else if (k0 < nz) {
+    const uint32_t q = (uint32_t)k0 * pm;
+    gc0 = (pc0 - pcm) * ldg(o.rdzw, mc + q);
+    gl0 = (pl0 - plm) * ldg(o.rdzw, ml + q);
+    gd0 = (pd0 - pdm) * ldg(o.rdzw, md + q);
+  }
+  T e0 = ldg(E, mc + (uint32_t)k0 * pm);
+  T fz0 = (k0 >= 1 && k0 < nz) ? e0 * ldg(o.cz, mc + (uint32_t)k0 * pm) : (T)0;
+#pragma unroll 1
+  for (int k = k0; k < k1; ++k) {
+    const uint32_t qm = (uint32_t)k * pm, qu = (uint32_t)k * pu, qv = (uint32_t)k * pv;
+    const uint32_t wk = mc + qm;  // own column at w level k (same plane layout)
+    const T wv = ldg(W, wk);
+    if (k == nz) {  // top w level: both rows are identities
+      stg(YW, wk, wv);
+      stg(YE, wk, e0 * ldg(o.rdzw, wk));
+      break;
     }
-    y[s.ou + p] = out;
-  } else if (seg == 1) {  // ---- L_v (euler.py:399, 416-420) ----
-    if (i >= nx || j > ny || k >= nz) return;
-    const int64_t p = s.v.at(i, j, k);
-    T out;
-    if (!s.py && (j == 0 || j == ny)) {
-      out = a.V[p];
-    } else {
-      const int f = (j == ny) ? 0 : j;
-      const int jm = f == 0 ? ny - 1 : f - 1;
-      const int64_t pf = s.v.at(i, f, k);
-      const T dpi = a.P[s.m.at(i, f, k)] - a.P[s.m.at(i, jm, k)];
-      const T gzv = (T)0.5 * (a.gzm(i, jm, k) + a.gzm(i, f, k));
-      const T uav = (T)0.5 * (a.um(i, jm, k) + a.um(i, f, k));
-      out = a.V[pf] + o.cva[pf] * (o.mvdy[s.v2(i, f)] * dpi - o.zev[pf] * gzv) + o.afv[s.v2(i, f)] * uav;
+    // ---- mass level k: gz_m at the three columns ----
+    T gc1, gl1, gd1, pc1 = 0, pl1 = 0, pd1 = 0;
+    if (k + 1 < nz) {
+      const uint32_t q1 = qm + pm;
+      pc1 = ldg(P, mc + q1); pl1 = ldg(P, ml + q1); pd1 = ldg(P, md + q1);
+      gc1 = (pc1 - pc0) * ldg(o.rdzw, mc + q1);
+      gl1 = (pl1 - pl0) * ldg(o.rdzw, ml + q1);
+      gd1 = (pd1 - pd0) * ldg(o.rdzw, md + q1);
+    } else {  // top closure: gz_w(nz) repeats the last interior difference
+      const uint32_t q1 = qm + pm;
+      gc1 = (pc0 - pcm) * ldg(o.rdzw, mc + q1);
+      gl1 = (pl0 - plm) * ldg(o.rdzw, ml + q1);
+      gd1 = (pd0 - pdm) * ldg(o.rdzw, md + q1);
     }
-    y[s.ov + p] = out;
-  } else if (seg == 2) {  // ---- L_w (euler.py:400, 426-427) ----
-    if (i >= nx || j >= ny) return;
-    const int64_t p = s.w.at(i, j, k);
-    T out;
-    if (k == 0 || k == nz) {
-      out = a.W[p];
-    } else {
-      out = o.dw[p] * a.W[p] + o.cw[p] * (a.P[s.m.at(i, j, k)] - a.P[s.m.at(i, j, k - 1)]);
+    const T gzc = half * (gc0 + gc1), gzl = half * (gl0 + gl1), gzd = half * (gd0 + gd1);
+    const uint32_t au = uc + qu, av = vc + qv;
+    const T u0 = ldg(U, au), u1 = ldg(U, au + 1), v0 = ldg(V, av), v1 = ldg(V, av + nx);
+    const T um_k = half * (u0 + u1), vm_k = half * (v0 + v1);
+    // ---- L_u at face i (euler.py:398, 411-415) ----
+    {
+      T out = u0;
+      if (uform) {
+        const uint32_t al = vl + qv;
+        const T vml = half * (ldg(V, al) + ldg(V, al + nx));
+        out = u0 + ldg(o.cua, au) * (mudx * (pc0 - pl0) - ldg(o.zxu, au) * (half * (gzl + gzc))) -
+              afu * (half * (vml + vm_k));
+      }
+      stg(YU, au, out);
+      if (i == nx - 1 && !s.px) stg(YU, au + 1, u1);   // closed east face: identity
+      if (i == 0 && s.px) stg(YU, ux + qu, out);       // periodic: face nx repeats face 0
     }
-    y[s.ow + p] = out;
-  } else if (seg == 3) {  // ---- L_pi (euler.py:298-318, 401) ----
-    if (i >= nx || j >= ny || k >= nz) return;
-    const int64_t p = s.m.at(i, j, k);
-    const T fx = a.U[s.u.at(i + 1, j, k)] * o.cx[s.u.at(i + 1, j, k)] - a.U[s.u.at(i, j, k)] * o.cx[s.u.at(i, j, k)];
-    const T fy = a.V[s.v.at(i, j + 1, k)] * o.cy[s.v.at(i, j + 1, k)] - a.V[s.v.at(i, j, k)] * o.cy[s.v.at(i, j, k)];
-    const T ftop = (k + 1 == nz) ? (T)0 : a.E[s.w.at(i, j, k + 1)] * o.cz[s.w.at(i, j, k + 1)];
-    const T fbot = (k == 0) ? (T)0 : a.E[s.w.at(i, j, k)] * o.cz[s.w.at(i, j, k)];
-    const int64_t q2 = s.m2(i, j);
-    y[s.om + p] = a.P[p] + o.cpm[p] * (o.mmx[q2] * fx + o.mmy[q2] * fy + (ftop - fbot));
-  } else {  // ---- L_eta (euler.py:429-437) ----
-    if (i >= nx || j >= ny) return;
-    const int64_t p = s.w.at(i, j, k);
-    T out;
-    if (k == 0 || k == nz) {
-      out = a.E[p] * o.rdzw[p];
-    } else {
-      const T uw = (T)0.5 * (a.um(i, j, k - 1) + a.um(i, j, k));
-      const T vw = (T)0.5 * (a.vm(i, j, k - 1) + a.vm(i, j, k));
-      out = o.alpha * (a.E[p] + o.kx[p] * uw + o.ky[p] * vw - a.W[p]);
+    // ---- L_v at face j (euler.py:399, 416-420) ----
+    {
+      T out = v0;
+      if (vform) {
+        const uint32_t ad = ud + qu;
+        const T umd = half * (ldg(U, ad) + ldg(U, ad + 1));
+        out = v0 + ldg(o.cva, av) * (mvdy * (pc0 - pd0) - ldg(o.zev, av) * (half * (gzd + gzc))) +
+              afv * (half * (umd + um_k));
+      }
+      stg(YV, av, out);
+      if (j == ny - 1 && !s.py) stg(YV, av + nx, v1);
+      if (j == 0 && s.py) stg(YV, vy + qv, out);
     }
-    y[s.oe + p] = out;
+    // ---- L_pi (euler.py:298-318, 401) ----
+    const uint32_t w1 = wk + pm;
+    const T e1 = ldg(E, w1);
+    const T fz1 = (k + 1 < nz) ? e1 * ldg(o.cz, w1) : (T)0;
+    {
+      const T fx = u1 * ldg(o.cx, au + 1) - u0 * ldg(o.cx, au);
+      const T fy = v1 * ldg(o.cy, av + nx) - v0 * ldg(o.cy, av);
+      stg(YP, wk, pc0 + ldg(o.cpm, wk) * (mmx * fx + mmy * fy + (fz1 - fz0)));
+    }
+    // ---- w level k: L_w (euler.py:400, 426-427), L_eta (euler.py:429-437) ----
+    if (k == 0) {
+      stg(YW, wk, wv);
+      stg(YE, wk, e0 * ldg(o.rdzw, wk));
+    } else {
+      stg(YW, wk, ldg(o.dw, wk) * wv + ldg(o.cw, wk) * (pc0 - pcm));
+      stg(YE, wk, alpha * (e0 + ldg(o.kx, wk) * (half * (umc_p + um_k)) +
+                           ldg(o.ky, wk) * (half * (vmc_p + vm_k)) - wv));
+    }
+    // k+1 becomes k
+    gc0 = gc1; gl0 = gl1; gd0 = gd1;
+    pcm = pc0; plm = pl0; pdm = pd0;
+    pc0 = pc1; pl0 = pl1; pd0 = pd1;
+    umc_p = um_k; vmc_p = vm_k;
+    e0 = e1; fz0 = fz1;
   }
 }
 
@@ -288,9 +353,16 @@ __global__ void __launch_bounds__(256) helmholtz_T2_kernel(OpT<T> o, const T *__
   xo[q] = out;
 }
 
+// Levels per thread of the forward operator: 8 chunks over the column give
+// ~8 waves per SIMD at 256x256 columns.
+inline int a_chunk(const Stag &s) { return (s.nz + 1 + 7) / 8; }
+
 template <typename T>
 int launch_A(const OpT<T> &o, const T *x, T *y, hipStream_t st, const int *stop) {
-  hipLaunchKernelGGL(helmholtz_kernel<T>, point_grid(o.s, 5), dim3(kTX, kTY), 0, st, o, x, y, stop);
+  if ((uint64_t)o.s.n * sizeof(T) >= (1ull << 32)) return SISL_ERR_ARG;  // 32-bit offsets
+  const int kc = a_chunk(o.s);
+  const int nb = ((o.s.nx + kTX - 1) / kTX) * ((o.s.ny + kTY - 1) / kTY) * ((o.s.nz + 1 + kc - 1) / kc);
+  hipLaunchKernelGGL(helmholtz_kernel<T>, dim3(nb), dim3(kTX, kTY), 0, st, o, x, y, kc, stop);
   SISL_CHECK_LAUNCH();
   return SISL_OK;
 }

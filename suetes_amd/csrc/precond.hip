@@ -2,73 +2,159 @@
 //
 // VerticalPreconditioner (suetes/regional3d/steppers.py:125-253): per column,
 // build the Helmholtz right-hand side, one Thomas solve, back-substitute w
-// and eta_dot.  One lane owns one (i, j) column and sweeps k in registers;
-// consecutive lanes own consecutive x columns, so every level's loads are
-// coalesced along x.  The Thomas factorisation (pivots, c') depends only on
-// the background state and is precomputed once per stepper.
+// and eta_dot.  Consecutive lanes own consecutive x columns, so every level's
+// loads are coalesced along x.  The Thomas factorisation (pivots, c') depends
+// only on the background state and is precomputed once per stepper.
 #include "common.h"
 
 namespace sisl {
 
-template <typename T>
-__global__ void __launch_bounds__(256) precond_kernel(OpT<T> o, const T *__restrict__ r,
-                                                      T *__restrict__ z,
-                                                      const int *__restrict__ stop) {
-  if (stop && *stop) return;
+// Forward M, segment-parallel.  A workgroup owns 64 consecutive columns of
+// one row j; wave `sg` owns levels [sg*L, sg*L + L) of those columns and keeps
+// them in registers.  With the Thomas factors fixed, forward and backward
+// substitution are first-order affine recurrences: each segment runs its
+// part with a zero incoming value while accumulating the product of the
+// recurrence coefficients, the segment ends are composed through LDS, and a
+// final multiply-add gives the exact sequential result up to rounding.  Every
+// input element is read once and every output element written once.
+constexpr int kMaxSeg = 8;  // at most 8 segments (512 threads): columns up to 256 levels
+
+template <typename T, int L>
+__global__ void __launch_bounds__(512) precond_seg_kernel(OpT<T> o, const T *__restrict__ r,
+                                                           T *__restrict__ z,
+                                                           const int *__restrict__ stop) {
+  if (stop && *stop) return;  // uniform over the launch
+  __shared__ T la[kMaxSeg][64], lb[kMaxSeg][64];
   const Stag &s = o.s;
   const int nx = s.nx, ny = s.ny, nz = s.nz;
-  const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (col >= (int64_t)nx * ny) return;
-  const int i = (int)(col % nx), j = (int)(col / nx);
+  const int nbx = (nx + 63) / 64;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int bx = tile % nbx, j = tile / nbx;
+  const int tx = threadIdx.x, sg = threadIdx.y, S = blockDim.y;
+  const int i = bx * 64 + tx;
+  const int ka = sg * L, kb = min(ka + L, nz);
+  const bool act = i < nx && ka < nz;
   const T *RU = r + s.ou, *RV = r + s.ov, *RW = r + s.ow, *RP = r + s.om, *RE = r + s.oe;
-  T *ZW = z + s.ow, *ZP = z + s.om, *ZE = z + s.oe;
-  const T a = o.alpha;
-  auto um = [&](int k) { return (T)0.5 * (RU[s.u.at(i, j, k)] + RU[s.u.at(i + 1, j, k)]); };
-  auto vm = [&](int k) { return (T)0.5 * (RV[s.v.at(i, j, k)] + RV[s.v.at(i, j + 1, k)]); };
-  const int64_t w0 = s.w.at(i, j, 0), wn = s.w.at(i, j, nz);
-  const T rw0 = RW[w0], rwn = RW[wn], re0 = RE[w0], ren = RE[wn];
-  // Sweep up: Helmholtz RHS (steppers.py:201-223) and Thomas forward elimination.
-  T wt_lo = 0, dprev = 0;
-  T um_k = um(0), vm_k = vm(0);
-  for (int k = 0; k < nz; ++k) {
-    T wt_hi = 0;
-    if (k + 1 < nz) {
-      const int64_t w = s.w.at(i, j, k + 1);
-      const T um1 = um(k + 1), vm1 = vm(k + 1);
-      const T kin = o.kx[w] * ((T)0.5 * (um_k + um1)) + o.ky[w] * ((T)0.5 * (vm_k + vm1));
-      const T re = RE[w];
-      const T wck = RW[w] * o.idw[w] + re / a - kin;
-      wt_hi = o.cz[w] * wck;
-      ZE[w] = re / a - kin;  // partial eta_dot back-substitution
-      um_k = um1;
-      vm_k = vm1;
+  T *ZU = z + s.ou, *ZV = z + s.ov, *ZW = z + s.ow, *ZP = z + s.om, *ZE = z + s.oe;
+  const T ra = (T)1 / o.alpha, half = (T)0.5;
+  const uint32_t pm = (uint32_t)nx * ny, pu = (uint32_t)(nx + 1) * ny, pv = (uint32_t)nx * (ny + 1);
+  const uint32_t mc = (uint32_t)j * nx + i, uc = (uint32_t)j * (nx + 1) + i, vc = (uint32_t)j * nx + i;
+  auto um = [&](int k) { const uint32_t q = uc + (uint32_t)k * pu; return half * (ldg(RU, q) + ldg(RU, q + 1)); };
+  auto vm = [&](int k) { const uint32_t q = vc + (uint32_t)k * pv; return half * (ldg(RV, q) + ldg(RV, q + nx)); };
+  T d[L], B[L], zep[L], rwi[L];
+  T dl = 0, bl = 1;
+  if (act) {
+    // ---- Helmholtz right-hand side (steppers.py:201-223) + local forward sweep ----
+    T um_k = um(ka), vm_k = vm(ka), wt_lo = 0;
+    if (ka >= 1) {
+      const uint32_t w = mc + (uint32_t)ka * pm;
+      const T kin = ldg(o.kx, w) * (half * (um(ka - 1) + um_k)) + ldg(o.ky, w) * (half * (vm(ka - 1) + vm_k));
+      zep[0] = ldg(RE, w) * ra - kin;
+      rwi[0] = ldg(RW, w) * ldg(o.idw, w);
+      wt_lo = ldg(o.cz, w) * (rwi[0] + zep[0]);
     }
-    const int64_t m = s.m.at(i, j, k);
-    const T rh = RP[m] * (T)1e-3 - o.lpi[m] * (wt_hi - wt_lo);
-    const T d = (rh - o.lower[m] * dprev) * o.invden[m];
-    ZP[m] = d;
-    dprev = d;
-    wt_lo = wt_hi;
-  }
-  // Sweep down: back substitution, w and eta_dot (steppers.py:230-245).
-  T xhi = 0;
-  for (int k = nz - 1; k >= 0; --k) {
-    const int64_t m = s.m.at(i, j, k);
-    const T x = ZP[m] - o.cprime[m] * xhi;
-    ZP[m] = x * (T)1000;
-    if (k + 1 <= nz - 1) {
-      const int64_t w = s.w.at(i, j, k + 1);
-      const T pw = RW[w] * o.idw[w] - o.gw[w] * (xhi - x);
-      ZW[w] = pw;
-      ZE[w] += pw;
+#pragma unroll
+    for (int t = 0; t < L; ++t) {
+      const int k = ka + t;
+      if (k < kb) {
+        T wt_hi = 0;
+        if (k + 1 < nz) {
+          const uint32_t w = mc + (uint32_t)(k + 1) * pm;
+          const T um1 = um(k + 1), vm1 = vm(k + 1);
+          const T kin = ldg(o.kx, w) * (half * (um_k + um1)) + ldg(o.ky, w) * (half * (vm_k + vm1));
+          const T zp = ldg(RE, w) * ra - kin, rw = ldg(RW, w) * ldg(o.idw, w);
+          wt_hi = ldg(o.cz, w) * (rw + zp);
+          if (t + 1 < L) { zep[t + 1] = zp; rwi[t + 1] = rw; }
+          um_k = um1;
+          vm_k = vm1;
+        }
+        const uint32_t m = mc + (uint32_t)k * pm;
+        const T rh = ldg(RP, m) * (T)1e-3 - ldg(o.lpi, m) * (wt_hi - wt_lo);
+        const T lo = ldg(o.lower, m), iv = ldg(o.invden, m);
+        dl = (rh - lo * dl) * iv;
+        bl = bl * (-lo * iv);
+        d[t] = dl;
+        B[t] = bl;
+        wt_lo = wt_hi;
+      }
     }
-    xhi = x;
   }
-  ZW[w0] = rw0;
-  ZW[wn] = rwn;
-  const int64_t q2 = s.m2(i, j);
-  ZE[w0] = re0 * o.dzw0[q2];
-  ZE[wn] = ren * o.dzwn[q2];
+  la[sg][tx] = dl;
+  lb[sg][tx] = bl;
+  __syncthreads();
+  T din = 0;
+  for (int t = 0; t < sg; ++t) din = la[t][tx] + lb[t][tx] * din;
+  __syncthreads();
+  // ---- local backward sweep: x = d - c' x_above ----
+  T xl = 0, gl = 1;
+  if (act) {
+#pragma unroll
+    for (int t = L - 1; t >= 0; --t) {
+      const int k = ka + t;
+      if (k < kb) {
+        const T dk = d[t] + B[t] * din;
+        const T cp = ldg(o.cprime, mc + (uint32_t)k * pm);
+        xl = dk - cp * xl;
+        gl = -cp * gl;
+        d[t] = xl;
+        B[t] = gl;
+      }
+    }
+  }
+  la[sg][tx] = xl;
+  lb[sg][tx] = gl;
+  __syncthreads();
+  T xin = 0;
+  for (int t = S - 1; t > sg; --t) xin = la[t][tx] + lb[t][tx] * xin;
+  T xtop = 0;
+  if (act) {
+#pragma unroll
+    for (int t = 0; t < L; ++t)
+      if (ka + t < kb) {
+        d[t] = d[t] + B[t] * xin;
+        xtop = d[t];
+      }
+  }
+  __syncthreads();
+  la[sg][tx] = xtop;
+  __syncthreads();
+  if (!act) return;
+  const T xbelow = sg > 0 ? la[sg - 1][tx] : (T)0;
+  // ---- outputs: pi, w and eta_dot back-substitution (steppers.py:230-245) ----
+#pragma unroll
+  for (int t = 0; t < L; ++t) {
+    const int k = ka + t;
+    if (k < kb) {
+      const uint32_t m = mc + (uint32_t)k * pm;
+      stg(ZP, m, d[t] * (T)1000);
+      if (k >= 1) {
+        const T xm1 = t == 0 ? xbelow : d[t > 0 ? t - 1 : 0];
+        const T pw = rwi[t] - ldg(o.gw, m) * (d[t] - xm1);
+        stg(ZW, m, pw);
+        stg(ZE, m, zep[t] + pw);
+      }
+    }
+  }
+  const uint32_t q2 = (uint32_t)s.m2(i, j);
+  if (ka == 0) {
+    stg(ZW, mc, ldg(RW, mc));
+    stg(ZE, mc, ldg(RE, mc) * ldg(o.dzw0, q2));
+  }
+  if (kb == nz) {
+    const uint32_t w = mc + (uint32_t)nz * pm;
+    stg(ZW, w, ldg(RW, w));
+    stg(ZE, w, ldg(RE, w) * ldg(o.dzwn, q2));
+  }
+  // ---- u, v pass through unchanged (steppers.py:247-253) ----
+  if (z != r) {
+    for (int k = ka; k < kb; ++k) {
+      const uint32_t qu = uc + (uint32_t)k * pu, qv = vc + (uint32_t)k * pv;
+      stg(ZU, qu, ldg(RU, qu));
+      if (i == nx - 1) stg(ZU, qu + 1, ldg(RU, qu + 1));
+      stg(ZV, qv, ldg(RV, qv));
+      if (j == ny - 1) stg(ZV, qv + nx, ldg(RV, qv + nx));
+    }
+  }
 }
 
 // Transpose: column sweeps of the reversed chain; the u/v cotangents of the
@@ -169,16 +255,33 @@ __global__ void __launch_bounds__(256) precond_T_faces_kernel(OpT<T> o, const T 
   rb[q] = out;
 }
 
+// Levels per segment: the smallest supported L with at most kMaxSeg
+// segments per column.
 template <typename T>
-int launch_M(const OpT<T> &o, const T *r, T *z, hipStream_t st, const int *stop) {
-  if (r != z) {
-    if (hipMemcpyAsync(z, r, sizeof(T) * o.s.ow, hipMemcpyDeviceToDevice, st) != hipSuccess)
-      return SISL_ERR_LAUNCH;
+static int launch_M_seg(const OpT<T> &o, const T *r, T *z, hipStream_t st, const int *stop) {
+  const int nz = o.s.nz;
+  static const int sizes[] = {2, 4, 8, 10, 12, 16, 24, 32};
+  int L = 0;
+  for (int c : sizes)
+    if ((nz + c - 1) / c <= kMaxSeg) { L = c; break; }
+  if (!L) return SISL_ERR_ARG;  // taller than kMaxSeg * 32 levels
+  const int S = (nz + L - 1) / L;
+  const dim3 grid((unsigned)(((o.s.nx + 63) / 64) * o.s.ny)), block(64, S);
+  switch (L) {
+#define SISL_M_CASE(c) \
+  case c: hipLaunchKernelGGL((precond_seg_kernel<T, c>), grid, block, 0, st, o, r, z, stop); break;
+    SISL_M_CASE(2) SISL_M_CASE(4) SISL_M_CASE(8) SISL_M_CASE(10) SISL_M_CASE(12) SISL_M_CASE(16)
+    SISL_M_CASE(24) SISL_M_CASE(32)
+#undef SISL_M_CASE
   }
-  const int64_t cols = (int64_t)o.s.nx * o.s.ny;
-  hipLaunchKernelGGL(precond_kernel<T>, dim3(grid_for(cols)), dim3(kBlock), 0, st, o, r, z, stop);
   SISL_CHECK_LAUNCH();
   return SISL_OK;
+}
+
+template <typename T>
+int launch_M(const OpT<T> &o, const T *r, T *z, hipStream_t st, const int *stop) {
+  if ((uint64_t)o.s.n * sizeof(T) >= (1ull << 32)) return SISL_ERR_ARG;  // 32-bit offsets
+  return launch_M_seg(o, r, z, st, stop);
 }
 
 template <typename T>
