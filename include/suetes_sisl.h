@@ -169,6 +169,8 @@ typedef struct SislGmresParams {
                               Givens residual estimate meets the target       */
   double inner_rtol;       /* mixed solver: inner FP32 cycle target (relative)*/
   int32_t max_outer;       /* mixed solver: refinement iterations             */
+  int32_t basis_fp16;      /* mixed solver: keep the inner Krylov basis in
+                              FP16 (scaled by sqrt(n)) instead of FP32        */
 } SislGmresParams;
 
 typedef struct SislGmresStats {

@@ -96,24 +96,41 @@ __global__ void __launch_bounds__(256) reduce_partials_kernel(const double *__re
   }
 }
 
-// 16-byte vector access: 2 doubles or 4 floats per lane per load.
+// Elements per lane per access: 16-byte loads of the working vectors (2
+// doubles or 4 floats); a basis stored in FP16 is read 4 halves (8 bytes)
+// at a time alongside FP32 working vectors.
 template <typename TV> struct Vec;
-template <> struct Vec<double> { static constexpr int N = 2; using T = double2; };
-template <> struct Vec<float> { static constexpr int N = 4; using T = float4; };
+template <> struct Vec<double> { static constexpr int N = 2; };
+template <> struct Vec<float> { static constexpr int N = 4; };
+template <> struct Vec<_Float16> { static constexpr int N = 4; };
 
-template <typename TV>
-__device__ __forceinline__ void vload(const TV *__restrict__ p, int64_t q, TV (&a)[Vec<TV>::N]) {
-  union { typename Vec<TV>::T v; TV e[Vec<TV>::N]; } u;
-  u.v = reinterpret_cast<const typename Vec<TV>::T *>(p)[q];
-#pragma unroll
-  for (int k = 0; k < Vec<TV>::N; ++k) a[k] = u.e[k];
+template <int Bytes> struct Raw;
+template <> struct Raw<16> { using T = uint4; };
+template <> struct Raw<8> { using T = uint2; };
+
+// Basis storage scale: a basis kept in FP16 holds sqrt(n) * v (entries O(1)
+// for a unit vector, far from FP16 overflow and underflow); other bases
+// hold v itself.
+template <typename TB>
+inline double basis_scale(int64_t n) {
+  return sizeof(TB) == 2 ? std::sqrt((double)n) : 1.0;
 }
-template <typename TV>
-__device__ __forceinline__ void vstore(TV *__restrict__ p, int64_t q, const TV (&a)[Vec<TV>::N]) {
-  union { typename Vec<TV>::T v; TV e[Vec<TV>::N]; } u;
+
+template <int N, typename T>
+__device__ __forceinline__ void vload(const T *__restrict__ p, int64_t q, T (&a)[N]) {
+  using R = typename Raw<sizeof(T) * N>::T;
+  union { R v; T e[N]; } u;
+  u.v = reinterpret_cast<const R *>(p)[q];
 #pragma unroll
-  for (int k = 0; k < Vec<TV>::N; ++k) u.e[k] = a[k];
-  reinterpret_cast<typename Vec<TV>::T *>(p)[q] = u.v;
+  for (int k = 0; k < N; ++k) a[k] = u.e[k];
+}
+template <int N, typename T>
+__device__ __forceinline__ void vstore(T *__restrict__ p, int64_t q, const T (&a)[N]) {
+  using R = typename Raw<sizeof(T) * N>::T;
+  union { R v; T e[N]; } u;
+#pragma unroll
+  for (int k = 0; k < N; ++k) u.e[k] = a[k];
+  reinterpret_cast<R *>(p)[q] = u.v;
 }
 
 // Vectors are visited in groups of 8 whose loads carry no branches (indices
@@ -128,12 +145,13 @@ __device__ __forceinline__ bool pass_skipped(const int *stop, const int *gate) {
   return (stop && *stop) || (gate && !*gate);
 }
 
-// partial[v][block] = sum over the block's elements of V[v] . w for every v in
-// [v0, v0+8) of every group, plus (slot nv) w . w.  FP64 accumulation.
-template <typename TV>
-__global__ void __launch_bounds__(256) dots_kernel(const TV *__restrict__ V, int64_t ldv, int nv,
+// partial[v][block] = vinv * sum over the block's elements of V[v] . w for
+// every v in [v0, v0+8) of every group, plus (slot nv) w . w.  FP64
+// accumulation; vinv undoes the basis storage scale.
+template <typename TB, typename TV>
+__global__ void __launch_bounds__(256) dots_kernel(const TB *__restrict__ V, int64_t ldv, int nv,
                                                    const TV *__restrict__ w, int64_t n,
-                                                   double *__restrict__ partials,
+                                                   double *__restrict__ partials, double vinv,
                                                    const int *__restrict__ stop,
                                                    const int *__restrict__ gate) {
   if (pass_skipped(stop, gate)) return;
@@ -146,7 +164,7 @@ __global__ void __launch_bounds__(256) dots_kernel(const TV *__restrict__ V, int
     double acc[1] = {0.0};
     for (int64_t q = t0; q < nq; q += stride) {
       TV we[N];
-      vload(w, q, we);
+      vload<N>(w, q, we);
 #pragma unroll
       for (int k = 0; k < N; ++k) acc[0] += (double)we[k] * (double)we[k];
     }
@@ -158,14 +176,15 @@ __global__ void __launch_bounds__(256) dots_kernel(const TV *__restrict__ V, int
     double acc[kGrp];
 #pragma unroll
     for (int r = 0; r < kGrp; ++r) acc[r] = 0;
-    const TV *base[kGrp];
+    const TB *base[kGrp];
 #pragma unroll
     for (int r = 0; r < kGrp; ++r) base[r] = V + (int64_t)min(g * kGrp + r, nv - 1) * ldv;
     for (int64_t q = t0; q < nq; q += stride) {
-      TV we[N], x[kGrp][N];
-      vload(w, q, we);
+      TV we[N];
+      TB x[kGrp][N];
+      vload<N>(w, q, we);
 #pragma unroll
-      for (int r = 0; r < kGrp; ++r) vload(base[r], q, x[r]);
+      for (int r = 0; r < kGrp; ++r) vload<N>(base[r], q, x[r]);
 #pragma unroll
       for (int r = 0; r < kGrp; ++r)
 #pragma unroll
@@ -177,23 +196,25 @@ __global__ void __launch_bounds__(256) dots_kernel(const TV *__restrict__ V, int
       for (int r = 0; r < kGrp; ++r) acc[r] += (double)base[r][e] * (double)w[e];
     }
     const int cnt = min(kGrp, nv - g * kGrp);
+#pragma unroll
+    for (int r = 0; r < kGrp; ++r) acc[r] *= vinv;
     block_store_group(acc, g * kGrp, cnt, partials);
   }
 }
 
-// w <- w - sum_v h[v] V[v] (h read from device memory), slot 0 <- w . w
-template <typename TV>
-__global__ void __launch_bounds__(256) update_kernel(const TV *__restrict__ V, int64_t ldv, int nv,
+// w <- w - sum_v h[v] vinv V[v] (h read from device memory), slot 0 <- w . w
+template <typename TB, typename TV>
+__global__ void __launch_bounds__(256) update_kernel(const TB *__restrict__ V, int64_t ldv, int nv,
                                                      const double *__restrict__ h,
                                                      TV *__restrict__ w, int64_t n,
-                                                     double *__restrict__ partials,
+                                                     double *__restrict__ partials, double vinv,
                                                      const int *__restrict__ stop,
                                                      const int *__restrict__ gate) {
   if (pass_skipped(stop, gate)) return;
   constexpr int N = Vec<TV>::N;
   __shared__ double hs[kMaxRestart + 8];
   const int nvr = ((nv + kGrp - 1) / kGrp) * kGrp;
-  for (int v = threadIdx.x; v < nvr; v += blockDim.x) hs[v] = v < nv ? h[v] : 0.0;
+  for (int v = threadIdx.x; v < nvr; v += blockDim.x) hs[v] = v < nv ? h[v] * vinv : 0.0;
   __syncthreads();
   const int64_t nq = n / N;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -201,15 +222,15 @@ __global__ void __launch_bounds__(256) update_kernel(const TV *__restrict__ V, i
   double ww = 0;
   for (int64_t q = t0; q < nq; q += stride) {
     TV we[N];
-    vload(w, q, we);
+    vload<N>(w, q, we);
     double s[N];
 #pragma unroll
     for (int k = 0; k < N; ++k) s[k] = (double)we[k];
 #pragma unroll 1
     for (int g = 0; g * kGrp < nv; ++g) {
-      TV x[kGrp][N];
+      TB x[kGrp][N];
 #pragma unroll
-      for (int r = 0; r < kGrp; ++r) vload(V + (int64_t)min(g * kGrp + r, nv - 1) * ldv, q, x[r]);
+      for (int r = 0; r < kGrp; ++r) vload<N>(V + (int64_t)min(g * kGrp + r, nv - 1) * ldv, q, x[r]);
 #pragma unroll
       for (int r = 0; r < kGrp; ++r) {
         const double hv = hs[g * kGrp + r];  // zero past nv
@@ -220,7 +241,7 @@ __global__ void __launch_bounds__(256) update_kernel(const TV *__restrict__ V, i
     TV sn[N];
 #pragma unroll
     for (int k = 0; k < N; ++k) sn[k] = (TV)s[k];
-    vstore(w, q, sn);
+    vstore<N>(w, q, sn);
 #pragma unroll
     for (int k = 0; k < N; ++k) ww += (double)sn[k] * (double)sn[k];
   }
@@ -289,34 +310,15 @@ __global__ void finalize_kernel(Small *sm, int j, int R, int early_exit) {
   if (early_exit && fabs(sm->g[j + 1]) <= sm->target) sm->broken = 1;
 }
 
-// V[j+1] = w * inv; nothing once the cycle has stopped (those columns are
-// outside the least squares).
-template <typename TV>
-__global__ void normalize_kernel(const TV *__restrict__ w, TV *__restrict__ out, int64_t n,
-                                 const Small *__restrict__ sm, int stopped_before) {
-  if (stopped_before) return;
-  const double inv = sm->inv;
-  constexpr int N = Vec<TV>::N;
-  const int64_t nq = n / N;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  for (int64_t q = t0; q < nq; q += stride) {
-    TV a[N];
-    vload(w, q, a);
-#pragma unroll
-    for (int k = 0; k < N; ++k) a[k] = inv != 0.0 ? (TV)((double)a[k] * inv) : (TV)0;
-    vstore(out, q, a);
-  }
-  if (t0 < n - nq * N) {
-    const int64_t e = nq * N + t0;
-    out[e] = inv != 0.0 ? (TV)((double)w[e] * inv) : (TV)0;
-  }
-}
-
-// V[j+1] = w * inv, gated on the device stop flag (used inside a cycle)
-template <typename TV>
-__global__ void normalize_gated_kernel(const TV *__restrict__ w, TV *__restrict__ out, int64_t n,
-                                       const Small *__restrict__ sm, const int *__restrict__ stop_before) {
+// V[j+1] = vscale * w * inv (basis storage) and, when `cur` is given, the
+// unscaled working copy cur = w * inv that the next operator apply reads;
+// nothing once the cycle has stopped (those columns are outside the least
+// squares).
+template <typename TB, typename TV>
+__global__ void normalize_gated_kernel(const TV *__restrict__ w, TB *__restrict__ out,
+                                       TV *__restrict__ cur, int64_t n, double vscale,
+                                       const Small *__restrict__ sm,
+                                       const int *__restrict__ stop_before) {
   if (*stop_before) return;
   const double inv = sm->inv;
   constexpr int N = Vec<TV>::N;
@@ -325,29 +327,41 @@ __global__ void normalize_gated_kernel(const TV *__restrict__ w, TV *__restrict_
   const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   for (int64_t q = t0; q < nq; q += stride) {
     TV a[N];
-    vload(w, q, a);
+    TB o[N];
+    vload<N>(w, q, a);
 #pragma unroll
-    for (int k = 0; k < N; ++k) a[k] = inv != 0.0 ? (TV)((double)a[k] * inv) : (TV)0;
-    vstore(out, q, a);
+    for (int k = 0; k < N; ++k) {
+      const double u = inv != 0.0 ? (double)a[k] * inv : 0.0;
+      a[k] = (TV)u;
+      o[k] = (TB)(u * vscale);
+    }
+    vstore<N>(out, q, o);
+    if (cur) vstore<N>(cur, q, a);
   }
   if (t0 < n - nq * N) {
     const int64_t e = nq * N + t0;
-    out[e] = inv != 0.0 ? (TV)((double)w[e] * inv) : (TV)0;
+    const double u = inv != 0.0 ? (double)w[e] * inv : 0.0;
+    out[e] = (TB)(u * vscale);
+    if (cur) cur[e] = (TV)u;
   }
 }
 
 // snapshot of the stop flag taken before a step's finalize
 __global__ void snapshot_kernel(const Small *sm, int *dst) { *dst = sm->broken; }
 
-// out = src / ||src|| safe-normalised (JAX _safe_normalize, thresh eps)
-template <typename TS, typename TD>
-__global__ void unit_kernel(const TS *__restrict__ src, TD *__restrict__ out, int64_t n,
-                            const double *__restrict__ norm2) {
+// u = src / ||src|| safe-normalised (JAX _safe_normalize, thresh eps):
+// out = vscale * u (basis storage), cur = u when given.
+template <typename TS, typename TB, typename TC>
+__global__ void unit_kernel(const TS *__restrict__ src, TB *__restrict__ out, TC *__restrict__ cur,
+                            int64_t n, double vscale, const double *__restrict__ norm2) {
   const double nr = sqrt(*norm2);
   const double inv = nr > 2.220446049250313e-16 ? 1.0 / nr : 0.0;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += stride)
-    out[e] = inv != 0.0 ? (TD)((double)src[e] * inv) : (TD)0;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += stride) {
+    const double u = inv != 0.0 ? (double)src[e] * inv : 0.0;
+    out[e] = (TB)(u * vscale);
+    if (cur) cur[e] = (TC)u;
+  }
 }
 
 // t = b - t
@@ -358,15 +372,16 @@ __global__ void residual_kernel(const T *__restrict__ b, T *__restrict__ t, int6
     t[e] = b[e] - t[e];
 }
 
-// x += sum_v y[v] V[v]   (x is FP64)
+// x += sum_v y[v] vinv V[v]   (x is FP64)
 template <typename TV>
 __global__ void __launch_bounds__(256) combine_kernel(const TV *__restrict__ V, int64_t ldv, int nv,
                                                       const double *__restrict__ y,
-                                                      double *__restrict__ x, int64_t n) {
+                                                      double *__restrict__ x, int64_t n,
+                                                      double vinv) {
   constexpr int N = Vec<TV>::N;
   __shared__ double ys[kMaxRestart + 8];
   const int nvr = ((nv + kGrp - 1) / kGrp) * kGrp;
-  for (int v = threadIdx.x; v < nvr; v += blockDim.x) ys[v] = v < nv ? y[v] : 0.0;
+  for (int v = threadIdx.x; v < nvr; v += blockDim.x) ys[v] = v < nv ? y[v] * vinv : 0.0;
   __syncthreads();
   const int64_t nq = n / N;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -379,7 +394,7 @@ __global__ void __launch_bounds__(256) combine_kernel(const TV *__restrict__ V, 
     for (int g = 0; g * kGrp < nv; ++g) {
       TV xv[kGrp][N];
 #pragma unroll
-      for (int r = 0; r < kGrp; ++r) vload(V + (int64_t)min(g * kGrp + r, nv - 1) * ldv, q, xv[r]);
+      for (int r = 0; r < kGrp; ++r) vload<N>(V + (int64_t)min(g * kGrp + r, nv - 1) * ldv, q, xv[r]);
 #pragma unroll
       for (int r = 0; r < kGrp; ++r)
 #pragma unroll
@@ -415,23 +430,25 @@ __global__ void reset_cycle_kernel(Small *sm, int R, double g0, double target) {
 
 // ---- host-side dispatch -------------------------------------------------------
 
-template <typename TV>
-static void dots(const TV *V, int64_t ldv, int nv, const TV *w, int64_t n, double *partials,
-                 hipStream_t s, const int *stop = nullptr, const int *gate = nullptr) {
-  hipLaunchKernelGGL(dots_kernel<TV>, dim3(kRedGrid), dim3(kBlock), 0, s, V, ldv, nv, w, n, partials,
-                     stop, gate);
+template <typename TB, typename TV>
+static void dots(const TB *V, int64_t ldv, int nv, const TV *w, int64_t n, double *partials,
+                 hipStream_t s, double vinv = 1.0, const int *stop = nullptr,
+                 const int *gate = nullptr) {
+  hipLaunchKernelGGL((dots_kernel<TB, TV>), dim3(kRedGrid), dim3(kBlock), 0, s, V, ldv, nv, w, n,
+                     partials, vinv, stop, gate);
 }
-template <typename TV>
-static void update(const TV *V, int64_t ldv, int nv, const double *h, TV *w, int64_t n,
-                   double *partials, hipStream_t s, const int *stop = nullptr,
+template <typename TB, typename TV>
+static void update(const TB *V, int64_t ldv, int nv, const double *h, TV *w, int64_t n,
+                   double *partials, hipStream_t s, double vinv = 1.0, const int *stop = nullptr,
                    const int *gate = nullptr) {
-  hipLaunchKernelGGL(update_kernel<TV>, dim3(kRedGrid), dim3(kBlock), 0, s, V, ldv, nv, h, w, n,
-                     partials, stop, gate);
+  hipLaunchKernelGGL((update_kernel<TB, TV>), dim3(kRedGrid), dim3(kBlock), 0, s, V, ldv, nv, h, w,
+                     n, partials, vinv, stop, gate);
 }
-template <typename TV>
-static void combine(const TV *V, int64_t ldv, int nv, const double *y, double *x, int64_t n,
-                    hipStream_t s) {
-  hipLaunchKernelGGL(combine_kernel<TV>, dim3(kRedGrid), dim3(kBlock), 0, s, V, ldv, nv, y, x, n);
+template <typename TB>
+static void combine(const TB *V, int64_t ldv, int nv, const double *y, double *x, int64_t n,
+                    double vinv, hipStream_t s) {
+  hipLaunchKernelGGL(combine_kernel<TB>, dim3(kRedGrid), dim3(kBlock), 0, s, V, ldv, nv, y, x, n,
+                     vinv);
 }
 static void reduce(const double *partials, int nslots, double *out, hipStream_t s,
                    const int *stop = nullptr, const int *gate = nullptr) {
@@ -444,7 +461,7 @@ static inline int64_t basis_ld(int64_t n) { return (n + 63) & ~(int64_t)63; }
 
 // Workspace carve-up (byte offsets).
 struct Layout {
-  int64_t V, w, t, scr, partials, small, flags, V32, w32, t32, scr32, total;
+  int64_t V, w, t, scr, partials, small, flags, V32, w32, t32, scr32, cur32, total;
 };
 
 static Layout layout(const Stag &s, int R, bool mixed) {
@@ -461,7 +478,8 @@ static Layout layout(const Stag &s, int R, bool mixed) {
   if (!mixed) {
     L.V = take(sizeof(double) * basis_ld(n) * (R + 1));
   } else {
-    L.V32 = take(sizeof(float) * basis_ld(n) * (R + 1));
+    L.V32 = take(sizeof(float) * basis_ld(n) * (R + 1));  // FP32 basis, or FP16 in its first half
+    L.cur32 = take(sizeof(float) * basis_ld(n));
     L.w32 = take(sizeof(float) * basis_ld(n));
     L.t32 = take(sizeof(float) * basis_ld(n));
     L.scr32 = take(sizeof(float) * 3 * nm);
@@ -505,17 +523,28 @@ static void host_lstsq(const double *H, int R, int s, double beta, double *y) {
   }
 }
 
-template <typename TV>
+// One restart cycle's Arnoldi process.  TV: type of the operator applies and
+// working vectors; TB: basis storage type (TV, or FP16 under the FP32
+// working type).  With an FP16 basis, `cur` holds the newest basis vector
+// unscaled in TV for the next operator apply.
+template <typename TV, typename TB = TV>
 struct Krylov {
   const OpT<TV> &op;
   int transpose;
-  TV *V, *w, *t, *scr;
+  TB *V;
+  TV *w, *t, *scr;
   double *partials;
   Small *sm;
   int *flags;   // flags[j]: stop flag snapshot before step j
   int64_t n;
   hipStream_t s;
+  TV *cur = nullptr;
   int64_t ldv() const { return basis_ld(n); }
+  double vscale() const { return basis_scale<TB>(n); }
+  const TV *basis_in(int j) const {
+    if constexpr (sizeof(TB) == sizeof(TV)) return reinterpret_cast<const TV *>(V) + (int64_t)j * ldv();
+    else return cur;
+  }
 
   // out <- (M A) in  (or M^T A^T); kernels skip when *stop != 0
   int apply(const TV *in, TV *out, const int *stop) {
@@ -533,22 +562,24 @@ struct Krylov {
     int rc;
     const int64_t ld = ldv();
     int *stop = flags + j;
+    const double vs = vscale(), vinv = 1.0 / vs;
     hipLaunchKernelGGL(snapshot_kernel, dim3(1), dim3(1), 0, s, (const Small *)sm, stop);
-    if ((rc = apply(V + (int64_t)j * ld, w, stop))) return rc;
+    if ((rc = apply(basis_in(j), w, stop))) return rc;
     const int nv = j + 1;
     const int *gate = &sm->gate2;
-    dots<TV>(V, ld, nv, w, n, partials, s, stop);
+    dots<TB, TV>(V, ld, nv, w, n, partials, s, vinv, stop);
     reduce(partials, nv + 1, sm->h1, s, stop);             // h1[0..nv-1], ||w||^2 in h1[nv]
-    update<TV>(V, ld, nv, sm->h1, w, n, partials, s, stop);
+    update<TB, TV>(V, ld, nv, sm->h1, w, n, partials, s, vinv, stop);
     reduce(partials, 1, sm->nn1a, s, stop);
     hipLaunchKernelGGL(decide_kernel, dim3(1), dim3(1), 0, s, sm, nv);
-    dots<TV>(V, ld, nv, w, n, partials, s, stop, gate);
+    dots<TB, TV>(V, ld, nv, w, n, partials, s, vinv, stop, gate);
     reduce(partials, nv, sm->h2, s, stop, gate);
-    update<TV>(V, ld, nv, sm->h2, w, n, partials, s, stop, gate);
+    update<TB, TV>(V, ld, nv, sm->h2, w, n, partials, s, vinv, stop, gate);
     reduce(partials, 1, sm->nn1b, s, stop, gate);
     hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(64), 0, s, sm, j, R, early_exit);
-    hipLaunchKernelGGL(normalize_gated_kernel<TV>, dim3(kRedGrid), dim3(kBlock), 0, s, (const TV *)w,
-                       V + (int64_t)(j + 1) * ld, n, (const Small *)sm, (const int *)stop);
+    hipLaunchKernelGGL((normalize_gated_kernel<TB, TV>), dim3(kRedGrid), dim3(kBlock), 0, s,
+                       (const TV *)w, V + (int64_t)(j + 1) * ld, cur, n, vs, (const Small *)sm,
+                       (const int *)stop);
     SISL_CHECK_LAUNCH();
     return SISL_OK;
   }
@@ -557,7 +588,7 @@ struct Krylov {
 // squared norm of a vector into *out (device)
 template <typename TV>
 static void norm2(const TV *x, int64_t n, double *partials, double *out, hipStream_t s) {
-  dots<TV>(x, n, 0, x, n, partials, s);
+  dots<TV, TV>(x, n, 0, x, n, partials, s);
   reduce(partials, 1, out, s);
 }
 
@@ -602,8 +633,9 @@ static int gmres64(const OpT<double> &op, const SislGmresParams &p, const double
       if ((r2 = launch_MT<double>(op, K.t, K.w, K.scr, s))) return r2;
     }
     norm2<double>(K.w, n, K.partials, scal_slot, s);
-    hipLaunchKernelGGL((unit_kernel<double, double>), dim3(kRedGrid), dim3(kBlock), 0, s,
-                       (const double *)K.w, K.V, n, (const double *)scal_slot);
+    hipLaunchKernelGGL((unit_kernel<double, double, double>), dim3(kRedGrid), dim3(kBlock), 0, s,
+                       (const double *)K.w, K.V, (double *)nullptr, n, 1.0,
+                       (const double *)scal_slot);
     SISL_CHECK_LAUNCH();
     return SISL_OK;
   };
@@ -630,7 +662,7 @@ static int gmres64(const OpT<double> &op, const SislGmresParams &p, const double
     host_lstsq(H.data(), R, steps, rnorm, y);
     if (hipMemcpyAsync(sm->y, y, sizeof(double) * steps, hipMemcpyHostToDevice, s) != hipSuccess)
       return SISL_ERR_LAUNCH;
-    combine<double>(K.V, K.ldv(), steps, sm->y, x, n, s);
+    combine<double>(K.V, K.ldv(), steps, sm->y, x, n, 1.0, s);
     if ((rc = residual(&sm->scal[2]))) return rc;
     if ((rc = sync_copy(hs, sm->scal, sizeof(double) * 3, s))) return rc;
     rnorm = std::sqrt(hs[2]);
@@ -647,7 +679,9 @@ static int gmres64(const OpT<double> &op, const SislGmresParams &p, const double
   return SISL_OK;
 }
 
-// Mixed: FP64 residual refinement around FP32 GMRES cycles with early exit.
+// Mixed: FP64 residual refinement around FP32 GMRES cycles with early exit;
+// TB is the inner basis storage type (float, or FP16 scaled by sqrt(n)).
+template <typename TB>
 static int gmres_mixed(const OpT<double> &op64, const OpT<float> &op32, const SislGmresParams &p,
                        const double *b, double *x, char *ws, SislGmresStats *st, hipStream_t s) {
   const int R = p.restart;
@@ -656,9 +690,10 @@ static int gmres_mixed(const OpT<double> &op64, const OpT<float> &op32, const Si
   Small *sm = (Small *)(ws + L.small);
   double *partials = (double *)(ws + L.partials);
   double *w64 = (double *)(ws + L.w), *t64 = (double *)(ws + L.t), *scr64 = (double *)(ws + L.scr);
-  Krylov<float> K{op32, p.transpose, (float *)(ws + L.V32), (float *)(ws + L.w32),
-                  (float *)(ws + L.t32), (float *)(ws + L.scr32), partials, sm,
-                  (int *)(ws + L.flags), n, s};
+  Krylov<float, TB> K{op32, p.transpose, (TB *)(ws + L.V32), (float *)(ws + L.w32),
+                      (float *)(ws + L.t32), (float *)(ws + L.scr32), partials, sm,
+                      (int *)(ws + L.flags), n, s};
+  if constexpr (sizeof(TB) != sizeof(float)) K.cur = (float *)(ws + L.cur32);
   int rc;
   double hs[4];
   auto M64 = [&](double *in, double *out) -> int {
@@ -689,8 +724,8 @@ static int gmres_mixed(const OpT<double> &op64, const OpT<float> &op32, const Si
     hipLaunchKernelGGL(residual_kernel<double>, dim3(kRedGrid), dim3(kBlock), 0, s, b, t64, n);
     if ((rc = M64(t64, w64))) return rc;
     norm2<double>(w64, n, partials, &sm->scal[2], s);
-    hipLaunchKernelGGL((unit_kernel<double, float>), dim3(kRedGrid), dim3(kBlock), 0, s,
-                       (const double *)w64, K.V, n, (const double *)&sm->scal[2]);
+    hipLaunchKernelGGL((unit_kernel<double, TB, float>), dim3(kRedGrid), dim3(kBlock), 0, s,
+                       (const double *)w64, K.V, K.cur, n, K.vscale(), (const double *)&sm->scal[2]);
     if ((rc = sync_copy(hs, sm->scal, sizeof(double) * 3, s))) return rc;
     rnorm = std::sqrt(hs[2]);
     if (!(rnorm > 2.220446049250313e-16) && !std::isnan(rnorm)) rnorm = 0.0;
@@ -706,7 +741,7 @@ static int gmres_mixed(const OpT<double> &op64, const OpT<float> &op32, const Si
     host_lstsq(H.data(), R, steps, rnorm, y);
     if (hipMemcpyAsync(sm->y, y, sizeof(double) * steps, hipMemcpyHostToDevice, s) != hipSuccess)
       return SISL_ERR_LAUNCH;
-    combine<float>(K.V, K.ldv(), steps, sm->y, x, n, s);
+    combine<TB>(K.V, K.ldv(), steps, sm->y, x, n, 1.0 / K.vscale(), s);
     ++outer;
   }
   if (st) {
@@ -750,7 +785,9 @@ int sisl_gmres_mixed(const SislOperator *op64, const SislOperator *op32, const S
   auto o64 = sisl::OpT<double>::make(*op64);
   auto o32 = sisl::OpT<float>::make(*op32);
   if (!o64.complete() || !o32.complete()) return SISL_ERR_ARG;
-  return sisl::gmres_mixed(o64, o32, *p, b, x, (char *)ws, st, sisl::as_stream(stream));
+  if (p->basis_fp16)
+    return sisl::gmres_mixed<_Float16>(o64, o32, *p, b, x, (char *)ws, st, sisl::as_stream(stream));
+  return sisl::gmres_mixed<float>(o64, o32, *p, b, x, (char *)ws, st, sisl::as_stream(stream));
 }
 
 int sisl_dot_f64(const double *x, const double *y, int64_t n, double *out, double *partials,

@@ -62,7 +62,9 @@ def build_dynamical_core(core_type, grid, operators, constants, initial_state, p
     history keys to ``initial_state`` in place.
 
     Extra keyword arguments of this implementation: ``precision`` ('fp64' or
-    'mixed'), ``inner_rtol`` (mixed solver), ``solver_early_exit``."""
+    'mixed'), ``inner_rtol``, ``inner_restart`` and ``inner_basis`` ('fp16'
+    or 'fp32': storage of the FP32 inner Krylov basis) of the mixed solver,
+    ``solver_early_exit``."""
     core = core_type.lower()
     if core == "split-explicit":
         raise NotImplementedError("the split-explicit core is not part of this build")
@@ -72,7 +74,8 @@ def build_dynamical_core(core_type, grid, operators, constants, initial_state, p
          "nu_h_factor": 0.1, "N_bv": 0.01, "alpha": 0.55, "use_limiter": False,
          "use_checkpointing": False, "solver_tol": 1e-12, "solver_maxiter": 100,
          "solver_restart": 100, "background_state": None, "precision": "fp64",
-         "inner_rtol": 1e-4, "inner_restart": None, "solver_early_exit": False}
+         "inner_rtol": 1e-4, "inner_restart": None, "inner_basis": "fp16",
+         "solver_early_exit": False}
     p.update(kwargs)
     ref = p["background_state"] if p["background_state"] is not None else initial_state
     physics = Euler3D(grid, operators, constants, dt=p["dt"], initial_era5_state=ref,
@@ -93,7 +96,8 @@ def build_dynamical_core(core_type, grid, operators, constants, initial_state, p
                             use_checkpointing=p["use_checkpointing"], solver_tol=p["solver_tol"],
                             solver_maxiter=p["solver_maxiter"], solver_restart=p["solver_restart"],
                             precision=p["precision"], inner_rtol=p["inner_rtol"],
-                            inner_restart=p["inner_restart"], solver_early_exit=p["solver_early_exit"])
+                            inner_restart=p["inner_restart"], inner_basis=p["inner_basis"],
+                            solver_early_exit=p["solver_early_exit"])
     return stepper, p["dt"]
 
 
@@ -250,12 +254,14 @@ class SemiImplicitSolver3D:
     """GMRES solve of the acoustic/gravity system (steppers.py:377-527).
 
     ``precision='fp64'`` runs the reference algorithm (every restart cycle
-    runs ``solver_restart`` Arnoldi steps); ``'mixed'`` runs FP32 operator,
-    preconditioner and Krylov basis inside an FP64 residual-refinement loop
-    with the same stopping test."""
+    runs ``solver_restart`` Arnoldi steps); ``'mixed'`` runs FP32 operator and
+    preconditioner with an FP16 (``inner_basis='fp16'``, scaled by sqrt(n))
+    or FP32 Krylov basis inside an FP64 residual-refinement loop with the
+    same stopping test."""
 
     def __init__(self, physics, dt, alpha=0.55, solver_tol=1e-4, solver_maxiter=20, solver_restart=20,
-                 precision="fp64", inner_rtol=1e-4, early_exit=False, inner_restart=None):
+                 precision="fp64", inner_rtol=1e-4, early_exit=False, inner_restart=None,
+                 inner_basis="fp16"):
         self.physics, self.dt, self.alpha = physics, dt, alpha
         self.pi_scale = 1000.0
         self.solver_tol, self.solver_maxiter, self.solver_restart = solver_tol, solver_maxiter, solver_restart
@@ -264,6 +270,9 @@ class SemiImplicitSolver3D:
         self.precision = precision
         self.inner_rtol = inner_rtol
         self.inner_restart = inner_restart
+        if inner_basis not in ("fp16", "fp32"):
+            raise ValueError("inner_basis must be 'fp16' or 'fp32'")
+        self.inner_basis = inner_basis
         self.early_exit = early_exit
         self._bg_id = None
         self.last_stats = []
@@ -296,7 +305,7 @@ class SemiImplicitSolver3D:
     def _params(self, transpose):
         return _lib.SislGmresParams(float(self.solver_tol), 0.0, self._restart(),
                                     int(self.solver_maxiter), int(transpose), int(bool(self.early_exit)),
-                                    float(self.inner_rtol), 0)
+                                    float(self.inner_rtol), 0, int(self.inner_basis == "fp16"))
 
     def solve_flat(self, b, x0, transpose=False):
         """Solve on packed scaled vectors; returns a new tensor."""
@@ -388,7 +397,7 @@ class SISLStepper3D:
 
     def __init__(self, physics, dt, alpha=0.55, use_limiter=False, use_checkpointing=False,
                  solver_tol=1e-4, solver_maxiter=20, solver_restart=20, precision="fp64",
-                 inner_rtol=1e-4, solver_early_exit=False, inner_restart=None):
+                 inner_rtol=1e-4, solver_early_exit=False, inner_restart=None, inner_basis="fp16"):
         self.physics, self.dt, self.alpha = physics, dt, alpha
         self.use_limiter = use_limiter
         self.precision = precision
@@ -399,7 +408,8 @@ class SISLStepper3D:
                                                     solver_maxiter=solver_maxiter,
                                                     solver_restart=solver_restart, precision=precision,
                                                     inner_rtol=inner_rtol, early_exit=solver_early_exit,
-                                                    inner_restart=inner_restart)
+                                                    inner_restart=inner_restart,
+                                                    inner_basis=inner_basis)
         self.use_checkpointing = use_checkpointing
         self.tracer_keys = physics.physics_suite.tracer_keys if physics.physics_suite is not None else []
         self._bg = None

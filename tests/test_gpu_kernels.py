@@ -205,14 +205,15 @@ def test_gmres_solution_matches_oracle():
     assert float(r.norm() / c.norm()) < 1e-9
 
 
-def test_mixed_solver_reaches_tolerance():
+@pytest.mark.parametrize("basis", ["fp16", "fp32"])
+def test_mixed_solver_reaches_tolerance(basis):
     cfg = small_lam()
     case, go, A_o, M_o, sol = _solver_pair(cfg)
     rng = np.random.default_rng(22)
     b = pack_ref(A_o(_rand_krylov(go, rng)))
     from suetes_amd.steppers import SemiImplicitSolver3D
     ms = SemiImplicitSolver3D(sol.physics, sol.dt, sol.alpha, solver_tol=1e-10, solver_maxiter=30,
-                              solver_restart=20, precision="mixed")
+                              solver_restart=20, precision="mixed", inner_basis=basis)
     ms.prepare(case.stepper.bg_precomputed())
     x = ms.solve_flat(b, torch.zeros_like(b))
     st = ms.last_stats[-1]
