@@ -136,6 +136,17 @@ int sisl_departure_f32(const SislGrid *g, int32_t loc, const float *u, const flo
  * scaled units, suetes/regional3d/steppers.py:439-461). */
 int sisl_helmholtz_apply_f64(const SislOperator *op, const double *x, double *y, void *stream);
 int sisl_helmholtz_apply_f32(const SislOperator *op, const float *x, float *y, void *stream);
+/* Autotuner of the operator apply: times every launch shape (levels per
+ * thread, loop unroll) on the caller's packed buffers x (input, any finite
+ * values) and y (scratch output) and keeps the fastest for this grid and
+ * element type (fp32 = 1: the _f32 entry points) for the rest of the
+ * process.  All shapes give bit-identical results.  Optional outputs: the
+ * chosen chunk and unroll. */
+int sisl_autotune_operator(const SislOperator *op, int32_t fp32, void *x, void *y, int32_t *chunk,
+                           int32_t *unroll, void *stream);
+/* Pin the operator launch shape for grid g and element type (chunk >= 2
+ * levels per thread, unroll 1 or 2), e.g. to reproduce a tuned choice. */
+int sisl_set_operator_shape(const SislGrid *g, int32_t fp32, int32_t chunk, int32_t unroll);
 /* y = A^T x (the transpose used by the GMRES adjoint solve,
  * steppers.py:473-481). scratch: 3 mass-point arrays. */
 int sisl_helmholtz_apply_T_f64(const SislOperator *op, const double *x, double *y,
@@ -193,6 +204,23 @@ int sisl_gmres_f64(const SislOperator *op64, const SislGmresParams *p, const dou
 int sisl_gmres_mixed(const SislOperator *op64, const SislOperator *op32,
                      const SislGmresParams *p, const double *b, double *x, void *workspace,
                      SislGmresStats *stats, void *stream);
+
+/* ---- kernel timing (measurement runs) ------------------------------------ */
+
+/* Timed kernels: operator apply, column preconditioner, the Krylov
+ * projection / update / normalisation passes, the tricubic gather. */
+enum SislTimedKernel {
+  SISL_T_HELMHOLTZ = 0, SISL_T_PRECOND = 1, SISL_T_KRYLOV_DOTS = 2, SISL_T_KRYLOV_UPDATE = 3,
+  SISL_T_KRYLOV_NORMALIZE = 4, SISL_T_TRICUBIC = 5, SISL_T_COUNT = 6
+};
+/* on = 1: clear the record and bracket every later launch of a timed kernel
+ * with HIP events on its stream; on = 0: stop recording (the record stays). */
+int sisl_timer_enable(int32_t on);
+/* Totals of one kernel (fp32 = 1: FP32 working-type variant) since the last
+ * enable: event time of all launches, number of launches, launches that did
+ * work (not retired by a device stop flag), algorithmic bytes of those. */
+int sisl_timer_read(int32_t kernel, int32_t fp32, double *total_ms, int64_t *launches,
+                    int64_t *working_launches, double *bytes);
 
 /* Packed-vector helpers used by the host layer. out = sum over d of x*y (FP64). */
 int sisl_dot_f64(const double *x, const double *y, int64_t n, double *out_dev, double *partials,

@@ -22,7 +22,13 @@ EXPORTED = [
     "sisl_helmholtz_apply_T_f32", "sisl_precond_apply_f64", "sisl_precond_apply_f32",
     "sisl_precond_apply_T_f64", "sisl_precond_apply_T_f32", "sisl_gmres_workspace_bytes_f64",
     "sisl_gmres_workspace_bytes_mixed", "sisl_gmres_f64", "sisl_gmres_mixed", "sisl_dot_f64",
+    "sisl_timer_enable", "sisl_timer_read", "sisl_autotune_operator",
+    "sisl_set_operator_shape",
 ]
+
+# timed kernels (enum SislTimedKernel)
+TIMED = {"helmholtz": 0, "precond": 1, "krylov_dots": 2, "krylov_update": 3,
+         "krylov_normalize": 4, "tricubic": 5}
 
 
 class SislGrid(C.Structure):
@@ -87,8 +93,27 @@ def lib():
                                        C.POINTER(SislGmresParams), vp, vp, vp,
                                        C.POINTER(SislGmresStats), vp]
         L.sisl_dot_f64.argtypes = [vp, vp, i64, vp, vp, vp]
+        L.sisl_autotune_operator.argtypes = [C.POINTER(SislOperator), i32, vp, vp, C.POINTER(i32),
+                                             C.POINTER(i32), vp]
+        L.sisl_set_operator_shape.argtypes = [C.POINTER(SislGrid), i32, i32, i32]
+        L.sisl_timer_enable.argtypes = [i32]
+        L.sisl_timer_read.argtypes = [i32, i32, C.POINTER(d), C.POINTER(i64), C.POINTER(i64),
+                                      C.POINTER(d)]
         _lib = L
     return _lib
+
+
+def timer_enable(on: bool):
+    check(lib().sisl_timer_enable(int(bool(on))), "sisl_timer_enable")
+
+
+def timer_read(kernel: str, fp32: bool):
+    """(total_ms, launches, working_launches, algorithmic_bytes) of one timed
+    kernel since the last timer_enable(True)."""
+    ms, n, nw, by = C.c_double(), C.c_int64(), C.c_int64(), C.c_double()
+    check(lib().sisl_timer_read(TIMED[kernel], int(bool(fp32)), C.byref(ms), C.byref(n), C.byref(nw),
+                                C.byref(by)), "sisl_timer_read")
+    return ms.value, n.value, nw.value, by.value
 
 
 def check(rc: int, what: str):

@@ -118,6 +118,10 @@ __device__ inline T wave_sum(T v) {
 }  // namespace sisl
 
 namespace sisl {
+// Optional launch timing (timing.hip): returns a record id or -1 when off.
+int timer_begin(int kind, bool fp32, double bytes, const int *stop, const int *gate, hipStream_t s);
+void timer_end(int id, hipStream_t s);
+
 // Launchers shared between translation units (explicitly instantiated for
 // float and double next to their kernels).
 // `stop` (device int, optional): the kernels return at once when *stop != 0

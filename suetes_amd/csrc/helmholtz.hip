@@ -5,6 +5,11 @@
 // u | v | w | pi*1000 | eta_dot*dz_w.  Every coefficient that the reference
 // recomputes per call (background averages, metrics, map factors) is
 // precomputed once per stepper by the host layer.
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
 #include "common.h"
 
 namespace sisl {
@@ -46,7 +51,7 @@ struct AView {
 // blockIdx.z enumerating (segment, level).
 constexpr int kTX = 64, kTY = 4;
 
-template <typename T>
+template <typename T, int UNR>
 __global__ void __launch_bounds__(256) helmholtz_kernel(OpT<T> o, const T *__restrict__ x,
                                                         T *__restrict__ y, int kchunk,
                                                         const int *__restrict__ stop) {
@@ -67,6 +72,20 @@ __global__ void __launch_bounds__(256) helmholtz_kernel(OpT<T> o, const T *__res
   const T mvdy = ldg(o.mvdy, (uint32_t)s.v2(i, j)), afv = ldg(o.afv, (uint32_t)s.v2(i, j));
   const T mmx = ldg(o.mmx, (uint32_t)s.m2(i, j)), mmy = ldg(o.mmy, (uint32_t)s.m2(i, j));
   const T half = (T)0.5, alpha = o.alpha;
+  // coefficient arrays as restrict locals: loads of level k+1 may move above stores of level k
+  const T *__restrict__ c_cua = o.cua;
+  const T *__restrict__ c_zxu = o.zxu;
+  const T *__restrict__ c_cx = o.cx;
+  const T *__restrict__ c_cva = o.cva;
+  const T *__restrict__ c_zev = o.zev;
+  const T *__restrict__ c_cy = o.cy;
+  const T *__restrict__ c_dw = o.dw;
+  const T *__restrict__ c_cw = o.cw;
+  const T *__restrict__ c_cz = o.cz;
+  const T *__restrict__ c_kx = o.kx;
+  const T *__restrict__ c_ky = o.ky;
+  const T *__restrict__ c_rdzw = o.rdzw;
+  const T *__restrict__ c_cpm = o.cpm;
   // 32-bit element offsets at level 0 and the plane strides: own column
   // (i, j), left column (il, j), lower column (i, jd).  Mass and w points
   // share the (i, j) plane layout.
@@ -92,25 +111,25 @@ __global__ void __launch_bounds__(256) helmholtz_kernel(OpT<T> o, const T *__res
     vmc_p = half * (ldg(V, vc + qv) + ldg(V, vc + qv + nx));
   }
   if (k0 == 0) {
-    gc0 = (ldg(P, mc + pm) - pc0) * ldg(o.rdzw, mc);
-    gl0 = (ldg(P, ml + pm) - pl0) * ldg(o.rdzw, ml);
-    gd0 = (ldg(P, md + pm) - pd0) * ldg(o.rdzw, md);
+    gc0 = (ldg(P, mc + pm) - pc0) * ldg(c_rdzw, mc);
+    gl0 = (ldg(P, ml + pm) - pl0) * ldg(c_rdzw, ml);
+    gd0 = (ldg(P, md + pm) - pd0) * ldg(c_rdzw, md);
   } else if (k0 < nz) {
     const uint32_t q = (uint32_t)k0 * pm;
-    gc0 = (pc0 - pcm) * ldg(o.rdzw, mc + q);
-    gl0 = (pl0 - plm) * ldg(o.rdzw, ml + q);
-    gd0 = (pd0 - pdm) * ldg(o.rdzw, md + q);
+    gc0 = (pc0 - pcm) * ldg(c_rdzw, mc + q);
+    gl0 = (pl0 - plm) * ldg(c_rdzw, ml + q);
+    gd0 = (pd0 - pdm) * ldg(c_rdzw, md + q);
   }
   T e0 = ldg(E, mc + (uint32_t)k0 * pm);
-  T fz0 = (k0 >= 1 && k0 < nz) ? e0 * ldg(o.cz, mc + (uint32_t)k0 * pm) : (T)0;
-#pragma unroll 1
+  T fz0 = (k0 >= 1 && k0 < nz) ? e0 * ldg(c_cz, mc + (uint32_t)k0 * pm) : (T)0;
+#pragma unroll UNR
   for (int k = k0; k < k1; ++k) {
     const uint32_t qm = (uint32_t)k * pm, qu = (uint32_t)k * pu, qv = (uint32_t)k * pv;
     const uint32_t wk = mc + qm;  // own column at w level k (same plane layout)
     const T wv = ldg(W, wk);
     if (k == nz) {  // top w level: both rows are identities
       stg(YW, wk, wv);
-      stg(YE, wk, e0 * ldg(o.rdzw, wk));
+      stg(YE, wk, e0 * ldg(c_rdzw, wk));
       break;
     }
     // ---- mass level k: gz_m at the three columns ----
@@ -118,14 +137,14 @@ __global__ void __launch_bounds__(256) helmholtz_kernel(OpT<T> o, const T *__res
     if (k + 1 < nz) {
       const uint32_t q1 = qm + pm;
       pc1 = ldg(P, mc + q1); pl1 = ldg(P, ml + q1); pd1 = ldg(P, md + q1);
-      gc1 = (pc1 - pc0) * ldg(o.rdzw, mc + q1);
-      gl1 = (pl1 - pl0) * ldg(o.rdzw, ml + q1);
-      gd1 = (pd1 - pd0) * ldg(o.rdzw, md + q1);
+      gc1 = (pc1 - pc0) * ldg(c_rdzw, mc + q1);
+      gl1 = (pl1 - pl0) * ldg(c_rdzw, ml + q1);
+      gd1 = (pd1 - pd0) * ldg(c_rdzw, md + q1);
     } else {  // top closure: gz_w(nz) repeats the last interior difference
       const uint32_t q1 = qm + pm;
-      gc1 = (pc0 - pcm) * ldg(o.rdzw, mc + q1);
-      gl1 = (pl0 - plm) * ldg(o.rdzw, ml + q1);
-      gd1 = (pd0 - pdm) * ldg(o.rdzw, md + q1);
+      gc1 = (pc0 - pcm) * ldg(c_rdzw, mc + q1);
+      gl1 = (pl0 - plm) * ldg(c_rdzw, ml + q1);
+      gd1 = (pd0 - pdm) * ldg(c_rdzw, md + q1);
     }
     const T gzc = half * (gc0 + gc1), gzl = half * (gl0 + gl1), gzd = half * (gd0 + gd1);
     const uint32_t au = uc + qu, av = vc + qv;
@@ -137,7 +156,7 @@ __global__ void __launch_bounds__(256) helmholtz_kernel(OpT<T> o, const T *__res
       if (uform) {
         const uint32_t al = vl + qv;
         const T vml = half * (ldg(V, al) + ldg(V, al + nx));
-        out = u0 + ldg(o.cua, au) * (mudx * (pc0 - pl0) - ldg(o.zxu, au) * (half * (gzl + gzc))) -
+        out = u0 + ldg(c_cua, au) * (mudx * (pc0 - pl0) - ldg(c_zxu, au) * (half * (gzl + gzc))) -
               afu * (half * (vml + vm_k));
       }
       stg(YU, au, out);
@@ -150,7 +169,7 @@ __global__ void __launch_bounds__(256) helmholtz_kernel(OpT<T> o, const T *__res
       if (vform) {
         const uint32_t ad = ud + qu;
         const T umd = half * (ldg(U, ad) + ldg(U, ad + 1));
-        out = v0 + ldg(o.cva, av) * (mvdy * (pc0 - pd0) - ldg(o.zev, av) * (half * (gzd + gzc))) +
+        out = v0 + ldg(c_cva, av) * (mvdy * (pc0 - pd0) - ldg(c_zev, av) * (half * (gzd + gzc))) +
               afv * (half * (umd + um_k));
       }
       stg(YV, av, out);
@@ -160,20 +179,20 @@ __global__ void __launch_bounds__(256) helmholtz_kernel(OpT<T> o, const T *__res
     // ---- L_pi (euler.py:298-318, 401) ----
     const uint32_t w1 = wk + pm;
     const T e1 = ldg(E, w1);
-    const T fz1 = (k + 1 < nz) ? e1 * ldg(o.cz, w1) : (T)0;
+    const T fz1 = (k + 1 < nz) ? e1 * ldg(c_cz, w1) : (T)0;
     {
-      const T fx = u1 * ldg(o.cx, au + 1) - u0 * ldg(o.cx, au);
-      const T fy = v1 * ldg(o.cy, av + nx) - v0 * ldg(o.cy, av);
-      stg(YP, wk, pc0 + ldg(o.cpm, wk) * (mmx * fx + mmy * fy + (fz1 - fz0)));
+      const T fx = u1 * ldg(c_cx, au + 1) - u0 * ldg(c_cx, au);
+      const T fy = v1 * ldg(c_cy, av + nx) - v0 * ldg(c_cy, av);
+      stg(YP, wk, pc0 + ldg(c_cpm, wk) * (mmx * fx + mmy * fy + (fz1 - fz0)));
     }
     // ---- w level k: L_w (euler.py:400, 426-427), L_eta (euler.py:429-437) ----
     if (k == 0) {
       stg(YW, wk, wv);
-      stg(YE, wk, e0 * ldg(o.rdzw, wk));
+      stg(YE, wk, e0 * ldg(c_rdzw, wk));
     } else {
-      stg(YW, wk, ldg(o.dw, wk) * wv + ldg(o.cw, wk) * (pc0 - pcm));
-      stg(YE, wk, alpha * (e0 + ldg(o.kx, wk) * (half * (umc_p + um_k)) +
-                           ldg(o.ky, wk) * (half * (vmc_p + vm_k)) - wv));
+      stg(YW, wk, ldg(c_dw, wk) * wv + ldg(c_cw, wk) * (pc0 - pcm));
+      stg(YE, wk, alpha * (e0 + ldg(c_kx, wk) * (half * (umc_p + um_k)) +
+                           ldg(c_ky, wk) * (half * (vmc_p + vm_k)) - wv));
     }
     // k+1 becomes k
     gc0 = gc1; gl0 = gl1; gd0 = gd1;
@@ -353,16 +372,43 @@ __global__ void __launch_bounds__(256) helmholtz_T2_kernel(OpT<T> o, const T *__
   xo[q] = out;
 }
 
-// Levels per thread of the forward operator: 8 chunks over the column give
-// ~8 waves per SIMD at 256x256 columns.
-inline int a_chunk(const Stag &s) { return (s.nz + 1 + 7) / 8; }
+// Launch shape of the forward operator: levels per thread (chunk) and the
+// unroll of the level loop.  Every shape computes bit-identical results; the
+// autotuner below picks one per grid and element type by timing, the
+// default is 8 chunks per column without unrolling.
+struct AShape {
+  int kc, unr;
+};
+static std::mutex g_tune_mu;
+static std::map<std::tuple<int, int, int, int>, AShape> g_tune;
+
+static AShape a_shape(const Stag &s, int bytes) {
+  std::lock_guard<std::mutex> g(g_tune_mu);
+  auto it = g_tune.find({s.nx, s.ny, s.nz, bytes});
+  if (it != g_tune.end()) return it->second;
+  return AShape{(s.nz + 1 + 7) / 8, 1};
+}
+
+template <typename T>
+static int launch_A_shape(const OpT<T> &o, const T *x, T *y, hipStream_t st, const int *stop, AShape sh) {
+  const int kc = sh.kc;
+  const int nb = ((o.s.nx + kTX - 1) / kTX) * ((o.s.ny + kTY - 1) / kTY) * ((o.s.nz + 1 + kc - 1) / kc);
+  if (sh.unr == 2)
+    hipLaunchKernelGGL((helmholtz_kernel<T, 2>), dim3(nb), dim3(kTX, kTY), 0, st, o, x, y, kc, stop);
+  else
+    hipLaunchKernelGGL((helmholtz_kernel<T, 1>), dim3(nb), dim3(kTX, kTY), 0, st, o, x, y, kc, stop);
+  return SISL_OK;
+}
 
 template <typename T>
 int launch_A(const OpT<T> &o, const T *x, T *y, hipStream_t st, const int *stop) {
   if ((uint64_t)o.s.n * sizeof(T) >= (1ull << 32)) return SISL_ERR_ARG;  // 32-bit offsets
-  const int kc = a_chunk(o.s);
-  const int nb = ((o.s.nx + kTX - 1) / kTX) * ((o.s.ny + kTY - 1) / kTY) * ((o.s.nz + 1 + kc - 1) / kc);
-  hipLaunchKernelGGL(helmholtz_kernel<T>, dim3(nb), dim3(kTX, kTY), 0, st, o, x, y, kc, stop);
+  const AShape sh = a_shape(o.s, (int)sizeof(T));
+  const Stag &s = o.s;
+  const double bytes = (double)sizeof(T) * (2.0 * s.n + 3.0 * s.u.n() + 3.0 * s.v.n() + 6.0 * s.w.n() + s.m.n());
+  const int tid = timer_begin(SISL_T_HELMHOLTZ, sizeof(T) == 4, bytes, stop, nullptr, st);
+  launch_A_shape(o, x, y, st, stop, sh);
+  timer_end(tid, st);
   SISL_CHECK_LAUNCH();
   return SISL_OK;
 }
@@ -377,6 +423,45 @@ int launch_AT(const OpT<T> &o, const T *y, T *x, T *scratch, hipStream_t st, con
   hipLaunchKernelGGL(helmholtz_T2_kernel<T>, point_grid(o.s, 5), dim3(kTX, kTY), 0, st, o, y,
                      (const T *)g, (const T *)um, (const T *)vm, x, stop);
   SISL_CHECK_LAUNCH();
+  return SISL_OK;
+}
+
+// Time every launch shape of the forward operator on the caller's buffers
+// and keep the fastest for this grid and element type.
+template <typename T>
+static int autotune_A(const OpT<T> &o, const T *x, T *y, hipStream_t st, int32_t *best_kc,
+                      int32_t *best_unr) {
+  if ((uint64_t)o.s.n * sizeof(T) >= (1ull << 32)) return SISL_ERR_ARG;
+  const int levels = o.s.nz + 1;
+  std::vector<AShape> cand;
+  for (int parts : {1, 2, 4, 6, 8, 12, 16})
+    for (int unr : {1, 2}) {
+      const int kc = (levels + parts - 1) / parts;
+      if (kc >= 2) cand.push_back({kc, unr});
+    }
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return SISL_ERR_LAUNCH;
+  AShape best{(levels + 7) / 8, 1};
+  float best_ms = 1e30f;
+  for (const AShape &c : cand) {
+    for (int r = 0; r < 2; ++r) launch_A_shape(o, x, y, st, nullptr, c);
+    hipEventRecord(e0, st);
+    for (int r = 0; r < 5; ++r) launch_A_shape(o, x, y, st, nullptr, c);
+    hipEventRecord(e1, st);
+    if (hipEventSynchronize(e1) != hipSuccess) return SISL_ERR_LAUNCH;
+    float ms = 0;
+    hipEventElapsedTime(&ms, e0, e1);
+    if (ms < best_ms) { best_ms = ms; best = c; }
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  SISL_CHECK_LAUNCH();
+  {
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    g_tune[{o.s.nx, o.s.ny, o.s.nz, (int)sizeof(T)}] = best;
+  }
+  if (best_kc) *best_kc = best.kc;
+  if (best_unr) *best_unr = best.unr;
   return SISL_OK;
 }
 
@@ -404,6 +489,25 @@ int sisl_helmholtz_apply_f32(const SislOperator *op, const float *x, float *y, v
   auto o = sisl::OpT<float>::make(*op);
   if (!o.complete()) return SISL_ERR_ARG;
   return sisl::launch_A(o, x, y, sisl::as_stream(s));
+}
+int sisl_set_operator_shape(const SislGrid *g, int32_t fp32, int32_t chunk, int32_t unroll) {
+  if (!g || chunk < 2 || (unroll != 1 && unroll != 2)) return SISL_ERR_ARG;
+  std::lock_guard<std::mutex> lk(sisl::g_tune_mu);
+  sisl::g_tune[{g->nx, g->ny, g->nz, fp32 ? 4 : 8}] = sisl::AShape{chunk, unroll};
+  return SISL_OK;
+}
+
+int sisl_autotune_operator(const SislOperator *op, int32_t fp32, void *x, void *y, int32_t *chunk,
+                           int32_t *unroll, void *s) {
+  if (!op || !x || !y || x == y) return SISL_ERR_ARG;
+  if (fp32) {
+    auto o = sisl::OpT<float>::make(*op);
+    if (!o.complete()) return SISL_ERR_ARG;
+    return sisl::autotune_A(o, (const float *)x, (float *)y, sisl::as_stream(s), chunk, unroll);
+  }
+  auto o = sisl::OpT<double>::make(*op);
+  if (!o.complete()) return SISL_ERR_ARG;
+  return sisl::autotune_A(o, (const double *)x, (double *)y, sisl::as_stream(s), chunk, unroll);
 }
 int sisl_helmholtz_apply_T_f64(const SislOperator *op, const double *x, double *y, double *scr,
                                void *s) {

@@ -390,8 +390,11 @@ template <typename T>
 int tricubic(int fnx, int fny, int fnz, const T *f, const T *co, T *out, int lim, void *st) {
   if (!f || !co || !out || fnx < 1 || fny < 1 || fnz < 1) return SISL_ERR_ARG;
   Dims d{fnx, fny, fnz};
+  const int tid = timer_begin(SISL_T_TRICUBIC, sizeof(T) == 4, (double)sizeof(T) * 5.0 * d.n(),
+                              nullptr, nullptr, as_stream(st));
   hipLaunchKernelGGL(tricubic_kernel<T>, dim3(grid_for(d.n())), dim3(kBlock), 0, as_stream(st),
                      d, f, co, out, lim);
+  timer_end(tid, as_stream(st));
   SISL_CHECK_LAUNCH();
   return SISL_OK;
 }

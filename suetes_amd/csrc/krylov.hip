@@ -202,6 +202,61 @@ __global__ void __launch_bounds__(256) dots_kernel(const TB *__restrict__ V, int
   }
 }
 
+// Single-sweep variant for nv <= 8*G: w is read once, all G groups of
+// projections and w . w accumulate together (8*G + 1 FP64 accumulators per
+// lane).  Same partial-slot layout as dots_kernel.
+template <typename TB, typename TV, int G>
+__global__ void __launch_bounds__(256) dots1_kernel(const TB *__restrict__ V, int64_t ldv, int nv,
+                                                    const TV *__restrict__ w, int64_t n,
+                                                    double *__restrict__ partials, double vinv,
+                                                    const int *__restrict__ stop,
+                                                    const int *__restrict__ gate) {
+  if (pass_skipped(stop, gate)) return;
+  constexpr int N = Vec<TV>::N;
+  const int64_t nq = n / N;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double acc[G][kGrp];
+  double ww = 0;
+#pragma unroll
+  for (int g = 0; g < G; ++g)
+#pragma unroll
+    for (int r = 0; r < kGrp; ++r) acc[g][r] = 0;
+  for (int64_t q = t0; q < nq; q += stride) {
+    TV we[N];
+    vload<N>(w, q, we);
+#pragma unroll
+    for (int k = 0; k < N; ++k) ww += (double)we[k] * (double)we[k];
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      TB x[kGrp][N];
+#pragma unroll
+      for (int r = 0; r < kGrp; ++r) vload<N>(V + (int64_t)min(g * kGrp + r, nv - 1) * ldv, q, x[r]);
+#pragma unroll
+      for (int r = 0; r < kGrp; ++r)
+#pragma unroll
+        for (int k = 0; k < N; ++k) acc[g][r] += (double)x[r][k] * (double)we[k];
+    }
+  }
+  if (t0 < n - nq * N) {
+    const int64_t e = nq * N + t0;
+    const double we = (double)w[e];
+    ww += we * we;
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+#pragma unroll
+      for (int r = 0; r < kGrp; ++r) acc[g][r] += (double)V[(int64_t)min(g * kGrp + r, nv - 1) * ldv + e] * we;
+  }
+  block_store_slot(ww, nv, partials);
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    if (g * kGrp >= nv) break;
+#pragma unroll
+    for (int r = 0; r < kGrp; ++r) acc[g][r] *= vinv;
+    block_store_group(acc[g], g * kGrp, min(kGrp, nv - g * kGrp), partials);
+  }
+}
+
 // w <- w - sum_v h[v] vinv V[v] (h read from device memory), slot 0 <- w . w
 template <typename TB, typename TV>
 __global__ void __launch_bounds__(256) update_kernel(const TB *__restrict__ V, int64_t ldv, int nv,
@@ -434,15 +489,32 @@ template <typename TB, typename TV>
 static void dots(const TB *V, int64_t ldv, int nv, const TV *w, int64_t n, double *partials,
                  hipStream_t s, double vinv = 1.0, const int *stop = nullptr,
                  const int *gate = nullptr) {
-  hipLaunchKernelGGL((dots_kernel<TB, TV>), dim3(kRedGrid), dim3(kBlock), 0, s, V, ldv, nv, w, n,
-                     partials, vinv, stop, gate);
+  const double bytes = (double)n * ((double)nv * sizeof(TB) + sizeof(TV));
+  const int tid = timer_begin(SISL_T_KRYLOV_DOTS, sizeof(TV) == 4, bytes, stop, gate, s);
+  const dim3 gr(kRedGrid), bl(kBlock);
+  if (nv <= 0)
+    hipLaunchKernelGGL((dots_kernel<TB, TV>), gr, bl, 0, s, V, ldv, nv, w, n, partials, vinv, stop, gate);
+  else if (nv <= kGrp)
+    hipLaunchKernelGGL((dots1_kernel<TB, TV, 1>), gr, bl, 0, s, V, ldv, nv, w, n, partials, vinv, stop, gate);
+  else if (nv <= 2 * kGrp)
+    hipLaunchKernelGGL((dots1_kernel<TB, TV, 2>), gr, bl, 0, s, V, ldv, nv, w, n, partials, vinv, stop, gate);
+  else if (nv <= 3 * kGrp)
+    hipLaunchKernelGGL((dots1_kernel<TB, TV, 3>), gr, bl, 0, s, V, ldv, nv, w, n, partials, vinv, stop, gate);
+  else if (nv <= 4 * kGrp)
+    hipLaunchKernelGGL((dots1_kernel<TB, TV, 4>), gr, bl, 0, s, V, ldv, nv, w, n, partials, vinv, stop, gate);
+  else
+    hipLaunchKernelGGL((dots_kernel<TB, TV>), gr, bl, 0, s, V, ldv, nv, w, n, partials, vinv, stop, gate);
+  timer_end(tid, s);
 }
 template <typename TB, typename TV>
 static void update(const TB *V, int64_t ldv, int nv, const double *h, TV *w, int64_t n,
                    double *partials, hipStream_t s, double vinv = 1.0, const int *stop = nullptr,
                    const int *gate = nullptr) {
+  const double bytes = (double)n * ((double)nv * sizeof(TB) + 2.0 * sizeof(TV));
+  const int tid = timer_begin(SISL_T_KRYLOV_UPDATE, sizeof(TV) == 4, bytes, stop, gate, s);
   hipLaunchKernelGGL((update_kernel<TB, TV>), dim3(kRedGrid), dim3(kBlock), 0, s, V, ldv, nv, h, w,
                      n, partials, vinv, stop, gate);
+  timer_end(tid, s);
 }
 template <typename TB>
 static void combine(const TB *V, int64_t ldv, int nv, const double *y, double *x, int64_t n,
@@ -577,9 +649,12 @@ struct Krylov {
     update<TB, TV>(V, ld, nv, sm->h2, w, n, partials, s, vinv, stop, gate);
     reduce(partials, 1, sm->nn1b, s, stop, gate);
     hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(64), 0, s, sm, j, R, early_exit);
+    const double nb = (double)n * (sizeof(TV) + sizeof(TB) + (cur ? sizeof(TV) : 0));
+    const int tid = timer_begin(SISL_T_KRYLOV_NORMALIZE, sizeof(TV) == 4, nb, stop, nullptr, s);
     hipLaunchKernelGGL((normalize_gated_kernel<TB, TV>), dim3(kRedGrid), dim3(kBlock), 0, s,
                        (const TV *)w, V + (int64_t)(j + 1) * ld, cur, n, vs, (const Small *)sm,
                        (const int *)stop);
+    timer_end(tid, s);
     SISL_CHECK_LAUNCH();
     return SISL_OK;
   }

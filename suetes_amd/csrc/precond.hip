@@ -267,6 +267,10 @@ static int launch_M_seg(const OpT<T> &o, const T *r, T *z, hipStream_t st, const
   if (!L) return SISL_ERR_ARG;  // taller than kMaxSeg * 32 levels
   const int S = (nz + L - 1) / L;
   const dim3 grid((unsigned)(((o.s.nx + 63) / 64) * o.s.ny)), block(64, S);
+  const Stag &g = o.s;
+  const double bytes = (double)sizeof(T) * (g.n + 7.0 * g.w.n() + 5.0 * g.m.n() + 2.0 * g.nx * g.ny +
+                                            (r != z ? (double)(g.u.n() + g.v.n()) : 0.0));
+  const int tid = timer_begin(SISL_T_PRECOND, sizeof(T) == 4, bytes, stop, nullptr, st);
   switch (L) {
 #define SISL_M_CASE(c) \
   case c: hipLaunchKernelGGL((precond_seg_kernel<T, c>), grid, block, 0, st, o, r, z, stop); break;
@@ -274,6 +278,7 @@ static int launch_M_seg(const OpT<T> &o, const T *r, T *z, hipStream_t st, const
     SISL_M_CASE(24) SISL_M_CASE(32)
 #undef SISL_M_CASE
   }
+  timer_end(tid, st);
   SISL_CHECK_LAUNCH();
   return SISL_OK;
 }

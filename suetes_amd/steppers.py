@@ -212,6 +212,29 @@ class _OperatorSet:
         self.dtype = dtype
 
 
+_TUNED = set()
+
+
+def _autotune(ops):
+    """Let the library time the launch shapes of the operator apply once per
+    grid and element type (the choice is kept for the process)."""
+    g = ops.struct.g
+    key = (g.nx, g.ny, g.nz, ops.dtype)
+    if key in _TUNED:
+        return
+    L = _lib.lib()
+    n = int(L.sisl_krylov_size(C.byref(g)))
+    dev = ops.tensors["cua"].device
+    x = torch.randn(n, dtype=ops.dtype, device=dev)
+    y = torch.empty_like(x)
+    kc, un = C.c_int32(), C.c_int32()
+    _lib.check(L.sisl_autotune_operator(C.byref(ops.struct), int(ops.dtype == torch.float32), x.data_ptr(),
+                                        y.data_ptr(), C.byref(kc), C.byref(un), _lib.stream_ptr(dev)),
+               "operator autotune")
+    _TUNED.add(key)
+    ops.tuned = (kc.value, un.value)
+
+
 class VerticalPreconditioner:
     """Column Helmholtz preconditioner (steppers.py:125-253) on the HIP kernel."""
 
@@ -293,6 +316,9 @@ class SemiImplicitSolver3D:
             raise ValueError("restart above 64 is not supported by the HIP GMRES")
         nbytes = (L.sisl_gmres_workspace_bytes_mixed if self.precision == "mixed"
                   else L.sisl_gmres_workspace_bytes_f64)(C.byref(self.op64.struct.g), R)
+        for ops in (self.op64, self.op32):
+            if ops is not None:
+                _autotune(ops)
         self.workspace = torch.empty(int(nbytes), dtype=torch.uint8, device=co["cua"].device)
         self._bg_id = key
         self._bg = bg

@@ -220,3 +220,40 @@ def test_mixed_solver_reaches_tolerance(basis):
     assert st["residual"] <= st["ptol"]
     x64 = sol.solve_flat(b, torch.zeros_like(b))
     assert float((x - x64).norm() / x64.norm()) < 1e-8
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_operator_launch_shapes_bit_identical_and_autotune(dtype):
+    """Every launch shape of the operator apply gives the same bits; the
+    autotuner returns a valid shape and leaves results unchanged."""
+    import ctypes as C
+    from suetes_amd import _lib
+    case, go, A_o, M_o, sol = _solver_pair(small_lam())
+    ops = sol.op64
+    if dtype == torch.float32:
+        from suetes_amd.steppers import _OperatorSet
+        co = sol.physics.operator_coefficients(case.stepper.bg_precomputed(), sol.dt, sol.alpha)
+        ops = _OperatorSet(sol.physics.grid, sol.physics.op.periodic_axes, co, sol.alpha, dtype)
+    L = _lib.lib()
+    g = ops.struct.g
+    n = int(L.sisl_krylov_size(C.byref(g)))
+    x = torch.randn(n, dtype=dtype, device=DEV)
+    fp32 = int(dtype == torch.float32)
+    fn = L.sisl_helmholtz_apply_f32 if fp32 else L.sisl_helmholtz_apply_f64
+    outs = []
+    for chunk in (2, 3, 5, 13):
+        for unr in (1, 2):
+            _lib.check(L.sisl_set_operator_shape(C.byref(g), fp32, chunk, unr), "shape")
+            y = torch.empty_like(x)
+            _lib.check(fn(C.byref(ops.struct), x.data_ptr(), y.data_ptr(), _lib.stream_ptr()), "A")
+            outs.append(y)
+    for y in outs[1:]:
+        assert torch.equal(y, outs[0])
+    kc, un = C.c_int32(), C.c_int32()
+    scratch = torch.empty_like(x)
+    _lib.check(L.sisl_autotune_operator(C.byref(ops.struct), fp32, x.data_ptr(), scratch.data_ptr(),
+                                        C.byref(kc), C.byref(un), _lib.stream_ptr()), "autotune")
+    assert kc.value >= 2 and un.value in (1, 2)
+    y = torch.empty_like(x)
+    _lib.check(fn(C.byref(ops.struct), x.data_ptr(), y.data_ptr(), _lib.stream_ptr()), "A")
+    assert torch.equal(y, outs[0])
