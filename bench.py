@@ -3,7 +3,8 @@
 
 Workload: the limited-area case ``lam`` of suetes_amd.workloads (256x256x80,
 dx=1 km, stretched SLEVE to 20 km, seeded Gaussian random terrain, N=0.01
-background, seeded +-2 m/s winds, 10-cell Davies rim, dt=10 s; GMRES restart
+background, seeded +-2 m/s winds with 10 km correlation, 10-cell Davies rim with
+equal inflow/outflow weights, dt=10 s; GMRES restart
 20, tol 1e-10).  One "step" = one full SISL step (departure points, explicit
 tendencies, six tricubic gathers, FFSL, two implicit solves, boundaries).
 
@@ -188,6 +189,9 @@ def main():
     ap.add_argument("--nz", type=int, default=80)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--err-steps", type=int, default=100,
+                    help="continue both paths to this many steps and report the field error "
+                         "(0: skip)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only launch the operator kernel and a plain copy of known size "
                          "(the run that hardware-counter passes profile)")
@@ -219,6 +223,17 @@ def main():
     err = fields_err(s_mx, s_64)
     st_mx = cmx.stepper.implicit_solver.last_stats[-4:]
     st_64 = c64.stepper.implicit_solver.last_stats[-4:]
+    err_long = None
+    done = args.warmup + args.steps
+    if args.err_steps > done:
+        a, b = s_64, s_mx
+        for i in range(done, args.err_steps):
+            a = c64.stepper.step(a, i * c64.dt, c64.forcing, c64.bc_fn)
+            b = cmx.stepper.step(b, i * cmx.dt, cmx.forcing, cmx.bc_fn)
+        err_long = {"steps": args.err_steps, "max_rel_l2": fields_err(b, a),
+                    "per_field": {k: float((b[k].double() - a[k].double()).norm() / a[k].double().norm())
+                                  for k in ("u", "v", "w", "pi", "th_v", "rho")}}
+        log(f"mixed vs fp64 after {args.err_steps} steps: {err_long['max_rel_l2']:.2e}")
     roof, table = instrumented_step(cmx, s_mx)
     log(f"dominant kernel {roof['kernel']}: {roof['achieved']:.0f} GB/s")
     adj = None
@@ -242,11 +257,13 @@ def main():
                      "the inner Krylov basis (FP64 inner products); FP64 outer residual refinement, "
                      "state and explicit tendencies",
         "config": {"workload": f"lam {args.nx}x{args.nx}x{args.nz} dx=1km dt=10s stretched-SLEVE "
-                               "GRF terrain, Davies rim 10, GMRES restart 20 tol 1e-10",
+                               "GRF terrain, +-2 m/s winds (10 km correlation), Davies rim 10 "
+                               "(equal inflow/outflow weights), GMRES restart 20 tol 1e-10",
                    "grid": [args.nx, args.nx, args.nz], "dt": 10.0},
         "fp64_steps_per_s": 1.0 / t_64, "fp64_ms_per_step": t_64 * 1e3,
         "speedup_mixed_vs_fp64": t_64 / t_mx,
         "field_err_mixed_vs_fp64": err,
+        "field_err_mixed_vs_fp64_long": err_long,
         "grid_point_updates_per_s": npts / t_mx,
         "solver_stats_mixed": st_mx, "solver_stats_fp64": st_64,
         "adjoint_forward_cost": adj,

@@ -1074,6 +1074,29 @@ class DaviesSponge:
         return out
 
 
+class XRelaxation:
+    """Symmetric cos^2 relaxation over ``cells`` columns at both x ends toward
+    the exterior state, for u, v, th_v, pi, rho; the boundary function of the
+    reference's mountain-wave benchmark
+    (benchmarks/physical/schaer_mountain_3d/run.py:103-126)."""
+
+    def __init__(self, nx, cells=8):
+        idx = np.arange(nx)
+        dist = np.minimum(idx, nx - 1 - idx)
+        self.mask = np.where(dist < cells, np.cos(0.5 * np.pi * dist / cells) ** 2, 0.0)[:, None, None]
+        self.mask_u = np.pad(self.mask, ((0, 1), (0, 0), (0, 0)), mode='edge')
+
+    def blend(self, st, ext):
+        out = {}
+        for k, v in st.items():
+            if k in ('u', 'v', 'th_v', 'pi', 'rho') and k in ext:
+                m = self.mask_u if k == 'u' else self.mask
+                out[k] = (1.0 - m) * v + m * ext[k]
+            else:
+                out[k] = v
+        return out
+
+
 def davies_masks(nx, ny, depth, loc):
     """boundaries.py:36-61 (float32 weights, as in the reference)."""
     nxp = nx + 1 if loc == 'u' else nx

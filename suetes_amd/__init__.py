@@ -11,12 +11,12 @@ from .operators import CGridOperator3D, Coords, tensor_product_interp_3d
 from .euler import Euler3D, HyperFilter
 from .steppers import (FluxFormAdvector, SemiImplicitSolver3D, SemiLagrangianAdvector3D, SISLStepper3D,
                        VerticalPreconditioner, build_dynamical_core, state_to_internal, state_to_reference)
-from .boundaries import BenchmarkSponge, DaviesSponge
+from .boundaries import BenchmarkSponge, DaviesSponge, XRelaxation
 
 __all__ = [
     "GalChenSigma", "HybridSigma", "ObliqueStereographic", "RegionalGrid3D", "SleveSimple",
     "StretchedSleveSimple", "CGridOperator3D", "Coords", "tensor_product_interp_3d", "Euler3D",
     "HyperFilter", "FluxFormAdvector", "SemiImplicitSolver3D", "SemiLagrangianAdvector3D", "SISLStepper3D",
-    "VerticalPreconditioner", "build_dynamical_core", "BenchmarkSponge", "DaviesSponge", "ref_view",
+    "VerticalPreconditioner", "build_dynamical_core", "BenchmarkSponge", "DaviesSponge", "XRelaxation", "ref_view",
     "to_internal", "state_to_internal", "state_to_reference",
 ]

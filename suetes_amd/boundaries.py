@@ -100,6 +100,42 @@ class DaviesSponge:
         return {k: (ref_view(v) if torch.is_tensor(v) and v.dim() == 3 else v) for k, v in out.items()}
 
 
+class XRelaxation:
+    """Symmetric cos^2 relaxation over ``cells`` columns at both x ends toward
+    the external state, for u, v, th_v, pi, rho: the boundary function of
+    the reference's mountain-wave benchmark
+    (benchmarks/physical/schaer_mountain_3d/run.py:103-126)."""
+
+    def __init__(self, grid, cells=8):
+        nx = grid.nx
+        idx = np.arange(nx)
+        dist = np.minimum(idx, nx - 1 - idx)
+        m = np.where(dist < cells, np.cos(0.5 * np.pi * dist / cells) ** 2, 0.0)
+        mu = np.concatenate([m, m[-1:]])
+        dev = grid.device
+        # internal layout: x is the last dimension
+        self.mask = torch.from_numpy(m).to(dev).reshape(1, 1, -1)
+        self.mask_u = torch.from_numpy(mu).to(dev).reshape(1, 1, -1)
+
+    def blend_internal(self, st, ext):
+        out = {}
+        for k, v in st.items():
+            if k in ("u", "v", "th_v", "pi", "rho") and k in ext:
+                m = self.mask_u if k == "u" else self.mask
+                out[k] = (1.0 - m) * v + m * ext[k]
+            else:
+                out[k] = v
+        return out
+
+    def blend(self, intermediate_state, external_state):
+        st = {k: (to_internal(v) if torch.is_tensor(v) and v.dim() == 3 else v)
+              for k, v in intermediate_state.items()}
+        ex = {k: (to_internal(v) if torch.is_tensor(v) and v.dim() == 3 else v)
+              for k, v in external_state.items()}
+        out = self.blend_internal(st, ex)
+        return {k: (ref_view(v) if torch.is_tensor(v) and v.dim() == 3 else v) for k, v in out.items()}
+
+
 class BenchmarkSponge:
     """Symmetric relaxation (boundaries.py:123-189)."""
 

@@ -7,7 +7,7 @@ import numpy as np
 import torch
 
 from . import geometry as G
-from .boundaries import DaviesSponge
+from .boundaries import DaviesSponge, XRelaxation
 from .operators import CGridOperator3D
 from .steppers import build_dynamical_core
 from .workloads import CONSTANTS, WORKLOADS, initial_state, make_terrain
@@ -76,4 +76,6 @@ def build_case(cfg, device="cuda", precision="fp64", **core_overrides):
     if cfg["boundary"] is not None and cfg["boundary"][0] == "davies":
         sponge = DaviesSponge(grid, op, **cfg["boundary"][1])
         bc = sponge.blend
+    elif cfg["boundary"] is not None and cfg["boundary"][0] == "xrelax":
+        bc = XRelaxation(grid, **cfg["boundary"][1]).blend
     return Case(cfg, grid, op, stepper, state, bc, forcing, dt)

@@ -38,26 +38,16 @@ __device__ __forceinline__ int offset_split(int X, T d, int n, T &t) {
   return b;
 }
 
-template <typename T>
-__global__ void __launch_bounds__(256) tricubic_kernel(Dims d, const T *__restrict__ f,
-                                                       const T *__restrict__ co,
-                                                       T *__restrict__ out, int lim) {
-  const int64_t n = d.n();
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n) return;
-  const int X = (int)(p % d.nx), Y = (int)((p / d.nx) % d.ny), Z = (int)(p / ((int64_t)d.nx * d.ny));
-  T tx, ty, tz;
-  const int x0 = offset_split(X, co[p], d.nx, tx);
-  const int y0 = offset_split(Y, co[n + p], d.ny, ty);
-  const int z0 = offset_split(Z, co[2 * n + p], d.nz, tz);
-  int ix[4] = {max(x0 - 1, 0), x0, min(x0 + 1, d.nx - 1), min(x0 + 2, d.nx - 1)};
-  int iy[4] = {max(y0 - 1, 0), y0, min(y0 + 1, d.ny - 1), min(y0 + 2, d.ny - 1)};
-  int iz[4] = {max(z0 - 1, 0), z0, min(z0 + 1, d.nz - 1), min(z0 + 2, d.nz - 1)};
+// Tricubic of one point from a tap accessor F(a, b, c), a, b, c in 0..3
+// relative to the stencil origin (x0-1, y0-1, z0-1); the accessor applies the
+// reference's index clamping (or reads a staged copy that holds clamped
+// values).  Cascade z -> y -> x as the reference does.
+template <typename T, typename Fetch>
+__device__ __forceinline__ T tricubic_point(T tx, T ty, T tz, int lim, Fetch F) {
   T wx[4], wy[4], wz[4];
   cr_weights(tx, wx);
   cr_weights(ty, wy);
   cr_weights(tz, wz);
-  // Cascade z -> y -> x as the reference does.
   T acc_x = 0;
 #pragma unroll
   for (int a = 0; a < 4; ++a) {
@@ -66,27 +56,99 @@ __global__ void __launch_bounds__(256) tricubic_kernel(Dims d, const T *__restri
     for (int b = 0; b < 4; ++b) {
       T acc_z = 0;
 #pragma unroll
-      for (int c = 0; c < 4; ++c) acc_z += wz[c] * f[d.at(ix[a], iy[b], iz[c])];
+      for (int c = 0; c < 4; ++c) acc_z += wz[c] * F(a, b, c);
       acc_y += wy[b] * acc_z;
     }
     acc_x += wx[a] * acc_y;
   }
-  if (lim) {
-    const int x1 = min(x0 + 1, d.nx - 1), y1 = min(y0 + 1, d.ny - 1), z1 = min(z0 + 1, d.nz - 1);
-    T lo = f[d.at(x0, y0, z0)], hi = lo;
-    const int cx[7] = {x1, x0, x1, x0, x1, x0, x1};
-    const int cy[7] = {y0, y1, y1, y0, y0, y1, y1};
-    const int cz[7] = {z0, z0, z0, z1, z1, z1, z1};
+  if (lim) {  // clip to the 8 cell corners (operators.py:188-207)
+    T lo = F(1, 1, 1), hi = lo;
 #pragma unroll
-    for (int q = 0; q < 7; ++q) {
-      const T nb = f[d.at(cx[q], cy[q], cz[q])];
+    for (int q = 1; q < 8; ++q) {
+      const T nb = F(1 + (q & 1), 1 + ((q >> 1) & 1), 1 + ((q >> 2) & 1));
       lo = nb < lo ? nb : lo;
       hi = nb > hi ? nb : hi;
     }
     acc_x = acc_x < lo ? lo : acc_x;
     acc_x = acc_x > hi ? hi : acc_x;
   }
-  out[p] = acc_x;
+  return acc_x;
+}
+
+// Staged tricubic gather.  A block owns a 64 x 4 tile of (x, y) columns and
+// marches a chunk of levels.  A ring of 6 LDS planes holds levels k-2 .. k+3
+// of the field over the tile plus halo ([x0-2, x0+70) x [y0-2, y0+7)), each
+// staged value being the field at the clamped index, so that every stencil
+// within the staged box reads LDS with the reference's clamping built in.
+// One new plane is staged per level (coalesced along x).  Points whose
+// stencil leaves the box (displacement beyond ~1 cell) gather from global
+// memory; both paths perform the same arithmetic in the same order.
+constexpr int kIX = 64, kIY = 4, kSX = kIX + 8, kSY = kIY + 5, kRing = 6;
+
+template <typename T>
+__global__ void __launch_bounds__(256) tricubic_kernel(Dims d, const T *__restrict__ f,
+                                                       const T *__restrict__ co,
+                                                       T *__restrict__ out, int lim, int kchunk) {
+  __shared__ T ring[kRing][kSY][kSX];
+  const int nx = d.nx, ny = d.ny, nz = d.nz;
+  const int nbx = (nx + kIX - 1) / kIX, nby = (ny + kIY - 1) / kIY;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int bx = tile % nbx, by = (tile / nbx) % nby, bz = tile / (nbx * nby);
+  const int tid = threadIdx.y * kIX + threadIdx.x;
+  const int xs = bx * kIX - 2, ys = by * kIY - 2;
+  const int X = bx * kIX + threadIdx.x, Y = by * kIY + threadIdx.y;
+  const bool own = X < nx && Y < ny;
+  const int k0 = bz * kchunk, k1 = min(k0 + kchunk, nz);
+  const uint32_t plane = (uint32_t)nx * ny;
+  const uint32_t n = plane * (uint32_t)nz;
+  auto stage = [&](int kk) {
+    const int slot = ((kk % kRing) + kRing) % kRing;
+    const uint32_t zoff = (uint32_t)min(max(kk, 0), nz - 1) * plane;
+    for (int q = tid; q < kSY * kSX; q += kIX * kIY) {
+      const int ly = q / kSX, lx = q - ly * kSX;
+      const int gx = min(max(xs + lx, 0), nx - 1), gy = min(max(ys + ly, 0), ny - 1);
+      ring[slot][ly][lx] = ldg(f, zoff + (uint32_t)gy * nx + gx);
+    }
+  };
+  for (int kk = k0 - 2; kk <= k0 + 2; ++kk) stage(kk);
+  for (int k = k0; k < k1; ++k) {
+    __syncthreads();  // level k-1 is done with plane k-3
+    stage(k + 3);
+    __syncthreads();
+    if (!own) continue;
+    const uint32_t p = (uint32_t)k * plane + (uint32_t)Y * nx + X;
+    T tx, ty, tz;
+    const int x0 = offset_split(X, ldg(co, p), nx, tx);
+    const int y0 = offset_split(Y, ldg(co, n + p), ny, ty);
+    const int z0 = offset_split(k, ldg(co, 2 * n + p), nz, tz);
+    const bool staged = x0 - 1 >= xs && x0 + 2 < xs + kSX && y0 - 1 >= ys && y0 + 2 < ys + kSY &&
+                        z0 - 1 >= k - 2 && z0 + 2 <= k + 3;
+    T r;
+    if (staged) {
+      // one LDS base per stencil plane; taps are constant offsets from it
+      const T *zb[4];
+      int slot = (z0 - 1) % kRing;
+      if (slot < 0) slot += kRing;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        zb[c] = &ring[slot][y0 - 1 - ys][x0 - 1 - xs];
+        slot = slot + 1 == kRing ? 0 : slot + 1;
+      }
+      r = tricubic_point(tx, ty, tz, lim, [&](int a, int b, int c) { return zb[c][b * kSX + a]; });
+    } else {
+      int ix[4], iy[4], iz[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        ix[q] = min(max(x0 - 1 + q, 0), nx - 1);
+        iy[q] = min(max(y0 - 1 + q, 0), ny - 1) * nx;
+        iz[q] = min(max(z0 - 1 + q, 0), nz - 1);
+      }
+      r = tricubic_point(tx, ty, tz, lim, [&](int a, int b, int c) {
+        return ldg(f, (uint32_t)iz[c] * plane + (uint32_t)iy[b] + (uint32_t)ix[a]);
+      });
+    }
+    stg(out, p, r);
+  }
 }
 
 // ---- deterministic transpose ------------------------------------------------
@@ -390,10 +452,14 @@ template <typename T>
 int tricubic(int fnx, int fny, int fnz, const T *f, const T *co, T *out, int lim, void *st) {
   if (!f || !co || !out || fnx < 1 || fny < 1 || fnz < 1) return SISL_ERR_ARG;
   Dims d{fnx, fny, fnz};
+  if ((uint64_t)d.n() * 3 * sizeof(T) >= (1ull << 32)) return SISL_ERR_ARG;  // 32-bit offsets
+  // ~8 level chunks per column keep enough blocks in flight
+  const int kc = max(1, (fnz + 7) / 8);
+  const int nb = ((fnx + kIX - 1) / kIX) * ((fny + kIY - 1) / kIY) * ((fnz + kc - 1) / kc);
   const int tid = timer_begin(SISL_T_TRICUBIC, sizeof(T) == 4, (double)sizeof(T) * 5.0 * d.n(),
                               nullptr, nullptr, as_stream(st));
-  hipLaunchKernelGGL(tricubic_kernel<T>, dim3(grid_for(d.n())), dim3(kBlock), 0, as_stream(st),
-                     d, f, co, out, lim);
+  hipLaunchKernelGGL(tricubic_kernel<T>, dim3(nb), dim3(kIX, kIY), 0, as_stream(st), d, f, co, out,
+                     lim, kc);
   timer_end(tid, as_stream(st));
   SISL_CHECK_LAUNCH();
   return SISL_OK;

@@ -10,11 +10,13 @@ options and boundary treatment of one benchmark/parity case:
                   reference's own idiom, benchmarks/physical/schaer_mountain_3d),
                   400x3x100, dx=250 m, top 20 km, stretched SLEVE,
                   h(x)=250 m exp(-(x/5 km)^2) cos^2(pi x/4 km), U=10 m/s,
-                  N=0.01 1/s, Rayleigh sponge above 12 km, dt=8 s.
+                  N=0.01 1/s, Rayleigh sponge above 12 km, 8-column x
+                  relaxation, dt=8 s.
 * ``lam``      -- limited-area step, 256x256x80, dx=1 km, stretched SLEVE to
                   20 km, terrain = seeded Gaussian random field (std 400 m,
                   20 km correlation, clipped >= 0), N=0.01 background, seeded
-                  +-2 m/s wind noise, 10-cell Davies rim, dt=10 s.
+                  +-2 m/s wind noise (10 km correlation), 10-cell Davies rim
+                  (equal inflow/outflow weights), dt=10 s.
 * ``adjoint``  -- 128x128x60, dx=1 km, terrain = 8 seeded Gaussian bumps
                   (sigma=5 km, <= 300 m) whose amplitudes are the control.
 
@@ -171,8 +173,24 @@ def bubble(n=64, dx=50.0, seed=0):
 
 
 @workload
-def lam(nx=256, ny=256, nz=80, dx=1000.0, top=20000.0, seed=0, steps=100):
-    """Config of the 3-D limited-area case (see module docstring)."""
+def lam(nx=256, ny=256, nz=80, dx=1000.0, top=20000.0, seed=0, steps=100, winds="smooth",
+        outflow_factor=1.0):
+    """Config of the 3-D limited-area case (see module docstring).
+
+    ``winds='smooth'`` (default): +-2 m/s noise with a 10 km horizontal
+    correlation; ``'white'``: independent U(-2, 2) per face.
+
+    ``outflow_factor`` of the Davies rim: 1.0 (default) relaxes inflow and
+    outflow points alike; the reference's class default 0.01 weakens the
+    relaxation where the boundary wind points outwards.  That switch is a
+    step function of the wind's sign, so the trajectory depends
+    discontinuously on rounding-size differences wherever a rim wind
+    crosses zero (at 256x256x80 an FP32-kernel run leaves the FP64 run by
+    1e-4 at step 40 and 0.3 at step 100, versus 2.8e-6 without the switch),
+    and with white-noise winds it also feeds a growing mode at the closed
+    faces (max |u| 4 -> 65 m/s in 100 FP64 steps)."""
+    init = (("smooth_winds", dict(seed=seed + 1, amp=2.0, corr=10000.0)) if winds == "smooth"
+            else ("winds", dict(seed=seed + 1, amp=2.0)))
     return {
         "name": "lam", "grid": dict(nx=nx, ny=ny, nz=nz, dx=dx, dy=dx, dz=top / nz, lat_center=45.0,
                                     lon_center=0.0),
@@ -181,8 +199,9 @@ def lam(nx=256, ny=256, nz=80, dx=1000.0, top=20000.0, seed=0, steps=100):
         "periodic_axes": (),
         "core": dict(dt=10.0, N_bv=0.01, alpha=0.55, solver_tol=1e-10, solver_maxiter=10,
                      solver_restart=20),
-        "N_bv": 0.01, "steps": steps, "boundary": ("davies", dict(sponge_depth=10, dt=10.0)),
-        "init": ("winds", dict(seed=seed + 1, amp=2.0)),
+        "N_bv": 0.01, "steps": steps,
+        "boundary": ("davies", dict(sponge_depth=10, dt=10.0, outflow_factor=outflow_factor)),
+        "init": init,
     }
 
 
@@ -195,9 +214,14 @@ def schaer(nx=400, nz=100, dx=250.0, top=20000.0, steps=500):
         "terrain": ("schaer", dict(h0=250.0, a=5000.0, lam=4000.0)),
         "transform": ("stretched_sleve", dict(stretch_kappa=2.5, scale_s=4000.0, n=1.35)),
         "periodic_axes": (1,),
-        "core": dict(dt=8.0, N_bv=0.01, alpha=0.55, damp_height=12000.0, max_damp=0.1,
-                     solver_tol=1e-10, solver_maxiter=10, solver_restart=20),
-        "N_bv": 0.01, "steps": steps, "boundary": ("davies", dict(sponge_depth=20, dt=8.0)),
+        # stepper options and boundary function of the reference's benchmark
+        # (benchmarks/physical/schaer_mountain_3d/run.py:103-136): alpha 0.5,
+        # no hyperdiffusion, Rayleigh damping 0.5 above 12 km, 8-column
+        # symmetric x relaxation; solver tolerance 1e-10, restart 20
+        "core": dict(dt=8.0, N_bv=0.01, alpha=0.5, damp_height=12000.0, max_damp=0.5,
+                     nu_div_factor=0.0, nu_h_factor=0.0, solver_tol=1e-10, solver_maxiter=10,
+                     solver_restart=20),
+        "N_bv": 0.01, "steps": steps, "boundary": ("xrelax", dict(cells=8)),
         "init": ("uniform", dict(u=10.0)),
     }
 
@@ -262,6 +286,23 @@ def initial_state(cfg, Z_m, x_m, y_m, z_m=None, c=CONSTANTS):
         rng = np.random.default_rng(kw["seed"])
         st["u"] = rng.uniform(-kw["amp"], kw["amp"], size=st["u"].shape)
         st["v"] = rng.uniform(-kw["amp"], kw["amp"], size=st["v"].shape)
+    elif kind == "smooth_winds":
+        # horizontally correlated noise (Gaussian random field of the given
+        # correlation length), the same at every level, scaled to max |.| = amp
+        xm, ym = _np(x_m).reshape(-1), _np(y_m).reshape(-1)
+        if xm.size != g["nx"]:
+            xm = _np(x_m)[:, 0, 0]
+            ym = _np(y_m)[0, :, 0]
+        xu = np.concatenate([xm - g["dx"] / 2.0, xm[-1:] + g["dx"] / 2.0])
+        yv = np.concatenate([ym - g["dy"] / 2.0, ym[-1:] + g["dy"] / 2.0])
+        Lx, Ly = g["nx"] * g["dx"], g["ny"] * g["dy"]
+        for key, xs, ys, sd in (("u", xu, ym, kw["seed"]), ("v", xm, yv, kw["seed"] + 1)):
+            fld = GaussianRandomTerrain(Lx, Ly, seed=sd, corr=kw["corr"], spacing=kw["corr"] / 4.0)
+            X, Y = np.meshgrid(xs, ys, indexing="ij")
+            raw = fld._raw(X, Y)
+            raw = raw - raw.mean()
+            raw *= kw["amp"] / np.abs(raw).max()
+            st[key] = np.repeat(raw[:, :, None], g["nz"], axis=2)
     elif kind == "uniform":
         st["u"] = np.full_like(st["u"], kw.get("u", 0.0))
         st["v"] = np.full_like(st["v"], kw.get("v", 0.0))

@@ -29,6 +29,8 @@ def oracle_case(cfg):
     if cfg["boundary"] is not None and cfg["boundary"][0] == "davies":
         sp = O.DaviesSponge(g, op, **cfg["boundary"][1])
         bc = sp.blend
+    elif cfg["boundary"] is not None and cfg["boundary"][0] == "xrelax":
+        bc = O.XRelaxation(gk["nx"], **cfg["boundary"][1]).blend
     return g, op, stepper, st, bc, ext
 
 

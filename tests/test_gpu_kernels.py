@@ -70,6 +70,28 @@ def test_tricubic_matches_oracle(shape, lim):
     assert rel_l2(from_int(out32), ref) < 2e-6
 
 
+@pytest.mark.parametrize("lim", [False, True])
+def test_tricubic_staged_and_far_departures_match_oracle(lim):
+    """Tiles of the staged gather (64 x 4 columns, level chunks) with both
+    near departure points (read from the staged planes) and far ones (read
+    from global memory), on a grid spanning several tiles."""
+    from suetes_amd.operators import tensor_product_interp_3d
+    rng = np.random.default_rng(40 + lim)
+    shape = (70, 9, 17)
+    f = rng.standard_normal(shape)
+    base = index_grid(shape)
+    near = rng.uniform(-0.9, 0.9, size=(3,) + shape)
+    far = rng.uniform(-6.0, 6.0, size=(3,) + shape)
+    pick = rng.random(shape) < 0.2
+    pts = base + np.where(pick[None], far, near)
+    ref = O.tricubic(f, pts, lim)
+    co = offsets_int(pts)
+    out = tensor_product_interp_3d(to_int(f), co, lim)
+    assert rel_l2(from_int(out), ref) < 1e-13
+    out32 = tensor_product_interp_3d(to_int(f, torch.float32), co.float(), lim)
+    assert rel_l2(from_int(out32), ref) < 2e-6
+
+
 def test_tricubic_adjoint_dot_and_determinism():
     from suetes_amd.operators import Coords, tensor_product_interp_3d, tricubic_transpose
     rng = np.random.default_rng(11)

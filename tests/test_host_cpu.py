@@ -112,6 +112,19 @@ def test_ffsl_and_sponge_match_oracle():
                    {k: torch.from_numpy(v.copy()) for k, v in ext.items()})
     for k in bo:
         np.testing.assert_allclose(tnp(bp[k]), bo[k], rtol=1e-15, atol=0)
+    # x-only relaxation of the mountain-wave benchmark: product vs oracle and
+    # its closed form (weight 1 on the edge columns, 0 from `cells` inward)
+    state["rho"] = st["rho"]
+    ext["rho"] = np.zeros_like(st["rho"]) + 0.5
+    xo = O.XRelaxation(go.nx, cells=3).blend(state, ext)
+    xp = S.XRelaxation(gp, cells=3).blend({k: torch.from_numpy(v.copy()) for k, v in state.items()},
+                                          {k: torch.from_numpy(v.copy()) for k, v in ext.items()})
+    for k in xo:
+        np.testing.assert_allclose(tnp(xp[k]), xo[k], rtol=1e-15, atol=0)
+    np.testing.assert_array_equal(xo["th_v"][0], ext["th_v"][0])
+    np.testing.assert_array_equal(xo["u"][-1], ext["u"][-1])
+    np.testing.assert_array_equal(xo["pi"][3:-3], state["pi"][3:-3])
+    np.testing.assert_array_equal(xo["w"], state["w"])
 
 
 def test_library_exports_every_declared_symbol():
