@@ -14,6 +14,7 @@ trajectory stays in the internal layout with no copies.
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes as C
 
 import torch
@@ -257,20 +258,94 @@ class VerticalPreconditioner:
         return unpack(z, rhs_scaled)
 
 
+def _operator_inputs(solver):
+    """(where, tensor) for every tensor the operator A can read that carries a
+    gradient: background entries, grid arrays and map factors, physics
+    arrays (e.g. the Rayleigh profile).  Empty unless the grid or the
+    background is differentiated (terrain controls)."""
+    ph = solver.physics
+    g = ph.grid
+    out = []
+    for k, v in solver._bg.items():
+        if torch.is_tensor(v) and v.requires_grad:
+            out.append((("bg", k), v))
+    for k, v in g.__dict__.items():
+        if torch.is_tensor(v) and v.requires_grad:
+            out.append((("grid", k), v))
+    for k, v in g.i_m_factors.items():
+        if v.requires_grad:
+            out.append((("mf", k), v))
+    for k, v in ph.__dict__.items():
+        if torch.is_tensor(v) and v.requires_grad:
+            out.append((("phys", k), v))
+    return out
+
+
+@contextlib.contextmanager
+def _substituted(solver, where, tensors):
+    """Temporarily read the operator's inputs from ``tensors``; yields the
+    background dict to use."""
+    ph = solver.physics
+    g = ph.grid
+    bg = dict(solver._bg)
+    mf = dict(g.i_m_factors)
+    saved = []
+    for (kind, k), t in zip(where, tensors):
+        if kind == "bg":
+            bg[k] = t
+        elif kind == "mf":
+            mf[k] = t
+        else:
+            obj = g if kind == "grid" else ph
+            saved.append((obj, k, obj.__dict__[k]))
+            obj.__dict__[k] = t
+    old_mf = g.__dict__["i_m_factors"]
+    g.__dict__["i_m_factors"] = mf
+    try:
+        yield bg
+    finally:
+        g.__dict__["i_m_factors"] = old_mf
+        for obj, k, v in saved:
+            obj.__dict__[k] = v
+
+
 class _GmresSolve(torch.autograd.Function):
     """x = A^{-1} b with the custom_linear_solve adjoint (steppers.py:473-485):
     the cotangent of b is the transposed solve with the transposed
-    preconditioner; x0 receives none."""
+    preconditioner, lambda = A^{-T} x_bar; x0 receives none.  When the
+    operator's own inputs carry gradients (``deps``: background, metrics),
+    they receive -lambda^T d(A x)/d(input), evaluated through the torch
+    restatement of A on fresh leaf copies (so each input gets its partial
+    derivative and derived inputs are not counted twice)."""
 
     @staticmethod
-    def forward(ctx, solver, b, x0):
+    def forward(ctx, solver, b, x0, where, *deps):
         ctx.solver = solver
-        return solver.solve_flat(b.detach(), x0.detach(), transpose=False)
+        x = solver.solve_flat(b.detach(), x0.detach(), transpose=False)
+        ctx.where = where
+        if deps:
+            ctx.save_for_backward(x, *deps)
+        return x
 
     @staticmethod
     def backward(ctx, gx):
-        y = ctx.solver.solve_flat(gx.contiguous(), torch.zeros_like(gx), transpose=True)
-        return None, y, None
+        solver = ctx.solver
+        lam = solver.solve_flat(gx.contiguous(), torch.zeros_like(gx), transpose=True)
+        if not ctx.where:
+            return None, lam, None, None
+        x, *deps = ctx.saved_tensors
+        op = solver.physics.op
+        keep = op.use_stop_grad
+        op.use_stop_grad = False  # the solve is traced without stop-gradients (steppers.py:463-487)
+        try:
+            with torch.enable_grad():
+                leaves = [d.detach().requires_grad_(True) for d in deps]
+                with _substituted(solver, ctx.where, leaves) as bg:
+                    y = solver.scaled_operator(x.detach(), bg)
+                grads = torch.autograd.grad(y, leaves, grad_outputs=-lam, allow_unused=True)
+        finally:
+            op.use_stop_grad = keep
+        return (None, lam, None, None) + tuple(grads)
 
 
 class SemiImplicitSolver3D:
@@ -303,7 +378,7 @@ class SemiImplicitSolver3D:
 
     # -- setup ------------------------------------------------------------------
     def prepare(self, bg):
-        key = id(bg["th_v"])
+        key = id(bg)  # the dict is held in self._bg, so its id stays unique
         if self._bg_id == key:
             return
         co = self.physics.operator_coefficients(bg, self.dt, self.alpha)
@@ -365,10 +440,27 @@ class SemiImplicitSolver3D:
                    "eta_dot": x0["eta_dot"] * dzw}
         else:
             x0s = rs
-        xs = _GmresSolve.apply(self, pack(rs), pack(x0s))
+        deps = _operator_inputs(self) if torch.is_grad_enabled() else []
+        where = tuple(w for w, _ in deps)
+        xs = _GmresSolve.apply(self, pack(rs), pack(x0s), where, *[t for _, t in deps])
         x = unpack(xs, rs)
         return {"u": x["u"], "v": x["v"], "w": x["w"], "pi": x["pi"] / self.pi_scale,
                 "eta_dot": x["eta_dot"] / dzw}
+
+    def scaled_operator(self, x_flat, bg):
+        """A on a packed vector in the solver's scaled units, as torch ops
+        (euler.py:374-439 through steppers.py:439-461)."""
+        g = self.physics.grid
+        nx, ny, nz = g.nx, g.ny, g.nz
+        like = {"u": torch.empty(nz, ny, nx + 1, device="meta"), "v": torch.empty(nz, ny + 1, nx, device="meta"),
+                "w": torch.empty(nz + 1, ny, nx, device="meta"), "pi": torch.empty(nz, ny, nx, device="meta"),
+                "eta_dot": torch.empty(nz + 1, ny, nx, device="meta")}
+        x = unpack(x_flat, like)
+        sp = {"u": x["u"], "v": x["v"], "w": x["w"], "pi": x["pi"] / self.pi_scale,
+              "eta_dot": x["eta_dot"] / bg["dz_w_full"]}
+        L = self.physics.linear_operator(sp, bg, self.dt, alpha=self.alpha)
+        return pack({"u": L["u"], "v": L["v"], "w": L["w"], "pi": L["pi"] * self.pi_scale,
+                     "eta_dot": L["eta_dot"]})
 
     def apply_A(self, x_flat, transpose=False, dtype=torch.float64):
         """y = A x (or A^T x) on packed scaled vectors with the HIP kernel."""
@@ -458,6 +550,8 @@ class SISLStepper3D:
         s = state_to_internal(state)
         if "eta_dot" not in s:
             s["eta_dot"] = torch.zeros_like(s["w"])
+        if checkpoint:
+            self.warm_caches(forcing, bc_fn)
         for i in range(int(num_steps)):
             t = t_start + i * self.dt
             if checkpoint and torch.is_grad_enabled():
@@ -467,6 +561,17 @@ class SISLStepper3D:
             else:
                 s = self.step_internal(s, t, forcing, bc_fn)
         return state_to_reference(s)
+
+    def warm_caches(self, forcing, bc_fn):
+        """Build the lazily cached background, solver coefficients and the
+        internal copy of a sponge's forcing now, outside any checkpointed
+        region, so that a recomputed step records the same tensors as its
+        forward pass."""
+        self.implicit_solver.prepare(self.bg_precomputed())
+        owner = getattr(bc_fn, "__self__", None)
+        if owner is not None and getattr(bc_fn, "__name__", "") == "blend" and hasattr(owner, "blend_internal"):
+            if self._ext_cache is None or self._ext_cache[0] is not forcing:
+                self._ext_cache = (forcing, state_to_internal(forcing))
 
     def bg_precomputed(self):
         if self._bg is None:

@@ -239,9 +239,12 @@ def adjoint(nx=128, ny=128, nz=60, dx=1000.0, top=15000.0, seed=3, steps=50):
         "terrain": ("bumps", dict(centres=centres, sigma=5000.0, amps=amps)),
         "transform": ("stretched_sleve", dict(stretch_kappa=2.5, scale_s=4000.0, n=1.35)),
         "periodic_axes": (),
-        "core": dict(dt=10.0, N_bv=0.01, alpha=0.55, solver_tol=1e-10, solver_maxiter=10,
+        # stepper options and x relaxation of the reference's optimal-topography
+        # experiment (experiments/gravity_wave_optimal_topography/run.py:70-140)
+        "core": dict(dt=10.0, N_bv=0.01, alpha=0.55, nu_div_factor=0.0, nu_h_factor=0.0,
+                     damp_height=12000.0, max_damp=0.5, solver_tol=1e-10, solver_maxiter=10,
                      solver_restart=20),
-        "N_bv": 0.01, "steps": steps, "boundary": ("davies", dict(sponge_depth=10, dt=10.0)),
+        "N_bv": 0.01, "steps": steps, "boundary": ("xrelax", dict(cells=10)),
         "init": ("uniform", dict(u=10.0)),
         "objective_box": (slice(int(0.55 * nx), int(0.55 * nx) + 20), slice(ny // 2 - 10, ny // 2 + 10),
                           slice(0, 10)),

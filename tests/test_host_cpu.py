@@ -141,3 +141,64 @@ def test_kernels_refuse_cpu_tensors():
     f = torch.zeros(4, 4, 4, dtype=torch.float64)
     with pytest.raises(RuntimeError, match="HIP device"):
         S.tensor_product_interp_3d(f, torch.zeros(3, 4, 4, 4, dtype=torch.float64))
+
+
+def _dense_solver(solver, bg):
+    """Replace the HIP GMRES of ``solver`` by a dense solve of the torch
+    operator (tiny grids only), so the solve's autograd rule runs on the CPU."""
+    n = int(sum(int(np.prod(s)) for s in _packed_shapes(solver.physics.grid)))
+    with torch.no_grad():
+        eye = torch.eye(n, dtype=torch.float64)
+        cols = [solver.scaled_operator(eye[:, q], {k: (v.detach() if torch.is_tensor(v) else v)
+                                                   for k, v in bg.items()}) for q in range(n)]
+        A = torch.stack(cols, dim=1)
+
+    def solve_flat(b, x0, transpose=False):
+        M = A.T if transpose else A
+        return torch.linalg.solve(M, b)
+
+    solver.solve_flat = solve_flat
+    solver._bg, solver._bg_id = bg, id(bg)
+    return A
+
+
+def _packed_shapes(g):
+    return [(g.nz, g.ny, g.nx + 1), (g.nz, g.ny + 1, g.nx), (g.nz + 1, g.ny, g.nx), (g.nz, g.ny, g.nx),
+            (g.nz + 1, g.ny, g.nx)]
+
+
+def test_solve_adjoint_reaches_terrain_through_the_operator():
+    """d/d(bump amplitudes) of <c, A(h)^{-1} b>: the implicit-solve rule
+    (transposed solve, then -lambda^T d(A x)/d(inputs) through the torch
+    operator) against central differences, with the HIP solve replaced by a
+    dense solve of the same operator on a 6x5x4 grid."""
+    from suetes_amd.cases import build_terrain_case
+    cfg = W.adjoint(nx=6, ny=5, nz=4)
+    cfg["terrain"][1]["centres"] = np.array([[-1500.0, 0.0], [1000.0, 500.0]])
+    cfg["terrain"][1]["amps"] = np.array([250.0, 150.0])
+    rng = np.random.default_rng(3)
+    g0 = build_terrain_case(cfg, torch.tensor([250.0, 150.0], dtype=torch.float64), device="cpu").grid
+    shapes = _packed_shapes(g0)
+    rhs = {k: torch.from_numpy(rng.standard_normal(s)) for k, s in zip(S.steppers.KEYS, shapes)}
+    cvec = torch.from_numpy(rng.standard_normal(sum(int(np.prod(s)) for s in shapes)))
+
+    def J(amps):
+        case = build_terrain_case(cfg, amps, device="cpu")
+        case.op.use_stop_grad = False  # differentiate the padded edge values too
+        sol = case.stepper.implicit_solver
+        bg = case.stepper.bg_precomputed()
+        _dense_solver(sol, bg)
+        x = sol.solve(rhs, bg)
+        return (S.steppers.pack(x) * cvec).sum()
+
+    a0 = torch.tensor([250.0, 150.0], dtype=torch.float64, requires_grad=True)
+    val = J(a0)
+    (grad,) = torch.autograd.grad(val, a0)
+    assert float(grad.abs().max()) > 0
+    for q in range(2):
+        h = 1e-2
+        e = torch.zeros(2, dtype=torch.float64)
+        e[q] = h
+        with torch.no_grad():
+            fd = (float(J(a0.detach() + e)) - float(J(a0.detach() - e))) / (2 * h)
+        assert abs(fd - float(grad[q])) <= 1e-6 * abs(fd), (q, fd, float(grad[q]))
