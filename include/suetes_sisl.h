@@ -222,6 +222,41 @@ int sisl_timer_enable(int32_t on);
 int sisl_timer_read(int32_t kernel, int32_t fp32, double *total_ms, int64_t *launches,
                     int64_t *working_launches, double *bytes);
 
+/* ---- batched ensemble (one launch per kernel over members) --------------- */
+
+/* At most 32 members per batched call, each on its own grid (uneven sizes). */
+
+/* tensor_product_interp_3d of one field per member: member m has field dims
+ * dims[3m..3m+2] = (fnx, fny, fnz) and its own field / offsets / output. */
+int sisl_tricubic_batched_f64(int32_t nmembers, const int32_t *dims, const double *const *field,
+                              const double *const *coords, double *const *out, int32_t use_limiter,
+                              void *stream);
+int sisl_tricubic_batched_f32(int32_t nmembers, const int32_t *dims, const float *const *field,
+                              const float *const *coords, float *const *out, int32_t use_limiter,
+                              void *stream);
+/* Departure points at staggering loc of every member (grids[m], dx[m], dy[m]). */
+int sisl_departure_batched_f64(int32_t nmembers, const SislGrid *grids, int32_t loc,
+                               const double *const *u, const double *const *v,
+                               const double *const *eta, const double *const *m2, const double *dx,
+                               const double *dy, double dt, int32_t iterations,
+                               double *const *scratch3, double *const *coords_out, void *stream);
+int sisl_departure_batched_f32(int32_t nmembers, const SislGrid *grids, int32_t loc,
+                               const float *const *u, const float *const *v,
+                               const float *const *eta, const float *const *m2, const double *dx,
+                               const double *dy, double dt, int32_t iterations,
+                               float *const *scratch3, float *const *coords_out, void *stream);
+/* Mixed-precision GMRES of every member's system (as sisl_gmres_mixed with
+ * the FP16 basis), all members in lockstep: every solver pass is one launch;
+ * the refinement loop and the Hessenberg least squares run on the device.
+ * Workspace: caller-owned, sized by _workspace_bytes; _setup records the
+ * members' operators in it (once per operator set); then any number of
+ * solves (p->transpose selects A^T / M^T). x: in = x0, out = solution. */
+int64_t sisl_gmres_batched_workspace_bytes(const SislGrid *grids, int32_t nmembers, int32_t restart);
+int sisl_gmres_batched_setup(const SislOperator *ops64, const SislOperator *ops32, int32_t nmembers,
+                             int32_t restart, void *workspace, void *stream);
+int sisl_gmres_batched(int32_t nmembers, const SislGmresParams *p, const double *const *b,
+                       double *const *x, void *workspace, SislGmresStats *stats, void *stream);
+
 /* Packed-vector helpers used by the host layer. out = sum over d of x*y (FP64). */
 int sisl_dot_f64(const double *x, const double *y, int64_t n, double *out_dev, double *partials,
                  void *stream);

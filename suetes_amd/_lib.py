@@ -23,7 +23,9 @@ EXPORTED = [
     "sisl_precond_apply_T_f64", "sisl_precond_apply_T_f32", "sisl_gmres_workspace_bytes_f64",
     "sisl_gmres_workspace_bytes_mixed", "sisl_gmres_f64", "sisl_gmres_mixed", "sisl_dot_f64",
     "sisl_timer_enable", "sisl_timer_read", "sisl_autotune_operator",
-    "sisl_set_operator_shape",
+    "sisl_set_operator_shape", "sisl_tricubic_batched_f64", "sisl_tricubic_batched_f32",
+    "sisl_departure_batched_f64", "sisl_departure_batched_f32", "sisl_gmres_batched_workspace_bytes",
+    "sisl_gmres_batched_setup", "sisl_gmres_batched",
 ]
 
 # timed kernels (enum SislTimedKernel)
@@ -96,6 +98,15 @@ def lib():
         L.sisl_autotune_operator.argtypes = [C.POINTER(SislOperator), i32, vp, vp, C.POINTER(i32),
                                              C.POINTER(i32), vp]
         L.sisl_set_operator_shape.argtypes = [C.POINTER(SislGrid), i32, i32, i32]
+        pp = C.POINTER(vp)
+        for p in ("f64", "f32"):
+            getattr(L, f"sisl_tricubic_batched_{p}").argtypes = [i32, C.POINTER(i32), pp, pp, pp, i32, vp]
+            getattr(L, f"sisl_departure_batched_{p}").argtypes = [i32, C.POINTER(SislGrid), i32, pp, pp, pp, pp,
+                                                                  C.POINTER(d), C.POINTER(d), d, i32, pp, pp, vp]
+        L.sisl_gmres_batched_workspace_bytes.restype = i64
+        L.sisl_gmres_batched_workspace_bytes.argtypes = [C.POINTER(SislGrid), i32, i32]
+        L.sisl_gmres_batched_setup.argtypes = [C.POINTER(SislOperator), C.POINTER(SislOperator), i32, i32, vp, vp]
+        L.sisl_gmres_batched.argtypes = [i32, C.POINTER(SislGmresParams), pp, pp, vp, C.POINTER(SislGmresStats), vp]
         L.sisl_timer_enable.argtypes = [i32]
         L.sisl_timer_read.argtypes = [i32, i32, C.POINTER(d), C.POINTER(i64), C.POINTER(i64),
                                       C.POINTER(d)]
@@ -114,6 +125,11 @@ def timer_read(kernel: str, fp32: bool):
     check(lib().sisl_timer_read(TIMED[kernel], int(bool(fp32)), C.byref(ms), C.byref(n), C.byref(nw),
                                 C.byref(by)), "sisl_timer_read")
     return ms.value, n.value, nw.value, by.value
+
+
+def ptr_array(tensors):
+    """ctypes array of the tensors' device pointers."""
+    return (C.c_void_p * len(tensors))(*[t.data_ptr() for t in tensors])
 
 
 def check(rc: int, what: str):

@@ -14,6 +14,7 @@
 namespace sisl {
 
 constexpr int kBlock = 256;
+constexpr int kMaxMembers = 32;  // members of one batched (ensemble) launch
 
 inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
 
@@ -118,6 +119,36 @@ __device__ inline T wave_sum(T v) {
 }  // namespace sisl
 
 namespace sisl {
+// One member's operator application in a batched launch (device table entry):
+// input x + j * xj, output y, transpose scratch scr (3 mass arrays), skipped
+// when stop is set and stop[j * sj] != 0.
+template <typename T>
+struct BOp {
+  OpT<T> o;
+  const T *x;
+  int64_t xj;
+  T *y;
+  T *scr;
+  const int *stop;
+  int sj;
+};
+// Batched launchers: member m = blockIdx.y over M members; `tab` in device
+// memory; `maxb` = the largest per-member block count (host-computed).
+template <typename T>
+int launch_A_batched(const BOp<T> *tab, int M, int maxb, int j, hipStream_t s);
+template <typename T>
+int launch_AT_batched(const BOp<T> *tab, int M, int maxb1, int maxb2, int j, hipStream_t s);
+template <typename T>
+int launch_M_batched(const BOp<T> *tab, int M, int maxb, int L, int S, int j, hipStream_t s);
+template <typename T>
+int launch_MT_batched(const BOp<T> *tab, int M, int maxcols, int maxfaces, int j, hipStream_t s);
+// Host-side launch geometry of one member (for maxb).
+int blocks_A(const Stag &s);
+int blocks_AT1(const Stag &s);
+int blocks_AT2(const Stag &s);
+int blocks_M(const Stag &s);
+int seg_levels(int nz);  // levels per segment L of the preconditioner (0: too tall)
+
 // Optional launch timing (timing.hip): returns a record id or -1 when off.
 int timer_begin(int kind, bool fp32, double bytes, const int *stop, const int *gate, hipStream_t s);
 void timer_end(int id, hipStream_t s);

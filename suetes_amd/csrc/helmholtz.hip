@@ -51,15 +51,21 @@ struct AView {
 // blockIdx.z enumerating (segment, level).
 constexpr int kTX = 64, kTY = 4;
 
+// Blocks of the forward operator at the default chunk (8 level chunks).
+__device__ __forceinline__ int blocks_A_dev(const Stag &s) {
+  const int kc = (s.nz + 1 + 7) / 8;
+  return ((s.nx + kTX - 1) / kTX) * ((s.ny + kTY - 1) / kTY) * ((s.nz + 1 + kc - 1) / kc);
+}
+
+// Body of the forward operator for launch block `blk` of `nblk` (a member of
+// a batched launch passes its own block index and count).
 template <typename T, int UNR>
-__global__ void __launch_bounds__(256) helmholtz_kernel(OpT<T> o, const T *__restrict__ x,
-                                                        T *__restrict__ y, int kchunk,
-                                                        const int *__restrict__ stop) {
-  if (stop && *stop) return;
+__device__ __forceinline__ void helmholtz_body(const OpT<T> &o, const T *__restrict__ x,
+                                               T *__restrict__ y, int kchunk, int blk, int nblk) {
   const Stag &s = o.s;
   const int nx = s.nx, ny = s.ny, nz = s.nz;
   const int nbx = (nx + kTX - 1) / kTX, nby = (ny + kTY - 1) / kTY;
-  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int tile = xcd_tile(blk, nblk);
   const int bx = tile % nbx, by = (tile / nbx) % nby, bz = tile / (nbx * nby);
   const int i = bx * kTX + threadIdx.x, j = by * kTY + threadIdx.y;
   if (i >= nx || j >= ny) return;
@@ -203,6 +209,15 @@ __global__ void __launch_bounds__(256) helmholtz_kernel(OpT<T> o, const T *__res
   }
 }
 
+template <typename T, int UNR>
+__global__ void __launch_bounds__(256) helmholtz_kernel(OpT<T> o, const T *__restrict__ x,
+                                                        T *__restrict__ y, int kchunk,
+                                                        const int *__restrict__ stop) {
+  if (stop && *stop) return;
+  helmholtz_body<T, UNR>(o, x, y, kchunk, blockIdx.x, gridDim.x);
+}
+
+
 inline dim3 point_grid(const Stag &s, int segments) {
   return dim3((unsigned)((s.nx + 1 + kTX - 1) / kTX), (unsigned)((s.ny + 1 + kTY - 1) / kTY),
               (unsigned)(segments * (s.nz + 1)));
@@ -244,15 +259,12 @@ struct ATView {
 };
 
 template <typename T>
-__global__ void __launch_bounds__(256) helmholtz_T1_kernel(OpT<T> o, const T *__restrict__ y,
-                                                           T *__restrict__ gzm_b,
-                                                           T *__restrict__ um_b,
-                                                           T *__restrict__ vm_b,
-                                                           const int *__restrict__ stop) {
-  if (stop && *stop) return;
+__device__ __forceinline__ void helmholtz_T1_body(const OpT<T> &o, const T *__restrict__ y,
+                                                  T *__restrict__ gzm_b, T *__restrict__ um_b,
+                                                  T *__restrict__ vm_b, int bx, int by, int bz) {
   const Stag &s = o.s;
   const int nx = s.nx, ny = s.ny, nz = s.nz;
-  const int i = blockIdx.x * kTX + threadIdx.x, j = blockIdx.y * kTY + threadIdx.y, k = blockIdx.z;
+  const int i = bx * kTX + threadIdx.x, j = by * kTY + threadIdx.y, k = bz;
   if (i >= nx || j >= ny || k >= nz) return;
   const int64_t p = s.m.at(i, j, k);
   ATView<T> t(o, y);
@@ -278,17 +290,25 @@ __global__ void __launch_bounds__(256) helmholtz_T1_kernel(OpT<T> o, const T *__
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) helmholtz_T2_kernel(OpT<T> o, const T *__restrict__ y,
-                                                           const T *__restrict__ gzm_b,
-                                                           const T *__restrict__ um_b,
-                                                           const T *__restrict__ vm_b,
-                                                           T *__restrict__ xo,
+__global__ void __launch_bounds__(256) helmholtz_T1_kernel(OpT<T> o, const T *__restrict__ y,
+                                                           T *__restrict__ gzm_b,
+                                                           T *__restrict__ um_b,
+                                                           T *__restrict__ vm_b,
                                                            const int *__restrict__ stop) {
   if (stop && *stop) return;
+  helmholtz_T1_body(o, y, gzm_b, um_b, vm_b, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+template <typename T>
+__device__ __forceinline__ void helmholtz_T2_body(const OpT<T> &o, const T *__restrict__ y,
+                                                  const T *__restrict__ gzm_b,
+                                                  const T *__restrict__ um_b,
+                                                  const T *__restrict__ vm_b, T *__restrict__ xo,
+                                                  int bx, int by, int bz) {
   const Stag &s = o.s;
   const int nx = s.nx, ny = s.ny, nz = s.nz;
-  const int seg = blockIdx.z / (nz + 1), k = blockIdx.z - seg * (nz + 1);
-  const int i = blockIdx.x * kTX + threadIdx.x, j = blockIdx.y * kTY + threadIdx.y;
+  const int seg = bz / (nz + 1), k = bz - seg * (nz + 1);
+  const int i = bx * kTX + threadIdx.x, j = by * kTY + threadIdx.y;
   ATView<T> t(o, y);
   T out;
   int64_t q;
@@ -370,6 +390,17 @@ __global__ void __launch_bounds__(256) helmholtz_T2_kernel(OpT<T> o, const T *__
     }
   }
   xo[q] = out;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) helmholtz_T2_kernel(OpT<T> o, const T *__restrict__ y,
+                                                           const T *__restrict__ gzm_b,
+                                                           const T *__restrict__ um_b,
+                                                           const T *__restrict__ vm_b,
+                                                           T *__restrict__ xo,
+                                                           const int *__restrict__ stop) {
+  if (stop && *stop) return;
+  helmholtz_T2_body(o, y, gzm_b, um_b, vm_b, xo, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
 // Launch shape of the forward operator: levels per thread (chunk) and the
@@ -464,6 +495,70 @@ static int autotune_A(const OpT<T> &o, const T *x, T *y, hipStream_t st, int32_t
   if (best_unr) *best_unr = best.unr;
   return SISL_OK;
 }
+
+// ---- batched (ensemble) launches -------------------------------------------
+
+int blocks_A(const Stag &s) {
+  const int kc = (s.nz + 1 + 7) / 8;
+  return ((s.nx + kTX - 1) / kTX) * ((s.ny + kTY - 1) / kTY) * ((s.nz + 1 + kc - 1) / kc);
+}
+int blocks_AT1(const Stag &s) { return ((s.nx + kTX - 1) / kTX) * ((s.ny + kTY - 1) / kTY) * s.nz; }
+int blocks_AT2(const Stag &s) {
+  return ((s.nx + 1 + kTX - 1) / kTX) * ((s.ny + 1 + kTY - 1) / kTY) * 5 * (s.nz + 1);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) helmholtz_batched_kernel(const BOp<T> *__restrict__ tab, int j) {
+  const BOp<T> &b = tab[blockIdx.y];
+  if (b.stop && b.stop[j * b.sj]) return;
+  const int nb = blocks_A_dev(b.o.s);
+  if ((int)blockIdx.x >= nb) return;
+  helmholtz_body<T, 1>(b.o, b.x + j * b.xj, b.y, (b.o.s.nz + 1 + 7) / 8, blockIdx.x, nb);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) helmholtz_T1_batched_kernel(const BOp<T> *__restrict__ tab, int j) {
+  const BOp<T> &b = tab[blockIdx.y];
+  if (b.stop && b.stop[j * b.sj]) return;
+  const Stag &s = b.o.s;
+  const int gx = (s.nx + kTX - 1) / kTX, gy = (s.ny + kTY - 1) / kTY;
+  const int lin = blockIdx.x;
+  if (lin >= gx * gy * s.nz) return;
+  const int64_t nm = s.m.n();
+  helmholtz_T1_body(b.o, b.x + j * b.xj, b.scr, b.scr + nm, b.scr + 2 * nm, lin % gx, (lin / gx) % gy,
+                    lin / (gx * gy));
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) helmholtz_T2_batched_kernel(const BOp<T> *__restrict__ tab, int j) {
+  const BOp<T> &b = tab[blockIdx.y];
+  if (b.stop && b.stop[j * b.sj]) return;
+  const Stag &s = b.o.s;
+  const int gx = (s.nx + 1 + kTX - 1) / kTX, gy = (s.ny + 1 + kTY - 1) / kTY;
+  const int lin = blockIdx.x;
+  if (lin >= gx * gy * 5 * (s.nz + 1)) return;
+  const int64_t nm = s.m.n();
+  helmholtz_T2_body(b.o, b.x + j * b.xj, (const T *)b.scr, (const T *)(b.scr + nm),
+                    (const T *)(b.scr + 2 * nm), b.y, lin % gx, (lin / gx) % gy, lin / (gx * gy));
+}
+
+template <typename T>
+int launch_A_batched(const BOp<T> *tab, int M, int maxb, int j, hipStream_t st) {
+  hipLaunchKernelGGL(helmholtz_batched_kernel<T>, dim3(maxb, M), dim3(kTX, kTY), 0, st, tab, j);
+  SISL_CHECK_LAUNCH();
+  return SISL_OK;
+}
+template <typename T>
+int launch_AT_batched(const BOp<T> *tab, int M, int maxb1, int maxb2, int j, hipStream_t st) {
+  hipLaunchKernelGGL(helmholtz_T1_batched_kernel<T>, dim3(maxb1, M), dim3(kTX, kTY), 0, st, tab, j);
+  hipLaunchKernelGGL(helmholtz_T2_batched_kernel<T>, dim3(maxb2, M), dim3(kTX, kTY), 0, st, tab, j);
+  SISL_CHECK_LAUNCH();
+  return SISL_OK;
+}
+template int launch_A_batched<double>(const BOp<double> *, int, int, int, hipStream_t);
+template int launch_A_batched<float>(const BOp<float> *, int, int, int, hipStream_t);
+template int launch_AT_batched<double>(const BOp<double> *, int, int, int, int, hipStream_t);
+template int launch_AT_batched<float>(const BOp<float> *, int, int, int, int, hipStream_t);
 
 template int launch_A<double>(const OpT<double> &, const double *, double *, hipStream_t, const int *);
 template int launch_A<float>(const OpT<float> &, const float *, float *, hipStream_t, const int *);

@@ -3,6 +3,8 @@
 //
 // Reference: suetes/regional3d/operators.py:134-209 (tricubic),
 //            suetes/regional3d/steppers.py:276-367 (trajectories).
+#include <algorithm>
+
 #include "common.h"
 
 namespace sisl {
@@ -86,13 +88,13 @@ __device__ __forceinline__ T tricubic_point(T tx, T ty, T tz, int lim, Fetch F) 
 constexpr int kIX = 64, kIY = 4, kSX = kIX + 8, kSY = kIY + 5, kRing = 6;
 
 template <typename T>
-__global__ void __launch_bounds__(256) tricubic_kernel(Dims d, const T *__restrict__ f,
-                                                       const T *__restrict__ co,
-                                                       T *__restrict__ out, int lim, int kchunk) {
+__device__ __forceinline__ void tricubic_body(const Dims &d, const T *__restrict__ f,
+                                              const T *__restrict__ co, T *__restrict__ out, int lim,
+                                              int kchunk, int blk, int nblk) {
   __shared__ T ring[kRing][kSY][kSX];
   const int nx = d.nx, ny = d.ny, nz = d.nz;
   const int nbx = (nx + kIX - 1) / kIX, nby = (ny + kIY - 1) / kIY;
-  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int tile = xcd_tile(blk, nblk);
   const int bx = tile % nbx, by = (tile / nbx) % nby, bz = tile / (nbx * nby);
   const int tid = threadIdx.y * kIX + threadIdx.x;
   const int xs = bx * kIX - 2, ys = by * kIY - 2;
@@ -149,6 +151,41 @@ __global__ void __launch_bounds__(256) tricubic_kernel(Dims d, const T *__restri
     }
     stg(out, p, r);
   }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) tricubic_kernel(Dims d, const T *__restrict__ f,
+                                                       const T *__restrict__ co,
+                                                       T *__restrict__ out, int lim, int kchunk) {
+  tricubic_body(d, f, co, out, lim, kchunk, blockIdx.x, gridDim.x);
+}
+
+// Launch geometry of the staged gather for one field: ~8 level chunks per
+// column keep enough blocks in flight.
+__host__ __device__ inline int tricubic_chunk(int fnz) { return (fnz + 7) / 8 > 1 ? (fnz + 7) / 8 : 1; }
+__host__ __device__ inline int tricubic_blocks(int fnx, int fny, int fnz) {
+  const int kc = tricubic_chunk(fnz);
+  return ((fnx + kIX - 1) / kIX) * ((fny + kIY - 1) / kIY) * ((fnz + kc - 1) / kc);
+}
+
+// Batched gather: member m = blockIdx.y, its table passed by value (no
+// device-side descriptor upload, so a captured graph replays it as is).
+template <typename T>
+struct TricubicBatch {
+  int M, lim;
+  Dims d[kMaxMembers];
+  const T *f[kMaxMembers];
+  const T *co[kMaxMembers];
+  T *out[kMaxMembers];
+};
+
+template <typename T>
+__global__ void __launch_bounds__(256) tricubic_batched_kernel(TricubicBatch<T> tb) {
+  const int m = blockIdx.y;
+  const Dims &d = tb.d[m];
+  const int nb = tricubic_blocks(d.nx, d.ny, d.nz);
+  if ((int)blockIdx.x >= nb) return;
+  tricubic_body(d, tb.f[m], tb.co[m], tb.out[m], tb.lim, tricubic_chunk(d.nz), blockIdx.x, nb);
 }
 
 // ---- deterministic transpose ------------------------------------------------
@@ -340,12 +377,11 @@ __global__ void __launch_bounds__(256) tricubic_adj_kernel(Dims d, const T *__re
 
 // Index-space velocities at staggering `loc` (steppers.py:276-316).
 template <typename T>
-__global__ void index_velocity_kernel(Stag s, int loc, const T *__restrict__ u,
-                                      const T *__restrict__ v, const T *__restrict__ e,
-                                      const T *__restrict__ m2, T rdx, T rdy,
-                                      T *__restrict__ out, Dims L) {
+__device__ __forceinline__ void index_velocity_body(const Stag &s, int loc, const T *__restrict__ u,
+                                                   const T *__restrict__ v, const T *__restrict__ e,
+                                                   const T *__restrict__ m2, T rdx, T rdy,
+                                                   T *__restrict__ out, const Dims &L, int64_t p) {
   const int64_t n = L.n();
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
   const int i = (int)(p % L.nx);
   const int j = (int)((p / L.nx) % L.ny);
@@ -394,6 +430,15 @@ __global__ void index_velocity_kernel(Stag s, int loc, const T *__restrict__ u,
   out[2 * n + p] = wl;
 }
 
+template <typename T>
+__global__ void index_velocity_kernel(Stag s, int loc, const T *__restrict__ u,
+                                      const T *__restrict__ v, const T *__restrict__ e,
+                                      const T *__restrict__ m2, T rdx, T rdy,
+                                      T *__restrict__ out, Dims L) {
+  index_velocity_body(s, loc, u, v, e, m2, rdx, rdy, out, L,
+                      (int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
 // map_coordinates(order=1, mode='nearest') of one field at X + d
 // (steppers.py:367): per axis i = floor(x), t = x - i, indices clamped.
 template <typename T>
@@ -429,10 +474,9 @@ __device__ __forceinline__ T trilinear_nearest(const T *__restrict__ f, Dims d, 
 // Midpoint trajectory iteration (steppers.py:318-360); writes the departure
 // offsets -alpha = x_d - x_a.
 template <typename T>
-__global__ void departure_kernel(Dims L, const T *__restrict__ vel, T dt, int iters,
-                                 T *__restrict__ out) {
+__device__ __forceinline__ void departure_body(const Dims &L, const T *__restrict__ vel, T dt,
+                                              int iters, T *__restrict__ out, int64_t p) {
   const int64_t n = L.n();
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
   const int X = (int)(p % L.nx), Y = (int)((p / L.nx) % L.ny), Z = (int)(p / ((int64_t)L.nx * L.ny));
   T ax = dt * vel[p], ay = dt * vel[n + p], az = dt * vel[2 * n + p];
@@ -449,13 +493,49 @@ __global__ void departure_kernel(Dims L, const T *__restrict__ vel, T dt, int it
 }
 
 template <typename T>
+__global__ void departure_kernel(Dims L, const T *__restrict__ vel, T dt, int iters,
+                                 T *__restrict__ out) {
+  departure_body(L, vel, dt, iters, out, (int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// Dims of staggering loc (0=m, 1=u, 2=v, 3=w) of a grid.
+inline __host__ __device__ Dims loc_dims(const SislGrid &g, int loc) {
+  return Dims{g.nx + (loc == 1 ? 1 : 0), g.ny + (loc == 2 ? 1 : 0), g.nz + (loc == 3 ? 1 : 0)};
+}
+
+// Batched trajectories: member m = blockIdx.y, table by value.
+template <typename T>
+struct DepartureBatch {
+  int M, loc, iters;
+  T dt;
+  SislGrid g[kMaxMembers];
+  const T *u[kMaxMembers], *v[kMaxMembers], *e[kMaxMembers], *m2[kMaxMembers];
+  T *scr[kMaxMembers], *out[kMaxMembers];
+  T rdx[kMaxMembers], rdy[kMaxMembers];
+};
+
+template <typename T>
+__global__ void index_velocity_batched_kernel(DepartureBatch<T> db) {
+  const int m = blockIdx.y;
+  const Stag s = Stag::make(db.g[m]);
+  index_velocity_body(s, db.loc, db.u[m], db.v[m], db.e[m], db.m2[m], db.rdx[m], db.rdy[m], db.scr[m],
+                      loc_dims(db.g[m], db.loc), (int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+template <typename T>
+__global__ void departure_batched_kernel(DepartureBatch<T> db) {
+  const int m = blockIdx.y;
+  departure_body(loc_dims(db.g[m], db.loc), (const T *)db.scr[m], db.dt, db.iters, db.out[m],
+                 (int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+template <typename T>
 int tricubic(int fnx, int fny, int fnz, const T *f, const T *co, T *out, int lim, void *st) {
   if (!f || !co || !out || fnx < 1 || fny < 1 || fnz < 1) return SISL_ERR_ARG;
   Dims d{fnx, fny, fnz};
   if ((uint64_t)d.n() * 3 * sizeof(T) >= (1ull << 32)) return SISL_ERR_ARG;  // 32-bit offsets
-  // ~8 level chunks per column keep enough blocks in flight
-  const int kc = max(1, (fnz + 7) / 8);
-  const int nb = ((fnx + kIX - 1) / kIX) * ((fny + kIY - 1) / kIY) * ((fnz + kc - 1) / kc);
+  const int kc = tricubic_chunk(fnz);
+  const int nb = tricubic_blocks(fnx, fny, fnz);
   const int tid = timer_begin(SISL_T_TRICUBIC, sizeof(T) == 4, (double)sizeof(T) * 5.0 * d.n(),
                               nullptr, nullptr, as_stream(st));
   hipLaunchKernelGGL(tricubic_kernel<T>, dim3(nb), dim3(kIX, kIY), 0, as_stream(st), d, f, co, out,
@@ -512,9 +592,93 @@ int departure(const SislGrid *g, int loc, const T *u, const T *v, const T *e, co
   return SISL_OK;
 }
 
+template <typename T>
+int tricubic_batched(int M, const int32_t *dims, const T *const *f, const T *const *co, T *const *out,
+                     int lim, void *st) {
+  if (M < 1 || M > kMaxMembers || !dims || !f || !co || !out) return SISL_ERR_ARG;
+  TricubicBatch<T> tb;
+  tb.M = M;
+  tb.lim = lim;
+  int maxb = 1;
+  double bytes = 0;
+  for (int m = 0; m < M; ++m) {
+    const Dims d{dims[3 * m], dims[3 * m + 1], dims[3 * m + 2]};
+    if (d.nx < 1 || d.ny < 1 || d.nz < 1 || !f[m] || !co[m] || !out[m]) return SISL_ERR_ARG;
+    if ((uint64_t)d.n() * 3 * sizeof(T) >= (1ull << 32)) return SISL_ERR_ARG;
+    tb.d[m] = d;
+    tb.f[m] = f[m];
+    tb.co[m] = co[m];
+    tb.out[m] = out[m];
+    maxb = max(maxb, tricubic_blocks(d.nx, d.ny, d.nz));
+    bytes += (double)sizeof(T) * 5.0 * d.n();
+  }
+  hipStream_t hs = as_stream(st);
+  const int tid = timer_begin(SISL_T_TRICUBIC, sizeof(T) == 4, bytes, nullptr, nullptr, hs);
+  hipLaunchKernelGGL(tricubic_batched_kernel<T>, dim3(maxb, M), dim3(kIX, kIY), 0, hs, tb);
+  timer_end(tid, hs);
+  SISL_CHECK_LAUNCH();
+  return SISL_OK;
+}
+
+template <typename T>
+int departure_batched(int M, const SislGrid *g, int loc, const T *const *u, const T *const *v,
+                      const T *const *e, const T *const *m2, const double *dx, const double *dy, double dt,
+                      int iters, T *const *scr, T *const *out, void *st) {
+  if (M < 1 || M > kMaxMembers || !g || !u || !v || !e || !m2 || !dx || !dy || !scr || !out || loc < 0 ||
+      loc > 3)
+    return SISL_ERR_ARG;
+  DepartureBatch<T> db;
+  db.M = M;
+  db.loc = loc;
+  db.iters = iters;
+  db.dt = (T)dt;
+  int64_t maxn = 1;
+  for (int m = 0; m < M; ++m) {
+    if (!u[m] || !v[m] || !e[m] || !m2[m] || !scr[m] || !out[m] || g[m].nx < 2 || g[m].nz < 2)
+      return SISL_ERR_ARG;
+    db.g[m] = g[m];
+    db.u[m] = u[m];
+    db.v[m] = v[m];
+    db.e[m] = e[m];
+    db.m2[m] = m2[m];
+    db.scr[m] = scr[m];
+    db.out[m] = out[m];
+    db.rdx[m] = (T)(1.0 / dx[m]);
+    db.rdy[m] = (T)(1.0 / dy[m]);
+    maxn = std::max<int64_t>(maxn, loc_dims(g[m], loc).n());
+  }
+  hipStream_t hs = as_stream(st);
+  const dim3 grid(grid_for(maxn), M);
+  hipLaunchKernelGGL(index_velocity_batched_kernel<T>, grid, dim3(kBlock), 0, hs, db);
+  hipLaunchKernelGGL(departure_batched_kernel<T>, grid, dim3(kBlock), 0, hs, db);
+  SISL_CHECK_LAUNCH();
+  return SISL_OK;
+}
+
 }  // namespace sisl
 
 extern "C" {
+
+int sisl_tricubic_batched_f64(int32_t M, const int32_t *dims, const double *const *f,
+                              const double *const *co, double *const *out, int32_t lim, void *s) {
+  return sisl::tricubic_batched<double>(M, dims, f, co, out, lim, s);
+}
+int sisl_tricubic_batched_f32(int32_t M, const int32_t *dims, const float *const *f,
+                              const float *const *co, float *const *out, int32_t lim, void *s) {
+  return sisl::tricubic_batched<float>(M, dims, f, co, out, lim, s);
+}
+int sisl_departure_batched_f64(int32_t M, const SislGrid *g, int32_t loc, const double *const *u,
+                               const double *const *v, const double *const *e,
+                               const double *const *m2, const double *dx, const double *dy, double dt,
+                               int32_t it, double *const *scr, double *const *out, void *s) {
+  return sisl::departure_batched<double>(M, g, loc, u, v, e, m2, dx, dy, dt, it, scr, out, s);
+}
+int sisl_departure_batched_f32(int32_t M, const SislGrid *g, int32_t loc, const float *const *u,
+                               const float *const *v, const float *const *e,
+                               const float *const *m2, const double *dx, const double *dy, double dt,
+                               int32_t it, float *const *scr, float *const *out, void *s) {
+  return sisl::departure_batched<float>(M, g, loc, u, v, e, m2, dx, dy, dt, it, scr, out, s);
+}
 
 int32_t sisl_scan_blocks(int64_t n) { return (int32_t)((n + sisl::kScanTile - 1) / sisl::kScanTile); }
 

@@ -12,7 +12,10 @@
 // "subtract + norm" pass, one normalisation pass.  Inner products accumulate
 // in FP64 whatever the basis type; reductions are two-stage and fixed-order,
 // so results are bit-reproducible run to run.
+#include <algorithm>
 #include <cmath>
+#include <map>
+#include <mutex>
 #include <cstring>
 #include <vector>
 
@@ -38,11 +41,14 @@ struct Small {
   int steps;    // Arnoldi steps completed in this cycle
   int gate2;    // second Gram-Schmidt pass required
   int pad;
+  // device-controlled outer loop (batched solver)
+  int done, outer, steps_total, pad2;
+  double b_norm, Mb_norm, ptol, rnorm;
 };
 
 // ---- block reduction helpers -------------------------------------------------
 // One value per thread -> partials[slot][block] (fixed-order, deterministic).
-__device__ __forceinline__ void block_store_slot(double v, int slot, double *partials) {
+__device__ __forceinline__ void block_store_slot(double v, int slot, double *partials, int blk) {
   __shared__ double red1[kBlock / 64];
   const double s = wave_sum(v);
   if ((threadIdx.x & 63) == 0) red1[threadIdx.x >> 6] = s;
@@ -51,14 +57,15 @@ __device__ __forceinline__ void block_store_slot(double v, int slot, double *par
     double t = 0;
 #pragma unroll
     for (int q = 0; q < kBlock / 64; ++q) t += red1[q];
-    partials[(int64_t)slot * kRedGrid + blockIdx.x] = t;
+    partials[(int64_t)slot * kRedGrid + blk] = t;
   }
   __syncthreads();
 }
 
 // kGrp values per thread -> partials[v0 + r][block] for r < cnt.
 template <int G>
-__device__ __forceinline__ void block_store_group(double (&acc)[G], int v0, int cnt, double *partials) {
+__device__ __forceinline__ void block_store_group(double (&acc)[G], int v0, int cnt, double *partials,
+                                                  int blk) {
   __shared__ double red[kBlock / 64][G];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
@@ -71,7 +78,7 @@ __device__ __forceinline__ void block_store_group(double (&acc)[G], int v0, int 
     double t = 0;
 #pragma unroll
     for (int q = 0; q < kBlock / 64; ++q) t += red[q][threadIdx.x];
-    partials[(int64_t)(v0 + threadIdx.x) * kRedGrid + blockIdx.x] = t;
+    partials[(int64_t)(v0 + threadIdx.x) * kRedGrid + blk] = t;
   }
   __syncthreads();
 }
@@ -149,16 +156,16 @@ __device__ __forceinline__ bool pass_skipped(const int *stop, const int *gate) {
 // every v in [v0, v0+8) of every group, plus (slot nv) w . w.  FP64
 // accumulation; vinv undoes the basis storage scale.
 template <typename TB, typename TV>
-__global__ void __launch_bounds__(256) dots_kernel(const TB *__restrict__ V, int64_t ldv, int nv,
-                                                   const TV *__restrict__ w, int64_t n,
-                                                   double *__restrict__ partials, double vinv,
-                                                   const int *__restrict__ stop,
-                                                   const int *__restrict__ gate) {
+__device__ __forceinline__ void dots_body(const TB *__restrict__ V, int64_t ldv, int nv,
+                                          const TV *__restrict__ w, int64_t n,
+                                          double *__restrict__ partials, double vinv,
+                                          const int *__restrict__ stop,
+                                          const int *__restrict__ gate, int blk, int nblk) {
   if (pass_skipped(stop, gate)) return;
   constexpr int N = Vec<TV>::N;
   const int64_t nq = n / N;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t stride = (int64_t)nblk * blockDim.x;
+  const int64_t t0 = (int64_t)blk * blockDim.x + threadIdx.x;
   const int64_t tail = n - nq * N;
   {  // w . w
     double acc[1] = {0.0};
@@ -169,7 +176,7 @@ __global__ void __launch_bounds__(256) dots_kernel(const TB *__restrict__ V, int
       for (int k = 0; k < N; ++k) acc[0] += (double)we[k] * (double)we[k];
     }
     if (t0 < tail) acc[0] += (double)w[nq * N + t0] * (double)w[nq * N + t0];
-    block_store_slot(acc[0], nv, partials);
+    block_store_slot(acc[0], nv, partials, blk);
   }
 #pragma unroll 1
   for (int g = 0; g * kGrp < nv; ++g) {
@@ -198,24 +205,33 @@ __global__ void __launch_bounds__(256) dots_kernel(const TB *__restrict__ V, int
     const int cnt = min(kGrp, nv - g * kGrp);
 #pragma unroll
     for (int r = 0; r < kGrp; ++r) acc[r] *= vinv;
-    block_store_group(acc, g * kGrp, cnt, partials);
+    block_store_group(acc, g * kGrp, cnt, partials, blk);
   }
+}
+
+template <typename TB, typename TV>
+__global__ void __launch_bounds__(256) dots_kernel(const TB *__restrict__ V, int64_t ldv, int nv,
+                                                   const TV *__restrict__ w, int64_t n,
+                                                   double *__restrict__ partials, double vinv,
+                                                   const int *__restrict__ stop,
+                                                   const int *__restrict__ gate) {
+  dots_body(V, ldv, nv, w, n, partials, vinv, stop, gate, blockIdx.x, gridDim.x);
 }
 
 // Single-sweep variant for nv <= 8*G: w is read once, all G groups of
 // projections and w . w accumulate together (8*G + 1 FP64 accumulators per
 // lane).  Same partial-slot layout as dots_kernel.
 template <typename TB, typename TV, int G>
-__global__ void __launch_bounds__(256) dots1_kernel(const TB *__restrict__ V, int64_t ldv, int nv,
-                                                    const TV *__restrict__ w, int64_t n,
-                                                    double *__restrict__ partials, double vinv,
-                                                    const int *__restrict__ stop,
-                                                    const int *__restrict__ gate) {
+__device__ __forceinline__ void dots1_body(const TB *__restrict__ V, int64_t ldv, int nv,
+                                           const TV *__restrict__ w, int64_t n,
+                                           double *__restrict__ partials, double vinv,
+                                           const int *__restrict__ stop,
+                                           const int *__restrict__ gate, int blk, int nblk) {
   if (pass_skipped(stop, gate)) return;
   constexpr int N = Vec<TV>::N;
   const int64_t nq = n / N;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t stride = (int64_t)nblk * blockDim.x;
+  const int64_t t0 = (int64_t)blk * blockDim.x + threadIdx.x;
   double acc[G][kGrp];
   double ww = 0;
 #pragma unroll
@@ -247,24 +263,32 @@ __global__ void __launch_bounds__(256) dots1_kernel(const TB *__restrict__ V, in
 #pragma unroll
       for (int r = 0; r < kGrp; ++r) acc[g][r] += (double)V[(int64_t)min(g * kGrp + r, nv - 1) * ldv + e] * we;
   }
-  block_store_slot(ww, nv, partials);
+  block_store_slot(ww, nv, partials, blk);
 #pragma unroll
   for (int g = 0; g < G; ++g) {
     if (g * kGrp >= nv) break;
 #pragma unroll
     for (int r = 0; r < kGrp; ++r) acc[g][r] *= vinv;
-    block_store_group(acc[g], g * kGrp, min(kGrp, nv - g * kGrp), partials);
+    block_store_group(acc[g], g * kGrp, min(kGrp, nv - g * kGrp), partials, blk);
   }
+}
+
+template <typename TB, typename TV, int G>
+__global__ void __launch_bounds__(256) dots1_kernel(const TB *__restrict__ V, int64_t ldv, int nv,
+                                                    const TV *__restrict__ w, int64_t n,
+                                                    double *__restrict__ partials, double vinv,
+                                                    const int *__restrict__ stop,
+                                                    const int *__restrict__ gate) {
+  dots1_body<TB, TV, G>(V, ldv, nv, w, n, partials, vinv, stop, gate, blockIdx.x, gridDim.x);
 }
 
 // w <- w - sum_v h[v] vinv V[v] (h read from device memory), slot 0 <- w . w
 template <typename TB, typename TV>
-__global__ void __launch_bounds__(256) update_kernel(const TB *__restrict__ V, int64_t ldv, int nv,
-                                                     const double *__restrict__ h,
-                                                     TV *__restrict__ w, int64_t n,
-                                                     double *__restrict__ partials, double vinv,
-                                                     const int *__restrict__ stop,
-                                                     const int *__restrict__ gate) {
+__device__ __forceinline__ void update_body(const TB *__restrict__ V, int64_t ldv, int nv,
+                                            const double *__restrict__ h, TV *__restrict__ w,
+                                            int64_t n, double *__restrict__ partials, double vinv,
+                                            const int *__restrict__ stop,
+                                            const int *__restrict__ gate, int blk, int nblk) {
   if (pass_skipped(stop, gate)) return;
   constexpr int N = Vec<TV>::N;
   __shared__ double hs[kMaxRestart + 8];
@@ -272,8 +296,8 @@ __global__ void __launch_bounds__(256) update_kernel(const TB *__restrict__ V, i
   for (int v = threadIdx.x; v < nvr; v += blockDim.x) hs[v] = v < nv ? h[v] * vinv : 0.0;
   __syncthreads();
   const int64_t nq = n / N;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t stride = (int64_t)nblk * blockDim.x;
+  const int64_t t0 = (int64_t)blk * blockDim.x + threadIdx.x;
   double ww = 0;
   for (int64_t q = t0; q < nq; q += stride) {
     TV we[N];
@@ -308,13 +332,27 @@ __global__ void __launch_bounds__(256) update_kernel(const TB *__restrict__ V, i
     w[e] = sn;
     ww += (double)sn * (double)sn;
   }
-  block_store_slot(ww, 0, partials);
+  block_store_slot(ww, 0, partials, blk);
+}
+
+template <typename TB, typename TV>
+__global__ void __launch_bounds__(256) update_kernel(const TB *__restrict__ V, int64_t ldv, int nv,
+                                                     const double *__restrict__ h,
+                                                     TV *__restrict__ w, int64_t n,
+                                                     double *__restrict__ partials, double vinv,
+                                                     const int *__restrict__ stop,
+                                                     const int *__restrict__ gate) {
+  update_body(V, ldv, nv, h, w, n, partials, vinv, stop, gate, blockIdx.x, gridDim.x);
 }
 
 // JAX _iterative_classical_gram_schmidt loop condition: a second pass runs
 // when ||r|| < ||q||/sqrt(2) after the first (norms below eps count as 0).
+__device__ void decide_body(Small *sm, int nv);
 __global__ void decide_kernel(Small *sm, int nv) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  decide_body(sm, nv);
+}
+__device__ void decide_body(Small *sm, int nv) {
   if (sm->broken) {
     sm->gate2 = 0;
     return;
@@ -330,8 +368,12 @@ __global__ void decide_kernel(Small *sm, int nv) {
 
 // Arnoldi bookkeeping for step j (one thread): JAX's H row, breakdown test
 // (v_norm_1 <= eps * v_norm_0), Givens residual estimate for early exit.
+__device__ void finalize_body(Small *sm, int j, int R, int early_exit);
 __global__ void finalize_kernel(Small *sm, int j, int R, int early_exit) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  finalize_body(sm, j, R, early_exit);
+}
+__device__ void finalize_body(Small *sm, int j, int R, int early_exit) {
   if (sm->broken) {
     sm->inv = 0.0;
     return;
@@ -370,16 +412,17 @@ __global__ void finalize_kernel(Small *sm, int j, int R, int early_exit) {
 // nothing once the cycle has stopped (those columns are outside the least
 // squares).
 template <typename TB, typename TV>
-__global__ void normalize_gated_kernel(const TV *__restrict__ w, TB *__restrict__ out,
-                                       TV *__restrict__ cur, int64_t n, double vscale,
-                                       const Small *__restrict__ sm,
-                                       const int *__restrict__ stop_before) {
+__device__ __forceinline__ void normalize_gated_body(const TV *__restrict__ w, TB *__restrict__ out,
+                                                     TV *__restrict__ cur, int64_t n, double vscale,
+                                                     const Small *__restrict__ sm,
+                                                     const int *__restrict__ stop_before, int blk,
+                                                     int nblk) {
   if (*stop_before) return;
   const double inv = sm->inv;
   constexpr int N = Vec<TV>::N;
   const int64_t nq = n / N;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t stride = (int64_t)nblk * blockDim.x;
+  const int64_t t0 = (int64_t)blk * blockDim.x + threadIdx.x;
   for (int64_t q = t0; q < nq; q += stride) {
     TV a[N];
     TB o[N];
@@ -399,6 +442,14 @@ __global__ void normalize_gated_kernel(const TV *__restrict__ w, TB *__restrict_
     out[e] = (TB)(u * vscale);
     if (cur) cur[e] = (TV)u;
   }
+}
+
+template <typename TB, typename TV>
+__global__ void normalize_gated_kernel(const TV *__restrict__ w, TB *__restrict__ out,
+                                       TV *__restrict__ cur, int64_t n, double vscale,
+                                       const Small *__restrict__ sm,
+                                       const int *__restrict__ stop_before) {
+  normalize_gated_body(w, out, cur, n, vscale, sm, stop_before, blockIdx.x, gridDim.x);
 }
 
 // snapshot of the stop flag taken before a step's finalize
@@ -429,18 +480,17 @@ __global__ void residual_kernel(const T *__restrict__ b, T *__restrict__ t, int6
 
 // x += sum_v y[v] vinv V[v]   (x is FP64)
 template <typename TV>
-__global__ void __launch_bounds__(256) combine_kernel(const TV *__restrict__ V, int64_t ldv, int nv,
-                                                      const double *__restrict__ y,
-                                                      double *__restrict__ x, int64_t n,
-                                                      double vinv) {
+__device__ __forceinline__ void combine_body(const TV *__restrict__ V, int64_t ldv, int nv,
+                                             const double *__restrict__ y, double *__restrict__ x,
+                                             int64_t n, double vinv, int blk, int nblk) {
   constexpr int N = Vec<TV>::N;
   __shared__ double ys[kMaxRestart + 8];
   const int nvr = ((nv + kGrp - 1) / kGrp) * kGrp;
   for (int v = threadIdx.x; v < nvr; v += blockDim.x) ys[v] = v < nv ? y[v] * vinv : 0.0;
   __syncthreads();
   const int64_t nq = n / N;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t stride = (int64_t)nblk * blockDim.x;
+  const int64_t t0 = (int64_t)blk * blockDim.x + threadIdx.x;
   for (int64_t q = t0; q < nq; q += stride) {
     double s[N];
 #pragma unroll
@@ -464,6 +514,14 @@ __global__ void __launch_bounds__(256) combine_kernel(const TV *__restrict__ V, 
     for (int v = 0; v < nv; ++v) sd += ys[v] * (double)V[(int64_t)v * ldv + e];
     x[e] += sd;
   }
+}
+
+template <typename TV>
+__global__ void __launch_bounds__(256) combine_kernel(const TV *__restrict__ V, int64_t ldv, int nv,
+                                                      const double *__restrict__ y,
+                                                      double *__restrict__ x, int64_t n,
+                                                      double vinv) {
+  combine_body(V, ldv, nv, y, x, n, vinv, blockIdx.x, gridDim.x);
 }
 
 __global__ void reset_cycle_kernel(Small *sm, int R, double g0, double target) {
@@ -829,6 +887,485 @@ static int gmres_mixed(const OpT<double> &op64, const OpT<float> &op32, const Si
   return SISL_OK;
 }
 
+// ---- batched (ensemble) mixed GMRES -------------------------------------------
+//
+// M members on their own grids advance in lockstep: every pass of the solver
+// (operator, preconditioner, projections, update, normalisation, reductions,
+// small per-member bookkeeping) is one launch over all members (member =
+// blockIdx.y, or blockIdx.x for one-block kernels).  The outer refinement is
+// controlled on the device (convergence test, the Hessenberg least squares by
+// one thread per member), so the host only reads the members' done flags once
+// per outer iteration.  Precision as gmres_mixed with the FP16 basis.
+
+using BT = _Float16;
+
+struct KMem {  // one member's solver state (device table entry)
+  BT *V;
+  float *w, *cur, *t, *scr;
+  double *w64, *t64, *scr64, *x64, *b64;
+  double *partials;
+  Small *sm;
+  int *flags;
+  int64_t n, ldv;
+  double vscale, vinv;
+};
+
+enum { kSelH1 = 0, kSelNn1a = 1, kSelH2 = 2, kSelNn1b = 3, kSelScal0 = 4 };
+
+__device__ __forceinline__ double *small_slot(Small *sm, int sel) {
+  switch (sel) {
+    case kSelH1: return sm->h1;
+    case kSelNn1a: return sm->nn1a;
+    case kSelH2: return sm->h2;
+    case kSelNn1b: return sm->nn1b;
+    default: return sm->scal + (sel - kSelScal0);
+  }
+}
+
+// Arnoldi-step gating of member m at step j: stop = flags[j]; `gated`
+// additionally requires the second Gram-Schmidt pass.
+template <int G>
+__global__ void __launch_bounds__(256) kb_dots1_kernel(const KMem *__restrict__ km, int nv, int j,
+                                                      int gated) {
+  const KMem &k = km[blockIdx.y];
+  dots1_body<BT, float, G>(k.V, k.ldv, nv, k.w, k.n, k.partials, k.vinv, k.flags + j,
+                           gated ? &k.sm->gate2 : nullptr, blockIdx.x, gridDim.x);
+}
+__global__ void __launch_bounds__(256) kb_dots_kernel(const KMem *__restrict__ km, int nv, int j,
+                                                     int gated) {
+  const KMem &k = km[blockIdx.y];
+  dots_body<BT, float>(k.V, k.ldv, nv, k.w, k.n, k.partials, k.vinv, k.flags + j,
+                       gated ? &k.sm->gate2 : nullptr, blockIdx.x, gridDim.x);
+}
+__global__ void __launch_bounds__(256) kb_update_kernel(const KMem *__restrict__ km, int nv, int j,
+                                                       int sel_h, int gated) {
+  const KMem &k = km[blockIdx.y];
+  update_body<BT, float>(k.V, k.ldv, nv, small_slot(k.sm, sel_h), k.w, k.n, k.partials, k.vinv,
+                         k.flags + j, gated ? &k.sm->gate2 : nullptr, blockIdx.x, gridDim.x);
+}
+// out[v] = sum over blocks of partials[v][b]; grid (nslots, M).  mode 0:
+// Arnoldi step j (stop = flags[j]); mode 1: outer loop (skip when done).
+__global__ void __launch_bounds__(256) kb_reduce_kernel(const KMem *__restrict__ km, int j, int sel,
+                                                       int gated, int mode) {
+  const KMem &k = km[blockIdx.y];
+  if (mode == 0 && k.flags[j]) return;
+  if (mode == 1 && k.sm->done) return;
+  if (gated && !k.sm->gate2) return;
+  __shared__ double red[kBlock / 64];
+  const int v = blockIdx.x;
+  double sacc = 0;
+  for (int b = threadIdx.x; b < kRedGrid; b += blockDim.x) sacc += k.partials[(int64_t)v * kRedGrid + b];
+  sacc = wave_sum(sacc);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = sacc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0;
+    for (int q = 0; q < kBlock / 64; ++q) t += red[q];
+    small_slot(k.sm, sel)[v] = t;
+  }
+}
+__global__ void kb_snapshot_kernel(const KMem *__restrict__ km, int j) {
+  const KMem &k = km[blockIdx.x];
+  if (threadIdx.x == 0) k.flags[j] = k.sm->broken;
+}
+__global__ void kb_decide_kernel(const KMem *__restrict__ km, int nv) {
+  if (threadIdx.x == 0) decide_body(km[blockIdx.x].sm, nv);
+}
+__global__ void kb_finalize_kernel(const KMem *__restrict__ km, int j, int R) {
+  if (threadIdx.x == 0) finalize_body(km[blockIdx.x].sm, j, R, 1);
+}
+__global__ void __launch_bounds__(256) kb_normalize_kernel(const KMem *__restrict__ km, int j) {
+  const KMem &k = km[blockIdx.y];
+  normalize_gated_body<BT, float>(k.w, k.V + (int64_t)(j + 1) * k.ldv, k.cur, k.n, k.vscale, k.sm,
+                                  k.flags + j, blockIdx.x, gridDim.x);
+}
+// ||vec||^2 into scal[slot]: vec 0 = b64, 1 = w64 (outer loop, skipped when done)
+__global__ void __launch_bounds__(256) kb_norm2_kernel(const KMem *__restrict__ km, int which) {
+  const KMem &k = km[blockIdx.y];
+  if (k.sm->done) return;
+  const double *v = which == 0 ? k.b64 : k.w64;
+  dots_body<double, double>(v, k.n, 0, v, k.n, k.partials, 1.0, nullptr, nullptr, blockIdx.x, gridDim.x);
+}
+__global__ void kb_residual_kernel(const KMem *__restrict__ km) {  // t64 = b64 - t64
+  const KMem &k = km[blockIdx.y];
+  if (k.sm->done) return;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < k.n; e += stride)
+    k.t64[e] = k.b64[e] - k.t64[e];
+}
+__global__ void kb_unit_kernel(const KMem *__restrict__ km) {  // V[0] = vscale w64/|w64|, cur
+  const KMem &k = km[blockIdx.y];
+  if (k.sm->done) return;
+  const double nr = sqrt(k.sm->scal[2]);
+  const double inv = nr > 2.220446049250313e-16 ? 1.0 / nr : 0.0;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < k.n; e += stride) {
+    const double u = inv != 0.0 ? k.w64[e] * inv : 0.0;
+    k.V[e] = (BT)(u * k.vscale);
+    k.cur[e] = (float)u;
+  }
+}
+// First kernel of a solve: clear the previous solve's done flag.
+__global__ void kb_begin_kernel(const KMem *__restrict__ km) {
+  if (threadIdx.x == 0) km[blockIdx.x].sm->done = 0;
+}
+// Start of the solve: norms in scal[0], scal[1] -> ptol (jax gmres semantics).
+__global__ void kb_init_kernel(const KMem *__restrict__ km, double tol, double atol) {
+  if (threadIdx.x != 0) return;
+  Small *sm = km[blockIdx.x].sm;
+  const double b_norm = sqrt(sm->scal[0]), Mb_norm = sqrt(sm->scal[1]);
+  const double at = fmax(tol * b_norm, atol);
+  sm->b_norm = b_norm;
+  sm->Mb_norm = Mb_norm;
+  sm->ptol = Mb_norm * fmin(1.0, at / b_norm);
+  sm->outer = 0;
+  sm->steps_total = 0;
+  sm->done = 0;
+}
+// Outer iteration: residual norm in scal[2]; convergence test; new cycle.
+__global__ void kb_reset_kernel(const KMem *__restrict__ km, int R, double inner_rtol, int max_outer) {
+  Small *sm = km[blockIdx.x].sm;
+  int *flags = km[blockIdx.x].flags;
+  const int t = threadIdx.x;
+  __shared__ int fin;
+  if (t == 0) {
+    fin = sm->done;
+    if (!fin) {
+      double rn = sqrt(sm->scal[2]);
+      if (!(rn > 2.220446049250313e-16) && !isnan(rn)) rn = 0.0;
+      sm->rnorm = rn;
+      if (!(rn > sm->ptol) || sm->outer >= max_outer) fin = 1;
+      sm->done = fin;
+    }
+  }
+  __syncthreads();
+  for (int q = t; q < R * (R + 1); q += blockDim.x) {
+    const int r = q / (R + 1), c = q % (R + 1);
+    sm->H[q] = r == c ? 1.0 : 0.0;
+  }
+  for (int q = t; q <= R; q += blockDim.x) flags[q] = fin;
+  if (t == 0) {
+    sm->broken = fin;
+    sm->steps = 0;
+    sm->gate2 = 0;
+    sm->inv = 0.0;
+    for (int v = 0; v <= R; ++v) sm->g[v] = 0.0;
+    sm->g[0] = sm->rnorm;
+    sm->target = fmax(inner_rtol * sm->rnorm, 0.5 * sm->ptol);
+  }
+}
+// Least squares of the cycle (JAX _lstsq normal equations, as host_lstsq) on
+// the device, one thread per member; y into sm->y.
+__global__ void kb_lstsq_kernel(const KMem *__restrict__ km, int R) {
+  __shared__ double G[kMaxRestart * kMaxRestart], r[kMaxRestart];
+  if (threadIdx.x != 0) return;
+  Small *sm = km[blockIdx.x].sm;
+  if (sm->done) return;
+  const int s = sm->steps;
+  const double beta = sm->rnorm;
+  for (int a = 0; a < s; ++a) {
+    for (int b = 0; b < s; ++b) {
+      double acc = 0;
+      for (int l = 0; l <= s; ++l) acc += sm->H[a * (R + 1) + l] * sm->H[b * (R + 1) + l];
+      G[a * s + b] = acc;
+    }
+    r[a] = sm->H[a * (R + 1)] * beta;
+  }
+  for (int c = 0; c < s; ++c) {
+    int p = c;
+    for (int q = c + 1; q < s; ++q)
+      if (fabs(G[q * s + c]) > fabs(G[p * s + c])) p = q;
+    if (p != c) {
+      for (int q = 0; q < s; ++q) { const double tq = G[c * s + q]; G[c * s + q] = G[p * s + q]; G[p * s + q] = tq; }
+      const double tr = r[c]; r[c] = r[p]; r[p] = tr;
+    }
+    const double d = G[c * s + c];
+    for (int q = c + 1; q < s; ++q) {
+      const double f = d != 0.0 ? G[q * s + c] / d : 0.0;
+      if (f == 0.0) continue;
+      for (int k2 = c; k2 < s; ++k2) G[q * s + k2] -= f * G[c * s + k2];
+      r[q] -= f * r[c];
+    }
+  }
+  for (int c = s - 1; c >= 0; --c) {
+    double acc = r[c];
+    for (int k2 = c + 1; k2 < s; ++k2) acc -= G[c * s + k2] * sm->y[k2];
+    const double d = G[c * s + c];
+    sm->y[c] = d != 0.0 ? acc / d : 0.0;
+  }
+  sm->steps_total += s;
+  sm->outer += 1;
+}
+__global__ void __launch_bounds__(256) kb_combine_kernel(const KMem *__restrict__ km) {
+  const KMem &k = km[blockIdx.y];
+  if (k.sm->done) return;
+  combine_body<BT>(k.V, k.ldv, k.sm->steps, k.sm->y, k.x64, k.n, k.vinv, blockIdx.x, gridDim.x);
+}
+// Copy between the caller's vectors and the members' workspace copies.
+struct PtrTable {
+  const double *src[kMaxMembers];
+  double *dst[kMaxMembers];
+  int64_t n[kMaxMembers];
+};
+__global__ void kb_copy_kernel(PtrTable t) {
+  const int m = blockIdx.y;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < t.n[m]; e += stride)
+    t.dst[m][e] = t.src[m][e];
+}
+
+// Host-side plan of one batched workspace (set up once per operator set).
+struct EnsPlan {
+  int M, R, L, S;
+  int maxA, maxAT1, maxAT2, maxM, maxCols, maxFaces;
+  int64_t maxN;
+  std::vector<int64_t> n;
+  KMem *km;                  // device table
+  BOp<float> *A32, *M32;     // inner: cur -> w (stop flags[j]); in-place w
+  BOp<float> *AT32, *MT32;   // inner transpose: cur -> t; t -> w
+  BOp<double> *A64, *M64;    // outer: x64 -> t64 (skip when done); t64 -> w64
+  BOp<double> *AT64, *MT64;
+  BOp<double> *Mb64, *MTb64; // start: b64 -> w64 (t64 for the transpose)
+  std::vector<Small *> sm;
+};
+static std::mutex g_ens_mu;
+static std::map<const void *, EnsPlan> g_ens;
+
+static int64_t ens_head_bytes(int M) {
+  return align256(sizeof(KMem) * M) + 10 * align256(sizeof(BOp<double>) * M);
+}
+static int64_t ens_member_bytes(const Stag &s, int R) {
+  const Layout L = layout(s, R, true);
+  return align256(L.total) + 2 * align256(sizeof(double) * s.n);
+}
+
+static int ens_setup(const SislOperator *ops64, const SislOperator *ops32, int M, int R, char *ws,
+                     hipStream_t st) {
+  EnsPlan P;
+  P.M = M;
+  P.R = R;
+  P.L = 0;
+  P.maxA = P.maxAT1 = P.maxAT2 = P.maxM = P.maxCols = P.maxFaces = 1;
+  P.maxN = 1;
+  std::vector<KMem> km(M);
+  std::vector<BOp<float>> a32(M), m32(M), at32(M), mt32(M);
+  std::vector<BOp<double>> a64(M), m64(M), at64(M), mt64(M), mb64(M), mtb64(M);
+  int64_t off = ens_head_bytes(M);
+  int S = 1;
+  for (int m = 0; m < M; ++m) {
+    const OpT<double> o64 = OpT<double>::make(ops64[m]);
+    const OpT<float> o32 = OpT<float>::make(ops32[m]);
+    if (!o64.complete() || !o32.complete()) return SISL_ERR_ARG;
+    const Stag &s = o64.s;
+    if ((uint64_t)s.n * sizeof(double) >= (1ull << 32)) return SISL_ERR_ARG;
+    const int Lm = seg_levels(s.nz);
+    if (!Lm) return SISL_ERR_ARG;
+    P.L = std::max(P.L, Lm);
+    const Layout Lay = layout(s, R, true);
+    char *base = ws + off;
+    KMem &k = km[m];
+    k.V = (BT *)(base + Lay.V32);
+    k.w = (float *)(base + Lay.w32);
+    k.cur = (float *)(base + Lay.cur32);
+    k.t = (float *)(base + Lay.t32);
+    k.scr = (float *)(base + Lay.scr32);
+    k.w64 = (double *)(base + Lay.w);
+    k.t64 = (double *)(base + Lay.t);
+    k.scr64 = (double *)(base + Lay.scr);
+    k.partials = (double *)(base + Lay.partials);
+    k.sm = (Small *)(base + Lay.small);
+    k.flags = (int *)(base + Lay.flags);
+    k.x64 = (double *)(base + align256(Lay.total));
+    k.b64 = (double *)(base + align256(Lay.total) + align256(sizeof(double) * s.n));
+    k.n = s.n;
+    k.ldv = basis_ld(s.n);
+    k.vscale = basis_scale<BT>(s.n);
+    k.vinv = 1.0 / k.vscale;
+    P.sm.push_back(k.sm);
+    P.n.push_back(s.n);
+    a32[m] = BOp<float>{o32, k.cur, 0, k.w, k.scr, k.flags, 1};
+    m32[m] = BOp<float>{o32, k.w, 0, k.w, k.scr, k.flags, 1};
+    at32[m] = BOp<float>{o32, k.cur, 0, k.t, k.scr, k.flags, 1};
+    mt32[m] = BOp<float>{o32, k.t, 0, k.w, k.scr, k.flags, 1};
+    a64[m] = BOp<double>{o64, k.x64, 0, k.t64, k.scr64, &k.sm->done, 0};
+    m64[m] = BOp<double>{o64, k.t64, 0, k.w64, k.scr64, &k.sm->done, 0};
+    at64[m] = BOp<double>{o64, k.x64, 0, k.t64, k.scr64, &k.sm->done, 0};
+    mt64[m] = BOp<double>{o64, k.t64, 0, k.w64, k.scr64, &k.sm->done, 0};
+    mb64[m] = BOp<double>{o64, k.b64, 0, k.w64, k.scr64, nullptr, 0};
+    mtb64[m] = BOp<double>{o64, k.b64, 0, k.w64, k.scr64, nullptr, 0};
+    P.maxA = std::max(P.maxA, blocks_A(s));
+    P.maxAT1 = std::max(P.maxAT1, blocks_AT1(s));
+    P.maxAT2 = std::max(P.maxAT2, blocks_AT2(s));
+    P.maxM = std::max(P.maxM, blocks_M(s));
+    P.maxCols = std::max(P.maxCols, s.nx * s.ny);
+    P.maxFaces = std::max<int>(P.maxFaces, (int)s.ow);
+    P.maxN = std::max(P.maxN, s.n);
+    off += ens_member_bytes(s, R);
+  }
+  for (int m = 0; m < M; ++m) S = std::max(S, (OpT<double>::make(ops64[m]).s.nz + P.L - 1) / P.L);
+  P.S = S;
+  // tables at the workspace head
+  char *h = ws;
+  P.km = (KMem *)h;
+  h += align256(sizeof(KMem) * M);
+  BOp<float> **f32[] = {&P.A32, &P.M32, &P.AT32, &P.MT32};
+  std::vector<BOp<float>> *v32[] = {&a32, &m32, &at32, &mt32};
+  for (int q = 0; q < 4; ++q) {
+    *f32[q] = (BOp<float> *)h;
+    h += align256(sizeof(BOp<double>) * M);
+  }
+  BOp<double> **f64[] = {&P.A64, &P.M64, &P.AT64, &P.MT64, &P.Mb64, &P.MTb64};
+  std::vector<BOp<double>> *v64[] = {&a64, &m64, &at64, &mt64, &mb64, &mtb64};
+  for (int q = 0; q < 6; ++q) {
+    *f64[q] = (BOp<double> *)h;
+    h += align256(sizeof(BOp<double>) * M);
+  }
+  if (hipMemcpyAsync(P.km, km.data(), sizeof(KMem) * M, hipMemcpyHostToDevice, st) != hipSuccess)
+    return SISL_ERR_LAUNCH;
+  for (int q = 0; q < 4; ++q)
+    if (hipMemcpyAsync(*f32[q], v32[q]->data(), sizeof(BOp<float>) * M, hipMemcpyHostToDevice, st) != hipSuccess)
+      return SISL_ERR_LAUNCH;
+  for (int q = 0; q < 6; ++q)
+    if (hipMemcpyAsync(*f64[q], v64[q]->data(), sizeof(BOp<double>) * M, hipMemcpyHostToDevice, st) != hipSuccess)
+      return SISL_ERR_LAUNCH;
+  if (hipStreamSynchronize(st) != hipSuccess) return SISL_ERR_LAUNCH;
+  std::lock_guard<std::mutex> g(g_ens_mu);
+  g_ens[ws] = std::move(P);
+  return SISL_OK;
+}
+
+static void kb_dots_dispatch(const EnsPlan &P, int nv, int j, int gated, hipStream_t st) {
+  const dim3 gr(kRedGrid, P.M), bl(kBlock);
+  if (nv <= kGrp) hipLaunchKernelGGL(kb_dots1_kernel<1>, gr, bl, 0, st, P.km, nv, j, gated);
+  else if (nv <= 2 * kGrp) hipLaunchKernelGGL(kb_dots1_kernel<2>, gr, bl, 0, st, P.km, nv, j, gated);
+  else if (nv <= 3 * kGrp) hipLaunchKernelGGL(kb_dots1_kernel<3>, gr, bl, 0, st, P.km, nv, j, gated);
+  else if (nv <= 4 * kGrp) hipLaunchKernelGGL(kb_dots1_kernel<4>, gr, bl, 0, st, P.km, nv, j, gated);
+  else hipLaunchKernelGGL(kb_dots_kernel, gr, bl, 0, st, P.km, nv, j, gated);
+}
+
+static int ens_arnoldi(const EnsPlan &P, int j, int transpose, hipStream_t st) {
+  int rc;
+  const int M = P.M, R = P.R, nv = j + 1;
+  hipLaunchKernelGGL(kb_snapshot_kernel, dim3(M), dim3(64), 0, st, (const KMem *)P.km, j);
+  if (!transpose) {
+    if ((rc = launch_A_batched<float>(P.A32, M, P.maxA, j, st))) return rc;
+    if ((rc = launch_M_batched<float>(P.M32, M, P.maxM, P.L, P.S, j, st))) return rc;
+  } else {
+    if ((rc = launch_AT_batched<float>(P.AT32, M, P.maxAT1, P.maxAT2, j, st))) return rc;
+    if ((rc = launch_MT_batched<float>(P.MT32, M, P.maxCols, P.maxFaces, j, st))) return rc;
+  }
+  kb_dots_dispatch(P, nv, j, 0, st);
+  hipLaunchKernelGGL(kb_reduce_kernel, dim3(nv + 1, M), dim3(kBlock), 0, st, (const KMem *)P.km, j, kSelH1, 0, 0);
+  hipLaunchKernelGGL(kb_update_kernel, dim3(kRedGrid, M), dim3(kBlock), 0, st, (const KMem *)P.km, nv, j, kSelH1, 0);
+  hipLaunchKernelGGL(kb_reduce_kernel, dim3(1, M), dim3(kBlock), 0, st, (const KMem *)P.km, j, kSelNn1a, 0, 0);
+  hipLaunchKernelGGL(kb_decide_kernel, dim3(M), dim3(64), 0, st, (const KMem *)P.km, nv);
+  kb_dots_dispatch(P, nv, j, 1, st);
+  hipLaunchKernelGGL(kb_reduce_kernel, dim3(nv, M), dim3(kBlock), 0, st, (const KMem *)P.km, j, kSelH2, 1, 0);
+  hipLaunchKernelGGL(kb_update_kernel, dim3(kRedGrid, M), dim3(kBlock), 0, st, (const KMem *)P.km, nv, j, kSelH2, 1);
+  hipLaunchKernelGGL(kb_reduce_kernel, dim3(1, M), dim3(kBlock), 0, st, (const KMem *)P.km, j, kSelNn1b, 1, 0);
+  hipLaunchKernelGGL(kb_finalize_kernel, dim3(M), dim3(64), 0, st, (const KMem *)P.km, j, R);
+  hipLaunchKernelGGL(kb_normalize_kernel, dim3(kRedGrid, M), dim3(kBlock), 0, st, (const KMem *)P.km, j);
+  SISL_CHECK_LAUNCH();
+  return SISL_OK;
+}
+
+static int ens_solve(const void *ws, const SislGmresParams &p, const double *const *b, double *const *x,
+                     SislGmresStats *stats, hipStream_t st) {
+  EnsPlan P;
+  {
+    std::lock_guard<std::mutex> g(g_ens_mu);
+    auto it = g_ens.find(ws);
+    if (it == g_ens.end()) return SISL_ERR_ARG;
+    P = it->second;
+  }
+  if (p.restart != P.R) return SISL_ERR_ARG;
+  const int M = P.M, R = P.R;
+  int rc;
+  PtrTable tin{}, tout{};
+  std::vector<KMem> km(M);
+  if (hipMemcpyAsync(km.data(), P.km, sizeof(KMem) * M, hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
+    return SISL_ERR_LAUNCH;
+  for (int m = 0; m < M; ++m) {
+    if (!b[m] || !x[m]) return SISL_ERR_ARG;
+    tin.src[m] = b[m];
+    tin.dst[m] = km[m].b64;
+    tin.n[m] = km[m].n;
+    tout.src[m] = x[m];
+    tout.dst[m] = km[m].x64;
+    tout.n[m] = km[m].n;
+  }
+  const dim3 cg(grid_for(P.maxN), M);
+  hipLaunchKernelGGL(kb_copy_kernel, cg, dim3(kBlock), 0, st, tin);
+  hipLaunchKernelGGL(kb_copy_kernel, cg, dim3(kBlock), 0, st, tout);
+  // ||b||, ||M b||, ptol
+  hipLaunchKernelGGL(kb_begin_kernel, dim3(M), dim3(64), 0, st, (const KMem *)P.km);
+  hipLaunchKernelGGL(kb_norm2_kernel, dim3(kRedGrid, M), dim3(kBlock), 0, st, (const KMem *)P.km, 0);
+  hipLaunchKernelGGL(kb_reduce_kernel, dim3(1, M), dim3(kBlock), 0, st, (const KMem *)P.km, 0, kSelScal0 + 0, 0, 1);
+  if (!p.transpose) {
+    if ((rc = launch_M_batched<double>(P.Mb64, M, P.maxM, P.L, P.S, 0, st))) return rc;
+  } else {
+    if ((rc = launch_MT_batched<double>(P.MTb64, M, P.maxCols, P.maxFaces, 0, st))) return rc;
+  }
+  hipLaunchKernelGGL(kb_norm2_kernel, dim3(kRedGrid, M), dim3(kBlock), 0, st, (const KMem *)P.km, 1);
+  hipLaunchKernelGGL(kb_reduce_kernel, dim3(1, M), dim3(kBlock), 0, st, (const KMem *)P.km, 0, kSelScal0 + 1, 0, 1);
+  hipLaunchKernelGGL(kb_init_kernel, dim3(M), dim3(64), 0, st, (const KMem *)P.km, p.tol, p.atol);
+  const int max_outer = p.max_outer > 0 ? p.max_outer : p.maxiter;
+  std::vector<int> done(M);
+  for (int outer = 0;; ++outer) {
+    // residual M (b - A x) in FP64, its norm, the first basis vector
+    if (!p.transpose) {
+      if ((rc = launch_A_batched<double>(P.A64, M, P.maxA, 0, st))) return rc;
+    } else {
+      if ((rc = launch_AT_batched<double>(P.AT64, M, P.maxAT1, P.maxAT2, 0, st))) return rc;
+    }
+    hipLaunchKernelGGL(kb_residual_kernel, dim3(kRedGrid, M), dim3(kBlock), 0, st, (const KMem *)P.km);
+    if (!p.transpose) {
+      if ((rc = launch_M_batched<double>(P.M64, M, P.maxM, P.L, P.S, 0, st))) return rc;
+    } else {
+      if ((rc = launch_MT_batched<double>(P.MT64, M, P.maxCols, P.maxFaces, 0, st))) return rc;
+    }
+    hipLaunchKernelGGL(kb_norm2_kernel, dim3(kRedGrid, M), dim3(kBlock), 0, st, (const KMem *)P.km, 1);
+    hipLaunchKernelGGL(kb_reduce_kernel, dim3(1, M), dim3(kBlock), 0, st, (const KMem *)P.km, 0, kSelScal0 + 2, 0, 1);
+    hipLaunchKernelGGL(kb_unit_kernel, dim3(kRedGrid, M), dim3(kBlock), 0, st, (const KMem *)P.km);
+    hipLaunchKernelGGL(kb_reset_kernel, dim3(M), dim3(kBlock), 0, st, (const KMem *)P.km, R, p.inner_rtol,
+                       max_outer);
+    // one read of the members' done flags per outer iteration
+    bool all = true;
+    for (int m = 0; m < M; ++m)
+      if (hipMemcpyAsync(&done[m], &P.sm[m]->done, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess)
+        return SISL_ERR_LAUNCH;
+    if (hipStreamSynchronize(st) != hipSuccess) return SISL_ERR_LAUNCH;
+    for (int m = 0; m < M; ++m) all = all && done[m];
+    if (all || outer > max_outer) break;
+    for (int j = 0; j < R; ++j)
+      if ((rc = ens_arnoldi(P, j, p.transpose, st))) return rc;
+    hipLaunchKernelGGL(kb_lstsq_kernel, dim3(M), dim3(64), 0, st, (const KMem *)P.km, R);
+    hipLaunchKernelGGL(kb_combine_kernel, dim3(kRedGrid, M), dim3(kBlock), 0, st, (const KMem *)P.km);
+  }
+  for (int m = 0; m < M; ++m) {
+    tout.src[m] = km[m].x64;
+    tout.dst[m] = x[m];
+  }
+  hipLaunchKernelGGL(kb_copy_kernel, cg, dim3(kBlock), 0, st, tout);
+  SISL_CHECK_LAUNCH();
+  if (stats) {
+    for (int m = 0; m < M; ++m) {
+      Small hs;
+      if (hipMemcpyAsync(&hs.done, &P.sm[m]->done, sizeof(int) * 4 + sizeof(double) * 4,
+                         hipMemcpyDeviceToHost, st) != hipSuccess ||
+          hipStreamSynchronize(st) != hipSuccess)
+        return SISL_ERR_LAUNCH;
+      stats[m].cycles = hs.outer;
+      stats[m].arnoldi_steps = hs.steps_total;
+      stats[m].residual = hs.rnorm;
+      stats[m].ptol = hs.ptol;
+      stats[m].b_norm = hs.b_norm;
+    }
+  }
+  return SISL_OK;
+}
+
 }  // namespace sisl
 
 extern "C" {
@@ -863,6 +1400,27 @@ int sisl_gmres_mixed(const SislOperator *op64, const SislOperator *op32, const S
   if (p->basis_fp16)
     return sisl::gmres_mixed<_Float16>(o64, o32, *p, b, x, (char *)ws, st, sisl::as_stream(stream));
   return sisl::gmres_mixed<float>(o64, o32, *p, b, x, (char *)ws, st, sisl::as_stream(stream));
+}
+
+int64_t sisl_gmres_batched_workspace_bytes(const SislGrid *grids, int32_t M, int32_t restart) {
+  if (!grids || M < 1 || M > sisl::kMaxMembers || restart < 1 || restart > sisl::kMaxRestart) return -1;
+  int64_t t = sisl::ens_head_bytes(M);
+  for (int m = 0; m < M; ++m) t += sisl::ens_member_bytes(sisl::Stag::make(grids[m]), restart);
+  return t;
+}
+
+int sisl_gmres_batched_setup(const SislOperator *ops64, const SislOperator *ops32, int32_t M,
+                             int32_t restart, void *ws, void *stream) {
+  if (!ops64 || !ops32 || !ws || M < 1 || M > sisl::kMaxMembers || restart < 1 ||
+      restart > sisl::kMaxRestart)
+    return SISL_ERR_ARG;
+  return sisl::ens_setup(ops64, ops32, M, restart, (char *)ws, sisl::as_stream(stream));
+}
+
+int sisl_gmres_batched(int32_t M, const SislGmresParams *p, const double *const *b, double *const *x,
+                       void *ws, SislGmresStats *stats, void *stream) {
+  if (!p || !b || !x || !ws || M < 1 || M > sisl::kMaxMembers) return SISL_ERR_ARG;
+  return sisl::ens_solve(ws, *p, b, x, stats, sisl::as_stream(stream));
 }
 
 int sisl_dot_f64(const double *x, const double *y, int64_t n, double *out, double *partials,

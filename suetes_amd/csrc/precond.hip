@@ -19,16 +19,16 @@ namespace sisl {
 // input element is read once and every output element written once.
 constexpr int kMaxSeg = 8;  // at most 8 segments (512 threads): columns up to 256 levels
 
+// Body for launch block `blk` of `nblk`; blockDim.y may exceed the segments
+// the column needs (the extra waves carry identity compositions).
 template <typename T, int L>
-__global__ void __launch_bounds__(512) precond_seg_kernel(OpT<T> o, const T *__restrict__ r,
-                                                           T *__restrict__ z,
-                                                           const int *__restrict__ stop) {
-  if (stop && *stop) return;  // uniform over the launch
+__device__ __forceinline__ void precond_seg_body(const OpT<T> &o, const T *__restrict__ r,
+                                                 T *__restrict__ z, int blk, int nblk) {
   __shared__ T la[kMaxSeg][64], lb[kMaxSeg][64];
   const Stag &s = o.s;
   const int nx = s.nx, ny = s.ny, nz = s.nz;
   const int nbx = (nx + 63) / 64;
-  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int tile = xcd_tile(blk, nblk);
   const int bx = tile % nbx, j = tile / nbx;
   const int tx = threadIdx.x, sg = threadIdx.y, S = blockDim.y;
   const int i = bx * 64 + tx;
@@ -157,18 +157,22 @@ __global__ void __launch_bounds__(512) precond_seg_kernel(OpT<T> o, const T *__r
   }
 }
 
+template <typename T, int L>
+__global__ void __launch_bounds__(512) precond_seg_kernel(OpT<T> o, const T *__restrict__ r,
+                                                           T *__restrict__ z,
+                                                           const int *__restrict__ stop) {
+  if (stop && *stop) return;  // uniform over the launch
+  precond_seg_body<T, L>(o, r, z, blockIdx.x, gridDim.x);
+}
+
 // Transpose: column sweeps of the reversed chain; the u/v cotangents of the
 // kinematic term leave through mass-point scratch and are gathered to faces.
 template <typename T>
-__global__ void __launch_bounds__(256) precond_T_kernel(OpT<T> o, const T *__restrict__ zb,
-                                                        T *__restrict__ rb,
-                                                        T *__restrict__ umb,
-                                                        T *__restrict__ vmb,
-                                                        const int *__restrict__ stop) {
-  if (stop && *stop) return;
+__device__ __forceinline__ void precond_T_body(const OpT<T> &o, const T *__restrict__ zb,
+                                               T *__restrict__ rb, T *__restrict__ umb,
+                                               T *__restrict__ vmb, int64_t col) {
   const Stag &s = o.s;
   const int nx = s.nx, ny = s.ny, nz = s.nz;
-  const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (col >= (int64_t)nx * ny) return;
   const int i = (int)(col % nx), j = (int)(col / nx);
   const T *ZW = zb + s.ow, *ZP = zb + s.om, *ZE = zb + s.oe;
@@ -226,14 +230,21 @@ __global__ void __launch_bounds__(256) precond_T_kernel(OpT<T> o, const T *__res
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) precond_T_faces_kernel(OpT<T> o, const T *__restrict__ zb,
-                                                              const T *__restrict__ umb,
-                                                              const T *__restrict__ vmb,
-                                                              T *__restrict__ rb,
-                                                              const int *__restrict__ stop) {
+__global__ void __launch_bounds__(256) precond_T_kernel(OpT<T> o, const T *__restrict__ zb,
+                                                        T *__restrict__ rb,
+                                                        T *__restrict__ umb,
+                                                        T *__restrict__ vmb,
+                                                        const int *__restrict__ stop) {
   if (stop && *stop) return;
+  precond_T_body(o, zb, rb, umb, vmb, (int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+template <typename T>
+__device__ __forceinline__ void precond_T_faces_body(const OpT<T> &o, const T *__restrict__ zb,
+                                                     const T *__restrict__ umb,
+                                                     const T *__restrict__ vmb,
+                                                     T *__restrict__ rb, int64_t q) {
   const Stag &s = o.s;
-  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= s.ow) return;
   const int nx = s.nx, ny = s.ny;
   T out = zb[q];
@@ -253,6 +264,16 @@ __global__ void __launch_bounds__(256) precond_T_faces_kernel(OpT<T> o, const T 
     out += (T)0.5 * acc;
   }
   rb[q] = out;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) precond_T_faces_kernel(OpT<T> o, const T *__restrict__ zb,
+                                                              const T *__restrict__ umb,
+                                                              const T *__restrict__ vmb,
+                                                              T *__restrict__ rb,
+                                                              const int *__restrict__ stop) {
+  if (stop && *stop) return;
+  precond_T_faces_body(o, zb, umb, vmb, rb, (int64_t)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // Levels per segment: the smallest supported L with at most kMaxSeg
@@ -300,6 +321,70 @@ int launch_MT(const OpT<T> &o, const T *zb, T *rb, T *scratch, hipStream_t st, c
   SISL_CHECK_LAUNCH();
   return SISL_OK;
 }
+
+// ---- batched (ensemble) launches -------------------------------------------
+
+int blocks_M(const Stag &s) { return ((s.nx + 63) / 64) * s.ny; }
+int seg_levels(int nz) {
+  static const int sizes[] = {2, 4, 8, 10, 12, 16, 24, 32};
+  for (int c : sizes)
+    if ((nz + c - 1) / c <= kMaxSeg) return c;
+  return 0;
+}
+
+template <typename T, int L>
+__global__ void __launch_bounds__(512) precond_seg_batched_kernel(const BOp<T> *__restrict__ tab, int j) {
+  const BOp<T> &b = tab[blockIdx.y];
+  if (b.stop && b.stop[j * b.sj]) return;
+  const int nb = ((b.o.s.nx + 63) / 64) * b.o.s.ny;
+  if ((int)blockIdx.x >= nb) return;
+  precond_seg_body<T, L>(b.o, b.x + j * b.xj, b.y, blockIdx.x, nb);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) precond_T_batched_kernel(const BOp<T> *__restrict__ tab, int j) {
+  const BOp<T> &b = tab[blockIdx.y];
+  if (b.stop && b.stop[j * b.sj]) return;
+  const int64_t nm = b.o.s.m.n();
+  precond_T_body(b.o, b.x + j * b.xj, b.y, b.scr, b.scr + nm,
+                 (int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) precond_T_faces_batched_kernel(const BOp<T> *__restrict__ tab, int j) {
+  const BOp<T> &b = tab[blockIdx.y];
+  if (b.stop && b.stop[j * b.sj]) return;
+  const int64_t nm = b.o.s.m.n();
+  precond_T_faces_body(b.o, b.x + j * b.xj, (const T *)b.scr, (const T *)(b.scr + nm), b.y,
+                       (int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+template <typename T>
+int launch_M_batched(const BOp<T> *tab, int M, int maxb, int L, int S, int j, hipStream_t st) {
+  const dim3 grid(maxb, M), block(64, S);
+  switch (L) {
+#define SISL_MB_CASE(c) \
+  case c: hipLaunchKernelGGL((precond_seg_batched_kernel<T, c>), grid, block, 0, st, tab, j); break;
+    SISL_MB_CASE(2) SISL_MB_CASE(4) SISL_MB_CASE(8) SISL_MB_CASE(10) SISL_MB_CASE(12) SISL_MB_CASE(16)
+    SISL_MB_CASE(24) SISL_MB_CASE(32)
+#undef SISL_MB_CASE
+    default: return SISL_ERR_ARG;
+  }
+  SISL_CHECK_LAUNCH();
+  return SISL_OK;
+}
+template <typename T>
+int launch_MT_batched(const BOp<T> *tab, int M, int maxcols, int maxfaces, int j, hipStream_t st) {
+  hipLaunchKernelGGL(precond_T_batched_kernel<T>, dim3(grid_for(maxcols), M), dim3(kBlock), 0, st, tab, j);
+  hipLaunchKernelGGL(precond_T_faces_batched_kernel<T>, dim3(grid_for(maxfaces), M), dim3(kBlock), 0, st,
+                     tab, j);
+  SISL_CHECK_LAUNCH();
+  return SISL_OK;
+}
+template int launch_M_batched<double>(const BOp<double> *, int, int, int, int, int, hipStream_t);
+template int launch_M_batched<float>(const BOp<float> *, int, int, int, int, int, hipStream_t);
+template int launch_MT_batched<double>(const BOp<double> *, int, int, int, int, hipStream_t);
+template int launch_MT_batched<float>(const BOp<float> *, int, int, int, int, hipStream_t);
 
 template int launch_M<double>(const OpT<double> &, const double *, double *, hipStream_t, const int *);
 template int launch_M<float>(const OpT<float> &, const float *, float *, hipStream_t, const int *);

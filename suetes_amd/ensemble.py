@@ -1,0 +1,377 @@
+"""Batched ensemble of SISL steppers on uneven domains.
+
+An ensemble advances every member one step in lockstep.  Each member runs
+its own unchanged ``SISLStepper3D.step_internal`` in a worker thread; the
+three hot operations of the step -- departure points (4 per step), tricubic
+gathers (6 per step) and implicit solves (2 per step) -- meet at a
+rendezvous, where the last member to arrive issues ONE batched launch for
+all members (``sisl_departure_batched_*``, ``sisl_tricubic_batched_*``,
+``sisl_gmres_batched``).  The per-member glue (tendencies, FFSL, EOS,
+boundaries) stays torch ops on device tensors.
+
+The batched tricubic and the batched solve are single autograd Functions over
+all members, so the reverse mode issues batched transposed solves as well.
+``integrate(..., checkpoint=True)`` checkpoints whole ensemble steps (a
+per-member checkpoint could not be recomputed in lockstep).
+
+Members must use ``precision='mixed'`` (the batched solver is the mixed
+FP32/FP16/FP64 one) and share ``dt``; at most 32 members.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import threading
+
+import torch
+
+from . import _lib
+from .operators import Coords, _Tricubic
+from .steppers import KEYS, LOC_ID, _operator_inputs, pack, state_to_internal, state_to_reference, unpack
+
+MAX_MEMBERS = 32
+
+
+class _Rendezvous:
+    """Calls number c of the members (one per member) form a batch: the last
+    member to arrive runs the batched function and hands each member its
+    result.  A failure in any member breaks the rendezvous for all."""
+
+    def __init__(self, M, timeout=600.0):
+        self.M = M
+        self.cv = threading.Condition()
+        self.count = [0] * M
+        self.pending = {}
+        self.results = {}
+        self.broken = None
+        self.timeout = timeout
+
+    def __call__(self, m, kind, payload, fn):
+        with self.cv:
+            if self.broken is not None:
+                raise RuntimeError("ensemble step aborted") from self.broken
+            c = self.count[m]
+            self.count[m] += 1
+            slot = self.pending.setdefault(c, {"kind": kind, "items": {}})
+            if slot["kind"] != kind:
+                self.broken = RuntimeError(f"members diverged: {slot['kind']} vs {kind}")
+                self.cv.notify_all()
+                raise self.broken
+            slot["items"][m] = payload
+            if len(slot["items"]) == self.M:
+                items = [slot["items"][i] for i in range(self.M)]
+                del self.pending[c]
+                try:
+                    res = fn(items)
+                except BaseException as e:  # noqa: BLE001 -- propagated to every member
+                    self.broken = e
+                    self.cv.notify_all()
+                    raise
+                self.results[c] = [res, self.M]
+                self.cv.notify_all()
+            else:
+                while c not in self.results:
+                    if self.broken is not None:
+                        raise RuntimeError("ensemble step aborted") from self.broken
+                    if not self.cv.wait(self.timeout):
+                        self.broken = TimeoutError("ensemble rendezvous timed out")
+                        self.cv.notify_all()
+                        raise self.broken
+            entry = self.results[c]
+            entry[1] -= 1
+            if entry[1] == 0:
+                del self.results[c]
+            return entry[0][m]
+
+    def fail(self, exc):
+        with self.cv:
+            if self.broken is None:
+                self.broken = exc
+            self.cv.notify_all()
+
+
+class _BatchedTricubic(torch.autograd.Function):
+    """Tricubic gathers of M fields in one launch; the adjoint is each
+    member's deterministic transpose."""
+
+    @staticmethod
+    def forward(ctx, lim, coords_list, *fields):
+        ctx.coords = coords_list
+        ctx.lim = lim
+        fields = [f.contiguous() for f in fields]
+        outs = [torch.empty_like(f) for f in fields]
+        M = len(fields)
+        dims = (C.c_int32 * (3 * M))()
+        for m, f in enumerate(fields):
+            _lib.require_device(f, coords_list[m].data)
+            fz, fy, fx = f.shape
+            dims[3 * m], dims[3 * m + 1], dims[3 * m + 2] = fx, fy, fz
+        L = _lib.lib()
+        fn = L.sisl_tricubic_batched_f64 if fields[0].dtype == torch.float64 else L.sisl_tricubic_batched_f32
+        _lib.check(fn(M, dims, _lib.ptr_array(fields), _lib.ptr_array([c.data for c in coords_list]),
+                      _lib.ptr_array(outs), int(bool(lim)), _lib.stream_ptr(fields[0].device)),
+                   "batched tricubic")
+        return tuple(outs)
+
+    @staticmethod
+    def backward(ctx, *grads):
+        if ctx.lim:
+            raise NotImplementedError("the adjoint of the limited tricubic is not implemented")
+        outs = []
+        for g, co in zip(grads, ctx.coords):
+            outs.append(None if g is None else _Tricubic.backward(_Ctx(co, False), g)[0])
+        return (None, None) + tuple(outs)
+
+
+class _Ctx:
+    def __init__(self, coords, lim):
+        self.coords, self.use_limiter = coords, lim
+
+
+class _BatchedSolve(torch.autograd.Function):
+    """x_m = A_m^{-1} b_m for every member with one batched mixed GMRES; the
+    cotangents are one batched transposed solve (x0 receives none)."""
+
+    @staticmethod
+    def forward(ctx, ens, *bx):
+        M = len(bx) // 2
+        ctx.ens = ens
+        return tuple(ens.solve_flat(list(bx[:M]), list(bx[M:]), transpose=False))
+
+    @staticmethod
+    def backward(ctx, *gx):
+        ens = ctx.ens
+        gx = [g.contiguous() if g is not None else None for g in gx]
+        ref = [g for g in gx if g is not None][0]
+        gx = [g if g is not None else torch.zeros_like(ref) for g in gx]
+        lam = ens.solve_flat(gx, [torch.zeros_like(g) for g in gx], transpose=True)
+        return (None,) + tuple(lam) + (None,) * len(gx)
+
+
+class Ensemble:
+    """Lockstep batched stepping of ``steppers`` (one ``SISLStepper3D`` per
+    member, each with its own grid)."""
+
+    def __init__(self, steppers):
+        if not 1 <= len(steppers) <= MAX_MEMBERS:
+            raise ValueError(f"an ensemble holds 1..{MAX_MEMBERS} members")
+        if any(s.precision != "mixed" for s in steppers):
+            raise ValueError("ensemble members must use precision='mixed'")
+        if len({float(s.dt) for s in steppers}) != 1:
+            raise ValueError("ensemble members must share dt")
+        self.steppers = list(steppers)
+        self.M = len(steppers)
+        self.dt = steppers[0].dt
+        self._workspace = None
+        self._ws_key = None
+        self.last_stats = []
+        self._rdv = None
+
+    # -- batched solve -----------------------------------------------------------
+    def _prepare(self, bgs):
+        key = tuple(id(b) for b in bgs)
+        if self._ws_key == key:
+            return
+        sols = [s.implicit_solver for s in self.steppers]
+        for sol, bg in zip(sols, bgs):
+            sol.prepare(bg)
+        R = sols[0]._restart()
+        if any(s._restart() != R for s in sols):
+            raise ValueError("ensemble members must share the (inner) restart length")
+        L = _lib.lib()
+        grids = (_lib.SislGrid * self.M)(*[s.op64.struct.g for s in sols])
+        nbytes = int(L.sisl_gmres_batched_workspace_bytes(grids, self.M, R))
+        if nbytes <= 0:
+            raise ValueError("batched solver: unsupported member grids")
+        dev = sols[0].op64.tensors["cua"].device
+        self._workspace = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+        ops64 = (_lib.SislOperator * self.M)(*[s.op64.struct for s in sols])
+        ops32 = (_lib.SislOperator * self.M)(*[s.op32.struct for s in sols])
+        _lib.check(L.sisl_gmres_batched_setup(ops64, ops32, self.M, R, self._workspace.data_ptr(),
+                                              _lib.stream_ptr(dev)), "batched solver setup")
+        self._ws_key = key
+        self._R = R
+
+    def solve_flat(self, bs, x0s, transpose=False):
+        sols = [s.implicit_solver for s in self.steppers]
+        xs = [x.detach().clone().contiguous() for x in x0s]
+        bs = [b.detach().contiguous() for b in bs]
+        p = sols[0]._params(transpose)
+        st = (_lib.SislGmresStats * self.M)()
+        L = _lib.lib()
+        _lib.check(L.sisl_gmres_batched(self.M, C.byref(p), _lib.ptr_array(bs), _lib.ptr_array(xs),
+                                        self._workspace.data_ptr(), st, _lib.stream_ptr(bs[0].device)),
+                   "batched GMRES")
+        self.last_stats.append([{"transpose": bool(transpose), "cycles": s.cycles,
+                                 "arnoldi_steps": s.arnoldi_steps, "residual": s.residual, "ptol": s.ptol}
+                                for s in st])
+        return xs
+
+    def _batched_solve(self, items):
+        bgs = [it[1] for it in items]
+        self._prepare(bgs)
+        bs, x0s, tmpl = [], [], []
+        for m, (rhs_prime, bg, x0) in enumerate(items):
+            sol = self.steppers[m].implicit_solver
+            if torch.is_grad_enabled() and _operator_inputs(sol):
+                raise NotImplementedError("ensemble solves do not propagate operator-input cotangents")
+            dzw = bg["dz_w_full"]
+            rs = {k: (rhs_prime[k] * sol.pi_scale if k == "pi" else rhs_prime[k]) for k in KEYS}
+            if x0 is not None:
+                x0s_ = {"u": x0["u"], "v": x0["v"], "w": x0["w"], "pi": x0["pi"] * sol.pi_scale,
+                        "eta_dot": x0["eta_dot"] * dzw}
+            else:
+                x0s_ = rs
+            bs.append(pack(rs))
+            x0s.append(pack(x0s_))
+            tmpl.append((rs, dzw, sol.pi_scale))
+        xs = _BatchedSolve.apply(self, *bs, *x0s)
+        out = []
+        for x, (rs, dzw, ps) in zip(xs, tmpl):
+            d = unpack(x, rs)
+            out.append({"u": d["u"], "v": d["v"], "w": d["w"], "pi": d["pi"] / ps, "eta_dot": d["eta_dot"] / dzw})
+        return out
+
+    # -- batched trajectories and gathers ------------------------------------------
+    def _batched_departure(self, items):
+        loc = items[0][1]
+        advs = [self.steppers[m].advector for m in range(self.M)]
+        dt_ = advs[0].coord_dtype
+        us, vs, es, m2s, outs, scrs, grids = [], [], [], [], [], [], []
+        for m, (state, _loc) in enumerate(items):
+            a = advs[m]
+            g = a.grid
+            us.append(state["u"].detach().to(dt_).contiguous())
+            vs.append(state["v"].detach().to(dt_).contiguous())
+            es.append(state["eta_dot"].detach().to(dt_).contiguous())
+            m2s.append(g.i_m_factors[loc].detach().to(dt_).contiguous())
+            shape = {"m": (g.nz, g.ny, g.nx), "u": (g.nz, g.ny, g.nx + 1),
+                     "v": (g.nz, g.ny + 1, g.nx), "w": (g.nz + 1, g.ny, g.nx)}[loc]
+            outs.append(torch.empty((3,) + shape, dtype=dt_, device=us[-1].device))
+            scrs.append(torch.empty_like(outs[-1]))
+            grids.append(a._cgrid)
+        L = _lib.lib()
+        fn = L.sisl_departure_batched_f64 if dt_ == torch.float64 else L.sisl_departure_batched_f32
+        dx = (C.c_double * self.M)(*[a.grid.dx for a in advs])
+        dy = (C.c_double * self.M)(*[a.grid.dy for a in advs])
+        with torch.no_grad():
+            _lib.check(fn(self.M, (_lib.SislGrid * self.M)(*grids), LOC_ID[loc], _lib.ptr_array(us),
+                          _lib.ptr_array(vs), _lib.ptr_array(es), _lib.ptr_array(m2s), dx, dy,
+                          float(self.dt), 2, _lib.ptr_array(scrs), _lib.ptr_array(outs),
+                          _lib.stream_ptr(us[0].device)), "batched departure points")
+        return [Coords(o) for o in outs]
+
+    def _batched_tricubic(self, items):
+        lim = bool(items[0][2])
+        fields, coords, cast = [], [], []
+        for field, co, _lim in items:
+            if not isinstance(co, Coords):
+                co = Coords(co)
+            c32 = co.data.dtype == torch.float32 and field.dtype == torch.float64
+            fields.append(field.float() if c32 else field)
+            cast.append(c32)
+            coords.append(co)
+        outs = _BatchedTricubic.apply(lim, coords, *fields)
+        return [o.double() if c else o for o, c in zip(outs, cast)]
+
+    # -- stepping --------------------------------------------------------------------
+    def _install(self, rdv):
+        """Route each member's hot calls through the rendezvous (instance
+        attributes shadow the class methods; removed by _uninstall)."""
+        for m, stp in enumerate(self.steppers):
+            adv, sol = stp.advector, stp.implicit_solver
+
+            def dep(state, loc="m", iterations=2, m=m):
+                return rdv(m, "departure:" + loc, (state, loc), self._batched_departure)
+
+            def cub(field, coords, use_limiter=False, m=m):
+                return rdv(m, "tricubic", (field, coords, use_limiter), self._batched_tricubic)
+
+            def solve(rhs_prime, bg, x0=None, m=m):
+                return rdv(m, "solve", (rhs_prime, bg, x0), self._batched_solve)
+            adv.compute_departure_indices = dep
+            adv.advect_cubic = cub
+            sol.solve = solve
+
+    def _uninstall(self):
+        for stp in self.steppers:
+            for obj, name in ((stp.advector, "compute_departure_indices"), (stp.advector, "advect_cubic"),
+                              (stp.implicit_solver, "solve")):
+                obj.__dict__.pop(name, None)
+
+    def step_internal(self, states, t, forcings, bc_fns):
+        """One lockstep step of every member (internal dicts in and out)."""
+        rdv = _Rendezvous(self.M)
+        self._install(rdv)
+        out = [None] * self.M
+        errs = []
+        grad = torch.is_grad_enabled()
+        stream = torch.cuda.current_stream()
+
+        def work(m):
+            try:
+                torch.set_grad_enabled(grad)
+                with torch.cuda.stream(stream):
+                    out[m] = self.steppers[m].step_internal(states[m], t, forcings[m], bc_fns[m])
+            except BaseException as e:  # noqa: BLE001 -- re-raised in the caller
+                errs.append(e)
+                rdv.fail(e)
+        try:
+            threads = [threading.Thread(target=work, args=(m,)) for m in range(self.M)]
+            for th in threads:
+                th.start()
+            for th in threads:
+                th.join()
+        finally:
+            self._uninstall()
+        if errs:
+            raise errs[0]
+        return out
+
+    def step(self, states, t, forcings, bc_fns):
+        """Reference-shaped states in and out, one per member."""
+        ins = [state_to_internal(s) for s in states]
+        return [state_to_reference(s) for s in self.step_internal(ins, t, forcings, bc_fns)]
+
+    def integrate(self, states, t_start, num_steps, forcings, bc_fns, checkpoint=False):
+        """``num_steps`` lockstep steps.  ``checkpoint=True``: one torch
+        checkpoint per ensemble step (the recomputation replays the step in
+        lockstep; departure points are recomputed from the same states)."""
+        from torch.utils.checkpoint import checkpoint as ckpt
+        sts = [state_to_internal(s) for s in states]
+        for s in sts:
+            if "eta_dot" not in s:
+                s["eta_dot"] = torch.zeros_like(s["w"])
+        if checkpoint:
+            for stp, f, b in zip(self.steppers, forcings, bc_fns):
+                stp.warm_caches(f, b)
+            self._prepare([stp.bg_precomputed() for stp in self.steppers])
+        for i in range(int(num_steps)):
+            t = t_start + i * self.dt
+            if checkpoint and torch.is_grad_enabled():
+                keys = [sorted(k for k, v in s.items() if torch.is_tensor(v)) for s in sts]
+                flat = [s[k] for s, ks in zip(sts, keys) for k in ks]
+                extra = [{k: v for k, v in s.items() if not torch.is_tensor(v)} for s in sts]
+
+                def fn(*tensors, t=t, keys=keys, extra=extra):
+                    it = iter(tensors)
+                    ins = []
+                    for ks, ex in zip(keys, extra):
+                        d = dict(ex)
+                        for k in ks:
+                            d[k] = next(it)
+                        ins.append(d)
+                    res = self.step_internal(ins, t, forcings, bc_fns)
+                    self._out_keys = [sorted(k for k, v in r.items() if torch.is_tensor(v)) for r in res]
+                    self._out_extra = [{k: v for k, v in r.items() if not torch.is_tensor(v)} for r in res]
+                    return tuple(r[k] for r, ks in zip(res, self._out_keys) for k in ks)
+                outs = ckpt(fn, *flat, use_reentrant=False)
+                it = iter(outs)
+                sts = []
+                for ks, ex in zip(self._out_keys, self._out_extra):
+                    d = dict(ex)
+                    for k in ks:
+                        d[k] = next(it)
+                    sts.append(d)
+            else:
+                sts = self.step_internal(sts, t, forcings, bc_fns)
+        return [state_to_reference(s) for s in sts]

@@ -251,6 +251,27 @@ def adjoint(nx=128, ny=128, nz=60, dx=1000.0, top=15000.0, seed=3, steps=50):
     }
 
 
+def ensemble_members(M=32, seed=5, steps=20, nx_range=(96, 192), nz_range=(48, 80), sizes=None):
+    """Configs of the batched-ensemble case: ``M`` limited-area members on
+    seeded uneven grids (nx = ny in ``nx_range`` by 16, nz in ``nz_range`` by
+    8, dx = 1 km, top 20 km; or the explicit (n, nz) list ``sizes``), each
+    with its own terrain and wind seeds and seeded theta noise of 0.5 K."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for m in range(M if sizes is None else len(sizes)):
+        if sizes is not None:
+            n, nz = sizes[m]
+        else:
+            n = int(rng.integers(nx_range[0] // 16, nx_range[1] // 16 + 1)) * 16
+            nz = int(rng.integers(nz_range[0] // 8, nz_range[1] // 8 + 1)) * 8
+        cfg = lam(nx=n, ny=n, nz=nz, seed=seed * 100 + m, steps=steps)
+        cfg["name"] = f"ensemble[{m}]"
+        kind, kw = cfg["init"]
+        cfg["init"] = (kind, dict(kw, theta_noise=0.5, noise_seed=seed * 1000 + m))
+        out.append(cfg)
+    return out
+
+
 def make_terrain(spec, grid_kw):
     if spec is None:
         return None
@@ -309,4 +330,8 @@ def initial_state(cfg, Z_m, x_m, y_m, z_m=None, c=CONSTANTS):
     elif kind == "uniform":
         st["u"] = np.full_like(st["u"], kw.get("u", 0.0))
         st["v"] = np.full_like(st["v"], kw.get("v", 0.0))
+    if kw.get("theta_noise", 0.0) > 0.0:
+        rng = np.random.default_rng(kw.get("noise_seed", 0))
+        st["th_v"] = st["th_v"] + rng.normal(0.0, kw["theta_noise"], size=st["th_v"].shape)
+        st["rho"] = eos_rho(st["th_v"], st["pi"], c)
     return st
