@@ -3,9 +3,9 @@
 An ensemble advances every member one step in lockstep.  Each member runs
 its own unchanged ``SISLStepper3D.step_internal`` in a worker thread; the
 three hot operations of the step -- departure points (4 per step), tricubic
-gathers (6 per step) and implicit solves (2 per step) -- meet at a
-rendezvous, where the last member to arrive issues ONE batched launch for
-all members (``sisl_departure_batched_*``, ``sisl_tricubic_batched_*``,
+gathers (6 per step) and implicit solves (2 per step) -- are collected
+from every member in turn (one member thread runs at a time), and the last
+member issues ONE batched launch for all members (``sisl_departure_batched_*``, ``sisl_tricubic_batched_*``,
 ``sisl_gmres_batched``).  The per-member glue (tendencies, FFSL, EOS,
 boundaries) stays torch ops on device tensors.
 
@@ -31,62 +31,107 @@ from .steppers import KEYS, LOC_ID, _operator_inputs, pack, state_to_internal, s
 MAX_MEMBERS = 32
 
 
-class _Rendezvous:
-    """Calls number c of the members (one per member) form a batch: the last
-    member to arrive runs the batched function and hands each member its
-    result.  A failure in any member breaks the rendezvous for all."""
+class _Baton:
+    """Lockstep schedule of the member threads: exactly one member runs at a
+    time (no interpreter-lock contention, so the host keeps ahead of the
+    device as in a single-threaded loop).  Member m runs until its next batched
+    call, deposits its arguments and passes the baton to member m+1; the last
+    member runs the batched function and passes the baton back to member 0.
+    Every member therefore executes the same call sequence in the order
+    0, 1, ..., M-1, batch.  A failure in any member releases and aborts all."""
 
-    def __init__(self, M, timeout=600.0):
+    def __init__(self, M):
         self.M = M
-        self.cv = threading.Condition()
-        self.count = [0] * M
-        self.pending = {}
-        self.results = {}
+        self.go = [threading.Semaphore(0) for _ in range(M)]
+        self.items = [None] * M
+        self.kind = None
+        self.result = None
         self.broken = None
-        self.timeout = timeout
+
+    def _check(self):
+        if self.broken is not None:
+            raise RuntimeError("ensemble step aborted") from self.broken
+
+    def enter(self, m):
+        self.go[m].acquire()
+        self._check()
 
     def __call__(self, m, kind, payload, fn):
-        with self.cv:
-            if self.broken is not None:
-                raise RuntimeError("ensemble step aborted") from self.broken
-            c = self.count[m]
-            self.count[m] += 1
-            slot = self.pending.setdefault(c, {"kind": kind, "items": {}})
-            if slot["kind"] != kind:
-                self.broken = RuntimeError(f"members diverged: {slot['kind']} vs {kind}")
-                self.cv.notify_all()
-                raise self.broken
-            slot["items"][m] = payload
-            if len(slot["items"]) == self.M:
-                items = [slot["items"][i] for i in range(self.M)]
-                del self.pending[c]
-                try:
-                    res = fn(items)
-                except BaseException as e:  # noqa: BLE001 -- propagated to every member
-                    self.broken = e
-                    self.cv.notify_all()
-                    raise
-                self.results[c] = [res, self.M]
-                self.cv.notify_all()
-            else:
-                while c not in self.results:
-                    if self.broken is not None:
-                        raise RuntimeError("ensemble step aborted") from self.broken
-                    if not self.cv.wait(self.timeout):
-                        self.broken = TimeoutError("ensemble rendezvous timed out")
-                        self.cv.notify_all()
-                        raise self.broken
-            entry = self.results[c]
-            entry[1] -= 1
-            if entry[1] == 0:
-                del self.results[c]
-            return entry[0][m]
+        self._check()
+        if m == 0:
+            self.kind = kind
+            self.items = [None] * self.M
+        elif kind != self.kind:
+            err = RuntimeError(f"members diverged: {self.kind} vs {kind}")
+            self.fail(err)
+            raise err
+        self.items[m] = payload
+        if m < self.M - 1:
+            self.go[m + 1].release()
+        else:
+            try:
+                self.result = fn(self.items)
+            except BaseException as e:  # noqa: BLE001 -- propagated to every member
+                self.fail(e)
+                raise
+            self.go[0].release()
+        self.go[m].acquire()
+        self._check()
+        return self.result[m]
+
+    def leave(self, m):
+        if m < self.M - 1:
+            self.go[m + 1].release()
 
     def fail(self, exc):
-        with self.cv:
-            if self.broken is None:
-                self.broken = exc
-            self.cv.notify_all()
+        if self.broken is None:
+            self.broken = exc
+        for g in self.go:
+            g.release()
+
+
+def run_lockstep(baton, body):
+    """Run ``body(m)`` for every member m in its own thread under ``baton``
+    and return the results in member order.
+
+    Grad mode and saved-tensor hooks are thread-local in torch, so each member
+    thread re-installs the caller's: a non-reentrant checkpoint around the
+    ensemble step then sees every tensor the members save (otherwise each
+    member step would keep its whole graph).  The baton runs the members one
+    at a time in a fixed order, so the tensors are saved in the same order in
+    the forward pass and in the checkpoint's recomputation, which matches them
+    by position.  The first failure is re-raised in the caller."""
+    M = baton.M
+    out = [None] * M
+    errs = []
+    grad = torch.is_grad_enabled()
+    hooks = torch._C._autograd._top_saved_tensors_default_hooks(False)
+
+    def work(m):
+        pushed = False
+        try:
+            baton.enter(m)
+            torch.set_grad_enabled(grad)
+            if hooks is not None:
+                torch._C._autograd._push_saved_tensors_default_hooks(*hooks)
+                pushed = True
+            out[m] = body(m)
+            baton.leave(m)
+        except BaseException as e:  # noqa: BLE001 -- re-raised in the caller
+            errs.append(e)
+            baton.fail(e)
+        finally:
+            if pushed:
+                torch._C._autograd._pop_saved_tensors_default_hooks()
+    threads = [threading.Thread(target=work, args=(m,)) for m in range(M)]
+    for th in threads:
+        th.start()
+    baton.go[0].release()
+    for th in threads:
+        th.join()
+    if errs:
+        raise errs[0]
+    return out
 
 
 class _BatchedTricubic(torch.autograd.Function):
@@ -95,7 +140,9 @@ class _BatchedTricubic(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, lim, coords_list, *fields):
-        ctx.coords = coords_list
+        # departure points through save_for_backward, so that a checkpointed
+        # ensemble step drops them after the forward pass
+        ctx.save_for_backward(*[c.data for c in coords_list])
         ctx.lim = lim
         fields = [f.contiguous() for f in fields]
         outs = [torch.empty_like(f) for f in fields]
@@ -117,8 +164,8 @@ class _BatchedTricubic(torch.autograd.Function):
         if ctx.lim:
             raise NotImplementedError("the adjoint of the limited tricubic is not implemented")
         outs = []
-        for g, co in zip(grads, ctx.coords):
-            outs.append(None if g is None else _Tricubic.backward(_Ctx(co, False), g)[0])
+        for g, co in zip(grads, ctx.saved_tensors):
+            outs.append(None if g is None else _Tricubic.backward(_Ctx(Coords(co), False), g)[0])
         return (None, None) + tuple(outs)
 
 
@@ -164,7 +211,6 @@ class Ensemble:
         self._workspace = None
         self._ws_key = None
         self.last_stats = []
-        self._rdv = None
 
     # -- batched solve -----------------------------------------------------------
     def _prepare(self, bgs):
@@ -275,7 +321,7 @@ class Ensemble:
 
     # -- stepping --------------------------------------------------------------------
     def _install(self, rdv):
-        """Route each member's hot calls through the rendezvous (instance
+        """Route each member's hot calls through the baton (instance
         attributes shadow the class methods; removed by _uninstall)."""
         for m, stp in enumerate(self.steppers):
             adv, sol = stp.advector, stp.implicit_solver
@@ -300,32 +346,17 @@ class Ensemble:
 
     def step_internal(self, states, t, forcings, bc_fns):
         """One lockstep step of every member (internal dicts in and out)."""
-        rdv = _Rendezvous(self.M)
-        self._install(rdv)
-        out = [None] * self.M
-        errs = []
-        grad = torch.is_grad_enabled()
+        baton = _Baton(self.M)
+        self._install(baton)
         stream = torch.cuda.current_stream()
 
-        def work(m):
-            try:
-                torch.set_grad_enabled(grad)
-                with torch.cuda.stream(stream):
-                    out[m] = self.steppers[m].step_internal(states[m], t, forcings[m], bc_fns[m])
-            except BaseException as e:  # noqa: BLE001 -- re-raised in the caller
-                errs.append(e)
-                rdv.fail(e)
+        def body(m):
+            with torch.cuda.stream(stream):
+                return self.steppers[m].step_internal(states[m], t, forcings[m], bc_fns[m])
         try:
-            threads = [threading.Thread(target=work, args=(m,)) for m in range(self.M)]
-            for th in threads:
-                th.start()
-            for th in threads:
-                th.join()
+            return run_lockstep(baton, body)
         finally:
             self._uninstall()
-        if errs:
-            raise errs[0]
-        return out
 
     def step(self, states, t, forcings, bc_fns):
         """Reference-shaped states in and out, one per member."""

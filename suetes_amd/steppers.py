@@ -394,7 +394,8 @@ class SemiImplicitSolver3D:
         for ops in (self.op64, self.op32):
             if ops is not None:
                 _autotune(ops)
-        self.workspace = torch.empty(int(nbytes), dtype=torch.uint8, device=co["cua"].device)
+        self._ws_bytes = int(nbytes)
+        self.workspace = None  # allocated at the first solve (an ensemble never needs it)
         self._bg_id = key
         self._bg = bg
 
@@ -417,6 +418,8 @@ class SemiImplicitSolver3D:
         p = self._params(transpose)
         L = _lib.lib()
         s = _lib.stream_ptr(b.device)
+        if self.workspace is None:
+            self.workspace = torch.empty(self._ws_bytes, dtype=torch.uint8, device=b.device)
         if self.precision == "mixed":
             rc = L.sisl_gmres_mixed(C.byref(self.op64.struct), C.byref(self.op32.struct), C.byref(p),
                                     b.data_ptr(), x.data_ptr(), self.workspace.data_ptr(), C.byref(st), s)
