@@ -208,10 +208,12 @@ int sisl_gmres_mixed(const SislOperator *op64, const SislOperator *op32,
 /* ---- kernel timing (measurement runs) ------------------------------------ */
 
 /* Timed kernels: operator apply, column preconditioner, the Krylov
- * projection / update / normalisation passes, the tricubic gather. */
+ * projection / update / normalisation passes, the tricubic gather, and the
+ * transposes of the operator, the preconditioner and the gather. */
 enum SislTimedKernel {
   SISL_T_HELMHOLTZ = 0, SISL_T_PRECOND = 1, SISL_T_KRYLOV_DOTS = 2, SISL_T_KRYLOV_UPDATE = 3,
-  SISL_T_KRYLOV_NORMALIZE = 4, SISL_T_TRICUBIC = 5, SISL_T_COUNT = 6
+  SISL_T_KRYLOV_NORMALIZE = 4, SISL_T_TRICUBIC = 5, SISL_T_HELMHOLTZ_T = 6, SISL_T_PRECOND_T = 7,
+  SISL_T_TRICUBIC_ADJ = 8, SISL_T_COUNT = 9
 };
 /* on = 1: clear the record and bracket every later launch of a timed kernel
  * with HIP events on its stream; on = 0: stop recording (the record stays). */

@@ -30,7 +30,7 @@ EXPORTED = [
 
 # timed kernels (enum SislTimedKernel)
 TIMED = {"helmholtz": 0, "precond": 1, "krylov_dots": 2, "krylov_update": 3,
-         "krylov_normalize": 4, "tricubic": 5}
+         "krylov_normalize": 4, "tricubic": 5, "helmholtz_T": 6, "precond_T": 7, "tricubic_adj": 8}
 
 
 class SislGrid(C.Structure):

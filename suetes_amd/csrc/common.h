@@ -141,12 +141,13 @@ int launch_AT_batched(const BOp<T> *tab, int M, int maxb1, int maxb2, int j, hip
 template <typename T>
 int launch_M_batched(const BOp<T> *tab, int M, int maxb, int L, int S, int j, hipStream_t s);
 template <typename T>
-int launch_MT_batched(const BOp<T> *tab, int M, int maxcols, int maxfaces, int j, hipStream_t s);
+int launch_MT_batched(const BOp<T> *tab, int M, int maxb, int L, int S, int maxf, int j, hipStream_t s);
 // Host-side launch geometry of one member (for maxb).
 int blocks_A(const Stag &s);
 int blocks_AT1(const Stag &s);
 int blocks_AT2(const Stag &s);
 int blocks_M(const Stag &s);
+int blocks_MT_faces(const Stag &s);
 int seg_levels(int nz);  // levels per segment L of the preconditioner (0: too tall)
 
 // Optional launch timing (timing.hip): returns a record id or -1 when off.

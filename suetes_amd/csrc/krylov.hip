@@ -1198,7 +1198,7 @@ static int ens_setup(const SislOperator *ops64, const SislOperator *ops32, int M
     P.maxAT2 = std::max(P.maxAT2, blocks_AT2(s));
     P.maxM = std::max(P.maxM, blocks_M(s));
     P.maxCols = std::max(P.maxCols, s.nx * s.ny);
-    P.maxFaces = std::max<int>(P.maxFaces, (int)s.ow);
+    P.maxFaces = std::max(P.maxFaces, blocks_MT_faces(s));
     P.maxN = std::max(P.maxN, s.n);
     off += ens_member_bytes(s, R);
   }
@@ -1252,7 +1252,7 @@ static int ens_arnoldi(const EnsPlan &P, int j, int transpose, hipStream_t st) {
     if ((rc = launch_M_batched<float>(P.M32, M, P.maxM, P.L, P.S, j, st))) return rc;
   } else {
     if ((rc = launch_AT_batched<float>(P.AT32, M, P.maxAT1, P.maxAT2, j, st))) return rc;
-    if ((rc = launch_MT_batched<float>(P.MT32, M, P.maxCols, P.maxFaces, j, st))) return rc;
+    if ((rc = launch_MT_batched<float>(P.MT32, M, P.maxM, P.L, P.S, P.maxFaces, j, st))) return rc;
   }
   kb_dots_dispatch(P, nv, j, 0, st);
   hipLaunchKernelGGL(kb_reduce_kernel, dim3(nv + 1, M), dim3(kBlock), 0, st, (const KMem *)P.km, j, kSelH1, 0, 0);
@@ -1305,7 +1305,7 @@ static int ens_solve(const void *ws, const SislGmresParams &p, const double *con
   if (!p.transpose) {
     if ((rc = launch_M_batched<double>(P.Mb64, M, P.maxM, P.L, P.S, 0, st))) return rc;
   } else {
-    if ((rc = launch_MT_batched<double>(P.MTb64, M, P.maxCols, P.maxFaces, 0, st))) return rc;
+    if ((rc = launch_MT_batched<double>(P.MTb64, M, P.maxM, P.L, P.S, P.maxFaces, 0, st))) return rc;
   }
   hipLaunchKernelGGL(kb_norm2_kernel, dim3(kRedGrid, M), dim3(kBlock), 0, st, (const KMem *)P.km, 1);
   hipLaunchKernelGGL(kb_reduce_kernel, dim3(1, M), dim3(kBlock), 0, st, (const KMem *)P.km, 0, kSelScal0 + 1, 0, 1);
@@ -1323,7 +1323,7 @@ static int ens_solve(const void *ws, const SislGmresParams &p, const double *con
     if (!p.transpose) {
       if ((rc = launch_M_batched<double>(P.M64, M, P.maxM, P.L, P.S, 0, st))) return rc;
     } else {
-      if ((rc = launch_MT_batched<double>(P.MT64, M, P.maxCols, P.maxFaces, 0, st))) return rc;
+      if ((rc = launch_MT_batched<double>(P.MT64, M, P.maxM, P.L, P.S, P.maxFaces, 0, st))) return rc;
     }
     hipLaunchKernelGGL(kb_norm2_kernel, dim3(kRedGrid, M), dim3(kBlock), 0, st, (const KMem *)P.km, 1);
     hipLaunchKernelGGL(kb_reduce_kernel, dim3(1, M), dim3(kBlock), 0, st, (const KMem *)P.km, 0, kSelScal0 + 2, 0, 1);

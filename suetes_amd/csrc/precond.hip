@@ -18,6 +18,8 @@ namespace sisl {
 // final multiply-add gives the exact sequential result up to rounding.  Every
 // input element is read once and every output element written once.
 constexpr int kMaxSeg = 8;  // at most 8 segments (512 threads): columns up to 256 levels
+// Workgroups of the segment-parallel kernels: one per 64 columns of a row.
+__host__ __device__ inline int blocks_M_dev(const Stag &s) { return ((s.nx + 63) / 64) * s.ny; }
 
 // Body for launch block `blk` of `nblk`; blockDim.y may exceed the segments
 // the column needs (the extra waves carry identity compositions).
@@ -165,105 +167,210 @@ __global__ void __launch_bounds__(512) precond_seg_kernel(OpT<T> o, const T *__r
   precond_seg_body<T, L>(o, r, z, blockIdx.x, gridDim.x);
 }
 
-// Transpose: column sweeps of the reversed chain; the u/v cotangents of the
-// kinematic term leave through mass-point scratch and are gathered to faces.
-template <typename T>
-__device__ __forceinline__ void precond_T_body(const OpT<T> &o, const T *__restrict__ zb,
-                                               T *__restrict__ rb, T *__restrict__ umb,
-                                               T *__restrict__ vmb, int64_t col) {
+// Transpose, segment-parallel like the forward solve.  M = B_out * T^{-1} *
+// B_in per column, with T = L U the Thomas factorisation (U unit upper with
+// super-diagonal c', L lower with pivots 1/invden and sub-diagonal `lower`).
+// The reversed chain: p = B_out^T zbar (the pi, w, eta_dot back-substitution
+// transposed), U^T y = p (forward recurrence y_k = p_k - c'_{k-1} y_{k-1}),
+// L^T x = y (backward recurrence x_k = invden_k (y_k - lower_{k+1} x_{k+1})),
+// then B_in^T x: the Helmholtz right-hand side transposed into w, eta_dot
+// and mass-point cotangents of the averaged u, v (umb, vmb), which the faces
+// kernel gathers to the u, v faces.  Wave `sg` owns levels [sg*L, sg*L + L)
+// of 64 columns of row j; both recurrences compose across segments through
+// LDS as in the forward solve.
+template <typename T, int L>
+__device__ __forceinline__ void precond_T_seg_body(const OpT<T> &o, const T *__restrict__ zb,
+                                                   T *__restrict__ rb, T *__restrict__ umb,
+                                                   T *__restrict__ vmb, int blk, int nblk) {
+  __shared__ T la[kMaxSeg][64], lb[kMaxSeg][64];
   const Stag &s = o.s;
   const int nx = s.nx, ny = s.ny, nz = s.nz;
-  if (col >= (int64_t)nx * ny) return;
-  const int i = (int)(col % nx), j = (int)(col / nx);
+  const int nbx = (nx + 63) / 64;
+  const int tile = xcd_tile(blk, nblk);
+  const int bx = tile % nbx, j = tile / nbx;
+  const int tx = threadIdx.x, sg = threadIdx.y, S = blockDim.y;
+  const int i = bx * 64 + tx;
+  const int ka = sg * L, kb = min(ka + L, nz);
+  const bool act = i < nx && ka < nz;
   const T *ZW = zb + s.ow, *ZP = zb + s.om, *ZE = zb + s.oe;
   T *RW = rb + s.ow, *RP = rb + s.om, *RE = rb + s.oe;
-  const T a = o.alpha;
-  // U^T y = pbar_p (forward), storing y in RP.
-  T yprev = 0;
-  for (int k = 0; k < nz; ++k) {
-    const int64_t m = s.m.at(i, j, k);
-    T pp = (T)1000 * ZP[m];
-    if (k >= 1) {
-      const int64_t w = s.w.at(i, j, k);
-      pp -= o.gw[w] * (ZW[w] + ZE[w]);
+  const T ra = (T)1 / o.alpha, half = (T)0.5;
+  const uint32_t pm = (uint32_t)nx * ny;
+  const uint32_t mc = (uint32_t)j * nx + i;
+  // zs[t] = zbar_w + zbar_eta at w level ka + t, ze[t] = zbar_eta there
+  T d[L], B[L], zs[L], ze[L];
+  T dl = 0, bl = 1;
+  if (act) {
+    // ---- p_k = 1000 zbar_pi - g_k + g_{k+1}, g = gw (zbar_w + zbar_eta) on interior w levels;
+    //      U^T y = p, local forward sweep ----
+    T g_lo = 0;
+    if (ka >= 1) {
+      const uint32_t w = mc + (uint32_t)ka * pm;
+      const T e = ldg(ZE, w);
+      zs[0] = ldg(ZW, w) + e;
+      ze[0] = e;
+      g_lo = ldg(o.gw, w) * zs[0];
     }
-    if (k + 1 <= nz - 1) {
-      const int64_t w = s.w.at(i, j, k + 1);
-      pp += o.gw[w] * (ZW[w] + ZE[w]);
+#pragma unroll
+    for (int t = 0; t < L; ++t) {
+      const int k = ka + t;
+      if (k < kb) {
+        T g_hi = 0;
+        if (k + 1 < nz) {
+          const uint32_t w = mc + (uint32_t)(k + 1) * pm;
+          const T e = ldg(ZE, w), sum = ldg(ZW, w) + e;
+          g_hi = ldg(o.gw, w) * sum;
+          if (t + 1 < L) { zs[t + 1] = sum; ze[t + 1] = e; }
+        }
+        const uint32_t m = mc + (uint32_t)k * pm;
+        const T pp = (T)1000 * ldg(ZP, m) - g_lo + g_hi;
+        const T a = k >= 1 ? -ldg(o.cprime, m - pm) : (T)0;
+        dl = pp + a * dl;
+        bl = a * bl;
+        d[t] = dl;
+        B[t] = bl;
+        g_lo = g_hi;
+      }
     }
-    const T y = (k == 0) ? pp : pp - o.cprime[s.m.at(i, j, k - 1)] * yprev;
-    RP[m] = y;
-    yprev = y;
   }
-  // L^T x = y (backward), then the Helmholtz-RHS transpose per w level.
-  T xhi = 0, divb_hi = 0, umb_hi = 0, vmb_hi = 0;
-  for (int k = nz - 1; k >= 0; --k) {
-    const int64_t m = s.m.at(i, j, k);
-    const T y = RP[m];
-    const T x = (k == nz - 1) ? y * o.invden[m] : (y - o.lower[s.m.at(i, j, k + 1)] * xhi) * o.invden[m];
-    RP[m] = x * (T)1e-3;
-    const T divb = -o.lpi[m] * x;
-    if (k + 1 <= nz - 1) {
-      const int64_t w = s.w.at(i, j, k + 1);
-      const T wckb = o.cz[w] * (divb - divb_hi);
-      const T ze = ZE[w];
-      RW[w] = o.idw[w] * (ZW[w] + ze + wckb);
-      RE[w] = (ze + wckb) / a;
-      const T kinb = -ze - wckb;
-      const T uwb = o.kx[w] * kinb, vwb = o.ky[w] * kinb;
-      umb[s.m.at(i, j, k + 1)] = umb_hi + (T)0.5 * uwb;
-      vmb[s.m.at(i, j, k + 1)] = vmb_hi + (T)0.5 * vwb;
-      umb_hi = (T)0.5 * uwb;
-      vmb_hi = (T)0.5 * vwb;
+  la[sg][tx] = dl;
+  lb[sg][tx] = bl;
+  __syncthreads();
+  T din = 0;
+  for (int t = 0; t < sg; ++t) din = la[t][tx] + lb[t][tx] * din;
+  __syncthreads();
+  // ---- L^T x = y, local backward sweep ----
+  T xl = 0, gl = 1;
+  if (act) {
+#pragma unroll
+    for (int t = L - 1; t >= 0; --t) {
+      const int k = ka + t;
+      if (k < kb) {
+        const uint32_t m = mc + (uint32_t)k * pm;
+        const T y = d[t] + B[t] * din;
+        const T iv = ldg(o.invden, m);
+        const T c = k + 1 < nz ? -iv * ldg(o.lower, m + pm) : (T)0;
+        xl = iv * y + c * xl;
+        gl = c * gl;
+        d[t] = xl;
+        B[t] = gl;
+      }
     }
-    xhi = x;
-    divb_hi = divb;
   }
-  umb[s.m.at(i, j, 0)] = umb_hi;
-  vmb[s.m.at(i, j, 0)] = vmb_hi;
-  const int64_t w0 = s.w.at(i, j, 0), wn = s.w.at(i, j, nz);
-  const int64_t q2 = s.m2(i, j);
-  RW[w0] = ZW[w0];
-  RW[wn] = ZW[wn];
-  RE[w0] = ZE[w0] * o.dzw0[q2];
-  RE[wn] = ZE[wn] * o.dzwn[q2];
+  la[sg][tx] = xl;
+  lb[sg][tx] = gl;
+  __syncthreads();
+  T xin = 0;
+  for (int t = S - 1; t > sg; --t) xin = la[t][tx] + lb[t][tx] * xin;
+  T xtop = 0;
+  if (act) {
+#pragma unroll
+    for (int t = 0; t < L; ++t)
+      if (ka + t < kb) {
+        d[t] = d[t] + B[t] * xin;
+        xtop = d[t];
+      }
+  }
+  __syncthreads();
+  la[sg][tx] = xtop;       // x at the segment's top level, for the segment above
+  lb[sg][tx] = act ? d[0] : (T)0;  // x at the segment's bottom level, for the segment below
+  __syncthreads();
+  if (!act) return;
+  const T xbelow = sg > 0 ? la[sg - 1][tx] : (T)0;
+  const T xabove = kb < nz ? lb[sg + 1][tx] : (T)0;
+  const uint32_t q2 = (uint32_t)s.m2(i, j);
+  // ---- outputs: pi; then per w level k in [ka, kb] the right-hand-side transpose
+  //      (w, eta_dot owned for k < kb; the top face only feeds umb, vmb) ----
+  T divb_lo = ka >= 1 ? -ldg(o.lpi, mc + (uint32_t)(ka - 1) * pm) * xbelow : (T)0;
+  T uw_lo = 0, vw_lo = 0;
+#pragma unroll
+  for (int t = 0; t <= L; ++t) {
+    const int k = ka + t;
+    if (k <= kb) {
+      const uint32_t m = mc + (uint32_t)k * pm;  // mass index of level k == w index of level k
+      const T xk = k < kb ? d[t < L ? t : L - 1] : xabove;
+      if (k < kb) stg(RP, m, xk * (T)1e-3);
+      const T divb_hi = k < nz ? -ldg(o.lpi, m) * xk : (T)0;
+      T uw = 0, vw = 0;
+      if (k == 0) {
+        stg(RW, m, ldg(ZW, m));
+        stg(RE, m, ldg(ZE, m) * ldg(o.dzw0, q2));
+      } else if (k == nz) {
+        stg(RW, m, ldg(ZW, m));
+        stg(RE, m, ldg(ZE, m) * ldg(o.dzwn, q2));
+      } else {
+        T zsum, e;
+        if (k < kb && t < L) {
+          zsum = zs[t < L ? t : L - 1];
+          e = ze[t < L ? t : L - 1];
+        } else {
+          e = ldg(ZE, m);
+          zsum = ldg(ZW, m) + e;
+        }
+        const T wckb = ldg(o.cz, m) * (divb_lo - divb_hi);
+        if (k < kb) {
+          stg(RW, m, ldg(o.idw, m) * (zsum + wckb));
+          stg(RE, m, (e + wckb) * ra);
+        }
+        const T kinb = -e - wckb;
+        uw = ldg(o.kx, m) * kinb;
+        vw = ldg(o.ky, m) * kinb;
+      }
+      if (k > ka) {
+        stg(umb, m - pm, half * uw_lo + half * uw);
+        stg(vmb, m - pm, half * vw_lo + half * vw);
+      }
+      uw_lo = uw;
+      vw_lo = vw;
+      divb_lo = divb_hi;
+    }
+  }
 }
 
-template <typename T>
-__global__ void __launch_bounds__(256) precond_T_kernel(OpT<T> o, const T *__restrict__ zb,
-                                                        T *__restrict__ rb,
-                                                        T *__restrict__ umb,
-                                                        T *__restrict__ vmb,
-                                                        const int *__restrict__ stop) {
+template <typename T, int L>
+__global__ void __launch_bounds__(512) precond_T_seg_kernel(OpT<T> o, const T *__restrict__ zb,
+                                                             T *__restrict__ rb,
+                                                             T *__restrict__ umb,
+                                                             T *__restrict__ vmb,
+                                                             const int *__restrict__ stop) {
   if (stop && *stop) return;
-  precond_T_body(o, zb, rb, umb, vmb, (int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+  precond_T_seg_body<T, L>(o, zb, rb, umb, vmb, blockIdx.x, gridDim.x);
 }
 
+// u, v faces of the transpose: zbar passes through and each face gathers half
+// of the mass-point cotangent of its (one or two) neighbouring columns.  A
+// 64x4 block covers 64 faces in x and 4 rows of one level; the thread at
+// (i, j, k) does the u face (i, j, k) and the v face (i, j, k).
+constexpr int kFTY = 4;
+__host__ __device__ inline int faces_blocks(const Stag &s) {
+  return ((s.nx + 1 + 63) / 64) * ((s.ny + 1 + kFTY - 1) / kFTY) * s.nz;
+}
 template <typename T>
 __device__ __forceinline__ void precond_T_faces_body(const OpT<T> &o, const T *__restrict__ zb,
                                                      const T *__restrict__ umb,
                                                      const T *__restrict__ vmb,
-                                                     T *__restrict__ rb, int64_t q) {
+                                                     T *__restrict__ rb, int blk) {
   const Stag &s = o.s;
-  if (q >= s.ow) return;
   const int nx = s.nx, ny = s.ny;
-  T out = zb[q];
-  if (q < s.ov) {
-    const int64_t p = q;
-    const int i = (int)(p % (nx + 1)), j = (int)((p / (nx + 1)) % ny), k = (int)(p / ((int64_t)(nx + 1) * ny));
+  const int nbx = (nx + 1 + 63) / 64, nby = (ny + 1 + kFTY - 1) / kFTY;
+  const int bx = blk % nbx, r = blk / nbx, by = r % nby, k = r / nby;
+  const int i = bx * 64 + threadIdx.x, j = by * kFTY + threadIdx.y;
+  const T half = (T)0.5;
+  const uint32_t mrow = ((uint32_t)k * ny + j) * nx;  // mass index of (0, j, k)
+  if (j < ny && i <= nx) {
+    const uint32_t q = ((uint32_t)k * ny + j) * (nx + 1) + i;
     T acc = 0;
-    if (i - 1 >= 0) acc += umb[s.m.at(i - 1, j, k)];
-    if (i <= nx - 1) acc += umb[s.m.at(i, j, k)];
-    out += (T)0.5 * acc;
-  } else {
-    const int64_t p = q - s.ov;
-    const int i = (int)(p % nx), j = (int)((p / nx) % (ny + 1)), k = (int)(p / ((int64_t)nx * (ny + 1)));
-    T acc = 0;
-    if (j - 1 >= 0) acc += vmb[s.m.at(i, j - 1, k)];
-    if (j <= ny - 1) acc += vmb[s.m.at(i, j, k)];
-    out += (T)0.5 * acc;
+    if (i >= 1) acc += ldg(umb, mrow + i - 1);
+    if (i < nx) acc += ldg(umb, mrow + i);
+    stg(rb + s.ou, q, ldg(zb + s.ou, q) + half * acc);
   }
-  rb[q] = out;
+  if (i < nx && j <= ny) {
+    const uint32_t q = ((uint32_t)k * (ny + 1) + j) * nx + i;
+    T acc = 0;
+    if (j >= 1) acc += ldg(vmb, mrow - nx + i);
+    if (j < ny) acc += ldg(vmb, mrow + i);
+    stg(rb + s.ov, q, ldg(zb + s.ov, q) + half * acc);
+  }
 }
 
 template <typename T>
@@ -273,7 +380,7 @@ __global__ void __launch_bounds__(256) precond_T_faces_kernel(OpT<T> o, const T 
                                                               T *__restrict__ rb,
                                                               const int *__restrict__ stop) {
   if (stop && *stop) return;
-  precond_T_faces_body(o, zb, umb, vmb, rb, (int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+  precond_T_faces_body(o, zb, umb, vmb, rb, blockIdx.x);
 }
 
 // Levels per segment: the smallest supported L with at most kMaxSeg
@@ -312,19 +419,39 @@ int launch_M(const OpT<T> &o, const T *r, T *z, hipStream_t st, const int *stop)
 
 template <typename T>
 int launch_MT(const OpT<T> &o, const T *zb, T *rb, T *scratch, hipStream_t st, const int *stop) {
-  const int64_t cols = (int64_t)o.s.nx * o.s.ny;
-  const int64_t nm = o.s.m.n();
-  hipLaunchKernelGGL(precond_T_kernel<T>, dim3(grid_for(cols)), dim3(kBlock), 0, st, o, zb, rb,
-                     scratch, scratch + nm, stop);
-  hipLaunchKernelGGL(precond_T_faces_kernel<T>, dim3(grid_for(o.s.ow)), dim3(kBlock), 0, st, o, zb,
-                     (const T *)scratch, (const T *)(scratch + nm), rb, stop);
+  if ((uint64_t)o.s.n * sizeof(T) >= (1ull << 32)) return SISL_ERR_ARG;  // 32-bit offsets
+  const int L = seg_levels(o.s.nz);
+  if (!L) return SISL_ERR_ARG;
+  const int S = (o.s.nz + L - 1) / L;
+  const Stag &g = o.s;
+  const int64_t nm = g.m.n();
+  T *umb = scratch, *vmb = scratch + nm;
+  const dim3 grid((unsigned)blocks_M(g)), block(64, S);
+  // column part: zbar_w, zbar_eta, zbar_pi; gw, cz, idw, kx, ky (w), cprime, invden, lower, lpi
+  // (mass), dzw0, dzwn (2-D) in; w, eta_dot, pi outputs and umb, vmb out.
+  // faces: zbar_u, zbar_v, umb, vmb in; u, v outputs out.
+  const double nw = (double)g.w.n(), nmf = (double)nm, nuv = (double)(g.u.n() + g.v.n());
+  const double bytes = (double)sizeof(T) * ((2 * nw + nmf) + (5 * nw + 4 * nmf + 2.0 * g.nx * g.ny) +
+                                            (2 * nw + 3 * nmf) + (nuv + 2 * nmf) + nuv);
+  const int tid = timer_begin(SISL_T_PRECOND_T, sizeof(T) == 4, bytes, stop, nullptr, st);
+  switch (L) {
+#define SISL_MT_CASE(c) \
+  case c: hipLaunchKernelGGL((precond_T_seg_kernel<T, c>), grid, block, 0, st, o, zb, rb, umb, vmb, stop); break;
+    SISL_MT_CASE(2) SISL_MT_CASE(4) SISL_MT_CASE(8) SISL_MT_CASE(10) SISL_MT_CASE(12) SISL_MT_CASE(16)
+    SISL_MT_CASE(24) SISL_MT_CASE(32)
+#undef SISL_MT_CASE
+    default: return SISL_ERR_ARG;
+  }
+  hipLaunchKernelGGL(precond_T_faces_kernel<T>, dim3((unsigned)faces_blocks(g)), dim3(64, kFTY), 0, st, o,
+                     zb, (const T *)umb, (const T *)vmb, rb, stop);
+  timer_end(tid, st);
   SISL_CHECK_LAUNCH();
   return SISL_OK;
 }
 
 // ---- batched (ensemble) launches -------------------------------------------
 
-int blocks_M(const Stag &s) { return ((s.nx + 63) / 64) * s.ny; }
+int blocks_M(const Stag &s) { return blocks_M_dev(s); }
 int seg_levels(int nz) {
   static const int sizes[] = {2, 4, 8, 10, 12, 16, 24, 32};
   for (int c : sizes)
@@ -341,22 +468,23 @@ __global__ void __launch_bounds__(512) precond_seg_batched_kernel(const BOp<T> *
   precond_seg_body<T, L>(b.o, b.x + j * b.xj, b.y, blockIdx.x, nb);
 }
 
-template <typename T>
-__global__ void __launch_bounds__(256) precond_T_batched_kernel(const BOp<T> *__restrict__ tab, int j) {
+template <typename T, int L>
+__global__ void __launch_bounds__(512) precond_T_seg_batched_kernel(const BOp<T> *__restrict__ tab, int j) {
   const BOp<T> &b = tab[blockIdx.y];
   if (b.stop && b.stop[j * b.sj]) return;
+  const int nb = blocks_M_dev(b.o.s);
+  if ((int)blockIdx.x >= nb) return;
   const int64_t nm = b.o.s.m.n();
-  precond_T_body(b.o, b.x + j * b.xj, b.y, b.scr, b.scr + nm,
-                 (int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+  precond_T_seg_body<T, L>(b.o, b.x + j * b.xj, b.y, b.scr, b.scr + nm, blockIdx.x, nb);
 }
 
 template <typename T>
 __global__ void __launch_bounds__(256) precond_T_faces_batched_kernel(const BOp<T> *__restrict__ tab, int j) {
   const BOp<T> &b = tab[blockIdx.y];
   if (b.stop && b.stop[j * b.sj]) return;
+  if ((int)blockIdx.x >= faces_blocks(b.o.s)) return;
   const int64_t nm = b.o.s.m.n();
-  precond_T_faces_body(b.o, b.x + j * b.xj, (const T *)b.scr, (const T *)(b.scr + nm), b.y,
-                       (int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+  precond_T_faces_body(b.o, b.x + j * b.xj, (const T *)b.scr, (const T *)(b.scr + nm), b.y, blockIdx.x);
 }
 
 template <typename T>
@@ -374,17 +502,25 @@ int launch_M_batched(const BOp<T> *tab, int M, int maxb, int L, int S, int j, hi
   return SISL_OK;
 }
 template <typename T>
-int launch_MT_batched(const BOp<T> *tab, int M, int maxcols, int maxfaces, int j, hipStream_t st) {
-  hipLaunchKernelGGL(precond_T_batched_kernel<T>, dim3(grid_for(maxcols), M), dim3(kBlock), 0, st, tab, j);
-  hipLaunchKernelGGL(precond_T_faces_batched_kernel<T>, dim3(grid_for(maxfaces), M), dim3(kBlock), 0, st,
-                     tab, j);
+int launch_MT_batched(const BOp<T> *tab, int M, int maxb, int L, int S, int maxf, int j, hipStream_t st) {
+  const dim3 grid(maxb, M), block(64, S);
+  switch (L) {
+#define SISL_MTB_CASE(c) \
+  case c: hipLaunchKernelGGL((precond_T_seg_batched_kernel<T, c>), grid, block, 0, st, tab, j); break;
+    SISL_MTB_CASE(2) SISL_MTB_CASE(4) SISL_MTB_CASE(8) SISL_MTB_CASE(10) SISL_MTB_CASE(12) SISL_MTB_CASE(16)
+    SISL_MTB_CASE(24) SISL_MTB_CASE(32)
+#undef SISL_MTB_CASE
+    default: return SISL_ERR_ARG;
+  }
+  hipLaunchKernelGGL(precond_T_faces_batched_kernel<T>, dim3(maxf, M), dim3(64, kFTY), 0, st, tab, j);
   SISL_CHECK_LAUNCH();
   return SISL_OK;
 }
+int blocks_MT_faces(const Stag &s) { return faces_blocks(s); }
 template int launch_M_batched<double>(const BOp<double> *, int, int, int, int, int, hipStream_t);
 template int launch_M_batched<float>(const BOp<float> *, int, int, int, int, int, hipStream_t);
-template int launch_MT_batched<double>(const BOp<double> *, int, int, int, int, hipStream_t);
-template int launch_MT_batched<float>(const BOp<float> *, int, int, int, int, hipStream_t);
+template int launch_MT_batched<double>(const BOp<double> *, int, int, int, int, int, int, hipStream_t);
+template int launch_MT_batched<float>(const BOp<float> *, int, int, int, int, int, int, hipStream_t);
 
 template int launch_M<double>(const OpT<double> &, const double *, double *, hipStream_t, const int *);
 template int launch_M<float>(const OpT<float> &, const float *, float *, hipStream_t, const int *);

@@ -12,7 +12,8 @@ boundaries) stays torch ops on device tensors.
 The batched tricubic and the batched solve are single autograd Functions over
 all members, so the reverse mode issues batched transposed solves as well.
 ``integrate(..., checkpoint=True)`` checkpoints whole ensemble steps (a
-per-member checkpoint could not be recomputed in lockstep).
+per-member checkpoint could not be recomputed in lockstep): each step is one
+autograd node that recomputes and differentiates itself in the backward pass.
 
 Members must use ``precision='mixed'`` (the batched solver is the mixed
 FP32/FP16/FP64 one) and share ``dt``; at most 32 members.
@@ -364,10 +365,12 @@ class Ensemble:
         return [state_to_reference(s) for s in self.step_internal(ins, t, forcings, bc_fns)]
 
     def integrate(self, states, t_start, num_steps, forcings, bc_fns, checkpoint=False):
-        """``num_steps`` lockstep steps.  ``checkpoint=True``: one torch
-        checkpoint per ensemble step (the recomputation replays the step in
-        lockstep; departure points are recomputed from the same states)."""
-        from torch.utils.checkpoint import checkpoint as ckpt
+        """``num_steps`` lockstep steps.  ``checkpoint=True``: each ensemble
+        step is one autograd node that keeps only its input states and, in
+        the backward pass, recomputes the step in lockstep and differentiates
+        it at once (the activation memory of one step, whatever the number of
+        steps).  Gradients flow to the member states; tensors captured by the
+        forcings or boundary closures receive none."""
         sts = [state_to_internal(s) for s in states]
         for s in sts:
             if "eta_dot" not in s:
@@ -379,30 +382,73 @@ class Ensemble:
         for i in range(int(num_steps)):
             t = t_start + i * self.dt
             if checkpoint and torch.is_grad_enabled():
-                keys = [sorted(k for k, v in s.items() if torch.is_tensor(v)) for s in sts]
-                flat = [s[k] for s, ks in zip(sts, keys) for k in ks]
-                extra = [{k: v for k, v in s.items() if not torch.is_tensor(v)} for s in sts]
-
-                def fn(*tensors, t=t, keys=keys, extra=extra):
-                    it = iter(tensors)
-                    ins = []
-                    for ks, ex in zip(keys, extra):
-                        d = dict(ex)
-                        for k in ks:
-                            d[k] = next(it)
-                        ins.append(d)
-                    res = self.step_internal(ins, t, forcings, bc_fns)
-                    self._out_keys = [sorted(k for k, v in r.items() if torch.is_tensor(v)) for r in res]
-                    self._out_extra = [{k: v for k, v in r.items() if not torch.is_tensor(v)} for r in res]
-                    return tuple(r[k] for r, ks in zip(res, self._out_keys) for k in ks)
-                outs = ckpt(fn, *flat, use_reentrant=False)
-                it = iter(outs)
-                sts = []
-                for ks, ex in zip(self._out_keys, self._out_extra):
-                    d = dict(ex)
-                    for k in ks:
-                        d[k] = next(it)
-                    sts.append(d)
+                spec, flat = _flatten(sts)
+                box = {}
+                outs = _EnsembleStep.apply(self, t, forcings, bc_fns, spec, box, *flat)
+                sts = _unflatten(box["spec"], outs)
             else:
                 sts = self.step_internal(sts, t, forcings, bc_fns)
         return [state_to_reference(s) for s in sts]
+
+
+def _flatten(states):
+    """Member state dicts -> (spec, tensors): spec keeps the keys and the
+    non-tensor entries."""
+    spec, flat = [], []
+    for s in states:
+        keys = sorted(k for k, v in s.items() if torch.is_tensor(v))
+        spec.append((keys, {k: v for k, v in s.items() if not torch.is_tensor(v)}))
+        flat.extend(s[k] for k in keys)
+    return spec, flat
+
+
+def _unflatten(spec, flat):
+    it = iter(flat)
+    out = []
+    for keys, extra in spec:
+        d = dict(extra)
+        for k in keys:
+            d[k] = next(it)
+        out.append(d)
+    return out
+
+
+class _EnsembleStep(torch.autograd.Function):
+    """One lockstep ensemble step as a single autograd node: the forward pass
+    runs without a graph and saves the input states; the backward pass
+    re-runs the step in lockstep with a graph and differentiates it.
+
+    Recomputing inside the node (rather than with torch's non-reentrant
+    checkpoint) keeps the steps apart in the backward pass: autograd orders
+    ready nodes by a per-thread sequence number, and the member threads of
+    every step start that count afresh, so the nodes of different steps would
+    otherwise interleave and several recomputed steps would be alive at once."""
+
+    @staticmethod
+    def forward(ctx, ens, t, forcings, bc_fns, spec, box, *flat):
+        ctx.ens, ctx.t, ctx.forcings, ctx.bc_fns, ctx.spec = ens, t, forcings, bc_fns, spec
+        ctx.save_for_backward(*flat)
+        with torch.no_grad():
+            res = ens.step_internal(_unflatten(spec, flat), t, forcings, bc_fns)
+        ospec, oflat = _flatten(res)
+        ins = {id(x) for x in flat}
+        oflat = [o.clone() if id(o) in ins else o for o in oflat]  # outputs never alias inputs
+        box["spec"] = ospec
+        return tuple(oflat)
+
+    @staticmethod
+    def backward(ctx, *gouts):
+        req = ctx.needs_input_grad[6:]
+        ins = [x.detach().requires_grad_(bool(r)) for x, r in zip(ctx.saved_tensors, req)]
+        with torch.enable_grad():
+            res = ctx.ens.step_internal(_unflatten(ctx.spec, ins), ctx.t, ctx.forcings, ctx.bc_fns)
+        _, oflat = _flatten(res)
+        outs, gs = [], []
+        for o, g in zip(oflat, gouts):
+            if g is not None and o.requires_grad:
+                outs.append(o)
+                gs.append(g)
+        targets = [x for x, r in zip(ins, req) if r]
+        grads = torch.autograd.grad(outs, targets, gs, allow_unused=True) if outs and targets else []
+        it = iter(grads)
+        return (None,) * 6 + tuple(next(it) if r else None for r in req)

@@ -148,3 +148,37 @@ def test_diverging_members_are_reported():
     baton = _Baton(2)
     with pytest.raises(RuntimeError, match="members diverged"):
         run_lockstep(baton, lambda m: baton(m, "a" if m == 0 else "b", m, lambda items: items))
+
+
+class _ToyEnsemble:
+    """Stands in for Ensemble: step_internal over member dicts in lockstep
+    (field 'c' passes through unchanged, 'x' is stepped)."""
+
+    def step_internal(self, states, t, forcings, bc_fns):
+        xs = _toy_step([s["x"] for s in states])
+        return [{"x": x * f, "c": s["c"], "n": s["n"] + 1} for x, s, f in zip(xs, states, forcings)]
+
+
+@pytest.mark.parametrize("steps", [1, 3])
+def test_recomputing_step_node_gradient(steps):
+    from suetes_amd.ensemble import _EnsembleStep, _flatten, _unflatten
+    ens = _ToyEnsemble()
+    xs = _inputs(M=3)
+    cs = [torch.full_like(x, 0.5).requires_grad_(True) for x in xs]
+    forc = [1.0, 0.9, 1.1]
+    sts = [{"x": x, "c": c, "n": 0} for x, c in zip(xs, cs)]
+    for i in range(steps):
+        spec, flat = _flatten(sts)
+        box = {}
+        outs = _EnsembleStep.apply(ens, float(i), forc, None, spec, box, *flat)
+        sts = _unflatten(box["spec"], outs)
+    assert all(s["n"] == steps for s in sts)
+    J = sum((s["x"] ** 2).sum() + (s["c"] * s["x"]).sum() for s in sts)
+    g = torch.autograd.grad(J, xs + cs)
+    ref = [{"x": x, "c": c, "n": 0} for x, c in zip(xs, cs)]
+    for i in range(steps):
+        ref = ens.step_internal(ref, float(i), forc, None)
+    Jr = sum((s["x"] ** 2).sum() + (s["c"] * s["x"]).sum() for s in ref)
+    gr = torch.autograd.grad(Jr, xs + cs)
+    for a, b in zip(g, gr):
+        assert torch.allclose(a, b, rtol=1e-13, atol=1e-300)
