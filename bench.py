@@ -68,7 +68,44 @@ KERNEL_NAMES = {
     "krylov_update": "update_kernel (CGS subtraction + norm)",
     "krylov_normalize": "normalize_gated_kernel (basis vector)",
     "tricubic": "tricubic_kernel (semi-Lagrangian gather)",
+    "helmholtz_T": "helmholtz_T_kernel (operator transpose A^T)",
+    "precond_T": "precond_T_seg_kernel + faces (preconditioner transpose M^T)",
+    "tricubic_adj": "tricubic_adj_kernel (deterministic gather transpose)",
 }
+
+
+def _timer_table():
+    from suetes_amd import _lib
+    table = []
+    for k in _lib.TIMED:
+        for fp32 in (True, False):
+            ms, n, nw, by = _lib.timer_read(k, fp32)
+            if n == 0:
+                continue
+            table.append({"kernel": KERNEL_NAMES[k], "type": "fp32" if fp32 else "fp64",
+                          "launches": n, "working_launches": nw, "total_ms": ms,
+                          "avg_launch_us": 1e3 * ms / n,
+                          "algorithmic_bytes_per_working_launch": by / max(nw, 1),
+                          "achieved_GBs": by / (ms * 1e-3) / 1e9 if ms > 0 else 0.0})
+    table.sort(key=lambda r: -r["total_ms"])
+    return table
+
+
+def instrumented_adjoint(case):
+    """Per-kernel launch timing over one value-and-gradient step (forward and
+    reverse pass; the transposed kernels appear only here)."""
+    import torch
+    from suetes_amd import _lib
+    st = dict(case.state)
+    u0 = st["u"].detach().clone().requires_grad_(True)
+    st["u"] = u0
+    torch.cuda.synchronize()
+    _lib.timer_enable(True)
+    out = case.stepper.step(st, 0.0, case.forcing, case.bc_fn)
+    torch.autograd.grad((out["w"] ** 2).mean(), u0)
+    torch.cuda.synchronize()
+    _lib.timer_enable(False)
+    return _timer_table()
 
 
 def instrumented_step(case, state):
@@ -83,18 +120,11 @@ def instrumented_step(case, state):
     case.stepper.step(state, 0.0, case.forcing, case.bc_fn)
     torch.cuda.synchronize()
     _lib.timer_enable(False)
-    table = []
-    for k in _lib.TIMED:
-        for fp32 in (True, False):
-            ms, n, nw, by = _lib.timer_read(k, fp32)
-            if n == 0:
-                continue
-            table.append({"kernel": KERNEL_NAMES[k], "type": "fp32" if fp32 else "fp64",
-                          "launches": n, "working_launches": nw, "total_ms": ms,
-                          "avg_launch_us": 1e3 * ms / n,
-                          "algorithmic_bytes_per_working_launch": by / max(nw, 1),
-                          "achieved_GBs": by / (ms * 1e-3) / 1e9 if ms > 0 else 0.0})
-    table.sort(key=lambda r: -r["total_ms"])
+    table = _timer_table()
+    return _roofline(table), table
+
+
+def _roofline(table):
     top = table[0]
     roof = {"kernel": f"{top['kernel']} [{top['type']}]", "bound": "hbm",
             "achieved": top["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -103,7 +133,7 @@ def instrumented_step(case, state):
             "algorithmic_bytes_per_launch": top["algorithmic_bytes_per_working_launch"],
             "method": "HIP events around every launch on its stream during one instrumented step; "
                       "achieved = algorithmic bytes of working launches / event time of all launches"}
-    return roof, table
+    return roof
 
 
 def copy_reference(nbytes=1 << 29, reps=5):
@@ -401,6 +431,7 @@ def main():
                    "fp64": {"ratio": r64, "t_forward_s": tf64, "t_value_and_grad_s": tvg64},
                    "speedup_value_and_grad_mixed_vs_fp64": tvg64 / tvg}
             log(f"adjoint/forward {r:.2f} (fp64 {r64:.2f}); value+grad speedup {tvg64 / tvg:.2f}")
+            adj["kernels_value_and_grad_mixed"] = instrumented_adjoint(cmx)
         except Exception as e:  # report, do not hide
             adj = {"error": repr(e)[:300]}
     cpu = None if args.no_cpu else cpu_baseline()

@@ -148,7 +148,8 @@ int sisl_autotune_operator(const SislOperator *op, int32_t fp32, void *x, void *
  * levels per thread, unroll 1 or 2), e.g. to reproduce a tuned choice. */
 int sisl_set_operator_shape(const SislGrid *g, int32_t fp32, int32_t chunk, int32_t unroll);
 /* y = A^T x (the transpose used by the GMRES adjoint solve,
- * steppers.py:473-481). scratch: 3 mass-point arrays. */
+ * steppers.py:473-481).  scratch: not used (may be NULL); kept so that the
+ * transposes of A and M share one signature shape. */
 int sisl_helmholtz_apply_T_f64(const SislOperator *op, const double *x, double *y,
                                double *scratch, void *stream);
 int sisl_helmholtz_apply_T_f32(const SislOperator *op, const float *x, float *y,

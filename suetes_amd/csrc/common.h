@@ -120,7 +120,7 @@ __device__ inline T wave_sum(T v) {
 
 namespace sisl {
 // One member's operator application in a batched launch (device table entry):
-// input x + j * xj, output y, transpose scratch scr (3 mass arrays), skipped
+// input x + j * xj, output y, transpose scratch scr (2 mass arrays), skipped
 // when stop is set and stop[j * sj] != 0.
 template <typename T>
 struct BOp {
@@ -137,15 +137,13 @@ struct BOp {
 template <typename T>
 int launch_A_batched(const BOp<T> *tab, int M, int maxb, int j, hipStream_t s);
 template <typename T>
-int launch_AT_batched(const BOp<T> *tab, int M, int maxb1, int maxb2, int j, hipStream_t s);
+int launch_AT_batched(const BOp<T> *tab, int M, int maxb, int j, hipStream_t s);
 template <typename T>
 int launch_M_batched(const BOp<T> *tab, int M, int maxb, int L, int S, int j, hipStream_t s);
 template <typename T>
 int launch_MT_batched(const BOp<T> *tab, int M, int maxb, int L, int S, int maxf, int j, hipStream_t s);
 // Host-side launch geometry of one member (for maxb).
 int blocks_A(const Stag &s);
-int blocks_AT1(const Stag &s);
-int blocks_AT2(const Stag &s);
 int blocks_M(const Stag &s);
 int blocks_MT_faces(const Stag &s);
 int seg_levels(int nz);  // levels per segment L of the preconditioner (0: too tall)
@@ -158,10 +156,22 @@ void timer_end(int id, hipStream_t s);
 // float and double next to their kernels).
 // `stop` (device int, optional): the kernels return at once when *stop != 0
 // (used to retire the remaining Arnoldi steps of a converged cycle).
+// Fused basis normalisation of an operator launch (FP32 operator only): the
+// input is the unnormalised Arnoldi vector, the outputs are scaled by *inv
+// and the launch stores the basis column vs * x * (*inv) into vout
+// (nb = 1: FP16, nb = 2: FP32).  nb = 0: a plain apply.
+struct Fuse {
+  const double *inv = nullptr;
+  void *vout = nullptr;
+  int nb = 0;
+  double vs = 1.0;
+};
 template <typename T>
-int launch_A(const OpT<T> &op, const T *x, T *y, hipStream_t s, const int *stop = nullptr);
+int launch_A(const OpT<T> &op, const T *x, T *y, hipStream_t s, const int *stop = nullptr,
+             const Fuse *fuse = nullptr);
 template <typename T>
-int launch_AT(const OpT<T> &op, const T *x, T *y, T *scratch, hipStream_t s, const int *stop = nullptr);
+int launch_AT(const OpT<T> &op, const T *x, T *y, T *scratch, hipStream_t s, const int *stop = nullptr,
+              const Fuse *fuse = nullptr);
 template <typename T>
 int launch_M(const OpT<T> &op, const T *r, T *z, hipStream_t s, const int *stop = nullptr);
 template <typename T>

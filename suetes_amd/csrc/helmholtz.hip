@@ -43,12 +43,11 @@ struct AView {
   }
 };
 
-// Launch geometry.  Forward operator: one thread per (i, j) column of a
+// Launch geometry (operator and transpose): one thread per (i, j) column of a
 // 64 x 4 tile, marching a chunk of `kchunk` levels upwards and carrying the
 // vertical neighbours in registers; it writes all five packed outputs of its
 // column, so every input array is read once per launch (plus tile halos, which
-// the XCD-aware tile order keeps in L2).  Transpose kernels: 3-D grids,
-// blockIdx.z enumerating (segment, level).
+// the XCD-aware tile order keeps in L2).
 constexpr int kTX = 64, kTY = 4;
 
 // Blocks of the forward operator at the default chunk (8 level chunks).
@@ -57,11 +56,40 @@ __device__ __forceinline__ int blocks_A_dev(const Stag &s) {
   return ((s.nx + kTX - 1) / kTX) * ((s.ny + kTY - 1) / kTY) * ((s.nz + 1 + kc - 1) / kc);
 }
 
+// Fused basis normalisation of the inner (FP32) Krylov solver.  With NB != 0
+// the input is the unnormalised Arnoldi vector w: the outputs are scaled by
+// 1 / ||w|| (A is linear) and every thread also stores the elements of the
+// normalised basis column vs * w / ||w|| that it owns -- the input elements at
+// its own output positions (NB = 1: FP16 storage, NB = 2: the working type).
+// This replaces a separate normalisation pass over w.
+template <typename T, int NB> struct BasisT { using type = T; };
+template <typename T> struct BasisT<T, 1> { using type = _Float16; };
+#define FUSED_NORMALISATION_SETUP                                                   \
+  using BT = typename BasisT<T, NB>::type;                                          \
+  T osc = 1, bsc = 1;                                                               \
+  BT *BU = nullptr, *BV = nullptr, *BW = nullptr, *BP = nullptr, *BE = nullptr;     \
+  if constexpr (NB != 0) {                                                          \
+    const double iv = *inv;                                                         \
+    osc = (T)iv;                                                                    \
+    bsc = (T)(iv * vs);                                                             \
+    BT *b_ = static_cast<BT *>(vout);                                               \
+    BU = b_ + s.ou; BV = b_ + s.ov; BW = b_ + s.ow; BP = b_ + s.om; BE = b_ + s.oe; \
+  }                                                                                 \
+  auto so = [&](T v) -> T {                                                         \
+    if constexpr (NB != 0) return v * osc;                                          \
+    else return v;                                                                  \
+  };                                                                                \
+  auto bput = [&](BT *b, uint32_t q, T v) {                                         \
+    if constexpr (NB != 0) stg(b, q, (BT)(v * bsc));                                \
+  };
+
 // Body of the forward operator for launch block `blk` of `nblk` (a member of
 // a batched launch passes its own block index and count).
-template <typename T, int UNR>
+template <typename T, int UNR, int NB = 0>
 __device__ __forceinline__ void helmholtz_body(const OpT<T> &o, const T *__restrict__ x,
-                                               T *__restrict__ y, int kchunk, int blk, int nblk) {
+                                               T *__restrict__ y, int kchunk, int blk, int nblk,
+                                               const double *inv = nullptr, void *vout = nullptr,
+                                               double vs = 1.0) {
   const Stag &s = o.s;
   const int nx = s.nx, ny = s.ny, nz = s.nz;
   const int nbx = (nx + kTX - 1) / kTX, nby = (ny + kTY - 1) / kTY;
@@ -73,6 +101,7 @@ __device__ __forceinline__ void helmholtz_body(const OpT<T> &o, const T *__restr
   const int il = i == 0 ? nx - 1 : i - 1, jd = j == 0 ? ny - 1 : j - 1;
   const T *U = x + s.ou, *V = x + s.ov, *W = x + s.ow, *P = x + s.om, *E = x + s.oe;
   T *YU = y + s.ou, *YV = y + s.ov, *YW = y + s.ow, *YP = y + s.om, *YE = y + s.oe;
+  FUSED_NORMALISATION_SETUP
   const bool uform = s.px || i > 0, vform = s.py || j > 0;  // else face 0 is the identity
   const T mudx = ldg(o.mudx, (uint32_t)s.u2(i, j)), afu = ldg(o.afu, (uint32_t)s.u2(i, j));
   const T mvdy = ldg(o.mvdy, (uint32_t)s.v2(i, j)), afv = ldg(o.afv, (uint32_t)s.v2(i, j));
@@ -134,8 +163,10 @@ __device__ __forceinline__ void helmholtz_body(const OpT<T> &o, const T *__restr
     const uint32_t wk = mc + qm;  // own column at w level k (same plane layout)
     const T wv = ldg(W, wk);
     if (k == nz) {  // top w level: both rows are identities
-      stg(YW, wk, wv);
-      stg(YE, wk, e0 * ldg(c_rdzw, wk));
+      stg(YW, wk, so(wv));
+      stg(YE, wk, so(e0 * ldg(c_rdzw, wk)));
+      bput(BW, wk, wv);
+      bput(BE, wk, e0);
       break;
     }
     // ---- mass level k: gz_m at the three columns ----
@@ -165,9 +196,16 @@ __device__ __forceinline__ void helmholtz_body(const OpT<T> &o, const T *__restr
         out = u0 + ldg(c_cua, au) * (mudx * (pc0 - pl0) - ldg(c_zxu, au) * (half * (gzl + gzc))) -
               afu * (half * (vml + vm_k));
       }
-      stg(YU, au, out);
-      if (i == nx - 1 && !s.px) stg(YU, au + 1, u1);   // closed east face: identity
-      if (i == 0 && s.px) stg(YU, ux + qu, out);       // periodic: face nx repeats face 0
+      stg(YU, au, so(out));
+      bput(BU, au, u0);
+      if (i == nx - 1 && !s.px) {                      // closed east face: identity
+        stg(YU, au + 1, so(u1));
+        bput(BU, au + 1, u1);
+      }
+      if (i == 0 && s.px) {                            // periodic: face nx repeats face 0
+        stg(YU, ux + qu, so(out));
+        if constexpr (NB != 0) bput(BU, ux + qu, ldg(U, ux + qu));
+      }
     }
     // ---- L_v at face j (euler.py:399, 416-420) ----
     {
@@ -178,9 +216,16 @@ __device__ __forceinline__ void helmholtz_body(const OpT<T> &o, const T *__restr
         out = v0 + ldg(c_cva, av) * (mvdy * (pc0 - pd0) - ldg(c_zev, av) * (half * (gzd + gzc))) +
               afv * (half * (umd + um_k));
       }
-      stg(YV, av, out);
-      if (j == ny - 1 && !s.py) stg(YV, av + nx, v1);
-      if (j == 0 && s.py) stg(YV, vy + qv, out);
+      stg(YV, av, so(out));
+      bput(BV, av, v0);
+      if (j == ny - 1 && !s.py) {
+        stg(YV, av + nx, so(v1));
+        bput(BV, av + nx, v1);
+      }
+      if (j == 0 && s.py) {
+        stg(YV, vy + qv, so(out));
+        if constexpr (NB != 0) bput(BV, vy + qv, ldg(V, vy + qv));
+      }
     }
     // ---- L_pi (euler.py:298-318, 401) ----
     const uint32_t w1 = wk + pm;
@@ -189,17 +234,20 @@ __device__ __forceinline__ void helmholtz_body(const OpT<T> &o, const T *__restr
     {
       const T fx = u1 * ldg(c_cx, au + 1) - u0 * ldg(c_cx, au);
       const T fy = v1 * ldg(c_cy, av + nx) - v0 * ldg(c_cy, av);
-      stg(YP, wk, pc0 + ldg(c_cpm, wk) * (mmx * fx + mmy * fy + (fz1 - fz0)));
+      stg(YP, wk, so(pc0 + ldg(c_cpm, wk) * (mmx * fx + mmy * fy + (fz1 - fz0))));
+      bput(BP, wk, pc0);
     }
     // ---- w level k: L_w (euler.py:400, 426-427), L_eta (euler.py:429-437) ----
     if (k == 0) {
-      stg(YW, wk, wv);
-      stg(YE, wk, e0 * ldg(c_rdzw, wk));
+      stg(YW, wk, so(wv));
+      stg(YE, wk, so(e0 * ldg(c_rdzw, wk)));
     } else {
-      stg(YW, wk, ldg(c_dw, wk) * wv + ldg(c_cw, wk) * (pc0 - pcm));
-      stg(YE, wk, alpha * (e0 + ldg(c_kx, wk) * (half * (umc_p + um_k)) +
-                           ldg(c_ky, wk) * (half * (vmc_p + vm_k)) - wv));
+      stg(YW, wk, so(ldg(c_dw, wk) * wv + ldg(c_cw, wk) * (pc0 - pcm)));
+      stg(YE, wk, so(alpha * (e0 + ldg(c_kx, wk) * (half * (umc_p + um_k)) +
+                              ldg(c_ky, wk) * (half * (vmc_p + vm_k)) - wv)));
     }
+    bput(BW, wk, wv);
+    bput(BE, wk, e0);
     // k+1 becomes k
     gc0 = gc1; gl0 = gl1; gd0 = gd1;
     pcm = pc0; plm = pl0; pdm = pd0;
@@ -209,256 +257,381 @@ __device__ __forceinline__ void helmholtz_body(const OpT<T> &o, const T *__restr
   }
 }
 
-template <typename T, int UNR>
+template <typename T, int UNR, int NB>
 __global__ void __launch_bounds__(256) helmholtz_kernel(OpT<T> o, const T *__restrict__ x,
                                                         T *__restrict__ y, int kchunk,
-                                                        const int *__restrict__ stop) {
+                                                        const int *__restrict__ stop,
+                                                        const double *__restrict__ inv, void *vout,
+                                                        double vs) {
   if (stop && *stop) return;
-  helmholtz_body<T, UNR>(o, x, y, kchunk, blockIdx.x, gridDim.x);
+  helmholtz_body<T, UNR, NB>(o, x, y, kchunk, blockIdx.x, gridDim.x, inv, vout, vs);
 }
 
 
-inline dim3 point_grid(const Stag &s, int segments) {
-  return dim3((unsigned)((s.nx + 1 + kTX - 1) / kTX), (unsigned)((s.ny + 1 + kTY - 1) / kTY),
-              (unsigned)(segments * (s.nz + 1)));
-}
-
-// ---- transpose ---------------------------------------------------------------
-// Phase 1 (per mass point): cotangents of the mass-point intermediates
-// gz_m, v_m, u_m.  Phase 2 (per packed element): gather into A^T y.
-
-template <typename T>
-struct ATView {
-  const OpT<T> &o;
-  const T *U, *V, *W, *P, *E;  // cotangents of L_u .. L_eta
-  __device__ ATView(const OpT<T> &op, const T *y)
-      : o(op), U(y + op.s.ou), V(y + op.s.ov), W(y + op.s.ow), P(y + op.s.om), E(y + op.s.oe) {}
-  // effective cotangent of the formula at u face f (periodic faces 0 / nx merged)
-  __device__ __forceinline__ T ut(int f, int j, int k) const {
-    const Stag &s = o.s;
-    if (s.px) {
-      if (f == s.nx) f = 0;
-      if (f == 0) return U[s.u.at(0, j, k)] + U[s.u.at(s.nx, j, k)];
-      return U[s.u.at(f, j, k)];
-    }
-    if (f == 0 || f == s.nx) return (T)0;
-    return U[s.u.at(f, j, k)];
-  }
-  __device__ __forceinline__ T vt(int i, int f, int k) const {
-    const Stag &s = o.s;
-    if (s.py) {
-      if (f == s.ny) f = 0;
-      if (f == 0) return V[s.v.at(i, 0, k)] + V[s.v.at(i, s.ny, k)];
-      return V[s.v.at(i, f, k)];
-    }
-    if (f == 0 || f == s.ny) return (T)0;
-    return V[s.v.at(i, f, k)];
-  }
-  __device__ __forceinline__ int ufold(int f) const { return (o.s.px && f == o.s.nx) ? 0 : f; }
-  __device__ __forceinline__ int vfold(int f) const { return (o.s.py && f == o.s.ny) ? 0 : f; }
-};
-
-template <typename T>
-__device__ __forceinline__ void helmholtz_T1_body(const OpT<T> &o, const T *__restrict__ y,
-                                                  T *__restrict__ gzm_b, T *__restrict__ um_b,
-                                                  T *__restrict__ vm_b, int bx, int by, int bz) {
+// ---- fused transpose -----------------------------------------------------------
+// x = A^T y in gather form, one thread per (i, j) column marching a chunk of
+// levels like the forward operator.  Per mass level the thread needs the
+// cotangents of three mass-point intermediates of the forward formulas:
+//   gzm_b = -1/2 sum over the adjacent u / v faces of (formula cotangent)
+//           * cua zxu (cva zev)                  (own cell, levels k-1..k+1)
+//   um_b  = 1/2 (adjacent v faces) * afv + alpha/2 (eta_dot rows k, k+1) * kx
+//                                                (own cell and the one to the left)
+//   vm_b  = -1/2 (adjacent u faces) * afu + alpha/2 (eta_dot rows k, k+1) * ky
+//                                                (own cell and the one below)
+// The own-cell values are carried up the column; the neighbours' are formed
+// from loads that neighbouring lanes also make (cache hits).  Outputs per
+// level: u face i, v face j (and the east / north faces at the domain edge),
+// pi, w and eta_dot of the column.
+template <typename T, int UNR, int NB = 0>
+__device__ __forceinline__ void helmholtz_T_body(const OpT<T> &o, const T *__restrict__ y,
+                                                 T *__restrict__ x, int kchunk, int blk, int nblk,
+                                                 const double *inv = nullptr, void *vout = nullptr,
+                                                 double vs = 1.0) {
   const Stag &s = o.s;
   const int nx = s.nx, ny = s.ny, nz = s.nz;
-  const int i = bx * kTX + threadIdx.x, j = by * kTY + threadIdx.y, k = bz;
-  if (i >= nx || j >= ny || k >= nz) return;
-  const int64_t p = s.m.at(i, j, k);
-  ATView<T> t(o, y);
-  const int fl = i, fr = t.ufold(i + 1);
-  const T ul = t.ut(fl, j, k), ur = t.ut(i + 1, j, k);
-  const int64_t pl = s.u.at(fl, j, k), pr = s.u.at(fr, j, k);
-  const int gl = j, gr = t.vfold(j + 1);
-  const T vl = t.vt(i, gl, k), vr = t.vt(i, j + 1, k);
-  const int64_t ql = s.v.at(i, gl, k), qr = s.v.at(i, gr, k);
-  T g = (T)-0.5 * (ul * o.cua[pl] * o.zxu[pl] + ur * o.cua[pr] * o.zxu[pr]);
-  g += (T)-0.5 * (vl * o.cva[ql] * o.zev[ql] + vr * o.cva[qr] * o.zev[qr]);
-  T vmb = (T)-0.5 * (ul * o.afu[s.u2(fl, j)] + ur * o.afu[s.u2(fr, j)]);
-  T umb = (T)0.5 * (vl * o.afv[s.v2(i, gl)] + vr * o.afv[s.v2(i, gr)]);
-  // L_eta interior levels k and k+1 read u_m/v_m at mass level k
-  T ek = 0, ek1 = 0, kxk = 0, kxk1 = 0, kyk = 0, kyk1 = 0;
-  if (k >= 1) { const int64_t w = s.w.at(i, j, k); ek = t.E[w]; kxk = o.kx[w]; kyk = o.ky[w]; }
-  if (k + 1 <= nz - 1) { const int64_t w = s.w.at(i, j, k + 1); ek1 = t.E[w]; kxk1 = o.kx[w]; kyk1 = o.ky[w]; }
-  umb += (T)0.5 * o.alpha * (ek * kxk + ek1 * kxk1);
-  vmb += (T)0.5 * o.alpha * (ek * kyk + ek1 * kyk1);
-  gzm_b[p] = g;
-  um_b[p] = umb;
-  vm_b[p] = vmb;
-}
-
-template <typename T>
-__global__ void __launch_bounds__(256) helmholtz_T1_kernel(OpT<T> o, const T *__restrict__ y,
-                                                           T *__restrict__ gzm_b,
-                                                           T *__restrict__ um_b,
-                                                           T *__restrict__ vm_b,
-                                                           const int *__restrict__ stop) {
-  if (stop && *stop) return;
-  helmholtz_T1_body(o, y, gzm_b, um_b, vm_b, blockIdx.x, blockIdx.y, blockIdx.z);
-}
-
-template <typename T>
-__device__ __forceinline__ void helmholtz_T2_body(const OpT<T> &o, const T *__restrict__ y,
-                                                  const T *__restrict__ gzm_b,
-                                                  const T *__restrict__ um_b,
-                                                  const T *__restrict__ vm_b, T *__restrict__ xo,
-                                                  int bx, int by, int bz) {
-  const Stag &s = o.s;
-  const int nx = s.nx, ny = s.ny, nz = s.nz;
-  const int seg = bz / (nz + 1), k = bz - seg * (nz + 1);
+  const int nbx = (nx + kTX - 1) / kTX, nby = (ny + kTY - 1) / kTY;
+  const int tile = xcd_tile(blk, nblk);
+  const int bx = tile % nbx, by = (tile / nbx) % nby, bz = tile / (nbx * nby);
   const int i = bx * kTX + threadIdx.x, j = by * kTY + threadIdx.y;
-  ATView<T> t(o, y);
-  T out;
-  int64_t q;
-  if (seg == 0) {  // ---- U bar ----
-    if (i > nx || j >= ny || k >= nz) return;
-    const int64_t p = s.u.at(i, j, k);
-    q = s.ou + p;
-    if (s.px) out = (i == 0) ? t.U[p] + t.U[s.u.at(nx, j, k)] : (i == nx ? (T)0 : t.U[p]);
-    else out = t.U[p];
-    T fl = 0;
-    if (i - 1 >= 0) { const int64_t m = s.m.at(i - 1, j, k); fl += o.cpm[m] * o.mmx[s.m2(i - 1, j)] * t.P[m]; }
-    if (i <= nx - 1) { const int64_t m = s.m.at(i, j, k); fl -= o.cpm[m] * o.mmx[s.m2(i, j)] * t.P[m]; }
-    out += o.cx[p] * fl;
-    T ub = 0;
-    if (i - 1 >= 0) ub += um_b[s.m.at(i - 1, j, k)];
-    if (i <= nx - 1) ub += um_b[s.m.at(i, j, k)];
-    out += (T)0.5 * ub;
-  } else if (seg == 1) {  // ---- V bar ----
-    if (i >= nx || j > ny || k >= nz) return;
-    const int64_t p = s.v.at(i, j, k);
-    q = s.ov + p;
-    if (s.py) out = (j == 0) ? t.V[p] + t.V[s.v.at(i, ny, k)] : (j == ny ? (T)0 : t.V[p]);
-    else out = t.V[p];
-    T fl = 0;
-    if (j - 1 >= 0) { const int64_t m = s.m.at(i, j - 1, k); fl += o.cpm[m] * o.mmy[s.m2(i, j - 1)] * t.P[m]; }
-    if (j <= ny - 1) { const int64_t m = s.m.at(i, j, k); fl -= o.cpm[m] * o.mmy[s.m2(i, j)] * t.P[m]; }
-    out += o.cy[p] * fl;
-    T vb = 0;
-    if (j - 1 >= 0) vb += vm_b[s.m.at(i, j - 1, k)];
-    if (j <= ny - 1) vb += vm_b[s.m.at(i, j, k)];
-    out += (T)0.5 * vb;
-  } else if (seg == 2) {  // ---- W bar ----
-    if (i >= nx || j >= ny) return;
-    const int64_t p = s.w.at(i, j, k);
-    q = s.ow + p;
-    if (k == 0 || k == nz) out = t.W[p];
-    else out = o.dw[p] * t.W[p] - o.alpha * t.E[p];
-  } else if (seg == 3) {  // ---- P bar ----
-    if (i >= nx || j >= ny || k >= nz) return;
-    const int64_t p = s.m.at(i, j, k);
-    q = s.om + p;
-    out = t.P[p];
-    {  // x faces i (P is the upper cell) and i+1 (P is the lower cell)
-      const int64_t a = s.u.at(i, j, k), b = s.u.at(t.ufold(i + 1), j, k);
-      out += t.ut(i, j, k) * o.cua[a] * o.mudx[s.u2(i, j)];
-      out -= t.ut(i + 1, j, k) * o.cua[b] * o.mudx[s.u2(t.ufold(i + 1), j)];
-    }
-    {
-      const int64_t a = s.v.at(i, j, k), b = s.v.at(i, t.vfold(j + 1), k);
-      out += t.vt(i, j, k) * o.cva[a] * o.mvdy[s.v2(i, j)];
-      out -= t.vt(i, j + 1, k) * o.cva[b] * o.mvdy[s.v2(i, t.vfold(j + 1))];
-    }
-    if (k >= 1) { const int64_t w = s.w.at(i, j, k); out += o.cw[w] * t.W[w]; }
-    if (k + 1 <= nz - 1) { const int64_t w = s.w.at(i, j, k + 1); out -= o.cw[w] * t.W[w]; }
-    // gz_w cotangent at level kk: 0.5 (gzm_b(kk-1) + gzm_b(kk))
-    auto gzwb = [&](int kk) {
-      T r = 0;
-      if (kk - 1 >= 0) r += gzm_b[s.m.at(i, j, kk - 1)];
-      if (kk <= nz - 1) r += gzm_b[s.m.at(i, j, kk)];
-      return (T)0.5 * r * o.rdzw[s.w.at(i, j, kk)];
-    };
-    // levels whose upper operand is pi(k)
-    if (k >= 1 && k <= nz - 1) out += gzwb(k);
-    if (k == 1) out += gzwb(0);
-    if (k == nz - 1) out += gzwb(nz);
-    // levels whose lower operand is pi(k)
-    if (k + 1 >= 1 && k + 1 <= nz - 1) out -= gzwb(k + 1);
-    if (k == 0) out -= gzwb(0);
-    if (k == nz - 2) out -= gzwb(nz);
-  } else {  // ---- E bar ----
-    if (i >= nx || j >= ny) return;
-    const int64_t p = s.w.at(i, j, k);
-    q = s.oe + p;
-    if (k == 0 || k == nz) {
-      out = t.E[p] * o.rdzw[p];
-    } else {
-      out = o.alpha * t.E[p] +
-            o.cz[p] * (o.cpm[s.m.at(i, j, k - 1)] * t.P[s.m.at(i, j, k - 1)] - o.cpm[s.m.at(i, j, k)] * t.P[s.m.at(i, j, k)]);
-    }
+  if (i >= nx || j >= ny) return;
+  const int k0 = bz * kchunk, k1 = min(k0 + kchunk, nz + 1);
+  const T *YU = y + s.ou, *YV = y + s.ov, *YW = y + s.ow, *YP = y + s.om, *YE = y + s.oe;
+  T *XU = x + s.ou, *XV = x + s.ov, *XW = x + s.ow, *XP = x + s.om, *XE = x + s.oe;
+  FUSED_NORMALISATION_SETUP
+  const bool px = s.px, py = s.py;
+  const bool hl = i >= 1, hd = j >= 1;  // the cell to the left / below exists
+  const T half = (T)0.5, mhalf = (T)-0.5, alpha = o.alpha, ha = half * o.alpha;
+  const T *__restrict__ c_cua = o.cua;
+  const T *__restrict__ c_zxu = o.zxu;
+  const T *__restrict__ c_cx = o.cx;
+  const T *__restrict__ c_cva = o.cva;
+  const T *__restrict__ c_zev = o.zev;
+  const T *__restrict__ c_cy = o.cy;
+  const T *__restrict__ c_dw = o.dw;
+  const T *__restrict__ c_cw = o.cw;
+  const T *__restrict__ c_cz = o.cz;
+  const T *__restrict__ c_kx = o.kx;
+  const T *__restrict__ c_ky = o.ky;
+  const T *__restrict__ c_rdzw = o.rdzw;
+  const T *__restrict__ c_cpm = o.cpm;
+  const uint32_t pm = (uint32_t)nx * ny, pu = (uint32_t)(nx + 1) * ny, pv = (uint32_t)nx * (ny + 1);
+  const uint32_t mc = (uint32_t)j * nx + i;
+  const uint32_t uc = (uint32_t)j * (nx + 1) + i, vc = (uint32_t)j * nx + i;
+  const uint32_t u0r = (uint32_t)j * (nx + 1);     // face 0 of row j
+  const uint32_t v0c = (uint32_t)i;                // face 0 of column i
+  // folded face indices of the face after the cell (periodic: face nx is face 0)
+  const int fr = (px && i + 1 == nx) ? 0 : i + 1;
+  const int gr = (py && j + 1 == ny) ? 0 : j + 1;
+  // 2-D coefficients: own faces, the faces of the left cell's rows, the lower row
+  const T mudx_i = ldg(o.mudx, (uint32_t)s.u2(i, j)), mudx_r = ldg(o.mudx, (uint32_t)s.u2(fr, j));
+  const T afu_i = ldg(o.afu, (uint32_t)s.u2(i, j)), afu_r = ldg(o.afu, (uint32_t)s.u2(fr, j));
+  const T mvdy_j = ldg(o.mvdy, (uint32_t)s.v2(i, j)), mvdy_r = ldg(o.mvdy, (uint32_t)s.v2(i, gr));
+  const T afv_j = ldg(o.afv, (uint32_t)s.v2(i, j)), afv_r = ldg(o.afv, (uint32_t)s.v2(i, gr));
+  const T afv_lj = hl ? ldg(o.afv, (uint32_t)s.v2(i - 1, j)) : (T)0;
+  const T afv_lr = hl ? ldg(o.afv, (uint32_t)s.v2(i - 1, gr)) : (T)0;
+  const T afu_dj = hd ? ldg(o.afu, (uint32_t)s.u2(i, j - 1)) : (T)0;
+  const T afu_dr = hd ? ldg(o.afu, (uint32_t)s.u2(fr, j - 1)) : (T)0;
+  const T mmx_c = ldg(o.mmx, (uint32_t)s.m2(i, j)), mmy_c = ldg(o.mmy, (uint32_t)s.m2(i, j));
+  const T mmx_l = hl ? ldg(o.mmx, (uint32_t)s.m2(i - 1, j)) : (T)0;
+  const T mmy_d = hd ? ldg(o.mmy, (uint32_t)s.m2(i, j - 1)) : (T)0;
+  // Formula cotangent of u face f of row `row` (level offset q): faces 0 / nx are
+  // identities unless periodic, where both carry the merged cotangent.
+  auto ut = [&](int f, uint32_t row, uint32_t q) -> T {
+    if (f == 0 || f == nx) return px ? ldg(YU, row + q) + ldg(YU, row + nx + q) : (T)0;
+    return ldg(YU, row + f + q);
+  };
+  auto vt = [&](int ii, int f, uint32_t q) -> T {
+    if (f == 0 || f == ny) return py ? ldg(YV, (uint32_t)ii + q) + ldg(YV, (uint32_t)ny * nx + ii + q) : (T)0;
+    return ldg(YV, (uint32_t)f * nx + ii + q);
+  };
+  // gzm_b of the own cell at mass level k from the four face cotangents
+  auto gzmb = [&](T uti, T utr, T vtj, T vtr, int k) -> T {
+    const uint32_t qu = (uint32_t)k * pu, qv = (uint32_t)k * pv;
+    const uint32_t al = uc + qu, ar = u0r + fr + qu, bl = vc + qv, br = (uint32_t)gr * nx + i + qv;
+    T g = mhalf * (uti * ldg(c_cua, al) * ldg(c_zxu, al) + utr * ldg(c_cua, ar) * ldg(c_zxu, ar));
+    g += mhalf * (vtj * ldg(c_cva, bl) * ldg(c_zev, bl) + vtr * ldg(c_cva, br) * ldg(c_zev, br));
+    return g;
+  };
+  // carried state
+  T ut_i = 0, ut_r = 0, vt_j = 0, vt_r = 0;  // own formula cotangents at level k
+  T gm1 = 0, g0 = 0;                          // gzm_b at k-1 and k
+  T cpmP_m1 = 0;                              // cpm * pibar at mass level k-1
+  T e0 = 0, e0l = 0, e0d = 0;                 // eta_dot cotangent at w level k (interior only)
+  T kx0 = 0, kx0l = 0, ky0 = 0, ky0d = 0;     // kx, ky at w level k
+  T cwW0 = 0;                                 // cw * wbar at w level k (interior only)
+  if (k0 < nz) {
+    const uint32_t qu = (uint32_t)k0 * pu, qv = (uint32_t)k0 * pv;
+    ut_i = ut(i, u0r, qu); ut_r = ut(i + 1, u0r, qu);
+    vt_j = vt(i, j, qv); vt_r = vt(i, j + 1, qv);
+    g0 = gzmb(ut_i, ut_r, vt_j, vt_r, k0);
   }
-  xo[q] = out;
+  if (k0 >= 1) {
+    const int km = k0 - 1;
+    const uint32_t qu = (uint32_t)km * pu, qv = (uint32_t)km * pv;
+    gm1 = gzmb(ut(i, u0r, qu), ut(i + 1, u0r, qu), vt(i, j, qv), vt(i, j + 1, qv), km);
+    const uint32_t m = mc + (uint32_t)km * pm;
+    cpmP_m1 = ldg(c_cpm, m) * ldg(YP, m);
+  }
+  if (k0 >= 1 && k0 <= nz - 1) {
+    const uint32_t w = mc + (uint32_t)k0 * pm;
+    e0 = ldg(YE, w);
+    if (hl) e0l = ldg(YE, w - 1);
+    if (hd) e0d = ldg(YE, w - nx);
+    cwW0 = ldg(c_cw, w) * ldg(YW, w);
+  }
+  if (k0 <= nz) {
+    const uint32_t w = mc + (uint32_t)k0 * pm;
+    kx0 = ldg(c_kx, w); ky0 = ldg(c_ky, w);
+    if (hl) kx0l = ldg(c_kx, w - 1);
+    if (hd) ky0d = ldg(c_ky, w - nx);
+  }
+#pragma unroll UNR
+  for (int k = k0; k < k1; ++k) {
+    const uint32_t qm = (uint32_t)k * pm, qu = (uint32_t)k * pu, qv = (uint32_t)k * pv;
+    const uint32_t wk = mc + qm;  // own column at w level k and at mass level k
+    if (k == nz) {  // top w level: identity rows
+      const T yw = ldg(YW, wk), ye = ldg(YE, wk);
+      stg(XW, wk, so(yw));
+      stg(XE, wk, so(ye * ldg(c_rdzw, wk)));
+      bput(BW, wk, yw);
+      bput(BE, wk, ye);
+      break;
+    }
+    // ---- level k+1 data (carried into the next iteration) ----
+    T ut_i1 = 0, ut_r1 = 0, vt_j1 = 0, vt_r1 = 0, g1 = 0;
+    T e1 = 0, e1l = 0, e1d = 0, cwW1 = 0;
+    const uint32_t w1 = wk + pm;
+    if (k + 1 < nz) {
+      const uint32_t qu1 = qu + pu, qv1 = qv + pv;
+      ut_i1 = ut(i, u0r, qu1); ut_r1 = ut(i + 1, u0r, qu1);
+      vt_j1 = vt(i, j, qv1); vt_r1 = vt(i, j + 1, qv1);
+      g1 = gzmb(ut_i1, ut_r1, vt_j1, vt_r1, k + 1);
+      e1 = ldg(YE, w1);
+      if (hl) e1l = ldg(YE, w1 - 1);
+      if (hd) e1d = ldg(YE, w1 - nx);
+      cwW1 = ldg(c_cw, w1) * ldg(YW, w1);
+    }
+    const T kx1 = ldg(c_kx, w1), ky1 = ldg(c_ky, w1);
+    const T kx1l = hl ? ldg(c_kx, w1 - 1) : (T)0, ky1d = hd ? ldg(c_ky, w1 - nx) : (T)0;
+    // ---- mass-point intermediates at level k ----
+    const T umb = half * (vt_j * afv_j + vt_r * afv_r) + ha * (e0 * kx0 + e1 * kx1);
+    const T vmb = mhalf * (ut_i * afu_i + ut_r * afu_r) + ha * (e0 * ky0 + e1 * ky1);
+    T umb_l = 0, vmb_d = 0;
+    if (hl) umb_l = half * (vt(i - 1, j, qv) * afv_lj + vt(i - 1, j + 1, qv) * afv_lr) + ha * (e0l * kx0l + e1l * kx1l);
+    if (hd) vmb_d = mhalf * (ut(i, u0r - (nx + 1), qu) * afu_dj + ut(i + 1, u0r - (nx + 1), qu) * afu_dr) +
+                    ha * (e0d * ky0d + e1d * ky1d);
+    const T yp = ldg(YP, wk);
+    const T cpmP = ldg(c_cpm, wk) * yp;
+    const T cpmPl = hl ? ldg(c_cpm, wk - 1) * ldg(YP, wk - 1) : (T)0;
+    const T cpmPd = hd ? ldg(c_cpm, wk - nx) * ldg(YP, wk - nx) : (T)0;
+    // ---- u face i (and the east face) ----
+    {
+      const uint32_t au = uc + qu;
+      const T yu = ldg(YU, au);
+      bput(BU, au, yu);
+      T idp = yu;
+      if (px && i == 0) idp = yu + ldg(YU, au + nx);
+      T fl = 0;
+      if (hl) fl += cpmPl * mmx_l;
+      fl -= cpmP * mmx_c;
+      T out = idp + ldg(c_cx, au) * fl;
+      T ub = 0;
+      if (hl) ub += umb_l;
+      ub += umb;
+      stg(XU, au, so(out + half * ub));
+      if (i == nx - 1) {
+        const T yn = ldg(YU, au + 1);
+        bput(BU, au + 1, yn);
+        const T idn = px ? (T)0 : yn;
+        stg(XU, au + 1, so(idn + ldg(c_cx, au + 1) * (cpmP * mmx_c) + half * umb));
+      }
+    }
+    // ---- v face j (and the north face) ----
+    {
+      const uint32_t av = vc + qv;
+      const T yv = ldg(YV, av);
+      bput(BV, av, yv);
+      T idp = yv;
+      if (py && j == 0) idp = yv + ldg(YV, av + (uint32_t)ny * nx);
+      T fl = 0;
+      if (hd) fl += cpmPd * mmy_d;
+      fl -= cpmP * mmy_c;
+      T out = idp + ldg(c_cy, av) * fl;
+      T vb = 0;
+      if (hd) vb += vmb_d;
+      vb += vmb;
+      stg(XV, av, so(out + half * vb));
+      if (j == ny - 1) {
+        const T yn = ldg(YV, av + nx);
+        bput(BV, av + nx, yn);
+        const T idn = py ? (T)0 : yn;
+        stg(XV, av + nx, so(idn + ldg(c_cy, av + nx) * (cpmP * mmy_c) + half * vmb));
+      }
+    }
+    // ---- pi ----
+    {
+      const uint32_t al = uc + qu, ar = u0r + fr + qu, bl = vc + qv, br = (uint32_t)gr * nx + i + qv;
+      T out = yp;
+      out += ut_i * ldg(c_cua, al) * mudx_i;
+      out -= ut_r * ldg(c_cua, ar) * mudx_r;
+      out += vt_j * ldg(c_cva, bl) * mvdy_j;
+      out -= vt_r * ldg(c_cva, br) * mvdy_r;
+      if (k >= 1) out += cwW0;
+      if (k + 1 <= nz - 1) out -= cwW1;
+      // pressure-gradient levels: gz_w(kk) = (pi(kk) - pi(kk-1)) rdzw(kk) with the edge closures
+      if (k >= 1) out += half * (gm1 + g0) * ldg(c_rdzw, wk);
+      if (k == 1) out += half * gm1 * ldg(c_rdzw, mc);
+      if (k == nz - 1) out += half * g0 * ldg(c_rdzw, mc + (uint32_t)nz * pm);
+      if (k + 1 <= nz - 1) out -= half * (g0 + g1) * ldg(c_rdzw, w1);
+      if (k == 0) out -= half * g0 * ldg(c_rdzw, mc);
+      if (k == nz - 2) out -= half * g1 * ldg(c_rdzw, mc + (uint32_t)nz * pm);
+      stg(XP, wk, so(out));
+      bput(BP, wk, yp);
+    }
+    // ---- w level k: w and eta_dot ----
+    {
+      const T yw = ldg(YW, wk);
+      bput(BW, wk, yw);
+      if (k == 0) {
+        const T ye = ldg(YE, wk);
+        bput(BE, wk, ye);
+        stg(XW, wk, so(yw));
+        stg(XE, wk, so(ye * ldg(c_rdzw, wk)));
+      } else {
+        bput(BE, wk, e0);
+        stg(XW, wk, so(ldg(c_dw, wk) * yw - alpha * e0));
+        stg(XE, wk, so(alpha * e0 + ldg(c_cz, wk) * (cpmP_m1 - cpmP)));
+      }
+    }
+    // k+1 becomes k
+    ut_i = ut_i1; ut_r = ut_r1; vt_j = vt_j1; vt_r = vt_r1;
+    gm1 = g0; g0 = g1;
+    cpmP_m1 = cpmP;
+    e0 = e1; e0l = e1l; e0d = e1d;
+    kx0 = kx1; kx0l = kx1l; ky0 = ky1; ky0d = ky1d;
+    cwW0 = cwW1;
+  }
 }
 
-template <typename T>
-__global__ void __launch_bounds__(256) helmholtz_T2_kernel(OpT<T> o, const T *__restrict__ y,
-                                                           const T *__restrict__ gzm_b,
-                                                           const T *__restrict__ um_b,
-                                                           const T *__restrict__ vm_b,
-                                                           T *__restrict__ xo,
-                                                           const int *__restrict__ stop) {
+template <typename T, int UNR, int NB>
+__global__ void __launch_bounds__(256) helmholtz_T_kernel(OpT<T> o, const T *__restrict__ y,
+                                                          T *__restrict__ x, int kchunk,
+                                                          const int *__restrict__ stop,
+                                                          const double *__restrict__ inv, void *vout,
+                                                          double vs) {
   if (stop && *stop) return;
-  helmholtz_T2_body(o, y, gzm_b, um_b, vm_b, xo, blockIdx.x, blockIdx.y, blockIdx.z);
+  helmholtz_T_body<T, UNR, NB>(o, y, x, kchunk, blockIdx.x, gridDim.x, inv, vout, vs);
 }
 
-// Launch shape of the forward operator: levels per thread (chunk) and the
-// unroll of the level loop.  Every shape computes bit-identical results; the
-// autotuner below picks one per grid and element type by timing, the
-// default is 8 chunks per column without unrolling.
+// Launch shape of the operator and of its transpose: levels per thread
+// (chunk) and the unroll of the level loop.  Every shape computes
+// bit-identical results; the autotuner below picks one per grid, element type
+// and direction by timing, the default is 8 chunks per column without
+// unrolling.
 struct AShape {
   int kc, unr;
 };
 static std::mutex g_tune_mu;
-static std::map<std::tuple<int, int, int, int>, AShape> g_tune;
+static std::map<std::tuple<int, int, int, int, int>, AShape> g_tune;  // (nx, ny, nz, bytes, transpose)
 
-static AShape a_shape(const Stag &s, int bytes) {
+static AShape a_shape(const Stag &s, int bytes, int tr = 0) {
   std::lock_guard<std::mutex> g(g_tune_mu);
-  auto it = g_tune.find({s.nx, s.ny, s.nz, bytes});
+  auto it = g_tune.find({s.nx, s.ny, s.nz, bytes, tr});
   if (it != g_tune.end()) return it->second;
   return AShape{(s.nz + 1 + 7) / 8, 1};
 }
 
-template <typename T>
-static int launch_A_shape(const OpT<T> &o, const T *x, T *y, hipStream_t st, const int *stop, AShape sh) {
+// Launch of kernel K<T, UNR, NB> (operator or transpose) in a given shape;
+// NB != 0 (fused basis normalisation) exists for the FP32 operator only.
+template <typename T, int UNR, int NB>
+struct AKern {
+  static void go(dim3 g, hipStream_t st, const OpT<T> &o, const T *x, T *y, int kc, const int *stop,
+                 const Fuse &f) {
+    hipLaunchKernelGGL((helmholtz_kernel<T, UNR, NB>), g, dim3(kTX, kTY), 0, st, o, x, y, kc, stop, f.inv,
+                       f.vout, f.vs);
+  }
+};
+template <typename T, int UNR, int NB>
+struct ATKern {
+  static void go(dim3 g, hipStream_t st, const OpT<T> &o, const T *x, T *y, int kc, const int *stop,
+                 const Fuse &f) {
+    hipLaunchKernelGGL((helmholtz_T_kernel<T, UNR, NB>), g, dim3(kTX, kTY), 0, st, o, x, y, kc, stop,
+                       f.inv, f.vout, f.vs);
+  }
+};
+template <template <typename, int, int> class K, typename T>
+static int launch_shape(const OpT<T> &o, const T *x, T *y, hipStream_t st, const int *stop, AShape sh,
+                        const Fuse &f) {
   const int kc = sh.kc;
-  const int nb = ((o.s.nx + kTX - 1) / kTX) * ((o.s.ny + kTY - 1) / kTY) * ((o.s.nz + 1 + kc - 1) / kc);
-  if (sh.unr == 2)
-    hipLaunchKernelGGL((helmholtz_kernel<T, 2>), dim3(nb), dim3(kTX, kTY), 0, st, o, x, y, kc, stop);
-  else
-    hipLaunchKernelGGL((helmholtz_kernel<T, 1>), dim3(nb), dim3(kTX, kTY), 0, st, o, x, y, kc, stop);
-  return SISL_OK;
+  const dim3 g((unsigned)(((o.s.nx + kTX - 1) / kTX) * ((o.s.ny + kTY - 1) / kTY) *
+                          ((o.s.nz + 1 + kc - 1) / kc)));
+  if (f.nb == 0) {
+    if (sh.unr == 2) K<T, 2, 0>::go(g, st, o, x, y, kc, stop, f);
+    else K<T, 1, 0>::go(g, st, o, x, y, kc, stop, f);
+    return SISL_OK;
+  }
+  if constexpr (sizeof(T) == 4) {
+    if (f.nb == 1) {
+      if (sh.unr == 2) K<T, 2, 1>::go(g, st, o, x, y, kc, stop, f);
+      else K<T, 1, 1>::go(g, st, o, x, y, kc, stop, f);
+      return SISL_OK;
+    }
+    if (f.nb == 2) {
+      if (sh.unr == 2) K<T, 2, 2>::go(g, st, o, x, y, kc, stop, f);
+      else K<T, 1, 2>::go(g, st, o, x, y, kc, stop, f);
+      return SISL_OK;
+    }
+  }
+  return SISL_ERR_ARG;
+}
+template <typename T>
+static int launch_A_shape(const OpT<T> &o, const T *x, T *y, hipStream_t st, const int *stop, AShape sh,
+                          const Fuse &f = Fuse{}) {
+  return launch_shape<AKern, T>(o, x, y, st, stop, sh, f);
 }
 
 template <typename T>
-int launch_A(const OpT<T> &o, const T *x, T *y, hipStream_t st, const int *stop) {
+int launch_A(const OpT<T> &o, const T *x, T *y, hipStream_t st, const int *stop, const Fuse *fuse) {
   if ((uint64_t)o.s.n * sizeof(T) >= (1ull << 32)) return SISL_ERR_ARG;  // 32-bit offsets
+  const Fuse f = fuse ? *fuse : Fuse{};
   const AShape sh = a_shape(o.s, (int)sizeof(T));
   const Stag &s = o.s;
-  const double bytes = (double)sizeof(T) * (2.0 * s.n + 3.0 * s.u.n() + 3.0 * s.v.n() + 6.0 * s.w.n() + s.m.n());
+  const double bytes = (double)sizeof(T) * (2.0 * s.n + 3.0 * s.u.n() + 3.0 * s.v.n() + 6.0 * s.w.n() + s.m.n()) +
+                       (f.nb == 1 ? 2.0 : f.nb == 2 ? (double)sizeof(T) : 0.0) * s.n;
   const int tid = timer_begin(SISL_T_HELMHOLTZ, sizeof(T) == 4, bytes, stop, nullptr, st);
-  launch_A_shape(o, x, y, st, stop, sh);
+  const int rc = launch_A_shape(o, x, y, st, stop, sh, f);
   timer_end(tid, st);
+  if (rc) return rc;
   SISL_CHECK_LAUNCH();
   return SISL_OK;
 }
 
 template <typename T>
-int launch_AT(const OpT<T> &o, const T *y, T *x, T *scratch, hipStream_t st, const int *stop) {
-  const int64_t nm = o.s.m.n();
-  T *g = scratch, *um = scratch + nm, *vm = scratch + 2 * nm;
-  const dim3 g1((unsigned)((o.s.nx + kTX - 1) / kTX), (unsigned)((o.s.ny + kTY - 1) / kTY),
-                (unsigned)o.s.nz);
-  hipLaunchKernelGGL(helmholtz_T1_kernel<T>, g1, dim3(kTX, kTY), 0, st, o, y, g, um, vm, stop);
-  hipLaunchKernelGGL(helmholtz_T2_kernel<T>, point_grid(o.s, 5), dim3(kTX, kTY), 0, st, o, y,
-                     (const T *)g, (const T *)um, (const T *)vm, x, stop);
+static int launch_AT_shape(const OpT<T> &o, const T *y, T *x, hipStream_t st, const int *stop, AShape sh,
+                           const Fuse &f = Fuse{}) {
+  return launch_shape<ATKern, T>(o, y, x, st, stop, sh, f);
+}
+
+template <typename T>
+int launch_AT(const OpT<T> &o, const T *y, T *x, T * /*scratch: unused*/, hipStream_t st, const int *stop,
+              const Fuse *fuse) {
+  if ((uint64_t)o.s.n * sizeof(T) >= (1ull << 32)) return SISL_ERR_ARG;  // 32-bit offsets
+  const Fuse f = fuse ? *fuse : Fuse{};
+  const AShape sh = a_shape(o.s, (int)sizeof(T), 1);
+  const Stag &s = o.s;
+  // same arrays as the forward operator: y in, x out, 13 coefficient arrays (+ the basis column)
+  const double bytes = (double)sizeof(T) * (2.0 * s.n + 3.0 * s.u.n() + 3.0 * s.v.n() + 6.0 * s.w.n() + s.m.n()) +
+                       (f.nb == 1 ? 2.0 : f.nb == 2 ? (double)sizeof(T) : 0.0) * s.n;
+  const int tid = timer_begin(SISL_T_HELMHOLTZ_T, sizeof(T) == 4, bytes, stop, nullptr, st);
+  const int rc = launch_AT_shape(o, y, x, st, stop, sh, f);
+  timer_end(tid, st);
+  if (rc) return rc;
   SISL_CHECK_LAUNCH();
   return SISL_OK;
 }
 
-// Time every launch shape of the forward operator on the caller's buffers
-// and keep the fastest for this grid and element type.
+// Time every launch shape of the operator and of its transpose on the
+// caller's buffers and keep the fastest of each for this grid and element type.
 template <typename T>
 static int autotune_A(const OpT<T> &o, const T *x, T *y, hipStream_t st, int32_t *best_kc,
                       int32_t *best_unr) {
@@ -472,27 +645,33 @@ static int autotune_A(const OpT<T> &o, const T *x, T *y, hipStream_t st, int32_t
     }
   hipEvent_t e0, e1;
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return SISL_ERR_LAUNCH;
-  AShape best{(levels + 7) / 8, 1};
-  float best_ms = 1e30f;
-  for (const AShape &c : cand) {
-    for (int r = 0; r < 2; ++r) launch_A_shape(o, x, y, st, nullptr, c);
-    hipEventRecord(e0, st);
-    for (int r = 0; r < 5; ++r) launch_A_shape(o, x, y, st, nullptr, c);
-    hipEventRecord(e1, st);
-    if (hipEventSynchronize(e1) != hipSuccess) return SISL_ERR_LAUNCH;
-    float ms = 0;
-    hipEventElapsedTime(&ms, e0, e1);
-    if (ms < best_ms) { best_ms = ms; best = c; }
+  AShape best[2] = {{(levels + 7) / 8, 1}, {(levels + 7) / 8, 1}};
+  for (int tr = 0; tr < 2; ++tr) {
+    float best_ms = 1e30f;
+    for (const AShape &c : cand) {
+      auto run = [&]() {
+        if (tr) launch_AT_shape(o, x, y, st, nullptr, c);
+        else launch_A_shape(o, x, y, st, nullptr, c);
+      };
+      for (int r = 0; r < 2; ++r) run();
+      hipEventRecord(e0, st);
+      for (int r = 0; r < 5; ++r) run();
+      hipEventRecord(e1, st);
+      if (hipEventSynchronize(e1) != hipSuccess) return SISL_ERR_LAUNCH;
+      float ms = 0;
+      hipEventElapsedTime(&ms, e0, e1);
+      if (ms < best_ms) { best_ms = ms; best[tr] = c; }
+    }
   }
   hipEventDestroy(e0);
   hipEventDestroy(e1);
   SISL_CHECK_LAUNCH();
   {
     std::lock_guard<std::mutex> g(g_tune_mu);
-    g_tune[{o.s.nx, o.s.ny, o.s.nz, (int)sizeof(T)}] = best;
+    for (int tr = 0; tr < 2; ++tr) g_tune[{o.s.nx, o.s.ny, o.s.nz, (int)sizeof(T), tr}] = best[tr];
   }
-  if (best_kc) *best_kc = best.kc;
-  if (best_unr) *best_unr = best.unr;
+  if (best_kc) *best_kc = best[0].kc;
+  if (best_unr) *best_unr = best[0].unr;
   return SISL_OK;
 }
 
@@ -502,44 +681,22 @@ int blocks_A(const Stag &s) {
   const int kc = (s.nz + 1 + 7) / 8;
   return ((s.nx + kTX - 1) / kTX) * ((s.ny + kTY - 1) / kTY) * ((s.nz + 1 + kc - 1) / kc);
 }
-int blocks_AT1(const Stag &s) { return ((s.nx + kTX - 1) / kTX) * ((s.ny + kTY - 1) / kTY) * s.nz; }
-int blocks_AT2(const Stag &s) {
-  return ((s.nx + 1 + kTX - 1) / kTX) * ((s.ny + 1 + kTY - 1) / kTY) * 5 * (s.nz + 1);
-}
-
 template <typename T>
 __global__ void __launch_bounds__(256) helmholtz_batched_kernel(const BOp<T> *__restrict__ tab, int j) {
   const BOp<T> &b = tab[blockIdx.y];
   if (b.stop && b.stop[j * b.sj]) return;
   const int nb = blocks_A_dev(b.o.s);
   if ((int)blockIdx.x >= nb) return;
-  helmholtz_body<T, 1>(b.o, b.x + j * b.xj, b.y, (b.o.s.nz + 1 + 7) / 8, blockIdx.x, nb);
+  helmholtz_body<T, 1, 0>(b.o, b.x + j * b.xj, b.y, (b.o.s.nz + 1 + 7) / 8, blockIdx.x, nb);
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) helmholtz_T1_batched_kernel(const BOp<T> *__restrict__ tab, int j) {
+__global__ void __launch_bounds__(256) helmholtz_T_batched_kernel(const BOp<T> *__restrict__ tab, int j) {
   const BOp<T> &b = tab[blockIdx.y];
   if (b.stop && b.stop[j * b.sj]) return;
-  const Stag &s = b.o.s;
-  const int gx = (s.nx + kTX - 1) / kTX, gy = (s.ny + kTY - 1) / kTY;
-  const int lin = blockIdx.x;
-  if (lin >= gx * gy * s.nz) return;
-  const int64_t nm = s.m.n();
-  helmholtz_T1_body(b.o, b.x + j * b.xj, b.scr, b.scr + nm, b.scr + 2 * nm, lin % gx, (lin / gx) % gy,
-                    lin / (gx * gy));
-}
-
-template <typename T>
-__global__ void __launch_bounds__(256) helmholtz_T2_batched_kernel(const BOp<T> *__restrict__ tab, int j) {
-  const BOp<T> &b = tab[blockIdx.y];
-  if (b.stop && b.stop[j * b.sj]) return;
-  const Stag &s = b.o.s;
-  const int gx = (s.nx + 1 + kTX - 1) / kTX, gy = (s.ny + 1 + kTY - 1) / kTY;
-  const int lin = blockIdx.x;
-  if (lin >= gx * gy * 5 * (s.nz + 1)) return;
-  const int64_t nm = s.m.n();
-  helmholtz_T2_body(b.o, b.x + j * b.xj, (const T *)b.scr, (const T *)(b.scr + nm),
-                    (const T *)(b.scr + 2 * nm), b.y, lin % gx, (lin / gx) % gy, lin / (gx * gy));
+  const int nb = blocks_A_dev(b.o.s);
+  if ((int)blockIdx.x >= nb) return;
+  helmholtz_T_body<T, 1, 0>(b.o, b.x + j * b.xj, b.y, (b.o.s.nz + 1 + 7) / 8, blockIdx.x, nb);
 }
 
 template <typename T>
@@ -549,23 +706,24 @@ int launch_A_batched(const BOp<T> *tab, int M, int maxb, int j, hipStream_t st) 
   return SISL_OK;
 }
 template <typename T>
-int launch_AT_batched(const BOp<T> *tab, int M, int maxb1, int maxb2, int j, hipStream_t st) {
-  hipLaunchKernelGGL(helmholtz_T1_batched_kernel<T>, dim3(maxb1, M), dim3(kTX, kTY), 0, st, tab, j);
-  hipLaunchKernelGGL(helmholtz_T2_batched_kernel<T>, dim3(maxb2, M), dim3(kTX, kTY), 0, st, tab, j);
+int launch_AT_batched(const BOp<T> *tab, int M, int maxb, int j, hipStream_t st) {
+  hipLaunchKernelGGL(helmholtz_T_batched_kernel<T>, dim3(maxb, M), dim3(kTX, kTY), 0, st, tab, j);
   SISL_CHECK_LAUNCH();
   return SISL_OK;
 }
 template int launch_A_batched<double>(const BOp<double> *, int, int, int, hipStream_t);
 template int launch_A_batched<float>(const BOp<float> *, int, int, int, hipStream_t);
-template int launch_AT_batched<double>(const BOp<double> *, int, int, int, int, hipStream_t);
-template int launch_AT_batched<float>(const BOp<float> *, int, int, int, int, hipStream_t);
+template int launch_AT_batched<double>(const BOp<double> *, int, int, int, hipStream_t);
+template int launch_AT_batched<float>(const BOp<float> *, int, int, int, hipStream_t);
 
-template int launch_A<double>(const OpT<double> &, const double *, double *, hipStream_t, const int *);
-template int launch_A<float>(const OpT<float> &, const float *, float *, hipStream_t, const int *);
+template int launch_A<double>(const OpT<double> &, const double *, double *, hipStream_t, const int *,
+                              const Fuse *);
+template int launch_A<float>(const OpT<float> &, const float *, float *, hipStream_t, const int *,
+                             const Fuse *);
 template int launch_AT<double>(const OpT<double> &, const double *, double *, double *, hipStream_t,
-                               const int *);
+                               const int *, const Fuse *);
 template int launch_AT<float>(const OpT<float> &, const float *, float *, float *, hipStream_t,
-                              const int *);
+                              const int *, const Fuse *);
 
 }  // namespace sisl
 
@@ -588,7 +746,7 @@ int sisl_helmholtz_apply_f32(const SislOperator *op, const float *x, float *y, v
 int sisl_set_operator_shape(const SislGrid *g, int32_t fp32, int32_t chunk, int32_t unroll) {
   if (!g || chunk < 2 || (unroll != 1 && unroll != 2)) return SISL_ERR_ARG;
   std::lock_guard<std::mutex> lk(sisl::g_tune_mu);
-  sisl::g_tune[{g->nx, g->ny, g->nz, fp32 ? 4 : 8}] = sisl::AShape{chunk, unroll};
+  sisl::g_tune[{g->nx, g->ny, g->nz, fp32 ? 4 : 8, 0}] = sisl::AShape{chunk, unroll};
   return SISL_OK;
 }
 
@@ -606,14 +764,14 @@ int sisl_autotune_operator(const SislOperator *op, int32_t fp32, void *x, void *
 }
 int sisl_helmholtz_apply_T_f64(const SislOperator *op, const double *x, double *y, double *scr,
                                void *s) {
-  if (!op || !x || !y || !scr || x == y) return SISL_ERR_ARG;
+  if (!op || !x || !y || x == y) return SISL_ERR_ARG;
   auto o = sisl::OpT<double>::make(*op);
   if (!o.complete()) return SISL_ERR_ARG;
   return sisl::launch_AT(o, x, y, scr, sisl::as_stream(s));
 }
 int sisl_helmholtz_apply_T_f32(const SislOperator *op, const float *x, float *y, float *scr,
                                void *s) {
-  if (!op || !x || !y || !scr || x == y) return SISL_ERR_ARG;
+  if (!op || !x || !y || x == y) return SISL_ERR_ARG;
   auto o = sisl::OpT<float>::make(*op);
   if (!o.complete()) return SISL_ERR_ARG;
   return sisl::launch_AT(o, x, y, scr, sisl::as_stream(s));

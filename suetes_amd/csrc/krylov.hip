@@ -604,7 +604,7 @@ static Layout layout(const Stag &s, int R, bool mixed) {
   L.partials = take(sizeof(double) * kPStride * kRedGrid);
   L.w = take(sizeof(double) * basis_ld(n));
   L.t = take(sizeof(double) * basis_ld(n));
-  L.scr = take(sizeof(double) * 3 * nm);
+  L.scr = take(sizeof(double) * 2 * nm);
   if (!mixed) {
     L.V = take(sizeof(double) * basis_ld(n) * (R + 1));
   } else {
@@ -612,7 +612,7 @@ static Layout layout(const Stag &s, int R, bool mixed) {
     L.cur32 = take(sizeof(float) * basis_ld(n));
     L.w32 = take(sizeof(float) * basis_ld(n));
     L.t32 = take(sizeof(float) * basis_ld(n));
-    L.scr32 = take(sizeof(float) * 3 * nm);
+    L.scr32 = take(sizeof(float) * 2 * nm);
   }
   L.total = o;
   return L;
@@ -676,37 +676,62 @@ struct Krylov {
     else return cur;
   }
 
-  // out <- (M A) in  (or M^T A^T); kernels skip when *stop != 0
-  int apply(const TV *in, TV *out, const int *stop) {
+  // out <- (M A) in  (or M^T A^T); kernels skip when *stop != 0; `fuse`:
+  // the operator launch also normalises `in` into a basis column
+  int apply(const TV *in, TV *out, const int *stop, const Fuse *fuse = nullptr) {
     int rc;
     if (!transpose) {
-      if ((rc = launch_A<TV>(op, in, out, s, stop))) return rc;
+      if ((rc = launch_A<TV>(op, in, out, s, stop, fuse))) return rc;
       return launch_M<TV>(op, out, out, s, stop);
     }
-    if ((rc = launch_AT<TV>(op, in, t, scr, s, stop))) return rc;
+    if ((rc = launch_AT<TV>(op, in, t, scr, s, stop, fuse))) return rc;
     return launch_MT<TV>(op, t, out, scr, s, stop);
   }
 
   // One Arnoldi step (JAX _kth_arnoldi_iteration) with JAX's iterated CGS.
+  //
+  // FP32 inner solver: the normalisation v_j = w_{j-1} / h_{j,j-1} is fused
+  // into the operator launch of step j (which reads the unnormalised w_{j-1},
+  // scales its outputs and stores basis column j), and the Arnoldi vector
+  // alternates between the buffers w (even j) and cur (odd j).  Step 0 reads
+  // the unit residual that starts the cycle.  FP64 solver: a separate
+  // normalisation pass after the step.
   int arnoldi(int j, int R, int early_exit) {
     int rc;
     const int64_t ld = ldv();
     int *stop = flags + j;
     const double vs = vscale(), vinv = 1.0 / vs;
+    constexpr bool fused = sizeof(TV) == 4;
     hipLaunchKernelGGL(snapshot_kernel, dim3(1), dim3(1), 0, s, (const Small *)sm, stop);
-    if ((rc = apply(basis_in(j), w, stop))) return rc;
+    TV *wj = w;
+    if constexpr (fused) {
+      wj = (j % 2 == 0) ? w : cur;
+      const TV *src = j == 0 ? basis_in(0) : (j % 2 == 0 ? cur : w);
+      Fuse f;
+      f.inv = &sm->inv;
+      f.vout = V + (int64_t)j * ld;
+      f.nb = sizeof(TB) == 2 ? 1 : 2;
+      f.vs = vs;
+      if ((rc = apply(src, wj, stop, j > 0 ? &f : nullptr))) return rc;
+    } else {
+      if ((rc = apply(basis_in(j), w, stop))) return rc;
+    }
     const int nv = j + 1;
     const int *gate = &sm->gate2;
-    dots<TB, TV>(V, ld, nv, w, n, partials, s, vinv, stop);
+    dots<TB, TV>(V, ld, nv, wj, n, partials, s, vinv, stop);
     reduce(partials, nv + 1, sm->h1, s, stop);             // h1[0..nv-1], ||w||^2 in h1[nv]
-    update<TB, TV>(V, ld, nv, sm->h1, w, n, partials, s, vinv, stop);
+    update<TB, TV>(V, ld, nv, sm->h1, wj, n, partials, s, vinv, stop);
     reduce(partials, 1, sm->nn1a, s, stop);
     hipLaunchKernelGGL(decide_kernel, dim3(1), dim3(1), 0, s, sm, nv);
-    dots<TB, TV>(V, ld, nv, w, n, partials, s, vinv, stop, gate);
+    dots<TB, TV>(V, ld, nv, wj, n, partials, s, vinv, stop, gate);
     reduce(partials, nv, sm->h2, s, stop, gate);
-    update<TB, TV>(V, ld, nv, sm->h2, w, n, partials, s, vinv, stop, gate);
+    update<TB, TV>(V, ld, nv, sm->h2, wj, n, partials, s, vinv, stop, gate);
     reduce(partials, 1, sm->nn1b, s, stop, gate);
     hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(64), 0, s, sm, j, R, early_exit);
+    if constexpr (fused) {
+      SISL_CHECK_LAUNCH();
+      return SISL_OK;
+    }
     const double nb = (double)n * (sizeof(TV) + sizeof(TB) + (cur ? sizeof(TV) : 0));
     const int tid = timer_begin(SISL_T_KRYLOV_NORMALIZE, sizeof(TV) == 4, nb, stop, nullptr, s);
     hipLaunchKernelGGL((normalize_gated_kernel<TB, TV>), dim3(kRedGrid), dim3(kBlock), 0, s,
@@ -826,7 +851,7 @@ static int gmres_mixed(const OpT<double> &op64, const OpT<float> &op32, const Si
   Krylov<float, TB> K{op32, p.transpose, (TB *)(ws + L.V32), (float *)(ws + L.w32),
                       (float *)(ws + L.t32), (float *)(ws + L.scr32), partials, sm,
                       (int *)(ws + L.flags), n, s};
-  if constexpr (sizeof(TB) != sizeof(float)) K.cur = (float *)(ws + L.cur32);
+  K.cur = (float *)(ws + L.cur32);  // second Arnoldi buffer (and the FP16 path's unit residual)
   int rc;
   double hs[4];
   auto M64 = [&](double *in, double *out) -> int {
@@ -1117,7 +1142,7 @@ __global__ void kb_copy_kernel(PtrTable t) {
 // Host-side plan of one batched workspace (set up once per operator set).
 struct EnsPlan {
   int M, R, L, S;
-  int maxA, maxAT1, maxAT2, maxM, maxCols, maxFaces;
+  int maxA, maxM, maxCols, maxFaces;
   int64_t maxN;
   std::vector<int64_t> n;
   KMem *km;                  // device table
@@ -1145,7 +1170,7 @@ static int ens_setup(const SislOperator *ops64, const SislOperator *ops32, int M
   P.M = M;
   P.R = R;
   P.L = 0;
-  P.maxA = P.maxAT1 = P.maxAT2 = P.maxM = P.maxCols = P.maxFaces = 1;
+  P.maxA = P.maxM = P.maxCols = P.maxFaces = 1;
   P.maxN = 1;
   std::vector<KMem> km(M);
   std::vector<BOp<float>> a32(M), m32(M), at32(M), mt32(M);
@@ -1194,8 +1219,6 @@ static int ens_setup(const SislOperator *ops64, const SislOperator *ops32, int M
     mb64[m] = BOp<double>{o64, k.b64, 0, k.w64, k.scr64, nullptr, 0};
     mtb64[m] = BOp<double>{o64, k.b64, 0, k.w64, k.scr64, nullptr, 0};
     P.maxA = std::max(P.maxA, blocks_A(s));
-    P.maxAT1 = std::max(P.maxAT1, blocks_AT1(s));
-    P.maxAT2 = std::max(P.maxAT2, blocks_AT2(s));
     P.maxM = std::max(P.maxM, blocks_M(s));
     P.maxCols = std::max(P.maxCols, s.nx * s.ny);
     P.maxFaces = std::max(P.maxFaces, blocks_MT_faces(s));
@@ -1251,7 +1274,7 @@ static int ens_arnoldi(const EnsPlan &P, int j, int transpose, hipStream_t st) {
     if ((rc = launch_A_batched<float>(P.A32, M, P.maxA, j, st))) return rc;
     if ((rc = launch_M_batched<float>(P.M32, M, P.maxM, P.L, P.S, j, st))) return rc;
   } else {
-    if ((rc = launch_AT_batched<float>(P.AT32, M, P.maxAT1, P.maxAT2, j, st))) return rc;
+    if ((rc = launch_AT_batched<float>(P.AT32, M, P.maxA, j, st))) return rc;
     if ((rc = launch_MT_batched<float>(P.MT32, M, P.maxM, P.L, P.S, P.maxFaces, j, st))) return rc;
   }
   kb_dots_dispatch(P, nv, j, 0, st);
@@ -1317,7 +1340,7 @@ static int ens_solve(const void *ws, const SislGmresParams &p, const double *con
     if (!p.transpose) {
       if ((rc = launch_A_batched<double>(P.A64, M, P.maxA, 0, st))) return rc;
     } else {
-      if ((rc = launch_AT_batched<double>(P.AT64, M, P.maxAT1, P.maxAT2, 0, st))) return rc;
+      if ((rc = launch_AT_batched<double>(P.AT64, M, P.maxA, 0, st))) return rc;
     }
     hipLaunchKernelGGL(kb_residual_kernel, dim3(kRedGrid, M), dim3(kBlock), 0, st, (const KMem *)P.km);
     if (!p.transpose) {

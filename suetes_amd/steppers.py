@@ -400,14 +400,30 @@ class SemiImplicitSolver3D:
         self._bg = bg
 
     def _restart(self):
-        if self.precision == "mixed" and self.inner_restart:
-            return int(self.inner_restart)
+        """Arnoldi steps per cycle.  The mixed solver's inner FP32 cycles
+        default to 10 steps (at most solver_restart): each cycle restarts from
+        a fresh FP64 residual, the number of Arnoldi steps to the tolerance
+        stays that of restart 20 on the limited-area workload, and the
+        Gram-Schmidt passes run over half as many basis vectors."""
+        if self.precision == "mixed":
+            if self.inner_restart:
+                return int(self.inner_restart)
+            return min(10, int(self.solver_restart))
         return int(self.solver_restart)
+
+    def _max_outer(self):
+        """Refinement passes of the mixed solver: each runs one inner cycle of
+        at most ``inner_restart`` Arnoldi steps, so the cap keeps the
+        reference's Arnoldi budget (solver_maxiter cycles of solver_restart)."""
+        if self.precision != "mixed":
+            return 0
+        R = self._restart()
+        return max(int(self.solver_maxiter), -(-int(self.solver_maxiter) * int(self.solver_restart) // R))
 
     def _params(self, transpose):
         return _lib.SislGmresParams(float(self.solver_tol), 0.0, self._restart(),
                                     int(self.solver_maxiter), int(transpose), int(bool(self.early_exit)),
-                                    float(self.inner_rtol), 0, int(self.inner_basis == "fp16"))
+                                    float(self.inner_rtol), self._max_outer(), int(self.inner_basis == "fp16"))
 
     def solve_flat(self, b, x0, transpose=False):
         """Solve on packed scaled vectors; returns a new tensor."""
@@ -474,10 +490,8 @@ class SemiImplicitSolver3D:
         s = _lib.stream_ptr(x.device)
         sfx = "f64" if dtype == torch.float64 else "f32"
         if transpose:
-            g = self.physics.grid
-            scr = torch.empty(3 * g.nx * g.ny * g.nz, dtype=dtype, device=x.device)
             rc = getattr(L, f"sisl_helmholtz_apply_T_{sfx}")(C.byref(ops.struct), x.data_ptr(), y.data_ptr(),
-                                                             scr.data_ptr(), s)
+                                                             None, s)
         else:
             rc = getattr(L, f"sisl_helmholtz_apply_{sfx}")(C.byref(ops.struct), x.data_ptr(), y.data_ptr(), s)
         _lib.check(rc, "operator apply")
@@ -492,7 +506,7 @@ class SemiImplicitSolver3D:
         sfx = "f64" if dtype == torch.float64 else "f32"
         if transpose:
             g = self.physics.grid
-            scr = torch.empty(3 * g.nx * g.ny * g.nz, dtype=dtype, device=x.device)
+            scr = torch.empty(2 * g.nx * g.ny * g.nz, dtype=dtype, device=x.device)
             rc = getattr(L, f"sisl_precond_apply_T_{sfx}")(C.byref(ops.struct), x.data_ptr(), y.data_ptr(),
                                                            scr.data_ptr(), s)
         else:

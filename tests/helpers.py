@@ -46,6 +46,11 @@ def small_bubble(n=16):
     return cfg
 
 
+def small_schaer(nx=40, nz=14):
+    """Mountain-wave slice: periodic in y (ny = 3) only, x relaxation."""
+    return W.schaer(nx=nx, nz=nz, dx=500.0, top=10000.0, steps=4)
+
+
 def to_np_ref(t):
     """Internal (Z,Y,X) or reference-view tensor -> reference-shaped NumPy."""
     return t.detach().cpu().numpy()

@@ -9,7 +9,7 @@ import pytest
 import torch
 
 from oracle import sisl as O
-from tests.helpers import oracle_case, rel_l2, small_bubble, small_lam
+from tests.helpers import oracle_case, rel_l2, small_bubble, small_lam, small_schaer
 
 pytestmark = pytest.mark.gpu
 DEV = "cuda"
@@ -189,7 +189,7 @@ def test_operator_and_preconditioner_match_oracle(cfgf):
         assert rel_l2(Mx32[k], M_o(x)[k]) < 2e-5, ("M32", k)
 
 
-@pytest.mark.parametrize("cfgf", [small_lam, small_bubble])
+@pytest.mark.parametrize("cfgf", [small_lam, small_bubble, small_schaer])
 def test_transposes_pass_dot_tests(cfgf):
     case, go, A_o, M_o, sol = _solver_pair(cfgf())
     rng = np.random.default_rng(9)
