@@ -206,6 +206,29 @@ int sisl_gmres_mixed(const SislOperator *op64, const SislOperator *op32,
                      const SislGmresParams *p, const double *b, double *x, void *workspace,
                      SislGmresStats *stats, void *stream);
 
+/* ---- staggered-grid operators (explicit tendencies) --------------------- */
+
+/* CGridOperator3D.avg (op 0) / diff (op 1) along one axis
+ * (suetes/regional3d/operators.py:65-132, edge closures operators.py:37-63):
+ * f is contiguous (outer, n, inner); out is (outer, n_out, inner) with
+ * n_out = n - 1 (pad 0), or n + 1 with the edge values repeated (pad 1,
+ * m -> u, v, w) or the periodic wrap value at both ends (pad 2).  diff
+ * multiplies by rd = 1 / spacing; m3 (optional, m3n elements) multiplies
+ * the output by a map factor indexed by (output flat index mod m3n). */
+int sisl_stagger_f64(const double *f, double *out, int64_t outer, int32_t n, int64_t inner, int32_t op,
+                     int32_t pad, double rd, const double *m3, int64_t m3n, void *stream);
+int sisl_stagger_f32(const float *f, float *out, int64_t outer, int32_t n, int64_t inner, int32_t op,
+                     int32_t pad, double rd, const float *m3, int64_t m3n, void *stream);
+/* Transpose: gf = (cotangent of f) from g (cotangent of out).  exact = 0:
+ * the repeated edge values of pad 1 pass nothing back (the reference's
+ * stop-gradient); exact = 1: they do. */
+int sisl_stagger_T_f64(const double *g, double *gf, int64_t outer, int32_t n, int64_t inner, int32_t op,
+                       int32_t pad, int32_t exact, double rd, const double *m3, int64_t m3n,
+                       void *stream);
+int sisl_stagger_T_f32(const float *g, float *gf, int64_t outer, int32_t n, int64_t inner, int32_t op,
+                       int32_t pad, int32_t exact, double rd, const float *m3, int64_t m3n,
+                       void *stream);
+
 /* ---- kernel timing (measurement runs) ------------------------------------ */
 
 /* Timed kernels: operator apply, column preconditioner, the Krylov

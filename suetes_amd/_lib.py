@@ -25,7 +25,8 @@ EXPORTED = [
     "sisl_timer_enable", "sisl_timer_read", "sisl_autotune_operator",
     "sisl_set_operator_shape", "sisl_tricubic_batched_f64", "sisl_tricubic_batched_f32",
     "sisl_departure_batched_f64", "sisl_departure_batched_f32", "sisl_gmres_batched_workspace_bytes",
-    "sisl_gmres_batched_setup", "sisl_gmres_batched",
+    "sisl_gmres_batched_setup", "sisl_gmres_batched", "sisl_stagger_f64", "sisl_stagger_f32",
+    "sisl_stagger_T_f64", "sisl_stagger_T_f32",
 ]
 
 # timed kernels (enum SislTimedKernel)
@@ -107,6 +108,9 @@ def lib():
         L.sisl_gmres_batched_workspace_bytes.argtypes = [C.POINTER(SislGrid), i32, i32]
         L.sisl_gmres_batched_setup.argtypes = [C.POINTER(SislOperator), C.POINTER(SislOperator), i32, i32, vp, vp]
         L.sisl_gmres_batched.argtypes = [i32, C.POINTER(SislGmresParams), pp, pp, vp, C.POINTER(SislGmresStats), vp]
+        for p in ("f64", "f32"):
+            getattr(L, f"sisl_stagger_{p}").argtypes = [vp, vp, i64, i32, i64, i32, i32, d, vp, i64, vp]
+            getattr(L, f"sisl_stagger_T_{p}").argtypes = [vp, vp, i64, i32, i64, i32, i32, i32, d, vp, i64, vp]
         L.sisl_timer_enable.argtypes = [i32]
         L.sisl_timer_read.argtypes = [i32, i32, C.POINTER(d), C.POINTER(i64), C.POINTER(i64),
                                       C.POINTER(d)]

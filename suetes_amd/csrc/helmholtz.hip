@@ -532,11 +532,12 @@ struct AShape {
   int kc, unr;
 };
 static std::mutex g_tune_mu;
-static std::map<std::tuple<int, int, int, int, int>, AShape> g_tune;  // (nx, ny, nz, bytes, transpose)
+static std::map<std::tuple<int, int, int, int, int>, AShape> g_tune;  // (nx, ny, nz, bytes, variant)
 
-static AShape a_shape(const Stag &s, int bytes, int tr = 0) {
+// variant = transpose + 2 * (fused basis normalisation)
+static AShape a_shape(const Stag &s, int bytes, int variant = 0) {
   std::lock_guard<std::mutex> g(g_tune_mu);
-  auto it = g_tune.find({s.nx, s.ny, s.nz, bytes, tr});
+  auto it = g_tune.find({s.nx, s.ny, s.nz, bytes, variant});
   if (it != g_tune.end()) return it->second;
   return AShape{(s.nz + 1 + 7) / 8, 1};
 }
@@ -594,7 +595,7 @@ template <typename T>
 int launch_A(const OpT<T> &o, const T *x, T *y, hipStream_t st, const int *stop, const Fuse *fuse) {
   if ((uint64_t)o.s.n * sizeof(T) >= (1ull << 32)) return SISL_ERR_ARG;  // 32-bit offsets
   const Fuse f = fuse ? *fuse : Fuse{};
-  const AShape sh = a_shape(o.s, (int)sizeof(T));
+  const AShape sh = a_shape(o.s, (int)sizeof(T), f.nb ? 2 : 0);
   const Stag &s = o.s;
   const double bytes = (double)sizeof(T) * (2.0 * s.n + 3.0 * s.u.n() + 3.0 * s.v.n() + 6.0 * s.w.n() + s.m.n()) +
                        (f.nb == 1 ? 2.0 : f.nb == 2 ? (double)sizeof(T) : 0.0) * s.n;
@@ -617,7 +618,7 @@ int launch_AT(const OpT<T> &o, const T *y, T *x, T * /*scratch: unused*/, hipStr
               const Fuse *fuse) {
   if ((uint64_t)o.s.n * sizeof(T) >= (1ull << 32)) return SISL_ERR_ARG;  // 32-bit offsets
   const Fuse f = fuse ? *fuse : Fuse{};
-  const AShape sh = a_shape(o.s, (int)sizeof(T), 1);
+  const AShape sh = a_shape(o.s, (int)sizeof(T), f.nb ? 3 : 1);
   const Stag &s = o.s;
   // same arrays as the forward operator: y in, x out, 13 coefficient arrays (+ the basis column)
   const double bytes = (double)sizeof(T) * (2.0 * s.n + 3.0 * s.u.n() + 3.0 * s.v.n() + 6.0 * s.w.n() + s.m.n()) +
@@ -630,8 +631,9 @@ int launch_AT(const OpT<T> &o, const T *y, T *x, T * /*scratch: unused*/, hipStr
   return SISL_OK;
 }
 
-// Time every launch shape of the operator and of its transpose on the
-// caller's buffers and keep the fastest of each for this grid and element type.
+// Time every launch shape of the operator and of its transpose (FP32 also
+// with the fused FP16 basis normalisation) on the caller's buffers and keep
+// the fastest of each for this grid and element type.
 template <typename T>
 static int autotune_A(const OpT<T> &o, const T *x, T *y, hipStream_t st, int32_t *best_kc,
                       int32_t *best_unr) {
@@ -643,32 +645,57 @@ static int autotune_A(const OpT<T> &o, const T *x, T *y, hipStream_t st, int32_t
       const int kc = (levels + parts - 1) / parts;
       if (kc >= 2) cand.push_back({kc, unr});
     }
+  const int nvar = sizeof(T) == 4 ? 4 : 2;
+  // fused variants: scale 1 from device memory, basis column into a scratch buffer
+  void *tmp = nullptr;
+  Fuse fz;
+  if (nvar == 4) {
+    if (hipMalloc(&tmp, 256 + 2 * (size_t)o.s.n) != hipSuccess) return SISL_ERR_LAUNCH;
+    const double one = 1.0;
+    if (hipMemcpy(tmp, &one, sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+      hipFree(tmp);
+      return SISL_ERR_LAUNCH;
+    }
+    fz.inv = (const double *)tmp;
+    fz.vout = (char *)tmp + 256;
+    fz.nb = 1;
+    fz.vs = 1.0;
+  }
   hipEvent_t e0, e1;
-  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return SISL_ERR_LAUNCH;
-  AShape best[2] = {{(levels + 7) / 8, 1}, {(levels + 7) / 8, 1}};
-  for (int tr = 0; tr < 2; ++tr) {
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    if (tmp) hipFree(tmp);
+    return SISL_ERR_LAUNCH;
+  }
+  AShape best[4];
+  for (int v = 0; v < 4; ++v) best[v] = {(levels + 7) / 8, 1};
+  int rc = SISL_OK;
+  for (int var = 0; var < nvar && rc == SISL_OK; ++var) {
+    const int tr = var & 1;
+    const Fuse f = var >= 2 ? fz : Fuse{};
     float best_ms = 1e30f;
     for (const AShape &c : cand) {
       auto run = [&]() {
-        if (tr) launch_AT_shape(o, x, y, st, nullptr, c);
-        else launch_A_shape(o, x, y, st, nullptr, c);
+        if (tr) launch_AT_shape(o, x, y, st, nullptr, c, f);
+        else launch_A_shape(o, x, y, st, nullptr, c, f);
       };
       for (int r = 0; r < 2; ++r) run();
       hipEventRecord(e0, st);
       for (int r = 0; r < 5; ++r) run();
       hipEventRecord(e1, st);
-      if (hipEventSynchronize(e1) != hipSuccess) return SISL_ERR_LAUNCH;
+      if (hipEventSynchronize(e1) != hipSuccess) { rc = SISL_ERR_LAUNCH; break; }
       float ms = 0;
       hipEventElapsedTime(&ms, e0, e1);
-      if (ms < best_ms) { best_ms = ms; best[tr] = c; }
+      if (ms < best_ms) { best_ms = ms; best[var] = c; }
     }
   }
   hipEventDestroy(e0);
   hipEventDestroy(e1);
+  if (tmp) hipFree(tmp);
+  if (rc) return rc;
   SISL_CHECK_LAUNCH();
   {
     std::lock_guard<std::mutex> g(g_tune_mu);
-    for (int tr = 0; tr < 2; ++tr) g_tune[{o.s.nx, o.s.ny, o.s.nz, (int)sizeof(T), tr}] = best[tr];
+    for (int var = 0; var < nvar; ++var) g_tune[{o.s.nx, o.s.ny, o.s.nz, (int)sizeof(T), var}] = best[var];
   }
   if (best_kc) *best_kc = best[0].kc;
   if (best_unr) *best_unr = best[0].unr;

@@ -46,6 +46,9 @@ struct Small {
   double b_norm, Mb_norm, ptol, rnorm;
 };
 
+__device__ void decide_body(Small *sm, int nv);
+__device__ void finalize_body(Small *sm, int j, int R, int early_exit);
+
 // ---- block reduction helpers -------------------------------------------------
 // One value per thread -> partials[slot][block] (fixed-order, deterministic).
 __device__ __forceinline__ void block_store_slot(double v, int slot, double *partials, int blk) {
@@ -100,6 +103,50 @@ __global__ void __launch_bounds__(256) reduce_partials_kernel(const double *__re
     double t = 0;
     for (int q = 0; q < kBlock / 64; ++q) t += red[q];
     out[v] = t;
+  }
+}
+
+// Reduction tail of a projection / update pass: one workgroup sums the
+// per-block partial sums of `nout` slots (wave q takes slots q, q + 16, ...;
+// lane l the blocks l, l + 64, ... in order, then a fixed butterfly -- a
+// fixed order, so results are reproducible) and then runs the scalar
+// bookkeeping that follows the pass: the second-pass decision (mode 1) or
+// the Arnoldi step's Hessenberg column, breakdown test and Givens update
+// (mode 2), which also records the stop flag that the next step's kernels
+// read.  One launch replaces the separate reduction, decision, bookkeeping
+// and flag-copy launches of the step.  A skipped pass (stopped cycle, or no
+// second Gram-Schmidt pass) reduces nothing but still runs its bookkeeping.
+constexpr int kTailThreads = 1024;
+struct Tail {
+  double *out = nullptr;  // reduced slots
+  int nout = 0;
+  int mode = 0;
+  Small *sm = nullptr;
+  int nv = 0, j = 0, R = 0, early_exit = 0;
+  int *next_flag = nullptr;
+};
+
+__global__ void __launch_bounds__(kTailThreads) tail_kernel(const double *__restrict__ partials, Tail t,
+                                                            const int *__restrict__ stop,
+                                                            const int *__restrict__ gate) {
+  const bool skipped = (stop && *stop) || (gate && !*gate);
+  if (!skipped) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    for (int v = wv; v < t.nout; v += nw) {
+      double acc = 0;
+      for (int b = lane; b < kRedGrid; b += 64) acc += partials[(int64_t)v * kRedGrid + b];
+      acc = wave_sum(acc);
+      if (lane == 0) t.out[v] = acc;
+    }
+  }
+  if (!t.mode) return;
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  if (t.mode == 1) {
+    decide_body(t.sm, t.nv);
+  } else {
+    finalize_body(t.sm, t.j, t.R, t.early_exit);
+    *t.next_flag = t.sm->broken;
   }
 }
 
@@ -347,11 +394,6 @@ __global__ void __launch_bounds__(256) update_kernel(const TB *__restrict__ V, i
 
 // JAX _iterative_classical_gram_schmidt loop condition: a second pass runs
 // when ||r|| < ||q||/sqrt(2) after the first (norms below eps count as 0).
-__device__ void decide_body(Small *sm, int nv);
-__global__ void decide_kernel(Small *sm, int nv) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  decide_body(sm, nv);
-}
 __device__ void decide_body(Small *sm, int nv) {
   if (sm->broken) {
     sm->gate2 = 0;
@@ -368,11 +410,6 @@ __device__ void decide_body(Small *sm, int nv) {
 
 // Arnoldi bookkeeping for step j (one thread): JAX's H row, breakdown test
 // (v_norm_1 <= eps * v_norm_0), Givens residual estimate for early exit.
-__device__ void finalize_body(Small *sm, int j, int R, int early_exit);
-__global__ void finalize_kernel(Small *sm, int j, int R, int early_exit) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  finalize_body(sm, j, R, early_exit);
-}
 __device__ void finalize_body(Small *sm, int j, int R, int early_exit) {
   if (sm->broken) {
     sm->inv = 0.0;
@@ -453,7 +490,6 @@ __global__ void normalize_gated_kernel(const TV *__restrict__ w, TB *__restrict_
 }
 
 // snapshot of the stop flag taken before a step's finalize
-__global__ void snapshot_kernel(const Small *sm, int *dst) { *dst = sm->broken; }
 
 // u = src / ||src|| safe-normalised (JAX _safe_normalize, thresh eps):
 // out = vscale * u (basis storage), cur = u when given.
@@ -516,15 +552,66 @@ __device__ __forceinline__ void combine_body(const TV *__restrict__ V, int64_t l
   }
 }
 
+// nvp (optional): the number of basis vectors, read from device memory
 template <typename TV>
 __global__ void __launch_bounds__(256) combine_kernel(const TV *__restrict__ V, int64_t ldv, int nv,
                                                       const double *__restrict__ y,
                                                       double *__restrict__ x, int64_t n,
-                                                      double vinv) {
-  combine_body(V, ldv, nv, y, x, n, vinv, blockIdx.x, gridDim.x);
+                                                      double vinv, const int *__restrict__ nvp) {
+  combine_body(V, ldv, nvp ? *nvp : nv, y, x, n, vinv, blockIdx.x, gridDim.x);
 }
 
-__global__ void reset_cycle_kernel(Small *sm, int R, double g0, double target) {
+// Least squares of one cycle, min || beta e1 - H^T y || over the `steps`
+// columns, by the normal equations with partial pivoting (JAX's
+// _gmres_batched solve of the small system), one thread.  G, r: scratch of
+// kMaxRestart^2 and kMaxRestart doubles.
+__device__ void lstsq_body(Small *sm, int R, double beta, double *G, double *r) {
+  const int s = sm->steps;
+  for (int a = 0; a < s; ++a) {
+    for (int b = 0; b < s; ++b) {
+      double acc = 0;
+      for (int l = 0; l <= s; ++l) acc += sm->H[a * (R + 1) + l] * sm->H[b * (R + 1) + l];
+      G[a * s + b] = acc;
+    }
+    r[a] = sm->H[a * (R + 1)] * beta;
+  }
+  for (int c = 0; c < s; ++c) {
+    int p = c;
+    for (int q = c + 1; q < s; ++q)
+      if (fabs(G[q * s + c]) > fabs(G[p * s + c])) p = q;
+    if (p != c) {
+      for (int q = 0; q < s; ++q) { const double tq = G[c * s + q]; G[c * s + q] = G[p * s + q]; G[p * s + q] = tq; }
+      const double tr = r[c]; r[c] = r[p]; r[p] = tr;
+    }
+    const double d = G[c * s + c];
+    for (int q = c + 1; q < s; ++q) {
+      const double f = d != 0.0 ? G[q * s + c] / d : 0.0;
+      if (f == 0.0) continue;
+      for (int k2 = c; k2 < s; ++k2) G[q * s + k2] -= f * G[c * s + k2];
+      r[q] -= f * r[c];
+    }
+  }
+  for (int c = s - 1; c >= 0; --c) {
+    double acc = r[c];
+    for (int k2 = c + 1; k2 < s; ++k2) acc -= G[c * s + k2] * sm->y[k2];
+    const double d = G[c * s + c];
+    sm->y[c] = d != 0.0 ? acc / d : 0.0;
+  }
+  sm->steps_total += s;
+}
+
+// Single-solve driver: the cycle's least squares on the device, so that the
+// host does not wait for the Hessenberg matrix (one synchronisation per
+// cycle, for the residual norm, remains).
+__global__ void lstsq_kernel(Small *sm, int R, double beta) {
+  __shared__ double G[kMaxRestart * kMaxRestart], r[kMaxRestart];
+  if (threadIdx.x != 0) return;
+  lstsq_body(sm, R, beta, G, r);
+}
+
+__global__ void zero_steps_total_kernel(Small *sm) { sm->steps_total = 0; }
+
+__global__ void reset_cycle_kernel(Small *sm, int R, double g0, double target, int *flags) {
   const int t = threadIdx.x;
   for (int q = t; q < R * (R + 1); q += blockDim.x) {
     const int r = q / (R + 1), c = q % (R + 1);
@@ -538,6 +625,7 @@ __global__ void reset_cycle_kernel(Small *sm, int R, double g0, double target) {
     for (int v = 0; v <= R; ++v) sm->g[v] = 0.0;
     sm->g[0] = g0;
     sm->target = target;
+    flags[0] = 0;  // stop flag of step 0
   }
 }
 
@@ -546,7 +634,7 @@ __global__ void reset_cycle_kernel(Small *sm, int R, double g0, double target) {
 template <typename TB, typename TV>
 static void dots(const TB *V, int64_t ldv, int nv, const TV *w, int64_t n, double *partials,
                  hipStream_t s, double vinv = 1.0, const int *stop = nullptr,
-                 const int *gate = nullptr) {
+                 const int *gate = nullptr, const Tail *tl = nullptr) {
   const double bytes = (double)n * ((double)nv * sizeof(TB) + sizeof(TV));
   const int tid = timer_begin(SISL_T_KRYLOV_DOTS, sizeof(TV) == 4, bytes, stop, gate, s);
   const dim3 gr(kRedGrid), bl(kBlock);
@@ -563,22 +651,24 @@ static void dots(const TB *V, int64_t ldv, int nv, const TV *w, int64_t n, doubl
   else
     hipLaunchKernelGGL((dots_kernel<TB, TV>), gr, bl, 0, s, V, ldv, nv, w, n, partials, vinv, stop, gate);
   timer_end(tid, s);
+  if (tl) hipLaunchKernelGGL(tail_kernel, dim3(1), dim3(kTailThreads), 0, s, (const double *)partials, *tl, stop, gate);
 }
 template <typename TB, typename TV>
 static void update(const TB *V, int64_t ldv, int nv, const double *h, TV *w, int64_t n,
                    double *partials, hipStream_t s, double vinv = 1.0, const int *stop = nullptr,
-                   const int *gate = nullptr) {
+                   const int *gate = nullptr, const Tail *tl = nullptr) {
   const double bytes = (double)n * ((double)nv * sizeof(TB) + 2.0 * sizeof(TV));
   const int tid = timer_begin(SISL_T_KRYLOV_UPDATE, sizeof(TV) == 4, bytes, stop, gate, s);
   hipLaunchKernelGGL((update_kernel<TB, TV>), dim3(kRedGrid), dim3(kBlock), 0, s, V, ldv, nv, h, w,
                      n, partials, vinv, stop, gate);
   timer_end(tid, s);
+  if (tl) hipLaunchKernelGGL(tail_kernel, dim3(1), dim3(kTailThreads), 0, s, (const double *)partials, *tl, stop, gate);
 }
 template <typename TB>
 static void combine(const TB *V, int64_t ldv, int nv, const double *y, double *x, int64_t n,
-                    double vinv, hipStream_t s) {
+                    double vinv, hipStream_t s, const int *nvp = nullptr) {
   hipLaunchKernelGGL(combine_kernel<TB>, dim3(kRedGrid), dim3(kBlock), 0, s, V, ldv, nv, y, x, n,
-                     vinv);
+                     vinv, nvp);
 }
 static void reduce(const double *partials, int nslots, double *out, hipStream_t s,
                    const int *stop = nullptr, const int *gate = nullptr) {
@@ -619,40 +709,6 @@ static Layout layout(const Stag &s, int R, bool mixed) {
 }
 
 // Least squares of JAX's _lstsq (normal equations) on the leading s steps.
-static void host_lstsq(const double *H, int R, int s, double beta, double *y) {
-  std::vector<double> G((size_t)s * s), r(s);
-  for (int a = 0; a < s; ++a) {
-    for (int b = 0; b < s; ++b) {
-      double acc = 0;
-      for (int l = 0; l <= s; ++l) acc += H[a * (R + 1) + l] * H[b * (R + 1) + l];
-      G[(size_t)a * s + b] = acc;
-    }
-    r[a] = H[a * (R + 1) + 0] * beta;
-  }
-  for (int c = 0; c < s; ++c) {  // Gaussian elimination, partial pivoting
-    int p = c;
-    for (int q = c + 1; q < s; ++q)
-      if (std::fabs(G[(size_t)q * s + c]) > std::fabs(G[(size_t)p * s + c])) p = q;
-    if (p != c) {
-      for (int q = 0; q < s; ++q) std::swap(G[(size_t)c * s + q], G[(size_t)p * s + q]);
-      std::swap(r[c], r[p]);
-    }
-    const double d = G[(size_t)c * s + c];
-    for (int q = c + 1; q < s; ++q) {
-      const double f = d != 0.0 ? G[(size_t)q * s + c] / d : 0.0;
-      if (f == 0.0) continue;
-      for (int k = c; k < s; ++k) G[(size_t)q * s + k] -= f * G[(size_t)c * s + k];
-      r[q] -= f * r[c];
-    }
-  }
-  for (int c = s - 1; c >= 0; --c) {
-    double acc = r[c];
-    for (int k = c + 1; k < s; ++k) acc -= G[(size_t)c * s + k] * y[k];
-    const double d = G[(size_t)c * s + c];
-    y[c] = d != 0.0 ? acc / d : 0.0;
-  }
-}
-
 // One restart cycle's Arnoldi process.  TV: type of the operator applies and
 // working vectors; TB: basis storage type (TV, or FP16 under the FP32
 // working type).  With an FP16 basis, `cur` holds the newest basis vector
@@ -702,7 +758,6 @@ struct Krylov {
     int *stop = flags + j;
     const double vs = vscale(), vinv = 1.0 / vs;
     constexpr bool fused = sizeof(TV) == 4;
-    hipLaunchKernelGGL(snapshot_kernel, dim3(1), dim3(1), 0, s, (const Small *)sm, stop);
     TV *wj = w;
     if constexpr (fused) {
       wj = (j % 2 == 0) ? w : cur;
@@ -718,16 +773,15 @@ struct Krylov {
     }
     const int nv = j + 1;
     const int *gate = &sm->gate2;
-    dots<TB, TV>(V, ld, nv, wj, n, partials, s, vinv, stop);
-    reduce(partials, nv + 1, sm->h1, s, stop);             // h1[0..nv-1], ||w||^2 in h1[nv]
-    update<TB, TV>(V, ld, nv, sm->h1, wj, n, partials, s, vinv, stop);
-    reduce(partials, 1, sm->nn1a, s, stop);
-    hipLaunchKernelGGL(decide_kernel, dim3(1), dim3(1), 0, s, sm, nv);
-    dots<TB, TV>(V, ld, nv, wj, n, partials, s, vinv, stop, gate);
-    reduce(partials, nv, sm->h2, s, stop, gate);
-    update<TB, TV>(V, ld, nv, sm->h2, wj, n, partials, s, vinv, stop, gate);
-    reduce(partials, 1, sm->nn1b, s, stop, gate);
-    hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(64), 0, s, sm, j, R, early_exit);
+    // each pass is followed by one tail launch: its reduction and the bookkeeping after it
+    Tail t{sm->h1, nv + 1, 0, sm, nv, j, R, early_exit, flags + j + 1};
+    dots<TB, TV>(V, ld, nv, wj, n, partials, s, vinv, stop, nullptr, &t);  // h1[0..nv-1], ||w||^2 in h1[nv]
+    t.out = sm->nn1a; t.nout = 1; t.mode = 1;                               // + second-pass decision
+    update<TB, TV>(V, ld, nv, sm->h1, wj, n, partials, s, vinv, stop, nullptr, &t);
+    t.out = sm->h2; t.nout = nv; t.mode = 0;
+    dots<TB, TV>(V, ld, nv, wj, n, partials, s, vinv, stop, gate, &t);
+    t.out = sm->nn1b; t.nout = 1; t.mode = 2;                               // + Arnoldi bookkeeping
+    update<TB, TV>(V, ld, nv, sm->h2, wj, n, partials, s, vinv, stop, gate, &t);
     if constexpr (fused) {
       SISL_CHECK_LAUNCH();
       return SISL_OK;
@@ -805,22 +859,15 @@ static int gmres64(const OpT<double> &op, const SislGmresParams &p, const double
   const double ptol = Mb_norm * std::fmin(1.0, atol / b_norm);  // NaN when b = 0: no cycle
   double rnorm = std::sqrt(hs[2]);
   if (!(rnorm > 2.220446049250313e-16)) rnorm = 0.0;
-  std::vector<double> H((size_t)R * (R + 1));
-  double y[kMaxRestart];
-  int k = 0, steps_total = 0;
+  hipLaunchKernelGGL(zero_steps_total_kernel, dim3(1), dim3(1), 0, s, sm);
+  int k = 0;
   while (k < p.maxiter && rnorm > ptol) {
     hipLaunchKernelGGL(reset_cycle_kernel, dim3(1), dim3(kBlock), 0, s, sm, R, rnorm,
-                       p.early_exit ? ptol : 0.0);
+                       p.early_exit ? ptol : 0.0, K.flags);
     for (int j = 0; j < R; ++j)
       if ((rc = K.arnoldi(j, R, p.early_exit))) return rc;
-    int steps = 0;
-    if ((rc = sync_copy(H.data(), sm->H, sizeof(double) * R * (R + 1), s))) return rc;
-    if ((rc = sync_copy(&steps, &sm->steps, sizeof(int), s))) return rc;
-    steps_total += steps;
-    host_lstsq(H.data(), R, steps, rnorm, y);
-    if (hipMemcpyAsync(sm->y, y, sizeof(double) * steps, hipMemcpyHostToDevice, s) != hipSuccess)
-      return SISL_ERR_LAUNCH;
-    combine<double>(K.V, K.ldv(), steps, sm->y, x, n, 1.0, s);
+    hipLaunchKernelGGL(lstsq_kernel, dim3(1), dim3(64), 0, s, sm, R, rnorm);
+    combine<double>(K.V, K.ldv(), R, sm->y, x, n, 1.0, s, &sm->steps);
     if ((rc = residual(&sm->scal[2]))) return rc;
     if ((rc = sync_copy(hs, sm->scal, sizeof(double) * 3, s))) return rc;
     rnorm = std::sqrt(hs[2]);
@@ -828,6 +875,8 @@ static int gmres64(const OpT<double> &op, const SislGmresParams &p, const double
     ++k;
   }
   if (st) {
+    int steps_total = 0;
+    if ((rc = sync_copy(&steps_total, &sm->steps_total, sizeof(int), s))) return rc;
     st->cycles = k;
     st->arnoldi_steps = steps_total;
     st->residual = rnorm;
@@ -868,9 +917,8 @@ static int gmres_mixed(const OpT<double> &op64, const OpT<float> &op32, const Si
   const double atol = std::fmax(p.tol * b_norm, p.atol);
   const double ptol = Mb_norm * std::fmin(1.0, atol / b_norm);
   const int max_outer = p.max_outer > 0 ? p.max_outer : p.maxiter;
-  std::vector<double> H((size_t)R * (R + 1));
-  double y[kMaxRestart];
-  int outer = 0, steps_total = 0;
+  hipLaunchKernelGGL(zero_steps_total_kernel, dim3(1), dim3(1), 0, s, sm);
+  int outer = 0;
   double rnorm = 0;
   for (;;) {
     // r = b - A x (FP64), z = M r (FP64)
@@ -889,20 +937,16 @@ static int gmres_mixed(const OpT<double> &op64, const OpT<float> &op32, const Si
     if (!(rnorm > 2.220446049250313e-16) && !std::isnan(rnorm)) rnorm = 0.0;
     if (!(rnorm > ptol) || outer >= max_outer) break;
     const double target = std::fmax(p.inner_rtol * rnorm, 0.5 * ptol);
-    hipLaunchKernelGGL(reset_cycle_kernel, dim3(1), dim3(kBlock), 0, s, sm, R, rnorm, target);
+    hipLaunchKernelGGL(reset_cycle_kernel, dim3(1), dim3(kBlock), 0, s, sm, R, rnorm, target, K.flags);
     for (int j = 0; j < R; ++j)
       if ((rc = K.arnoldi(j, R, 1))) return rc;
-    int steps = 0;
-    if ((rc = sync_copy(H.data(), sm->H, sizeof(double) * R * (R + 1), s))) return rc;
-    if ((rc = sync_copy(&steps, &sm->steps, sizeof(int), s))) return rc;
-    steps_total += steps;
-    host_lstsq(H.data(), R, steps, rnorm, y);
-    if (hipMemcpyAsync(sm->y, y, sizeof(double) * steps, hipMemcpyHostToDevice, s) != hipSuccess)
-      return SISL_ERR_LAUNCH;
-    combine<TB>(K.V, K.ldv(), steps, sm->y, x, n, 1.0 / K.vscale(), s);
+    hipLaunchKernelGGL(lstsq_kernel, dim3(1), dim3(64), 0, s, sm, R, rnorm);
+    combine<TB>(K.V, K.ldv(), R, sm->y, x, n, 1.0 / K.vscale(), s, &sm->steps);
     ++outer;
   }
   if (st) {
+    int steps_total = 0;
+    if ((rc = sync_copy(&steps_total, &sm->steps_total, sizeof(int), s))) return rc;
     st->cycles = outer;
     st->arnoldi_steps = steps_total;
     st->residual = rnorm;
@@ -1079,46 +1123,14 @@ __global__ void kb_reset_kernel(const KMem *__restrict__ km, int R, double inner
     sm->target = fmax(inner_rtol * sm->rnorm, 0.5 * sm->ptol);
   }
 }
-// Least squares of the cycle (JAX _lstsq normal equations, as host_lstsq) on
-// the device, one thread per member; y into sm->y.
+// Least squares of the cycle (lstsq_body) on the device, one thread per
+// member; y into sm->y.
 __global__ void kb_lstsq_kernel(const KMem *__restrict__ km, int R) {
   __shared__ double G[kMaxRestart * kMaxRestart], r[kMaxRestart];
   if (threadIdx.x != 0) return;
   Small *sm = km[blockIdx.x].sm;
   if (sm->done) return;
-  const int s = sm->steps;
-  const double beta = sm->rnorm;
-  for (int a = 0; a < s; ++a) {
-    for (int b = 0; b < s; ++b) {
-      double acc = 0;
-      for (int l = 0; l <= s; ++l) acc += sm->H[a * (R + 1) + l] * sm->H[b * (R + 1) + l];
-      G[a * s + b] = acc;
-    }
-    r[a] = sm->H[a * (R + 1)] * beta;
-  }
-  for (int c = 0; c < s; ++c) {
-    int p = c;
-    for (int q = c + 1; q < s; ++q)
-      if (fabs(G[q * s + c]) > fabs(G[p * s + c])) p = q;
-    if (p != c) {
-      for (int q = 0; q < s; ++q) { const double tq = G[c * s + q]; G[c * s + q] = G[p * s + q]; G[p * s + q] = tq; }
-      const double tr = r[c]; r[c] = r[p]; r[p] = tr;
-    }
-    const double d = G[c * s + c];
-    for (int q = c + 1; q < s; ++q) {
-      const double f = d != 0.0 ? G[q * s + c] / d : 0.0;
-      if (f == 0.0) continue;
-      for (int k2 = c; k2 < s; ++k2) G[q * s + k2] -= f * G[c * s + k2];
-      r[q] -= f * r[c];
-    }
-  }
-  for (int c = s - 1; c >= 0; --c) {
-    double acc = r[c];
-    for (int k2 = c + 1; k2 < s; ++k2) acc -= G[c * s + k2] * sm->y[k2];
-    const double d = G[c * s + c];
-    sm->y[c] = d != 0.0 ? acc / d : 0.0;
-  }
-  sm->steps_total += s;
+  lstsq_body(sm, R, sm->rnorm, G, r);
   sm->outer += 1;
 }
 __global__ void __launch_bounds__(256) kb_combine_kernel(const KMem *__restrict__ km) {

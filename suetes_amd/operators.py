@@ -29,6 +29,34 @@ class CGridOperator3D:
     def m3(self, loc):
         return self.grid.i_m_factors[loc][None]
 
+    def _stagger(self, f, axis, from_loc, to_loc, op):
+        """avg (op 0) / diff (op 1) on a device tensor in one HIP launch
+        (forward and transpose); None when the call needs the tensor path
+        (host tensors, mixed dtypes)."""
+        if not f.is_cuda or f.dim() != 3 or f.dtype not in (torch.float64, torch.float32):
+            return None
+        d = DIM[axis]
+        n = f.shape[d]
+        if n < 2:
+            return None
+        if axis in self.periodic_axes and from_loc == "m" and to_loc in ("u", "v"):
+            pad = 2
+        elif from_loc == "m" and to_loc in ("u", "v", "w"):
+            pad = 1
+        else:
+            pad = 0
+        m3 = None
+        if op == 1 and axis in (0, 1):
+            m3 = self.grid.i_m_factors[to_loc]
+            if m3.dtype != f.dtype or m3.requires_grad:
+                return None
+        shape = list(f.shape)
+        shape[d] = n + 1 if pad else n - 1
+        if m3 is not None and tuple(m3.shape) != tuple(shape[1:]):
+            return None
+        rd = 1.0 / self.grid.delta[axis] if op == 1 else 0.0
+        return _Stagger.apply(f, d, pad, op, rd, m3, not self.use_stop_grad, tuple(shape))
+
     def _apply_padding(self, f, axis, from_loc, to_loc):
         # operators.py:37-63: duplicate the edge values (detached when use_stop_grad)
         if from_loc == "m" and to_loc in ("u", "v", "w"):
@@ -42,6 +70,9 @@ class CGridOperator3D:
 
     def diff(self, f, axis, from_loc, to_loc):
         # operators.py:65-103
+        out = self._stagger(f, axis, from_loc, to_loc, 1)
+        if out is not None:
+            return out
         d = DIM[axis]
         delta = self.grid.delta[axis]
         n = f.shape[d]
@@ -57,6 +88,9 @@ class CGridOperator3D:
 
     def avg(self, f, axis, from_loc, to_loc):
         # operators.py:105-132
+        out = self._stagger(f, axis, from_loc, to_loc, 0)
+        if out is not None:
+            return out
         d = DIM[axis]
         n = f.shape[d]
         a = 0.5 * (f.narrow(d, 0, n - 1) + f.narrow(d, 1, n - 1))
@@ -64,6 +98,61 @@ class CGridOperator3D:
             b = 0.5 * (f.narrow(d, 0, 1) + f.narrow(d, n - 1, 1))
             return torch.cat([b, a, b], dim=d)
         return self._apply_padding(a, axis, from_loc, to_loc)
+
+
+_STAGGER_FNS = {}
+
+
+def _stagger_fn(transpose, dtype):
+    key = (transpose, dtype)
+    fn = _STAGGER_FNS.get(key)
+    if fn is None:
+        sfx = "f64" if dtype == torch.float64 else "f32"
+        fn = getattr(_lib.lib(), f"sisl_stagger_T_{sfx}" if transpose else f"sisl_stagger_{sfx}")
+        _STAGGER_FNS[key] = fn
+    return fn
+
+
+def _stagger_launch(transpose, src, out_shape, d, pad, op, rd, m3, exact):
+    src = src.contiguous()
+    out = torch.empty(out_shape, dtype=src.dtype, device=src.device)
+    shape = out_shape if transpose else src.shape
+    outer = shape[0] if d >= 1 else 1
+    if d == 2:
+        outer *= shape[1]
+    inner = shape[2] if d <= 1 else 1
+    if d == 0:
+        inner *= shape[1]
+    n = shape[d]
+    m3p, m3n = (m3.data_ptr(), m3.numel()) if m3 is not None else (None, 0)
+    stream = torch.cuda.current_stream(src.device).cuda_stream
+    if transpose:
+        rc = _stagger_fn(True, src.dtype)(src.data_ptr(), out.data_ptr(), outer, n, inner, op, pad,
+                                          int(exact), rd, m3p, m3n, stream)
+    else:
+        rc = _stagger_fn(False, src.dtype)(src.data_ptr(), out.data_ptr(), outer, n, inner, op, pad,
+                                           rd, m3p, m3n, stream)
+    if rc:
+        _lib.check(rc, "staggered operator")
+    return out
+
+
+class _Stagger(torch.autograd.Function):
+    """One staggered average / difference (with edge closure and map factor)
+    as a HIP launch; the backward is its gather-form transpose."""
+
+    @staticmethod
+    def forward(ctx, f, d, pad, op, rd, m3, exact, out_shape):
+        m3c = m3.contiguous() if m3 is not None else None
+        ctx.args = (d, pad, op, rd, exact, tuple(f.shape))
+        ctx.m3 = m3c
+        return _stagger_launch(False, f, out_shape, d, pad, op, rd, m3c, exact)
+
+    @staticmethod
+    def backward(ctx, g):
+        d, pad, op, rd, exact, in_shape = ctx.args
+        gf = _stagger_launch(True, g, in_shape, d, pad, op, rd, ctx.m3, exact)
+        return gf, None, None, None, None, None, None, None
 
 
 # ---------------------------------------------------------------------------
