@@ -577,6 +577,118 @@ int tricubic_adj(int fnx, int fny, int fnz, const T *co, const int32_t *perm, co
   return SISL_OK;
 }
 
+// ---- adjoint over sorted rows -------------------------------------------------
+// The plan's perm lists the points sorted by base cell key = at(bx, by, bz)
+// (x fastest), so the sources whose base lies in x-range [x0, x1] of one
+// (by, bz) row occupy one contiguous range of sorted positions.  A
+// destination (i, j, k) can only receive from bases in [i-2, i+1] x [j-2, j+1]
+// x [k-2, k+1] (clamped to the grid), i.e. from 16 such ranges.  Per sorted
+// position the plan extension keeps the source's 12 Catmull-Rom weights and
+// its base x; each call gathers g into sorted order once; the transpose then
+// sums, per destination, weight x weight x weight x g over its 16 ranges in a
+// fixed order (deterministic).
+
+// weight of index i of an axis of length n in the 4-tap stencil of base b
+template <typename T>
+__device__ __forceinline__ T tap_weight(int i, int b, int n, const T *__restrict__ w) {
+  if (b >= 1 && b + 2 <= n - 1) return w[i - b + 1];  // no clamping: exactly one tap
+  T s = 0;
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+    if (min(max(b - 1 + t, 0), n - 1) == i) s += w[t];
+  return s;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) adj_weights_kernel(Dims d, const T *__restrict__ co,
+                                                          const int32_t *__restrict__ perm,
+                                                          T *__restrict__ ws, int32_t *__restrict__ bxs) {
+  const int64_t n = d.n();
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const int64_t p = perm[r];
+  const int X = (int)(p % d.nx), Y = (int)((p / d.nx) % d.ny), Z = (int)(p / ((int64_t)d.nx * d.ny));
+  T tx, ty, tz;
+  const int bx = offset_split(X, co[p], d.nx, tx);
+  offset_split(Y, co[n + p], d.ny, ty);
+  offset_split(Z, co[2 * n + p], d.nz, tz);
+  T w[12];
+  cr_weights(tx, w);
+  cr_weights(ty, w + 4);
+  cr_weights(tz, w + 8);
+#pragma unroll
+  for (int a = 0; a < 12; ++a) ws[r * 12 + a] = w[a];
+  bxs[r] = bx;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) adj_gather_kernel(int64_t n, const int32_t *__restrict__ perm,
+                                                         const T *__restrict__ g, T *__restrict__ gs) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < n) gs[r] = g[perm[r]];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) tricubic_adj_rows_kernel(Dims d, const int32_t *__restrict__ start,
+                                                                const T *__restrict__ ws,
+                                                                const int32_t *__restrict__ bxs,
+                                                                const T *__restrict__ gs,
+                                                                T *__restrict__ gf) {
+  const uint32_t n = (uint32_t)d.n();
+  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const int nx = d.nx, ny = d.ny, nz = d.nz;
+  const int i = (int)(q % (uint32_t)nx);
+  const uint32_t rq = q / (uint32_t)nx;
+  const int j = (int)(rq % (uint32_t)ny), k = (int)(rq / (uint32_t)ny);
+  const int x0 = max(i - 2, 0), x1 = min(i + 1, nx - 1);
+  T acc = 0;
+  for (int bz = max(k - 2, 0); bz <= min(k + 1, nz - 1); ++bz)
+    for (int by = max(j - 2, 0); by <= min(j + 1, ny - 1); ++by) {
+      const uint32_t row = ((uint32_t)bz * ny + by) * nx;
+      const int32_t r0 = start[row + x0], r1 = start[row + x1 + 1];
+      for (int32_t r = r0; r < r1; ++r) {
+        const T *w = ws + (int64_t)r * 12;
+        const T wxi = tap_weight(i, bxs[r], nx, w);
+        const T wyj = tap_weight(j, by, ny, w + 4);
+        const T wzk = tap_weight(k, bz, nz, w + 8);
+        acc += wxi * (wyj * (wzk * gs[r]));
+      }
+    }
+  gf[q] = acc;
+}
+
+template <typename T>
+int tricubic_adj_weights(int fnx, int fny, int fnz, const T *co, const int32_t *perm, T *ws,
+                         int32_t *bxs, void *st) {
+  if (!co || !perm || !ws || !bxs) return SISL_ERR_ARG;
+  Dims d{fnx, fny, fnz};
+  if (d.n() >= (int64_t)1 << 31) return SISL_ERR_ARG;
+  hipLaunchKernelGGL(adj_weights_kernel<T>, dim3(grid_for(d.n())), dim3(kBlock), 0, as_stream(st), d, co,
+                     perm, ws, bxs);
+  SISL_CHECK_LAUNCH();
+  return SISL_OK;
+}
+
+template <typename T>
+int tricubic_adj_rows(int fnx, int fny, int fnz, const int32_t *perm, const int32_t *start, const T *ws,
+                      const int32_t *bxs, const T *g, T *gs, T *gf, void *st) {
+  if (!perm || !start || !ws || !bxs || !g || !gs || !gf || g == gf) return SISL_ERR_ARG;
+  Dims d{fnx, fny, fnz};
+  const int64_t n = d.n();
+  if (n >= (int64_t)1 << 31) return SISL_ERR_ARG;
+  hipStream_t hs = as_stream(st);
+  // g in, gs out and in, gf out; plan: start, perm, 12 weights and base x per point
+  const double bytes = (double)n * (4.0 * sizeof(T) + 12.0 * sizeof(T) + 3.0 * 4.0);
+  const int tid = timer_begin(SISL_T_TRICUBIC_ADJ, sizeof(T) == 4, bytes, nullptr, nullptr, hs);
+  hipLaunchKernelGGL(adj_gather_kernel<T>, dim3(grid_for(n)), dim3(kBlock), 0, hs, n, perm, g, gs);
+  hipLaunchKernelGGL(tricubic_adj_rows_kernel<T>, dim3(grid_for(n)), dim3(kBlock), 0, hs, d, start, ws, bxs,
+                     (const T *)gs, gf);
+  timer_end(tid, hs);
+  SISL_CHECK_LAUNCH();
+  return SISL_OK;
+}
+
 template <typename T>
 int departure(const SislGrid *g, int loc, const T *u, const T *v, const T *e, const T *m2,
               double dx, double dy, double dt, int iters, T *scratch3, T *out, void *st) {
@@ -717,6 +829,25 @@ int sisl_departure_f32(const SislGrid *g, int32_t loc, const float *u, const flo
                        const float *e, const float *m2, double dx, double dy, double dt,
                        int32_t it, float *scr, float *out, void *s) {
   return sisl::departure<float>(g, loc, u, v, e, m2, dx, dy, dt, it, scr, out, s);
+}
+
+int sisl_tricubic_adj_weights_f64(int32_t a, int32_t b, int32_t c, const double *co, const int32_t *perm,
+                                  double *weights, int32_t *base_x, void *s) {
+  return sisl::tricubic_adj_weights<double>(a, b, c, co, perm, weights, base_x, s);
+}
+int sisl_tricubic_adj_weights_f32(int32_t a, int32_t b, int32_t c, const float *co, const int32_t *perm,
+                                  float *weights, int32_t *base_x, void *s) {
+  return sisl::tricubic_adj_weights<float>(a, b, c, co, perm, weights, base_x, s);
+}
+int sisl_tricubic_adj_rows_f64(int32_t a, int32_t b, int32_t c, const int32_t *perm, const int32_t *start,
+                               const double *weights, const int32_t *base_x, const double *g,
+                               double *g_sorted, double *gf, void *s) {
+  return sisl::tricubic_adj_rows<double>(a, b, c, perm, start, weights, base_x, g, g_sorted, gf, s);
+}
+int sisl_tricubic_adj_rows_f32(int32_t a, int32_t b, int32_t c, const int32_t *perm, const int32_t *start,
+                               const float *weights, const int32_t *base_x, const float *g,
+                               float *g_sorted, float *gf, void *s) {
+  return sisl::tricubic_adj_rows<float>(a, b, c, perm, start, weights, base_x, g, g_sorted, gf, s);
 }
 
 }  // extern "C"

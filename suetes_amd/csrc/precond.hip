@@ -338,12 +338,14 @@ __global__ void __launch_bounds__(512) precond_T_seg_kernel(OpT<T> o, const T *_
 }
 
 // u, v faces of the transpose: zbar passes through and each face gathers half
-// of the mass-point cotangent of its (one or two) neighbouring columns.  A
-// 64x4 block covers 64 faces in x and 4 rows of one level; the thread at
-// (i, j, k) does the u face (i, j, k) and the v face (i, j, k).
+// of the mass-point cotangent of its (one or two) neighbouring columns.  One
+// thread per cell (i, j, k) of a 64x4 tile of one level: it does the u face i
+// and the v face j, and the cells of the last column / row also the east /
+// north domain faces, so that every thread has work when nx and ny are
+// multiples of 64 and 4.
 constexpr int kFTY = 4;
 __host__ __device__ inline int faces_blocks(const Stag &s) {
-  return ((s.nx + 1 + 63) / 64) * ((s.ny + 1 + kFTY - 1) / kFTY) * s.nz;
+  return ((s.nx + 63) / 64) * ((s.ny + kFTY - 1) / kFTY) * s.nz;
 }
 template <typename T>
 __device__ __forceinline__ void precond_T_faces_body(const OpT<T> &o, const T *__restrict__ zb,
@@ -352,24 +354,28 @@ __device__ __forceinline__ void precond_T_faces_body(const OpT<T> &o, const T *_
                                                      T *__restrict__ rb, int blk) {
   const Stag &s = o.s;
   const int nx = s.nx, ny = s.ny;
-  const int nbx = (nx + 1 + 63) / 64, nby = (ny + 1 + kFTY - 1) / kFTY;
+  const int nbx = (nx + 63) / 64, nby = (ny + kFTY - 1) / kFTY;
   const int bx = blk % nbx, r = blk / nbx, by = r % nby, k = r / nby;
   const int i = bx * 64 + threadIdx.x, j = by * kFTY + threadIdx.y;
+  if (i >= nx || j >= ny) return;
   const T half = (T)0.5;
-  const uint32_t mrow = ((uint32_t)k * ny + j) * nx;  // mass index of (0, j, k)
-  if (j < ny && i <= nx) {
+  const uint32_t m = ((uint32_t)k * ny + j) * nx + i;  // mass index of (i, j, k)
+  const T uc = ldg(umb, m), vc = ldg(vmb, m);
+  {
     const uint32_t q = ((uint32_t)k * ny + j) * (nx + 1) + i;
     T acc = 0;
-    if (i >= 1) acc += ldg(umb, mrow + i - 1);
-    if (i < nx) acc += ldg(umb, mrow + i);
+    if (i >= 1) acc += ldg(umb, m - 1);
+    acc += uc;
     stg(rb + s.ou, q, ldg(zb + s.ou, q) + half * acc);
+    if (i == nx - 1) stg(rb + s.ou, q + 1, ldg(zb + s.ou, q + 1) + half * uc);
   }
-  if (i < nx && j <= ny) {
+  {
     const uint32_t q = ((uint32_t)k * (ny + 1) + j) * nx + i;
     T acc = 0;
-    if (j >= 1) acc += ldg(vmb, mrow - nx + i);
-    if (j < ny) acc += ldg(vmb, mrow + i);
+    if (j >= 1) acc += ldg(vmb, m - nx);
+    acc += vc;
     stg(rb + s.ov, q, ldg(zb + s.ov, q) + half * acc);
+    if (j == ny - 1) stg(rb + s.ov, q + nx, ldg(zb + s.ov, q + nx) + half * vc);
   }
 }
 

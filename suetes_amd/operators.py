@@ -177,6 +177,23 @@ class TricubicPlan:
         _lib.check(fn(fx, fy, fz, coords.data_ptr(), self.key.data_ptr(), self.perm.data_ptr(),
                       self.start.data_ptr(), count.data_ptr(), sums.data_ptr(), _lib.stream_ptr(dev)),
                    "tricubic adjoint plan")
+        self.dims = (fx, fy, fz)
+        self.weights = None
+        self.base_x = None
+
+    def ensure_rows(self, coords: torch.Tensor):
+        """Per sorted position: the 12 Catmull-Rom weights and the base x of
+        the source point (for the sorted-row transpose)."""
+        if self.weights is not None:
+            return
+        fx, fy, fz = self.dims
+        n = fx * fy * fz
+        self.weights = torch.empty(12 * n, dtype=coords.dtype, device=coords.device)
+        self.base_x = torch.empty(n, dtype=torch.int32, device=coords.device)
+        L = _lib.lib()
+        fn = L.sisl_tricubic_adj_weights_f64 if coords.dtype == torch.float64 else L.sisl_tricubic_adj_weights_f32
+        _lib.check(fn(fx, fy, fz, coords.data_ptr(), self.perm.data_ptr(), self.weights.data_ptr(),
+                      self.base_x.data_ptr(), _lib.stream_ptr(coords.device)), "tricubic adjoint weights")
 
 
 class Coords:
@@ -226,16 +243,7 @@ class _Tricubic(torch.autograd.Function):
     def backward(ctx, g):
         if ctx.use_limiter:
             raise NotImplementedError("the adjoint of the limited tricubic is not implemented")
-        co = ctx.coords.data
-        plan = ctx.coords.plan()
-        g = g.contiguous()
-        gf = torch.empty_like(g)
-        fz, fy, fx = g.shape
-        L = _lib.lib()
-        fn = L.sisl_tricubic_adj_f64 if g.dtype == torch.float64 else L.sisl_tricubic_adj_f32
-        _lib.check(fn(fx, fy, fz, co.data_ptr(), plan.perm.data_ptr(), plan.start.data_ptr(),
-                      g.data_ptr(), gf.data_ptr(), _lib.stream_ptr(g.device)), "tricubic adjoint")
-        return gf, None, None
+        return tricubic_transpose(g.contiguous(), ctx.coords), None, None
 
 
 def tensor_product_interp_3d(field, coords, use_limiter=False):
@@ -252,15 +260,30 @@ def tensor_product_interp_3d(field, coords, use_limiter=False):
     return _Tricubic.apply(field, coords, use_limiter)
 
 
-def tricubic_transpose(g, coords):
-    """Explicit T^T g (deterministic), for tests and custom adjoints."""
+def tricubic_transpose(g, coords, method="rows"):
+    """Explicit T^T g (deterministic), the adjoint of the unlimited gather.
+    ``method='rows'``: the sorted-row kernel (plan weights built once per
+    coordinate set); ``'segments'``: the per-destination segment gather."""
     if not isinstance(coords, Coords):
         coords = Coords(coords)
+    _lib.require_device(g, coords.data)
     plan = coords.plan()
+    g = g.contiguous()
     gf = torch.empty_like(g)
     fz, fy, fx = g.shape
     L = _lib.lib()
-    fn = L.sisl_tricubic_adj_f64 if g.dtype == torch.float64 else L.sisl_tricubic_adj_f32
-    _lib.check(fn(fx, fy, fz, coords.data.data_ptr(), plan.perm.data_ptr(), plan.start.data_ptr(),
-                  g.data_ptr(), gf.data_ptr(), _lib.stream_ptr(g.device)), "tricubic adjoint")
+    sfx = "f64" if g.dtype == torch.float64 else "f32"
+    if method == "rows":
+        plan.ensure_rows(coords.data)
+        gs = torch.empty_like(g)
+        _lib.check(getattr(L, f"sisl_tricubic_adj_rows_{sfx}")(
+            fx, fy, fz, plan.perm.data_ptr(), plan.start.data_ptr(), plan.weights.data_ptr(),
+            plan.base_x.data_ptr(), g.data_ptr(), gs.data_ptr(), gf.data_ptr(), _lib.stream_ptr(g.device)),
+            "tricubic adjoint")
+    elif method == "segments":
+        _lib.check(getattr(L, f"sisl_tricubic_adj_{sfx}")(
+            fx, fy, fz, coords.data.data_ptr(), plan.perm.data_ptr(), plan.start.data_ptr(), g.data_ptr(),
+            gf.data_ptr(), _lib.stream_ptr(g.device)), "tricubic adjoint")
+    else:
+        raise ValueError(f"unknown method {method!r}")
     return gf

@@ -92,8 +92,13 @@ def test_tricubic_staged_and_far_departures_match_oracle(lim):
     assert rel_l2(from_int(out32), ref) < 2e-6
 
 
-def test_tricubic_adjoint_dot_and_determinism():
-    from suetes_amd.operators import Coords, tensor_product_interp_3d, tricubic_transpose
+@pytest.mark.parametrize("method", ["rows", "segments"])
+def test_tricubic_adjoint_dot_and_determinism(method):
+    from functools import partial
+
+    from suetes_amd.operators import Coords, tensor_product_interp_3d
+    from suetes_amd.operators import tricubic_transpose as tt
+    tricubic_transpose = partial(tt, method=method)
     rng = np.random.default_rng(11)
     shape = (13, 11, 9)
     pts = np.stack([rng.uniform(-1, s, size=shape) for s in shape])
@@ -121,6 +126,21 @@ def test_tricubic_adjoint_dot_and_determinism():
     gg = rng.standard_normal(small)
     ref = (Tm.T @ gg.ravel()).reshape(small)
     assert rel_l2(from_int(tricubic_transpose(to_int(gg), c2)), ref) < 1e-13
+
+
+def test_tricubic_adjoint_methods_agree():
+    """The sorted-row transpose and the segment transpose are the same
+    operator (summation order aside), in FP64 and FP32."""
+    from suetes_amd.operators import Coords, tricubic_transpose
+    rng = np.random.default_rng(12)
+    shape = (40, 33, 21)
+    pts = np.stack([rng.uniform(-1.5, s + 0.5, size=shape) for s in shape])
+    for dt, tol in ((torch.float64, 1e-14), (torch.float32, 1e-6)):
+        co = Coords(offsets_int(pts).to(dt))
+        g = to_int(rng.standard_normal(shape), dt)
+        a = tricubic_transpose(g, co, method="rows")
+        b = tricubic_transpose(g, co, method="segments")
+        assert rel_l2(from_int(a), from_int(b)) < tol, dt
 
 
 @pytest.mark.parametrize("cfgf", [small_lam, small_bubble])
