@@ -563,50 +563,77 @@ __global__ void __launch_bounds__(256) combine_kernel(const TV *__restrict__ V, 
 
 // Least squares of one cycle, min || beta e1 - H^T y || over the `steps`
 // columns, by the normal equations with partial pivoting (JAX's
-// _gmres_batched solve of the small system), one thread.  G, r: scratch of
-// kMaxRestart^2 and kMaxRestart doubles.
-__device__ void lstsq_body(Small *sm, int R, double beta, double *G, double *r) {
+// _gmres_batched solve of the small system), by one workgroup: H is staged
+// in LDS, each entry of G = H H^T is one thread's dot product (fixed order),
+// the elimination updates the rows below the pivot in parallel, the back
+// substitution is one thread's.  Same arithmetic as the sequential
+// algorithm.  All threads of the block call it.
+constexpr int kLsqThreads = 256;
+__device__ void lstsq_block(Small *sm, int R, double beta) {
+  __shared__ double Hs[kMaxRestart * (kMaxRestart + 1)];
+  __shared__ double G[kMaxRestart * kMaxRestart];
+  __shared__ double r[kMaxRestart], f[kMaxRestart], y[kMaxRestart];
+  __shared__ int piv;
+  const int t = threadIdx.x, nt = blockDim.x;
   const int s = sm->steps;
-  for (int a = 0; a < s; ++a) {
-    for (int b = 0; b < s; ++b) {
-      double acc = 0;
-      for (int l = 0; l <= s; ++l) acc += sm->H[a * (R + 1) + l] * sm->H[b * (R + 1) + l];
-      G[a * s + b] = acc;
-    }
-    r[a] = sm->H[a * (R + 1)] * beta;
+  for (int q = t; q < s * (R + 1); q += nt) Hs[q] = sm->H[q];
+  __syncthreads();
+  for (int q = t; q < s * s; q += nt) {
+    const int a = q / s, b = q % s;
+    double acc = 0;
+    for (int l = 0; l <= s; ++l) acc += Hs[a * (R + 1) + l] * Hs[b * (R + 1) + l];
+    G[q] = acc;
   }
+  for (int a = t; a < s; a += nt) r[a] = Hs[a * (R + 1)] * beta;
+  __syncthreads();
   for (int c = 0; c < s; ++c) {
-    int p = c;
-    for (int q = c + 1; q < s; ++q)
-      if (fabs(G[q * s + c]) > fabs(G[p * s + c])) p = q;
+    if (t == 0) {
+      int p = c;
+      for (int q = c + 1; q < s; ++q)
+        if (fabs(G[q * s + c]) > fabs(G[p * s + c])) p = q;
+      piv = p;
+    }
+    __syncthreads();
+    const int p = piv;
     if (p != c) {
-      for (int q = 0; q < s; ++q) { const double tq = G[c * s + q]; G[c * s + q] = G[p * s + q]; G[p * s + q] = tq; }
-      const double tr = r[c]; r[c] = r[p]; r[p] = tr;
+      for (int q = t; q < s; q += nt) {
+        const double tq = G[c * s + q];
+        G[c * s + q] = G[p * s + q];
+        G[p * s + q] = tq;
+      }
+      if (t == 0) { const double tr = r[c]; r[c] = r[p]; r[p] = tr; }
     }
+    __syncthreads();
     const double d = G[c * s + c];
-    for (int q = c + 1; q < s; ++q) {
-      const double f = d != 0.0 ? G[q * s + c] / d : 0.0;
-      if (f == 0.0) continue;
-      for (int k2 = c; k2 < s; ++k2) G[q * s + k2] -= f * G[c * s + k2];
-      r[q] -= f * r[c];
+    for (int q = c + 1 + t; q < s; q += nt) f[q] = d != 0.0 ? G[q * s + c] / d : 0.0;
+    __syncthreads();
+    const int w = s - c;  // columns c..s-1
+    for (int e = t; e < (s - c - 1) * w; e += nt) {
+      const int q = c + 1 + e / w, k2 = c + e % w;
+      if (f[q] != 0.0) G[q * s + k2] -= f[q] * G[c * s + k2];
     }
+    for (int q = c + 1 + t; q < s; q += nt)
+      if (f[q] != 0.0) r[q] -= f[q] * r[c];
+    __syncthreads();
   }
-  for (int c = s - 1; c >= 0; --c) {
-    double acc = r[c];
-    for (int k2 = c + 1; k2 < s; ++k2) acc -= G[c * s + k2] * sm->y[k2];
-    const double d = G[c * s + c];
-    sm->y[c] = d != 0.0 ? acc / d : 0.0;
+  if (t == 0) {
+    for (int c = s - 1; c >= 0; --c) {
+      double acc = r[c];
+      for (int k2 = c + 1; k2 < s; ++k2) acc -= G[c * s + k2] * y[k2];
+      const double d = G[c * s + c];
+      y[c] = d != 0.0 ? acc / d : 0.0;
+    }
+    sm->steps_total += s;
   }
-  sm->steps_total += s;
+  __syncthreads();
+  for (int c = t; c < s; c += nt) sm->y[c] = y[c];
 }
 
 // Single-solve driver: the cycle's least squares on the device, so that the
 // host does not wait for the Hessenberg matrix (one synchronisation per
 // cycle, for the residual norm, remains).
-__global__ void lstsq_kernel(Small *sm, int R, double beta) {
-  __shared__ double G[kMaxRestart * kMaxRestart], r[kMaxRestart];
-  if (threadIdx.x != 0) return;
-  lstsq_body(sm, R, beta, G, r);
+__global__ void __launch_bounds__(kLsqThreads) lstsq_kernel(Small *sm, int R, double beta) {
+  lstsq_block(sm, R, beta);
 }
 
 __global__ void zero_steps_total_kernel(Small *sm) { sm->steps_total = 0; }
@@ -866,7 +893,7 @@ static int gmres64(const OpT<double> &op, const SislGmresParams &p, const double
                        p.early_exit ? ptol : 0.0, K.flags);
     for (int j = 0; j < R; ++j)
       if ((rc = K.arnoldi(j, R, p.early_exit))) return rc;
-    hipLaunchKernelGGL(lstsq_kernel, dim3(1), dim3(64), 0, s, sm, R, rnorm);
+    hipLaunchKernelGGL(lstsq_kernel, dim3(1), dim3(kLsqThreads), 0, s, sm, R, rnorm);
     combine<double>(K.V, K.ldv(), R, sm->y, x, n, 1.0, s, &sm->steps);
     if ((rc = residual(&sm->scal[2]))) return rc;
     if ((rc = sync_copy(hs, sm->scal, sizeof(double) * 3, s))) return rc;
@@ -940,7 +967,7 @@ static int gmres_mixed(const OpT<double> &op64, const OpT<float> &op32, const Si
     hipLaunchKernelGGL(reset_cycle_kernel, dim3(1), dim3(kBlock), 0, s, sm, R, rnorm, target, K.flags);
     for (int j = 0; j < R; ++j)
       if ((rc = K.arnoldi(j, R, 1))) return rc;
-    hipLaunchKernelGGL(lstsq_kernel, dim3(1), dim3(64), 0, s, sm, R, rnorm);
+    hipLaunchKernelGGL(lstsq_kernel, dim3(1), dim3(kLsqThreads), 0, s, sm, R, rnorm);
     combine<TB>(K.V, K.ldv(), R, sm->y, x, n, 1.0 / K.vscale(), s, &sm->steps);
     ++outer;
   }
@@ -1123,15 +1150,13 @@ __global__ void kb_reset_kernel(const KMem *__restrict__ km, int R, double inner
     sm->target = fmax(inner_rtol * sm->rnorm, 0.5 * sm->ptol);
   }
 }
-// Least squares of the cycle (lstsq_body) on the device, one thread per
+// Least squares of the cycle (lstsq_block) on the device, one workgroup per
 // member; y into sm->y.
-__global__ void kb_lstsq_kernel(const KMem *__restrict__ km, int R) {
-  __shared__ double G[kMaxRestart * kMaxRestart], r[kMaxRestart];
-  if (threadIdx.x != 0) return;
+__global__ void __launch_bounds__(kLsqThreads) kb_lstsq_kernel(const KMem *__restrict__ km, int R) {
   Small *sm = km[blockIdx.x].sm;
-  if (sm->done) return;
-  lstsq_body(sm, R, sm->rnorm, G, r);
-  sm->outer += 1;
+  if (sm->done) return;  // uniform over the block
+  lstsq_block(sm, R, sm->rnorm);
+  if (threadIdx.x == 0) sm->outer += 1;
 }
 __global__ void __launch_bounds__(256) kb_combine_kernel(const KMem *__restrict__ km) {
   const KMem &k = km[blockIdx.y];
@@ -1375,7 +1400,7 @@ static int ens_solve(const void *ws, const SislGmresParams &p, const double *con
     if (all || outer > max_outer) break;
     for (int j = 0; j < R; ++j)
       if ((rc = ens_arnoldi(P, j, p.transpose, st))) return rc;
-    hipLaunchKernelGGL(kb_lstsq_kernel, dim3(M), dim3(64), 0, st, (const KMem *)P.km, R);
+    hipLaunchKernelGGL(kb_lstsq_kernel, dim3(M), dim3(kLsqThreads), 0, st, (const KMem *)P.km, R);
     hipLaunchKernelGGL(kb_combine_kernel, dim3(kRedGrid, M), dim3(kBlock), 0, st, (const KMem *)P.km);
   }
   for (int m = 0; m < M; ++m) {
