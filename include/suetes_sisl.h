@@ -245,6 +245,26 @@ int sisl_stagger_T_f32(const float *g, float *gf, int64_t outer, int32_t n, int6
                        int32_t pad, int32_t exact, double rd, const float *m3, int64_t m3n,
                        void *stream);
 
+/* ---- flux-form semi-Lagrangian transport ------------------------------- */
+
+/* FluxFormAdvector.advect_1d (suetes/regional3d/steppers.py:552-594) along
+ * one axis of a contiguous (outer, n, inner) array: mass (outer, n, inner),
+ * face Courant numbers cfl (outer, n + 1, inner), out like mass; periodic =
+ * 1 wraps the line.  scratch: outer * (n + 1) * inner elements. */
+int sisl_ffsl_1d_f64(const double *mass, const double *cfl, double *out, double *scratch, int64_t outer,
+                     int32_t n, int64_t inner, int32_t periodic, void *stream);
+int sisl_ffsl_1d_f32(const float *mass, const float *cfl, float *out, float *scratch, int64_t outer,
+                     int32_t n, int64_t inner, int32_t periodic, void *stream);
+/* Transpose: cotangents of mass and of cfl from the cotangent of out (the
+ * clamp of a closed line passes half a cotangent at its bounds, as the
+ * tensor operators' maximum / minimum).  scratch: 2 * outer * (n+1) * inner. */
+int sisl_ffsl_1d_T_f64(const double *mass, const double *cfl, const double *gout, double *gmass,
+                       double *gcfl, double *scratch, int64_t outer, int32_t n, int64_t inner,
+                       int32_t periodic, void *stream);
+int sisl_ffsl_1d_T_f32(const float *mass, const float *cfl, const float *gout, float *gmass, float *gcfl,
+                       float *scratch, int64_t outer, int32_t n, int64_t inner, int32_t periodic,
+                       void *stream);
+
 /* ---- kernel timing (measurement runs) ------------------------------------ */
 
 /* Timed kernels: operator apply, column preconditioner, the Krylov

@@ -28,6 +28,7 @@ EXPORTED = [
     "sisl_gmres_batched_setup", "sisl_gmres_batched", "sisl_stagger_f64", "sisl_stagger_f32",
     "sisl_stagger_T_f64", "sisl_stagger_T_f32", "sisl_tricubic_adj_weights_f64",
     "sisl_tricubic_adj_weights_f32", "sisl_tricubic_adj_rows_f64", "sisl_tricubic_adj_rows_f32",
+    "sisl_ffsl_1d_f64", "sisl_ffsl_1d_f32", "sisl_ffsl_1d_T_f64", "sisl_ffsl_1d_T_f32",
 ]
 
 # timed kernels (enum SislTimedKernel)
@@ -114,6 +115,9 @@ def lib():
         for p in ("f64", "f32"):
             getattr(L, f"sisl_stagger_{p}").argtypes = [vp, vp, i64, i32, i64, i32, i32, d, vp, i64, vp]
             getattr(L, f"sisl_stagger_T_{p}").argtypes = [vp, vp, i64, i32, i64, i32, i32, i32, d, vp, i64, vp]
+        for p in ("f64", "f32"):
+            getattr(L, f"sisl_ffsl_1d_{p}").argtypes = [vp, vp, vp, vp, i64, i32, i64, i32, vp]
+            getattr(L, f"sisl_ffsl_1d_T_{p}").argtypes = [vp, vp, vp, vp, vp, vp, i64, i32, i64, i32, vp]
         L.sisl_timer_enable.argtypes = [i32]
         L.sisl_timer_read.argtypes = [i32, i32, C.POINTER(d), C.POINTER(i64), C.POINTER(i64),
                                       C.POINTER(d)]

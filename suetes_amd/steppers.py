@@ -140,6 +140,53 @@ class SemiLagrangianAdvector3D:
 
 
 # ---------------------------------------------------------------------------
+def _ffsl_launch(transpose, mass, cfl, dim, periodic, gout=None):
+    shape = mass.shape
+    outer = 1
+    for k in range(dim):
+        outer *= shape[k]
+    inner = 1
+    for k in range(dim + 1, 3):
+        inner *= shape[k]
+    n = shape[dim]
+    faces = outer * (n + 1) * inner
+    sfx = "f64" if mass.dtype == torch.float64 else "f32"
+    L = _lib.lib()
+    stream = _lib.stream_ptr(mass.device)
+    if not transpose:
+        out = torch.empty_like(mass)
+        scr = torch.empty(faces, dtype=mass.dtype, device=mass.device)
+        _lib.check(getattr(L, f"sisl_ffsl_1d_{sfx}")(mass.data_ptr(), cfl.data_ptr(), out.data_ptr(),
+                                                     scr.data_ptr(), outer, n, inner, int(periodic), stream),
+                   "flux-form transport")
+        return out
+    gm = torch.empty_like(mass)
+    gc = torch.empty_like(cfl)
+    scr = torch.empty(2 * faces, dtype=mass.dtype, device=mass.device)
+    _lib.check(getattr(L, f"sisl_ffsl_1d_T_{sfx}")(mass.data_ptr(), cfl.data_ptr(), gout.data_ptr(),
+                                                   gm.data_ptr(), gc.data_ptr(), scr.data_ptr(), outer, n,
+                                                   inner, int(periodic), stream), "flux-form transport adjoint")
+    return gm, gc
+
+
+class _Ffsl(torch.autograd.Function):
+    """One split FFSL sweep (advect_1d) as a HIP launch; the backward gives
+    the mass and Courant-number cotangents in one more."""
+
+    @staticmethod
+    def forward(ctx, mass, cfl, dim, periodic):
+        mass, cfl = mass.contiguous(), cfl.contiguous()
+        ctx.save_for_backward(mass, cfl)
+        ctx.dim, ctx.periodic = dim, periodic
+        return _ffsl_launch(False, mass, cfl, dim, periodic)
+
+    @staticmethod
+    def backward(ctx, g):
+        mass, cfl = ctx.saved_tensors
+        gm, gc = _ffsl_launch(True, mass, cfl, ctx.dim, ctx.periodic, gout=g.contiguous())
+        return gm, gc, None, None
+
+
 class FluxFormAdvector:
     """Split flux-form semi-Lagrangian transport (steppers.py:530-647).
 
@@ -152,6 +199,9 @@ class FluxFormAdvector:
 
     @staticmethod
     def advect_1d(mass, cfl, dim, periodic=False):
+        if mass.is_cuda and mass.dim() == 3 and mass.dtype in (torch.float64, torch.float32) \
+                and cfl.dtype == mass.dtype:
+            return _Ffsl.apply(mass, cfl, dim, bool(periodic))
         N = mass.shape[dim]
         shape = [1] * mass.dim()
         shape[dim] = N + 1
