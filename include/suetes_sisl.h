@@ -199,6 +199,8 @@ typedef struct SislGmresParams {
   int32_t max_outer;       /* mixed solver: refinement iterations             */
   int32_t basis_fp16;      /* mixed solver: keep the inner Krylov basis in
                               FP16 (scaled by sqrt(n)) instead of FP32        */
+  int32_t coef_bf16;       /* mixed solver: the FP32 operator's 3-D coefficient
+                              arrays are bfloat16 (2-D ones stay FP32)        */
 } SislGmresParams;
 
 typedef struct SislGmresStats {

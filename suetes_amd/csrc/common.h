@@ -108,6 +108,18 @@ __device__ __forceinline__ void stg(T *base, uint32_t idx, T v) {
   *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + idx * (uint32_t)sizeof(T)) = v;
 }
 
+// Coefficient storage: the working type T, or bfloat16 bits (uint16_t) for
+// the inner solver of the mixed path; loads convert to T.
+using bf16_t = uint16_t;
+template <typename T, typename TC>
+__device__ __forceinline__ T ldc(const TC *base, uint32_t idx) {
+  if constexpr (sizeof(TC) == 2) {
+    return (T)__uint_as_float((uint32_t)ldg(base, idx) << 16);
+  } else {
+    return (T)ldg(base, idx);
+  }
+}
+
 // Wave64 sum reduction through DPP/shuffles.
 template <typename T>
 __device__ inline T wave_sum(T v) {
@@ -165,6 +177,7 @@ struct Fuse {
   void *vout = nullptr;
   int nb = 0;
   double vs = 1.0;
+  int cb = 0;  // 1: the 3-D coefficient arrays are bfloat16 (FP32 operator only)
 };
 template <typename T>
 int launch_A(const OpT<T> &op, const T *x, T *y, hipStream_t s, const int *stop = nullptr,
@@ -172,8 +185,10 @@ int launch_A(const OpT<T> &op, const T *x, T *y, hipStream_t s, const int *stop 
 template <typename T>
 int launch_AT(const OpT<T> &op, const T *x, T *y, T *scratch, hipStream_t s, const int *stop = nullptr,
               const Fuse *fuse = nullptr);
+// cb = 1: the 3-D coefficient arrays of `op` are bfloat16 (FP32 operator only)
 template <typename T>
-int launch_M(const OpT<T> &op, const T *r, T *z, hipStream_t s, const int *stop = nullptr);
+int launch_M(const OpT<T> &op, const T *r, T *z, hipStream_t s, const int *stop = nullptr, int cb = 0);
 template <typename T>
-int launch_MT(const OpT<T> &op, const T *r, T *z, T *scratch, hipStream_t s, const int *stop = nullptr);
+int launch_MT(const OpT<T> &op, const T *r, T *z, T *scratch, hipStream_t s, const int *stop = nullptr,
+              int cb = 0);
 }  // namespace sisl

@@ -14,35 +14,6 @@
 
 namespace sisl {
 
-template <typename T>
-struct AView {
-  const OpT<T> &o;
-  const T *U, *V, *W, *P, *E;
-  __device__ AView(const OpT<T> &op, const T *x)
-      : o(op), U(x + op.s.ou), V(x + op.s.ov), W(x + op.s.ow), P(x + op.s.om), E(x + op.s.oe) {}
-
-  // d(pi)/dz at a w level times the 1/dz_w metric, with the edge closure of
-  // op.diff(m->w) (operators.py:37-63, euler.py:252).
-  __device__ __forceinline__ T gzw(int i, int j, int k) const {
-    const Stag &s = o.s;
-    int ka = k, kb = k - 1;
-    if (k == 0) { ka = 1; kb = 0; }
-    else if (k == s.nz) { ka = s.nz - 1; kb = s.nz - 2; }
-    return (P[s.m.at(i, j, ka)] - P[s.m.at(i, j, kb)]) * o.rdzw[s.w.at(i, j, k)];
-  }
-  __device__ __forceinline__ T gzm(int i, int j, int k) const {
-    return (T)0.5 * (gzw(i, j, k) + gzw(i, j, k + 1));
-  }
-  __device__ __forceinline__ T um(int i, int j, int k) const {
-    const Stag &s = o.s;
-    return (T)0.5 * (U[s.u.at(i, j, k)] + U[s.u.at(i + 1, j, k)]);
-  }
-  __device__ __forceinline__ T vm(int i, int j, int k) const {
-    const Stag &s = o.s;
-    return (T)0.5 * (V[s.v.at(i, j, k)] + V[s.v.at(i, j + 1, k)]);
-  }
-};
-
 // Launch geometry (operator and transpose): one thread per (i, j) column of a
 // 64 x 4 tile, marching a chunk of `kchunk` levels upwards and carrying the
 // vertical neighbours in registers; it writes all five packed outputs of its
@@ -85,7 +56,7 @@ template <typename T> struct BasisT<T, 1> { using type = _Float16; };
 
 // Body of the forward operator for launch block `blk` of `nblk` (a member of
 // a batched launch passes its own block index and count).
-template <typename T, int UNR, int NB = 0>
+template <typename T, int UNR, int NB = 0, typename TC = T>
 __device__ __forceinline__ void helmholtz_body(const OpT<T> &o, const T *__restrict__ x,
                                                T *__restrict__ y, int kchunk, int blk, int nblk,
                                                const double *inv = nullptr, void *vout = nullptr,
@@ -108,19 +79,19 @@ __device__ __forceinline__ void helmholtz_body(const OpT<T> &o, const T *__restr
   const T mmx = ldg(o.mmx, (uint32_t)s.m2(i, j)), mmy = ldg(o.mmy, (uint32_t)s.m2(i, j));
   const T half = (T)0.5, alpha = o.alpha;
   // coefficient arrays as restrict locals: loads of level k+1 may move above stores of level k
-  const T *__restrict__ c_cua = o.cua;
-  const T *__restrict__ c_zxu = o.zxu;
-  const T *__restrict__ c_cx = o.cx;
-  const T *__restrict__ c_cva = o.cva;
-  const T *__restrict__ c_zev = o.zev;
-  const T *__restrict__ c_cy = o.cy;
-  const T *__restrict__ c_dw = o.dw;
-  const T *__restrict__ c_cw = o.cw;
-  const T *__restrict__ c_cz = o.cz;
-  const T *__restrict__ c_kx = o.kx;
-  const T *__restrict__ c_ky = o.ky;
-  const T *__restrict__ c_rdzw = o.rdzw;
-  const T *__restrict__ c_cpm = o.cpm;
+  const TC *__restrict__ c_cua = reinterpret_cast<const TC *>(o.cua);
+  const TC *__restrict__ c_zxu = reinterpret_cast<const TC *>(o.zxu);
+  const TC *__restrict__ c_cx = reinterpret_cast<const TC *>(o.cx);
+  const TC *__restrict__ c_cva = reinterpret_cast<const TC *>(o.cva);
+  const TC *__restrict__ c_zev = reinterpret_cast<const TC *>(o.zev);
+  const TC *__restrict__ c_cy = reinterpret_cast<const TC *>(o.cy);
+  const TC *__restrict__ c_dw = reinterpret_cast<const TC *>(o.dw);
+  const TC *__restrict__ c_cw = reinterpret_cast<const TC *>(o.cw);
+  const TC *__restrict__ c_cz = reinterpret_cast<const TC *>(o.cz);
+  const TC *__restrict__ c_kx = reinterpret_cast<const TC *>(o.kx);
+  const TC *__restrict__ c_ky = reinterpret_cast<const TC *>(o.ky);
+  const TC *__restrict__ c_rdzw = reinterpret_cast<const TC *>(o.rdzw);
+  const TC *__restrict__ c_cpm = reinterpret_cast<const TC *>(o.cpm);
   // 32-bit element offsets at level 0 and the plane strides: own column
   // (i, j), left column (il, j), lower column (i, jd).  Mass and w points
   // share the (i, j) plane layout.
@@ -146,17 +117,17 @@ __device__ __forceinline__ void helmholtz_body(const OpT<T> &o, const T *__restr
     vmc_p = half * (ldg(V, vc + qv) + ldg(V, vc + qv + nx));
   }
   if (k0 == 0) {
-    gc0 = (ldg(P, mc + pm) - pc0) * ldg(c_rdzw, mc);
-    gl0 = (ldg(P, ml + pm) - pl0) * ldg(c_rdzw, ml);
-    gd0 = (ldg(P, md + pm) - pd0) * ldg(c_rdzw, md);
+    gc0 = (ldg(P, mc + pm) - pc0) * ldc<T>(c_rdzw, mc);
+    gl0 = (ldg(P, ml + pm) - pl0) * ldc<T>(c_rdzw, ml);
+    gd0 = (ldg(P, md + pm) - pd0) * ldc<T>(c_rdzw, md);
   } else if (k0 < nz) {
     const uint32_t q = (uint32_t)k0 * pm;
-    gc0 = (pc0 - pcm) * ldg(c_rdzw, mc + q);
-    gl0 = (pl0 - plm) * ldg(c_rdzw, ml + q);
-    gd0 = (pd0 - pdm) * ldg(c_rdzw, md + q);
+    gc0 = (pc0 - pcm) * ldc<T>(c_rdzw, mc + q);
+    gl0 = (pl0 - plm) * ldc<T>(c_rdzw, ml + q);
+    gd0 = (pd0 - pdm) * ldc<T>(c_rdzw, md + q);
   }
   T e0 = ldg(E, mc + (uint32_t)k0 * pm);
-  T fz0 = (k0 >= 1 && k0 < nz) ? e0 * ldg(c_cz, mc + (uint32_t)k0 * pm) : (T)0;
+  T fz0 = (k0 >= 1 && k0 < nz) ? e0 * ldc<T>(c_cz, mc + (uint32_t)k0 * pm) : (T)0;
 #pragma unroll UNR
   for (int k = k0; k < k1; ++k) {
     const uint32_t qm = (uint32_t)k * pm, qu = (uint32_t)k * pu, qv = (uint32_t)k * pv;
@@ -164,7 +135,7 @@ __device__ __forceinline__ void helmholtz_body(const OpT<T> &o, const T *__restr
     const T wv = ldg(W, wk);
     if (k == nz) {  // top w level: both rows are identities
       stg(YW, wk, so(wv));
-      stg(YE, wk, so(e0 * ldg(c_rdzw, wk)));
+      stg(YE, wk, so(e0 * ldc<T>(c_rdzw, wk)));
       bput(BW, wk, wv);
       bput(BE, wk, e0);
       break;
@@ -174,14 +145,14 @@ __device__ __forceinline__ void helmholtz_body(const OpT<T> &o, const T *__restr
     if (k + 1 < nz) {
       const uint32_t q1 = qm + pm;
       pc1 = ldg(P, mc + q1); pl1 = ldg(P, ml + q1); pd1 = ldg(P, md + q1);
-      gc1 = (pc1 - pc0) * ldg(c_rdzw, mc + q1);
-      gl1 = (pl1 - pl0) * ldg(c_rdzw, ml + q1);
-      gd1 = (pd1 - pd0) * ldg(c_rdzw, md + q1);
+      gc1 = (pc1 - pc0) * ldc<T>(c_rdzw, mc + q1);
+      gl1 = (pl1 - pl0) * ldc<T>(c_rdzw, ml + q1);
+      gd1 = (pd1 - pd0) * ldc<T>(c_rdzw, md + q1);
     } else {  // top closure: gz_w(nz) repeats the last interior difference
       const uint32_t q1 = qm + pm;
-      gc1 = (pc0 - pcm) * ldg(c_rdzw, mc + q1);
-      gl1 = (pl0 - plm) * ldg(c_rdzw, ml + q1);
-      gd1 = (pd0 - pdm) * ldg(c_rdzw, md + q1);
+      gc1 = (pc0 - pcm) * ldc<T>(c_rdzw, mc + q1);
+      gl1 = (pl0 - plm) * ldc<T>(c_rdzw, ml + q1);
+      gd1 = (pd0 - pdm) * ldc<T>(c_rdzw, md + q1);
     }
     const T gzc = half * (gc0 + gc1), gzl = half * (gl0 + gl1), gzd = half * (gd0 + gd1);
     const uint32_t au = uc + qu, av = vc + qv;
@@ -193,7 +164,7 @@ __device__ __forceinline__ void helmholtz_body(const OpT<T> &o, const T *__restr
       if (uform) {
         const uint32_t al = vl + qv;
         const T vml = half * (ldg(V, al) + ldg(V, al + nx));
-        out = u0 + ldg(c_cua, au) * (mudx * (pc0 - pl0) - ldg(c_zxu, au) * (half * (gzl + gzc))) -
+        out = u0 + ldc<T>(c_cua, au) * (mudx * (pc0 - pl0) - ldc<T>(c_zxu, au) * (half * (gzl + gzc))) -
               afu * (half * (vml + vm_k));
       }
       stg(YU, au, so(out));
@@ -213,7 +184,7 @@ __device__ __forceinline__ void helmholtz_body(const OpT<T> &o, const T *__restr
       if (vform) {
         const uint32_t ad = ud + qu;
         const T umd = half * (ldg(U, ad) + ldg(U, ad + 1));
-        out = v0 + ldg(c_cva, av) * (mvdy * (pc0 - pd0) - ldg(c_zev, av) * (half * (gzd + gzc))) +
+        out = v0 + ldc<T>(c_cva, av) * (mvdy * (pc0 - pd0) - ldc<T>(c_zev, av) * (half * (gzd + gzc))) +
               afv * (half * (umd + um_k));
       }
       stg(YV, av, so(out));
@@ -230,21 +201,21 @@ __device__ __forceinline__ void helmholtz_body(const OpT<T> &o, const T *__restr
     // ---- L_pi (euler.py:298-318, 401) ----
     const uint32_t w1 = wk + pm;
     const T e1 = ldg(E, w1);
-    const T fz1 = (k + 1 < nz) ? e1 * ldg(c_cz, w1) : (T)0;
+    const T fz1 = (k + 1 < nz) ? e1 * ldc<T>(c_cz, w1) : (T)0;
     {
-      const T fx = u1 * ldg(c_cx, au + 1) - u0 * ldg(c_cx, au);
-      const T fy = v1 * ldg(c_cy, av + nx) - v0 * ldg(c_cy, av);
-      stg(YP, wk, so(pc0 + ldg(c_cpm, wk) * (mmx * fx + mmy * fy + (fz1 - fz0))));
+      const T fx = u1 * ldc<T>(c_cx, au + 1) - u0 * ldc<T>(c_cx, au);
+      const T fy = v1 * ldc<T>(c_cy, av + nx) - v0 * ldc<T>(c_cy, av);
+      stg(YP, wk, so(pc0 + ldc<T>(c_cpm, wk) * (mmx * fx + mmy * fy + (fz1 - fz0))));
       bput(BP, wk, pc0);
     }
     // ---- w level k: L_w (euler.py:400, 426-427), L_eta (euler.py:429-437) ----
     if (k == 0) {
       stg(YW, wk, so(wv));
-      stg(YE, wk, so(e0 * ldg(c_rdzw, wk)));
+      stg(YE, wk, so(e0 * ldc<T>(c_rdzw, wk)));
     } else {
-      stg(YW, wk, so(ldg(c_dw, wk) * wv + ldg(c_cw, wk) * (pc0 - pcm)));
-      stg(YE, wk, so(alpha * (e0 + ldg(c_kx, wk) * (half * (umc_p + um_k)) +
-                              ldg(c_ky, wk) * (half * (vmc_p + vm_k)) - wv)));
+      stg(YW, wk, so(ldc<T>(c_dw, wk) * wv + ldc<T>(c_cw, wk) * (pc0 - pcm)));
+      stg(YE, wk, so(alpha * (e0 + ldc<T>(c_kx, wk) * (half * (umc_p + um_k)) +
+                              ldc<T>(c_ky, wk) * (half * (vmc_p + vm_k)) - wv)));
     }
     bput(BW, wk, wv);
     bput(BE, wk, e0);
@@ -257,14 +228,14 @@ __device__ __forceinline__ void helmholtz_body(const OpT<T> &o, const T *__restr
   }
 }
 
-template <typename T, int UNR, int NB>
+template <typename T, int UNR, int NB, typename TC>
 __global__ void __launch_bounds__(256) helmholtz_kernel(OpT<T> o, const T *__restrict__ x,
                                                         T *__restrict__ y, int kchunk,
                                                         const int *__restrict__ stop,
                                                         const double *__restrict__ inv, void *vout,
                                                         double vs) {
   if (stop && *stop) return;
-  helmholtz_body<T, UNR, NB>(o, x, y, kchunk, blockIdx.x, gridDim.x, inv, vout, vs);
+  helmholtz_body<T, UNR, NB, TC>(o, x, y, kchunk, blockIdx.x, gridDim.x, inv, vout, vs);
 }
 
 
@@ -282,7 +253,7 @@ __global__ void __launch_bounds__(256) helmholtz_kernel(OpT<T> o, const T *__res
 // from loads that neighbouring lanes also make (cache hits).  Outputs per
 // level: u face i, v face j (and the east / north faces at the domain edge),
 // pi, w and eta_dot of the column.
-template <typename T, int UNR, int NB = 0>
+template <typename T, int UNR, int NB = 0, typename TC = T>
 __device__ __forceinline__ void helmholtz_T_body(const OpT<T> &o, const T *__restrict__ y,
                                                  T *__restrict__ x, int kchunk, int blk, int nblk,
                                                  const double *inv = nullptr, void *vout = nullptr,
@@ -301,19 +272,19 @@ __device__ __forceinline__ void helmholtz_T_body(const OpT<T> &o, const T *__res
   const bool px = s.px, py = s.py;
   const bool hl = i >= 1, hd = j >= 1;  // the cell to the left / below exists
   const T half = (T)0.5, mhalf = (T)-0.5, alpha = o.alpha, ha = half * o.alpha;
-  const T *__restrict__ c_cua = o.cua;
-  const T *__restrict__ c_zxu = o.zxu;
-  const T *__restrict__ c_cx = o.cx;
-  const T *__restrict__ c_cva = o.cva;
-  const T *__restrict__ c_zev = o.zev;
-  const T *__restrict__ c_cy = o.cy;
-  const T *__restrict__ c_dw = o.dw;
-  const T *__restrict__ c_cw = o.cw;
-  const T *__restrict__ c_cz = o.cz;
-  const T *__restrict__ c_kx = o.kx;
-  const T *__restrict__ c_ky = o.ky;
-  const T *__restrict__ c_rdzw = o.rdzw;
-  const T *__restrict__ c_cpm = o.cpm;
+  const TC *__restrict__ c_cua = reinterpret_cast<const TC *>(o.cua);
+  const TC *__restrict__ c_zxu = reinterpret_cast<const TC *>(o.zxu);
+  const TC *__restrict__ c_cx = reinterpret_cast<const TC *>(o.cx);
+  const TC *__restrict__ c_cva = reinterpret_cast<const TC *>(o.cva);
+  const TC *__restrict__ c_zev = reinterpret_cast<const TC *>(o.zev);
+  const TC *__restrict__ c_cy = reinterpret_cast<const TC *>(o.cy);
+  const TC *__restrict__ c_dw = reinterpret_cast<const TC *>(o.dw);
+  const TC *__restrict__ c_cw = reinterpret_cast<const TC *>(o.cw);
+  const TC *__restrict__ c_cz = reinterpret_cast<const TC *>(o.cz);
+  const TC *__restrict__ c_kx = reinterpret_cast<const TC *>(o.kx);
+  const TC *__restrict__ c_ky = reinterpret_cast<const TC *>(o.ky);
+  const TC *__restrict__ c_rdzw = reinterpret_cast<const TC *>(o.rdzw);
+  const TC *__restrict__ c_cpm = reinterpret_cast<const TC *>(o.cpm);
   const uint32_t pm = (uint32_t)nx * ny, pu = (uint32_t)(nx + 1) * ny, pv = (uint32_t)nx * (ny + 1);
   const uint32_t mc = (uint32_t)j * nx + i;
   const uint32_t uc = (uint32_t)j * (nx + 1) + i, vc = (uint32_t)j * nx + i;
@@ -348,8 +319,8 @@ __device__ __forceinline__ void helmholtz_T_body(const OpT<T> &o, const T *__res
   auto gzmb = [&](T uti, T utr, T vtj, T vtr, int k) -> T {
     const uint32_t qu = (uint32_t)k * pu, qv = (uint32_t)k * pv;
     const uint32_t al = uc + qu, ar = u0r + fr + qu, bl = vc + qv, br = (uint32_t)gr * nx + i + qv;
-    T g = mhalf * (uti * ldg(c_cua, al) * ldg(c_zxu, al) + utr * ldg(c_cua, ar) * ldg(c_zxu, ar));
-    g += mhalf * (vtj * ldg(c_cva, bl) * ldg(c_zev, bl) + vtr * ldg(c_cva, br) * ldg(c_zev, br));
+    T g = mhalf * (uti * ldc<T>(c_cua, al) * ldc<T>(c_zxu, al) + utr * ldc<T>(c_cua, ar) * ldc<T>(c_zxu, ar));
+    g += mhalf * (vtj * ldc<T>(c_cva, bl) * ldc<T>(c_zev, bl) + vtr * ldc<T>(c_cva, br) * ldc<T>(c_zev, br));
     return g;
   };
   // carried state
@@ -370,20 +341,20 @@ __device__ __forceinline__ void helmholtz_T_body(const OpT<T> &o, const T *__res
     const uint32_t qu = (uint32_t)km * pu, qv = (uint32_t)km * pv;
     gm1 = gzmb(ut(i, u0r, qu), ut(i + 1, u0r, qu), vt(i, j, qv), vt(i, j + 1, qv), km);
     const uint32_t m = mc + (uint32_t)km * pm;
-    cpmP_m1 = ldg(c_cpm, m) * ldg(YP, m);
+    cpmP_m1 = ldc<T>(c_cpm, m) * ldg(YP, m);
   }
   if (k0 >= 1 && k0 <= nz - 1) {
     const uint32_t w = mc + (uint32_t)k0 * pm;
     e0 = ldg(YE, w);
     if (hl) e0l = ldg(YE, w - 1);
     if (hd) e0d = ldg(YE, w - nx);
-    cwW0 = ldg(c_cw, w) * ldg(YW, w);
+    cwW0 = ldc<T>(c_cw, w) * ldg(YW, w);
   }
   if (k0 <= nz) {
     const uint32_t w = mc + (uint32_t)k0 * pm;
-    kx0 = ldg(c_kx, w); ky0 = ldg(c_ky, w);
-    if (hl) kx0l = ldg(c_kx, w - 1);
-    if (hd) ky0d = ldg(c_ky, w - nx);
+    kx0 = ldc<T>(c_kx, w); ky0 = ldc<T>(c_ky, w);
+    if (hl) kx0l = ldc<T>(c_kx, w - 1);
+    if (hd) ky0d = ldc<T>(c_ky, w - nx);
   }
 #pragma unroll UNR
   for (int k = k0; k < k1; ++k) {
@@ -392,7 +363,7 @@ __device__ __forceinline__ void helmholtz_T_body(const OpT<T> &o, const T *__res
     if (k == nz) {  // top w level: identity rows
       const T yw = ldg(YW, wk), ye = ldg(YE, wk);
       stg(XW, wk, so(yw));
-      stg(XE, wk, so(ye * ldg(c_rdzw, wk)));
+      stg(XE, wk, so(ye * ldc<T>(c_rdzw, wk)));
       bput(BW, wk, yw);
       bput(BE, wk, ye);
       break;
@@ -409,10 +380,10 @@ __device__ __forceinline__ void helmholtz_T_body(const OpT<T> &o, const T *__res
       e1 = ldg(YE, w1);
       if (hl) e1l = ldg(YE, w1 - 1);
       if (hd) e1d = ldg(YE, w1 - nx);
-      cwW1 = ldg(c_cw, w1) * ldg(YW, w1);
+      cwW1 = ldc<T>(c_cw, w1) * ldg(YW, w1);
     }
-    const T kx1 = ldg(c_kx, w1), ky1 = ldg(c_ky, w1);
-    const T kx1l = hl ? ldg(c_kx, w1 - 1) : (T)0, ky1d = hd ? ldg(c_ky, w1 - nx) : (T)0;
+    const T kx1 = ldc<T>(c_kx, w1), ky1 = ldc<T>(c_ky, w1);
+    const T kx1l = hl ? ldc<T>(c_kx, w1 - 1) : (T)0, ky1d = hd ? ldc<T>(c_ky, w1 - nx) : (T)0;
     // ---- mass-point intermediates at level k ----
     const T umb = half * (vt_j * afv_j + vt_r * afv_r) + ha * (e0 * kx0 + e1 * kx1);
     const T vmb = mhalf * (ut_i * afu_i + ut_r * afu_r) + ha * (e0 * ky0 + e1 * ky1);
@@ -421,9 +392,9 @@ __device__ __forceinline__ void helmholtz_T_body(const OpT<T> &o, const T *__res
     if (hd) vmb_d = mhalf * (ut(i, u0r - (nx + 1), qu) * afu_dj + ut(i + 1, u0r - (nx + 1), qu) * afu_dr) +
                     ha * (e0d * ky0d + e1d * ky1d);
     const T yp = ldg(YP, wk);
-    const T cpmP = ldg(c_cpm, wk) * yp;
-    const T cpmPl = hl ? ldg(c_cpm, wk - 1) * ldg(YP, wk - 1) : (T)0;
-    const T cpmPd = hd ? ldg(c_cpm, wk - nx) * ldg(YP, wk - nx) : (T)0;
+    const T cpmP = ldc<T>(c_cpm, wk) * yp;
+    const T cpmPl = hl ? ldc<T>(c_cpm, wk - 1) * ldg(YP, wk - 1) : (T)0;
+    const T cpmPd = hd ? ldc<T>(c_cpm, wk - nx) * ldg(YP, wk - nx) : (T)0;
     // ---- u face i (and the east face) ----
     {
       const uint32_t au = uc + qu;
@@ -434,7 +405,7 @@ __device__ __forceinline__ void helmholtz_T_body(const OpT<T> &o, const T *__res
       T fl = 0;
       if (hl) fl += cpmPl * mmx_l;
       fl -= cpmP * mmx_c;
-      T out = idp + ldg(c_cx, au) * fl;
+      T out = idp + ldc<T>(c_cx, au) * fl;
       T ub = 0;
       if (hl) ub += umb_l;
       ub += umb;
@@ -443,7 +414,7 @@ __device__ __forceinline__ void helmholtz_T_body(const OpT<T> &o, const T *__res
         const T yn = ldg(YU, au + 1);
         bput(BU, au + 1, yn);
         const T idn = px ? (T)0 : yn;
-        stg(XU, au + 1, so(idn + ldg(c_cx, au + 1) * (cpmP * mmx_c) + half * umb));
+        stg(XU, au + 1, so(idn + ldc<T>(c_cx, au + 1) * (cpmP * mmx_c) + half * umb));
       }
     }
     // ---- v face j (and the north face) ----
@@ -456,7 +427,7 @@ __device__ __forceinline__ void helmholtz_T_body(const OpT<T> &o, const T *__res
       T fl = 0;
       if (hd) fl += cpmPd * mmy_d;
       fl -= cpmP * mmy_c;
-      T out = idp + ldg(c_cy, av) * fl;
+      T out = idp + ldc<T>(c_cy, av) * fl;
       T vb = 0;
       if (hd) vb += vmb_d;
       vb += vmb;
@@ -465,26 +436,26 @@ __device__ __forceinline__ void helmholtz_T_body(const OpT<T> &o, const T *__res
         const T yn = ldg(YV, av + nx);
         bput(BV, av + nx, yn);
         const T idn = py ? (T)0 : yn;
-        stg(XV, av + nx, so(idn + ldg(c_cy, av + nx) * (cpmP * mmy_c) + half * vmb));
+        stg(XV, av + nx, so(idn + ldc<T>(c_cy, av + nx) * (cpmP * mmy_c) + half * vmb));
       }
     }
     // ---- pi ----
     {
       const uint32_t al = uc + qu, ar = u0r + fr + qu, bl = vc + qv, br = (uint32_t)gr * nx + i + qv;
       T out = yp;
-      out += ut_i * ldg(c_cua, al) * mudx_i;
-      out -= ut_r * ldg(c_cua, ar) * mudx_r;
-      out += vt_j * ldg(c_cva, bl) * mvdy_j;
-      out -= vt_r * ldg(c_cva, br) * mvdy_r;
+      out += ut_i * ldc<T>(c_cua, al) * mudx_i;
+      out -= ut_r * ldc<T>(c_cua, ar) * mudx_r;
+      out += vt_j * ldc<T>(c_cva, bl) * mvdy_j;
+      out -= vt_r * ldc<T>(c_cva, br) * mvdy_r;
       if (k >= 1) out += cwW0;
       if (k + 1 <= nz - 1) out -= cwW1;
       // pressure-gradient levels: gz_w(kk) = (pi(kk) - pi(kk-1)) rdzw(kk) with the edge closures
-      if (k >= 1) out += half * (gm1 + g0) * ldg(c_rdzw, wk);
-      if (k == 1) out += half * gm1 * ldg(c_rdzw, mc);
-      if (k == nz - 1) out += half * g0 * ldg(c_rdzw, mc + (uint32_t)nz * pm);
-      if (k + 1 <= nz - 1) out -= half * (g0 + g1) * ldg(c_rdzw, w1);
-      if (k == 0) out -= half * g0 * ldg(c_rdzw, mc);
-      if (k == nz - 2) out -= half * g1 * ldg(c_rdzw, mc + (uint32_t)nz * pm);
+      if (k >= 1) out += half * (gm1 + g0) * ldc<T>(c_rdzw, wk);
+      if (k == 1) out += half * gm1 * ldc<T>(c_rdzw, mc);
+      if (k == nz - 1) out += half * g0 * ldc<T>(c_rdzw, mc + (uint32_t)nz * pm);
+      if (k + 1 <= nz - 1) out -= half * (g0 + g1) * ldc<T>(c_rdzw, w1);
+      if (k == 0) out -= half * g0 * ldc<T>(c_rdzw, mc);
+      if (k == nz - 2) out -= half * g1 * ldc<T>(c_rdzw, mc + (uint32_t)nz * pm);
       stg(XP, wk, so(out));
       bput(BP, wk, yp);
     }
@@ -496,11 +467,11 @@ __device__ __forceinline__ void helmholtz_T_body(const OpT<T> &o, const T *__res
         const T ye = ldg(YE, wk);
         bput(BE, wk, ye);
         stg(XW, wk, so(yw));
-        stg(XE, wk, so(ye * ldg(c_rdzw, wk)));
+        stg(XE, wk, so(ye * ldc<T>(c_rdzw, wk)));
       } else {
         bput(BE, wk, e0);
-        stg(XW, wk, so(ldg(c_dw, wk) * yw - alpha * e0));
-        stg(XE, wk, so(alpha * e0 + ldg(c_cz, wk) * (cpmP_m1 - cpmP)));
+        stg(XW, wk, so(ldc<T>(c_dw, wk) * yw - alpha * e0));
+        stg(XE, wk, so(alpha * e0 + ldc<T>(c_cz, wk) * (cpmP_m1 - cpmP)));
       }
     }
     // k+1 becomes k
@@ -513,14 +484,14 @@ __device__ __forceinline__ void helmholtz_T_body(const OpT<T> &o, const T *__res
   }
 }
 
-template <typename T, int UNR, int NB>
+template <typename T, int UNR, int NB, typename TC>
 __global__ void __launch_bounds__(256) helmholtz_T_kernel(OpT<T> o, const T *__restrict__ y,
                                                           T *__restrict__ x, int kchunk,
                                                           const int *__restrict__ stop,
                                                           const double *__restrict__ inv, void *vout,
                                                           double vs) {
   if (stop && *stop) return;
-  helmholtz_T_body<T, UNR, NB>(o, y, x, kchunk, blockIdx.x, gridDim.x, inv, vout, vs);
+  helmholtz_T_body<T, UNR, NB, TC>(o, y, x, kchunk, blockIdx.x, gridDim.x, inv, vout, vs);
 }
 
 // Launch shape of the operator and of its transpose: levels per thread
@@ -544,47 +515,57 @@ static AShape a_shape(const Stag &s, int bytes, int variant = 0) {
 
 // Launch of kernel K<T, UNR, NB> (operator or transpose) in a given shape;
 // NB != 0 (fused basis normalisation) exists for the FP32 operator only.
-template <typename T, int UNR, int NB>
+template <typename T, int UNR, int NB, typename TC>
 struct AKern {
   static void go(dim3 g, hipStream_t st, const OpT<T> &o, const T *x, T *y, int kc, const int *stop,
                  const Fuse &f) {
-    hipLaunchKernelGGL((helmholtz_kernel<T, UNR, NB>), g, dim3(kTX, kTY), 0, st, o, x, y, kc, stop, f.inv,
+    hipLaunchKernelGGL((helmholtz_kernel<T, UNR, NB, TC>), g, dim3(kTX, kTY), 0, st, o, x, y, kc, stop, f.inv,
                        f.vout, f.vs);
   }
 };
-template <typename T, int UNR, int NB>
+template <typename T, int UNR, int NB, typename TC>
 struct ATKern {
   static void go(dim3 g, hipStream_t st, const OpT<T> &o, const T *x, T *y, int kc, const int *stop,
                  const Fuse &f) {
-    hipLaunchKernelGGL((helmholtz_T_kernel<T, UNR, NB>), g, dim3(kTX, kTY), 0, st, o, x, y, kc, stop,
+    hipLaunchKernelGGL((helmholtz_T_kernel<T, UNR, NB, TC>), g, dim3(kTX, kTY), 0, st, o, x, y, kc, stop,
                        f.inv, f.vout, f.vs);
   }
 };
-template <template <typename, int, int> class K, typename T>
-static int launch_shape(const OpT<T> &o, const T *x, T *y, hipStream_t st, const int *stop, AShape sh,
-                        const Fuse &f) {
-  const int kc = sh.kc;
-  const dim3 g((unsigned)(((o.s.nx + kTX - 1) / kTX) * ((o.s.ny + kTY - 1) / kTY) *
-                          ((o.s.nz + 1 + kc - 1) / kc)));
+template <template <typename, int, int, typename> class K, typename T, typename TC>
+static int launch_shape_tc(dim3 g, const OpT<T> &o, const T *x, T *y, hipStream_t st, const int *stop, int kc,
+                           int unr, const Fuse &f) {
   if (f.nb == 0) {
-    if (sh.unr == 2) K<T, 2, 0>::go(g, st, o, x, y, kc, stop, f);
-    else K<T, 1, 0>::go(g, st, o, x, y, kc, stop, f);
+    if (unr == 2) K<T, 2, 0, TC>::go(g, st, o, x, y, kc, stop, f);
+    else K<T, 1, 0, TC>::go(g, st, o, x, y, kc, stop, f);
     return SISL_OK;
   }
   if constexpr (sizeof(T) == 4) {
     if (f.nb == 1) {
-      if (sh.unr == 2) K<T, 2, 1>::go(g, st, o, x, y, kc, stop, f);
-      else K<T, 1, 1>::go(g, st, o, x, y, kc, stop, f);
+      if (unr == 2) K<T, 2, 1, TC>::go(g, st, o, x, y, kc, stop, f);
+      else K<T, 1, 1, TC>::go(g, st, o, x, y, kc, stop, f);
       return SISL_OK;
     }
     if (f.nb == 2) {
-      if (sh.unr == 2) K<T, 2, 2>::go(g, st, o, x, y, kc, stop, f);
-      else K<T, 1, 2>::go(g, st, o, x, y, kc, stop, f);
+      if (unr == 2) K<T, 2, 2, TC>::go(g, st, o, x, y, kc, stop, f);
+      else K<T, 1, 2, TC>::go(g, st, o, x, y, kc, stop, f);
       return SISL_OK;
     }
   }
   return SISL_ERR_ARG;
 }
+template <template <typename, int, int, typename> class K, typename T>
+static int launch_shape(const OpT<T> &o, const T *x, T *y, hipStream_t st, const int *stop, AShape sh,
+                        const Fuse &f) {
+  const int kc = sh.kc;
+  const dim3 g((unsigned)(((o.s.nx + kTX - 1) / kTX) * ((o.s.ny + kTY - 1) / kTY) *
+                          ((o.s.nz + 1 + kc - 1) / kc)));
+  if (f.cb) {
+    if constexpr (sizeof(T) == 4) return launch_shape_tc<K, T, bf16_t>(g, o, x, y, st, stop, kc, sh.unr, f);
+    return SISL_ERR_ARG;
+  }
+  return launch_shape_tc<K, T, T>(g, o, x, y, st, stop, kc, sh.unr, f);
+}
+
 template <typename T>
 static int launch_A_shape(const OpT<T> &o, const T *x, T *y, hipStream_t st, const int *stop, AShape sh,
                           const Fuse &f = Fuse{}) {
@@ -597,7 +578,8 @@ int launch_A(const OpT<T> &o, const T *x, T *y, hipStream_t st, const int *stop,
   const Fuse f = fuse ? *fuse : Fuse{};
   const AShape sh = a_shape(o.s, (int)sizeof(T), f.nb ? 2 : 0);
   const Stag &s = o.s;
-  const double bytes = (double)sizeof(T) * (2.0 * s.n + 3.0 * s.u.n() + 3.0 * s.v.n() + 6.0 * s.w.n() + s.m.n()) +
+  const double sc = f.cb ? 2.0 : (double)sizeof(T);  // bytes per coefficient element
+  const double bytes = (double)sizeof(T) * 2.0 * s.n + sc * (3.0 * s.u.n() + 3.0 * s.v.n() + 6.0 * s.w.n() + s.m.n()) +
                        (f.nb == 1 ? 2.0 : f.nb == 2 ? (double)sizeof(T) : 0.0) * s.n;
   const int tid = timer_begin(SISL_T_HELMHOLTZ, sizeof(T) == 4, bytes, stop, nullptr, st);
   const int rc = launch_A_shape(o, x, y, st, stop, sh, f);
@@ -621,7 +603,8 @@ int launch_AT(const OpT<T> &o, const T *y, T *x, T * /*scratch: unused*/, hipStr
   const AShape sh = a_shape(o.s, (int)sizeof(T), f.nb ? 3 : 1);
   const Stag &s = o.s;
   // same arrays as the forward operator: y in, x out, 13 coefficient arrays (+ the basis column)
-  const double bytes = (double)sizeof(T) * (2.0 * s.n + 3.0 * s.u.n() + 3.0 * s.v.n() + 6.0 * s.w.n() + s.m.n()) +
+  const double sc = f.cb ? 2.0 : (double)sizeof(T);  // bytes per coefficient element
+  const double bytes = (double)sizeof(T) * 2.0 * s.n + sc * (3.0 * s.u.n() + 3.0 * s.v.n() + 6.0 * s.w.n() + s.m.n()) +
                        (f.nb == 1 ? 2.0 : f.nb == 2 ? (double)sizeof(T) : 0.0) * s.n;
   const int tid = timer_begin(SISL_T_HELMHOLTZ_T, sizeof(T) == 4, bytes, stop, nullptr, st);
   const int rc = launch_AT_shape(o, y, x, st, stop, sh, f);
@@ -714,7 +697,7 @@ __global__ void __launch_bounds__(256) helmholtz_batched_kernel(const BOp<T> *__
   if (b.stop && b.stop[j * b.sj]) return;
   const int nb = blocks_A_dev(b.o.s);
   if ((int)blockIdx.x >= nb) return;
-  helmholtz_body<T, 1, 0>(b.o, b.x + j * b.xj, b.y, (b.o.s.nz + 1 + 7) / 8, blockIdx.x, nb);
+  helmholtz_body<T, 1, 0, T>(b.o, b.x + j * b.xj, b.y, (b.o.s.nz + 1 + 7) / 8, blockIdx.x, nb);
 }
 
 template <typename T>
@@ -723,7 +706,7 @@ __global__ void __launch_bounds__(256) helmholtz_T_batched_kernel(const BOp<T> *
   if (b.stop && b.stop[j * b.sj]) return;
   const int nb = blocks_A_dev(b.o.s);
   if ((int)blockIdx.x >= nb) return;
-  helmholtz_T_body<T, 1, 0>(b.o, b.x + j * b.xj, b.y, (b.o.s.nz + 1 + 7) / 8, blockIdx.x, nb);
+  helmholtz_T_body<T, 1, 0, T>(b.o, b.x + j * b.xj, b.y, (b.o.s.nz + 1 + 7) / 8, blockIdx.x, nb);
 }
 
 template <typename T>

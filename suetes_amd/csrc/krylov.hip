@@ -752,6 +752,7 @@ struct Krylov {
   int64_t n;
   hipStream_t s;
   TV *cur = nullptr;
+  int cb = 0;  // 1: the operator's 3-D coefficient arrays are bfloat16
   int64_t ldv() const { return basis_ld(n); }
   double vscale() const { return basis_scale<TB>(n); }
   const TV *basis_in(int j) const {
@@ -763,12 +764,14 @@ struct Krylov {
   // the operator launch also normalises `in` into a basis column
   int apply(const TV *in, TV *out, const int *stop, const Fuse *fuse = nullptr) {
     int rc;
+    Fuse f = fuse ? *fuse : Fuse{};
+    f.cb = cb;
     if (!transpose) {
-      if ((rc = launch_A<TV>(op, in, out, s, stop, fuse))) return rc;
-      return launch_M<TV>(op, out, out, s, stop);
+      if ((rc = launch_A<TV>(op, in, out, s, stop, &f))) return rc;
+      return launch_M<TV>(op, out, out, s, stop, cb);
     }
-    if ((rc = launch_AT<TV>(op, in, t, scr, s, stop, fuse))) return rc;
-    return launch_MT<TV>(op, t, out, scr, s, stop);
+    if ((rc = launch_AT<TV>(op, in, t, scr, s, stop, &f))) return rc;
+    return launch_MT<TV>(op, t, out, scr, s, stop, cb);
   }
 
   // One Arnoldi step (JAX _kth_arnoldi_iteration) with JAX's iterated CGS.
@@ -928,6 +931,7 @@ static int gmres_mixed(const OpT<double> &op64, const OpT<float> &op32, const Si
                       (float *)(ws + L.t32), (float *)(ws + L.scr32), partials, sm,
                       (int *)(ws + L.flags), n, s};
   K.cur = (float *)(ws + L.cur32);  // second Arnoldi buffer (and the FP16 path's unit residual)
+  K.cb = p.coef_bf16 ? 1 : 0;
   int rc;
   double hs[4];
   auto M64 = [&](double *in, double *out) -> int {

@@ -23,11 +23,21 @@ __host__ __device__ inline int blocks_M_dev(const Stag &s) { return ((s.nx + 63)
 
 // Body for launch block `blk` of `nblk`; blockDim.y may exceed the segments
 // the column needs (the extra waves carry identity compositions).
-template <typename T, int L>
+template <typename T, int L, typename TC = T>
 __device__ __forceinline__ void precond_seg_body(const OpT<T> &o, const T *__restrict__ r,
                                                  T *__restrict__ z, int blk, int nblk) {
   __shared__ T la[kMaxSeg][64], lb[kMaxSeg][64];
   const Stag &s = o.s;
+  // 3-D coefficient arrays (T or bfloat16 storage)
+  const TC *__restrict__ k_kx = reinterpret_cast<const TC *>(o.kx);
+  const TC *__restrict__ k_ky = reinterpret_cast<const TC *>(o.ky);
+  const TC *__restrict__ k_idw = reinterpret_cast<const TC *>(o.idw);
+  const TC *__restrict__ k_cz = reinterpret_cast<const TC *>(o.cz);
+  const TC *__restrict__ k_lpi = reinterpret_cast<const TC *>(o.lpi);
+  const TC *__restrict__ k_lower = reinterpret_cast<const TC *>(o.lower);
+  const TC *__restrict__ k_invden = reinterpret_cast<const TC *>(o.invden);
+  const TC *__restrict__ k_cprime = reinterpret_cast<const TC *>(o.cprime);
+  const TC *__restrict__ k_gw = reinterpret_cast<const TC *>(o.gw);
   const int nx = s.nx, ny = s.ny, nz = s.nz;
   const int nbx = (nx + 63) / 64;
   const int tile = xcd_tile(blk, nblk);
@@ -50,10 +60,10 @@ __device__ __forceinline__ void precond_seg_body(const OpT<T> &o, const T *__res
     T um_k = um(ka), vm_k = vm(ka), wt_lo = 0;
     if (ka >= 1) {
       const uint32_t w = mc + (uint32_t)ka * pm;
-      const T kin = ldg(o.kx, w) * (half * (um(ka - 1) + um_k)) + ldg(o.ky, w) * (half * (vm(ka - 1) + vm_k));
+      const T kin = ldc<T>(k_kx, w) * (half * (um(ka - 1) + um_k)) + ldc<T>(k_ky, w) * (half * (vm(ka - 1) + vm_k));
       zep[0] = ldg(RE, w) * ra - kin;
-      rwi[0] = ldg(RW, w) * ldg(o.idw, w);
-      wt_lo = ldg(o.cz, w) * (rwi[0] + zep[0]);
+      rwi[0] = ldg(RW, w) * ldc<T>(k_idw, w);
+      wt_lo = ldc<T>(k_cz, w) * (rwi[0] + zep[0]);
     }
 #pragma unroll
     for (int t = 0; t < L; ++t) {
@@ -63,16 +73,16 @@ __device__ __forceinline__ void precond_seg_body(const OpT<T> &o, const T *__res
         if (k + 1 < nz) {
           const uint32_t w = mc + (uint32_t)(k + 1) * pm;
           const T um1 = um(k + 1), vm1 = vm(k + 1);
-          const T kin = ldg(o.kx, w) * (half * (um_k + um1)) + ldg(o.ky, w) * (half * (vm_k + vm1));
-          const T zp = ldg(RE, w) * ra - kin, rw = ldg(RW, w) * ldg(o.idw, w);
-          wt_hi = ldg(o.cz, w) * (rw + zp);
+          const T kin = ldc<T>(k_kx, w) * (half * (um_k + um1)) + ldc<T>(k_ky, w) * (half * (vm_k + vm1));
+          const T zp = ldg(RE, w) * ra - kin, rw = ldg(RW, w) * ldc<T>(k_idw, w);
+          wt_hi = ldc<T>(k_cz, w) * (rw + zp);
           if (t + 1 < L) { zep[t + 1] = zp; rwi[t + 1] = rw; }
           um_k = um1;
           vm_k = vm1;
         }
         const uint32_t m = mc + (uint32_t)k * pm;
-        const T rh = ldg(RP, m) * (T)1e-3 - ldg(o.lpi, m) * (wt_hi - wt_lo);
-        const T lo = ldg(o.lower, m), iv = ldg(o.invden, m);
+        const T rh = ldg(RP, m) * (T)1e-3 - ldc<T>(k_lpi, m) * (wt_hi - wt_lo);
+        const T lo = ldc<T>(k_lower, m), iv = ldc<T>(k_invden, m);
         dl = (rh - lo * dl) * iv;
         bl = bl * (-lo * iv);
         d[t] = dl;
@@ -95,7 +105,7 @@ __device__ __forceinline__ void precond_seg_body(const OpT<T> &o, const T *__res
       const int k = ka + t;
       if (k < kb) {
         const T dk = d[t] + B[t] * din;
-        const T cp = ldg(o.cprime, mc + (uint32_t)k * pm);
+        const T cp = ldc<T>(k_cprime, mc + (uint32_t)k * pm);
         xl = dk - cp * xl;
         gl = -cp * gl;
         d[t] = xl;
@@ -131,7 +141,7 @@ __device__ __forceinline__ void precond_seg_body(const OpT<T> &o, const T *__res
       stg(ZP, m, d[t] * (T)1000);
       if (k >= 1) {
         const T xm1 = t == 0 ? xbelow : d[t > 0 ? t - 1 : 0];
-        const T pw = rwi[t] - ldg(o.gw, m) * (d[t] - xm1);
+        const T pw = rwi[t] - ldc<T>(k_gw, m) * (d[t] - xm1);
         stg(ZW, m, pw);
         stg(ZE, m, zep[t] + pw);
       }
@@ -159,12 +169,12 @@ __device__ __forceinline__ void precond_seg_body(const OpT<T> &o, const T *__res
   }
 }
 
-template <typename T, int L>
+template <typename T, int L, typename TC>
 __global__ void __launch_bounds__(512) precond_seg_kernel(OpT<T> o, const T *__restrict__ r,
                                                            T *__restrict__ z,
                                                            const int *__restrict__ stop) {
   if (stop && *stop) return;  // uniform over the launch
-  precond_seg_body<T, L>(o, r, z, blockIdx.x, gridDim.x);
+  precond_seg_body<T, L, TC>(o, r, z, blockIdx.x, gridDim.x);
 }
 
 // Transpose, segment-parallel like the forward solve.  M = B_out * T^{-1} *
@@ -178,12 +188,22 @@ __global__ void __launch_bounds__(512) precond_seg_kernel(OpT<T> o, const T *__r
 // kernel gathers to the u, v faces.  Wave `sg` owns levels [sg*L, sg*L + L)
 // of 64 columns of row j; both recurrences compose across segments through
 // LDS as in the forward solve.
-template <typename T, int L>
+template <typename T, int L, typename TC = T>
 __device__ __forceinline__ void precond_T_seg_body(const OpT<T> &o, const T *__restrict__ zb,
                                                    T *__restrict__ rb, T *__restrict__ umb,
                                                    T *__restrict__ vmb, int blk, int nblk) {
   __shared__ T la[kMaxSeg][64], lb[kMaxSeg][64];
   const Stag &s = o.s;
+  // 3-D coefficient arrays (T or bfloat16 storage)
+  const TC *__restrict__ k_kx = reinterpret_cast<const TC *>(o.kx);
+  const TC *__restrict__ k_ky = reinterpret_cast<const TC *>(o.ky);
+  const TC *__restrict__ k_idw = reinterpret_cast<const TC *>(o.idw);
+  const TC *__restrict__ k_cz = reinterpret_cast<const TC *>(o.cz);
+  const TC *__restrict__ k_lpi = reinterpret_cast<const TC *>(o.lpi);
+  const TC *__restrict__ k_lower = reinterpret_cast<const TC *>(o.lower);
+  const TC *__restrict__ k_invden = reinterpret_cast<const TC *>(o.invden);
+  const TC *__restrict__ k_cprime = reinterpret_cast<const TC *>(o.cprime);
+  const TC *__restrict__ k_gw = reinterpret_cast<const TC *>(o.gw);
   const int nx = s.nx, ny = s.ny, nz = s.nz;
   const int nbx = (nx + 63) / 64;
   const int tile = xcd_tile(blk, nblk);
@@ -209,7 +229,7 @@ __device__ __forceinline__ void precond_T_seg_body(const OpT<T> &o, const T *__r
       const T e = ldg(ZE, w);
       zs[0] = ldg(ZW, w) + e;
       ze[0] = e;
-      g_lo = ldg(o.gw, w) * zs[0];
+      g_lo = ldc<T>(k_gw, w) * zs[0];
     }
 #pragma unroll
     for (int t = 0; t < L; ++t) {
@@ -219,12 +239,12 @@ __device__ __forceinline__ void precond_T_seg_body(const OpT<T> &o, const T *__r
         if (k + 1 < nz) {
           const uint32_t w = mc + (uint32_t)(k + 1) * pm;
           const T e = ldg(ZE, w), sum = ldg(ZW, w) + e;
-          g_hi = ldg(o.gw, w) * sum;
+          g_hi = ldc<T>(k_gw, w) * sum;
           if (t + 1 < L) { zs[t + 1] = sum; ze[t + 1] = e; }
         }
         const uint32_t m = mc + (uint32_t)k * pm;
         const T pp = (T)1000 * ldg(ZP, m) - g_lo + g_hi;
-        const T a = k >= 1 ? -ldg(o.cprime, m - pm) : (T)0;
+        const T a = k >= 1 ? -ldc<T>(k_cprime, m - pm) : (T)0;
         dl = pp + a * dl;
         bl = a * bl;
         d[t] = dl;
@@ -248,8 +268,8 @@ __device__ __forceinline__ void precond_T_seg_body(const OpT<T> &o, const T *__r
       if (k < kb) {
         const uint32_t m = mc + (uint32_t)k * pm;
         const T y = d[t] + B[t] * din;
-        const T iv = ldg(o.invden, m);
-        const T c = k + 1 < nz ? -iv * ldg(o.lower, m + pm) : (T)0;
+        const T iv = ldc<T>(k_invden, m);
+        const T c = k + 1 < nz ? -iv * ldc<T>(k_lower, m + pm) : (T)0;
         xl = iv * y + c * xl;
         gl = c * gl;
         d[t] = xl;
@@ -281,7 +301,7 @@ __device__ __forceinline__ void precond_T_seg_body(const OpT<T> &o, const T *__r
   const uint32_t q2 = (uint32_t)s.m2(i, j);
   // ---- outputs: pi; then per w level k in [ka, kb] the right-hand-side transpose
   //      (w, eta_dot owned for k < kb; the top face only feeds umb, vmb) ----
-  T divb_lo = ka >= 1 ? -ldg(o.lpi, mc + (uint32_t)(ka - 1) * pm) * xbelow : (T)0;
+  T divb_lo = ka >= 1 ? -ldc<T>(k_lpi, mc + (uint32_t)(ka - 1) * pm) * xbelow : (T)0;
   T uw_lo = 0, vw_lo = 0;
 #pragma unroll
   for (int t = 0; t <= L; ++t) {
@@ -290,7 +310,7 @@ __device__ __forceinline__ void precond_T_seg_body(const OpT<T> &o, const T *__r
       const uint32_t m = mc + (uint32_t)k * pm;  // mass index of level k == w index of level k
       const T xk = k < kb ? d[t < L ? t : L - 1] : xabove;
       if (k < kb) stg(RP, m, xk * (T)1e-3);
-      const T divb_hi = k < nz ? -ldg(o.lpi, m) * xk : (T)0;
+      const T divb_hi = k < nz ? -ldc<T>(k_lpi, m) * xk : (T)0;
       T uw = 0, vw = 0;
       if (k == 0) {
         stg(RW, m, ldg(ZW, m));
@@ -307,14 +327,14 @@ __device__ __forceinline__ void precond_T_seg_body(const OpT<T> &o, const T *__r
           e = ldg(ZE, m);
           zsum = ldg(ZW, m) + e;
         }
-        const T wckb = ldg(o.cz, m) * (divb_lo - divb_hi);
+        const T wckb = ldc<T>(k_cz, m) * (divb_lo - divb_hi);
         if (k < kb) {
-          stg(RW, m, ldg(o.idw, m) * (zsum + wckb));
+          stg(RW, m, ldc<T>(k_idw, m) * (zsum + wckb));
           stg(RE, m, (e + wckb) * ra);
         }
         const T kinb = -e - wckb;
-        uw = ldg(o.kx, m) * kinb;
-        vw = ldg(o.ky, m) * kinb;
+        uw = ldc<T>(k_kx, m) * kinb;
+        vw = ldc<T>(k_ky, m) * kinb;
       }
       if (k > ka) {
         stg(umb, m - pm, half * uw_lo + half * uw);
@@ -327,14 +347,14 @@ __device__ __forceinline__ void precond_T_seg_body(const OpT<T> &o, const T *__r
   }
 }
 
-template <typename T, int L>
+template <typename T, int L, typename TC>
 __global__ void __launch_bounds__(512) precond_T_seg_kernel(OpT<T> o, const T *__restrict__ zb,
                                                              T *__restrict__ rb,
                                                              T *__restrict__ umb,
                                                              T *__restrict__ vmb,
                                                              const int *__restrict__ stop) {
   if (stop && *stop) return;
-  precond_T_seg_body<T, L>(o, zb, rb, umb, vmb, blockIdx.x, gridDim.x);
+  precond_T_seg_body<T, L, TC>(o, zb, rb, umb, vmb, blockIdx.x, gridDim.x);
 }
 
 // u, v faces of the transpose: zbar passes through and each face gathers half
@@ -391,7 +411,7 @@ __global__ void __launch_bounds__(256) precond_T_faces_kernel(OpT<T> o, const T 
 
 // Levels per segment: the smallest supported L with at most kMaxSeg
 // segments per column.
-template <typename T>
+template <typename T, typename TC>
 static int launch_M_seg(const OpT<T> &o, const T *r, T *z, hipStream_t st, const int *stop) {
   const int nz = o.s.nz;
   static const int sizes[] = {2, 4, 8, 10, 12, 16, 24, 32};
@@ -402,12 +422,16 @@ static int launch_M_seg(const OpT<T> &o, const T *r, T *z, hipStream_t st, const
   const int S = (nz + L - 1) / L;
   const dim3 grid((unsigned)(((o.s.nx + 63) / 64) * o.s.ny)), block(64, S);
   const Stag &g = o.s;
-  const double bytes = (double)sizeof(T) * (g.n + 7.0 * g.w.n() + 5.0 * g.m.n() + 2.0 * g.nx * g.ny +
-                                            (r != z ? (double)(g.u.n() + g.v.n()) : 0.0));
+  // packed vector in and out, 5 w-level and 4 mass-level coefficient arrays (T or bfloat16), w and
+  // eta_dot ends, 2-D end factors
+  const double sc = (double)sizeof(TC);
+  const double bytes = (double)sizeof(T) * (g.n + 2.0 * g.w.n() + g.m.n() + 2.0 * g.nx * g.ny +
+                                            (r != z ? (double)(g.u.n() + g.v.n()) : 0.0)) +
+                       sc * (5.0 * g.w.n() + 4.0 * g.m.n());
   const int tid = timer_begin(SISL_T_PRECOND, sizeof(T) == 4, bytes, stop, nullptr, st);
   switch (L) {
 #define SISL_M_CASE(c) \
-  case c: hipLaunchKernelGGL((precond_seg_kernel<T, c>), grid, block, 0, st, o, r, z, stop); break;
+  case c: hipLaunchKernelGGL((precond_seg_kernel<T, c, TC>), grid, block, 0, st, o, r, z, stop); break;
     SISL_M_CASE(2) SISL_M_CASE(4) SISL_M_CASE(8) SISL_M_CASE(10) SISL_M_CASE(12) SISL_M_CASE(16)
     SISL_M_CASE(24) SISL_M_CASE(32)
 #undef SISL_M_CASE
@@ -418,14 +442,17 @@ static int launch_M_seg(const OpT<T> &o, const T *r, T *z, hipStream_t st, const
 }
 
 template <typename T>
-int launch_M(const OpT<T> &o, const T *r, T *z, hipStream_t st, const int *stop) {
+int launch_M(const OpT<T> &o, const T *r, T *z, hipStream_t st, const int *stop, int cb) {
   if ((uint64_t)o.s.n * sizeof(T) >= (1ull << 32)) return SISL_ERR_ARG;  // 32-bit offsets
-  return launch_M_seg(o, r, z, st, stop);
+  if (cb) {
+    if constexpr (sizeof(T) == 4) return launch_M_seg<T, bf16_t>(o, r, z, st, stop);
+    return SISL_ERR_ARG;
+  }
+  return launch_M_seg<T, T>(o, r, z, st, stop);
 }
 
-template <typename T>
-int launch_MT(const OpT<T> &o, const T *zb, T *rb, T *scratch, hipStream_t st, const int *stop) {
-  if ((uint64_t)o.s.n * sizeof(T) >= (1ull << 32)) return SISL_ERR_ARG;  // 32-bit offsets
+template <typename T, typename TC>
+static int launch_MT_tc(const OpT<T> &o, const T *zb, T *rb, T *scratch, hipStream_t st, const int *stop) {
   const int L = seg_levels(o.s.nz);
   if (!L) return SISL_ERR_ARG;
   const int S = (o.s.nz + L - 1) / L;
@@ -437,12 +464,13 @@ int launch_MT(const OpT<T> &o, const T *zb, T *rb, T *scratch, hipStream_t st, c
   // (mass), dzw0, dzwn (2-D) in; w, eta_dot, pi outputs and umb, vmb out.
   // faces: zbar_u, zbar_v, umb, vmb in; u, v outputs out.
   const double nw = (double)g.w.n(), nmf = (double)nm, nuv = (double)(g.u.n() + g.v.n());
-  const double bytes = (double)sizeof(T) * ((2 * nw + nmf) + (5 * nw + 4 * nmf + 2.0 * g.nx * g.ny) +
-                                            (2 * nw + 3 * nmf) + (nuv + 2 * nmf) + nuv);
+  const double bytes = (double)sizeof(T) * ((2 * nw + nmf) + 2.0 * g.nx * g.ny + (2 * nw + 3 * nmf) +
+                                            (nuv + 2 * nmf) + nuv) +
+                       (double)sizeof(TC) * (5 * nw + 4 * nmf);
   const int tid = timer_begin(SISL_T_PRECOND_T, sizeof(T) == 4, bytes, stop, nullptr, st);
   switch (L) {
 #define SISL_MT_CASE(c) \
-  case c: hipLaunchKernelGGL((precond_T_seg_kernel<T, c>), grid, block, 0, st, o, zb, rb, umb, vmb, stop); break;
+  case c: hipLaunchKernelGGL((precond_T_seg_kernel<T, c, TC>), grid, block, 0, st, o, zb, rb, umb, vmb, stop); break;
     SISL_MT_CASE(2) SISL_MT_CASE(4) SISL_MT_CASE(8) SISL_MT_CASE(10) SISL_MT_CASE(12) SISL_MT_CASE(16)
     SISL_MT_CASE(24) SISL_MT_CASE(32)
 #undef SISL_MT_CASE
@@ -453,6 +481,16 @@ int launch_MT(const OpT<T> &o, const T *zb, T *rb, T *scratch, hipStream_t st, c
   timer_end(tid, st);
   SISL_CHECK_LAUNCH();
   return SISL_OK;
+}
+
+template <typename T>
+int launch_MT(const OpT<T> &o, const T *zb, T *rb, T *scratch, hipStream_t st, const int *stop, int cb) {
+  if ((uint64_t)o.s.n * sizeof(T) >= (1ull << 32)) return SISL_ERR_ARG;  // 32-bit offsets
+  if (cb) {
+    if constexpr (sizeof(T) == 4) return launch_MT_tc<T, bf16_t>(o, zb, rb, scratch, st, stop);
+    return SISL_ERR_ARG;
+  }
+  return launch_MT_tc<T, T>(o, zb, rb, scratch, st, stop);
 }
 
 // ---- batched (ensemble) launches -------------------------------------------
@@ -471,7 +509,7 @@ __global__ void __launch_bounds__(512) precond_seg_batched_kernel(const BOp<T> *
   if (b.stop && b.stop[j * b.sj]) return;
   const int nb = ((b.o.s.nx + 63) / 64) * b.o.s.ny;
   if ((int)blockIdx.x >= nb) return;
-  precond_seg_body<T, L>(b.o, b.x + j * b.xj, b.y, blockIdx.x, nb);
+  precond_seg_body<T, L, T>(b.o, b.x + j * b.xj, b.y, blockIdx.x, nb);
 }
 
 template <typename T, int L>
@@ -481,7 +519,7 @@ __global__ void __launch_bounds__(512) precond_T_seg_batched_kernel(const BOp<T>
   const int nb = blocks_M_dev(b.o.s);
   if ((int)blockIdx.x >= nb) return;
   const int64_t nm = b.o.s.m.n();
-  precond_T_seg_body<T, L>(b.o, b.x + j * b.xj, b.y, b.scr, b.scr + nm, blockIdx.x, nb);
+  precond_T_seg_body<T, L, T>(b.o, b.x + j * b.xj, b.y, b.scr, b.scr + nm, blockIdx.x, nb);
 }
 
 template <typename T>
@@ -528,12 +566,12 @@ template int launch_M_batched<float>(const BOp<float> *, int, int, int, int, int
 template int launch_MT_batched<double>(const BOp<double> *, int, int, int, int, int, int, hipStream_t);
 template int launch_MT_batched<float>(const BOp<float> *, int, int, int, int, int, int, hipStream_t);
 
-template int launch_M<double>(const OpT<double> &, const double *, double *, hipStream_t, const int *);
-template int launch_M<float>(const OpT<float> &, const float *, float *, hipStream_t, const int *);
+template int launch_M<double>(const OpT<double> &, const double *, double *, hipStream_t, const int *, int);
+template int launch_M<float>(const OpT<float> &, const float *, float *, hipStream_t, const int *, int);
 template int launch_MT<double>(const OpT<double> &, const double *, double *, double *, hipStream_t,
-                               const int *);
+                               const int *, int);
 template int launch_MT<float>(const OpT<float> &, const float *, float *, float *, hipStream_t,
-                              const int *);
+                              const int *, int);
 
 }  // namespace sisl
 

@@ -248,20 +248,23 @@ def test_gmres_solution_matches_oracle():
 
 
 @pytest.mark.parametrize("basis", ["fp16", "fp32"])
-def test_mixed_solver_reaches_tolerance(basis):
+@pytest.mark.parametrize("coefs", ["bf16", "fp32"])
+def test_mixed_solver_reaches_tolerance(basis, coefs):
     cfg = small_lam()
     case, go, A_o, M_o, sol = _solver_pair(cfg)
     rng = np.random.default_rng(22)
     b = pack_ref(A_o(_rand_krylov(go, rng)))
     from suetes_amd.steppers import SemiImplicitSolver3D
     ms = SemiImplicitSolver3D(sol.physics, sol.dt, sol.alpha, solver_tol=1e-10, solver_maxiter=30,
-                              solver_restart=20, precision="mixed", inner_basis=basis)
+                              solver_restart=20, precision="mixed", inner_basis=basis,
+                              inner_coefficients=coefs)
     ms.prepare(case.stepper.bg_precomputed())
-    x = ms.solve_flat(b, torch.zeros_like(b))
-    st = ms.last_stats[-1]
-    assert st["residual"] <= st["ptol"]
-    x64 = sol.solve_flat(b, torch.zeros_like(b))
-    assert float((x - x64).norm() / x64.norm()) < 1e-8
+    for transpose in (False, True):
+        x = ms.solve_flat(b, torch.zeros_like(b), transpose=transpose)
+        st = ms.last_stats[-1]
+        assert st["residual"] <= st["ptol"], (transpose, st)
+        x64 = sol.solve_flat(b, torch.zeros_like(b), transpose=transpose)
+        assert float((x - x64).norm() / x64.norm()) < 1e-8, transpose
 
 
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
