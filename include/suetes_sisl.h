@@ -267,6 +267,14 @@ int sisl_ffsl_1d_T_f32(const float *mass, const float *cfl, const float *gout, f
                        float *scratch, int64_t outer, int32_t n, int64_t inner, int32_t periodic,
                        void *stream);
 
+/* ---- launch graphs ------------------------------------------------------ */
+
+/* on = 1 (default): the mixed solver captures each inner Arnoldi cycle into
+ * a hipGraph once (per workspace, operator, restart length, direction and
+ * coefficient storage) and replays it for every later cycle; on = 0: eager
+ * launches (bit-identical results).  No counterpart in the reference. */
+int sisl_set_graphs(int32_t on);
+
 /* ---- kernel timing (measurement runs) ------------------------------------ */
 
 /* Timed kernels: operator apply, column preconditioner, the Krylov

@@ -29,6 +29,7 @@ EXPORTED = [
     "sisl_stagger_T_f64", "sisl_stagger_T_f32", "sisl_tricubic_adj_weights_f64",
     "sisl_tricubic_adj_weights_f32", "sisl_tricubic_adj_rows_f64", "sisl_tricubic_adj_rows_f32",
     "sisl_ffsl_1d_f64", "sisl_ffsl_1d_f32", "sisl_ffsl_1d_T_f64", "sisl_ffsl_1d_T_f32",
+    "sisl_set_graphs",
 ]
 
 # timed kernels (enum SislTimedKernel)
@@ -119,6 +120,7 @@ def lib():
         for p in ("f64", "f32"):
             getattr(L, f"sisl_ffsl_1d_{p}").argtypes = [vp, vp, vp, vp, i64, i32, i64, i32, vp]
             getattr(L, f"sisl_ffsl_1d_T_{p}").argtypes = [vp, vp, vp, vp, vp, vp, i64, i32, i64, i32, vp]
+        L.sisl_set_graphs.argtypes = [i32]
         L.sisl_timer_enable.argtypes = [i32]
         L.sisl_timer_read.argtypes = [i32, i32, C.POINTER(d), C.POINTER(i64), C.POINTER(i64),
                                       C.POINTER(d)]

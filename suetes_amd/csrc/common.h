@@ -161,6 +161,7 @@ int blocks_MT_faces(const Stag &s);
 int seg_levels(int nz);  // levels per segment L of the preconditioner (0: too tall)
 
 // Optional launch timing (timing.hip): returns a record id or -1 when off.
+bool timer_on();
 int timer_begin(int kind, bool fp32, double bytes, const int *stop, const int *gate, hipStream_t s);
 void timer_end(int id, hipStream_t s);
 

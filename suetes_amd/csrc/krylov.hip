@@ -17,6 +17,7 @@
 #include <map>
 #include <mutex>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "common.h"
@@ -916,6 +917,115 @@ static int gmres64(const OpT<double> &op, const SislGmresParams &p, const double
   return SISL_OK;
 }
 
+// ---- hipGraph of the inner Arnoldi cycle ----------------------------------------
+// One inner cycle is R Arnoldi steps of ~10 launches each with fixed
+// arguments (workspace and coefficient pointers, step index), gated on the
+// device by the cycle's stop flags and free of host synchronisation.  It is
+// captured once -- on a private stream of default priority, since the
+// caller's stream may be the legacy default stream -- and replayed on the
+// caller's stream for every later cycle with the same workspace, operator,
+// restart length, direction and coefficient storage.  The captured launches
+// are the ones the eager path makes (no memory-set nodes); with launch timing
+// on, the eager path runs so that every launch is timed.
+static std::mutex g_graph_mu;
+static bool g_graphs_on = true;
+static std::map<std::string, hipGraphExec_t> g_graph_cache;
+static std::map<int, hipStream_t> g_capture_streams;
+
+static hipStream_t capture_stream() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+  auto it = g_capture_streams.find(dev);
+  if (it != g_capture_streams.end()) return it->second;
+  hipStream_t cs = nullptr;
+  if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) return nullptr;
+  g_capture_streams[dev] = cs;
+  return cs;
+}
+
+template <typename T>
+static void key_add(std::string &k, const T &v) {
+  k.append(reinterpret_cast<const char *>(&v), sizeof(T));
+}
+
+template <typename TB>
+static std::string cycle_key(const Krylov<float, TB> &K, int R) {
+  std::string k;
+  int dev = 0;
+  hipGetDevice(&dev);
+  key_add(k, dev);
+  key_add(k, R);
+  key_add(k, (int)sizeof(TB));
+  key_add(k, K.transpose);
+  key_add(k, K.cb);
+  key_add(k, K.n);
+  const OpT<float> &o = K.op;
+  const int dims[5] = {o.s.nx, o.s.ny, o.s.nz, o.s.px, o.s.py};
+  key_add(k, dims);
+  key_add(k, o.alpha);
+  const void *ptrs[] = {K.V, K.w, K.t, K.scr, K.partials, K.sm, K.flags, K.cur,
+                        o.cua, o.zxu, o.cx, o.cva, o.zev, o.cy, o.dw, o.cw, o.cz, o.kx, o.ky, o.rdzw,
+                        o.cpm, o.mudx, o.mvdy, o.afu, o.afv, o.mmx, o.mmy, o.idw, o.gw, o.lpi, o.lower,
+                        o.invden, o.cprime, o.dzw0, o.dzwn};
+  key_add(k, ptrs);
+  return k;
+}
+
+// R Arnoldi steps of one inner cycle: replay of the captured graph, or the
+// eager launches.
+template <typename TB>
+static int run_inner_cycle(Krylov<float, TB> &K, int R) {
+  bool use_graph;
+  {
+    std::lock_guard<std::mutex> g(g_graph_mu);
+    use_graph = g_graphs_on;
+  }
+  if (!use_graph || timer_on()) {
+    for (int j = 0; j < R; ++j) {
+      const int rc = K.arnoldi(j, R, 1);
+      if (rc) return rc;
+    }
+    return SISL_OK;
+  }
+  const std::string key = cycle_key(K, R);
+  hipGraphExec_t exec = nullptr;
+  {
+    std::lock_guard<std::mutex> g(g_graph_mu);
+    auto it = g_graph_cache.find(key);
+    if (it != g_graph_cache.end()) exec = it->second;
+  }
+  if (!exec) {
+    hipStream_t cs;
+    {
+      std::lock_guard<std::mutex> g(g_graph_mu);
+      cs = capture_stream();
+    }
+    if (!cs) return SISL_ERR_LAUNCH;
+    const hipStream_t orig = K.s;
+    K.s = cs;
+    if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+      K.s = orig;
+      return SISL_ERR_LAUNCH;
+    }
+    int rc = SISL_OK;
+    for (int j = 0; j < R && rc == SISL_OK; ++j) rc = K.arnoldi(j, R, 1);
+    hipGraph_t graph = nullptr;
+    const hipError_t ec = hipStreamEndCapture(cs, &graph);
+    K.s = orig;
+    if (rc || ec != hipSuccess || !graph) {
+      if (graph) hipGraphDestroy(graph);
+      return rc ? rc : SISL_ERR_LAUNCH;
+    }
+    const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    if (ei != hipSuccess) return SISL_ERR_LAUNCH;
+    std::lock_guard<std::mutex> g(g_graph_mu);
+    g_graph_cache[key] = exec;
+  }
+  if (hipGraphLaunch(exec, K.s) != hipSuccess) return SISL_ERR_LAUNCH;
+  return SISL_OK;
+}
+
 // Mixed: FP64 residual refinement around FP32 GMRES cycles with early exit;
 // TB is the inner basis storage type (float, or FP16 scaled by sqrt(n)).
 template <typename TB>
@@ -969,8 +1079,7 @@ static int gmres_mixed(const OpT<double> &op64, const OpT<float> &op32, const Si
     if (!(rnorm > ptol) || outer >= max_outer) break;
     const double target = std::fmax(p.inner_rtol * rnorm, 0.5 * ptol);
     hipLaunchKernelGGL(reset_cycle_kernel, dim3(1), dim3(kBlock), 0, s, sm, R, rnorm, target, K.flags);
-    for (int j = 0; j < R; ++j)
-      if ((rc = K.arnoldi(j, R, 1))) return rc;
+    if ((rc = run_inner_cycle(K, R))) return rc;
     hipLaunchKernelGGL(lstsq_kernel, dim3(1), dim3(kLsqThreads), 0, s, sm, R, rnorm);
     combine<TB>(K.V, K.ldv(), R, sm->y, x, n, 1.0 / K.vscale(), s, &sm->steps);
     ++outer;
@@ -1433,6 +1542,12 @@ static int ens_solve(const void *ws, const SislGmresParams &p, const double *con
 }  // namespace sisl
 
 extern "C" {
+
+int sisl_set_graphs(int32_t on) {
+  std::lock_guard<std::mutex> g(sisl::g_graph_mu);
+  sisl::g_graphs_on = on != 0;
+  return SISL_OK;
+}
 
 int64_t sisl_gmres_workspace_bytes_f64(const SislGrid *g, int32_t restart) {
   if (!g || restart < 1 || restart > sisl::kMaxRestart) return -1;

@@ -48,6 +48,8 @@ __global__ void skip_probe_kernel(const int *stop, const int *gate, int *out) {
   *out = ((stop && *stop) || (gate && !*gate)) ? 1 : 0;
 }
 
+bool timer_on() { return timer().on; }
+
 int timer_begin(int kind, bool fp32, double bytes, const int *stop, const int *gate, hipStream_t s) {
   Timer &t = timer();
   if (!t.on) return -1;

@@ -302,3 +302,36 @@ def test_operator_launch_shapes_bit_identical_and_autotune(dtype):
     y = torch.empty_like(x)
     _lib.check(fn(C.byref(ops.struct), x.data_ptr(), y.data_ptr(), _lib.stream_ptr()), "A")
     assert torch.equal(y, outs[0])
+
+
+def test_inner_cycle_graph_replays_match_eager_launches():
+    """The mixed solver's inner Arnoldi cycle captured once as a hipGraph and
+    replayed: every replay (with eager GPU work and synchronisations in
+    between) gives bit-identical solutions to the eager launches, forward
+    and transposed."""
+    from suetes_amd import _lib
+    from suetes_amd.steppers import SemiImplicitSolver3D
+    cfg = small_lam()
+    case, go, A_o, M_o, sol = _solver_pair(cfg)
+    rng = np.random.default_rng(23)
+    b = pack_ref(A_o(_rand_krylov(go, rng)))
+    ms = SemiImplicitSolver3D(sol.physics, sol.dt, sol.alpha, solver_tol=1e-10, solver_maxiter=30,
+                              solver_restart=20, precision="mixed")
+    ms.prepare(case.stepper.bg_precomputed())
+    L = _lib.lib()
+    try:
+        for transpose in (False, True):
+            L.sisl_set_graphs(0)
+            ref = ms.solve_flat(b, torch.zeros_like(b), transpose=transpose)
+            cyc = ms.last_stats[-1]["cycles"]
+            assert cyc >= 3  # several cycles: the graph is replayed within one solve too
+            L.sisl_set_graphs(1)
+            for rep in range(3):
+                x = ms.solve_flat(b, torch.zeros_like(b), transpose=transpose)
+                torch.cuda.synchronize()
+                assert torch.equal(x, ref), (transpose, rep)
+                junk = torch.randn_like(b) * 2.0 + 1.0  # eager work between replays
+                torch.cuda.synchronize()
+                del junk
+    finally:
+        L.sisl_set_graphs(1)
