@@ -25,6 +25,11 @@ FLAGS = [
     # packed FP32 instructions stay off (they have produced wrong results on
     # these parts); plain float code is scheduled as scalar VALU ops instead.
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # multiply-adds are fused within one source expression only, never across
+    # statements: the rounding of every result is then fixed by the source,
+    # so the unrolled and chunked launch shapes of a kernel give the same bits
+    # (the default contraction fuses what unrolling happens to bring together).
+    "-ffp-contract=on",
     "-munsafe-fp-atomics", "-Wno-unused-result",
 ]
 

@@ -159,6 +159,9 @@ int blocks_A(const Stag &s);
 int blocks_M(const Stag &s);
 int blocks_MT_faces(const Stag &s);
 int seg_levels(int nz);  // levels per segment L of the preconditioner (0: too tall)
+// Generation of the operator launch shapes (advances when a shape is tuned or
+// pinned): a captured launch sequence is valid for one generation.
+uint64_t shape_generation();
 
 // Optional launch timing (timing.hip): returns a record id or -1 when off.
 bool timer_on();

@@ -504,6 +504,14 @@ struct AShape {
 };
 static std::mutex g_tune_mu;
 static std::map<std::tuple<int, int, int, int, int>, AShape> g_tune;  // (nx, ny, nz, bytes, variant)
+static uint64_t g_tune_generation = 0;  // advanced on every change of g_tune
+
+// Captured launch sequences embed the shapes in force when they were
+// recorded: they are valid while the generation is unchanged.
+uint64_t shape_generation() {
+  std::lock_guard<std::mutex> g(g_tune_mu);
+  return g_tune_generation;
+}
 
 // variant = transpose + 2 * (fused basis normalisation)
 static AShape a_shape(const Stag &s, int bytes, int variant = 0) {
@@ -679,6 +687,7 @@ static int autotune_A(const OpT<T> &o, const T *x, T *y, hipStream_t st, int32_t
   {
     std::lock_guard<std::mutex> g(g_tune_mu);
     for (int var = 0; var < nvar; ++var) g_tune[{o.s.nx, o.s.ny, o.s.nz, (int)sizeof(T), var}] = best[var];
+    ++g_tune_generation;
   }
   if (best_kc) *best_kc = best[0].kc;
   if (best_unr) *best_unr = best[0].unr;
@@ -756,7 +765,8 @@ int sisl_helmholtz_apply_f32(const SislOperator *op, const float *x, float *y, v
 int sisl_set_operator_shape(const SislGrid *g, int32_t fp32, int32_t chunk, int32_t unroll) {
   if (!g || chunk < 2 || (unroll != 1 && unroll != 2)) return SISL_ERR_ARG;
   std::lock_guard<std::mutex> lk(sisl::g_tune_mu);
-  sisl::g_tune[{g->nx, g->ny, g->nz, fp32 ? 4 : 8, 0}] = sisl::AShape{chunk, unroll};
+  for (int var = 0; var < 4; ++var) sisl::g_tune[{g->nx, g->ny, g->nz, fp32 ? 4 : 8, var}] = sisl::AShape{chunk, unroll};
+  ++sisl::g_tune_generation;
   return SISL_OK;
 }
 

@@ -924,12 +924,24 @@ static int gmres64(const OpT<double> &op, const SislGmresParams &p, const double
 // captured once -- on a private stream of default priority, since the
 // caller's stream may be the legacy default stream -- and replayed on the
 // caller's stream for every later cycle with the same workspace, operator,
-// restart length, direction and coefficient storage.  The captured launches
+// restart length, direction, coefficient storage and operator launch shapes
+// (a change of shapes retires every cached graph).  The captured launches
 // are the ones the eager path makes (no memory-set nodes); with launch timing
 // on, the eager path runs so that every launch is timed.
 static std::mutex g_graph_mu;
 static bool g_graphs_on = true;
 static std::map<std::string, hipGraphExec_t> g_graph_cache;
+static uint64_t g_graph_generation = 0;  // shape generation of the cached graphs
+constexpr size_t kMaxGraphs = 64;
+
+// Destroy every cached graph (after the device has drained, so that none is
+// still executing).  Caller holds g_graph_mu.
+static void purge_graphs() {
+  if (g_graph_cache.empty()) return;
+  hipDeviceSynchronize();
+  for (auto &e : g_graph_cache) hipGraphExecDestroy(e.second);
+  g_graph_cache.clear();
+}
 static std::map<int, hipStream_t> g_capture_streams;
 
 static hipStream_t capture_stream() {
@@ -963,6 +975,7 @@ static std::string cycle_key(const Krylov<float, TB> &K, int R) {
   const int dims[5] = {o.s.nx, o.s.ny, o.s.nz, o.s.px, o.s.py};
   key_add(k, dims);
   key_add(k, o.alpha);
+  key_add(k, shape_generation());  // the captured operator launches embed the current shapes
   const void *ptrs[] = {K.V, K.w, K.t, K.scr, K.partials, K.sm, K.flags, K.cur,
                         o.cua, o.zxu, o.cx, o.cva, o.zev, o.cy, o.dw, o.cw, o.cz, o.kx, o.ky, o.rdzw,
                         o.cpm, o.mudx, o.mvdy, o.afu, o.afv, o.mmx, o.mmy, o.idw, o.gw, o.lpi, o.lower,
@@ -991,6 +1004,11 @@ static int run_inner_cycle(Krylov<float, TB> &K, int R) {
   hipGraphExec_t exec = nullptr;
   {
     std::lock_guard<std::mutex> g(g_graph_mu);
+    const uint64_t gen = shape_generation();
+    if (gen != g_graph_generation || g_graph_cache.size() >= kMaxGraphs) {
+      purge_graphs();  // graphs of older shapes never match again; bound the cache
+      g_graph_generation = gen;
+    }
     auto it = g_graph_cache.find(key);
     if (it != g_graph_cache.end()) exec = it->second;
   }

@@ -47,27 +47,42 @@ class HyperFilter:
 
     def __init__(self, grid, operators, nu_h=1e6, nu_v=1e6):
         self.grid, self.op, self.nu_h, self.nu_v = grid, operators, nu_h, nu_v
+        self._metric_cache = None
+
+    def _metrics(self, bg):
+        """Terrain metrics of the Laplacian (the same in every call of a step):
+        dz / dz_w, z_x at u, z_y at v and their mass-point averages.  Kept
+        across calls while neither the heights nor the background spacing
+        carry a gradient (terrain controls recompute them every call)."""
+        op, g = self.op, self.grid
+        zm, dzw = g.i_Z_m, bg["dz_w_full"]
+        c = self._metric_cache
+        if c is not None and c[0] is zm and c[1] is dzw:
+            return c[2]
+        rz = g.dz / dzw
+        zxu = op.diff(zm, 0, "m", "u")
+        zev = op.diff(zm, 1, "m", "v")
+        vals = (rz, zxu, zev, op.avg(zxu, 0, "u", "m"), op.avg(zev, 1, "v", "m"))
+        if not (zm.requires_grad or dzw.requires_grad):
+            self._metric_cache = (zm, dzw, vals)
+        return vals
 
     def _cartesian_horizontal_laplacian(self, f, bg):
-        op, g = self.op, self.grid
-        rz = g.dz / bg["dz_w_full"]
+        op = self.op
+        rz, zxu, zev, zxm, zem = self._metrics(bg)
         dfx = op.diff(f, 0, "m", "u")
         dfz_m = op.avg(op.diff(f, 2, "m", "w") * rz, 2, "w", "m")
         dfz_u = op.avg(dfz_m, 0, "m", "u")
-        zxu = op.diff(g.i_Z_m, 0, "m", "u")
         Gx = dfx - zxu * dfz_u
         dfy = op.diff(f, 1, "m", "v")
         dfz_v = op.avg(dfz_m, 1, "m", "v")
-        zev = op.diff(g.i_Z_m, 1, "m", "v")
         Gy = dfy - zev * dfz_v
         dGx = op.diff(Gx, 0, "u", "m")
         Gxm = op.avg(Gx, 0, "u", "m")
         dGxz = op.avg(op.diff(Gxm, 2, "m", "w") * rz, 2, "w", "m")
-        zxm = op.avg(zxu, 0, "u", "m")
         dGy = op.diff(Gy, 1, "v", "m")
         Gym = op.avg(Gy, 1, "v", "m")
         dGyz = op.avg(op.diff(Gym, 2, "m", "w") * rz, 2, "w", "m")
-        zem = op.avg(zev, 1, "v", "m")
         return (dGx - zxm * dGxz) + (dGy - zem * dGyz)
 
     def _vertical_laplacian(self, f, bg):

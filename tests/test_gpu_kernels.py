@@ -267,10 +267,12 @@ def test_mixed_solver_reaches_tolerance(basis, coefs):
         assert float((x - x64).norm() / x64.norm()) < 1e-8, transpose
 
 
+@pytest.mark.parametrize("transpose", [False, True])
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
-def test_operator_launch_shapes_bit_identical_and_autotune(dtype):
-    """Every launch shape of the operator apply gives the same bits; the
-    autotuner returns a valid shape and leaves results unchanged."""
+def test_operator_launch_shapes_bit_identical_and_autotune(dtype, transpose):
+    """Every launch shape of the operator apply (and of its transpose) gives
+    the same bits; the autotuner returns a valid shape and leaves results
+    unchanged."""
     import ctypes as C
     from suetes_amd import _lib
     case, go, A_o, M_o, sol = _solver_pair(small_lam())
@@ -284,24 +286,66 @@ def test_operator_launch_shapes_bit_identical_and_autotune(dtype):
     n = int(L.sisl_krylov_size(C.byref(g)))
     x = torch.randn(n, dtype=dtype, device=DEV)
     fp32 = int(dtype == torch.float32)
-    fn = L.sisl_helmholtz_apply_f32 if fp32 else L.sisl_helmholtz_apply_f64
-    outs = []
-    for chunk in (2, 3, 5, 13):
+
+    def apply():
+        y = torch.empty_like(x)
+        if transpose:
+            fn = L.sisl_helmholtz_apply_T_f32 if fp32 else L.sisl_helmholtz_apply_T_f64
+            rc = fn(C.byref(ops.struct), x.data_ptr(), y.data_ptr(), None, _lib.stream_ptr())
+        else:
+            fn = L.sisl_helmholtz_apply_f32 if fp32 else L.sisl_helmholtz_apply_f64
+            rc = fn(C.byref(ops.struct), x.data_ptr(), y.data_ptr(), _lib.stream_ptr())
+        _lib.check(rc, "A")
+        return y
+
+    outs, shapes = [], []
+    for chunk in (2, 3, 5, 13, 81):
         for unr in (1, 2):
             _lib.check(L.sisl_set_operator_shape(C.byref(g), fp32, chunk, unr), "shape")
-            y = torch.empty_like(x)
-            _lib.check(fn(C.byref(ops.struct), x.data_ptr(), y.data_ptr(), _lib.stream_ptr()), "A")
-            outs.append(y)
-    for y in outs[1:]:
-        assert torch.equal(y, outs[0])
+            outs.append(apply())
+            shapes.append((chunk, unr))
+    for y, sh in zip(outs[1:], shapes[1:]):
+        assert torch.equal(y, outs[0]), (sh, (y - outs[0]).abs().max().item())
     kc, un = C.c_int32(), C.c_int32()
     scratch = torch.empty_like(x)
     _lib.check(L.sisl_autotune_operator(C.byref(ops.struct), fp32, x.data_ptr(), scratch.data_ptr(),
                                         C.byref(kc), C.byref(un), _lib.stream_ptr()), "autotune")
     assert kc.value >= 2 and un.value in (1, 2)
-    y = torch.empty_like(x)
-    _lib.check(fn(C.byref(ops.struct), x.data_ptr(), y.data_ptr(), _lib.stream_ptr()), "A")
-    assert torch.equal(y, outs[0])
+    assert torch.equal(apply(), outs[0])
+
+
+@pytest.mark.parametrize("transpose", [False, True])
+def test_mixed_solve_independent_of_launch_shapes(transpose):
+    """The mixed solver (whose inner cycles use the operator kernels with the
+    fused basis normalisation, in FP32 and bfloat16 coefficient storage)
+    gives the same bits whatever launch shapes are pinned, with the inner
+    cycle replayed from a captured graph or launched eagerly."""
+    import ctypes as C
+    from suetes_amd import _lib
+    from suetes_amd.steppers import SemiImplicitSolver3D
+    case, go, A_o, M_o, sol = _solver_pair(small_lam())
+    rng = np.random.default_rng(29)
+    b = pack_ref(A_o(_rand_krylov(go, rng)))
+    ms = SemiImplicitSolver3D(sol.physics, sol.dt, sol.alpha, solver_tol=1e-10, solver_maxiter=30,
+                              solver_restart=20, precision="mixed")
+    ms.prepare(case.stepper.bg_precomputed())
+    L = _lib.lib()
+    g = ms.op64.struct.g
+    outs = []
+    try:
+        for graphs in (0, 1):
+            L.sisl_set_graphs(graphs)
+            for chunk, unr in ((2, 1), (5, 2), (13, 1), (4, 2)):
+                for fp32 in (0, 1):
+                    _lib.check(L.sisl_set_operator_shape(C.byref(g), fp32, chunk, unr), "shape")
+                x = ms.solve_flat(b, torch.zeros_like(b), transpose=transpose)
+                torch.cuda.synchronize()
+                outs.append(((graphs, chunk, unr), x))
+    finally:
+        L.sisl_set_graphs(1)
+    ref = outs[0][1]
+    for tag, x in outs[1:]:
+        assert torch.equal(x, ref), (tag, (x - ref).abs().max().item())
 
 
 def test_inner_cycle_graph_replays_match_eager_launches():
