@@ -26,7 +26,7 @@ import threading
 import torch
 
 from . import _lib
-from .operators import Coords, _Tricubic
+from .operators import Coords, _tricubic_adjoint
 from .steppers import KEYS, LOC_ID, _operator_inputs, pack, state_to_internal, state_to_reference, unpack
 
 MAX_MEMBERS = 32
@@ -144,6 +144,8 @@ class _BatchedTricubic(torch.autograd.Function):
         # departure points through save_for_backward, so that a checkpointed
         # ensemble step drops them after the forward pass
         ctx.save_for_backward(*[c.data for c in coords_list])
+        ctx.shares = [c._share for c in coords_list]
+        ctx.toks = [sh.register() for sh in ctx.shares]
         ctx.lim = lim
         fields = [f.contiguous() for f in fields]
         outs = [torch.empty_like(f) for f in fields]
@@ -165,14 +167,13 @@ class _BatchedTricubic(torch.autograd.Function):
         if ctx.lim:
             raise NotImplementedError("the adjoint of the limited tricubic is not implemented")
         outs = []
-        for g, co in zip(grads, ctx.saved_tensors):
-            outs.append(None if g is None else _Tricubic.backward(_Ctx(Coords(co), False), g)[0])
+        for g, co, sh, tok in zip(grads, ctx.saved_tensors, ctx.shares, ctx.toks):
+            if g is None:
+                sh.done(tok)
+                outs.append(None)
+            else:
+                outs.append(_tricubic_adjoint(g, co, sh, tok))
         return (None, None) + tuple(outs)
-
-
-class _Ctx:
-    def __init__(self, coords, lim):
-        self.coords, self.use_limiter = coords, lim
 
 
 class _BatchedSolve(torch.autograd.Function):

@@ -8,6 +8,7 @@ the HIP gather; its backward is the deterministic segmented transpose.
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 
 import torch
 
@@ -196,14 +197,42 @@ class TricubicPlan:
                       self.base_x.data_ptr(), _lib.stream_ptr(coords.device)), "tricubic adjoint weights")
 
 
+class _Pending:
+    """Token of one gather whose adjoint has not run yet."""
+    __slots__ = ("__weakref__",)
+
+
+class _PlanShare:
+    """The adjoint plan of one coordinate set, shared by the adjoints of every
+    field interpolated with it.  ``pending`` holds (weakly) a token per gather
+    whose adjoint is still to come; the last adjoint releases the plan, so a
+    long checkpointed backward pass holds the plans of one step at a time.
+    Tokens of gathers whose autograd nodes die without a backward (a
+    checkpoint's recomputation, outputs outside the loss) vanish with them."""
+    __slots__ = ("plan", "pending")
+
+    def __init__(self):
+        self.plan, self.pending = None, weakref.WeakSet()
+
+    def register(self):
+        tok = _Pending()
+        self.pending.add(tok)
+        return tok
+
+    def done(self, tok):
+        self.pending.discard(tok)
+        if not self.pending:
+            self.plan = None
+
+
 class Coords:
     """Departure points of one staggering: tensor (3, Z, Y, X) of index-space
     offsets x_d - x_a (component 0 along x) plus a lazily built adjoint plan.
     ``absolute()`` returns the reference's coordinates (steppers.py:360)."""
 
-    def __init__(self, data: torch.Tensor):
+    def __init__(self, data: torch.Tensor, share: "_PlanShare" = None):
         self.data = data
-        self._plan = None
+        self._share = share if share is not None else _PlanShare()
 
     def absolute(self):
         _, nz, ny, nx = self.data.shape
@@ -214,9 +243,10 @@ class Coords:
         return torch.stack([self.data[a] + idx[a] for a in range(3)])
 
     def plan(self):
-        if self._plan is None:
-            self._plan = TricubicPlan(self.data, tuple(self.data.shape[1:]))
-        return self._plan
+        sh = self._share
+        if sh.plan is None:
+            sh.plan = TricubicPlan(self.data, tuple(self.data.shape[1:]))
+        return sh.plan
 
 
 def _tricubic_fwd(field, coords, use_limiter):
@@ -232,10 +262,23 @@ def _tricubic_fwd(field, coords, use_limiter):
     return out
 
 
+def _tricubic_adjoint(g, data, share, tok):
+    """Adjoint of one gather (token ``tok``) with departure offsets ``data``:
+    the plan comes from (or goes to) ``share``, which drops it after the last
+    pending adjoint."""
+    gf = tricubic_transpose(g.contiguous(), Coords(data, share))
+    share.done(tok)
+    return gf
+
+
 class _Tricubic(torch.autograd.Function):
     @staticmethod
     def forward(ctx, field, coords_obj, use_limiter):
-        ctx.coords = coords_obj
+        # the offsets go through save_for_backward (a checkpointed step drops
+        # them after its forward pass); only the small plan holder stays
+        ctx.save_for_backward(coords_obj.data)
+        ctx.share = coords_obj._share
+        ctx.tok = ctx.share.register()
         ctx.use_limiter = use_limiter
         return _tricubic_fwd(field.contiguous(), coords_obj.data, use_limiter)
 
@@ -243,7 +286,8 @@ class _Tricubic(torch.autograd.Function):
     def backward(ctx, g):
         if ctx.use_limiter:
             raise NotImplementedError("the adjoint of the limited tricubic is not implemented")
-        return tricubic_transpose(g.contiguous(), ctx.coords), None, None
+        (data,) = ctx.saved_tensors
+        return _tricubic_adjoint(g, data, ctx.share, ctx.tok), None, None
 
 
 def tensor_product_interp_3d(field, coords, use_limiter=False):

@@ -92,6 +92,39 @@ def test_tricubic_staged_and_far_departures_match_oracle(lim):
     assert rel_l2(from_int(out32), ref) < 2e-6
 
 
+@pytest.mark.parametrize("checkpointed", [False, True])
+def test_tricubic_adjoint_plan_shared_then_released(monkeypatch, checkpointed):
+    """Fields gathered at one coordinate set share one adjoint plan, built
+    at the first adjoint and released by the last (also when a checkpoint
+    recomputes the gathers, whose recomputed nodes never run a backward);
+    the gradients are the explicit transposes."""
+    from torch.utils.checkpoint import checkpoint
+    from suetes_amd import operators as ops
+    rng = np.random.default_rng(13)
+    shape = (10, 9, 8)
+    pts = np.stack([rng.uniform(-1, s, size=shape) for s in shape])
+    co = ops.Coords(offsets_int(pts))
+    fs = [to_int(rng.standard_normal(shape)).requires_grad_(True) for _ in range(3)]
+    gs = [to_int(rng.standard_normal(shape)) for _ in range(3)]
+    built = []
+    orig = ops.TricubicPlan.__init__
+
+    def counting(self, *a, **k):
+        built.append(1)
+        orig(self, *a, **k)
+    monkeypatch.setattr(ops.TricubicPlan, "__init__", counting)
+
+    def objective(*fields):
+        return sum((ops.tensor_product_interp_3d(f, co) * g).sum() for f, g in zip(fields, gs))
+    J = checkpoint(objective, *fs, use_reentrant=False) if checkpointed else objective(*fs)
+    assert len(co._share.pending) == 3 and co._share.plan is None
+    grads = torch.autograd.grad(J, fs)
+    assert len(built) == 1
+    assert len(co._share.pending) == 0 and co._share.plan is None
+    for gr, g in zip(grads, gs):
+        assert torch.equal(gr, ops.tricubic_transpose(g, ops.Coords(co.data.clone())))
+
+
 @pytest.mark.parametrize("method", ["rows", "segments"])
 def test_tricubic_adjoint_dot_and_determinism(method):
     from functools import partial
