@@ -182,6 +182,7 @@ struct Fuse {
   int nb = 0;
   double vs = 1.0;
   int cb = 0;  // 1: the 3-D coefficient arrays are bfloat16 (FP32 operator only)
+  const void *res = nullptr;  // packed vector b: store b - A x (not with nb)
 };
 template <typename T>
 int launch_A(const OpT<T> &op, const T *x, T *y, hipStream_t s, const int *stop = nullptr,

@@ -27,19 +27,23 @@ __device__ __forceinline__ int blocks_A_dev(const Stag &s) {
   return ((s.nx + kTX - 1) / kTX) * ((s.ny + kTY - 1) / kTY) * ((s.nz + 1 + kc - 1) / kc);
 }
 
-// Fused basis normalisation of the inner (FP32) Krylov solver.  With NB != 0
+// Fused basis normalisation of the inner (FP32) Krylov solver.  With NB = 1, 2
 // the input is the unnormalised Arnoldi vector w: the outputs are scaled by
 // 1 / ||w|| (A is linear) and every thread also stores the elements of the
 // normalised basis column vs * w / ||w|| that it owns -- the input elements at
 // its own output positions (NB = 1: FP16 storage, NB = 2: the working type).
 // This replaces a separate normalisation pass over w.
+//
+// NB = 3 (FP64 only) is the residual epilogue of the FP64 refinement instead:
+// every output is stored as res - (A x), res the packed right-hand side.
 template <typename T, int NB> struct BasisT { using type = T; };
 template <typename T> struct BasisT<T, 1> { using type = _Float16; };
 #define FUSED_NORMALISATION_SETUP                                                   \
   using BT = typename BasisT<T, NB>::type;                                          \
+  constexpr bool kNorm = NB == 1 || NB == 2;                                        \
   T osc = 1, bsc = 1;                                                               \
   BT *BU = nullptr, *BV = nullptr, *BW = nullptr, *BP = nullptr, *BE = nullptr;     \
-  if constexpr (NB != 0) {                                                          \
+  if constexpr (kNorm) {                                                            \
     const double iv = *inv;                                                         \
     osc = (T)iv;                                                                    \
     bsc = (T)(iv * vs);                                                             \
@@ -47,11 +51,11 @@ template <typename T> struct BasisT<T, 1> { using type = _Float16; };
     BU = b_ + s.ou; BV = b_ + s.ov; BW = b_ + s.ow; BP = b_ + s.om; BE = b_ + s.oe; \
   }                                                                                 \
   auto so = [&](T v) -> T {                                                         \
-    if constexpr (NB != 0) return v * osc;                                          \
+    if constexpr (kNorm) return v * osc;                                            \
     else return v;                                                                  \
   };                                                                                \
   auto bput = [&](BT *b, uint32_t q, T v) {                                         \
-    if constexpr (NB != 0) stg(b, q, (BT)(v * bsc));                                \
+    if constexpr (kNorm) stg(b, q, (BT)(v * bsc));                                  \
   };
 
 // Body of the forward operator for launch block `blk` of `nblk` (a member of
@@ -60,7 +64,7 @@ template <typename T, int UNR, int NB = 0, typename TC = T>
 __device__ __forceinline__ void helmholtz_body(const OpT<T> &o, const T *__restrict__ x,
                                                T *__restrict__ y, int kchunk, int blk, int nblk,
                                                const double *inv = nullptr, void *vout = nullptr,
-                                               double vs = 1.0) {
+                                               double vs = 1.0, const T *__restrict__ res = nullptr) {
   const Stag &s = o.s;
   const int nx = s.nx, ny = s.ny, nz = s.nz;
   const int nbx = (nx + kTX - 1) / kTX, nby = (ny + kTY - 1) / kTY;
@@ -73,6 +77,11 @@ __device__ __forceinline__ void helmholtz_body(const OpT<T> &o, const T *__restr
   const T *U = x + s.ou, *V = x + s.ov, *W = x + s.ow, *P = x + s.om, *E = x + s.oe;
   T *YU = y + s.ou, *YV = y + s.ov, *YW = y + s.ow, *YP = y + s.om, *YE = y + s.oe;
   FUSED_NORMALISATION_SETUP
+  // output store (NB = 3: the residual res - A x)
+  auto put = [&](T *Y_, uint32_t q, T v) {
+    if constexpr (NB == 3) v = ldg(res + (Y_ - y), q) - v;
+    stg(Y_, q, so(v));
+  };
   const bool uform = s.px || i > 0, vform = s.py || j > 0;  // else face 0 is the identity
   const T mudx = ldg(o.mudx, (uint32_t)s.u2(i, j)), afu = ldg(o.afu, (uint32_t)s.u2(i, j));
   const T mvdy = ldg(o.mvdy, (uint32_t)s.v2(i, j)), afv = ldg(o.afv, (uint32_t)s.v2(i, j));
@@ -134,8 +143,8 @@ __device__ __forceinline__ void helmholtz_body(const OpT<T> &o, const T *__restr
     const uint32_t wk = mc + qm;  // own column at w level k (same plane layout)
     const T wv = ldg(W, wk);
     if (k == nz) {  // top w level: both rows are identities
-      stg(YW, wk, so(wv));
-      stg(YE, wk, so(e0 * ldc<T>(c_rdzw, wk)));
+      put(YW, wk, wv);
+      put(YE, wk, e0 * ldc<T>(c_rdzw, wk));
       bput(BW, wk, wv);
       bput(BE, wk, e0);
       break;
@@ -167,15 +176,15 @@ __device__ __forceinline__ void helmholtz_body(const OpT<T> &o, const T *__restr
         out = u0 + ldc<T>(c_cua, au) * (mudx * (pc0 - pl0) - ldc<T>(c_zxu, au) * (half * (gzl + gzc))) -
               afu * (half * (vml + vm_k));
       }
-      stg(YU, au, so(out));
+      put(YU, au, out);
       bput(BU, au, u0);
       if (i == nx - 1 && !s.px) {                      // closed east face: identity
-        stg(YU, au + 1, so(u1));
+        put(YU, au + 1, u1);
         bput(BU, au + 1, u1);
       }
       if (i == 0 && s.px) {                            // periodic: face nx repeats face 0
-        stg(YU, ux + qu, so(out));
-        if constexpr (NB != 0) bput(BU, ux + qu, ldg(U, ux + qu));
+        put(YU, ux + qu, out);
+        if constexpr (kNorm) bput(BU, ux + qu, ldg(U, ux + qu));
       }
     }
     // ---- L_v at face j (euler.py:399, 416-420) ----
@@ -187,15 +196,15 @@ __device__ __forceinline__ void helmholtz_body(const OpT<T> &o, const T *__restr
         out = v0 + ldc<T>(c_cva, av) * (mvdy * (pc0 - pd0) - ldc<T>(c_zev, av) * (half * (gzd + gzc))) +
               afv * (half * (umd + um_k));
       }
-      stg(YV, av, so(out));
+      put(YV, av, out);
       bput(BV, av, v0);
       if (j == ny - 1 && !s.py) {
-        stg(YV, av + nx, so(v1));
+        put(YV, av + nx, v1);
         bput(BV, av + nx, v1);
       }
       if (j == 0 && s.py) {
-        stg(YV, vy + qv, so(out));
-        if constexpr (NB != 0) bput(BV, vy + qv, ldg(V, vy + qv));
+        put(YV, vy + qv, out);
+        if constexpr (kNorm) bput(BV, vy + qv, ldg(V, vy + qv));
       }
     }
     // ---- L_pi (euler.py:298-318, 401) ----
@@ -205,17 +214,17 @@ __device__ __forceinline__ void helmholtz_body(const OpT<T> &o, const T *__restr
     {
       const T fx = u1 * ldc<T>(c_cx, au + 1) - u0 * ldc<T>(c_cx, au);
       const T fy = v1 * ldc<T>(c_cy, av + nx) - v0 * ldc<T>(c_cy, av);
-      stg(YP, wk, so(pc0 + ldc<T>(c_cpm, wk) * (mmx * fx + mmy * fy + (fz1 - fz0))));
+      put(YP, wk, pc0 + ldc<T>(c_cpm, wk) * (mmx * fx + mmy * fy + (fz1 - fz0)));
       bput(BP, wk, pc0);
     }
     // ---- w level k: L_w (euler.py:400, 426-427), L_eta (euler.py:429-437) ----
     if (k == 0) {
-      stg(YW, wk, so(wv));
-      stg(YE, wk, so(e0 * ldc<T>(c_rdzw, wk)));
+      put(YW, wk, wv);
+      put(YE, wk, e0 * ldc<T>(c_rdzw, wk));
     } else {
-      stg(YW, wk, so(ldc<T>(c_dw, wk) * wv + ldc<T>(c_cw, wk) * (pc0 - pcm)));
-      stg(YE, wk, so(alpha * (e0 + ldc<T>(c_kx, wk) * (half * (umc_p + um_k)) +
-                              ldc<T>(c_ky, wk) * (half * (vmc_p + vm_k)) - wv)));
+      put(YW, wk, ldc<T>(c_dw, wk) * wv + ldc<T>(c_cw, wk) * (pc0 - pcm));
+      put(YE, wk, alpha * (e0 + ldc<T>(c_kx, wk) * (half * (umc_p + um_k)) +
+                              ldc<T>(c_ky, wk) * (half * (vmc_p + vm_k)) - wv));
     }
     bput(BW, wk, wv);
     bput(BE, wk, e0);
@@ -233,9 +242,9 @@ __global__ void __launch_bounds__(256) helmholtz_kernel(OpT<T> o, const T *__res
                                                         T *__restrict__ y, int kchunk,
                                                         const int *__restrict__ stop,
                                                         const double *__restrict__ inv, void *vout,
-                                                        double vs) {
+                                                        double vs, const T *__restrict__ res) {
   if (stop && *stop) return;
-  helmholtz_body<T, UNR, NB, TC>(o, x, y, kchunk, blockIdx.x, gridDim.x, inv, vout, vs);
+  helmholtz_body<T, UNR, NB, TC>(o, x, y, kchunk, blockIdx.x, gridDim.x, inv, vout, vs, res);
 }
 
 
@@ -257,7 +266,7 @@ template <typename T, int UNR, int NB = 0, typename TC = T>
 __device__ __forceinline__ void helmholtz_T_body(const OpT<T> &o, const T *__restrict__ y,
                                                  T *__restrict__ x, int kchunk, int blk, int nblk,
                                                  const double *inv = nullptr, void *vout = nullptr,
-                                                 double vs = 1.0) {
+                                                 double vs = 1.0, const T *__restrict__ res = nullptr) {
   const Stag &s = o.s;
   const int nx = s.nx, ny = s.ny, nz = s.nz;
   const int nbx = (nx + kTX - 1) / kTX, nby = (ny + kTY - 1) / kTY;
@@ -269,6 +278,11 @@ __device__ __forceinline__ void helmholtz_T_body(const OpT<T> &o, const T *__res
   const T *YU = y + s.ou, *YV = y + s.ov, *YW = y + s.ow, *YP = y + s.om, *YE = y + s.oe;
   T *XU = x + s.ou, *XV = x + s.ov, *XW = x + s.ow, *XP = x + s.om, *XE = x + s.oe;
   FUSED_NORMALISATION_SETUP
+  // output store (NB = 3: the residual res - A^T y)
+  auto put = [&](T *X_, uint32_t q, T v) {
+    if constexpr (NB == 3) v = ldg(res + (X_ - x), q) - v;
+    stg(X_, q, so(v));
+  };
   const bool px = s.px, py = s.py;
   const bool hl = i >= 1, hd = j >= 1;  // the cell to the left / below exists
   const T half = (T)0.5, mhalf = (T)-0.5, alpha = o.alpha, ha = half * o.alpha;
@@ -362,8 +376,8 @@ __device__ __forceinline__ void helmholtz_T_body(const OpT<T> &o, const T *__res
     const uint32_t wk = mc + qm;  // own column at w level k and at mass level k
     if (k == nz) {  // top w level: identity rows
       const T yw = ldg(YW, wk), ye = ldg(YE, wk);
-      stg(XW, wk, so(yw));
-      stg(XE, wk, so(ye * ldc<T>(c_rdzw, wk)));
+      put(XW, wk, yw);
+      put(XE, wk, ye * ldc<T>(c_rdzw, wk));
       bput(BW, wk, yw);
       bput(BE, wk, ye);
       break;
@@ -409,12 +423,12 @@ __device__ __forceinline__ void helmholtz_T_body(const OpT<T> &o, const T *__res
       T ub = 0;
       if (hl) ub += umb_l;
       ub += umb;
-      stg(XU, au, so(out + half * ub));
+      put(XU, au, out + half * ub);
       if (i == nx - 1) {
         const T yn = ldg(YU, au + 1);
         bput(BU, au + 1, yn);
         const T idn = px ? (T)0 : yn;
-        stg(XU, au + 1, so(idn + ldc<T>(c_cx, au + 1) * (cpmP * mmx_c) + half * umb));
+        put(XU, au + 1, idn + ldc<T>(c_cx, au + 1) * (cpmP * mmx_c) + half * umb);
       }
     }
     // ---- v face j (and the north face) ----
@@ -431,12 +445,12 @@ __device__ __forceinline__ void helmholtz_T_body(const OpT<T> &o, const T *__res
       T vb = 0;
       if (hd) vb += vmb_d;
       vb += vmb;
-      stg(XV, av, so(out + half * vb));
+      put(XV, av, out + half * vb);
       if (j == ny - 1) {
         const T yn = ldg(YV, av + nx);
         bput(BV, av + nx, yn);
         const T idn = py ? (T)0 : yn;
-        stg(XV, av + nx, so(idn + ldc<T>(c_cy, av + nx) * (cpmP * mmy_c) + half * vmb));
+        put(XV, av + nx, idn + ldc<T>(c_cy, av + nx) * (cpmP * mmy_c) + half * vmb);
       }
     }
     // ---- pi ----
@@ -456,7 +470,7 @@ __device__ __forceinline__ void helmholtz_T_body(const OpT<T> &o, const T *__res
       if (k + 1 <= nz - 1) out -= half * (g0 + g1) * ldc<T>(c_rdzw, w1);
       if (k == 0) out -= half * g0 * ldc<T>(c_rdzw, mc);
       if (k == nz - 2) out -= half * g1 * ldc<T>(c_rdzw, mc + (uint32_t)nz * pm);
-      stg(XP, wk, so(out));
+      put(XP, wk, out);
       bput(BP, wk, yp);
     }
     // ---- w level k: w and eta_dot ----
@@ -466,12 +480,12 @@ __device__ __forceinline__ void helmholtz_T_body(const OpT<T> &o, const T *__res
       if (k == 0) {
         const T ye = ldg(YE, wk);
         bput(BE, wk, ye);
-        stg(XW, wk, so(yw));
-        stg(XE, wk, so(ye * ldc<T>(c_rdzw, wk)));
+        put(XW, wk, yw);
+        put(XE, wk, ye * ldc<T>(c_rdzw, wk));
       } else {
         bput(BE, wk, e0);
-        stg(XW, wk, so(ldc<T>(c_dw, wk) * yw - alpha * e0));
-        stg(XE, wk, so(alpha * e0 + ldc<T>(c_cz, wk) * (cpmP_m1 - cpmP)));
+        put(XW, wk, ldc<T>(c_dw, wk) * yw - alpha * e0);
+        put(XE, wk, alpha * e0 + ldc<T>(c_cz, wk) * (cpmP_m1 - cpmP));
       }
     }
     // k+1 becomes k
@@ -489,9 +503,9 @@ __global__ void __launch_bounds__(256) helmholtz_T_kernel(OpT<T> o, const T *__r
                                                           T *__restrict__ x, int kchunk,
                                                           const int *__restrict__ stop,
                                                           const double *__restrict__ inv, void *vout,
-                                                          double vs) {
+                                                          double vs, const T *__restrict__ res) {
   if (stop && *stop) return;
-  helmholtz_T_body<T, UNR, NB, TC>(o, y, x, kchunk, blockIdx.x, gridDim.x, inv, vout, vs);
+  helmholtz_T_body<T, UNR, NB, TC>(o, y, x, kchunk, blockIdx.x, gridDim.x, inv, vout, vs, res);
 }
 
 // Launch shape of the operator and of its transpose: levels per thread
@@ -522,13 +536,14 @@ static AShape a_shape(const Stag &s, int bytes, int variant = 0) {
 }
 
 // Launch of kernel K<T, UNR, NB> (operator or transpose) in a given shape;
-// NB != 0 (fused basis normalisation) exists for the FP32 operator only.
+// NB = 1, 2 (fused basis normalisation) exist for the FP32 operator only, NB =
+// 3 (residual epilogue) for the FP64 one.
 template <typename T, int UNR, int NB, typename TC>
 struct AKern {
   static void go(dim3 g, hipStream_t st, const OpT<T> &o, const T *x, T *y, int kc, const int *stop,
                  const Fuse &f) {
     hipLaunchKernelGGL((helmholtz_kernel<T, UNR, NB, TC>), g, dim3(kTX, kTY), 0, st, o, x, y, kc, stop, f.inv,
-                       f.vout, f.vs);
+                       f.vout, f.vs, static_cast<const T *>(f.res));
   }
 };
 template <typename T, int UNR, int NB, typename TC>
@@ -536,12 +551,21 @@ struct ATKern {
   static void go(dim3 g, hipStream_t st, const OpT<T> &o, const T *x, T *y, int kc, const int *stop,
                  const Fuse &f) {
     hipLaunchKernelGGL((helmholtz_T_kernel<T, UNR, NB, TC>), g, dim3(kTX, kTY), 0, st, o, x, y, kc, stop,
-                       f.inv, f.vout, f.vs);
+                       f.inv, f.vout, f.vs, static_cast<const T *>(f.res));
   }
 };
 template <template <typename, int, int, typename> class K, typename T, typename TC>
 static int launch_shape_tc(dim3 g, const OpT<T> &o, const T *x, T *y, hipStream_t st, const int *stop, int kc,
                            int unr, const Fuse &f) {
+  if (f.res) {
+    if (f.nb) return SISL_ERR_ARG;
+    if constexpr (sizeof(T) == 8) {
+      if (unr == 2) K<T, 2, 3, TC>::go(g, st, o, x, y, kc, stop, f);
+      else K<T, 1, 3, TC>::go(g, st, o, x, y, kc, stop, f);
+      return SISL_OK;
+    }
+    return SISL_ERR_ARG;
+  }
   if (f.nb == 0) {
     if (unr == 2) K<T, 2, 0, TC>::go(g, st, o, x, y, kc, stop, f);
     else K<T, 1, 0, TC>::go(g, st, o, x, y, kc, stop, f);
@@ -588,7 +612,8 @@ int launch_A(const OpT<T> &o, const T *x, T *y, hipStream_t st, const int *stop,
   const Stag &s = o.s;
   const double sc = f.cb ? 2.0 : (double)sizeof(T);  // bytes per coefficient element
   const double bytes = (double)sizeof(T) * 2.0 * s.n + sc * (3.0 * s.u.n() + 3.0 * s.v.n() + 6.0 * s.w.n() + s.m.n()) +
-                       (f.nb == 1 ? 2.0 : f.nb == 2 ? (double)sizeof(T) : 0.0) * s.n;
+                       (f.nb == 1 ? 2.0 : f.nb == 2 ? (double)sizeof(T) : 0.0) * s.n +
+                       (f.res ? (double)sizeof(T) * s.n : 0.0);
   const int tid = timer_begin(SISL_T_HELMHOLTZ, sizeof(T) == 4, bytes, stop, nullptr, st);
   const int rc = launch_A_shape(o, x, y, st, stop, sh, f);
   timer_end(tid, st);
@@ -613,7 +638,8 @@ int launch_AT(const OpT<T> &o, const T *y, T *x, T * /*scratch: unused*/, hipStr
   // same arrays as the forward operator: y in, x out, 13 coefficient arrays (+ the basis column)
   const double sc = f.cb ? 2.0 : (double)sizeof(T);  // bytes per coefficient element
   const double bytes = (double)sizeof(T) * 2.0 * s.n + sc * (3.0 * s.u.n() + 3.0 * s.v.n() + 6.0 * s.w.n() + s.m.n()) +
-                       (f.nb == 1 ? 2.0 : f.nb == 2 ? (double)sizeof(T) : 0.0) * s.n;
+                       (f.nb == 1 ? 2.0 : f.nb == 2 ? (double)sizeof(T) : 0.0) * s.n +
+                       (f.res ? (double)sizeof(T) * s.n : 0.0);
   const int tid = timer_begin(SISL_T_HELMHOLTZ_T, sizeof(T) == 4, bytes, stop, nullptr, st);
   const int rc = launch_AT_shape(o, y, x, st, stop, sh, f);
   timer_end(tid, st);

@@ -507,14 +507,6 @@ __global__ void unit_kernel(const TS *__restrict__ src, TB *__restrict__ out, TC
   }
 }
 
-// t = b - t
-template <typename T>
-__global__ void residual_kernel(const T *__restrict__ b, T *__restrict__ t, int64_t n) {
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += stride)
-    t[e] = b[e] - t[e];
-}
-
 // x += sum_v y[v] vinv V[v]   (x is FP64)
 template <typename TV>
 __device__ __forceinline__ void combine_body(const TV *__restrict__ V, int64_t ldv, int nv,
@@ -864,15 +856,15 @@ static int gmres64(const OpT<double> &op, const SislGmresParams &p, const double
     norm2<double>(K.t, n, K.partials, &sm->scal[1], s);
   }
   // residual M (b - A x), unit vector into V[0]
+  Fuse rb;  // the operator launch stores b - A x
+  rb.res = b;
   auto residual = [&](double *scal_slot) -> int {
     int r2;
     if (!p.transpose) {
-      if ((r2 = launch_A<double>(op, x, K.w, s))) return r2;
-      hipLaunchKernelGGL(residual_kernel<double>, dim3(kRedGrid), dim3(kBlock), 0, s, b, K.w, n);
+      if ((r2 = launch_A<double>(op, x, K.w, s, nullptr, &rb))) return r2;
       if ((r2 = launch_M<double>(op, K.w, K.w, s))) return r2;
     } else {
-      if ((r2 = launch_AT<double>(op, x, K.t, K.scr, s))) return r2;
-      hipLaunchKernelGGL(residual_kernel<double>, dim3(kRedGrid), dim3(kBlock), 0, s, b, K.t, n);
+      if ((r2 = launch_AT<double>(op, x, K.t, K.scr, s, nullptr, &rb))) return r2;
       if ((r2 = launch_MT<double>(op, K.t, K.w, K.scr, s))) return r2;
     }
     norm2<double>(K.w, n, K.partials, scal_slot, s);
@@ -1076,17 +1068,18 @@ static int gmres_mixed(const OpT<double> &op64, const OpT<float> &op32, const Si
   const double atol = std::fmax(p.tol * b_norm, p.atol);
   const double ptol = Mb_norm * std::fmin(1.0, atol / b_norm);
   const int max_outer = p.max_outer > 0 ? p.max_outer : p.maxiter;
+  Fuse rb;  // the FP64 operator launch stores b - A x
+  rb.res = b;
   hipLaunchKernelGGL(zero_steps_total_kernel, dim3(1), dim3(1), 0, s, sm);
   int outer = 0;
   double rnorm = 0;
   for (;;) {
-    // r = b - A x (FP64), z = M r (FP64)
+    // r = b - A x (FP64, stored by the operator launch), z = M r (FP64)
     if (!p.transpose) {
-      if ((rc = launch_A<double>(op64, x, t64, s))) return rc;
+      if ((rc = launch_A<double>(op64, x, t64, s, nullptr, &rb))) return rc;
     } else {
-      if ((rc = launch_AT<double>(op64, x, t64, scr64, s))) return rc;
+      if ((rc = launch_AT<double>(op64, x, t64, scr64, s, nullptr, &rb))) return rc;
     }
-    hipLaunchKernelGGL(residual_kernel<double>, dim3(kRedGrid), dim3(kBlock), 0, s, b, t64, n);
     if ((rc = M64(t64, w64))) return rc;
     norm2<double>(w64, n, partials, &sm->scal[2], s);
     hipLaunchKernelGGL((unit_kernel<double, TB, float>), dim3(kRedGrid), dim3(kBlock), 0, s,
