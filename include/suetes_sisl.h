@@ -203,6 +203,10 @@ typedef struct SislGmresParams {
                               FP16 (scaled by sqrt(n)) instead of FP32        */
   int32_t coef_bf16;       /* mixed solver: the FP32 operator's 3-D coefficient
                               arrays are bfloat16 (2-D ones stay FP32)        */
+  int32_t inner_single_cgs;/* mixed solver: one classical Gram-Schmidt pass per
+                              inner Arnoldi step (0: iterated CGS as in the
+                              FP64 solver, a second pass when the first
+                              removed most of the vector)                     */
 } SislGmresParams;
 
 typedef struct SislGmresStats {

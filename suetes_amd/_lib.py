@@ -55,7 +55,7 @@ class SislGmresParams(C.Structure):
     _fields_ = [("tol", C.c_double), ("atol", C.c_double), ("restart", C.c_int32),
                 ("maxiter", C.c_int32), ("transpose", C.c_int32), ("early_exit", C.c_int32),
                 ("inner_rtol", C.c_double), ("max_outer", C.c_int32), ("basis_fp16", C.c_int32),
-                ("coef_bf16", C.c_int32)]
+                ("coef_bf16", C.c_int32), ("inner_single_cgs", C.c_int32)]
 
 
 class SislGmresStats(C.Structure):

@@ -746,6 +746,7 @@ struct Krylov {
   hipStream_t s;
   TV *cur = nullptr;
   int cb = 0;  // 1: the operator's 3-D coefficient arrays are bfloat16
+  int single_cgs = 0;  // 1: one Gram-Schmidt pass per step (no second pass)
   int64_t ldv() const { return basis_ld(n); }
   double vscale() const { return basis_scale<TB>(n); }
   const TV *basis_in(int j) const {
@@ -799,12 +800,18 @@ struct Krylov {
     // each pass is followed by one tail launch: its reduction and the bookkeeping after it
     Tail t{sm->h1, nv + 1, 0, sm, nv, j, R, early_exit, flags + j + 1};
     dots<TB, TV>(V, ld, nv, wj, n, partials, s, vinv, stop, nullptr, &t);  // h1[0..nv-1], ||w||^2 in h1[nv]
-    t.out = sm->nn1a; t.nout = 1; t.mode = 1;                               // + second-pass decision
-    update<TB, TV>(V, ld, nv, sm->h1, wj, n, partials, s, vinv, stop, nullptr, &t);
-    t.out = sm->h2; t.nout = nv; t.mode = 0;
-    dots<TB, TV>(V, ld, nv, wj, n, partials, s, vinv, stop, gate, &t);
-    t.out = sm->nn1b; t.nout = 1; t.mode = 2;                               // + Arnoldi bookkeeping
-    update<TB, TV>(V, ld, nv, sm->h2, wj, n, partials, s, vinv, stop, gate, &t);
+    if (single_cgs) {
+      // one pass: gate2 stays 0 from the cycle reset, so the bookkeeping reads h1, nn1a
+      t.out = sm->nn1a; t.nout = 1; t.mode = 2;
+      update<TB, TV>(V, ld, nv, sm->h1, wj, n, partials, s, vinv, stop, nullptr, &t);
+    } else {
+      t.out = sm->nn1a; t.nout = 1; t.mode = 1;                             // + second-pass decision
+      update<TB, TV>(V, ld, nv, sm->h1, wj, n, partials, s, vinv, stop, nullptr, &t);
+      t.out = sm->h2; t.nout = nv; t.mode = 0;
+      dots<TB, TV>(V, ld, nv, wj, n, partials, s, vinv, stop, gate, &t);
+      t.out = sm->nn1b; t.nout = 1; t.mode = 2;                             // + Arnoldi bookkeeping
+      update<TB, TV>(V, ld, nv, sm->h2, wj, n, partials, s, vinv, stop, gate, &t);
+    }
     if constexpr (fused) {
       SISL_CHECK_LAUNCH();
       return SISL_OK;
@@ -962,6 +969,7 @@ static std::string cycle_key(const Krylov<float, TB> &K, int R) {
   key_add(k, (int)sizeof(TB));
   key_add(k, K.transpose);
   key_add(k, K.cb);
+  key_add(k, K.single_cgs);
   key_add(k, K.n);
   const OpT<float> &o = K.op;
   const int dims[5] = {o.s.nx, o.s.ny, o.s.nz, o.s.px, o.s.py};
@@ -1052,6 +1060,7 @@ static int gmres_mixed(const OpT<double> &op64, const OpT<float> &op32, const Si
                       (int *)(ws + L.flags), n, s};
   K.cur = (float *)(ws + L.cur32);  // second Arnoldi buffer (and the FP16 path's unit residual)
   K.cb = p.coef_bf16 ? 1 : 0;
+  K.single_cgs = p.inner_single_cgs ? 1 : 0;
   int rc;
   double hs[4];
   auto M64 = [&](double *in, double *out) -> int {
@@ -1427,7 +1436,7 @@ static void kb_dots_dispatch(const EnsPlan &P, int nv, int j, int gated, hipStre
   else hipLaunchKernelGGL(kb_dots_kernel, gr, bl, 0, st, P.km, nv, j, gated);
 }
 
-static int ens_arnoldi(const EnsPlan &P, int j, int transpose, hipStream_t st) {
+static int ens_arnoldi(const EnsPlan &P, int j, int transpose, int single_cgs, hipStream_t st) {
   int rc;
   const int M = P.M, R = P.R, nv = j + 1;
   hipLaunchKernelGGL(kb_snapshot_kernel, dim3(M), dim3(64), 0, st, (const KMem *)P.km, j);
@@ -1442,11 +1451,13 @@ static int ens_arnoldi(const EnsPlan &P, int j, int transpose, hipStream_t st) {
   hipLaunchKernelGGL(kb_reduce_kernel, dim3(nv + 1, M), dim3(kBlock), 0, st, (const KMem *)P.km, j, kSelH1, 0, 0);
   hipLaunchKernelGGL(kb_update_kernel, dim3(kRedGrid, M), dim3(kBlock), 0, st, (const KMem *)P.km, nv, j, kSelH1, 0);
   hipLaunchKernelGGL(kb_reduce_kernel, dim3(1, M), dim3(kBlock), 0, st, (const KMem *)P.km, j, kSelNn1a, 0, 0);
-  hipLaunchKernelGGL(kb_decide_kernel, dim3(M), dim3(64), 0, st, (const KMem *)P.km, nv);
-  kb_dots_dispatch(P, nv, j, 1, st);
-  hipLaunchKernelGGL(kb_reduce_kernel, dim3(nv, M), dim3(kBlock), 0, st, (const KMem *)P.km, j, kSelH2, 1, 0);
-  hipLaunchKernelGGL(kb_update_kernel, dim3(kRedGrid, M), dim3(kBlock), 0, st, (const KMem *)P.km, nv, j, kSelH2, 1);
-  hipLaunchKernelGGL(kb_reduce_kernel, dim3(1, M), dim3(kBlock), 0, st, (const KMem *)P.km, j, kSelNn1b, 1, 0);
+  if (!single_cgs) {  // one pass: gate2 stays 0 from the cycle reset, the bookkeeping reads h1, nn1a
+    hipLaunchKernelGGL(kb_decide_kernel, dim3(M), dim3(64), 0, st, (const KMem *)P.km, nv);
+    kb_dots_dispatch(P, nv, j, 1, st);
+    hipLaunchKernelGGL(kb_reduce_kernel, dim3(nv, M), dim3(kBlock), 0, st, (const KMem *)P.km, j, kSelH2, 1, 0);
+    hipLaunchKernelGGL(kb_update_kernel, dim3(kRedGrid, M), dim3(kBlock), 0, st, (const KMem *)P.km, nv, j, kSelH2, 1);
+    hipLaunchKernelGGL(kb_reduce_kernel, dim3(1, M), dim3(kBlock), 0, st, (const KMem *)P.km, j, kSelNn1b, 1, 0);
+  }
   hipLaunchKernelGGL(kb_finalize_kernel, dim3(M), dim3(64), 0, st, (const KMem *)P.km, j, R);
   hipLaunchKernelGGL(kb_normalize_kernel, dim3(kRedGrid, M), dim3(kBlock), 0, st, (const KMem *)P.km, j);
   SISL_CHECK_LAUNCH();
@@ -1523,7 +1534,7 @@ static int ens_solve(const void *ws, const SislGmresParams &p, const double *con
     for (int m = 0; m < M; ++m) all = all && done[m];
     if (all || outer > max_outer) break;
     for (int j = 0; j < R; ++j)
-      if ((rc = ens_arnoldi(P, j, p.transpose, st))) return rc;
+      if ((rc = ens_arnoldi(P, j, p.transpose, p.inner_single_cgs, st))) return rc;
     hipLaunchKernelGGL(kb_lstsq_kernel, dim3(M), dim3(kLsqThreads), 0, st, (const KMem *)P.km, R);
     hipLaunchKernelGGL(kb_combine_kernel, dim3(kRedGrid, M), dim3(kBlock), 0, st, (const KMem *)P.km);
   }

@@ -280,9 +280,10 @@ def test_gmres_solution_matches_oracle():
     assert float(r.norm() / c.norm()) < 1e-9
 
 
+@pytest.mark.parametrize("gs", ["single", "iterated"])
 @pytest.mark.parametrize("basis", ["fp16", "fp32"])
 @pytest.mark.parametrize("coefs", ["bf16", "fp32"])
-def test_mixed_solver_reaches_tolerance(basis, coefs):
+def test_mixed_solver_reaches_tolerance(basis, coefs, gs):
     cfg = small_lam()
     case, go, A_o, M_o, sol = _solver_pair(cfg)
     rng = np.random.default_rng(22)
@@ -290,7 +291,7 @@ def test_mixed_solver_reaches_tolerance(basis, coefs):
     from suetes_amd.steppers import SemiImplicitSolver3D
     ms = SemiImplicitSolver3D(sol.physics, sol.dt, sol.alpha, solver_tol=1e-10, solver_maxiter=30,
                               solver_restart=20, precision="mixed", inner_basis=basis,
-                              inner_coefficients=coefs)
+                              inner_coefficients=coefs, inner_gram_schmidt=gs)
     ms.prepare(case.stepper.bg_precomputed())
     for transpose in (False, True):
         x = ms.solve_flat(b, torch.zeros_like(b), transpose=transpose)
