@@ -117,22 +117,23 @@ int sisl_tricubic_adj_f64(int32_t fnx, int32_t fny, int32_t fnz, const double *c
 int sisl_tricubic_adj_f32(int32_t fnx, int32_t fny, int32_t fnz, const float *coords,
                           const int32_t *perm, const int32_t *start, const float *gout,
                           float *gfield, void *stream);
-/* Faster transpose over the same plan: per sorted position the source's 12
- * Catmull-Rom weights (x, y, z, 4 each; weights[12n]) and base x index
- * (base_x[n]), built once per coordinate set; then gf = T^T g with g gathered
- * into sorted order (g_sorted[n], scratch) and, per destination, a fixed-order
- * sum over the 16 (y, z) rows of candidate bases -- deterministic, the same
- * operator as sisl_tricubic_adj_* up to summation order. */
-int sisl_tricubic_adj_weights_f64(int32_t fnx, int32_t fny, int32_t fnz, const double *coords,
-                                  const int32_t *perm, double *weights, int32_t *base_x, void *stream);
-int sisl_tricubic_adj_weights_f32(int32_t fnx, int32_t fny, int32_t fnz, const float *coords,
-                                  const int32_t *perm, float *weights, int32_t *base_x, void *stream);
+/* Faster transpose over the same plan: per sorted position the source's
+ * fractions (tx, ty, tz) and base x index as four values (frac[4n]), built
+ * once per coordinate set; then gf = T^T g with g gathered into sorted order
+ * (g_sorted[n], scratch) and, per destination, a fixed-order sum over the 16
+ * (y, z) rows of candidate bases (sources staged in LDS per 64 x 4 tile) --
+ * deterministic, the same operator as sisl_tricubic_adj_* up to summation
+ * order. */
+int sisl_tricubic_adj_frac_f64(int32_t fnx, int32_t fny, int32_t fnz, const double *coords,
+                               const int32_t *perm, double *frac, void *stream);
+int sisl_tricubic_adj_frac_f32(int32_t fnx, int32_t fny, int32_t fnz, const float *coords,
+                               const int32_t *perm, float *frac, void *stream);
 int sisl_tricubic_adj_rows_f64(int32_t fnx, int32_t fny, int32_t fnz, const int32_t *perm,
-                               const int32_t *start, const double *weights, const int32_t *base_x,
-                               const double *g, double *g_sorted, double *gf, void *stream);
+                               const int32_t *start, const double *frac, const double *g,
+                               double *g_sorted, double *gf, void *stream);
 int sisl_tricubic_adj_rows_f32(int32_t fnx, int32_t fny, int32_t fnz, const int32_t *perm,
-                               const int32_t *start, const float *weights, const int32_t *base_x,
-                               const float *g, float *g_sorted, float *gf, void *stream);
+                               const int32_t *start, const float *frac, const float *g,
+                               float *g_sorted, float *gf, void *stream);
 
 /* _get_index_velocities + compute_departure_indices
  * (suetes/regional3d/steppers.py:276-360): departure points at staggering

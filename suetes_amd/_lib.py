@@ -26,8 +26,8 @@ EXPORTED = [
     "sisl_set_operator_shape", "sisl_tricubic_batched_f64", "sisl_tricubic_batched_f32",
     "sisl_departure_batched_f64", "sisl_departure_batched_f32", "sisl_gmres_batched_workspace_bytes",
     "sisl_gmres_batched_setup", "sisl_gmres_batched", "sisl_stagger_f64", "sisl_stagger_f32",
-    "sisl_stagger_T_f64", "sisl_stagger_T_f32", "sisl_tricubic_adj_weights_f64",
-    "sisl_tricubic_adj_weights_f32", "sisl_tricubic_adj_rows_f64", "sisl_tricubic_adj_rows_f32",
+    "sisl_stagger_T_f64", "sisl_stagger_T_f32", "sisl_tricubic_adj_frac_f64",
+    "sisl_tricubic_adj_frac_f32", "sisl_tricubic_adj_rows_f64", "sisl_tricubic_adj_rows_f32",
     "sisl_ffsl_1d_f64", "sisl_ffsl_1d_f32", "sisl_ffsl_1d_T_f64", "sisl_ffsl_1d_T_f32",
     "sisl_set_graphs",
 ]
@@ -88,8 +88,8 @@ def lib():
             getattr(L, f"sisl_tricubic_{p}").argtypes = [i32, i32, i32, vp, vp, vp, i32, vp]
             getattr(L, f"sisl_tricubic_adj_plan_{p}").argtypes = [i32, i32, i32, vp, vp, vp, vp, vp, vp, vp]
             getattr(L, f"sisl_tricubic_adj_{p}").argtypes = [i32, i32, i32, vp, vp, vp, vp, vp, vp]
-            getattr(L, f"sisl_tricubic_adj_weights_{p}").argtypes = [i32, i32, i32, vp, vp, vp, vp, vp]
-            getattr(L, f"sisl_tricubic_adj_rows_{p}").argtypes = [i32, i32, i32, vp, vp, vp, vp, vp, vp, vp, vp]
+            getattr(L, f"sisl_tricubic_adj_frac_{p}").argtypes = [i32, i32, i32, vp, vp, vp, vp]
+            getattr(L, f"sisl_tricubic_adj_rows_{p}").argtypes = [i32, i32, i32, vp, vp, vp, vp, vp, vp, vp]
             getattr(L, f"sisl_departure_{p}").argtypes = [C.POINTER(SislGrid), i32, vp, vp, vp, vp,
                                                           d, d, d, i32, vp, vp, vp]
             getattr(L, f"sisl_helmholtz_apply_{p}").argtypes = [C.POINTER(SislOperator), vp, vp, vp]

@@ -599,10 +599,11 @@ __device__ __forceinline__ T tap_weight(int i, int b, int n, const T *__restrict
   return s;
 }
 
+// Sorted-row plan: per sorted position r the source's fractions (tx, ty, tz)
+// and base x, as four values of T (the base exactly representable).
 template <typename T>
-__global__ void __launch_bounds__(256) adj_weights_kernel(Dims d, const T *__restrict__ co,
-                                                          const int32_t *__restrict__ perm,
-                                                          T *__restrict__ ws, int32_t *__restrict__ bxs) {
+__global__ void __launch_bounds__(256) adj_frac_kernel(Dims d, const T *__restrict__ co,
+                                                       const int32_t *__restrict__ perm, T *__restrict__ fr) {
   const int64_t n = d.n();
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= n) return;
@@ -612,13 +613,10 @@ __global__ void __launch_bounds__(256) adj_weights_kernel(Dims d, const T *__res
   const int bx = offset_split(X, co[p], d.nx, tx);
   offset_split(Y, co[n + p], d.ny, ty);
   offset_split(Z, co[2 * n + p], d.nz, tz);
-  T w[12];
-  cr_weights(tx, w);
-  cr_weights(ty, w + 4);
-  cr_weights(tz, w + 8);
-#pragma unroll
-  for (int a = 0; a < 12; ++a) ws[r * 12 + a] = w[a];
-  bxs[r] = bx;
+  fr[4 * r] = tx;
+  fr[4 * r + 1] = ty;
+  fr[4 * r + 2] = tz;
+  fr[4 * r + 3] = (T)bx;
 }
 
 template <typename T>
@@ -628,62 +626,189 @@ __global__ void __launch_bounds__(256) adj_gather_kernel(int64_t n, const int32_
   if (r < n) gs[r] = g[perm[r]];
 }
 
+// Contribution of sorted source (fractions f[0..2], base x f[3], value gv) to
+// destination (i, j, k) from its (by, bz) row: the product of the three
+// tap weights (clamped taps summed), accumulated as wx (wy (wz g)).
 template <typename T>
-__global__ void __launch_bounds__(256) tricubic_adj_rows_kernel(Dims d, const int32_t *__restrict__ start,
-                                                                const T *__restrict__ ws,
-                                                                const int32_t *__restrict__ bxs,
+__device__ __forceinline__ T adj_term(const T *f, T gv, int i, int j, int k, int by, int bz, Dims d) {
+  T wx[4], wy[4], wz[4];
+  cr_weights(f[0], wx);
+  cr_weights(f[1], wy);
+  cr_weights(f[2], wz);
+  const T a = tap_weight(i, (int)f[3], d.nx, wx);
+  const T b = tap_weight(j, by, d.ny, wy);
+  const T c = tap_weight(k, bz, d.nz, wz);
+  return a * (b * (c * gv));
+}
+
+// Transpose gather over the sorted-row plan.  One workgroup per 64 x 4 tile
+// of destinations on one level k.  The sources of the 4 x 7 (bz, by) rows
+// that can reach the tile (bases x in [i0-2, i0+64]) are staged in LDS, each
+// with the work that does not depend on the destination column done once:
+// its effective x weights for the four columns bx-1..bx+2 (clamped taps
+// summed) and, for each of the tile's four rows j, wy(j) (wz(k) g).  Every
+// destination then sums, in a fixed order (bz, by, sorted position), wx times
+// that product over the sources of the 16 rows and 4 bases around it -- the
+// same products in the same order as the plain loop, which a tile whose
+// sources do not fit in LDS runs from global memory.
+constexpr int kAdjTX = 64, kAdjTY = 4, kAdjRows = 4 * (kAdjTY + 3), kAdjW = kAdjTX + 4;
+template <typename T> struct AdjCap { static constexpr int v = 2048; };
+template <> struct AdjCap<double> { static constexpr int v = 1024; };
+
+template <typename T>
+__global__ void __launch_bounds__(256) tricubic_adj_tile_kernel(Dims d, const int32_t *__restrict__ start,
+                                                                const T *__restrict__ fr,
                                                                 const T *__restrict__ gs,
                                                                 T *__restrict__ gf) {
-  const uint32_t n = (uint32_t)d.n();
-  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= n) return;
+  constexpr int cap = AdjCap<T>::v;
+  __shared__ T sx[cap * 4];        // effective x weights of columns bx-1 .. bx+2
+  __shared__ T sy[cap * kAdjTY];   // wy(j0 + t) (wz(k) g) for the tile's rows
+  __shared__ int sb[cap];          // base x
+  __shared__ int st[kAdjRows][kAdjW];  // per row: sorted index of the first source of bases xa..xb+1
+  __shared__ int off[kAdjRows + 1];
   const int nx = d.nx, ny = d.ny, nz = d.nz;
-  const int i = (int)(q % (uint32_t)nx);
-  const uint32_t rq = q / (uint32_t)nx;
-  const int j = (int)(rq % (uint32_t)ny), k = (int)(rq / (uint32_t)ny);
+  const int nbx = (nx + kAdjTX - 1) / kAdjTX, nby = (ny + kAdjTY - 1) / kAdjTY;
+  const int tile = (int)blockIdx.x;
+  const int i0 = (tile % nbx) * kAdjTX, j0 = ((tile / nbx) % nby) * kAdjTY, k = tile / (nbx * nby);
+  static_assert(kAdjTX * kAdjTY == 256, "the staging loops assume 256 threads");
+  const int tid = threadIdx.y * kAdjTX + threadIdx.x;
+  const int xa = max(i0 - 2, 0), xb = min(i0 + kAdjTX, nx - 1);
+  const int wspan = xb + 2 - xa;  // start entries per row: bases xa..xb and the end of xb
+  // row slot q = dz * (kAdjTY + 3) + dy: bz = k - 2 + dz, by = j0 - 2 + dy.
+  // Every load of a thread is issued before any is used (the loops below have
+  // fixed trip counts and are unrolled): the staging is latency-bound.
+  constexpr int kStE = kAdjRows * kAdjW, kStPer = (kStE + 255) / 256;
+  {
+    int sv[kStPer];
+#pragma unroll
+    for (int u = 0; u < kStPer; ++u) {
+      const int e = tid + 256 * u;
+      const int q = e / kAdjW, x = e % kAdjW;
+      const int bz = k - 2 + q / (kAdjTY + 3), by = j0 - 2 + q % (kAdjTY + 3);
+      sv[u] = (e < kStE && x < wspan && bz >= 0 && bz < nz && by >= 0 && by < ny)
+                  ? start[((uint32_t)bz * ny + by) * nx + xa + x]
+                  : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < kStPer; ++u) {
+      const int e = tid + 256 * u;
+      if (e < kStE) st[e / kAdjW][e % kAdjW] = sv[u];
+    }
+  }
+  __syncthreads();
+  if (tid == 0) {
+    int c[kAdjRows];
+#pragma unroll
+    for (int q = 0; q < kAdjRows; ++q) c[q] = st[q][wspan - 1] - st[q][0];
+    int o = 0;
+#pragma unroll
+    for (int q = 0; q < kAdjRows; ++q) {
+      off[q] = o;
+      o += c[q];
+    }
+    off[kAdjRows] = o;
+  }
+  __syncthreads();
+  const int total = off[kAdjRows];
+  const bool staged = total <= cap;
+  if (staged) {
+    constexpr int kPer = cap / 256;
+    T f[kPer][4], gv[kPer];
+    int qs[kPer];
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      const int e = tid + 256 * u;
+      qs[u] = -1;
+      if (e < total) {
+        int lo = 0, hi = kAdjRows - 1;  // the row q with off[q] <= e < off[q + 1]
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (off[mid] <= e) lo = mid;
+          else hi = mid - 1;
+        }
+        qs[u] = lo;
+        const uint32_t r = (uint32_t)(st[lo][0] + (e - off[lo]));
+#pragma unroll
+        for (int c = 0; c < 4; ++c) f[u][c] = fr[4 * r + c];
+        gv[u] = gs[r];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      const int e = tid + 256 * u, q = qs[u];
+      if (q < 0) continue;
+      const int bz = k - 2 + q / (kAdjTY + 3), by = j0 - 2 + q % (kAdjTY + 3);
+      T wx[4], wy[4], wz[4];
+      cr_weights(f[u][0], wx);
+      cr_weights(f[u][1], wy);
+      cr_weights(f[u][2], wz);
+      const int bx = (int)f[u][3];
+      sb[e] = bx;
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const int ii = bx - 1 + t;
+        sx[4 * e + t] = (ii >= 0 && ii < nx) ? tap_weight(ii, bx, nx, wx) : (T)0;
+      }
+      const T zg = tap_weight(k, bz, nz, wz) * gv[u];
+#pragma unroll
+      for (int t = 0; t < kAdjTY; ++t) {
+        const int jj = j0 + t;
+        sy[kAdjTY * e + t] = (jj < ny && jj >= by - 1 && jj <= by + 2) ? tap_weight(jj, by, ny, wy) * zg : (T)0;
+      }
+    }
+  }
+  __syncthreads();
+  const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
+  if (i >= nx || j >= ny) return;
   const int x0 = max(i - 2, 0), x1 = min(i + 1, nx - 1);
   T acc = 0;
   for (int bz = max(k - 2, 0); bz <= min(k + 1, nz - 1); ++bz)
     for (int by = max(j - 2, 0); by <= min(j + 1, ny - 1); ++by) {
-      const uint32_t row = ((uint32_t)bz * ny + by) * nx;
-      const int32_t r0 = start[row + x0], r1 = start[row + x1 + 1];
-      for (int32_t r = r0; r < r1; ++r) {
-        const T *w = ws + (int64_t)r * 12;
-        const T wxi = tap_weight(i, bxs[r], nx, w);
-        const T wyj = tap_weight(j, by, ny, w + 4);
-        const T wzk = tap_weight(k, bz, nz, w + 8);
-        acc += wxi * (wyj * (wzk * gs[r]));
+      const int q = (bz - k + 2) * (kAdjTY + 3) + (by - j0 + 2);
+      const int ra = st[q][x0 - xa], re = st[q][x1 + 1 - xa];
+      if (staged) {
+        const int base = off[q] - st[q][0];
+        for (int r = ra; r < re; ++r) {
+          const int e = base + r;
+          acc += sx[4 * e + (i - sb[e] + 1)] * sy[kAdjTY * e + threadIdx.y];
+        }
+      } else {
+        for (int r = ra; r < re; ++r) {
+          T f[4];
+#pragma unroll
+          for (int c = 0; c < 4; ++c) f[c] = fr[4 * (int64_t)r + c];
+          acc += adj_term(f, gs[r], i, j, k, by, bz, d);
+        }
       }
     }
-  gf[q] = acc;
+  gf[((uint32_t)k * ny + j) * nx + i] = acc;
 }
 
 template <typename T>
-int tricubic_adj_weights(int fnx, int fny, int fnz, const T *co, const int32_t *perm, T *ws,
-                         int32_t *bxs, void *st) {
-  if (!co || !perm || !ws || !bxs) return SISL_ERR_ARG;
+int tricubic_adj_frac(int fnx, int fny, int fnz, const T *co, const int32_t *perm, T *fr, void *st) {
+  if (!co || !perm || !fr) return SISL_ERR_ARG;
   Dims d{fnx, fny, fnz};
-  if (d.n() >= (int64_t)1 << 31) return SISL_ERR_ARG;
-  hipLaunchKernelGGL(adj_weights_kernel<T>, dim3(grid_for(d.n())), dim3(kBlock), 0, as_stream(st), d, co,
-                     perm, ws, bxs);
+  if (d.n() >= (int64_t)1 << 29) return SISL_ERR_ARG;  // 4n entries with 32-bit indices
+  hipLaunchKernelGGL(adj_frac_kernel<T>, dim3(grid_for(d.n())), dim3(kBlock), 0, as_stream(st), d, co, perm, fr);
   SISL_CHECK_LAUNCH();
   return SISL_OK;
 }
 
 template <typename T>
-int tricubic_adj_rows(int fnx, int fny, int fnz, const int32_t *perm, const int32_t *start, const T *ws,
-                      const int32_t *bxs, const T *g, T *gs, T *gf, void *st) {
-  if (!perm || !start || !ws || !bxs || !g || !gs || !gf || g == gf) return SISL_ERR_ARG;
+int tricubic_adj_rows(int fnx, int fny, int fnz, const int32_t *perm, const int32_t *start, const T *fr,
+                      const T *g, T *gs, T *gf, void *st) {
+  if (!perm || !start || !fr || !g || !gs || !gf || g == gf) return SISL_ERR_ARG;
   Dims d{fnx, fny, fnz};
   const int64_t n = d.n();
-  if (n >= (int64_t)1 << 31) return SISL_ERR_ARG;
+  if (n >= (int64_t)1 << 29) return SISL_ERR_ARG;
   hipStream_t hs = as_stream(st);
-  // g in, gs out and in, gf out; plan: start, perm, 12 weights and base x per point
-  const double bytes = (double)n * (4.0 * sizeof(T) + 12.0 * sizeof(T) + 3.0 * 4.0);
+  // g in, gs out and in, gf out; plan: perm, start, 4 values per point
+  const double bytes = (double)n * (8.0 * sizeof(T) + 2.0 * 4.0);
   const int tid = timer_begin(SISL_T_TRICUBIC_ADJ, sizeof(T) == 4, bytes, nullptr, nullptr, hs);
   hipLaunchKernelGGL(adj_gather_kernel<T>, dim3(grid_for(n)), dim3(kBlock), 0, hs, n, perm, g, gs);
-  hipLaunchKernelGGL(tricubic_adj_rows_kernel<T>, dim3(grid_for(n)), dim3(kBlock), 0, hs, d, start, ws, bxs,
-                     (const T *)gs, gf);
+  const int nbx = (fnx + kAdjTX - 1) / kAdjTX, nby = (fny + kAdjTY - 1) / kAdjTY;
+  hipLaunchKernelGGL(tricubic_adj_tile_kernel<T>, dim3((unsigned)(nbx * nby * fnz)), dim3(kAdjTX, kAdjTY), 0, hs,
+                     d, start, fr, (const T *)gs, gf);
   timer_end(tid, hs);
   SISL_CHECK_LAUNCH();
   return SISL_OK;
@@ -831,23 +956,21 @@ int sisl_departure_f32(const SislGrid *g, int32_t loc, const float *u, const flo
   return sisl::departure<float>(g, loc, u, v, e, m2, dx, dy, dt, it, scr, out, s);
 }
 
-int sisl_tricubic_adj_weights_f64(int32_t a, int32_t b, int32_t c, const double *co, const int32_t *perm,
-                                  double *weights, int32_t *base_x, void *s) {
-  return sisl::tricubic_adj_weights<double>(a, b, c, co, perm, weights, base_x, s);
+int sisl_tricubic_adj_frac_f64(int32_t a, int32_t b, int32_t c, const double *co, const int32_t *perm,
+                               double *frac, void *s) {
+  return sisl::tricubic_adj_frac<double>(a, b, c, co, perm, frac, s);
 }
-int sisl_tricubic_adj_weights_f32(int32_t a, int32_t b, int32_t c, const float *co, const int32_t *perm,
-                                  float *weights, int32_t *base_x, void *s) {
-  return sisl::tricubic_adj_weights<float>(a, b, c, co, perm, weights, base_x, s);
+int sisl_tricubic_adj_frac_f32(int32_t a, int32_t b, int32_t c, const float *co, const int32_t *perm,
+                               float *frac, void *s) {
+  return sisl::tricubic_adj_frac<float>(a, b, c, co, perm, frac, s);
 }
 int sisl_tricubic_adj_rows_f64(int32_t a, int32_t b, int32_t c, const int32_t *perm, const int32_t *start,
-                               const double *weights, const int32_t *base_x, const double *g,
-                               double *g_sorted, double *gf, void *s) {
-  return sisl::tricubic_adj_rows<double>(a, b, c, perm, start, weights, base_x, g, g_sorted, gf, s);
+                               const double *frac, const double *g, double *g_sorted, double *gf, void *s) {
+  return sisl::tricubic_adj_rows<double>(a, b, c, perm, start, frac, g, g_sorted, gf, s);
 }
 int sisl_tricubic_adj_rows_f32(int32_t a, int32_t b, int32_t c, const int32_t *perm, const int32_t *start,
-                               const float *weights, const int32_t *base_x, const float *g,
-                               float *g_sorted, float *gf, void *s) {
-  return sisl::tricubic_adj_rows<float>(a, b, c, perm, start, weights, base_x, g, g_sorted, gf, s);
+                               const float *frac, const float *g, float *g_sorted, float *gf, void *s) {
+  return sisl::tricubic_adj_rows<float>(a, b, c, perm, start, frac, g, g_sorted, gf, s);
 }
 
 }  // extern "C"

@@ -179,22 +179,20 @@ class TricubicPlan:
                       self.start.data_ptr(), count.data_ptr(), sums.data_ptr(), _lib.stream_ptr(dev)),
                    "tricubic adjoint plan")
         self.dims = (fx, fy, fz)
-        self.weights = None
-        self.base_x = None
+        self.frac = None
 
     def ensure_rows(self, coords: torch.Tensor):
-        """Per sorted position: the 12 Catmull-Rom weights and the base x of
+        """Per sorted position: the fractions (tx, ty, tz) and the base x of
         the source point (for the sorted-row transpose)."""
-        if self.weights is not None:
+        if self.frac is not None:
             return
         fx, fy, fz = self.dims
         n = fx * fy * fz
-        self.weights = torch.empty(12 * n, dtype=coords.dtype, device=coords.device)
-        self.base_x = torch.empty(n, dtype=torch.int32, device=coords.device)
+        self.frac = torch.empty(4 * n, dtype=coords.dtype, device=coords.device)
         L = _lib.lib()
-        fn = L.sisl_tricubic_adj_weights_f64 if coords.dtype == torch.float64 else L.sisl_tricubic_adj_weights_f32
-        _lib.check(fn(fx, fy, fz, coords.data_ptr(), self.perm.data_ptr(), self.weights.data_ptr(),
-                      self.base_x.data_ptr(), _lib.stream_ptr(coords.device)), "tricubic adjoint weights")
+        fn = L.sisl_tricubic_adj_frac_f64 if coords.dtype == torch.float64 else L.sisl_tricubic_adj_frac_f32
+        _lib.check(fn(fx, fy, fz, coords.data_ptr(), self.perm.data_ptr(), self.frac.data_ptr(),
+                      _lib.stream_ptr(coords.device)), "tricubic adjoint plan fractions")
 
 
 class _Pending:
@@ -306,7 +304,7 @@ def tensor_product_interp_3d(field, coords, use_limiter=False):
 
 def tricubic_transpose(g, coords, method="rows"):
     """Explicit T^T g (deterministic), the adjoint of the unlimited gather.
-    ``method='rows'``: the sorted-row kernel (plan weights built once per
+    ``method='rows'``: the sorted-row kernel (plan fractions built once per
     coordinate set); ``'segments'``: the per-destination segment gather."""
     if not isinstance(coords, Coords):
         coords = Coords(coords)
@@ -321,8 +319,8 @@ def tricubic_transpose(g, coords, method="rows"):
         plan.ensure_rows(coords.data)
         gs = torch.empty_like(g)
         _lib.check(getattr(L, f"sisl_tricubic_adj_rows_{sfx}")(
-            fx, fy, fz, plan.perm.data_ptr(), plan.start.data_ptr(), plan.weights.data_ptr(),
-            plan.base_x.data_ptr(), g.data_ptr(), gs.data_ptr(), gf.data_ptr(), _lib.stream_ptr(g.device)),
+            fx, fy, fz, plan.perm.data_ptr(), plan.start.data_ptr(), plan.frac.data_ptr(),
+            g.data_ptr(), gs.data_ptr(), gf.data_ptr(), _lib.stream_ptr(g.device)),
             "tricubic adjoint")
     elif method == "segments":
         _lib.check(getattr(L, f"sisl_tricubic_adj_{sfx}")(

@@ -176,6 +176,30 @@ def test_tricubic_adjoint_methods_agree():
         assert rel_l2(from_int(a), from_int(b)) < tol, dt
 
 
+def test_tricubic_adjoint_rows_clustered_sources():
+    """Departure points crowded into a few cells (a tile's staged sources
+    overflow LDS and the row kernel reads them from global memory): the row
+    transpose still equals the segment transpose and passes the dot test."""
+    from suetes_amd.operators import Coords, tensor_product_interp_3d, tricubic_transpose
+    rng = np.random.default_rng(14)
+    shape = (70, 12, 9)
+    pts = np.stack([rng.uniform(20.0, 23.0, size=shape), rng.uniform(4.0, 6.0, size=shape),
+                    rng.uniform(3.0, 5.0, size=shape)])
+    # a destination next to the cluster sums thousands of terms: the two
+    # summation orders differ by ~sqrt(terms) * eps in FP32
+    for dt, tol in ((torch.float64, 1e-14), (torch.float32, 2e-5)):
+        co = Coords(offsets_int(pts).to(dt))
+        f = to_int(rng.standard_normal(shape), dt)
+        g = to_int(rng.standard_normal(shape), dt)
+        a = tricubic_transpose(g, co, method="rows")
+        b = tricubic_transpose(g, co, method="segments")
+        assert rel_l2(from_int(a), from_int(b)) < tol, dt
+        lhs = float((tensor_product_interp_3d(f, co) * g).double().sum())
+        rhs = float((f * a).double().sum())
+        assert abs(lhs - rhs) <= (1e-12 if dt == torch.float64 else 1e-4) * max(abs(lhs), 1.0), dt
+        assert torch.equal(a, tricubic_transpose(g, Coords(co.data.clone()), method="rows"))
+
+
 @pytest.mark.parametrize("cfgf", [small_lam, small_bubble])
 def test_departure_points_match_oracle(cfgf):
     cfg = cfgf()
