@@ -507,6 +507,33 @@ __global__ void unit_kernel(const TS *__restrict__ src, TB *__restrict__ out, TC
   }
 }
 
+// ||w||^2 (FP64, fixed-order partials into slot 0) and the FP32 copy of w:
+// the unnormalised start vector of an inner cycle, which the cycle's first
+// operator launch normalises (one read of w instead of a norm pass and a
+// separate normalisation pass).
+__global__ void __launch_bounds__(256) norm2_to_f32_kernel(const double *__restrict__ w,
+                                                           float *__restrict__ out, int64_t n,
+                                                           double *__restrict__ partials) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  double acc = 0;
+  const int64_t nq = n / 2;
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += stride) {
+    double e[2];
+    vload<2>(w, q, e);
+    acc += e[0] * e[0] + e[1] * e[1];
+    float2 f;
+    f.x = (float)e[0];
+    f.y = (float)e[1];
+    reinterpret_cast<float2 *>(out)[q] = f;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0 && (n & 1)) {
+    const double v = w[n - 1];
+    acc += v * v;
+    out[n - 1] = (float)v;
+  }
+  block_store_slot(acc, 0, partials, blockIdx.x);
+}
+
 // x += sum_v y[v] vinv V[v]   (x is FP64)
 template <typename TV>
 __device__ __forceinline__ void combine_body(const TV *__restrict__ V, int64_t ldv, int nv,
@@ -641,7 +668,7 @@ __global__ void reset_cycle_kernel(Small *sm, int R, double g0, double target, i
     sm->broken = 0;
     sm->steps = 0;
     sm->gate2 = 0;
-    sm->inv = 0.0;
+    sm->inv = g0 > 0.0 ? 1.0 / g0 : 0.0;  // normalises the cycle's start vector (fused first step)
     for (int v = 0; v <= R; ++v) sm->g[v] = 0.0;
     sm->g[0] = g0;
     sm->target = target;
@@ -784,14 +811,17 @@ struct Krylov {
     constexpr bool fused = sizeof(TV) == 4;
     TV *wj = w;
     if constexpr (fused) {
+      // step 0 reads the cycle's unnormalised start vector (cur), later
+      // steps the previous unnormalised Arnoldi vector; every launch scales
+      // by 1 / norm and stores basis column j
       wj = (j % 2 == 0) ? w : cur;
-      const TV *src = j == 0 ? basis_in(0) : (j % 2 == 0 ? cur : w);
+      const TV *src = j % 2 == 0 ? cur : w;
       Fuse f;
       f.inv = &sm->inv;
       f.vout = V + (int64_t)j * ld;
       f.nb = sizeof(TB) == 2 ? 1 : 2;
       f.vs = vs;
-      if ((rc = apply(src, wj, stop, j > 0 ? &f : nullptr))) return rc;
+      if ((rc = apply(src, wj, stop, &f))) return rc;
     } else {
       if ((rc = apply(basis_in(j), w, stop))) return rc;
     }
@@ -1090,9 +1120,10 @@ static int gmres_mixed(const OpT<double> &op64, const OpT<float> &op32, const Si
       if ((rc = launch_AT<double>(op64, x, t64, scr64, s, nullptr, &rb))) return rc;
     }
     if ((rc = M64(t64, w64))) return rc;
-    norm2<double>(w64, n, partials, &sm->scal[2], s);
-    hipLaunchKernelGGL((unit_kernel<double, TB, float>), dim3(kRedGrid), dim3(kBlock), 0, s,
-                       (const double *)w64, K.V, K.cur, n, K.vscale(), (const double *)&sm->scal[2]);
+    // ||z||^2 and the FP32 start vector of the next cycle in one pass over z
+    hipLaunchKernelGGL(norm2_to_f32_kernel, dim3(kRedGrid), dim3(kBlock), 0, s, (const double *)w64, K.cur, n,
+                       partials);
+    reduce(partials, 1, &sm->scal[2], s);
     if ((rc = sync_copy(hs, sm->scal, sizeof(double) * 3, s))) return rc;
     rnorm = std::sqrt(hs[2]);
     if (!(rnorm > 2.220446049250313e-16) && !std::isnan(rnorm)) rnorm = 0.0;
