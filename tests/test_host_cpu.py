@@ -202,3 +202,33 @@ def test_solve_adjoint_reaches_terrain_through_the_operator():
         with torch.no_grad():
             fd = (float(J(a0.detach() + e)) - float(J(a0.detach() - e))) / (2 * h)
         assert abs(fd - float(grad[q])) <= 1e-6 * abs(fd), (q, fd, float(grad[q]))
+
+
+def test_tricubic_plan_holder_releases_after_last_pending_adjoint():
+    """The adjoint plan of a coordinate set stays while any gather's adjoint
+    is pending and goes with the last one; tokens of gathers whose autograd
+    nodes die without a backward (a checkpoint's recomputation) vanish from
+    the pending set by themselves."""
+    import gc
+
+    from suetes_amd.operators import _PlanShare
+    sh = _PlanShare()
+    toks = [sh.register() for _ in range(3)]
+    recomputed = sh.register()
+    sh.plan = object()  # built at the first adjoint
+    sh.done(toks[0])
+    assert sh.plan is not None and len(sh.pending) == 3
+    del recomputed  # its node died without a backward
+    gc.collect()
+    assert len(sh.pending) == 2
+    sh.done(toks[1])
+    assert sh.plan is not None
+    sh.done(toks[2])
+    assert sh.plan is None and len(sh.pending) == 0
+
+
+def test_mixed_solver_options_are_validated():
+    from suetes_amd.steppers import SemiImplicitSolver3D
+    for kw in ({"inner_gram_schmidt": "twice"}, {"inner_coefficients": "fp16"}, {"inner_basis": "bf16"}):
+        with pytest.raises(ValueError):
+            SemiImplicitSolver3D(None, 1.0, precision="mixed", **kw)
