@@ -232,3 +232,14 @@ def test_mixed_solver_options_are_validated():
     for kw in ({"inner_gram_schmidt": "twice"}, {"inner_coefficients": "fp16"}, {"inner_basis": "bf16"}):
         with pytest.raises(ValueError):
             SemiImplicitSolver3D(None, 1.0, precision="mixed", **kw)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """One GPU per box: --gpus above 1 exits with a message, before touching a device."""
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode != 0
+    assert "one GPU" in (r.stdout + r.stderr)
