@@ -1097,6 +1097,51 @@ class XRelaxation:
         return out
 
 
+class BenchmarkSponge:
+    """Symmetric cos^2 relaxation along x and/or y (boundaries.py:123-189):
+    per-axis float32 weights cos^2(pi/2 * d / depth) for distance d from the
+    nearer edge below ``depth`` (0 beyond), combined as 1 - (1-wx)(1-wy);
+    u / v masks repeat the last mass column / row."""
+
+    def __init__(self, nx, ny=None, sponge_depth=10, axes=('x',), blend_vars=None):
+        self.nx, self.ny, self.depth, self.axes = nx, ny, sponge_depth, axes
+        if blend_vars is None:
+            blend_vars = (['u', 'v', 'th_v', 'pi', 'rho', 'q_tr'] if ny is None
+                          else ['u', 'v', 'w', 'th_v', 'pi', 'rho', 'q_tr'])
+        self.blend_vars = blend_vars
+
+        def axis_weight(n):
+            # boundaries.py:145-158 (float32 index, distance and cosine)
+            idx = np.arange(n, dtype=np.float32)
+            dist = np.minimum(idx, np.float32(n - 1) - idx)
+            arg = np.float32(0.5 * np.pi) * dist / np.float32(sponge_depth)
+            return np.where(dist < sponge_depth, np.cos(arg) ** 2, np.float32(0.0)).astype(np.float32)
+
+        wx = axis_weight(nx) if 'x' in axes else np.zeros(nx, np.float32)
+        if ny is not None:
+            wy = axis_weight(ny) if 'y' in axes else np.zeros(ny, np.float32)
+            mm = (1.0 - (1.0 - wx[:, None]) * (1.0 - wy[None, :])).astype(np.float32)[:, :, None]
+            self.mask_m = mm
+            self.mask_u = np.pad(mm, ((0, 1), (0, 0), (0, 0)), mode='edge')
+            self.mask_v = np.pad(mm, ((0, 0), (0, 1), (0, 0)), mode='edge')
+        else:
+            self.mask_m = wx[:, None, None]
+            self.mask_u = np.pad(self.mask_m, ((0, 1), (0, 0), (0, 0)), mode='edge')
+            self.mask_v = self.mask_m
+        self.mask_w = self.mask_m
+
+    def blend(self, st, ext):
+        # boundaries.py:174-189
+        out = {}
+        for k, v in st.items():
+            if k in self.blend_vars and k in ext:
+                m = {'u': self.mask_u, 'v': self.mask_v, 'w': self.mask_w}.get(k, self.mask_m)
+                out[k] = (1.0 - m) * v + m * ext[k]
+            else:
+                out[k] = v
+        return out
+
+
 def davies_masks(nx, ny, depth, loc):
     """boundaries.py:36-61 (float32 weights, as in the reference)."""
     nxp = nx + 1 if loc == 'u' else nx

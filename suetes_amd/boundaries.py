@@ -36,6 +36,10 @@ def davies_masks(nx, ny, depth, loc):
 
 
 class DaviesSponge:
+    # blend() computes no field from rho (the stepper may skip transporting
+    # rho when the equation of state overwrites it)
+    rho_independent = True
+
     def __init__(self, grid, operators, sponge_depth=30, dt=30.0, tau_bndy_factor=10.0,
                  outflow_factor=0.01):
         self.grid, self.op, self.depth = grid, operators, sponge_depth
@@ -105,6 +109,7 @@ class XRelaxation:
     the external state, for u, v, th_v, pi, rho: the boundary function of
     the reference's mountain-wave benchmark
     (benchmarks/physical/schaer_mountain_3d/run.py:103-126)."""
+    rho_independent = True  # rho is blended on its own; no other field reads it
 
     def __init__(self, grid, cells=8):
         nx = grid.nx
@@ -138,6 +143,7 @@ class XRelaxation:
 
 class BenchmarkSponge:
     """Symmetric relaxation (boundaries.py:123-189)."""
+    rho_independent = True  # rho is blended on its own; no other field reads it
 
     def __init__(self, nx, ny=None, sponge_depth=10, axes=("x",), blend_vars=None, device=None):
         self.nx, self.ny, self.sponge_depth, self.axes = nx, ny, sponge_depth, axes

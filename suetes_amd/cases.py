@@ -29,6 +29,7 @@ def identity_bc(state, forcing):
 
 
 identity_bc.suetes_internal = True
+identity_bc.suetes_rho_independent = True
 
 
 @dataclass
@@ -56,7 +57,11 @@ def to_device_state(st_np, device):
     return out
 
 
-def build_case(cfg, device="cuda", precision="fp64", **core_overrides):
+def build_case(cfg, device="cuda", precision="fp64", physics_suite=None, extra_fields=None,
+               **core_overrides):
+    """Grid, stepper, initial state and boundary function of workload
+    ``cfg`` on ``device``.  ``extra_fields``: reference-shaped NumPy arrays
+    added to the initial state (e.g. tracers of ``physics_suite``)."""
     if isinstance(cfg, str):
         cfg = WORKLOADS[cfg]()
     gk = cfg["grid"]
@@ -66,12 +71,14 @@ def build_case(cfg, device="cuda", precision="fp64", **core_overrides):
                             device=device)
     op = CGridOperator3D(grid, cfg["periodic_axes"])
     st_np = initial_state(cfg, grid.Z_m, grid.x_m, grid.y_m)
+    st_np.update(extra_fields or {})
     state = to_device_state(st_np, device)
     forcing = dict(state)
     core = dict(cfg["core"])
     core["precision"] = precision
     core.update(core_overrides)
-    stepper, dt = build_dynamical_core("sisl", grid, op, CONSTANTS, state, **core)
+    stepper, dt = build_dynamical_core("sisl", grid, op, CONSTANTS, state, physics_suite=physics_suite,
+                                       **core)
     bc = identity_bc
     if cfg["boundary"] is not None and cfg["boundary"][0] == "davies":
         sponge = DaviesSponge(grid, op, **cfg["boundary"][1])

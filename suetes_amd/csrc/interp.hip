@@ -25,10 +25,17 @@ __device__ __forceinline__ void cr_weights(T t, T w[4]) {
 // to 256 would keep only ~1.5e-5 cells in FP32).  Split X + d into base
 // index and fraction, clamped to [0, n-1] like the reference's coordinates
 // (operators.py:169-174).
+// The floor is bounded to [-(n+1), n+1] before the integer conversion, so a
+// far or non-finite offset clamps to the nearer edge instead of overflowing.
+template <typename T>
+__device__ __forceinline__ T bounded_floor(T fl, int n) {
+  return fmax(fmin(fl, (T)(n + 1)), (T)(-(n + 1)));
+}
+
 template <typename T>
 __device__ __forceinline__ int offset_split(int X, T d, int n, T &t) {
   const T fl = floor(d);
-  int b = X + (int)fl;
+  int b = X + (int)bounded_floor(fl, n);
   t = d - fl;
   if (b < 0) {
     b = 0;
@@ -444,7 +451,7 @@ __global__ void index_velocity_kernel(Stag s, int loc, const T *__restrict__ u,
 template <typename T>
 __device__ __forceinline__ void lin_split(int X, T d, int n, int &i0, int &i1, T &w1) {
   const T fl = floor(d);
-  const int i = X + (int)fl;
+  const int i = X + (int)bounded_floor(fl, n);
   w1 = d - fl;
   i0 = min(max(i, 0), n - 1);
   i1 = min(max(i + 1, 0), n - 1);

@@ -24,6 +24,11 @@
 
 namespace sisl {
 
+// min(1, r) with NaN kept, as jnp.minimum / np.minimum do (std::fmin and
+// fmin return the non-NaN operand): with b = 0 and atol = 0 the stopping
+// threshold is NaN and no cycle runs, so x0 is returned unchanged.
+__host__ __device__ inline double nan_min1(double r) { return r != r ? r : (r < 1.0 ? r : 1.0); }
+
 constexpr int kRedGrid = 1024;  // blocks of the reduction passes
 constexpr int kMaxRestart = 64;
 constexpr int kPStride = kMaxRestart + 2;  // partial sums per block
@@ -916,9 +921,9 @@ static int gmres64(const OpT<double> &op, const SislGmresParams &p, const double
   const double b_norm = std::sqrt(hs[0]);
   const double Mb_norm = std::sqrt(hs[1]);
   const double atol = std::fmax(p.tol * b_norm, p.atol);
-  const double ptol = Mb_norm * std::fmin(1.0, atol / b_norm);  // NaN when b = 0: no cycle
+  const double ptol = Mb_norm * nan_min1(atol / b_norm);  // NaN when b = 0 and atol = 0: no cycle
   double rnorm = std::sqrt(hs[2]);
-  if (!(rnorm > 2.220446049250313e-16)) rnorm = 0.0;
+  if (!(rnorm > 2.220446049250313e-16) && !std::isnan(rnorm)) rnorm = 0.0;
   hipLaunchKernelGGL(zero_steps_total_kernel, dim3(1), dim3(1), 0, s, sm);
   int k = 0;
   while (k < p.maxiter && rnorm > ptol) {
@@ -943,7 +948,7 @@ static int gmres64(const OpT<double> &op, const SislGmresParams &p, const double
     st->ptol = ptol;
     st->b_norm = b_norm;
   }
-  return SISL_OK;
+  return std::isfinite(rnorm) ? SISL_OK : SISL_ERR_NOT_FINITE;
 }
 
 // ---- hipGraph of the inner Arnoldi cycle ----------------------------------------
@@ -1105,7 +1110,7 @@ static int gmres_mixed(const OpT<double> &op64, const OpT<float> &op32, const Si
   const double b_norm = std::sqrt(hs[0]);
   const double Mb_norm = std::sqrt(hs[1]);
   const double atol = std::fmax(p.tol * b_norm, p.atol);
-  const double ptol = Mb_norm * std::fmin(1.0, atol / b_norm);
+  const double ptol = Mb_norm * nan_min1(atol / b_norm);
   const int max_outer = p.max_outer > 0 ? p.max_outer : p.maxiter;
   Fuse rb;  // the FP64 operator launch stores b - A x
   rb.res = b;
@@ -1144,7 +1149,7 @@ static int gmres_mixed(const OpT<double> &op64, const OpT<float> &op32, const Si
     st->ptol = ptol;
     st->b_norm = b_norm;
   }
-  return SISL_OK;
+  return std::isfinite(rnorm) ? SISL_OK : SISL_ERR_NOT_FINITE;
 }
 
 // ---- batched (ensemble) mixed GMRES -------------------------------------------
@@ -1277,7 +1282,7 @@ __global__ void kb_init_kernel(const KMem *__restrict__ km, double tol, double a
   const double at = fmax(tol * b_norm, atol);
   sm->b_norm = b_norm;
   sm->Mb_norm = Mb_norm;
-  sm->ptol = Mb_norm * fmin(1.0, at / b_norm);
+  sm->ptol = Mb_norm * nan_min1(at / b_norm);
   sm->outer = 0;
   sm->steps_total = 0;
   sm->done = 0;
@@ -1575,13 +1580,16 @@ static int ens_solve(const void *ws, const SislGmresParams &p, const double *con
   }
   hipLaunchKernelGGL(kb_copy_kernel, cg, dim3(kBlock), 0, st, tout);
   SISL_CHECK_LAUNCH();
-  if (stats) {
-    for (int m = 0; m < M; ++m) {
-      Small hs;
-      if (hipMemcpyAsync(&hs.done, &P.sm[m]->done, sizeof(int) * 4 + sizeof(double) * 4,
-                         hipMemcpyDeviceToHost, st) != hipSuccess ||
-          hipStreamSynchronize(st) != hipSuccess)
-        return SISL_ERR_LAUNCH;
+  // per-member outcome (one small copy per member after the solve)
+  bool finite = true;
+  for (int m = 0; m < M; ++m) {
+    Small hs;
+    if (hipMemcpyAsync(&hs.done, &P.sm[m]->done, sizeof(int) * 4 + sizeof(double) * 4,
+                       hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+      return SISL_ERR_LAUNCH;
+    finite = finite && std::isfinite(hs.rnorm);
+    if (stats) {
       stats[m].cycles = hs.outer;
       stats[m].arnoldi_steps = hs.steps_total;
       stats[m].residual = hs.rnorm;
@@ -1589,7 +1597,7 @@ static int ens_solve(const void *ws, const SislGmresParams &p, const double *con
       stats[m].b_norm = hs.b_norm;
     }
   }
-  return SISL_OK;
+  return finite ? SISL_OK : SISL_ERR_NOT_FINITE;
 }
 
 }  // namespace sisl

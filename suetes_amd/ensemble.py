@@ -184,14 +184,16 @@ class _BatchedSolve(torch.autograd.Function):
     def forward(ctx, ens, *bx):
         M = len(bx) // 2
         ctx.ens = ens
+        # members have uneven grids: a missing cotangent becomes zeros of its
+        # own member's packed length
+        ctx.like = [(b.numel(), b.dtype, b.device) for b in bx[:M]]
         return tuple(ens.solve_flat(list(bx[:M]), list(bx[M:]), transpose=False))
 
     @staticmethod
     def backward(ctx, *gx):
         ens = ctx.ens
-        gx = [g.contiguous() if g is not None else None for g in gx]
-        ref = [g for g in gx if g is not None][0]
-        gx = [g if g is not None else torch.zeros_like(ref) for g in gx]
+        gx = [g.contiguous() if g is not None else torch.zeros(n, dtype=dt, device=dev)
+              for g, (n, dt, dev) in zip(gx, ctx.like)]
         lam = ens.solve_flat(gx, [torch.zeros_like(g) for g in gx], transpose=True)
         return (None,) + tuple(lam) + (None,) * len(gx)
 

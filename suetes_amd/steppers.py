@@ -700,6 +700,19 @@ class SISLStepper3D:
         out = bc_fn(state_to_reference(st), forcing)
         return state_to_internal(out)
 
+    def _rho_transport_observed(self, bc_fn, state):
+        """Whether anything reads the FFSL-transported density of a step: a
+        tracer (weighted by it), a physics suite's state update, or a
+        boundary function not known to treat rho independently of the other
+        fields."""
+        if any(k in state for k in self.tracer_keys) or self.physics.physics_suite is not None:
+            return True
+        if getattr(bc_fn, "suetes_rho_independent", False):
+            return False
+        owner = getattr(bc_fn, "__self__", None)
+        return not (owner is not None and getattr(bc_fn, "__name__", "") == "blend"
+                    and getattr(owner, "rho_independent", False))
+
     def _explicit(self, sp, bg, ml_params):
         ph = self.physics
         if ph.physics_suite is None:
@@ -796,7 +809,13 @@ class SISLStepper3D:
         rhs_pi = cub(pi_in, co["m"], lim)
         th_adv = cub(th_in, co["m"], lim)
         th_next = th_adv + a * dt * tn["th_v"] + th_bg
-        rho_next = self.ffsl_advector.advect_3d_split(state["rho"], state, bg)
+        if self._rho_transport_observed(bc_fn, state):
+            rho_next = self.ffsl_advector.advect_3d_split(state["rho"], state, bg)
+        else:
+            # nothing reads the transported density: the equation of state
+            # overwrites rho below (steppers.py:1056), there are no tracers to
+            # weight and the boundary function blends rho on its own
+            rho_next = state["rho"]
         tracers_next = {}
         for key in self.tracer_keys:
             if key in state:

@@ -243,3 +243,70 @@ def test_bench_refuses_more_than_one_gpu():
                        text=True, timeout=120)
     assert r.returncode != 0
     assert "one GPU" in (r.stdout + r.stderr)
+
+
+@pytest.mark.parametrize("ny,axes", [(None, ("x",)), (10, ("x", "y")), (10, ("y",))])
+def test_benchmark_sponge_matches_oracle(ny, axes):
+    """boundaries.py:123-189: masks and blend of the product against the
+    oracle (float32 weights, bit for bit)."""
+    nx, nz, depth = 14, 5, 4
+    so = O.BenchmarkSponge(nx, ny, sponge_depth=depth, axes=axes)
+    sp = S.BenchmarkSponge(nx, ny, sponge_depth=depth, axes=axes)
+    nyy = ny if ny is not None else 3
+    rng = np.random.default_rng(7)
+    shapes = {"u": (nx + 1, nyy, nz), "v": (nx, nyy + 1, nz), "w": (nx, nyy, nz + 1), "th_v": (nx, nyy, nz),
+              "pi": (nx, nyy, nz), "rho": (nx, nyy, nz), "q_tr": (nx, nyy, nz), "eta_dot": (nx, nyy, nz + 1)}
+    if ny is not None:  # w is blended with the mass mask: same shape as the mass points in the reference
+        shapes["w"] = (nx, nyy, nz)
+    state = {k: rng.standard_normal(s) for k, s in shapes.items()}
+    ext = {k: rng.standard_normal(s) for k, s in shapes.items()}
+    bo = so.blend(state, ext)
+    bp = sp.blend({k: torch.from_numpy(v.copy()) for k, v in state.items()},
+                  {k: torch.from_numpy(v.copy()) for k, v in ext.items()})
+    assert set(bo) == set(bp)
+    for k in bo:
+        np.testing.assert_allclose(tnp(bp[k]), bo[k], rtol=1e-15, atol=1e-15, err_msg=k)
+
+
+def test_dlpack_state_round_trip_is_zero_copy_and_exact():
+    """The drop-in swaps state with the caller through DLPack
+    (torch.from_dlpack / np.from_dlpack): a NumPy state dict enters as
+    tensors that share its memory, the internal layout and back is exact, and
+    the results leave as NumPy arrays over the tensors' memory."""
+    from suetes_amd.steppers import state_to_internal, state_to_reference
+    rng = np.random.default_rng(11)
+    nx, ny, nz = 6, 5, 4
+    st = {"u": rng.standard_normal((nx + 1, ny, nz)), "v": rng.standard_normal((nx, ny + 1, nz)),
+          "w": rng.standard_normal((nx, ny, nz + 1)), "pi": rng.standard_normal((nx, ny, nz)),
+          "th_v": rng.standard_normal((nx, ny, nz)), "rho": rng.standard_normal((nx, ny, nz)),
+          "eta_dot": rng.standard_normal((nx, ny, nz + 1))}
+    tens = {k: torch.from_dlpack(v) for k, v in st.items()}
+    for k, v in st.items():
+        assert tens[k].data_ptr() == v.__array_interface__["data"][0]   # shared memory
+    back = state_to_reference(state_to_internal(tens))
+    for k, v in st.items():
+        out = np.from_dlpack(back[k].contiguous())
+        np.testing.assert_array_equal(out, v)
+    # a write through the NumPy view is seen by the tensor (no copy on entry)
+    st["pi"][0, 0, 0] = 123.0
+    assert float(tens["pi"][0, 0, 0]) == 123.0
+
+
+def test_rho_transport_skipped_only_when_unobserved():
+    """The stepper transports rho when a tracer, a physics suite or an
+    unknown boundary function can observe it."""
+    from suetes_amd.steppers import SISLStepper3D
+    from suetes_amd.cases import identity_bc
+
+    class Ph:
+        physics_suite = None
+
+    st = SISLStepper3D.__new__(SISLStepper3D)
+    st.physics, st.tracer_keys = Ph(), ["q"]
+    sponge = S.BenchmarkSponge(8, 6)
+    assert not st._rho_transport_observed(identity_bc, {"rho": 1})
+    assert not st._rho_transport_observed(sponge.blend, {"rho": 1})
+    assert st._rho_transport_observed(lambda s, f: s, {"rho": 1})
+    assert st._rho_transport_observed(identity_bc, {"rho": 1, "q": 2})
+    Ph.physics_suite = object()
+    assert st._rho_transport_observed(identity_bc, {"rho": 1})
