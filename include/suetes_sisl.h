@@ -30,7 +30,9 @@ enum SislStatus {
   SISL_OK = 0,
   SISL_ERR_ARG = -1,      /* null pointer / bad size / bad option          */
   SISL_ERR_LAUNCH = -2,   /* a HIP launch or runtime call failed           */
-  SISL_ERR_NOT_FINITE = -3/* GMRES produced a non-finite residual          */
+  SISL_ERR_NOT_FINITE = -3,/* GMRES produced a non-finite residual         */
+  SISL_ERR_TOO_LARGE = -4 /* input exceeds the fixed scratch of an
+                             allocation-free entry point                    */
 };
 
 /* Grid descriptor shared by every entry point. */
@@ -273,6 +275,17 @@ int sisl_ffsl_1d_T_f64(const double *mass, const double *cfl, const double *gout
 int sisl_ffsl_1d_T_f32(const float *mass, const float *cfl, const float *gout, float *gmass, float *gcfl,
                        float *scratch, int64_t outer, int32_t n, int64_t inner, int32_t periodic,
                        void *stream);
+
+/* Allocation-free variant of sisl_ffsl_1d_f64.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The cumulative-mass scratch is a fixed device array inside
+ * the library (64 MiB per device, held until the process ends):
+ * outer * (n + 1) * inner must not exceed 8,388,608 elements, else
+ * SISL_ERR_TOO_LARGE.  Calls that overlap on one device (e.g. on two
+ * streams) share that scratch and race; serialise them. */
+int sisl_ffsl_1d_f64_noalloc(const double *mass, const double *cfl, double *out, int64_t outer, int32_t n,
+                             int64_t inner, int32_t periodic, void *stream);
 
 /* ---- launch graphs ------------------------------------------------------ */
 

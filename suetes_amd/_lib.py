@@ -12,7 +12,8 @@ import torch
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libsuetes_sisl.so")
 
-ERRORS = {-1: "bad argument", -2: "HIP launch failed", -3: "non-finite residual"}
+ERRORS = {-1: "bad argument", -2: "HIP launch failed", -3: "non-finite residual",
+          -4: "input exceeds the fixed scratch of an allocation-free entry point"}
 
 EXPORTED = [
     "sisl_krylov_size", "sisl_tricubic_f64", "sisl_tricubic_f32", "sisl_scan_blocks",
@@ -29,7 +30,7 @@ EXPORTED = [
     "sisl_stagger_T_f64", "sisl_stagger_T_f32", "sisl_tricubic_adj_frac_f64",
     "sisl_tricubic_adj_frac_f32", "sisl_tricubic_adj_rows_f64", "sisl_tricubic_adj_rows_f32",
     "sisl_ffsl_1d_f64", "sisl_ffsl_1d_f32", "sisl_ffsl_1d_T_f64", "sisl_ffsl_1d_T_f32",
-    "sisl_set_graphs",
+    "sisl_set_graphs", "sisl_ffsl_1d_f64_noalloc",
 ]
 
 # timed kernels (enum SislTimedKernel)
@@ -120,6 +121,7 @@ def lib():
         for p in ("f64", "f32"):
             getattr(L, f"sisl_ffsl_1d_{p}").argtypes = [vp, vp, vp, vp, i64, i32, i64, i32, vp]
             getattr(L, f"sisl_ffsl_1d_T_{p}").argtypes = [vp, vp, vp, vp, vp, vp, i64, i32, i64, i32, vp]
+        L.sisl_ffsl_1d_f64_noalloc.argtypes = [vp, vp, vp, i64, i32, i64, i32, vp]
         L.sisl_set_graphs.argtypes = [i32]
         L.sisl_timer_enable.argtypes = [i32]
         L.sisl_timer_read.argtypes = [i32, i32, C.POINTER(d), C.POINTER(i64), C.POINTER(i64),

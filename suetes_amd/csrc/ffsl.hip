@@ -18,6 +18,8 @@
 // for the mass cotangent, and forms the Courant-number cotangent
 // -(M_b - M_a) times the derivative of the clamp, with ties at the clamp
 // bounds split in half like the tensor operators' maximum / minimum.
+#include <mutex>
+
 #include "common.h"
 
 namespace sisl {
@@ -160,9 +162,52 @@ static int ffsl_launch(bool transpose, const T *mass, const T *cfl, const T *gou
   return SISL_OK;
 }
 
+// ---- allocation-free sweep ----------------------------------------------------
+// Fixed scratch of sisl_ffsl_1d_f64_noalloc: the cumulative-mass lines of
+// one FP64 sweep, outer * (n + 1) * inner <= kFfslScratchMax elements
+// (64 MiB, enough for a 256 x 256 x 80 field along any axis).  Every device
+// the library runs on holds its own copy until the process ends.  In host
+// code `scratch` names a host-side stand-in, never its device memory: the
+// device address comes from hipGetSymbolAddress, once per device.
+constexpr int64_t kFfslScratchMax = int64_t(1) << 23;
+static __device__ __align__(256) double scratch[kFfslScratchMax];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double *g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device (looked up on the first
+// call on each device, with that device current).
+static int scratch_on_current_device(double **out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return SISL_ERR_LAUNCH;
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return SISL_ERR_LAUNCH;
+    g_scratch_ptr[dev] = static_cast<double *>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return SISL_OK;
+}
+
 }  // namespace sisl
 
 extern "C" {
+
+int sisl_ffsl_1d_f64_noalloc(const double *mass, const double *cfl, double *out, int64_t outer, int32_t n,
+                             int64_t inner, int32_t periodic, void *stream) {
+  if (!mass || !cfl || !out || n < 1 || outer < 1 || inner < 1) return SISL_ERR_ARG;
+  if (outer > sisl::kFfslScratchMax || inner > sisl::kFfslScratchMax ||
+      outer * (int64_t)(n + 1) * inner > sisl::kFfslScratchMax)
+    return SISL_ERR_TOO_LARGE;
+  double *ms = nullptr;
+  if (const int rc = sisl::scratch_on_current_device(&ms)) return rc;
+  // every cumulative-mass element is written by its line's thread before it
+  // is read, so the scratch needs no clearing between calls
+  return sisl::ffsl_launch<double>(false, mass, cfl, nullptr, out, nullptr, ms, outer, n, inner, periodic,
+                                   stream);
+}
 
 int sisl_ffsl_1d_f64(const double *mass, const double *cfl, double *out, double *scratch, int64_t outer,
                      int32_t n, int64_t inner, int32_t periodic, void *stream) {
