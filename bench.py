@@ -197,34 +197,66 @@ def adjoint_cost(case, nsteps=1):
     return (tvg - tf) / tf, tf, tvg
 
 
-def cpu_baseline(seconds=20.0):
-    """The FP64 NumPy oracle, single thread, timing whole steps of the same
-    workload on a 48x48x80 sub-grid and scaling by grid points to 256x256x80."""
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(repeats=5, sub=48):
+    """The FP64 NumPy oracle of the same workload on a sub x sub x 80 sub-grid
+    (same generator, seed and settings), one step from the initial state
+    timed `repeats` times after an untimed warm-up, median taken
+    (cpu_gpu_short_scaling/run.py:135-167 protocol); scaled by grid points
+    to 256x256x80.  The BLAS / OpenMP pools get every host core (the
+    Gram-Schmidt projections and updates of the oracle's GMRES are BLAS
+    calls); NumPy's elementwise stencil work runs on one thread.  The
+    sub-grid's Arnoldi steps per time step are reported beside the GPU's at
+    full size, since the per-step solver work scales with them."""
+    import statistics
     try:
         from threadpoolctl import threadpool_limits
     except Exception:  # pragma: no cover
         threadpool_limits = None
     from suetes_amd import workloads as W
     from tests.helpers import oracle_case
-    cfg = W.lam(nx=48, ny=48, nz=80)
-    g, op, st, s, bc, ext = oracle_case(cfg)
-    ctx = threadpool_limits(limits=1) if threadpool_limits else None
-    t0 = time.perf_counter()
-    n = 0
+    # every host core this process may use: the CPU share the environment
+    # grants (OMP_NUM_THREADS) when it names one, else all cores
+    cores = int(os.environ.get("OMP_NUM_THREADS", 0) or 0) or os.cpu_count() or 1
+    cfg = W.lam(nx=sub, ny=sub, nz=80)
+    g, op, st, s0, bc, ext = oracle_case(cfg)
+    ctx = threadpool_limits(limits=cores) if threadpool_limits else None
+    times, arnoldi = [], []
     try:
-        while True:
-            s = st.step(s, n * 10.0, ext, bc)
-            n += 1
-            if time.perf_counter() - t0 > seconds or n >= 3:
-                break
+        for rep in range(repeats + 1):
+            s = {k: (v.copy() if hasattr(v, "copy") else v) for k, v in s0.items()}
+            st.solver.stats = {}
+            t0 = time.perf_counter()
+            st.step(s, 0.0, ext, bc)
+            dt = time.perf_counter() - t0
+            if rep:  # the first step is the warm-up
+                times.append(dt)
+                arnoldi.append(st.solver.stats.get("cycles", 0) * st.solver.solver_restart)
     finally:
         if ctx is not None:
             ctx.__exit__(None, None, None)
-    dt = (time.perf_counter() - t0) / n
-    scale = (48 * 48 * 80) / (256 * 256 * 80)
-    return {"value": scale / dt, "unit": "steps/s", "cores": 1, "kind": "port",
-            "sample": f"{n} FP64 oracle steps of the same workload on a 48x48x80 sub-grid "
-                      f"({dt:.2f} s/step), scaled by grid points to 256x256x80"}
+    med = statistics.median(times)
+    log(f"cpu baseline: {med:.2f} s per {sub}x{sub}x80 step on {cores} threads")
+    scale = (sub * sub * 80) / (256 * 256 * 80)
+    return {"value": scale / med, "unit": "steps/s", "cores": cores, "kind": "port",
+            "cpu_model": _cpu_model(), "os_cpu_count": os.cpu_count(),
+            "median_s_per_substep": med, "repeats_s": times,
+            "oracle_arnoldi_steps_per_step": arnoldi[-1] if arnoldi else None,
+            "sample": f"median of {repeats} FP64 oracle steps (after one warm-up) of the same workload on a "
+                      f"{sub}x{sub}x80 sub-grid ({med:.2f} s/step), scaled by grid points to 256x256x80; "
+                      f"BLAS pool {cores} threads, elementwise NumPy on one; the sub-grid solve took "
+                      f"{arnoldi[-1] if arnoldi else '?'} Arnoldi steps per time step (GPU FP64 at full size: "
+                      "see solver_stats_fp64)"}
 
 
 def terrain_gradient(args):
@@ -371,6 +403,37 @@ def ensemble_run(args):
     print(json.dumps(out), flush=True)
 
 
+def literal_variant(args):
+    """The limited-area workload exactly as written: white-noise +-2 m/s
+    winds and the Davies rim's class default outflow factor 0.01
+    (boundaries.py:21,110-113).  Times mixed and FP64 steps and reports the
+    mixed-vs-FP64 error after `args.literal_steps` steps (the headline uses
+    smooth winds and equal rim weights, whose error stays smooth)."""
+    from suetes_amd import workloads as W
+    from suetes_amd.cases import build_case
+    cfg = W.lam(nx=args.nx, ny=args.nx, nz=args.nz, winds="white", outflow_factor=0.01)
+    c64 = build_case(cfg, device="cuda", precision="fp64")
+    cmx = build_case(cfg, device="cuda", precision="mixed")
+    log("literal workload (white winds, outflow factor 0.01): timing")
+    t_mx, s_mx = time_steps(cmx, args.steps, args.warmup)
+    t_64, s_64 = time_steps(c64, args.steps, args.warmup)
+    done = args.warmup + args.steps
+    errs = {}
+    a, b = s_64, s_mx
+    for i in range(done, args.literal_steps):
+        a = c64.stepper.step(a, i * c64.dt, c64.forcing, c64.bc_fn)
+        b = cmx.stepper.step(b, i * cmx.dt, cmx.forcing, cmx.bc_fn)
+        if (i + 1) % 20 == 0 or i + 1 == args.literal_steps:
+            errs[i + 1] = fields_err(b, a)
+    umax = float(a["u"].abs().max())
+    log(f"literal: mixed {t_mx*1e3:.1f} ms, fp64 {t_64*1e3:.1f} ms, errors {errs}")
+    return {"workload": f"lam {args.nx}x{args.nx}x{args.nz}, white U(-2,2) m/s winds, Davies rim 10 with "
+                        "outflow_factor 0.01 (the class default)",
+            "ms_per_step_mixed": t_mx * 1e3, "ms_per_step_fp64": t_64 * 1e3,
+            "speedup_mixed_vs_fp64": t_64 / t_mx,
+            "max_rel_l2_mixed_vs_fp64_by_step": errs, "fp64_max_abs_u_at_end": umax}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -382,6 +445,10 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--err-steps", type=int, default=100,
                     help="continue both paths to this many steps and report the field error "
+                         "(0: skip)")
+    ap.add_argument("--literal-steps", type=int, default=100,
+                    help="also time the workload as written (white +-2 m/s winds, Davies outflow "
+                         "factor 0.01) and report its mixed-vs-FP64 error after this many steps "
                          "(0: skip)")
     ap.add_argument("--terrain-gradient", action="store_true",
                     help="run the terrain-control reverse-mode case instead (128x128x60, 50 steps)")
@@ -454,15 +521,21 @@ def main():
             adj["kernels_value_and_grad_mixed"] = instrumented_adjoint(cmx)
         except Exception as e:  # report, do not hide
             adj = {"error": repr(e)[:300]}
+    literal = None
+    if args.literal_steps > 0:
+        literal = literal_variant(args)
     cpu = None if args.no_cpu else cpu_baseline()
     npts = args.nx * args.nx * args.nz
     out = {
         "metric": METRIC, "value": 1.0 / t_mx, "unit": "steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": t_mx * 1e3, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "precision": "FP32 operator, preconditioner, departure points and tricubic; FP16 storage of "
-                     "the inner Krylov basis (FP64 inner products); FP64 outer residual refinement, "
-                     "state and explicit tendencies",
+        "vs_baseline": None, "dtype": "f32 (f16 Krylov basis, bf16 operator coefficients; f64 refinement)",
+        "data": "synthetic",
+        "precision": "FP32 arithmetic in the operator, preconditioner, departure points and tricubic; "
+                     "storage: FP16 inner Krylov basis (scaled by sqrt(n); FP64 inner products), bfloat16 "
+                     "3-D coefficient arrays of the inner operator and preconditioner (forward solves; "
+                     "transposed solves read FP32); FP64 outer residual refinement, state and "
+                     "explicit tendencies",
         "config": {"workload": f"lam {args.nx}x{args.nx}x{args.nz} dx=1km dt=10s stretched-SLEVE "
                                "GRF terrain, +-2 m/s winds (10 km correlation), Davies rim 10 "
                                "(equal inflow/outflow weights), GMRES restart 20 tol 1e-10",
@@ -475,6 +548,7 @@ def main():
         "solver_stats_mixed": st_mx, "solver_stats_fp64": st_64,
         "adjoint_forward_cost": adj,
         "roofline": roof, "kernels_mixed_step": table, "cpu_baseline": cpu,
+        "literal_config": literal,
         "autotuned_operator_shape": {
             "fp32": getattr(cmx.stepper.implicit_solver.op32, "tuned", None),
             "fp64": getattr(c64.stepper.implicit_solver.op64, "tuned", None)},

@@ -2,8 +2,13 @@
  * suetes_sisl.h -- C ABI of the MI355X-native SISL dynamical-core kernels.
  *
  * Every entry point takes plain device pointers, sizes and a HIP stream
- * (passed as void*), allocates nothing (all buffers are caller-owned) and
- * returns 0 or a negative SislStatus.  Suffix _f64 / _f32 is the element
+ * (passed as void*), allocates no device memory (all buffers are
+ * caller-owned; the one exception is the measurement timer, whose
+ * sisl_timer_enable keeps a 4 MiB launch record of its own) and returns 0 or
+ * a negative SislStatus.  Launches are ordered on the caller's stream; the
+ * process-wide tables (tuned launch shapes, captured solver cycles, the
+ * timer record) are guarded by locks, so calls on different streams with
+ * their own buffers may run concurrently.  Suffix _f64 / _f32 is the element
  * type of every array argument; Krylov inner products, norms and the
  * Hessenberg least squares are FP64 in both variants.
  *
@@ -159,10 +164,13 @@ int sisl_helmholtz_apply_f32(const SislOperator *op, const float *x, float *y, v
  * thread, loop unroll) on the caller's packed buffers x (input, any finite
  * values) and y (scratch output) and keeps the fastest for this grid and
  * element type (fp32 = 1: the _f32 entry points) for the rest of the
- * process.  All shapes give bit-identical results.  Optional outputs: the
- * chosen chunk and unroll. */
-int sisl_autotune_operator(const SislOperator *op, int32_t fp32, void *x, void *y, int32_t *chunk,
-                           int32_t *unroll, void *stream);
+ * process.  All shapes give bit-identical results.  `scratch`: caller-owned
+ * device memory of sisl_autotune_scratch_bytes(g, fp32) bytes (NULL when
+ * that is 0), used by the fused-normalisation variants.  Optional outputs:
+ * the chosen chunk and unroll.  Synchronises on its own timing events. */
+int64_t sisl_autotune_scratch_bytes(const SislGrid *g, int32_t fp32);
+int sisl_autotune_operator(const SislOperator *op, int32_t fp32, void *x, void *y, void *scratch,
+                           int32_t *chunk, int32_t *unroll, void *stream);
 /* Pin the operator launch shape for grid g and element type (chunk >= 2
  * levels per thread, unroll 1 or 2) of the operator and of its transpose,
  * with and without the fused basis normalisation, e.g. to reproduce a tuned
@@ -256,6 +264,75 @@ int sisl_stagger_T_f32(const float *g, float *gf, int64_t outer, int32_t n, int6
                        int32_t pad, int32_t exact, double rd, const float *m3, int64_t m3n,
                        void *stream);
 
+/* ---- fused explicit tendencies and predictor-corrector (FP64) ----------- */
+
+/*
+ * Static fields of the explicit tendencies (the background and the grid's
+ * metrics; precomputed once per stepper by the host layer).  Device
+ * pointers to FP64 arrays in the internal layout; [u2] [v2] [m2] are 2-D
+ * (per column), [u] [v] [w] [m] 3-D as in SislOperator.
+ */
+typedef struct SislTendency {
+  SislGrid g;
+  double dx, dy, dz;      /* grid spacings                                    */
+  double cp, grav;        /* constants['cp'], constants['g']                  */
+  double alpha, dt;       /* off-centring, time step                          */
+  double nu_h, nu_div;    /* hyperdiffusion, divergence-damping coefficients  */
+  const double *mm;       /* [m2] map factor at mass points                   */
+  const double *mu, *mv;  /* [u2] [v2] map factors at u / v faces             */
+  const double *fu, *fv;  /* [u2] [v2] Coriolis parameter                     */
+  const double *dmdx, *dmdy; /* [m2] map-factor derivatives                   */
+  const double *th_bg, *pi_bg;            /* [m] background theta, Exner     */
+  const double *thu_bg, *thv_bg, *thw_bg; /* [u] [v] [w] its averages        */
+  const double *zxu, *zev;  /* [u] [v] terrain slopes (differences of Z_m)    */
+  const double *zxm, *zem;  /* [m] their mass-point averages                  */
+  const double *rz;         /* [w] dz / dz_w_full                             */
+  const double *zxiw, *zetaw; /* [w] z_xi_w, z_eta_w                          */
+  const double *dzw, *dzm;  /* [w] dz_w_full, [m] dz_m_full                   */
+  const double *cpi;        /* [m] C_pi                                       */
+  const double *dthm;       /* [m] background d theta / dz at mass points     */
+  const double *cx, *cy;    /* [u] [v] rho th dz / m (linear Exner flux)      */
+  const double *cxp, *cyp;  /* [u] [v] rho dz / m (theta' Exner flux)         */
+  const double *czs, *czp;  /* [w] dz_w rho_w th_w, dz_w rho_w                */
+} SislTendency;
+
+/* HyperFilter (suetes/regional3d/diffusion.py:93-172): the terrain-corrected
+ * Cartesian horizontal Laplacian applied twice to u_m, v_m, w_m and theta'
+ * of the state (u, v, w, th); lap2 receives the four mass fields (4 x N).
+ * work: sisl_hyperdiffusion_workspace_f64(g) elements. */
+int64_t sisl_hyperdiffusion_workspace_f64(const SislGrid *g);
+int sisl_hyperdiffusion_f64(const SislTendency *t, const double *u, const double *v, const double *w,
+                            const double *th, double *work, double *lap2, void *stream);
+/* Euler3D.get_tendencies(is_explicit=True) (suetes/regional3d/euler.py:
+ * 208-372) with nonlinear_pi_tendency and metric_curvature_tendencies
+ * (euler.py:162-206), combined into the step's arrival fields
+ * (suetes/regional3d/steppers.py:776-850): u_in, v_in, w_in, pi_in (pi'),
+ * th_in (theta'), the kinematic residual res (w points) and theta's
+ * tendency tth.  lap2: output of sisl_hyperdiffusion_f64 (NULL when
+ * nu_h = 0).  Optional: x0 (packed u | v | w | pi' 1000 | eta dz_w, the
+ * first solve's initial guess) and tend (raw tendencies tu | tv | tw | tpi
+ * in their own layouts, N_u + N_v + N_w + N elements). */
+int sisl_explicit_f64(const SislTendency *t, const double *u, const double *v, const double *w,
+                      const double *pi, const double *th, const double *eta, const double *lap2,
+                      double *u_in, double *v_in, double *w_in, double *pi_in, double *th_in,
+                      double *res, double *tth, double *x0, double *tend, void *stream);
+/* One predictor-corrector pass (steppers.py:911-968): theta at n+1 from the
+ * advected theta', its tendency and the guess's w; then the packed
+ * right-hand side b = u | v | w | pi 1000 | eta of the implicit solve from
+ * the advected fields (rhs_u / rhs_v with their lateral faces already set),
+ * the raw advected kinematic residual reta, the guess (g_*: physical units)
+ * and the winds of the metric and kinematic-bottom terms (m_u, m_v). */
+int sisl_predictor_f64(const SislTendency *t, const double *rhs_u, const double *rhs_v,
+                       const double *rhs_w, const double *rhs_pi, const double *th_adv,
+                       const double *tth, const double *w_n, const double *reta, const double *g_pi,
+                       const double *g_u, const double *g_v, const double *g_w, const double *g_eta,
+                       const double *m_u, const double *m_v, double *th_next, double *b, void *stream);
+/* Divergence damping of the solved winds and the solution's unscaling
+ * (steppers.py:1003-1035): from the packed solution x, u and v after
+ * damping, pi = x_pi / 1000 + pi_bg and eta = x_eta / dz_w. */
+int sisl_final_f64(const SislTendency *t, const double *x, double *u, double *v, double *pi, double *eta,
+                   void *stream);
+
 /* ---- flux-form semi-Lagrangian transport ------------------------------- */
 
 /* FluxFormAdvector.advect_1d (suetes/regional3d/steppers.py:552-594) along
@@ -306,7 +383,9 @@ enum SislTimedKernel {
   SISL_T_TRICUBIC_ADJ = 8, SISL_T_COUNT = 9
 };
 /* on = 1: clear the record and bracket every later launch of a timed kernel
- * with HIP events on its stream; on = 0: stop recording (the record stays). */
+ * with HIP events on its stream; on = 0: stop recording (the record stays).
+ * The first enable allocates the record's device flags (4 MiB, kept for the
+ * process): a measurement facility, not for allocation-free callers. */
 int sisl_timer_enable(int32_t on);
 /* Totals of one kernel (fp32 = 1: FP32 working-type variant) since the last
  * enable: event time of all launches, number of launches, launches that did

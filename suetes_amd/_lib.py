@@ -23,7 +23,7 @@ EXPORTED = [
     "sisl_helmholtz_apply_T_f32", "sisl_precond_apply_f64", "sisl_precond_apply_f32",
     "sisl_precond_apply_T_f64", "sisl_precond_apply_T_f32", "sisl_gmres_workspace_bytes_f64",
     "sisl_gmres_workspace_bytes_mixed", "sisl_gmres_f64", "sisl_gmres_mixed", "sisl_dot_f64",
-    "sisl_timer_enable", "sisl_timer_read", "sisl_autotune_operator",
+    "sisl_timer_enable", "sisl_timer_read", "sisl_autotune_operator", "sisl_autotune_scratch_bytes",
     "sisl_set_operator_shape", "sisl_tricubic_batched_f64", "sisl_tricubic_batched_f32",
     "sisl_departure_batched_f64", "sisl_departure_batched_f32", "sisl_gmres_batched_workspace_bytes",
     "sisl_gmres_batched_setup", "sisl_gmres_batched", "sisl_stagger_f64", "sisl_stagger_f32",
@@ -103,8 +103,10 @@ def lib():
                                        C.POINTER(SislGmresParams), vp, vp, vp,
                                        C.POINTER(SislGmresStats), vp]
         L.sisl_dot_f64.argtypes = [vp, vp, i64, vp, vp, vp]
-        L.sisl_autotune_operator.argtypes = [C.POINTER(SislOperator), i32, vp, vp, C.POINTER(i32),
+        L.sisl_autotune_operator.argtypes = [C.POINTER(SislOperator), i32, vp, vp, vp, C.POINTER(i32),
                                              C.POINTER(i32), vp]
+        L.sisl_autotune_scratch_bytes.restype = i64
+        L.sisl_autotune_scratch_bytes.argtypes = [C.POINTER(SislGrid), i32]
         L.sisl_set_operator_shape.argtypes = [C.POINTER(SislGrid), i32, i32, i32]
         pp = C.POINTER(vp)
         for p in ("f64", "f32"):

@@ -651,8 +651,17 @@ int launch_AT(const OpT<T> &o, const T *y, T *x, T * /*scratch: unused*/, hipStr
 // Time every launch shape of the operator and of its transpose (FP32 also
 // with the fused FP16 basis normalisation) on the caller's buffers and keep
 // the fastest of each for this grid and element type.
+__global__ void set_one_kernel(double *p) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *p = 1.0;
+}
+
+// Bytes of caller scratch the autotuner needs: FP32 times the fused-
+// normalisation variants, which read a scale (1.0) from device memory and
+// store an FP16 basis column; FP64 needs none.
+int64_t autotune_scratch_bytes(const Stag &s, bool fp32) { return fp32 ? 256 + 2 * (int64_t)s.n : 0; }
+
 template <typename T>
-static int autotune_A(const OpT<T> &o, const T *x, T *y, hipStream_t st, int32_t *best_kc,
+static int autotune_A(const OpT<T> &o, const T *x, T *y, void *scratch, hipStream_t st, int32_t *best_kc,
                       int32_t *best_unr) {
   if ((uint64_t)o.s.n * sizeof(T) >= (1ull << 32)) return SISL_ERR_ARG;
   const int levels = o.s.nz + 1;
@@ -663,24 +672,21 @@ static int autotune_A(const OpT<T> &o, const T *x, T *y, hipStream_t st, int32_t
       if (kc >= 2) cand.push_back({kc, unr});
     }
   const int nvar = sizeof(T) == 4 ? 4 : 2;
-  // fused variants: scale 1 from device memory, basis column into a scratch buffer
-  void *tmp = nullptr;
+  // fused variants: scale 1 from device memory, basis column into the caller's scratch
   Fuse fz;
   if (nvar == 4) {
-    if (hipMalloc(&tmp, 256 + 2 * (size_t)o.s.n) != hipSuccess) return SISL_ERR_LAUNCH;
-    const double one = 1.0;
-    if (hipMemcpy(tmp, &one, sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
-      hipFree(tmp);
-      return SISL_ERR_LAUNCH;
-    }
-    fz.inv = (const double *)tmp;
-    fz.vout = (char *)tmp + 256;
+    if (!scratch) return SISL_ERR_ARG;
+    hipLaunchKernelGGL(set_one_kernel, dim3(1), dim3(64), 0, st, static_cast<double *>(scratch));
+    SISL_CHECK_LAUNCH();
+    fz.inv = static_cast<const double *>(scratch);
+    fz.vout = static_cast<char *>(scratch) + 256;
     fz.nb = 1;
     fz.vs = 1.0;
   }
   hipEvent_t e0, e1;
-  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
-    if (tmp) hipFree(tmp);
+  if (hipEventCreate(&e0) != hipSuccess) return SISL_ERR_LAUNCH;
+  if (hipEventCreate(&e1) != hipSuccess) {
+    hipEventDestroy(e0);
     return SISL_ERR_LAUNCH;
   }
   AShape best[4];
@@ -707,7 +713,6 @@ static int autotune_A(const OpT<T> &o, const T *x, T *y, hipStream_t st, int32_t
   }
   hipEventDestroy(e0);
   hipEventDestroy(e1);
-  if (tmp) hipFree(tmp);
   if (rc) return rc;
   SISL_CHECK_LAUNCH();
   {
@@ -796,17 +801,21 @@ int sisl_set_operator_shape(const SislGrid *g, int32_t fp32, int32_t chunk, int3
   return SISL_OK;
 }
 
-int sisl_autotune_operator(const SislOperator *op, int32_t fp32, void *x, void *y, int32_t *chunk,
-                           int32_t *unroll, void *s) {
+int64_t sisl_autotune_scratch_bytes(const SislGrid *g, int32_t fp32) {
+  if (!g) return -1;
+  return sisl::autotune_scratch_bytes(sisl::Stag::make(*g), fp32 != 0);
+}
+int sisl_autotune_operator(const SislOperator *op, int32_t fp32, void *x, void *y, void *scratch,
+                           int32_t *chunk, int32_t *unroll, void *s) {
   if (!op || !x || !y || x == y) return SISL_ERR_ARG;
   if (fp32) {
     auto o = sisl::OpT<float>::make(*op);
     if (!o.complete()) return SISL_ERR_ARG;
-    return sisl::autotune_A(o, (const float *)x, (float *)y, sisl::as_stream(s), chunk, unroll);
+    return sisl::autotune_A(o, (const float *)x, (float *)y, scratch, sisl::as_stream(s), chunk, unroll);
   }
   auto o = sisl::OpT<double>::make(*op);
   if (!o.complete()) return SISL_ERR_ARG;
-  return sisl::autotune_A(o, (const double *)x, (double *)y, sisl::as_stream(s), chunk, unroll);
+  return sisl::autotune_A(o, (const double *)x, (double *)y, scratch, sisl::as_stream(s), chunk, unroll);
 }
 int sisl_helmholtz_apply_T_f64(const SislOperator *op, const double *x, double *y, double *scr,
                                void *s) {

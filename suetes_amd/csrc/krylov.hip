@@ -964,16 +964,23 @@ static int gmres64(const OpT<double> &op, const SislGmresParams &p, const double
 // on, the eager path runs so that every launch is timed.
 static std::mutex g_graph_mu;
 static bool g_graphs_on = true;
-static std::map<std::string, hipGraphExec_t> g_graph_cache;
+// A cached graph and an event recorded after its latest replay.
+struct CachedGraph {
+  hipGraphExec_t exec;
+  hipEvent_t done;
+};
+static std::map<std::string, CachedGraph> g_graph_cache;
 static uint64_t g_graph_generation = 0;  // shape generation of the cached graphs
 constexpr size_t kMaxGraphs = 64;
 
-// Destroy every cached graph (after the device has drained, so that none is
-// still executing).  Caller holds g_graph_mu.
+// Destroy every cached graph once its latest replay has finished (waits on
+// that replay's event only, not on the device).  Caller holds g_graph_mu.
 static void purge_graphs() {
-  if (g_graph_cache.empty()) return;
-  hipDeviceSynchronize();
-  for (auto &e : g_graph_cache) hipGraphExecDestroy(e.second);
+  for (auto &e : g_graph_cache) {
+    hipEventSynchronize(e.second.done);
+    hipGraphExecDestroy(e.second.exec);
+    hipEventDestroy(e.second.done);
+  }
   g_graph_cache.clear();
 }
 static std::map<int, hipStream_t> g_capture_streams;
@@ -1036,7 +1043,7 @@ static int run_inner_cycle(Krylov<float, TB> &K, int R) {
     return SISL_OK;
   }
   const std::string key = cycle_key(K, R);
-  hipGraphExec_t exec = nullptr;
+  bool cached;
   {
     std::lock_guard<std::mutex> g(g_graph_mu);
     const uint64_t gen = shape_generation();
@@ -1044,10 +1051,10 @@ static int run_inner_cycle(Krylov<float, TB> &K, int R) {
       purge_graphs();  // graphs of older shapes never match again; bound the cache
       g_graph_generation = gen;
     }
-    auto it = g_graph_cache.find(key);
-    if (it != g_graph_cache.end()) exec = it->second;
+    cached = g_graph_cache.count(key) != 0;
   }
-  if (!exec) {
+  if (!cached) {
+    hipGraphExec_t exec = nullptr;
     hipStream_t cs;
     {
       std::lock_guard<std::mutex> g(g_graph_mu);
@@ -1072,10 +1079,33 @@ static int run_inner_cycle(Krylov<float, TB> &K, int R) {
     const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
     hipGraphDestroy(graph);
     if (ei != hipSuccess) return SISL_ERR_LAUNCH;
+    hipEvent_t done = nullptr;
+    if (hipEventCreateWithFlags(&done, hipEventDisableTiming) != hipSuccess) {
+      hipGraphExecDestroy(exec);
+      return SISL_ERR_LAUNCH;
+    }
     std::lock_guard<std::mutex> g(g_graph_mu);
-    g_graph_cache[key] = exec;
+    auto it = g_graph_cache.find(key);
+    if (it != g_graph_cache.end()) {  // another thread cached the same cycle meanwhile
+      hipGraphExecDestroy(exec);
+      hipEventDestroy(done);
+    } else {
+      g_graph_cache[key] = CachedGraph{exec, done};
+    }
   }
-  if (hipGraphLaunch(exec, K.s) != hipSuccess) return SISL_ERR_LAUNCH;
+  // launch under the lock, so that no purge destroys the graph in between;
+  // the event marks this replay for a later purge
+  std::lock_guard<std::mutex> g(g_graph_mu);
+  auto it = g_graph_cache.find(key);
+  if (it == g_graph_cache.end()) {  // purged meanwhile: run the cycle eagerly
+    for (int j = 0; j < R; ++j) {
+      const int rc = K.arnoldi(j, R, 1);
+      if (rc) return rc;
+    }
+    return SISL_OK;
+  }
+  if (hipGraphLaunch(it->second.exec, K.s) != hipSuccess) return SISL_ERR_LAUNCH;
+  if (hipEventRecord(it->second.done, K.s) != hipSuccess) return SISL_ERR_LAUNCH;
   return SISL_OK;
 }
 

@@ -1,0 +1,580 @@
+// Fused explicit tendencies, hyperdiffusion and the predictor-corrector
+// right-hand side of one SISL step (FP64, the state's precision).
+//
+// The reference evaluates these as ~740 array expressions per step
+// (suetes/regional3d/euler.py:208-372 explicit tendencies,
+// suetes/regional3d/diffusion.py:93-172 terrain-following hyperdiffusion,
+// suetes/regional3d/steppers.py:742-1003 step glue).  Here they are:
+//
+//   hyper_faces / hyper_mass  the terrain-corrected Cartesian horizontal
+//                             Laplacian of four mass fields (u_m, v_m, w_m,
+//                             theta'), applied twice for del^4: a face pass
+//                             (Gx at u, Gy at v) and a mass pass;
+//   explicit_kernel           every explicit tendency and the arrival-point
+//                             fields of the step (u_in .. th_in, the kinematic
+//                             residual, theta's tendency, the packed x0);
+//   thnext_kernel, rhs_kernel the predictor-corrector: theta at n+1 and the
+//                             packed right-hand side of the implicit solve;
+//   final_kernel              divergence damping and unscaling of the
+//                             solution.
+//
+// Staggered operators follow CGridOperator3D (operators.py:37-132): a face
+// value between mass points a and b is their difference / average; at a
+// closed boundary the first / last interior face value is repeated, at a
+// periodic one the face between the last and the first cell is used at
+// both ends; horizontal differences carry the map factor of the target
+// location.  Each thread evaluates one output point from its neighbours
+// (the inputs stay in L2 across neighbouring threads), in the reference's
+// operation order, so results agree with the array formulation to rounding.
+#include "common.h"
+
+namespace sisl {
+
+struct TendCtx {
+  int nx, ny, nz, px, py;
+  double rdx, rdy, rdz, dz;
+  double cp, grav, alpha, dt, nu_h, nu_div;
+  const double *mm, *mu, *mv, *fu, *fv, *dmdx, *dmdy;
+  const double *th_bg, *pi_bg, *thu_bg, *thv_bg, *thw_bg, *zxu, *zev, *zxm, *zem, *rz, *zxiw, *zetaw;
+  const double *dzw, *dzm, *cpi, *dthm, *cx, *cy, *cxp, *cyp, *czs, *czp;
+
+  // element offsets (internal layout, x fastest)
+  __device__ __forceinline__ int m(int i, int j, int k) const { return (k * ny + j) * nx + i; }
+  __device__ __forceinline__ int u(int i, int j, int k) const { return (k * ny + j) * (nx + 1) + i; }
+  __device__ __forceinline__ int v(int i, int j, int k) const { return (k * (ny + 1) + j) * nx + i; }
+  __device__ __forceinline__ int w(int i, int j, int k) const { return (k * ny + j) * nx + i; }
+  __device__ __forceinline__ int m2(int i, int j) const { return j * nx + i; }
+  __device__ __forceinline__ int u2(int i, int j) const { return j * (nx + 1) + i; }
+  __device__ __forceinline__ int v2(int i, int j) const { return j * nx + i; }
+  __device__ __forceinline__ int nm() const { return nx * ny * nz; }
+  __device__ __forceinline__ int nu() const { return (nx + 1) * ny * nz; }
+  __device__ __forceinline__ int nv() const { return nx * (ny + 1) * nz; }
+  __device__ __forceinline__ int nw() const { return nx * ny * (nz + 1); }
+};
+
+// Mass points (a, b) whose difference / average gives face f (0..n) along an
+// axis of n mass points (operators.py:37-103: closed ends repeat the first /
+// last interior face, periodic ends use the face between cell n-1 and 0).
+__device__ __forceinline__ void face_pair(int f, int n, int per, int &a, int &b) {
+  if (per && (f == 0 || f == n)) {
+    a = n - 1;
+    b = 0;
+    return;
+  }
+  const int g = min(max(f, 1), n - 1);
+  a = g - 1;
+  b = g;
+}
+// The mass level below w level kk whose vertical difference gives it (closed ends).
+__device__ __forceinline__ int wlev(int kk, int nz) { return min(max(kk, 1), nz - 1); }
+
+// diff_z(F, m->w) * rz at w level kk of column c (F a mass field).
+__device__ __forceinline__ double dzw_rz(const TendCtx &t, const double *F, int i, int j, int kk) {
+  const int g = wlev(kk, t.nz);
+  return ((F[t.m(i, j, g)] - F[t.m(i, j, g - 1)]) * t.rdz) * t.rz[t.w(i, j, kk)];
+}
+// avg_z(diff_z(F) * rz) at mass level k (the vertical derivative on mass points).
+__device__ __forceinline__ double dz_mass(const TendCtx &t, const double *F, int i, int j, int k) {
+  return 0.5 * (dzw_rz(t, F, i, j, k) + dzw_rz(t, F, i, j, k + 1));
+}
+
+// ---- hyperdiffusion (diffusion.py:93-129, twice: del^4) ----------------------------
+
+// Base fields of the four tendencies (diffusion.py:136-160): u_m, v_m, w_m, theta'.
+__global__ void __launch_bounds__(256) hyper_base_kernel(TendCtx t, const double *__restrict__ U,
+                                                         const double *__restrict__ V,
+                                                         const double *__restrict__ W,
+                                                         const double *__restrict__ TH, double *__restrict__ F) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int N = t.nm();
+  if (p >= N) return;
+  const int i = p % t.nx, j = (p / t.nx) % t.ny, k = p / (t.nx * t.ny);
+  F[p] = 0.5 * (U[t.u(i, j, k)] + U[t.u(i + 1, j, k)]);
+  F[N + p] = 0.5 * (V[t.v(i, j, k)] + V[t.v(i, j + 1, k)]);
+  F[2 * N + p] = 0.5 * (W[t.w(i, j, k)] + W[t.w(i, j, k + 1)]);
+  F[3 * N + p] = TH[p] - t.th_bg[p];
+}
+
+// Face pass: Gx = dF/dx - z_x dF/dz at u faces, Gy likewise at v faces, for
+// nf mass fields F (stride nm) into GX (stride nu), GY (stride nv).
+__global__ void __launch_bounds__(256) hyper_faces_kernel(TendCtx t, int nf, const double *__restrict__ F,
+                                                          double *__restrict__ GX, double *__restrict__ GY) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int sx = t.nx + 1, sy = t.ny + 1;
+  if (p >= sx * sy * t.nz) return;
+  const int i = p % sx, j = (p / sx) % sy, k = p / (sx * sy);
+  const int N = t.nm(), Nu = t.nu(), Nv = t.nv();
+  if (j < t.ny) {
+    int a, b;
+    face_pair(i, t.nx, t.px, a, b);
+    const double mf = t.mu[t.u2(i, j)], zx = t.zxu[t.u(i, j, k)];
+    for (int f = 0; f < nf; ++f) {
+      const double *Ff = F + (size_t)f * N;
+      const double dfx = ((Ff[t.m(b, j, k)] - Ff[t.m(a, j, k)]) * t.rdx) * mf;
+      const double dfz = 0.5 * (dz_mass(t, Ff, a, j, k) + dz_mass(t, Ff, b, j, k));
+      GX[(size_t)f * Nu + t.u(i, j, k)] = dfx - zx * dfz;
+    }
+  }
+  if (i < t.nx) {
+    int a, b;
+    face_pair(j, t.ny, t.py, a, b);
+    const double mf = t.mv[t.v2(i, j)], ze = t.zev[t.v(i, j, k)];
+    for (int f = 0; f < nf; ++f) {
+      const double *Ff = F + (size_t)f * N;
+      const double dfy = ((Ff[t.m(i, b, k)] - Ff[t.m(i, a, k)]) * t.rdy) * mf;
+      const double dfz = 0.5 * (dz_mass(t, Ff, i, a, k) + dz_mass(t, Ff, i, b, k));
+      GY[(size_t)f * Nv + t.v(i, j, k)] = dfy - ze * dfz;
+    }
+  }
+}
+
+// Mass pass: L = (dGx/dx - z_x dGx_m/dz) + (dGy/dy - z_y dGy_m/dz).
+__device__ __forceinline__ double gx_m(const TendCtx &t, const double *G, int i, int j, int k) {
+  return 0.5 * (G[t.u(i, j, k)] + G[t.u(i + 1, j, k)]);
+}
+__device__ __forceinline__ double gy_m(const TendCtx &t, const double *G, int i, int j, int k) {
+  return 0.5 * (G[t.v(i, j, k)] + G[t.v(i, j + 1, k)]);
+}
+__global__ void __launch_bounds__(256) hyper_mass_kernel(TendCtx t, int nf, const double *__restrict__ GX,
+                                                         const double *__restrict__ GY, double *__restrict__ L) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int N = t.nm();
+  if (p >= N) return;
+  const int i = p % t.nx, j = (p / t.nx) % t.ny, k = p / (t.nx * t.ny);
+  const int Nu = t.nu(), Nv = t.nv();
+  const double mmv = t.mm[t.m2(i, j)];
+  const int g0 = wlev(k, t.nz), g1 = wlev(k + 1, t.nz);
+  const double rz0 = t.rz[t.w(i, j, k)], rz1 = t.rz[t.w(i, j, k + 1)];
+  for (int f = 0; f < nf; ++f) {
+    const double *Gx = GX + (size_t)f * Nu, *Gy = GY + (size_t)f * Nv;
+    const double dGx = ((Gx[t.u(i + 1, j, k)] - Gx[t.u(i, j, k)]) * t.rdx) * mmv;
+    const double ex0 = ((gx_m(t, Gx, i, j, g0) - gx_m(t, Gx, i, j, g0 - 1)) * t.rdz) * rz0;
+    const double ex1 = ((gx_m(t, Gx, i, j, g1) - gx_m(t, Gx, i, j, g1 - 1)) * t.rdz) * rz1;
+    const double dGxz = 0.5 * (ex0 + ex1);
+    const double dGy = ((Gy[t.v(i, j + 1, k)] - Gy[t.v(i, j, k)]) * t.rdy) * mmv;
+    const double ey0 = ((gy_m(t, Gy, i, j, g0) - gy_m(t, Gy, i, j, g0 - 1)) * t.rdz) * rz0;
+    const double ey1 = ((gy_m(t, Gy, i, j, g1) - gy_m(t, Gy, i, j, g1 - 1)) * t.rdz) * rz1;
+    const double dGyz = 0.5 * (ey0 + ey1);
+    L[(size_t)f * N + p] = (dGx - t.zxm[p] * dGxz) + (dGy - t.zem[p] * dGyz);
+  }
+}
+
+// ---- explicit tendencies and arrival fields (euler.py:208-372, steppers.py:776-850) --
+
+struct ExplicitIO {
+  const double *u, *v, *w, *pi, *th, *eta;
+  const double *lap2;  // del^4 of (u_m, v_m, w_m, theta'), 4 mass fields; null when nu_h = 0
+  double *u_in, *v_in, *w_in, *pi_in, *th_in, *res, *tth;
+  double *x0;    // optional: packed u | v | w | pi' * 1000 | eta * dz_w (the first solve's x0)
+  double *tend;  // optional: raw tendencies tu | tv | tw | tpi (the arrays' own layouts)
+};
+
+__device__ __forceinline__ double pip(const TendCtx &t, const double *PI, int q) { return PI[q] - t.pi_bg[q]; }
+__device__ __forceinline__ double thp(const TendCtx &t, const double *TH, int q) { return TH[q] - t.th_bg[q]; }
+// d pi' / dz at w level kk (diff_z(pi') * rz), and its mass-point average
+__device__ __forceinline__ double gzw_pi(const TendCtx &t, const double *PI, int i, int j, int kk) {
+  const int g = wlev(kk, t.nz);
+  return ((pip(t, PI, t.m(i, j, g)) - pip(t, PI, t.m(i, j, g - 1))) * t.rdz) * t.rz[t.w(i, j, kk)];
+}
+__device__ __forceinline__ double gzm_pi(const TendCtx &t, const double *PI, int i, int j, int k) {
+  return 0.5 * (gzw_pi(t, PI, i, j, k) + gzw_pi(t, PI, i, j, k + 1));
+}
+// the same for a perturbation field given directly (no background to subtract)
+__device__ __forceinline__ double gzw_p(const TendCtx &t, const double *P, int i, int j, int kk) {
+  const int g = wlev(kk, t.nz);
+  return ((P[t.m(i, j, g)] - P[t.m(i, j, g - 1)]) * t.rdz) * t.rz[t.w(i, j, kk)];
+}
+__device__ __forceinline__ double gzm_p(const TendCtx &t, const double *P, int i, int j, int k) {
+  return 0.5 * (gzw_p(t, P, i, j, k) + gzw_p(t, P, i, j, k + 1));
+}
+__device__ __forceinline__ double um_at(const TendCtx &t, const double *U, int i, int j, int k) {
+  return 0.5 * (U[t.u(i, j, k)] + U[t.u(i + 1, j, k)]);
+}
+__device__ __forceinline__ double vm_at(const TendCtx &t, const double *V, int i, int j, int k) {
+  return 0.5 * (V[t.v(i, j, k)] + V[t.v(i, j + 1, k)]);
+}
+// map-curvature mass term vm dm/dx - um dm/dy (euler.py:188-206)
+__device__ __forceinline__ double curv_m(const TendCtx &t, const double *U, const double *V, int i, int j, int k) {
+  return vm_at(t, V, i, j, k) * t.dmdx[t.m2(i, j)] - um_at(t, U, i, j, k) * t.dmdy[t.m2(i, j)];
+}
+// -nu_h del^4 of base field f at mass point q
+__device__ __forceinline__ double hyp(const TendCtx &t, const double *L2, int f, int q) {
+  return L2 ? (-t.nu_h) * L2[(size_t)f * t.nm() + q] : 0.0;
+}
+
+// Divergence of a horizontal flux pair (face arrays FXc * FX, FYc * FY) and of a
+// vertical flux at mass point (i, j, k): -C_pi (m (dxx + dyy) + dzz)
+// (euler.py:298-318 and 162-186).  `thx`, `thy`, `thz` multiply the fluxes when
+// given (theta' at the faces), else 1.
+template <typename FU, typename FV, typename FZ>
+__device__ __forceinline__ double pi_div(const TendCtx &t, int i, int j, int k, FU fxu, FV fyv, FZ fzw) {
+  const int q = t.m(i, j, k);
+  const double mmv = t.mm[t.m2(i, j)];
+  const double dzm = t.dzm[q];
+  const double dxx = (((fxu(i + 1) - fxu(i)) * t.rdx) * mmv) / dzm;
+  const double dyy = (((fyv(j + 1) - fyv(j)) * t.rdy) * mmv) / dzm;
+  const double dzz = (((fzw(k + 1) - fzw(k)) * t.rdz) * t.dz) / dzm;
+  return (-t.cpi[q]) * (mmv * (dxx + dyy) + dzz);
+}
+
+__global__ void __launch_bounds__(256) explicit_kernel(TendCtx t, ExplicitIO io) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int sx = t.nx + 1, sy = t.ny + 1;
+  if (p >= sx * sy * (t.nz + 1)) return;
+  const int i = p % sx, j = (p / sx) % sy, k = p / (sx * sy);
+  const int nx = t.nx, ny = t.ny, nz = t.nz;
+  const double a = t.alpha, dt = t.dt, oma = 1.0 - t.alpha;
+  const double *U = io.u, *V = io.v, *W = io.w, *PI = io.pi, *TH = io.th, *E = io.eta;
+  const int Nu = t.nu(), Nv = t.nv(), Nw = t.nw(), N = t.nm();
+  // ---- u face (i, j, k) ----
+  if (j < ny && k < nz) {
+    int ia, ib;
+    face_pair(i, nx, t.px, ia, ib);
+    const int qu = t.u(i, j, k);
+    const double gx = ((pip(t, PI, t.m(ib, j, k)) - pip(t, PI, t.m(ia, j, k))) * t.rdx) * t.mu[t.u2(i, j)];
+    const double gzu = 0.5 * (gzm_pi(t, PI, ia, j, k) + gzm_pi(t, PI, ib, j, k));
+    const double thu = t.thu_bg[qu] + 0.5 * (thp(t, TH, t.m(ia, j, k)) + thp(t, TH, t.m(ib, j, k)));
+    const double vau = 0.5 * (vm_at(t, V, ia, j, k) + vm_at(t, V, ib, j, k));
+    double tu = ((-t.cp) * thu) * (gx - t.zxu[qu] * gzu);
+    tu = tu + t.fu[t.u2(i, j)] * vau;
+    tu = tu + (0.5 * (curv_m(t, U, V, ia, j, k) + curv_m(t, U, V, ib, j, k))) * vau;
+    const double pd = io.lap2 ? 0.5 * (hyp(t, io.lap2, 0, t.m(ia, j, k)) + hyp(t, io.lap2, 0, t.m(ib, j, k))) : 0.0;
+    if (io.lap2) tu = tu + pd;
+    io.u_in[qu] = U[qu] + dt * (oma * tu + a * pd);
+    if (io.tend) io.tend[qu] = tu;
+    if (io.x0) io.x0[qu] = U[qu];
+  }
+  // ---- v face (i, j, k) ----
+  if (i < nx && k < nz) {
+    int ja, jb;
+    face_pair(j, ny, t.py, ja, jb);
+    const int qv = t.v(i, j, k);
+    const double gy = ((pip(t, PI, t.m(i, jb, k)) - pip(t, PI, t.m(i, ja, k))) * t.rdy) * t.mv[t.v2(i, j)];
+    const double gzv = 0.5 * (gzm_pi(t, PI, i, ja, k) + gzm_pi(t, PI, i, jb, k));
+    const double thv = t.thv_bg[qv] + 0.5 * (thp(t, TH, t.m(i, ja, k)) + thp(t, TH, t.m(i, jb, k)));
+    const double uav = 0.5 * (um_at(t, U, i, ja, k) + um_at(t, U, i, jb, k));
+    double tv = ((-t.cp) * thv) * (gy - t.zev[qv] * gzv);
+    tv = tv - t.fv[t.v2(i, j)] * uav;
+    tv = tv + (-(0.5 * (curv_m(t, U, V, i, ja, k) + curv_m(t, U, V, i, jb, k)))) * uav;
+    const double pd = io.lap2 ? 0.5 * (hyp(t, io.lap2, 1, t.m(i, ja, k)) + hyp(t, io.lap2, 1, t.m(i, jb, k))) : 0.0;
+    if (io.lap2) tv = tv + pd;
+    io.v_in[qv] = V[qv] + dt * (oma * tv + a * pd);
+    if (io.tend) io.tend[Nu + qv] = tv;
+    if (io.x0) io.x0[Nu + qv] = V[qv];
+  }
+  // ---- w level (i, j, k), k = 0..nz ----
+  if (i < nx && j < ny) {
+    const int qw = t.w(i, j, k);
+    const int g = wlev(k, nz);
+    const double thpw = 0.5 * (thp(t, TH, t.m(i, j, g - 1)) + thp(t, TH, t.m(i, j, g)));
+    const double thw = t.thw_bg[qw] + thpw;
+    const double buoy = t.grav * (thpw / t.thw_bg[qw]);
+    double tw = ((-t.cp) * thw) * gzw_pi(t, PI, i, j, k) + buoy;
+    const double pd = io.lap2 ? 0.5 * (hyp(t, io.lap2, 2, t.m(i, j, g - 1)) + hyp(t, io.lap2, 2, t.m(i, j, g))) : 0.0;
+    if (io.lap2) tw = tw + pd;
+    io.w_in[qw] = W[qw] + dt * (oma * tw + a * pd);
+    if (io.tend) io.tend[Nu + Nv + qw] = tw;
+    // kinematic residual dz_w eta + u_w z_xi + v_w z_eta - w (steppers.py:837-842)
+    const double uw = 0.5 * (um_at(t, U, i, j, g - 1) + um_at(t, U, i, j, g));
+    const double vw = 0.5 * (vm_at(t, V, i, j, g - 1) + vm_at(t, V, i, j, g));
+    io.res[qw] = ((t.dzw[qw] * E[qw] + uw * t.zxiw[qw]) + vw * t.zetaw[qw]) - W[qw];
+    if (io.x0) {
+      io.x0[Nu + Nv + qw] = W[qw];
+      io.x0[Nu + Nv + Nw + N + qw] = E[qw] * t.dzw[qw];
+    }
+  }
+  // ---- mass point (i, j, k) ----
+  if (i < nx && j < ny && k < nz) {
+    const int q = t.m(i, j, k);
+    // linear Exner tendency: background fluxes (euler.py:298-318)
+    const double tpi = pi_div(
+        t, i, j, k, [&](int f) { return U[t.u(f, j, k)] * t.cx[t.u(f, j, k)]; },
+        [&](int f) { return V[t.v(i, f, k)] * t.cy[t.v(i, f, k)]; },
+        [&](int kk) { return (kk == 0 || kk == nz) ? 0.0 : E[t.w(i, j, kk)] * t.czs[t.w(i, j, kk)]; });
+    // theta' Exner flux (euler.py:162-186): theta' averaged to the faces
+    auto thpu = [&](int f) {
+      int ia, ib;
+      face_pair(f, nx, t.px, ia, ib);
+      return 0.5 * (thp(t, TH, t.m(ia, j, k)) + thp(t, TH, t.m(ib, j, k)));
+    };
+    auto thpv = [&](int f) {
+      int ja, jb;
+      face_pair(f, ny, t.py, ja, jb);
+      return 0.5 * (thp(t, TH, t.m(i, ja, k)) + thp(t, TH, t.m(i, jb, k)));
+    };
+    const double nl = pi_div(
+        t, i, j, k, [&](int f) { return U[t.u(f, j, k)] * thpu(f) * t.cxp[t.u(f, j, k)]; },
+        [&](int f) { return V[t.v(i, f, k)] * thpv(f) * t.cyp[t.v(i, f, k)]; },
+        [&](int kk) {
+          if (kk == 0 || kk == nz) return 0.0;
+          const double tw_ = 0.5 * (thp(t, TH, t.m(i, j, kk - 1)) + thp(t, TH, t.m(i, j, kk)));
+          return E[t.w(i, j, kk)] * tw_ * t.czp[t.w(i, j, kk)];
+        });
+    const double wm = 0.5 * (W[t.w(i, j, k)] + W[t.w(i, j, k + 1)]);
+    double tth = (-wm) * t.dthm[q];
+    if (io.lap2) tth = tth + hyp(t, io.lap2, 3, q);
+    const double pp = pip(t, PI, q);
+    io.pi_in[q] = pp + (oma * dt) * (tpi + nl);
+    io.th_in[q] = thp(t, TH, q) + (oma * dt) * tth;
+    io.tth[q] = tth;
+    if (io.tend) io.tend[Nu + Nv + Nw + q] = tpi;
+    if (io.x0) io.x0[Nu + Nv + Nw + q] = pp * 1000.0;
+  }
+}
+
+// ---- predictor-corrector (steppers.py:911-998) --------------------------------------
+
+struct PredictorIO {
+  const double *rhs_u, *rhs_v, *rhs_w, *rhs_pi, *th_adv, *tth, *w_n, *reta;
+  const double *g_pi, *g_u, *g_v, *g_w, *g_eta;  // current guess (physical units)
+  const double *m_u, *m_v;                       // winds of the metric and kinematic terms
+  double *th_next, *b;                           // theta at n+1; packed right-hand side
+};
+
+// theta at n+1 from the current guess's w (steppers.py:911-919).
+__global__ void __launch_bounds__(256) thnext_kernel(TendCtx t, PredictorIO io) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= t.nm()) return;
+  const int i = p % t.nx, j = (p / t.nx) % t.ny, k = p / (t.nx * t.ny);
+  const double wm = 0.5 * (io.w_n[t.w(i, j, k)] + io.w_n[t.w(i, j, k + 1)]);
+  const double wmn = 0.5 * (io.g_w[t.w(i, j, k)] + io.g_w[t.w(i, j, k + 1)]);
+  const double tth1 = io.tth[p] + (wm - wmn) * t.dthm[p];
+  io.th_next[p] = (io.th_adv[p] + (t.alpha * t.dt) * tth1) + t.th_bg[p];
+}
+
+// Packed right-hand side u | v | w | pi * 1000 | eta of the implicit solve
+// (steppers.py:920-968): the nonlinear pressure-gradient and buoyancy
+// corrections with theta at n+1, the map-curvature terms of the current
+// winds, the kinematic bottom row and the theta' Exner flux of the guess.
+__global__ void __launch_bounds__(256) rhs_kernel(TendCtx t, PredictorIO io) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int sx = t.nx + 1, sy = t.ny + 1;
+  if (p >= sx * sy * (t.nz + 1)) return;
+  const int i = p % sx, j = (p / sx) % sy, k = p / (sx * sy);
+  const int nx = t.nx, ny = t.ny, nz = t.nz;
+  const double adt = t.alpha * t.dt;
+  const double cad = (t.cp * t.alpha) * t.dt;
+  const int Nu = t.nu(), Nv = t.nv(), Nw = t.nw();
+  const double *TN = io.th_next;
+  auto tp1 = [&](int q) { return TN[q] - t.th_bg[q]; };
+  if (j < ny && k < nz) {
+    int ia, ib;
+    face_pair(i, nx, t.px, ia, ib);
+    const int qu = t.u(i, j, k);
+    const double thu1 = 0.5 * (tp1(t.m(ia, j, k)) + tp1(t.m(ib, j, k)));
+    const double gx = ((io.g_pi[t.m(ib, j, k)] - io.g_pi[t.m(ia, j, k)]) * t.rdx) * t.mu[t.u2(i, j)];
+    const double gzu = 0.5 * (gzm_p(t, io.g_pi, ia, j, k) + gzm_p(t, io.g_pi, ib, j, k));
+    const double nlu = ((-cad) * thu1) * (gx - t.zxu[qu] * gzu);
+    const double vau = 0.5 * (vm_at(t, io.m_v, ia, j, k) + vm_at(t, io.m_v, ib, j, k));
+    const double mun = (0.5 * (curv_m(t, io.m_u, io.m_v, ia, j, k) + curv_m(t, io.m_u, io.m_v, ib, j, k))) * vau;
+    io.b[qu] = (io.rhs_u[qu] + nlu) + adt * mun;
+  }
+  if (i < nx && k < nz) {
+    int ja, jb;
+    face_pair(j, ny, t.py, ja, jb);
+    const int qv = t.v(i, j, k);
+    const double thv1 = 0.5 * (tp1(t.m(i, ja, k)) + tp1(t.m(i, jb, k)));
+    const double gy = ((io.g_pi[t.m(i, jb, k)] - io.g_pi[t.m(i, ja, k)]) * t.rdy) * t.mv[t.v2(i, j)];
+    const double gzv = 0.5 * (gzm_p(t, io.g_pi, i, ja, k) + gzm_p(t, io.g_pi, i, jb, k));
+    const double nlv = ((-cad) * thv1) * (gy - t.zev[qv] * gzv);
+    const double uav = 0.5 * (um_at(t, io.m_u, i, ja, k) + um_at(t, io.m_u, i, jb, k));
+    const double mvn =
+        (-(0.5 * (curv_m(t, io.m_u, io.m_v, i, ja, k) + curv_m(t, io.m_u, io.m_v, i, jb, k)))) * uav;
+    io.b[Nu + qv] = (io.rhs_v[qv] + nlv) + adt * mvn;
+  }
+  if (i < nx && j < ny) {
+    const int qw = t.w(i, j, k);
+    double bw;
+    if (k == 0) {  // kinematic bottom: m_w (u_w z_xi + v_w z_eta) of the current winds
+      const double uw = 0.5 * (um_at(t, io.m_u, i, j, 0) + um_at(t, io.m_u, i, j, 1));
+      const double vw = 0.5 * (vm_at(t, io.m_v, i, j, 0) + vm_at(t, io.m_v, i, j, 1));
+      bw = t.mm[t.m2(i, j)] * (uw * t.zxiw[qw] + vw * t.zetaw[qw]);
+    } else if (k == nz) {
+      bw = 0.0;
+    } else {
+      const double thw1 = 0.5 * (tp1(t.m(i, j, k - 1)) + tp1(t.m(i, j, k)));
+      const double buoy = adt * (t.grav * (thw1 / t.thw_bg[qw]));
+      const double nlw = ((-cad) * thw1) * gzw_p(t, io.g_pi, i, j, k);
+      bw = (io.rhs_w[qw] + buoy) + nlw;
+    }
+    io.b[Nu + Nv + qw] = bw;
+    // eta row: -(1 - alpha) times the advected kinematic residual, ends zero
+    const double re = (k == 0 || k == nz) ? 0.0 : (-(1.0 - t.alpha)) * io.reta[qw];
+    io.b[Nu + Nv + Nw + t.nm() + qw] = re;
+  }
+  if (i < nx && j < ny && k < nz) {
+    const int q = t.m(i, j, k);
+    auto thu = [&](int f) {
+      int ia, ib;
+      face_pair(f, nx, t.px, ia, ib);
+      return 0.5 * (tp1(t.m(ia, j, k)) + tp1(t.m(ib, j, k)));
+    };
+    auto thv = [&](int f) {
+      int ja, jb;
+      face_pair(f, ny, t.py, ja, jb);
+      return 0.5 * (tp1(t.m(i, ja, k)) + tp1(t.m(i, jb, k)));
+    };
+    const double nld = pi_div(
+        t, i, j, k, [&](int f) { return io.g_u[t.u(f, j, k)] * thu(f) * t.cxp[t.u(f, j, k)]; },
+        [&](int f) { return io.g_v[t.v(i, f, k)] * thv(f) * t.cyp[t.v(i, f, k)]; },
+        [&](int kk) {
+          if (kk == 0 || kk == nz) return 0.0;
+          const double tw_ = 0.5 * (tp1(t.m(i, j, kk - 1)) + tp1(t.m(i, j, kk)));
+          return io.g_eta[t.w(i, j, kk)] * tw_ * t.czp[t.w(i, j, kk)];
+        });
+    io.b[Nu + Nv + Nw + q] = (io.rhs_pi[q] + adt * nld) * 1000.0;
+  }
+}
+
+// ---- divergence damping and unscaling (steppers.py:1003-1035) ----------------------
+
+struct FinalIO {
+  const double *x;  // packed solution u | v | w | pi * 1000 | eta * dz_w
+  double *u, *v, *pi, *eta;
+};
+
+__global__ void __launch_bounds__(256) final_kernel(TendCtx t, FinalIO io) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int sx = t.nx + 1, sy = t.ny + 1;
+  if (p >= sx * sy * (t.nz + 1)) return;
+  const int i = p % sx, j = (p / sx) % sy, k = p / (sx * sy);
+  const int nx = t.nx, ny = t.ny, nz = t.nz;
+  const int Nu = t.nu(), Nv = t.nv(), Nw = t.nw(), N = t.nm();
+  const double *XU = io.x, *XV = io.x + Nu;
+  const double kd = t.nu_div * t.dt;
+  auto div = [&](int ii, int jj) {
+    const double mmv = t.mm[t.m2(ii, jj)];
+    return ((XU[t.u(ii + 1, jj, k)] - XU[t.u(ii, jj, k)]) * t.rdx) * mmv +
+           ((XV[t.v(ii, jj + 1, k)] - XV[t.v(ii, jj, k)]) * t.rdy) * mmv;
+  };
+  if (j < ny && k < nz) {
+    int ia, ib;
+    face_pair(i, nx, t.px, ia, ib);
+    const int qu = t.u(i, j, k);
+    const double dd = ((div(ib, j) - div(ia, j)) * t.rdx) * t.mu[t.u2(i, j)];
+    io.u[qu] = XU[qu] + kd * dd;
+  }
+  if (i < nx && k < nz) {
+    int ja, jb;
+    face_pair(j, ny, t.py, ja, jb);
+    const int qv = t.v(i, j, k);
+    const double dd = ((div(i, jb) - div(i, ja)) * t.rdy) * t.mv[t.v2(i, j)];
+    io.v[qv] = XV[qv] + kd * dd;
+  }
+  if (i < nx && j < ny) {
+    const int qw = t.w(i, j, k);
+    io.eta[qw] = io.x[Nu + Nv + Nw + N + qw] / t.dzw[qw];
+    if (k < nz) {
+      const int q = t.m(i, j, k);
+      io.pi[q] = io.x[Nu + Nv + Nw + q] / 1000.0 + t.pi_bg[q];
+    }
+  }
+}
+
+static bool make_ctx(const SislTendency *d, TendCtx &t) {
+  if (!d) return false;
+  const SislGrid &g = d->g;
+  if (g.nx < 2 || g.ny < 2 || g.nz < 2) return false;
+  if ((int64_t)(g.nx + 1) * (g.ny + 1) * (g.nz + 1) * 4 >= (int64_t(1) << 31)) return false;
+  t.nx = g.nx; t.ny = g.ny; t.nz = g.nz; t.px = g.periodic_x != 0; t.py = g.periodic_y != 0;
+  t.rdx = 1.0 / d->dx; t.rdy = 1.0 / d->dy; t.rdz = 1.0 / d->dz; t.dz = d->dz;
+  t.cp = d->cp; t.grav = d->grav; t.alpha = d->alpha; t.dt = d->dt; t.nu_h = d->nu_h; t.nu_div = d->nu_div;
+  t.mm = d->mm; t.mu = d->mu; t.mv = d->mv; t.fu = d->fu; t.fv = d->fv; t.dmdx = d->dmdx; t.dmdy = d->dmdy;
+  t.th_bg = d->th_bg; t.pi_bg = d->pi_bg; t.thu_bg = d->thu_bg; t.thv_bg = d->thv_bg; t.thw_bg = d->thw_bg;
+  t.zxu = d->zxu; t.zev = d->zev; t.zxm = d->zxm; t.zem = d->zem; t.rz = d->rz; t.zxiw = d->zxiw;
+  t.zetaw = d->zetaw; t.dzw = d->dzw; t.dzm = d->dzm; t.cpi = d->cpi; t.dthm = d->dthm; t.cx = d->cx;
+  t.cy = d->cy; t.cxp = d->cxp; t.cyp = d->cyp; t.czs = d->czs; t.czp = d->czp;
+  const void *ptrs[] = {t.mm, t.mu, t.mv, t.fu, t.fv, t.dmdx, t.dmdy, t.th_bg, t.pi_bg, t.thu_bg, t.thv_bg,
+                        t.thw_bg, t.zxu, t.zev, t.zxm, t.zem, t.rz, t.zxiw, t.zetaw, t.dzw, t.dzm, t.cpi,
+                        t.dthm, t.cx, t.cy, t.cxp, t.cyp, t.czs, t.czp};
+  for (const void *q : ptrs)
+    if (!q) return false;
+  return true;
+}
+
+static unsigned nblocks(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+}  // namespace sisl
+
+extern "C" {
+
+int64_t sisl_hyperdiffusion_workspace_f64(const SislGrid *g) {
+  if (!g) return -1;
+  const int64_t N = (int64_t)g->nx * g->ny * g->nz;
+  const int64_t Nu = (int64_t)(g->nx + 1) * g->ny * g->nz, Nv = (int64_t)g->nx * (g->ny + 1) * g->nz;
+  return 4 * (2 * N + Nu + Nv);  // elements: base fields, first Laplacian, Gx, Gy
+}
+
+int sisl_hyperdiffusion_f64(const SislTendency *d, const double *u, const double *v, const double *w,
+                            const double *th, double *work, double *lap2, void *stream) {
+  sisl::TendCtx t;
+  if (!sisl::make_ctx(d, t) || !u || !v || !w || !th || !work || !lap2) return SISL_ERR_ARG;
+  hipStream_t s = sisl::as_stream(stream);
+  const int64_t N = (int64_t)t.nx * t.ny * t.nz;
+  const int64_t Nu = (int64_t)(t.nx + 1) * t.ny * t.nz;
+  double *F = work, *L1 = work + 4 * N, *GX = work + 8 * N, *GY = GX + 4 * Nu;
+  const unsigned bm = sisl::nblocks(N), bf = sisl::nblocks((int64_t)(t.nx + 1) * (t.ny + 1) * t.nz);
+  hipLaunchKernelGGL(sisl::hyper_base_kernel, dim3(bm), dim3(sisl::kBlock), 0, s, t, u, v, w, th, F);
+  hipLaunchKernelGGL(sisl::hyper_faces_kernel, dim3(bf), dim3(sisl::kBlock), 0, s, t, 4, (const double *)F, GX, GY);
+  hipLaunchKernelGGL(sisl::hyper_mass_kernel, dim3(bm), dim3(sisl::kBlock), 0, s, t, 4, (const double *)GX,
+                     (const double *)GY, L1);
+  hipLaunchKernelGGL(sisl::hyper_faces_kernel, dim3(bf), dim3(sisl::kBlock), 0, s, t, 4, (const double *)L1, GX, GY);
+  hipLaunchKernelGGL(sisl::hyper_mass_kernel, dim3(bm), dim3(sisl::kBlock), 0, s, t, 4, (const double *)GX,
+                     (const double *)GY, lap2);
+  SISL_CHECK_LAUNCH();
+  return SISL_OK;
+}
+
+int sisl_explicit_f64(const SislTendency *d, const double *u, const double *v, const double *w, const double *pi,
+                      const double *th, const double *eta, const double *lap2, double *u_in, double *v_in,
+                      double *w_in, double *pi_in, double *th_in, double *res, double *tth, double *x0,
+                      double *tend, void *stream) {
+  sisl::TendCtx t;
+  if (!sisl::make_ctx(d, t) || !u || !v || !w || !pi || !th || !eta || !u_in || !v_in || !w_in || !pi_in ||
+      !th_in || !res || !tth)
+    return SISL_ERR_ARG;
+  if (!lap2 && t.nu_h != 0.0) return SISL_ERR_ARG;
+  sisl::ExplicitIO io{u, v, w, pi, th, eta, t.nu_h != 0.0 ? lap2 : nullptr, u_in, v_in, w_in, pi_in, th_in,
+                      res, tth, x0, tend};
+  const int64_t n = (int64_t)(t.nx + 1) * (t.ny + 1) * (t.nz + 1);
+  hipLaunchKernelGGL(sisl::explicit_kernel, dim3(sisl::nblocks(n)), dim3(sisl::kBlock), 0, sisl::as_stream(stream),
+                     t, io);
+  SISL_CHECK_LAUNCH();
+  return SISL_OK;
+}
+
+int sisl_predictor_f64(const SislTendency *d, const double *rhs_u, const double *rhs_v, const double *rhs_w,
+                       const double *rhs_pi, const double *th_adv, const double *tth, const double *w_n,
+                       const double *reta, const double *g_pi, const double *g_u, const double *g_v,
+                       const double *g_w, const double *g_eta, const double *m_u, const double *m_v,
+                       double *th_next, double *b, void *stream) {
+  sisl::TendCtx t;
+  if (!sisl::make_ctx(d, t)) return SISL_ERR_ARG;
+  const void *req[] = {rhs_u, rhs_v, rhs_w, rhs_pi, th_adv, tth, w_n, reta, g_pi, g_u, g_v, g_w, g_eta, m_u, m_v,
+                       th_next, b};
+  for (const void *q : req)
+    if (!q) return SISL_ERR_ARG;
+  sisl::PredictorIO io{rhs_u, rhs_v, rhs_w, rhs_pi, th_adv, tth, w_n, reta, g_pi, g_u, g_v, g_w, g_eta,
+                       m_u, m_v, th_next, b};
+  hipStream_t s = sisl::as_stream(stream);
+  const int64_t N = (int64_t)t.nx * t.ny * t.nz;
+  const int64_t n = (int64_t)(t.nx + 1) * (t.ny + 1) * (t.nz + 1);
+  hipLaunchKernelGGL(sisl::thnext_kernel, dim3(sisl::nblocks(N)), dim3(sisl::kBlock), 0, s, t, io);
+  hipLaunchKernelGGL(sisl::rhs_kernel, dim3(sisl::nblocks(n)), dim3(sisl::kBlock), 0, s, t, io);
+  SISL_CHECK_LAUNCH();
+  return SISL_OK;
+}
+
+int sisl_final_f64(const SislTendency *d, const double *x, double *u, double *v, double *pi, double *eta,
+                   void *stream) {
+  sisl::TendCtx t;
+  if (!sisl::make_ctx(d, t) || !x || !u || !v || !pi || !eta) return SISL_ERR_ARG;
+  sisl::FinalIO io{x, u, v, pi, eta};
+  const int64_t n = (int64_t)(t.nx + 1) * (t.ny + 1) * (t.nz + 1);
+  hipLaunchKernelGGL(sisl::final_kernel, dim3(sisl::nblocks(n)), dim3(sisl::kBlock), 0, sisl::as_stream(stream),
+                     t, io);
+  SISL_CHECK_LAUNCH();
+  return SISL_OK;
+}
+
+}  // extern "C"

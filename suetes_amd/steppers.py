@@ -290,10 +290,13 @@ def _autotune(ops):
     dev = ops.tensors["cua"].device
     x = torch.randn(n, dtype=ops.dtype, device=dev)
     y = torch.empty_like(x)
+    fp32 = int(ops.dtype == torch.float32)
+    nb = int(L.sisl_autotune_scratch_bytes(C.byref(g), fp32))
+    scr = torch.empty(max(nb, 1), dtype=torch.uint8, device=dev)
     kc, un = C.c_int32(), C.c_int32()
-    _lib.check(L.sisl_autotune_operator(C.byref(ops.struct), int(ops.dtype == torch.float32), x.data_ptr(),
-                                        y.data_ptr(), C.byref(kc), C.byref(un), _lib.stream_ptr(dev)),
-               "operator autotune")
+    _lib.check(L.sisl_autotune_operator(C.byref(ops.struct), fp32, x.data_ptr(), y.data_ptr(),
+                                        scr.data_ptr() if nb > 0 else None, C.byref(kc), C.byref(un),
+                                        _lib.stream_ptr(dev)), "operator autotune")
     _TUNED.add(key)
     ops.tuned = (kc.value, un.value)
 

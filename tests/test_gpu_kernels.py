@@ -365,9 +365,13 @@ def test_operator_launch_shapes_bit_identical_and_autotune(dtype, transpose):
     for y, sh in zip(outs[1:], shapes[1:]):
         assert torch.equal(y, outs[0]), (sh, (y - outs[0]).abs().max().item())
     kc, un = C.c_int32(), C.c_int32()
-    scratch = torch.empty_like(x)
-    _lib.check(L.sisl_autotune_operator(C.byref(ops.struct), fp32, x.data_ptr(), scratch.data_ptr(),
-                                        C.byref(kc), C.byref(un), _lib.stream_ptr()), "autotune")
+    out = torch.empty_like(x)
+    nb = int(L.sisl_autotune_scratch_bytes(C.byref(g), fp32))
+    assert nb == (256 + 2 * x.numel() if fp32 else 0)
+    tune_scr = torch.empty(max(nb, 1), dtype=torch.uint8, device=x.device)
+    _lib.check(L.sisl_autotune_operator(C.byref(ops.struct), fp32, x.data_ptr(), out.data_ptr(),
+                                        tune_scr.data_ptr() if nb else None, C.byref(kc), C.byref(un),
+                                        _lib.stream_ptr()), "autotune")
     assert kc.value >= 2 and un.value in (1, 2)
     assert torch.equal(apply(), outs[0])
 
