@@ -30,7 +30,8 @@ EXPORTED = [
     "sisl_stagger_T_f64", "sisl_stagger_T_f32", "sisl_tricubic_adj_frac_f64",
     "sisl_tricubic_adj_frac_f32", "sisl_tricubic_adj_rows_f64", "sisl_tricubic_adj_rows_f32",
     "sisl_ffsl_1d_f64", "sisl_ffsl_1d_f32", "sisl_ffsl_1d_T_f64", "sisl_ffsl_1d_T_f32",
-    "sisl_set_graphs", "sisl_ffsl_1d_f64_noalloc",
+    "sisl_set_graphs", "sisl_ffsl_1d_f64_noalloc", "sisl_hyperdiffusion_workspace_f64",
+    "sisl_hyperdiffusion_f64", "sisl_explicit_f64", "sisl_predictor_f64", "sisl_final_f64",
 ]
 
 # timed kernels (enum SislTimedKernel)

@@ -281,6 +281,17 @@ class Ensemble:
             out.append({"u": d["u"], "v": d["v"], "w": d["w"], "pi": d["pi"] / ps, "eta_dot": d["eta_dot"] / dzw})
         return out
 
+    def _batched_solve_packed(self, items):
+        """The members' solves on packed scaled vectors (the fused step's
+        form): one batched solve, packed solutions back."""
+        self._prepare([it[2] for it in items])
+        for m in range(len(items)):
+            if torch.is_grad_enabled() and _operator_inputs(self.steppers[m].implicit_solver):
+                raise NotImplementedError("ensemble solves do not propagate operator-input cotangents")
+        bs = [it[0].contiguous() for it in items]
+        x0s = [it[1].contiguous() for it in items]
+        return list(_BatchedSolve.apply(self, *bs, *x0s))
+
     # -- batched trajectories and gathers ------------------------------------------
     def _batched_departure(self, items):
         loc = items[0][1]
@@ -338,14 +349,18 @@ class Ensemble:
 
             def solve(rhs_prime, bg, x0=None, m=m):
                 return rdv(m, "solve", (rhs_prime, bg, x0), self._batched_solve)
+
+            def solve_packed(b, x0, bg, m=m):
+                return rdv(m, "solve_packed", (b, x0, bg), self._batched_solve_packed)
             adv.compute_departure_indices = dep
             adv.advect_cubic = cub
             sol.solve = solve
+            sol.solve_packed = solve_packed
 
     def _uninstall(self):
         for stp in self.steppers:
             for obj, name in ((stp.advector, "compute_departure_indices"), (stp.advector, "advect_cubic"),
-                              (stp.implicit_solver, "solve")):
+                              (stp.implicit_solver, "solve"), (stp.implicit_solver, "solve_packed")):
                 obj.__dict__.pop(name, None)
 
     def step_internal(self, states, t, forcings, bc_fns):

@@ -540,6 +540,15 @@ class SemiImplicitSolver3D:
                                     "ptol": st.ptol})
         return x
 
+    def solve_packed(self, b, x0, bg_precomputed):
+        """The same solve on packed scaled vectors (u | v | w | pi 1000 | eta
+        and the initial guess with eta scaled by dz_w), returning the packed
+        scaled solution; autograd as :meth:`solve`."""
+        self.prepare(bg_precomputed)
+        deps = _operator_inputs(self) if torch.is_grad_enabled() else []
+        where = tuple(w for w, _ in deps)
+        return _GmresSolve.apply(self, b, x0, where, *[t for _, t in deps])
+
     def solve(self, rhs_prime, bg_precomputed, x0=None):
         """Internal dicts in, internal dict out (steppers.py:401-496)."""
         self.prepare(bg_precomputed)
@@ -645,6 +654,11 @@ class SISLStepper3D:
         self._ext_cache = None
         self.record = None
         self.record_coords = None
+        # run the explicit tendencies, hyperdiffusion, predictor-corrector
+        # right-hand sides and divergence damping as the fused HIP kernels
+        # (csrc/tendency.hip) where they apply; False: torch ops throughout
+        self.fused_glue = True
+        self._glue = None
 
     # -- public API (reference shapes) -----------------------------------------
     def step(self, state, t, forcing, bc_fn, ml_params=None):
@@ -750,6 +764,176 @@ class SISLStepper3D:
             traj = state
         return {loc: self.advector.compute_departure_indices(traj, loc) for loc in ("u", "v", "w", "m")}
 
+    # -- fused glue ----------------------------------------------------------------
+    def _like(self):
+        g = self.physics.grid
+        nx, ny, nz = g.nx, g.ny, g.nz
+        return {"u": torch.empty(nz, ny, nx + 1, device="meta"), "v": torch.empty(nz, ny + 1, nx, device="meta"),
+                "w": torch.empty(nz + 1, ny, nx, device="meta"), "pi": torch.empty(nz, ny, nx, device="meta"),
+                "eta_dot": torch.empty(nz + 1, ny, nx, device="meta")}
+
+    def _fused_applies(self, state, bg):
+        """The fused kernels cover steps without a physics suite (tracers come
+        with one), with FP64 device state and with a background and grid
+        that carry no gradient (terrain controls run the torch ops, whose
+        reverse mode reaches the metrics)."""
+        if not self.fused_glue or self.physics.physics_suite is not None:
+            return False
+        u = state["u"]
+        if not (torch.is_tensor(u) and u.is_cuda and u.dtype == torch.float64):
+            return False
+        if any(k in state for k in self.tracer_keys):
+            return False
+        self.implicit_solver.prepare(bg)
+        return not _operator_inputs(self.implicit_solver)
+
+    def _fused(self, bg):
+        from .glue import FusedGlue
+        if self._glue is None or self._glue.bg is not bg:
+            self._glue = FusedGlue(self, bg)
+        return self._glue
+
+    def _explicit_stage_torch(self, u, v, w, pi, th, eta):
+        """Torch restatement of the explicit stage (steppers.py:776-842):
+        (u_in, v_in, w_in, pi_in, th_in, res_n, theta tendency)."""
+        ph, op, g = self.physics, self.physics.op, self.physics.grid
+        a, dt = self.alpha, self.dt
+        bg = self.bg_precomputed()
+        thp_n = th - ph.i_theta_bg
+        sp_n = {"u": u, "v": v, "w": w, "th_v": th, "pi": pi - ph.i_pi_bg, "eta_dot": eta,
+                "th_v_prime_u": op.avg(thp_n, 0, "m", "u"), "th_v_prime_v": op.avg(thp_n, 1, "m", "v"),
+                "th_v_prime_w": op.avg(thp_n, 2, "m", "w")}
+        tn = ph.get_tendencies(sp_n, bg, True, None)
+        u_in = u + dt * ((1.0 - a) * tn["u"] + a * tn.get("phys_diff_u", 0.0))
+        v_in = v + dt * ((1.0 - a) * tn["v"] + a * tn.get("phys_diff_v", 0.0))
+        w_in = w + dt * ((1.0 - a) * tn["w"] + a * tn.get("phys_diff_w", 0.0))
+        nl_n = ph.nonlinear_pi_tendency(sp_n, thp_n, bg)
+        pi_in = sp_n["pi"] + (1.0 - a) * dt * (tn["pi"] + nl_n)
+        th_in = thp_n + (1.0 - a) * dt * tn["th_v"]
+        uw = op.avg(op.avg(u, 0, "u", "m"), 2, "m", "w")
+        vw = op.avg(op.avg(v, 1, "v", "m"), 2, "m", "w")
+        res_n = bg["dz_w_full"] * eta + uw * g.i_z_xi_w + vw * g.i_z_eta_w - w
+        return u_in, v_in, w_in, pi_in, th_in, res_n, tn["th_v"]
+
+    def _predictor_torch(self, rhs_u, rhs_v, rhs_w, rhs_pi, th_adv, tth, w_n, reta, g_pi, g_u, g_v, g_w,
+                         g_eta, m_u, m_v):
+        """Torch restatement of one predictor-corrector pass
+        (steppers.py:911-968): (packed right-hand side, theta at n+1)."""
+        ph, op, g, c = self.physics, self.physics.op, self.physics.grid, self.physics.c
+        a, dt = self.alpha, self.dt
+        bg = self.bg_precomputed()
+        th_bg = ph.i_theta_bg
+        dthw = op.diff(bg["th_v"], 2, "m", "w") * (g.dz / bg["dz_w_full"])
+        dthm = op.avg(dthw, 2, "w", "m")
+        w_m = op.avg(w_n, 2, "w", "m")
+        w_m_next = op.avg(g_w, 2, "w", "m")
+        zxu = op.diff(g.i_Z_m, 0, "m", "u")
+        zev = op.diff(g.i_Z_m, 1, "m", "v")
+        cad = c["cp"] * a * dt
+        tth1 = tth + (w_m - w_m_next) * dthm
+        th_next = th_adv + a * dt * tth1 + th_bg
+        thp1 = th_next - th_bg
+        thu1 = op.avg(thp1, 0, "m", "u")
+        thv1 = op.avg(thp1, 1, "m", "v")
+        thw1 = op.avg(thp1, 2, "m", "w")
+        buoy = a * dt * (c["g"] * (thw1 / bg["th_v_w"]))
+        gx = op.diff(g_pi, 0, "m", "u")
+        gy = op.diff(g_pi, 1, "m", "v")
+        gzw = op.diff(g_pi, 2, "m", "w") * (g.dz / bg["dz_w_full"])
+        gzm = op.avg(gzw, 2, "w", "m")
+        gzu = op.avg(gzm, 0, "m", "u")
+        gzv = op.avg(gzm, 1, "m", "v")
+        nlu = -cad * thu1 * (gx - zxu * gzu)
+        nlv = -cad * thv1 * (gy - zev * gzv)
+        nlw = -cad * thw1 * gzw
+        guess = {"u": g_u, "v": g_v, "w": g_w, "pi": g_pi, "eta_dot": g_eta}
+        nld = ph.nonlinear_pi_tendency(guess, thp1, bg)
+        mun, mvn = ph.metric_curvature_tendencies(m_u, m_v)
+        rhs_u_k = rhs_u + nlu + a * dt * mun
+        rhs_v_k = rhs_v + nlv + a * dt * mvn
+        rhs_w_k = rhs_w + buoy + nlw
+        rhs_pi_k = rhs_pi + a * dt * nld
+        uwg = op.avg(op.avg(m_u, 0, "u", "m"), 2, "m", "w")
+        vwg = op.avg(op.avg(m_v, 1, "v", "m"), 2, "m", "w")
+        kb = g.i_m_factors["w"] * (uwg[0] * g.i_z_xi_w[0] + vwg[0] * g.i_z_eta_w[0])
+        rhs_w_k = _set_slice(_set_slice(rhs_w_k, 0, 0, kb[None]), 0, -1, torch.zeros_like(kb[None]))
+        R_eta_adv = _zero_ends(-(1.0 - a) * reta, 0)
+        ps = self.implicit_solver.pi_scale
+        b = pack({"u": rhs_u_k, "v": rhs_v_k, "w": rhs_w_k, "pi": rhs_pi_k * ps, "eta_dot": R_eta_adv})
+        return b, th_next
+
+    def _final_torch(self, x):
+        """Torch restatement of the unscaling and divergence damping
+        (steppers.py:1003-1035): (u, v, pi, eta) from the packed solution."""
+        ph, op = self.physics, self.physics.op
+        bg = self.bg_precomputed()
+        d = unpack(x, self._like())
+        u, v = d["u"], d["v"]
+        pi = d["pi"] / self.implicit_solver.pi_scale
+        eta = d["eta_dot"] / bg["dz_w_full"]
+        div = op.diff(u, 0, "u", "m") + op.diff(v, 1, "v", "m")
+        u = u + ph.nu_div * self.dt * op.diff(div, 0, "m", "u")
+        v = v + ph.nu_div * self.dt * op.diff(div, 1, "m", "v")
+        return u, v, pi + ph.i_pi_bg, eta
+
+    def _advected_winds(self, u_in, v_in, co, state):
+        """Advected u, v with the lateral faces of the reference (steppers.py:
+        853-870): locked to the old state on closed faces, face n repeating
+        face 0 on periodic axes."""
+        op, lim, cub = self.physics.op, self.use_limiter, self.advector.advect_cubic
+        rhs_u = cub(u_in, co["u"], lim)
+        rhs_v = cub(v_in, co["v"], lim)
+        if 0 in op.periodic_axes:
+            rhs_u = _set_slice(rhs_u, 2, -1, rhs_u[:, :, :1])
+        else:
+            rhs_u = _set_slice(_set_slice(rhs_u, 2, 0, state["u"][:, :, :1]), 2, -1, state["u"][:, :, -1:])
+        if 1 in op.periodic_axes:
+            rhs_v = _set_slice(rhs_v, 1, -1, rhs_v[:, :1])
+        else:
+            rhs_v = _set_slice(_set_slice(rhs_v, 1, 0, state["v"][:, :1]), 1, -1, state["v"][:, -1:])
+        return rhs_u, rhs_v
+
+    def _step_fused(self, state, co, bg, forcing, bc_fn):
+        """The step with its glue on the fused kernels (same arithmetic as the
+        torch path of :meth:`step_internal`)."""
+        from .glue import ExplicitFn, FinalFn, PredictorFn
+        ph, c = self.physics, self.physics.c
+        glue = self._fused(bg)
+        u_in, v_in, w_in, pi_in, th_in, res_n, tth, x = ExplicitFn.apply(
+            glue, state["u"], state["v"], state["w"], state["pi"], state["th_v"], state["eta_dot"])
+        lim, cub = self.use_limiter, self.advector.advect_cubic
+        reta = cub(res_n, co["w"], lim)
+        rhs_u, rhs_v = self._advected_winds(u_in, v_in, co, state)
+        rhs_w = cub(w_in, co["w"], lim)
+        rhs_pi = cub(pi_in, co["m"], lim)
+        th_adv = cub(th_in, co["m"], lim)
+        if self._rho_transport_observed(bc_fn, state):
+            rho_next = self.ffsl_advector.advect_3d_split(state["rho"], state, bg)
+        else:
+            rho_next = state["rho"]  # overwritten by the equation of state below
+        g_u, g_v, g_w, g_eta = state["u"], state["v"], state["w"], state["eta_dot"]
+        g_pi = state["pi"] - ph.i_pi_bg
+        m_u, m_v = rhs_u, rhs_v
+        solver, like = self.implicit_solver, self._like()
+        dzw = bg["dz_w_full"]
+        for _ in range(2):
+            b, th_next = PredictorFn.apply(glue, rhs_u, rhs_v, rhs_w, rhs_pi, th_adv, tth, state["w"], reta,
+                                           g_pi, g_u, g_v, g_w, g_eta, m_u, m_v)
+            x = solver.solve_packed(b, x, bg)
+            d = unpack(x, like)
+            g_u, g_v, g_w = d["u"], d["v"], d["w"]
+            g_pi, g_eta = d["pi"] / solver.pi_scale, d["eta_dot"] / dzw
+            m_u, m_v = g_u, g_v
+        u_o, v_o, pi_o, eta_o = FinalFn.apply(glue, x)
+        out = {"u": u_o, "v": v_o, "w": g_w, "pi": pi_o, "rho": rho_next, "th_v": th_next, "eta_dot": eta_o}
+        out = self._call_bc(bc_fn, out, forcing)
+        out["rho"] = c["p0"] / (c["Rd"] * out["th_v"]) * (out["pi"] ** (c["cvd"] / c["Rd"]))
+        out["u_prev"], out["v_prev"], out["w_prev"] = state["u"], state["v"], state["w"]
+        out["eta_dot_prev"] = state["eta_dot"]
+        out["tend_th_v_prev"] = tth
+        out["is_first_step"] = 0.0
+        return out
+
     def step_internal(self, state, t, forcing, bc_fn, ml_params=None, coords=None):
         """One step on an internal dict.  ``coords`` (from
         :meth:`departure_points`) replays stored departure points, e.g. when a
@@ -764,6 +948,8 @@ class SISLStepper3D:
         if self.record_coords is not None:
             self.record_coords.append(co)
         bg = self.bg_precomputed()
+        if self._fused_applies(state, bg):
+            return self._step_fused(state, co, bg, forcing, bc_fn)
         th_bg, pi_bg = ph.i_theta_bg, ph.i_pi_bg
         thp_n = state["th_v"] - th_bg
         sp_n = {"u": state["u"], "v": state["v"], "w": state["w"], "th_v": state["th_v"],
