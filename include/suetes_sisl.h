@@ -142,6 +142,25 @@ int sisl_tricubic_adj_rows_f32(int32_t fnx, int32_t fny, int32_t fnz, const int3
                                const int32_t *start, const float *frac, const float *g,
                                float *g_sorted, float *gf, void *stream);
 
+/* Adjoint of the LIMITED gather (use_limiter = 1; operators.py:188-207,
+ * clip to the chained min / max of the 8 cell corners), with the balanced
+ * tie rule of JAX's minimum / maximum (half the cotangent to each operand
+ * that equals the result).  split: per output point, the interpolant's
+ * share gi[n] (to be transposed by sisl_tricubic_adj_rows_* / _adj_*) and
+ * the corners' shares gc[8 n] (corner q = ox + 2 oy + 4 oz at stride n);
+ * corners: gf += the corner shares gathered per cell over the plan
+ * (deterministic). */
+int sisl_tricubic_limited_adj_split_f64(int32_t fnx, int32_t fny, int32_t fnz, const double *field,
+                                        const double *coords, const double *gout, double *gi, double *gc,
+                                        void *stream);
+int sisl_tricubic_limited_adj_split_f32(int32_t fnx, int32_t fny, int32_t fnz, const float *field,
+                                        const float *coords, const float *gout, float *gi, float *gc,
+                                        void *stream);
+int sisl_tricubic_limited_adj_corners_f64(int32_t fnx, int32_t fny, int32_t fnz, const int32_t *perm,
+                                          const int32_t *start, const double *gc, double *gf, void *stream);
+int sisl_tricubic_limited_adj_corners_f32(int32_t fnx, int32_t fny, int32_t fnz, const int32_t *perm,
+                                          const int32_t *start, const float *gc, float *gf, void *stream);
+
 /* _get_index_velocities + compute_departure_indices
  * (suetes/regional3d/steppers.py:276-360): departure points at staggering
  * loc (0=m, 1=u, 2=v, 3=w) from trajectory winds u, v, eta_dot, written as

@@ -32,6 +32,8 @@ EXPORTED = [
     "sisl_ffsl_1d_f64", "sisl_ffsl_1d_f32", "sisl_ffsl_1d_T_f64", "sisl_ffsl_1d_T_f32",
     "sisl_set_graphs", "sisl_ffsl_1d_f64_noalloc", "sisl_hyperdiffusion_workspace_f64",
     "sisl_hyperdiffusion_f64", "sisl_explicit_f64", "sisl_predictor_f64", "sisl_final_f64",
+    "sisl_tricubic_limited_adj_split_f64", "sisl_tricubic_limited_adj_split_f32",
+    "sisl_tricubic_limited_adj_corners_f64", "sisl_tricubic_limited_adj_corners_f32",
 ]
 
 # timed kernels (enum SislTimedKernel)
@@ -92,6 +94,8 @@ def lib():
             getattr(L, f"sisl_tricubic_adj_{p}").argtypes = [i32, i32, i32, vp, vp, vp, vp, vp, vp]
             getattr(L, f"sisl_tricubic_adj_frac_{p}").argtypes = [i32, i32, i32, vp, vp, vp, vp]
             getattr(L, f"sisl_tricubic_adj_rows_{p}").argtypes = [i32, i32, i32, vp, vp, vp, vp, vp, vp, vp]
+            getattr(L, f"sisl_tricubic_limited_adj_split_{p}").argtypes = [i32, i32, i32, vp, vp, vp, vp, vp, vp]
+            getattr(L, f"sisl_tricubic_limited_adj_corners_{p}").argtypes = [i32, i32, i32, vp, vp, vp, vp, vp]
             getattr(L, f"sisl_departure_{p}").argtypes = [C.POINTER(SislGrid), i32, vp, vp, vp, vp,
                                                           d, d, d, i32, vp, vp, vp]
             getattr(L, f"sisl_helmholtz_apply_{p}").argtypes = [C.POINTER(SislOperator), vp, vp, vp]

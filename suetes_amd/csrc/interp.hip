@@ -899,9 +899,159 @@ int departure_batched(int M, const SislGrid *g, int loc, const T *const *u, cons
   return SISL_OK;
 }
 
+// ---- adjoint of the limited gather ------------------------------------------------
+// The limiter is clip(f, lo, hi) = min(max(f, lo), hi) with lo / hi the
+// chained binary minimum / maximum of the 8 cell corners in the reference's
+// order (operators.py:188-207).  JAX differentiates a binary min / max with
+// the balanced rule: the cotangent goes to the operand equal to the result,
+// split in half when both are.  Per output point the split kernel recomputes
+// the unclipped interpolant and both chains and writes the interpolant's
+// share of the cotangent (gi, transposed by the unlimited adjoint) and the
+// eight corners' shares (gc, corner q = ox + 2 oy + 4 oz at stride n); the
+// corner kernel gathers the corner shares per destination cell over the
+// plan's base-cell buckets in a fixed order (deterministic).
+
+// d min(a, b) / da and / db, and the same for max, with balanced ties
+template <typename T>
+__device__ __forceinline__ void dmin(T a, T b, T &da, T &db) {
+  da = a < b ? (T)1 : (a == b ? (T)0.5 : (T)0);
+  db = b < a ? (T)1 : (a == b ? (T)0.5 : (T)0);
+}
+template <typename T>
+__device__ __forceinline__ void dmax(T a, T b, T &da, T &db) {
+  da = a > b ? (T)1 : (a == b ? (T)0.5 : (T)0);
+  db = b > a ? (T)1 : (a == b ? (T)0.5 : (T)0);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) tricubic_limit_split_kernel(Dims d, const T *__restrict__ f,
+                                                                   const T *__restrict__ co,
+                                                                   const T *__restrict__ g,
+                                                                   T *__restrict__ gi, T *__restrict__ gc) {
+  const int64_t n = d.n();
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int nx = d.nx, ny = d.ny, nz = d.nz;
+  const int X = (int)(p % nx), Y = (int)((p / nx) % ny), Z = (int)(p / ((int64_t)nx * ny));
+  T tx, ty, tz;
+  const int x0 = offset_split(X, co[p], nx, tx);
+  const int y0 = offset_split(Y, co[n + p], ny, ty);
+  const int z0 = offset_split(Z, co[2 * n + p], nz, tz);
+  int ix[4], iy[4], iz[4];
+  for (int q = 0; q < 4; ++q) {
+    ix[q] = min(max(x0 - 1 + q, 0), nx - 1);
+    iy[q] = min(max(y0 - 1 + q, 0), ny - 1);
+    iz[q] = min(max(z0 - 1 + q, 0), nz - 1);
+  }
+  auto F = [&](int a, int b, int c) { return f[d.at(ix[a], iy[b], iz[c])]; };
+  const T fi = tricubic_point(tx, ty, tz, 0, F);
+  T v[8], cmin[8], cmax[8];
+  for (int q = 0; q < 8; ++q) v[q] = F(1 + (q & 1), 1 + ((q >> 1) & 1), 1 + ((q >> 2) & 1));
+  cmin[0] = cmax[0] = v[0];
+  for (int q = 1; q < 8; ++q) {
+    cmin[q] = v[q] < cmin[q - 1] ? v[q] : cmin[q - 1];
+    cmax[q] = v[q] > cmax[q - 1] ? v[q] : cmax[q - 1];
+  }
+  const T lo = cmin[7], hi = cmax[7];
+  const T m = fi < lo ? lo : fi;  // max(f, lo)
+  T dm_f, dm_lo, do_m, do_hi;
+  dmax(fi, lo, dm_f, dm_lo);
+  dmin(m, hi, do_m, do_hi);
+  const T gp = g[p];
+  const T gm = gp * do_m;
+  gi[p] = gm * dm_f;
+  T glo = gm * dm_lo, ghi = gp * do_hi;
+  T gv[8];
+  for (int q = 7; q >= 1; --q) {  // back through the chains
+    T a_, b_, c_, e_;
+    dmin(cmin[q - 1], v[q], a_, b_);
+    dmax(cmax[q - 1], v[q], c_, e_);
+    gv[q] = glo * b_ + ghi * e_;
+    glo *= a_;
+    ghi *= c_;
+  }
+  gv[0] = glo + ghi;
+  for (int q = 0; q < 8; ++q) gc[q * n + p] = gv[q];
+}
+
+// (base, corner offset) pairs whose clamped corner is cell i on an axis of n cells
+__device__ __forceinline__ int corner_candidates(int i, int n, int *b, int *o) {
+  int c = 0;
+  b[c] = i; o[c] = 0; ++c;
+  if (i >= 1) { b[c] = i - 1; o[c] = 1; ++c; }
+  if (i == n - 1) { b[c] = i; o[c] = 1; ++c; }  // x1 = min(x0 + 1, n - 1) = x0
+  return c;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) tricubic_limit_corner_kernel(Dims d, const int32_t *__restrict__ perm,
+                                                                    const int32_t *__restrict__ start,
+                                                                    const T *__restrict__ gc,
+                                                                    T *__restrict__ gf) {
+  const int64_t n = d.n();
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int X = (int)(p % d.nx), Y = (int)((p / d.nx) % d.ny), Z = (int)(p / ((int64_t)d.nx * d.ny));
+  int bx[3], ox[3], by[3], oy[3], bz[3], oz[3];
+  const int cx = corner_candidates(X, d.nx, bx, ox);
+  const int cy = corner_candidates(Y, d.ny, by, oy);
+  const int cz = corner_candidates(Z, d.nz, bz, oz);
+  T acc = 0;
+  for (int c = 0; c < cz; ++c)
+    for (int b = 0; b < cy; ++b)
+      for (int a = 0; a < cx; ++a) {
+        const int64_t key = d.at(bx[a], by[b], bz[c]);
+        const int q = ox[a] + 2 * oy[b] + 4 * oz[c];
+        const int32_t s = start[key], e = start[key + 1];
+        for (int32_t r = s; r < e; ++r) acc += gc[q * n + perm[r]];
+      }
+  gf[p] += acc;
+}
+
+template <typename T>
+int tricubic_limited_split(int fnx, int fny, int fnz, const T *f, const T *co, const T *g, T *gi, T *gc,
+                           void *st) {
+  if (!f || !co || !g || !gi || !gc || fnx < 1 || fny < 1 || fnz < 1) return SISL_ERR_ARG;
+  Dims d{fnx, fny, fnz};
+  if (8 * d.n() >= ((int64_t)1 << 31)) return SISL_ERR_ARG;
+  hipLaunchKernelGGL(tricubic_limit_split_kernel<T>, dim3(grid_for(d.n())), dim3(kBlock), 0, as_stream(st), d, f,
+                     co, g, gi, gc);
+  SISL_CHECK_LAUNCH();
+  return SISL_OK;
+}
+
+template <typename T>
+int tricubic_limited_corners(int fnx, int fny, int fnz, const int32_t *perm, const int32_t *start, const T *gc,
+                             T *gf, void *st) {
+  if (!perm || !start || !gc || !gf || fnx < 1 || fny < 1 || fnz < 1) return SISL_ERR_ARG;
+  Dims d{fnx, fny, fnz};
+  if (8 * d.n() >= ((int64_t)1 << 31)) return SISL_ERR_ARG;
+  hipLaunchKernelGGL(tricubic_limit_corner_kernel<T>, dim3(grid_for(d.n())), dim3(kBlock), 0, as_stream(st), d,
+                     perm, start, gc, gf);
+  SISL_CHECK_LAUNCH();
+  return SISL_OK;
+}
+
 }  // namespace sisl
 
 extern "C" {
+
+int sisl_tricubic_limited_adj_split_f64(int32_t a, int32_t b, int32_t c, const double *field, const double *co,
+                                        const double *gout, double *gi, double *gc, void *s) {
+  return sisl::tricubic_limited_split<double>(a, b, c, field, co, gout, gi, gc, s);
+}
+int sisl_tricubic_limited_adj_split_f32(int32_t a, int32_t b, int32_t c, const float *field, const float *co,
+                                        const float *gout, float *gi, float *gc, void *s) {
+  return sisl::tricubic_limited_split<float>(a, b, c, field, co, gout, gi, gc, s);
+}
+int sisl_tricubic_limited_adj_corners_f64(int32_t a, int32_t b, int32_t c, const int32_t *perm,
+                                          const int32_t *start, const double *gc, double *gf, void *s) {
+  return sisl::tricubic_limited_corners<double>(a, b, c, perm, start, gc, gf, s);
+}
+int sisl_tricubic_limited_adj_corners_f32(int32_t a, int32_t b, int32_t c, const int32_t *perm,
+                                          const int32_t *start, const float *gc, float *gf, void *s) {
+  return sisl::tricubic_limited_corners<float>(a, b, c, perm, start, gc, gf, s);
+}
 
 int sisl_tricubic_batched_f64(int32_t M, const int32_t *dims, const double *const *f,
                               const double *const *co, double *const *out, int32_t lim, void *s) {

@@ -269,21 +269,52 @@ def _tricubic_adjoint(g, data, share, tok):
     return gf
 
 
+def _limited_adjoint(g, field, data, share, tok):
+    """Adjoint of one limited gather: the interpolant's share of the
+    cotangent through the unlimited transpose, the 8 corners' shares
+    gathered per cell over the same plan (both deterministic)."""
+    g = g.contiguous()
+    fz, fy, fx = g.shape
+    L = _lib.lib()
+    sfx = "f64" if g.dtype == torch.float64 else "f32"
+    gi = torch.empty_like(g)
+    gc = torch.empty((8,) + tuple(g.shape), dtype=g.dtype, device=g.device)
+    s = _lib.stream_ptr(g.device)
+    _lib.check(getattr(L, f"sisl_tricubic_limited_adj_split_{sfx}")(
+        fx, fy, fz, field.data_ptr(), data.data_ptr(), g.data_ptr(), gi.data_ptr(), gc.data_ptr(), s),
+        "limited tricubic adjoint")
+    co = Coords(data, share)
+    gf = tricubic_transpose(gi, co)
+    plan = co.plan()
+    _lib.check(getattr(L, f"sisl_tricubic_limited_adj_corners_{sfx}")(
+        fx, fy, fz, plan.perm.data_ptr(), plan.start.data_ptr(), gc.data_ptr(), gf.data_ptr(), s),
+        "limited tricubic adjoint corners")
+    share.done(tok)
+    return gf
+
+
 class _Tricubic(torch.autograd.Function):
     @staticmethod
     def forward(ctx, field, coords_obj, use_limiter):
         # the offsets go through save_for_backward (a checkpointed step drops
-        # them after its forward pass); only the small plan holder stays
-        ctx.save_for_backward(coords_obj.data)
+        # them after its forward pass); only the small plan holder stays.
+        # The limited gather's adjoint also needs the field (its clip
+        # branches and corner extrema).
+        field = field.contiguous()
+        if use_limiter:
+            ctx.save_for_backward(coords_obj.data, field)
+        else:
+            ctx.save_for_backward(coords_obj.data)
         ctx.share = coords_obj._share
         ctx.tok = ctx.share.register()
         ctx.use_limiter = use_limiter
-        return _tricubic_fwd(field.contiguous(), coords_obj.data, use_limiter)
+        return _tricubic_fwd(field, coords_obj.data, use_limiter)
 
     @staticmethod
     def backward(ctx, g):
         if ctx.use_limiter:
-            raise NotImplementedError("the adjoint of the limited tricubic is not implemented")
+            data, field = ctx.saved_tensors
+            return _limited_adjoint(g, field, data, ctx.share, ctx.tok), None, None
         (data,) = ctx.saved_tensors
         return _tricubic_adjoint(g, data, ctx.share, ctx.tok), None, None
 
