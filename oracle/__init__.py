@@ -20,4 +20,8 @@ Parity status per function:
   JAX is not installed and its source is not vendored.
 * DaviesSponge float32 masks: the cos() rounding of XLA is parity unpinned
   (the oracle and the product both use NumPy float32).
+* ``split_explicit``: the split-explicit RK3 core (steppers.py:1076-1737),
+  pinned by the reference's own regressions (theta' Exner flux, closed-box
+  budgets), the upwind flux on cell averages of polynomials, the coupled
+  off-centred column equations and a resting atmosphere.
 """

@@ -1010,7 +1010,12 @@ class Stepper:
 
 def build_dynamical_core(core_type, grid, operators, constants, initial_state,
                          physics_suite=None, interior_mask=None, gs_mode='jax', **kwargs):
-    """build_dynamical_core('sisl', ...) (steppers.py:12-88)."""
+    """build_dynamical_core('sisl', ...) (steppers.py:12-88); 'split-explicit'
+    goes to oracle.split_explicit (steppers.py:95-119)."""
+    if core_type.lower() == 'split-explicit':
+        from .split_explicit import build_split_explicit
+        return build_split_explicit(grid, operators, constants, initial_state, physics_suite=physics_suite,
+                                    interior_mask=interior_mask, **kwargs)
     if core_type.lower() != 'sisl':
         raise ValueError(f"Unknown core_type: '{core_type}'. Use 'sisl' or 'split-explicit'.")
     p = {'dt': 120.0, 'damp_height': 9000.0, 'max_damp': 3.0, 'nu_div_factor': 0.1,
