@@ -352,6 +352,48 @@ int sisl_predictor_f64(const SislTendency *t, const double *rhs_u, const double 
 int sisl_final_f64(const SislTendency *t, const double *x, double *u, double *v, double *pi, double *eta,
                    void *stream);
 
+/* ---- split-explicit core: acoustic substeps (FP64) -------------------- */
+
+/*
+ * Static fields of the acoustic substeps of SplitExplicitStepper3D
+ * (suetes/regional3d/steppers.py:1472-1737), precomputed once per stepper
+ * from the reference background; layouts as in SislTendency.
+ */
+typedef struct SislAcoustic {
+  SislGrid g;
+  double dx, dy, dz;      /* grid spacings                                    */
+  double cp, alpha;       /* constants['cp'], acoustic off-centring            */
+  double nu_div;          /* divergence-damping coefficient                   */
+  const double *mu, *mv;  /* [u2] [v2] map factors at u / v faces             */
+  const double *mm, *mw;  /* [m2] map factors at mass points / w columns      */
+  const double *thu, *thv, *thw; /* [u] [v] [w] background theta averages     */
+  const double *zxu, *zev;  /* [u] [v] terrain slopes (differences of Z_m)    */
+  const double *rz;         /* [w] dz / dz_w_full                             */
+  const double *dzm;        /* [m] dz_m_full                                  */
+  const double *cpi;        /* [m] C_pi                                       */
+  const double *cx, *cy;    /* [u] [v] rho th dz / m (horizontal Exner flux)  */
+  const double *rtw;        /* [w] rho_w th_w (vertical Exner flux)           */
+  const double *zxiw, *zetaw; /* [w] z_xi_w, z_eta_w                          */
+  const double *tau;        /* [w] Rayleigh damping rate                      */
+} SislAcoustic;
+
+/* Elements of the work buffer of sisl_acoustic_loop_f64 (-1: bad grid). */
+int64_t sisl_acoustic_workspace_f64(const SislGrid *g);
+/* SplitExplicitStepper3D._acoustic_loop (steppers.py:1472-1651) with
+ * _solve_vertical_acoustic_column (steppers.py:1653-1737): nsteps substeps
+ * of length dtau on the deviation (u, v, w, pi) = (u'', v'', w'', pi'') from
+ * the stage state (uc, vc, wc), updated in place.  su, sv, sw, sp: slow
+ * tendencies of the stage; ub, vb: the deviation's values at the start of
+ * the loop (its lateral boundary faces); lower, cprime, invden [m]: Thomas
+ * factors of the column matrix for this dtau (sub-diagonal, c'_k and
+ * 1 / (main_k - lower_k c'_{k-1})).  work: sisl_acoustic_workspace_f64(g)
+ * elements.  Needs nx, ny, nz >= 2. */
+int sisl_acoustic_loop_f64(const SislAcoustic *a, int32_t nsteps, double dtau, const double *lower,
+                           const double *cprime, const double *invden, double *u, double *v, double *w,
+                           double *pi, const double *uc, const double *vc, const double *wc, const double *su,
+                           const double *sv, const double *sw, const double *sp, const double *ub,
+                           const double *vb, double *work, void *stream);
+
 /* ---- flux-form semi-Lagrangian transport ------------------------------- */
 
 /* FluxFormAdvector.advect_1d (suetes/regional3d/steppers.py:552-594) along

@@ -58,10 +58,11 @@ def to_device_state(st_np, device):
 
 
 def build_case(cfg, device="cuda", precision="fp64", physics_suite=None, extra_fields=None,
-               **core_overrides):
+               core_type="sisl", **core_overrides):
     """Grid, stepper, initial state and boundary function of workload
     ``cfg`` on ``device``.  ``extra_fields``: reference-shaped NumPy arrays
-    added to the initial state (e.g. tracers of ``physics_suite``)."""
+    added to the initial state (e.g. tracers of ``physics_suite``);
+    ``core_type``: 'sisl' or 'split-explicit'."""
     if isinstance(cfg, str):
         cfg = WORKLOADS[cfg]()
     gk = cfg["grid"]
@@ -77,7 +78,7 @@ def build_case(cfg, device="cuda", precision="fp64", physics_suite=None, extra_f
     core = dict(cfg["core"])
     core["precision"] = precision
     core.update(core_overrides)
-    stepper, dt = build_dynamical_core("sisl", grid, op, CONSTANTS, state, physics_suite=physics_suite,
+    stepper, dt = build_dynamical_core(core_type, grid, op, CONSTANTS, state, physics_suite=physics_suite,
                                        **core)
     bc = identity_bc
     if cfg["boundary"] is not None and cfg["boundary"][0] == "davies":

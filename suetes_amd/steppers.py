@@ -70,7 +70,8 @@ def build_dynamical_core(core_type, grid, operators, constants, initial_state, p
     or 'iterated') of the mixed solver, ``solver_early_exit``."""
     core = core_type.lower()
     if core == "split-explicit":
-        raise NotImplementedError("the split-explicit core is not part of this build")
+        return _build_split_explicit(grid, operators, constants, initial_state, physics_suite, interior_mask,
+                                     **kwargs)
     if core != "sisl":
         raise ValueError(f"Unknown core_type: '{core_type}'. Use 'sisl' or 'split-explicit'.")
     p = {"dt": 120.0, "damp_height": 9000.0, "max_damp": 3.0, "nu_div_factor": 0.1,
@@ -104,6 +105,25 @@ def build_dynamical_core(core_type, grid, operators, constants, initial_state, p
                             inner_gram_schmidt=p["inner_gram_schmidt"],
                             solver_early_exit=p["solver_early_exit"])
     return stepper, p["dt"]
+
+
+def _build_split_explicit(grid, operators, constants, initial_state, physics_suite, interior_mask, **kwargs):
+    """The 'split-explicit' branch of the factory (steppers.py:95-119): its
+    own defaults; keys of the SISL branch are ignored as the reference's
+    dict update ignores them, except a precision other than FP64, which this
+    core does not have."""
+    from .split_explicit import SplitExplicitStepper3D
+    p = {"dt": 40.0, "ns": 3, "damp_height": 12000.0, "max_damp": 3.0, "nu_div_factor": 0.03,
+         "nu_h_factor": 0.03, "N_bv": 0.01, "alpha": 0.55, "background_state": None}
+    p.update(kwargs)
+    if p.get("precision", "fp64") != "fp64":
+        raise ValueError("the split-explicit core runs in FP64 only")
+    ref = p["background_state"] if p["background_state"] is not None else initial_state
+    physics = Euler3D(grid, operators, constants, dt=p["dt"], initial_era5_state=ref,
+                      damp_height=p["damp_height"], max_damp=p["max_damp"],
+                      nu_div_factor=p["nu_div_factor"], nu_h_factor=p["nu_h_factor"], N_bv=p["N_bv"],
+                      physics_suite=physics_suite, interior_mask=interior_mask)
+    return SplitExplicitStepper3D(physics, dt=p["dt"], ns=p["ns"], alpha=p["alpha"]), p["dt"]
 
 
 # ---------------------------------------------------------------------------

@@ -16,7 +16,7 @@ def oracle_transform(spec):
             "stretched_sleve": O.StretchedSleveSimple}[kind](**kw)
 
 
-def oracle_case(cfg, physics_suite=None, extra_fields=None):
+def oracle_case(cfg, physics_suite=None, extra_fields=None, core_type="sisl"):
     gk = cfg["grid"]
     terrain = W.make_terrain(cfg["terrain"], gk)
     g = O.Grid(gk["nx"], gk["ny"], gk["nz"], gk["dx"], gk["dy"], gk["dz"], gk["lat_center"],
@@ -25,7 +25,7 @@ def oracle_case(cfg, physics_suite=None, extra_fields=None):
     st = W.initial_state(cfg, g.Z_m, g.x_m, g.y_m)
     st.update({k: v.copy() for k, v in (extra_fields or {}).items()})
     ext = {k: v.copy() for k, v in st.items()}
-    stepper, dt = O.build_dynamical_core("sisl", g, op, C, st, physics_suite=physics_suite, **cfg["core"])
+    stepper, dt = O.build_dynamical_core(core_type, g, op, C, st, physics_suite=physics_suite, **cfg["core"])
     bc = (lambda s, f: s)
     if cfg["boundary"] is not None and cfg["boundary"][0] == "davies":
         sp = O.DaviesSponge(g, op, **cfg["boundary"][1])

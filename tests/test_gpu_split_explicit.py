@@ -1,0 +1,195 @@
+"""GPU parity of the split-explicit core (SplitExplicitStepper3D,
+steppers.py:1076-1737, with the acoustic loop on csrc/acoustic.hip) against
+the FP64 oracle, and its reverse mode.
+
+Bars (FP64 throughout, this core has no mixed path):
+* full steps vs the oracle, rel-L2 <= 1e-12 per field after 2-3 steps: the
+  limited-area case (terrain, SLEVE levels, Davies rim, open lateral faces,
+  divergence damping, Rayleigh damping) and the periodic warm bubble.  Over
+  terrain w and the contravariant velocity eta_dot dz_w are differences of
+  terms of the wind's size (the kinematic condition w = u z_x + v z_y +
+  eta_dot dz_w), 35-125x larger than themselves in these cases, so their
+  rounding is measured against the wind: |err| / |(u, v, w)| <= 1e-12 (their
+  plain rel-L2 stays below 1e-11);
+* the HIP acoustic loop vs its torch restatement on the device: 1e-13;
+* the reference's closed-box regression on the device (density and passive
+  tracer mass to 1e-10, rho q equal to the FFSL transport to 1e-12,
+  tests/test_core_formula_regressions.py:238-292);
+* reverse mode (the loop's backward re-evaluates the torch restatement):
+  central differences of the HIP forward along a random direction of the
+  initial theta, relative 1e-6.
+"""
+import numpy as np
+import pytest
+import torch
+
+from tests.helpers import oracle_case, rel_l2, small_bubble, small_lam
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda"
+FIELDS = ("u", "v", "w", "pi", "th_v", "rho", "eta_dot")
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+def _lam(nx=20, ny=18, nz=12):
+    cfg = small_lam(nx=nx, ny=ny, nz=nz)
+    cfg["core"] = dict(cfg["core"], dt=4.0, ns=4, nu_div_factor=0.05, nu_h_factor=0.05, damp_height=5000.0,
+                       max_damp=0.2)
+    return cfg
+
+
+def _bubble():
+    cfg = small_bubble()
+    cfg["core"] = dict(cfg["core"], dt=0.5, ns=5)
+    return cfg
+
+
+@pytest.mark.parametrize("name,cfg_fn,steps", [("lam 20x18x12", _lam, 3),
+                                               ("lam 48x40x20", lambda: _lam(48, 40, 20), 2),
+                                               ("bubble 16^3 periodic", _bubble, 3)])
+def test_split_explicit_step_matches_oracle(name, cfg_fn, steps):
+    from suetes_amd.cases import build_case
+    print(f"[case] split-explicit {name}: {steps} steps vs the FP64 oracle", flush=True)
+    cfg = cfg_fn()
+    go, opo, sto, s_o, bc_o, ext_o = oracle_case(cfg, core_type="split-explicit")
+    case = build_case(cfg, device=DEV, core_type="split-explicit")
+    assert case.stepper.fused_acoustic
+    s_p = case.state
+    dzw = go.dz_w_full
+    with torch.no_grad():
+        for i in range(steps):
+            s_o = sto.step(s_o, i * case.dt, ext_o, bc_o)
+            s_p = case.stepper.step(s_p, i * case.dt, case.forcing, case.bc_fn)
+            errs = {k: rel_l2(_np(s_p[k]), s_o[k]) for k in FIELDS}
+            wind = np.sqrt(sum(np.linalg.norm(s_o[k]) ** 2 for k in ("u", "v", "w")))
+            vs_wind = {"w": np.linalg.norm(_np(s_p["w"]) - s_o["w"]) / wind,
+                       "eta_dot dz_w": np.linalg.norm((_np(s_p["eta_dot"]) - s_o["eta_dot"]) * dzw) / wind}
+            print(f"  step {i + 1}: " + " ".join(f"{k} {e:.1e}" for k, e in errs.items())
+                  + " | vs the wind: " + " ".join(f"{k} {e:.1e}" for k, e in vs_wind.items()), flush=True)
+            assert max(e for k, e in errs.items() if k not in ("w", "eta_dot")) < 1e-12, errs
+            assert max(vs_wind.values()) < 1e-12 and max(errs["w"], errs["eta_dot"]) < 1e-11, (errs, vs_wind)
+    assert case.stepper._kern is not None  # the HIP loop ran
+
+
+def test_acoustic_kernels_match_torch_restatement():
+    from suetes_amd.cases import build_case
+    print("[case] split-explicit lam 20x18x12: HIP acoustic loop vs torch restatement", flush=True)
+    cfg = _lam()
+    out = {}
+    for fused in (True, False):
+        case = build_case(cfg, device=DEV, core_type="split-explicit")
+        case.stepper.fused_acoustic = fused
+        with torch.no_grad():
+            out[fused] = case.stepper.integrate(case.state, 0.0, 2, case.forcing, case.bc_fn)
+    for k in FIELDS:
+        e = rel_l2(_np(out[True][k]), _np(out[False][k]))
+        print(f"  {k}: {e:.1e}", flush=True)
+        assert e < 1e-13, (k, e)
+
+
+class Tracer:
+    tracer_keys = ["q_test"]
+
+    def get_explicit_tendencies(self, *a, **k):
+        return {}
+
+    def apply_state_updates(self, s):
+        return s
+
+
+def test_split_explicit_closed_box_budgets_on_device():
+    # tests/test_core_formula_regressions.py:238-292 through the product
+    from suetes_amd import geometry as G
+    from suetes_amd.euler import Euler3D
+    from suetes_amd.operators import CGridOperator3D
+    from suetes_amd.split_explicit import SplitExplicitStepper3D
+    from suetes_amd.cases import identity_bc
+    print("[case] split-explicit closed box 8x8x5: density and tracer budgets", flush=True)
+    C = {"g": 9.81, "cp": 1004.0, "Rd": 287.0, "cvd": 717.0, "p0": 100000.0}
+    dt = 0.02
+    g = G.RegionalGrid3D(8, 8, 5, 2000.0, 2000.0, 500.0, 45.0, 0.0, device=DEV)
+    op = CGridOperator3D(g, ())
+    ph = Euler3D(g, op, C, dt=dt, N_bv=0.0, damp_height=1e9, max_damp=0.0, nu_div_factor=0.0, nu_h_factor=0.0,
+                 physics_suite=Tracer())
+    kw = dict(dtype=torch.float64, device=DEV)
+    nx, ny, nz = 8, 8, 5
+    iu = torch.arange(nx + 1, **kw)[None, None, :]
+    jv = torch.arange(ny + 1, **kw)[None, :, None]
+    x = torch.arange(nx, **kw)[None, None, :]
+    y = torch.arange(ny, **kw)[None, :, None]
+    bgr = ph.bg_reference()
+    s = {"u": (2.0 * torch.sin(np.pi * iu / nx)).expand(nz, ny, nx + 1).contiguous(),
+         "v": (-1.5 * torch.sin(np.pi * jv / ny)).expand(nz, ny + 1, nx).contiguous(),
+         "w": torch.zeros(nz + 1, ny, nx, **kw), "eta_dot": torch.zeros(nz + 1, ny, nx, **kw),
+         "pi": bgr["pi"].clone(), "th_v": bgr["th_v"].clone(), "rho": bgr["rho"].clone()}
+    s["q_test"] = (1.0 + 0.2 * torch.sin(2 * np.pi * x / nx) * torch.cos(2 * np.pi * y / ny)).expand(nz, ny, nx) \
+        .contiguous()
+    vol = g.dx * g.dy * g.i_dz_m_full / g.i_m_factors["m"][None] ** 2
+    m0, t0 = float((s["rho"] * vol).sum()), float((s["rho"] * s["q_test"] * vol).sum())
+    st = SplitExplicitStepper3D(ph, dt, ns=2, alpha=0.5)
+    with torch.no_grad():
+        tm = st.ffsl_advector.advect_3d_split(s["rho"] * s["q_test"], s, st.bg_precomputed())
+        r = st.step_internal(dict(s), 0.0, None, identity_bc)
+    m1, t1 = float((r["rho"] * vol).sum()), float((r["rho"] * r["q_test"] * vol).sum())
+    local = float((r["rho"] * r["q_test"] - tm).norm() / tm.norm())
+    print(f"  rho drift {abs(m1 - m0) / m0:.1e} tracer drift {abs(t1 - t0) / t0:.1e} local {local:.1e}",
+          flush=True)
+    assert abs(m1 - m0) / m0 < 1e-10 and abs(t1 - t0) / t0 < 1e-10 and local < 1e-12
+    assert st._kern is not None
+
+
+def test_split_explicit_reverse_mode_matches_central_differences():
+    from suetes_amd.cases import build_case
+    from suetes_amd.steppers import state_to_internal
+    print("[case] split-explicit lam 20x18x12: adjoint of 2 steps vs FD in theta", flush=True)
+    cfg = _lam()
+    case = build_case(cfg, device=DEV, core_type="split-explicit")
+    case.op.use_stop_grad = False
+    st = case.stepper
+    s0 = state_to_internal(case.state)
+    gen = torch.Generator(device="cpu").manual_seed(11)
+    wgt = torch.randn(s0["w"].shape, generator=gen, dtype=torch.float64).to(DEV)
+
+    def J(th):
+        s = dict(s0)
+        s["th_v"] = th
+        for i in range(2):
+            s = st.step_internal(s, i * case.dt, case.forcing, case.bc_fn)
+        return (s["w"] * wgt).sum() + (s["u"] ** 2).mean()
+    th0 = s0["th_v"].detach() + 0.2 * torch.randn(s0["th_v"].shape, generator=gen, dtype=torch.float64).to(DEV)
+    d = torch.randn(th0.shape, generator=gen, dtype=torch.float64).to(DEV)
+    th = th0.clone().requires_grad_(True)
+    (g,) = torch.autograd.grad(J(th), th)
+    dJ = float((g * d).sum())
+    h = 1e-4
+    with torch.no_grad():
+        fd = (float(J(th0 + h * d)) - float(J(th0 - h * d))) / (2 * h)
+    print(f"  adjoint {dJ:.12e} fd {fd:.12e} rel {abs(fd - dJ) / abs(dJ):.1e}", flush=True)
+    assert abs(fd - dJ) <= 1e-6 * abs(dJ)
+
+
+def test_split_explicit_factory_defaults_and_precision_guard():
+    from suetes_amd import geometry as G
+    from suetes_amd.operators import CGridOperator3D
+    from suetes_amd.steppers import build_dynamical_core
+    from suetes_amd.split_explicit import SplitExplicitStepper3D
+    print("[case] split-explicit factory defaults", flush=True)
+    C = {"g": 9.81, "cp": 1004.0, "Rd": 287.0, "cvd": 717.0, "p0": 100000.0}
+    g = G.RegionalGrid3D(6, 5, 4, 1000.0, 1000.0, 500.0, 45.0, 0.0, device=DEV)
+    op = CGridOperator3D(g, ())
+    st, dt = build_dynamical_core("split-explicit", g, op, C, _rest(g), solver_tol=1e-9)
+    assert isinstance(st, SplitExplicitStepper3D) and dt == 40.0 and st.ns == 3 and st.alpha == 0.55
+    assert (st.ns_stage1, st.ns_stage2, st.ns_stage3) == (1, 2, 3)
+    ph = st.physics
+    assert abs(ph.nu_div - 0.03 * 1000.0 ** 2 / (4 * 40.0)) < 1e-12
+    with pytest.raises(ValueError):
+        build_dynamical_core("split-explicit", g, op, C, _rest(g), precision="mixed")
+
+
+def _rest(g):
+    from suetes_amd.cases import analytic_background_torch
+    th, pi = analytic_background_torch(g.i_Z_m, 0.01)
+    return {"th_v": th.permute(2, 1, 0), "pi": pi.permute(2, 1, 0)}
