@@ -434,6 +434,59 @@ def literal_variant(args):
             "max_rel_l2_mixed_vs_fp64_by_step": errs, "fp64_max_abs_u_at_end": umax}
 
 
+def acoustic_substep_bytes(nx, ny, nz):
+    """Algorithmic HBM bytes of one acoustic substep (csrc/acoustic.hip, FP64):
+    every array each of the four kernels must read or write once.
+    predictor: pi, rz, 5 u-face and 5 v-face inputs, 2 + 2 outputs;
+    column_rhs: 4 mass, 7 w-level, 2 + 2 face inputs, 2 outputs;
+    column: 5 mass inputs, 2 outputs; corrector: 2 mass, 7 + 7 face and
+    5 w-level inputs, 3 outputs."""
+    nm, nu, nv, nw = nx * ny * nz, (nx + 1) * ny * nz, nx * (ny + 1) * nz, nx * ny * (nz + 1)
+    return 8 * (15 * nm + 17 * nu + 17 * nv + 15 * nw)
+
+
+def split_explicit_leg(args, steps=3, warmup=1):
+    """The split-explicit core (steppers.py:1076-1737) on the limited-area
+    grid (dt 6 s, ns 6: 2 + 3 + 6 acoustic substeps of 1 s per step): FP64
+    ms per step, and the acoustic substep (four kernels behind one C call)
+    timed with HIP events on its stream against its algorithmic bytes."""
+    import torch
+    from suetes_amd import workloads as W
+    from suetes_amd.cases import build_case
+    from suetes_amd.steppers import state_to_internal
+    cfg = W.lam(nx=args.nx, ny=args.nx, nz=args.nz)
+    cfg["core"] = dict(cfg["core"], dt=6.0, ns=6)
+    case = build_case(cfg, device="cuda", core_type="split-explicit")
+    log("split-explicit: timing")
+    t, s = time_steps(case, steps, warmup)
+    st = case.stepper
+    kern = st._kern
+    ist = state_to_internal(s)
+    with torch.no_grad():
+        bg = st.bg_precomputed()
+        ist["eta_dot"], ist["w"] = st._diagnose_eta_dot(ist)
+        slow = st._compute_slow_tendencies(ist, bg, {}, {}, None)
+        args_loop = (ist["u"], ist["v"], ist["w"], ist["pi"], ist["u"], ist["v"], ist["w"],
+                     slow["u"], slow["v"], slow["w"], slow["pi"])
+        n = 20
+        dev0 = [torch.zeros_like(x) for x in args_loop[:4]]
+        kern.loop(st.dtau_stage3, 2, *dev0, *args_loop[4:])
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        kern.loop(st.dtau_stage3, n, *dev0, *args_loop[4:])
+        e1.record()
+        torch.cuda.synchronize()
+    us = e0.elapsed_time(e1) * 1e3 / n
+    nb = acoustic_substep_bytes(args.nx, args.nx, args.nz)
+    log(f"split-explicit {t*1e3:.1f} ms/step; acoustic substep {us:.0f} us, {nb / us / 1e3:.0f} GB/s")
+    return {"workload": f"lam {args.nx}x{args.nx}x{args.nz}, split-explicit dt 6 s, ns 6 (2+3+6 substeps), "
+                        "FP64", "ms_per_step": t * 1e3, "steps": steps, "warmup": warmup,
+            "acoustic_substep": {"avg_us": us, "algorithmic_bytes": nb, "achieved_GBs": nb / us / 1e3,
+                                 "frac_of_hbm_peak": nb / us / 1e3 / HBM_PEAK_GBS,
+                                 "method": "HIP events around 20 substeps of one C call on its stream"},
+            "max_abs_w_at_end": float(s["w"].abs().max())}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -457,6 +510,10 @@ def main():
                     help="run the batched ensemble case instead (32 uneven members)")
     ap.add_argument("--members", type=int, default=32)
     ap.add_argument("--ens-steps", type=int, default=20)
+    ap.add_argument("--no-split-explicit", action="store_true",
+                    help="skip the split-explicit core's timing")
+    ap.add_argument("--split-explicit-only", action="store_true",
+                    help="only time the split-explicit core (the run its profiles come from)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only launch the operator kernel and a plain copy of known size "
                          "(the run that hardware-counter passes profile)")
@@ -476,6 +533,10 @@ def main():
         if args.nx == 256 and args.nz == 80:
             args.nx, args.nz = 128, 60
         terrain_gradient(args)
+        return
+    if args.split_explicit_only:
+        out = {"split_explicit": split_explicit_leg(args), "copy_bytes": copy_reference()}
+        print(json.dumps(out), flush=True)
         return
     cfg = W.lam(nx=args.nx, ny=args.nx, nz=args.nz)
     log(f"building workload lam {args.nx}x{args.nx}x{args.nz}")
@@ -524,6 +585,7 @@ def main():
     literal = None
     if args.literal_steps > 0:
         literal = literal_variant(args)
+    split = None if args.no_split_explicit else split_explicit_leg(args)
     cpu = None if args.no_cpu else cpu_baseline()
     npts = args.nx * args.nx * args.nz
     out = {
@@ -549,6 +611,7 @@ def main():
         "adjoint_forward_cost": adj,
         "roofline": roof, "kernels_mixed_step": table, "cpu_baseline": cpu,
         "literal_config": literal,
+        "split_explicit": split,
         "autotuned_operator_shape": {
             "fp32": getattr(cmx.stepper.implicit_solver.op32, "tuned", None),
             "fp64": getattr(c64.stepper.implicit_solver.op64, "tuned", None)},

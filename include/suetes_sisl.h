@@ -352,6 +352,20 @@ int sisl_predictor_f64(const SislTendency *t, const double *rhs_u, const double 
 int sisl_final_f64(const SislTendency *t, const double *x, double *u, double *v, double *pi, double *eta,
                    void *stream);
 
+/* SplitExplicitStepper3D._compute_slow_tendencies (suetes/regional3d/
+ * steppers.py:1358-1470) with _upwind_flux_3rd_order (steppers.py:1310-1356)
+ * on the stage state (u, v, w, pi, th, eta): the explicit tendencies as in
+ * sisl_explicit_f64 with the theta' Exner flux, plus -nu_h del^4 of the stage
+ * (lap2s) and of the step's start state (lap20) -- each an output of
+ * sisl_hyperdiffusion_f64, NULL to omit, t->nu_h their coefficient -- plus
+ * the advection of theta', u_m, v_m and w_m averaged to their points.
+ * eps_l: the flux magnitude below which the upwind bias is off (1e-12).
+ * adv: 4 N elements of work; tend: tu | tv | tw | tpi | tth
+ * (N_u + N_v + N_w + 2 N elements). */
+int sisl_slow_tendencies_f64(const SislTendency *t, const double *u, const double *v, const double *w,
+                             const double *pi, const double *th, const double *eta, const double *lap2s,
+                             const double *lap20, double eps_l, double *adv, double *tend, void *stream);
+
 /* ---- split-explicit core: acoustic substeps (FP64) -------------------- */
 
 /*

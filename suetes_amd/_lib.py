@@ -32,7 +32,7 @@ EXPORTED = [
     "sisl_ffsl_1d_f64", "sisl_ffsl_1d_f32", "sisl_ffsl_1d_T_f64", "sisl_ffsl_1d_T_f32",
     "sisl_set_graphs", "sisl_ffsl_1d_f64_noalloc", "sisl_hyperdiffusion_workspace_f64",
     "sisl_hyperdiffusion_f64", "sisl_explicit_f64", "sisl_predictor_f64", "sisl_final_f64",
-    "sisl_acoustic_workspace_f64", "sisl_acoustic_loop_f64",
+    "sisl_acoustic_workspace_f64", "sisl_acoustic_loop_f64", "sisl_slow_tendencies_f64",
     "sisl_tricubic_limited_adj_split_f64", "sisl_tricubic_limited_adj_split_f32",
     "sisl_tricubic_limited_adj_corners_f64", "sisl_tricubic_limited_adj_corners_f32",
 ]

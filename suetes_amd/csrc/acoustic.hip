@@ -24,8 +24,10 @@
 // closed boundary the first / last interior face value is repeated, at a
 // periodic one the face between the last and the first cell is used at
 // both ends; horizontal differences carry the map factor of the target
-// location.  Threads read their neighbours (the inputs of one launch stay in
-// L2 across neighbouring threads) in the internal x-fastest layout.
+// location.  Threads read their neighbours in the internal x-fastest layout;
+// the block grids are padded to a multiple of the 8 XCDs and renumbered so
+// that each XCD runs one contiguous slab of levels, and the rows and levels
+// a thread reads around its own are then in its XCD's L2.
 #include "common.h"
 
 namespace sisl {
@@ -94,7 +96,7 @@ __global__ void __launch_bounds__(256) ac_predictor_kernel(
     const double *__restrict__ UC, const double *__restrict__ VC, const double *__restrict__ SU,
     const double *__restrict__ SV, const double *__restrict__ UB, const double *__restrict__ VB,
     double *__restrict__ UH, double *__restrict__ VH, double *__restrict__ GX, double *__restrict__ GY) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int p = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   const int sx = t.nx + 1, sy = t.ny + 1;
   if (p >= sx * sy * t.nz) return;
   const int i = p % sx, j = (p / sx) % sy, k = p / (sx * sy);
@@ -173,7 +175,7 @@ __global__ void __launch_bounds__(256) ac_column_rhs_kernel(
     AcCtx t, const double *__restrict__ P, const double *__restrict__ W, const double *__restrict__ SW,
     const double *__restrict__ SP, const double *__restrict__ UH, const double *__restrict__ VH,
     double *__restrict__ WE, double *__restrict__ R) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int p = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (p >= t.nx * t.ny * t.nz) return;
   const int i = p % t.nx, j = (p / t.nx) % t.ny, k = p / (t.nx * t.ny);
   double we0, we1;
@@ -228,7 +230,7 @@ __global__ void __launch_bounds__(256) ac_corrector_kernel(
     const double *__restrict__ GX, const double *__restrict__ GY, const double *__restrict__ UB,
     const double *__restrict__ VB, const double *__restrict__ WE, double *__restrict__ U,
     double *__restrict__ V, double *__restrict__ W) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int p = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   const int sx = t.nx + 1, sy = t.ny + 1;
   if (p >= sx * sy * (t.nz + 1)) return;
   const int i = p % sx, j = (p / sx) % sy, k = p / (sx * sy);
@@ -323,15 +325,16 @@ extern "C" int sisl_acoustic_loop_f64(const SislAcoustic *a, int32_t nsteps, dou
   (void)nm;
   hipStream_t s = as_stream(stream);
   const int64_t n_faces = (int64_t)(g.nx + 1) * (g.ny + 1) * g.nz;
+  // block counts rounded up to a multiple of the XCD count (xcd_tile)
+  auto xgrid = [](int64_t n) { return dim3((grid_for(n) + kXcds - 1) / kXcds * kXcds); };
   for (int step = 0; step < nsteps; ++step) {
-    hipLaunchKernelGGL(ac_predictor_kernel, dim3(grid_for(n_faces)), dim3(kBlock), 0, s, t, pi, u, v, uc, vc, su,
-                       sv, ub, vb, UH, VH, GX, GY);
-    hipLaunchKernelGGL(ac_column_rhs_kernel, dim3(grid_for(nm)), dim3(kBlock), 0, s, t, pi, w, sw, sp, UH, VH,
-                       WE, R);
+    hipLaunchKernelGGL(ac_predictor_kernel, xgrid(n_faces), dim3(kBlock), 0, s, t, pi, u, v, uc, vc, su, sv, ub,
+                       vb, UH, VH, GX, GY);
+    hipLaunchKernelGGL(ac_column_rhs_kernel, xgrid(nm), dim3(kBlock), 0, s, t, pi, w, sw, sp, UH, VH, WE, R);
     hipLaunchKernelGGL(ac_column_kernel, dim3(grid_for((int64_t)g.nx * g.ny, 64)), dim3(64), 0, s, t, lower,
                        cprime, invden, R, pi);
-    hipLaunchKernelGGL(ac_corrector_kernel, dim3(grid_for(span)), dim3(kBlock), 0, s, t, pi, R, UH, VH, uc, vc,
-                       wc, su, sv, GX, GY, ub, vb, WE, u, v, w);
+    hipLaunchKernelGGL(ac_corrector_kernel, xgrid(span), dim3(kBlock), 0, s, t, pi, R, UH, VH, uc, vc, wc, su, sv,
+                       GX, GY, ub, vb, WE, u, v, w);
     SISL_CHECK_LAUNCH();
   }
   return SISL_OK;

@@ -322,6 +322,183 @@ __global__ void __launch_bounds__(256) explicit_kernel(TendCtx t, ExplicitIO io)
   }
 }
 
+// ---- slow tendencies of the split-explicit core (steppers.py:1358-1470) --------------
+//
+// adv_kernel: the flux-form advection terms adv(q) of theta', u_m, v_m and w_m
+// at mass points, with the third-order upwind-biased face values
+// (steppers.py:1310-1356: fourth-order centred part plus a twelfth of the third
+// difference signed by the face flux, the sign gated to 0 below eps_l; edge
+// padding on closed axes, wrap on periodic ones).  slow_kernel: the explicit
+// tendencies of the stage state (as explicit_kernel) plus the hyperdiffusion of
+// the stage and of the step's start state and the averaged advection terms.
+
+struct AdvIO {
+  const double *u, *v, *w, *th, *eta;
+  double eps_l;
+  double *adv;  // 4 mass fields: adv(theta'), adv(u_m), adv(v_m), adv(w_m)
+};
+
+__device__ __forceinline__ int pad_index(int idx, int n, int per) {
+  return per ? ((idx % n) + n) % n : min(max(idx, 0), n - 1);
+}
+__device__ __forceinline__ double upwind3(double flux, double qm2, double qm1, double q0, double qp1, double eps) {
+  const double q4 = (7.0 / 12.0) * (q0 + qm1) - (1.0 / 12.0) * (qp1 + qm2);
+  const double qb = (1.0 / 12.0) * ((qp1 - qm2) - 3.0 * (q0 - qm1));
+  const double sg = fabs(flux) < eps ? 0.0 : (flux > 0.0 ? 1.0 : (flux < 0.0 ? -1.0 : 0.0));
+  return flux * (q4 + sg * qb);
+}
+// the advected mass field F at (i, j, k): 0 theta', 1 u_m, 2 v_m, 3 w_m
+template <int F>
+__device__ __forceinline__ double adv_q(const TendCtx &t, const AdvIO &io, int i, int j, int k) {
+  if constexpr (F == 0) return io.th[t.m(i, j, k)] - t.th_bg[t.m(i, j, k)];
+  else if constexpr (F == 1) return 0.5 * (io.u[t.u(i, j, k)] + io.u[t.u(i + 1, j, k)]);
+  else if constexpr (F == 2) return 0.5 * (io.v[t.v(i, j, k)] + io.v[t.v(i, j + 1, k)]);
+  else return 0.5 * (io.w[t.w(i, j, k)] + io.w[t.w(i, j, k + 1)]);
+}
+template <int F>
+__device__ __forceinline__ double adv_flux_x(const TendCtx &t, const AdvIO &io, int f, int j, int k) {
+  const double U = io.u[t.u(f, j, k)] / t.mu[t.u2(f, j)];
+  return upwind3(U, adv_q<F>(t, io, pad_index(f - 2, t.nx, t.px), j, k),
+                 adv_q<F>(t, io, pad_index(f - 1, t.nx, t.px), j, k), adv_q<F>(t, io, pad_index(f, t.nx, t.px), j, k),
+                 adv_q<F>(t, io, pad_index(f + 1, t.nx, t.px), j, k), io.eps_l);
+}
+template <int F>
+__device__ __forceinline__ double adv_flux_y(const TendCtx &t, const AdvIO &io, int i, int f, int k) {
+  const double V = io.v[t.v(i, f, k)] / t.mv[t.v2(i, f)];
+  return upwind3(V, adv_q<F>(t, io, i, pad_index(f - 2, t.ny, t.py), k),
+                 adv_q<F>(t, io, i, pad_index(f - 1, t.ny, t.py), k), adv_q<F>(t, io, i, pad_index(f, t.ny, t.py), k),
+                 adv_q<F>(t, io, i, pad_index(f + 1, t.ny, t.py), k), io.eps_l);
+}
+template <int F>
+__device__ __forceinline__ double adv_flux_z(const TendCtx &t, const AdvIO &io, int i, int j, int kk) {
+  const double W = io.eta[t.w(i, j, kk)] * t.dzw[t.w(i, j, kk)];
+  return upwind3(W, adv_q<F>(t, io, i, j, pad_index(kk - 2, t.nz, 0)), adv_q<F>(t, io, i, j, pad_index(kk - 1, t.nz, 0)),
+                 adv_q<F>(t, io, i, j, pad_index(kk, t.nz, 0)), adv_q<F>(t, io, i, j, pad_index(kk + 1, t.nz, 0)),
+                 io.eps_l);
+}
+// adv(q) = -m^2 (ax - q divU + ay - q divV) - (az - q divW) at (i, j, k)
+template <int F>
+__device__ __forceinline__ double adv_term(const TendCtx &t, const AdvIO &io, int i, int j, int k) {
+  const double mmv = t.mm[t.m2(i, j)];
+  const double rzm = t.dz / t.dzm[t.m(i, j, k)];
+  const double q = adv_q<F>(t, io, i, j, k);
+  const double ax = ((adv_flux_x<F>(t, io, i + 1, j, k) - adv_flux_x<F>(t, io, i, j, k)) * t.rdx) * mmv;
+  const double ay = ((adv_flux_y<F>(t, io, i, j + 1, k) - adv_flux_y<F>(t, io, i, j, k)) * t.rdy) * mmv;
+  const double az = ((adv_flux_z<F>(t, io, i, j, k + 1) - adv_flux_z<F>(t, io, i, j, k)) * t.rdz) * rzm;
+  const double divU = ((io.u[t.u(i + 1, j, k)] / t.mu[t.u2(i + 1, j)] - io.u[t.u(i, j, k)] / t.mu[t.u2(i, j)]) *
+                       t.rdx) * mmv;
+  const double divV = ((io.v[t.v(i, j + 1, k)] / t.mv[t.v2(i, j + 1)] - io.v[t.v(i, j, k)] / t.mv[t.v2(i, j)]) *
+                       t.rdy) * mmv;
+  const double divW = ((io.eta[t.w(i, j, k + 1)] * t.dzw[t.w(i, j, k + 1)] -
+                        io.eta[t.w(i, j, k)] * t.dzw[t.w(i, j, k)]) * t.rdz) * rzm;
+  return (-(mmv * mmv)) * (((ax - q * divU) + ay) - q * divV) - (az - q * divW);
+}
+
+__global__ void __launch_bounds__(256) adv_kernel(TendCtx t, AdvIO io) {
+  const int p = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  const int N = t.nm();
+  if (p >= N) return;
+  const int i = p % t.nx, j = (p / t.nx) % t.ny, k = p / (t.nx * t.ny);
+  io.adv[p] = adv_term<0>(t, io, i, j, k);
+  io.adv[N + p] = adv_term<1>(t, io, i, j, k);
+  io.adv[2 * N + p] = adv_term<2>(t, io, i, j, k);
+  io.adv[3 * N + p] = adv_term<3>(t, io, i, j, k);
+}
+
+struct SlowIO {
+  const double *u, *v, *w, *pi, *th, *eta;
+  const double *lap2s, *lap20;  // del^4 of the stage / start state (4 mass fields each); null: omitted
+  const double *adv;            // output of adv_kernel
+  double *tend;                 // tu | tv | tw | tpi | tth
+};
+
+__global__ void __launch_bounds__(256) slow_kernel(TendCtx t, SlowIO io) {
+  const int p = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  const int sx = t.nx + 1, sy = t.ny + 1;
+  if (p >= sx * sy * (t.nz + 1)) return;
+  const int i = p % sx, j = (p / sx) % sy, k = p / (sx * sy);
+  const int nx = t.nx, ny = t.ny, nz = t.nz;
+  const double *U = io.u, *V = io.v, *W = io.w, *PI = io.pi, *TH = io.th, *E = io.eta;
+  const int Nu = t.nu(), Nv = t.nv(), Nw = t.nw(), N = t.nm();
+  const double *A = io.adv;
+  if (j < ny && k < nz) {
+    int ia, ib;
+    face_pair(i, nx, t.px, ia, ib);
+    const int qu = t.u(i, j, k);
+    const double gx = ((pip(t, PI, t.m(ib, j, k)) - pip(t, PI, t.m(ia, j, k))) * t.rdx) * t.mu[t.u2(i, j)];
+    const double gzu = 0.5 * (gzm_pi(t, PI, ia, j, k) + gzm_pi(t, PI, ib, j, k));
+    const double thu = t.thu_bg[qu] + 0.5 * (thp(t, TH, t.m(ia, j, k)) + thp(t, TH, t.m(ib, j, k)));
+    const double vau = 0.5 * (vm_at(t, V, ia, j, k) + vm_at(t, V, ib, j, k));
+    double tu = ((-t.cp) * thu) * (gx - t.zxu[qu] * gzu);
+    tu = tu + t.fu[t.u2(i, j)] * vau;
+    tu = tu + (0.5 * (curv_m(t, U, V, ia, j, k) + curv_m(t, U, V, ib, j, k))) * vau;
+    if (io.lap2s) tu = tu + 0.5 * (hyp(t, io.lap2s, 0, t.m(ia, j, k)) + hyp(t, io.lap2s, 0, t.m(ib, j, k)));
+    if (io.lap20) tu = tu + 0.5 * (hyp(t, io.lap20, 0, t.m(ia, j, k)) + hyp(t, io.lap20, 0, t.m(ib, j, k)));
+    tu = tu + 0.5 * (A[N + t.m(ia, j, k)] + A[N + t.m(ib, j, k)]);
+    io.tend[qu] = tu;
+  }
+  if (i < nx && k < nz) {
+    int ja, jb;
+    face_pair(j, ny, t.py, ja, jb);
+    const int qv = t.v(i, j, k);
+    const double gy = ((pip(t, PI, t.m(i, jb, k)) - pip(t, PI, t.m(i, ja, k))) * t.rdy) * t.mv[t.v2(i, j)];
+    const double gzv = 0.5 * (gzm_pi(t, PI, i, ja, k) + gzm_pi(t, PI, i, jb, k));
+    const double thv = t.thv_bg[qv] + 0.5 * (thp(t, TH, t.m(i, ja, k)) + thp(t, TH, t.m(i, jb, k)));
+    const double uav = 0.5 * (um_at(t, U, i, ja, k) + um_at(t, U, i, jb, k));
+    double tv = ((-t.cp) * thv) * (gy - t.zev[qv] * gzv);
+    tv = tv - t.fv[t.v2(i, j)] * uav;
+    tv = tv + (-(0.5 * (curv_m(t, U, V, i, ja, k) + curv_m(t, U, V, i, jb, k)))) * uav;
+    if (io.lap2s) tv = tv + 0.5 * (hyp(t, io.lap2s, 1, t.m(i, ja, k)) + hyp(t, io.lap2s, 1, t.m(i, jb, k)));
+    if (io.lap20) tv = tv + 0.5 * (hyp(t, io.lap20, 1, t.m(i, ja, k)) + hyp(t, io.lap20, 1, t.m(i, jb, k)));
+    tv = tv + 0.5 * (A[2 * N + t.m(i, ja, k)] + A[2 * N + t.m(i, jb, k)]);
+    io.tend[Nu + qv] = tv;
+  }
+  if (i < nx && j < ny) {
+    const int qw = t.w(i, j, k);
+    const int g = wlev(k, nz);
+    const double thpw = 0.5 * (thp(t, TH, t.m(i, j, g - 1)) + thp(t, TH, t.m(i, j, g)));
+    const double thw = t.thw_bg[qw] + thpw;
+    const double buoy = t.grav * (thpw / t.thw_bg[qw]);
+    double tw = ((-t.cp) * thw) * gzw_pi(t, PI, i, j, k) + buoy;
+    if (io.lap2s) tw = tw + 0.5 * (hyp(t, io.lap2s, 2, t.m(i, j, g - 1)) + hyp(t, io.lap2s, 2, t.m(i, j, g)));
+    if (io.lap20) tw = tw + 0.5 * (hyp(t, io.lap20, 2, t.m(i, j, g - 1)) + hyp(t, io.lap20, 2, t.m(i, j, g)));
+    tw = tw + 0.5 * (A[3 * N + t.m(i, j, g - 1)] + A[3 * N + t.m(i, j, g)]);
+    io.tend[Nu + Nv + qw] = tw;
+  }
+  if (i < nx && j < ny && k < nz) {
+    const int q = t.m(i, j, k);
+    const double tpi = pi_div(
+        t, i, j, k, [&](int f) { return U[t.u(f, j, k)] * t.cx[t.u(f, j, k)]; },
+        [&](int f) { return V[t.v(i, f, k)] * t.cy[t.v(i, f, k)]; },
+        [&](int kk) { return (kk == 0 || kk == nz) ? 0.0 : E[t.w(i, j, kk)] * t.czs[t.w(i, j, kk)]; });
+    auto thpu = [&](int f) {
+      int ia, ib;
+      face_pair(f, nx, t.px, ia, ib);
+      return 0.5 * (thp(t, TH, t.m(ia, j, k)) + thp(t, TH, t.m(ib, j, k)));
+    };
+    auto thpv = [&](int f) {
+      int ja, jb;
+      face_pair(f, ny, t.py, ja, jb);
+      return 0.5 * (thp(t, TH, t.m(i, ja, k)) + thp(t, TH, t.m(i, jb, k)));
+    };
+    const double nl = pi_div(
+        t, i, j, k, [&](int f) { return U[t.u(f, j, k)] * thpu(f) * t.cxp[t.u(f, j, k)]; },
+        [&](int f) { return V[t.v(i, f, k)] * thpv(f) * t.cyp[t.v(i, f, k)]; },
+        [&](int kk) {
+          if (kk == 0 || kk == nz) return 0.0;
+          const double tw_ = 0.5 * (thp(t, TH, t.m(i, j, kk - 1)) + thp(t, TH, t.m(i, j, kk)));
+          return E[t.w(i, j, kk)] * tw_ * t.czp[t.w(i, j, kk)];
+        });
+    const double wm = 0.5 * (W[t.w(i, j, k)] + W[t.w(i, j, k + 1)]);
+    double tth = (-wm) * t.dthm[q];
+    if (io.lap2s) tth = tth + hyp(t, io.lap2s, 3, q);
+    if (io.lap20) tth = tth + hyp(t, io.lap20, 3, q);
+    tth = tth + A[q];
+    io.tend[Nu + Nv + Nw + q] = tpi + nl;
+    io.tend[Nu + Nv + Nw + N + q] = tth;
+  }
+}
+
 // ---- predictor-corrector (steppers.py:911-998) --------------------------------------
 
 struct PredictorIO {
@@ -539,6 +716,24 @@ int sisl_explicit_f64(const SislTendency *d, const double *u, const double *v, c
   const int64_t n = (int64_t)(t.nx + 1) * (t.ny + 1) * (t.nz + 1);
   hipLaunchKernelGGL(sisl::explicit_kernel, dim3(sisl::nblocks(n)), dim3(sisl::kBlock), 0, sisl::as_stream(stream),
                      t, io);
+  SISL_CHECK_LAUNCH();
+  return SISL_OK;
+}
+
+int sisl_slow_tendencies_f64(const SislTendency *d, const double *u, const double *v, const double *w,
+                             const double *pi, const double *th, const double *eta, const double *lap2s,
+                             const double *lap20, double eps_l, double *adv, double *tend, void *stream) {
+  sisl::TendCtx t;
+  if (!sisl::make_ctx(d, t) || !u || !v || !w || !pi || !th || !eta || !adv || !tend) return SISL_ERR_ARG;
+  if ((lap2s || lap20) && t.nu_h == 0.0) return SISL_ERR_ARG;
+  hipStream_t s = sisl::as_stream(stream);
+  const int64_t N = (int64_t)t.nx * t.ny * t.nz;
+  const int64_t span = (int64_t)(t.nx + 1) * (t.ny + 1) * (t.nz + 1);
+  auto xgrid = [](int64_t n) { return dim3((sisl::nblocks(n) + sisl::kXcds - 1) / sisl::kXcds * sisl::kXcds); };
+  sisl::AdvIO a{u, v, w, th, eta, eps_l, adv};
+  hipLaunchKernelGGL(sisl::adv_kernel, xgrid(N), dim3(sisl::kBlock), 0, s, t, a);
+  sisl::SlowIO io{u, v, w, pi, th, eta, lap2s, lap20, adv, tend};
+  hipLaunchKernelGGL(sisl::slow_kernel, xgrid(span), dim3(sisl::kBlock), 0, s, t, io);
   SISL_CHECK_LAUNCH();
   return SISL_OK;
 }

@@ -57,6 +57,62 @@ def _cat_levels(first, middle, last):
     return torch.cat([first.unsqueeze(0), middle, last.unsqueeze(0)], dim=0)
 
 
+class _SlowKernels:
+    """The slow tendencies of a stage on csrc/tendency.hip (forward only):
+    the static fields of the fused explicit tendencies, with the
+    hyperdiffusion coefficient of the physics' HyperFilter."""
+
+    def __init__(self, stepper, bg):
+        from .glue import FusedGlue, _bind as bind_glue
+        ph = stepper.physics
+        self.glue = FusedGlue(stepper, bg)
+        self.glue.struct.nu_h = float(ph.diffusion.nu_h)
+        self.L = bind_glue(_lib.lib())
+        L = self.L
+        if not getattr(L, "_slow_bound", False):
+            vp = C.c_void_p
+            L.sisl_slow_tendencies_f64.argtypes = [C.POINTER(glue_struct_type())] + [vp] * 8 + [C.c_double,
+                                                                                                vp, vp, vp]
+            L._slow_bound = True
+        self.eps_l = float(ph.c.get("eps_l", 1e-12))
+        self.hyper_on = float(ph.diffusion.nu_h) != 0.0
+        self._work = None
+
+    def lap2(self, u, v, w, th):
+        """del^4 of (u_m, v_m, w_m, theta') of a state (4 mass fields)."""
+        L, g = self.L, self.glue
+        if self._work is None:
+            n = int(L.sisl_hyperdiffusion_workspace_f64(C.byref(g.struct.g)))
+            self._work = torch.empty(n, dtype=torch.float64, device=u.device)
+        out = torch.empty(4 * th.numel(), dtype=torch.float64, device=u.device)
+        _lib.check(L.sisl_hyperdiffusion_f64(C.byref(g.struct), u.data_ptr(), v.data_ptr(), w.data_ptr(),
+                                             th.data_ptr(), self._work.data_ptr(), out.data_ptr(),
+                                             _lib.stream_ptr(u.device)), "hyperdiffusion")
+        return out
+
+    def tendencies(self, cur, lap2s, lap20):
+        u, v, w, pi, th, eta = (cur[k].detach().contiguous() for k in ("u", "v", "w", "pi", "th_v", "eta_dot"))
+        nz, ny, nx = pi.shape
+        nu, nv, nw, nm = u.numel(), v.numel(), w.numel(), pi.numel()
+        adv = torch.empty(4 * nm, dtype=torch.float64, device=u.device)
+        tend = torch.empty(nu + nv + nw + 2 * nm, dtype=torch.float64, device=u.device)
+        ptr = (lambda x: x.data_ptr() if x is not None else None)
+        _lib.check(self.L.sisl_slow_tendencies_f64(C.byref(self.glue.struct), u.data_ptr(), v.data_ptr(),
+                                                   w.data_ptr(), pi.data_ptr(), th.data_ptr(), eta.data_ptr(),
+                                                   ptr(lap2s), ptr(lap20), self.eps_l, adv.data_ptr(),
+                                                   tend.data_ptr(), _lib.stream_ptr(u.device)),
+                   "slow tendencies")
+        o = [0, nu, nu + nv, nu + nv + nw, nu + nv + nw + nm, nu + nv + nw + 2 * nm]
+        return {"u": tend[o[0]:o[1]].view(u.shape), "v": tend[o[1]:o[2]].view(v.shape),
+                "w": tend[o[2]:o[3]].view(w.shape), "pi": tend[o[3]:o[4]].view(pi.shape),
+                "th_v": tend[o[4]:o[5]].view(th.shape)}
+
+
+def glue_struct_type():
+    from .glue import SislTendency
+    return SislTendency
+
+
 class _AcousticKernels:
     """Static fields of the acoustic kernels for one stepper (built once: the
     background is the reference state of the run and never changes)."""
@@ -183,9 +239,12 @@ class SplitExplicitStepper3D:
         self._bg = None
         self._ext_cache = None
         # run the acoustic loop on the HIP kernels (csrc/acoustic.hip) for FP64
-        # device states; False: the torch restatement throughout
+        # device states, and (forward passes without a physics suite) the slow
+        # tendencies on csrc/tendency.hip; False: the torch restatement
         self.fused_acoustic = True
+        self.fused_slow = True
         self._kern = None
+        self._slow = None
 
     # -- public API (reference shapes) -----------------------------------------
     def step(self, state, t, forcing, bc_fn, ml_params=None):
@@ -223,6 +282,8 @@ class SplitExplicitStepper3D:
         for k in st:
             if k not in sp:
                 sp[k] = st[k]
+        if self._slow_fused_applies(st):
+            return self._step_fused_slow(st, bg, forcing, bc_fn)
         phys_tends, diff_tends = {}, {}
         if ph.physics_suite is not None:
             phys_tends = ph.physics_suite.get_explicit_tendencies(sp, bg, interior_mask=ph.interior_mask,
@@ -259,6 +320,46 @@ class SplitExplicitStepper3D:
         nxt["rho"] = c["p0"] / (c["Rd"] * nxt["th_v"]) * (nxt["pi"] ** (c["cvd"] / c["Rd"]))
         for k, rq in tmass.items():
             nxt[k] = rq / (nxt["rho"] + eps)
+        return nxt
+
+    def _slow_fused_applies(self, st):
+        ph = self.physics
+        if not (self.fused_slow and self._fused_applies(st) and ph.physics_suite is None):
+            return False
+        return not (torch.is_grad_enabled() and any(torch.is_tensor(v) and v.requires_grad for v in st.values()))
+
+    def _step_fused_slow(self, st, bg, forcing, bc_fn):
+        """The step with the slow tendencies on the HIP kernels (no physics
+        suite, no tracers, no gradient): the start state's hyperdiffusion is
+        the diff_tends term of every stage and, with nu_div > 0, also the
+        stage-1 term (the same state)."""
+        ph = self.physics
+        c = ph.c
+        if self._slow is None:
+            self._slow = _SlowKernels(self, bg)
+        sk = self._slow
+        with_start = sk.hyper_on and (ph.nu_h > 0.0 or ph.nu_div > 0.0)
+        with_stage = sk.hyper_on and ph.nu_div > 0.0
+
+        def lap2(s):
+            return sk.lap2(*(s[k].detach().contiguous() for k in ("u", "v", "w", "th_v")))
+        lap0 = lap2(st) if (with_start or with_stage) else None
+        nu_h = ph.nu_h
+        ph.nu_h = 0.0  # hidden during the stages, as in the torch path
+        try:
+            s1 = sk.tendencies(st, lap0 if with_stage else None, lap0 if with_start else None)
+            star = self._acoustic_loop(st, st, s1, bg, self.dtau_stage1, self.ns_stage1)
+            star["eta_dot"], star["w"] = self._diagnose_eta_dot(star)
+            s2 = sk.tendencies(star, lap2(star) if with_stage else None, lap0 if with_start else None)
+            dstar = self._acoustic_loop(st, star, s2, bg, self.dtau_stage2, self.ns_stage2)
+            dstar["eta_dot"], dstar["w"] = self._diagnose_eta_dot(dstar)
+            s3 = sk.tendencies(dstar, lap2(dstar) if with_stage else None, lap0 if with_start else None)
+            nxt = self._acoustic_loop(st, dstar, s3, bg, self.dtau_stage3, self.ns_stage3)
+        finally:
+            ph.nu_h = nu_h
+        nxt = self._call_bc(bc_fn, nxt, forcing)
+        nxt["eta_dot"], nxt["w"] = self._diagnose_eta_dot(nxt)
+        nxt["rho"] = c["p0"] / (c["Rd"] * nxt["th_v"]) * (nxt["pi"] ** (c["cvd"] / c["Rd"]))
         return nxt
 
     # -- steppers.py:1271-1309 ----------------------------------------------------
