@@ -335,6 +335,57 @@ __global__ void __launch_bounds__(256) dots1_kernel(const TB *__restrict__ V, in
   dots1_body<TB, TV, G>(V, ldv, nv, w, n, partials, vinv, stop, gate, blockIdx.x, gridDim.x);
 }
 
+// Exactly NV basis vectors (no clamped group members): the same sums in the
+// same order as dots1_body, so the same bits, with no redundant loads of the
+// last vector -- faster than the group of 8 for short Arnoldi prefixes.
+template <typename TB, typename TV, int NV>
+__global__ void __launch_bounds__(256) dotsx_kernel(const TB *__restrict__ V, int64_t ldv,
+                                                    const TV *__restrict__ w, int64_t n,
+                                                    double *__restrict__ partials, double vinv,
+                                                    const int *__restrict__ stop,
+                                                    const int *__restrict__ gate) {
+  if (pass_skipped(stop, gate)) return;
+  constexpr int N = Vec<TV>::N;
+  const int64_t nq = n / N;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double acc[NV];
+  double ww = 0;
+#pragma unroll
+  for (int r = 0; r < NV; ++r) acc[r] = 0;
+  for (int64_t q = t0; q < nq; q += stride) {
+    TV we[N];
+    vload<N>(w, q, we);
+#pragma unroll
+    for (int k = 0; k < N; ++k) ww += (double)we[k] * (double)we[k];
+    TB x[NV][N];
+#pragma unroll
+    for (int r = 0; r < NV; ++r) vload<N>(V + (int64_t)r * ldv, q, x[r]);
+#pragma unroll
+    for (int r = 0; r < NV; ++r)
+#pragma unroll
+      for (int k = 0; k < N; ++k) acc[r] += (double)x[r][k] * (double)we[k];
+  }
+  if (t0 < n - nq * N) {
+    const int64_t e = nq * N + t0;
+    const double we = (double)w[e];
+    ww += we * we;
+#pragma unroll
+    for (int r = 0; r < NV; ++r) acc[r] += (double)V[(int64_t)r * ldv + e] * we;
+  }
+  block_store_slot(ww, NV, partials, blockIdx.x);
+#pragma unroll
+  for (int g = 0; g * kGrp < NV; ++g) {
+    double a[kGrp];
+#pragma unroll
+    for (int r = 0; r < kGrp; ++r) {
+      const int v = g * kGrp + r;
+      a[r] = v < NV ? acc[v < NV ? v : 0] * vinv : 0.0;
+    }
+    block_store_group(a, g * kGrp, min(kGrp, NV - g * kGrp), partials, blockIdx.x);
+  }
+}
+
 // w <- w - sum_v h[v] vinv V[v] (h read from device memory), slot 0 <- w . w
 template <typename TB, typename TV>
 __device__ __forceinline__ void update_body(const TB *__restrict__ V, int64_t ldv, int nv,
@@ -396,6 +447,56 @@ __global__ void __launch_bounds__(256) update_kernel(const TB *__restrict__ V, i
                                                      const int *__restrict__ stop,
                                                      const int *__restrict__ gate) {
   update_body(V, ldv, nv, h, w, n, partials, vinv, stop, gate, blockIdx.x, gridDim.x);
+}
+
+// update_kernel for exactly NV basis vectors (same bits, no clamped loads).
+template <typename TB, typename TV, int NV>
+__global__ void __launch_bounds__(256) updatex_kernel(const TB *__restrict__ V, int64_t ldv,
+                                                      const double *__restrict__ h,
+                                                      TV *__restrict__ w, int64_t n,
+                                                      double *__restrict__ partials, double vinv,
+                                                      const int *__restrict__ stop,
+                                                      const int *__restrict__ gate) {
+  if (pass_skipped(stop, gate)) return;
+  constexpr int N = Vec<TV>::N;
+  __shared__ double hs[NV];
+  for (int v = threadIdx.x; v < NV; v += blockDim.x) hs[v] = h[v] * vinv;
+  __syncthreads();
+  const int64_t nq = n / N;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double ww = 0;
+  for (int64_t q = t0; q < nq; q += stride) {
+    TV we[N];
+    vload<N>(w, q, we);
+    double sd[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) sd[k] = (double)we[k];
+    TB x[NV][N];
+#pragma unroll
+    for (int r = 0; r < NV; ++r) vload<N>(V + (int64_t)r * ldv, q, x[r]);
+#pragma unroll
+    for (int r = 0; r < NV; ++r) {
+      const double hv = hs[r];
+#pragma unroll
+      for (int k = 0; k < N; ++k) sd[k] -= hv * (double)x[r][k];
+    }
+    TV sn[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) sn[k] = (TV)sd[k];
+    vstore<N>(w, q, sn);
+#pragma unroll
+    for (int k = 0; k < N; ++k) ww += (double)sn[k] * (double)sn[k];
+  }
+  if (t0 < n - nq * N) {
+    const int64_t e = nq * N + t0;
+    double sd = (double)w[e];
+    for (int v = 0; v < NV; ++v) sd -= hs[v] * (double)V[(int64_t)v * ldv + e];
+    const TV sn = (TV)sd;
+    w[e] = sn;
+    ww += (double)sn * (double)sn;
+  }
+  block_store_slot(ww, 0, partials, blockIdx.x);
 }
 
 // JAX _iterative_classical_gram_schmidt loop condition: a second pass runs
@@ -690,8 +791,17 @@ static void dots(const TB *V, int64_t ldv, int nv, const TV *w, int64_t n, doubl
   const double bytes = (double)n * ((double)nv * sizeof(TB) + sizeof(TV));
   const int tid = timer_begin(SISL_T_KRYLOV_DOTS, sizeof(TV) == 4, bytes, stop, gate, s);
   const dim3 gr(kRedGrid), bl(kBlock);
+  // exact-count variants where they measured faster (short prefixes, and
+  // 9 / 10 vectors, which the two-group variant would load as 16)
+#define SISL_DOTSX(K) hipLaunchKernelGGL((dotsx_kernel<TB, TV, K>), gr, bl, 0, s, V, ldv, w, n, partials, vinv, stop, gate)
   if (nv <= 0)
     hipLaunchKernelGGL((dots_kernel<TB, TV>), gr, bl, 0, s, V, ldv, nv, w, n, partials, vinv, stop, gate);
+  else if (nv == 1) SISL_DOTSX(1);
+  else if (nv == 2) SISL_DOTSX(2);
+  else if (nv == 3) SISL_DOTSX(3);
+  else if (nv == 4) SISL_DOTSX(4);
+  else if (nv == 9) SISL_DOTSX(9);
+  else if (nv == 10) SISL_DOTSX(10);
   else if (nv <= kGrp)
     hipLaunchKernelGGL((dots1_kernel<TB, TV, 1>), gr, bl, 0, s, V, ldv, nv, w, n, partials, vinv, stop, gate);
   else if (nv <= 2 * kGrp)
@@ -711,8 +821,15 @@ static void update(const TB *V, int64_t ldv, int nv, const double *h, TV *w, int
                    const int *gate = nullptr, const Tail *tl = nullptr) {
   const double bytes = (double)n * ((double)nv * sizeof(TB) + 2.0 * sizeof(TV));
   const int tid = timer_begin(SISL_T_KRYLOV_UPDATE, sizeof(TV) == 4, bytes, stop, gate, s);
-  hipLaunchKernelGGL((update_kernel<TB, TV>), dim3(kRedGrid), dim3(kBlock), 0, s, V, ldv, nv, h, w,
-                     n, partials, vinv, stop, gate);
+#define SISL_UPDX(K) hipLaunchKernelGGL((updatex_kernel<TB, TV, K>), dim3(kRedGrid), dim3(kBlock), 0, s, V, ldv, h, w, n, partials, vinv, stop, gate)
+  if (nv == 1) SISL_UPDX(1);
+  else if (nv == 2) SISL_UPDX(2);
+  else if (nv == 3) SISL_UPDX(3);
+  else if (nv == 4) SISL_UPDX(4);
+  else if (nv == 5) SISL_UPDX(5);
+  else
+    hipLaunchKernelGGL((update_kernel<TB, TV>), dim3(kRedGrid), dim3(kBlock), 0, s, V, ldv, nv, h, w,
+                       n, partials, vinv, stop, gate);
   timer_end(tid, s);
   if (tl) hipLaunchKernelGGL(tail_kernel, dim3(1), dim3(kTailThreads), 0, s, (const double *)partials, *tl, stop, gate);
 }

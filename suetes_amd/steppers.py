@@ -796,13 +796,19 @@ class SISLStepper3D:
         """The fused kernels cover steps without a physics suite (tracers come
         with one), with FP64 device state and with a background and grid
         that carry no gradient (terrain controls run the torch ops, whose
-        reverse mode reaches the metrics)."""
+        reverse mode reaches the metrics).  A step whose state is being
+        differentiated also runs the torch ops: their autograd graph keeps
+        the intermediates, which costs less than re-evaluating the torch
+        restatement of each fused group in the backward pass (value and
+        gradient at 256x256x80: 235 vs 240 ms mixed, 700 vs 707 ms FP64)."""
         if not self.fused_glue or self.physics.physics_suite is not None:
             return False
         u = state["u"]
         if not (torch.is_tensor(u) and u.is_cuda and u.dtype == torch.float64):
             return False
         if any(k in state for k in self.tracer_keys):
+            return False
+        if torch.is_grad_enabled() and any(torch.is_tensor(v) and v.requires_grad for v in state.values()):
             return False
         self.implicit_solver.prepare(bg)
         return not _operator_inputs(self.implicit_solver)
