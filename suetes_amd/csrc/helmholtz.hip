@@ -248,6 +248,367 @@ __global__ void __launch_bounds__(256) helmholtz_kernel(OpT<T> o, const T *__res
 }
 
 
+// ---- two columns per thread ------------------------------------------------------
+// The forward operator with each thread owning the adjacent columns (i, i+1),
+// i even (nx even): column i+1 takes every left-neighbour value (pi and its
+// vertical difference, the v-face average, the u face between the two) from
+// column i's registers, and the pair's loads of mass-point, w-level and
+// v-face arrays are single two-element vector loads (u-face rows have an odd
+// stride, so those stay scalar).  Each output is the same expression of the
+// same operands as in helmholtz_body, so the results are the same bits.
+template <typename T> struct Vec2T;
+template <> struct Vec2T<float> { using type = float2; };
+template <> struct Vec2T<double> { using type = double2; };
+template <typename T>
+__device__ __forceinline__ void ldg2(const T *base, uint32_t idx, T &a, T &b) {
+  using V = typename Vec2T<T>::type;
+  const V v = *reinterpret_cast<const V *>(reinterpret_cast<const char *>(base) + idx * (uint32_t)sizeof(T));
+  a = v.x;
+  b = v.y;
+}
+template <typename T>
+__device__ __forceinline__ void stg2(T *base, uint32_t idx, T a, T b) {
+  using V = typename Vec2T<T>::type;
+  V v;
+  v.x = a;
+  v.y = b;
+  *reinterpret_cast<V *>(reinterpret_cast<char *>(base) + idx * (uint32_t)sizeof(T)) = v;
+}
+// a pair of coefficients (bfloat16 storage: one 32-bit load, low half first)
+template <typename T, typename TC>
+__device__ __forceinline__ void ldc2(const TC *base, uint32_t idx, T &a, T &b) {
+  if constexpr (sizeof(TC) == 2) {
+    const uint32_t v = *reinterpret_cast<const uint32_t *>(reinterpret_cast<const char *>(base) + idx * 2u);
+    a = (T)__uint_as_float(v << 16);
+    b = (T)__uint_as_float(v & 0xffff0000u);
+  } else {
+    ldg2<T>(reinterpret_cast<const T *>(base), idx, a, b);
+  }
+}
+// a pair of basis elements (FP16 or the working type).  FP16 elements are
+// stored one by one: packing the two conversions into one register lets the
+// compiler fuse the scaling multiply and the conversion into mixed-precision
+// instructions that round once, not as the one-column kernel does (a product
+// rounded to FP32, then to FP16), and a few elements per column then differ
+// in their last bit.
+template <typename B>
+__device__ __forceinline__ void stb2(B *base, uint32_t idx, B a, B b) {
+  if constexpr (sizeof(B) == 2) {
+    stg(base, idx, a);
+    stg(base, idx + 1, b);
+  } else {
+    stg2<B>(base, idx, a, b);
+  }
+}
+
+constexpr int kTX2 = 2 * kTX;  // columns per tile row of the two-column form
+
+template <typename T, int NB = 0, typename TC = T>
+__device__ __forceinline__ void helmholtz2_body(const OpT<T> &o, const T *__restrict__ x,
+                                                T *__restrict__ y, int kchunk, int blk, int nblk,
+                                                const double *inv = nullptr, void *vout = nullptr,
+                                                double vs = 1.0, const T *__restrict__ res = nullptr) {
+  const Stag &s = o.s;
+  const int nx = s.nx, ny = s.ny, nz = s.nz;
+  const int nbx = (nx + kTX2 - 1) / kTX2, nby = (ny + kTY - 1) / kTY;
+  const int tile = xcd_tile(blk, nblk);
+  const int bx = tile % nbx, by = (tile / nbx) % nby, bz = tile / (nbx * nby);
+  const int i = bx * kTX2 + 2 * threadIdx.x, j = by * kTY + threadIdx.y;  // columns i, i + 1
+  if (i >= nx || j >= ny) return;
+  const int k0 = bz * kchunk, k1 = min(k0 + kchunk, nz + 1);
+  const int il = i == 0 ? nx - 1 : i - 1, jd = j == 0 ? ny - 1 : j - 1;
+  const T *U = x + s.ou, *V = x + s.ov, *W = x + s.ow, *P = x + s.om, *E = x + s.oe;
+  T *YU = y + s.ou, *YV = y + s.ov, *YW = y + s.ow, *YP = y + s.om, *YE = y + s.oe;
+  FUSED_NORMALISATION_SETUP
+  auto put = [&](T *Y_, uint32_t q, T v) {
+    if constexpr (NB == 3) v = ldg(res + (Y_ - y), q) - v;
+    stg(Y_, q, so(v));
+  };
+  auto put2 = [&](T *Y_, uint32_t q, T va, T vb) {
+    if constexpr (NB == 3) {
+      T ra, rb;
+      ldg2(res + (Y_ - y), q, ra, rb);
+      va = ra - va;
+      vb = rb - vb;
+    }
+    stg2(Y_, q, so(va), so(vb));
+  };
+  auto bput2 = [&](BT *b, uint32_t q, T va, T vb) {
+    if constexpr (kNorm) stb2<BT>(b, q, (BT)(va * bsc), (BT)(vb * bsc));
+  };
+  const bool uform0 = s.px || i > 0, vform = s.py || j > 0;  // column i + 1 > 0 always has its formula
+  const T mudx0 = ldg(o.mudx, (uint32_t)s.u2(i, j)), mudx1 = ldg(o.mudx, (uint32_t)s.u2(i + 1, j));
+  const T afu0 = ldg(o.afu, (uint32_t)s.u2(i, j)), afu1 = ldg(o.afu, (uint32_t)s.u2(i + 1, j));
+  T mvdy[2], afv[2], mmx[2], mmy[2];
+  ldg2(o.mvdy, (uint32_t)s.v2(i, j), mvdy[0], mvdy[1]);
+  ldg2(o.afv, (uint32_t)s.v2(i, j), afv[0], afv[1]);
+  ldg2(o.mmx, (uint32_t)s.m2(i, j), mmx[0], mmx[1]);
+  ldg2(o.mmy, (uint32_t)s.m2(i, j), mmy[0], mmy[1]);
+  const T half = (T)0.5, alpha = o.alpha;
+  const TC *__restrict__ c_cua = reinterpret_cast<const TC *>(o.cua);
+  const TC *__restrict__ c_zxu = reinterpret_cast<const TC *>(o.zxu);
+  const TC *__restrict__ c_cx = reinterpret_cast<const TC *>(o.cx);
+  const TC *__restrict__ c_cva = reinterpret_cast<const TC *>(o.cva);
+  const TC *__restrict__ c_zev = reinterpret_cast<const TC *>(o.zev);
+  const TC *__restrict__ c_cy = reinterpret_cast<const TC *>(o.cy);
+  const TC *__restrict__ c_dw = reinterpret_cast<const TC *>(o.dw);
+  const TC *__restrict__ c_cw = reinterpret_cast<const TC *>(o.cw);
+  const TC *__restrict__ c_cz = reinterpret_cast<const TC *>(o.cz);
+  const TC *__restrict__ c_kx = reinterpret_cast<const TC *>(o.kx);
+  const TC *__restrict__ c_ky = reinterpret_cast<const TC *>(o.ky);
+  const TC *__restrict__ c_rdzw = reinterpret_cast<const TC *>(o.rdzw);
+  const TC *__restrict__ c_cpm = reinterpret_cast<const TC *>(o.cpm);
+  const uint32_t pm = (uint32_t)nx * ny, pu = (uint32_t)(nx + 1) * ny, pv = (uint32_t)nx * (ny + 1);
+  const uint32_t mc = (uint32_t)j * nx + i, ml = (uint32_t)j * nx + il, md = (uint32_t)jd * nx + i;
+  const uint32_t uc = (uint32_t)j * (nx + 1) + i, ud = (uint32_t)jd * (nx + 1) + i;
+  const uint32_t vc = (uint32_t)j * nx + i, vl = (uint32_t)j * nx + il;
+  const uint32_t ux = (uint32_t)j * (nx + 1) + nx;
+  const uint32_t vy = (uint32_t)ny * nx + i;
+  // carried state per column (index 0: column i, 1: column i + 1); the left
+  // column of column i is carried separately (pl*, gl0), that of i + 1 is column i
+  T pc0[2] = {0, 0}, pd0[2] = {0, 0}, pcm[2] = {0, 0}, pdm[2] = {0, 0}, gc0[2] = {0, 0}, gd0[2] = {0, 0};
+  T pl0 = 0, plm = 0, gl0 = 0;
+  T umc_p[2] = {0, 0}, vmc_p[2] = {0, 0};
+  if (k0 < nz) {
+    const uint32_t q = (uint32_t)k0 * pm;
+    ldg2(P, mc + q, pc0[0], pc0[1]);
+    pl0 = ldg(P, ml + q);
+    ldg2(P, md + q, pd0[0], pd0[1]);
+  }
+  if (k0 >= 1) {
+    const uint32_t q = (uint32_t)(k0 - 1) * pm;
+    ldg2(P, mc + q, pcm[0], pcm[1]);
+    plm = ldg(P, ml + q);
+    ldg2(P, md + q, pdm[0], pdm[1]);
+    const uint32_t qu = (uint32_t)(k0 - 1) * pu, qv = (uint32_t)(k0 - 1) * pv;
+    const T ua = ldg(U, uc + qu), ub = ldg(U, uc + qu + 1), ue = ldg(U, uc + qu + 2);
+    umc_p[0] = half * (ua + ub);
+    umc_p[1] = half * (ub + ue);
+    T va[2], vb[2];
+    ldg2(V, vc + qv, va[0], va[1]);
+    ldg2(V, vc + qv + nx, vb[0], vb[1]);
+    vmc_p[0] = half * (va[0] + vb[0]);
+    vmc_p[1] = half * (va[1] + vb[1]);
+  }
+  if (k0 == 0) {
+    T p1[2], d1[2], r[2], rd[2];
+    ldg2(P, mc + pm, p1[0], p1[1]);
+    ldg2(P, md + pm, d1[0], d1[1]);
+    ldc2<T>(c_rdzw, mc, r[0], r[1]);
+    ldc2<T>(c_rdzw, md, rd[0], rd[1]);
+    for (int c = 0; c < 2; ++c) {
+      gc0[c] = (p1[c] - pc0[c]) * r[c];
+      gd0[c] = (d1[c] - pd0[c]) * rd[c];
+    }
+    gl0 = (ldg(P, ml + pm) - pl0) * ldc<T>(c_rdzw, ml);
+  } else if (k0 < nz) {
+    const uint32_t q = (uint32_t)k0 * pm;
+    T r[2], rd[2];
+    ldc2<T>(c_rdzw, mc + q, r[0], r[1]);
+    ldc2<T>(c_rdzw, md + q, rd[0], rd[1]);
+    for (int c = 0; c < 2; ++c) {
+      gc0[c] = (pc0[c] - pcm[c]) * r[c];
+      gd0[c] = (pd0[c] - pdm[c]) * rd[c];
+    }
+    gl0 = (pl0 - plm) * ldc<T>(c_rdzw, ml + q);
+  }
+  T e0[2], fz0[2] = {0, 0};
+  ldg2(E, mc + (uint32_t)k0 * pm, e0[0], e0[1]);
+  if (k0 >= 1 && k0 < nz) {
+    T cz[2];
+    ldc2<T>(c_cz, mc + (uint32_t)k0 * pm, cz[0], cz[1]);
+    fz0[0] = e0[0] * cz[0];
+    fz0[1] = e0[1] * cz[1];
+  }
+  for (int k = k0; k < k1; ++k) {
+    const uint32_t qm = (uint32_t)k * pm, qu = (uint32_t)k * pu, qv = (uint32_t)k * pv;
+    const uint32_t wk = mc + qm;
+    T wv[2];
+    ldg2(W, wk, wv[0], wv[1]);
+    if (k == nz) {
+      T r[2];
+      ldc2<T>(c_rdzw, wk, r[0], r[1]);
+      put2(YW, wk, wv[0], wv[1]);
+      put2(YE, wk, e0[0] * r[0], e0[1] * r[1]);
+      bput2(BW, wk, wv[0], wv[1]);
+      bput2(BE, wk, e0[0], e0[1]);
+      break;
+    }
+    // ---- mass level k: gz_m of the columns ----
+    T gc1[2], gd1[2], pc1[2] = {0, 0}, pd1[2] = {0, 0}, gl1, pl1 = 0;
+    {
+      const uint32_t q1 = qm + pm;
+      T r[2], rd[2];
+      ldc2<T>(c_rdzw, mc + q1, r[0], r[1]);
+      ldc2<T>(c_rdzw, md + q1, rd[0], rd[1]);
+      const T rl = ldc<T>(c_rdzw, ml + q1);
+      if (k + 1 < nz) {
+        ldg2(P, mc + q1, pc1[0], pc1[1]);
+        pl1 = ldg(P, ml + q1);
+        ldg2(P, md + q1, pd1[0], pd1[1]);
+        for (int c = 0; c < 2; ++c) {
+          gc1[c] = (pc1[c] - pc0[c]) * r[c];
+          gd1[c] = (pd1[c] - pd0[c]) * rd[c];
+        }
+        gl1 = (pl1 - pl0) * rl;
+      } else {  // top closure: gz_w(nz) repeats the last interior difference
+        for (int c = 0; c < 2; ++c) {
+          gc1[c] = (pc0[c] - pcm[c]) * r[c];
+          gd1[c] = (pd0[c] - pdm[c]) * rd[c];
+        }
+        gl1 = (pl0 - plm) * rl;
+      }
+    }
+    const T gzc0 = half * (gc0[0] + gc1[0]), gzc1 = half * (gc0[1] + gc1[1]);
+    const T gzl = half * (gl0 + gl1);
+    const T gzd0 = half * (gd0[0] + gd1[0]), gzd1 = half * (gd0[1] + gd1[1]);
+    const uint32_t au = uc + qu, av = vc + qv;
+    const T ua = ldg(U, au), ub = ldg(U, au + 1), ue = ldg(U, au + 2);  // faces i, i+1, i+2
+    T v0[2], v1[2];
+    ldg2(V, av, v0[0], v0[1]);
+    ldg2(V, av + nx, v1[0], v1[1]);
+    const T um0 = half * (ua + ub), um1 = half * (ub + ue);
+    const T vm0 = half * (v0[0] + v1[0]), vm1 = half * (v0[1] + v1[1]);
+    // ---- L_u at faces i and i+1 ----
+    {
+      T out0 = ua;
+      if (uform0) {
+        const uint32_t al = vl + qv;
+        const T vml = half * (ldg(V, al) + ldg(V, al + nx));
+        out0 = ua + ldc<T>(c_cua, au) * (mudx0 * (pc0[0] - pl0) - ldc<T>(c_zxu, au) * (half * (gzl + gzc0))) -
+               afu0 * (half * (vml + vm0));
+      }
+      const T out1 = ub + ldc<T>(c_cua, au + 1) * (mudx1 * (pc0[1] - pc0[0]) -
+                                                   ldc<T>(c_zxu, au + 1) * (half * (gzc0 + gzc1))) -
+                     afu1 * (half * (vm0 + vm1));
+      put(YU, au, out0);
+      put(YU, au + 1, out1);
+      if constexpr (kNorm) {
+        stg(BU, au, (BT)(ua * bsc));
+        stg(BU, au + 1, (BT)(ub * bsc));
+      }
+      if (i + 1 == nx - 1 && !s.px) {  // closed east face: identity
+        put(YU, au + 2, ue);
+        if constexpr (kNorm) stg(BU, au + 2, (BT)(ue * bsc));
+      }
+      if (i == 0 && s.px) {  // periodic: face nx repeats face 0
+        put(YU, ux + qu, out0);
+        if constexpr (kNorm) stg(BU, ux + qu, (BT)(ldg(U, ux + qu) * bsc));
+      }
+    }
+    // ---- L_v at face j of both columns ----
+    {
+      T out[2] = {v0[0], v0[1]};
+      if (vform) {
+        const uint32_t ad = ud + qu;
+        const T da = ldg(U, ad), db = ldg(U, ad + 1), de = ldg(U, ad + 2);
+        const T umd[2] = {half * (da + db), half * (db + de)};
+        const T umk[2] = {um0, um1};
+        const T gzc[2] = {gzc0, gzc1}, gzd[2] = {gzd0, gzd1};
+        T cva[2], zev[2];
+        ldc2<T>(c_cva, av, cva[0], cva[1]);
+        ldc2<T>(c_zev, av, zev[0], zev[1]);
+        for (int c = 0; c < 2; ++c)
+          out[c] = v0[c] + cva[c] * (mvdy[c] * (pc0[c] - pd0[c]) - zev[c] * (half * (gzd[c] + gzc[c]))) +
+                   afv[c] * (half * (umd[c] + umk[c]));
+      }
+      put2(YV, av, out[0], out[1]);
+      bput2(BV, av, v0[0], v0[1]);
+      if (j == ny - 1 && !s.py) {
+        put2(YV, av + nx, v1[0], v1[1]);
+        bput2(BV, av + nx, v1[0], v1[1]);
+      }
+      if (j == 0 && s.py) {
+        put2(YV, vy + qv, out[0], out[1]);
+        if constexpr (kNorm) {
+          T na, nb_;
+          ldg2(V, vy + qv, na, nb_);
+          bput2(BV, vy + qv, na, nb_);
+        }
+      }
+    }
+    // ---- L_pi ----
+    const uint32_t w1 = wk + pm;
+    T e1[2], fz1[2] = {0, 0};
+    ldg2(E, w1, e1[0], e1[1]);
+    if (k + 1 < nz) {
+      T cz[2];
+      ldc2<T>(c_cz, w1, cz[0], cz[1]);
+      fz1[0] = e1[0] * cz[0];
+      fz1[1] = e1[1] * cz[1];
+    }
+    {
+      const T cxa = ldc<T>(c_cx, au), cxb = ldc<T>(c_cx, au + 1), cxe = ldc<T>(c_cx, au + 2);
+      const T fx0 = ub * cxb - ua * cxa, fx1 = ue * cxe - ub * cxb;
+      T cya[2], cyb[2], cpm[2];
+      ldc2<T>(c_cy, av, cya[0], cya[1]);
+      ldc2<T>(c_cy, av + nx, cyb[0], cyb[1]);
+      ldc2<T>(c_cpm, wk, cpm[0], cpm[1]);
+      const T fy0 = v1[0] * cyb[0] - v0[0] * cya[0], fy1 = v1[1] * cyb[1] - v0[1] * cya[1];
+      put2(YP, wk, pc0[0] + cpm[0] * (mmx[0] * fx0 + mmy[0] * fy0 + (fz1[0] - fz0[0])),
+           pc0[1] + cpm[1] * (mmx[1] * fx1 + mmy[1] * fy1 + (fz1[1] - fz0[1])));
+      bput2(BP, wk, pc0[0], pc0[1]);
+    }
+    // ---- w level k ----
+    if (k == 0) {
+      T r[2];
+      ldc2<T>(c_rdzw, wk, r[0], r[1]);
+      put2(YW, wk, wv[0], wv[1]);
+      put2(YE, wk, e0[0] * r[0], e0[1] * r[1]);
+    } else {
+      T dw[2], cw[2], kx[2], ky[2];
+      ldc2<T>(c_dw, wk, dw[0], dw[1]);
+      ldc2<T>(c_cw, wk, cw[0], cw[1]);
+      ldc2<T>(c_kx, wk, kx[0], kx[1]);
+      ldc2<T>(c_ky, wk, ky[0], ky[1]);
+      const T umk[2] = {um0, um1}, vmk[2] = {vm0, vm1};
+      T yw[2], ye[2];
+      for (int c = 0; c < 2; ++c) {
+        yw[c] = dw[c] * wv[c] + cw[c] * (pc0[c] - pcm[c]);
+        ye[c] = alpha * (e0[c] + kx[c] * (half * (umc_p[c] + umk[c])) + ky[c] * (half * (vmc_p[c] + vmk[c])) - wv[c]);
+      }
+      put2(YW, wk, yw[0], yw[1]);
+      put2(YE, wk, ye[0], ye[1]);
+    }
+    bput2(BW, wk, wv[0], wv[1]);
+    bput2(BE, wk, e0[0], e0[1]);
+    // k+1 becomes k
+    for (int c = 0; c < 2; ++c) {
+      gc0[c] = gc1[c];
+      gd0[c] = gd1[c];
+      pcm[c] = pc0[c];
+      pdm[c] = pd0[c];
+      pc0[c] = pc1[c];
+      pd0[c] = pd1[c];
+      e0[c] = e1[c];
+      fz0[c] = fz1[c];
+    }
+    gl0 = gl1;
+    plm = pl0;
+    pl0 = pl1;
+    umc_p[0] = um0;
+    umc_p[1] = um1;
+    vmc_p[0] = vm0;
+    vmc_p[1] = vm1;
+  }
+}
+
+template <typename T, int NB, typename TC>
+__global__ void __launch_bounds__(256) helmholtz2_kernel(OpT<T> o, const T *__restrict__ x,
+                                                         T *__restrict__ y, int kchunk,
+                                                         const int *__restrict__ stop,
+                                                         const double *__restrict__ inv, void *vout,
+                                                         double vs, const T *__restrict__ res) {
+  if (stop && *stop) return;
+  helmholtz2_body<T, NB, TC>(o, x, y, kchunk, blockIdx.x, gridDim.x, inv, vout, vs, res);
+}
+
+// The two-column form needs even nx and even offsets of the packed vector's
+// components (aligned pair loads and stores).
+__host__ __device__ inline bool two_column_ok(const Stag &s) {
+  return s.nx % 2 == 0 && s.ov % 2 == 0 && s.ow % 2 == 0 && s.om % 2 == 0 && s.oe % 2 == 0;
+}
+
 // ---- fused transpose -----------------------------------------------------------
 // x = A^T y in gather form, one thread per (i, j) column marching a chunk of
 // levels like the forward operator.  Per mass level the thread needs the
@@ -509,7 +870,8 @@ __global__ void __launch_bounds__(256) helmholtz_T_kernel(OpT<T> o, const T *__r
 }
 
 // Launch shape of the operator and of its transpose: levels per thread
-// (chunk) and the unroll of the level loop.  Every shape computes
+// (chunk) and the unroll of the level loop (1, 2), or 3: the forward
+// operator with two columns per thread (its transpose then runs unroll 1).  Every shape computes
 // bit-identical results; the autotuner below picks one per grid, element type
 // and direction by timing, the default is 8 chunks per column without
 // unrolling.
@@ -598,9 +960,56 @@ static int launch_shape(const OpT<T> &o, const T *x, T *y, hipStream_t st, const
   return launch_shape_tc<K, T, T>(g, o, x, y, st, stop, kc, sh.unr, f);
 }
 
+// The two-column forward kernel (shape unroll value 3) in its variants.
+template <typename T, int NB, typename TC>
+static void go2(dim3 g, hipStream_t st, const OpT<T> &o, const T *x, T *y, int kc, const int *stop, const Fuse &f) {
+  hipLaunchKernelGGL((helmholtz2_kernel<T, NB, TC>), g, dim3(kTX, kTY), 0, st, o, x, y, kc, stop, f.inv, f.vout,
+                     f.vs, static_cast<const T *>(f.res));
+}
+template <typename T, typename TC>
+static int launch2_tc(dim3 g, const OpT<T> &o, const T *x, T *y, hipStream_t st, const int *stop, int kc,
+                      const Fuse &f) {
+  if (f.res) {
+    if (f.nb) return SISL_ERR_ARG;
+    if constexpr (sizeof(T) == 8) {
+      go2<T, 3, TC>(g, st, o, x, y, kc, stop, f);
+      return SISL_OK;
+    }
+    return SISL_ERR_ARG;
+  }
+  if (f.nb == 0) {
+    go2<T, 0, TC>(g, st, o, x, y, kc, stop, f);
+    return SISL_OK;
+  }
+  if constexpr (sizeof(T) == 4) {
+    if (f.nb == 1) {
+      go2<T, 1, TC>(g, st, o, x, y, kc, stop, f);
+      return SISL_OK;
+    }
+    if (f.nb == 2) {
+      go2<T, 2, TC>(g, st, o, x, y, kc, stop, f);
+      return SISL_OK;
+    }
+  }
+  return SISL_ERR_ARG;
+}
+
 template <typename T>
 static int launch_A_shape(const OpT<T> &o, const T *x, T *y, hipStream_t st, const int *stop, AShape sh,
                           const Fuse &f = Fuse{}) {
+  if (sh.unr == 3) {
+    if (two_column_ok(o.s)) {
+      const int kc = sh.kc;
+      const dim3 g((unsigned)(((o.s.nx + kTX2 - 1) / kTX2) * ((o.s.ny + kTY - 1) / kTY) *
+                              ((o.s.nz + 1 + kc - 1) / kc)));
+      if (f.cb) {
+        if constexpr (sizeof(T) == 4) return launch2_tc<T, bf16_t>(g, o, x, y, st, stop, kc, f);
+        return SISL_ERR_ARG;
+      }
+      return launch2_tc<T, T>(g, o, x, y, st, stop, kc, f);
+    }
+    sh.unr = 1;  // odd nx: one column per thread
+  }
   return launch_shape<AKern, T>(o, x, y, st, stop, sh, f);
 }
 
@@ -625,6 +1034,7 @@ int launch_A(const OpT<T> &o, const T *x, T *y, hipStream_t st, const int *stop,
 template <typename T>
 static int launch_AT_shape(const OpT<T> &o, const T *y, T *x, hipStream_t st, const int *stop, AShape sh,
                            const Fuse &f = Fuse{}) {
+  if (sh.unr == 3) sh.unr = 1;  // the two-column form exists for the forward operator only
   return launch_shape<ATKern, T>(o, y, x, st, stop, sh, f);
 }
 
@@ -667,8 +1077,9 @@ static int autotune_A(const OpT<T> &o, const T *x, T *y, void *scratch, hipStrea
   const int levels = o.s.nz + 1;
   std::vector<AShape> cand;
   for (int parts : {1, 2, 4, 6, 8, 12, 16})
-    for (int unr : {1, 2}) {
+    for (int unr : {1, 2, 3}) {
       const int kc = (levels + parts - 1) / parts;
+      if (unr == 3 && !two_column_ok(o.s)) continue;
       if (kc >= 2) cand.push_back({kc, unr});
     }
   const int nvar = sizeof(T) == 4 ? 4 : 2;
@@ -697,6 +1108,7 @@ static int autotune_A(const OpT<T> &o, const T *x, T *y, void *scratch, hipStrea
     const Fuse f = var >= 2 ? fz : Fuse{};
     float best_ms = 1e30f;
     for (const AShape &c : cand) {
+      if (tr && c.unr == 3) continue;  // forward only
       auto run = [&]() {
         if (tr) launch_AT_shape(o, x, y, st, nullptr, c, f);
         else launch_A_shape(o, x, y, st, nullptr, c, f);
@@ -794,7 +1206,7 @@ int sisl_helmholtz_apply_f32(const SislOperator *op, const float *x, float *y, v
   return sisl::launch_A(o, x, y, sisl::as_stream(s));
 }
 int sisl_set_operator_shape(const SislGrid *g, int32_t fp32, int32_t chunk, int32_t unroll) {
-  if (!g || chunk < 2 || (unroll != 1 && unroll != 2)) return SISL_ERR_ARG;
+  if (!g || chunk < 2 || unroll < 1 || unroll > 3) return SISL_ERR_ARG;
   std::lock_guard<std::mutex> lk(sisl::g_tune_mu);
   for (int var = 0; var < 4; ++var) sisl::g_tune[{g->nx, g->ny, g->nz, fp32 ? 4 : 8, var}] = sisl::AShape{chunk, unroll};
   ++sisl::g_tune_generation;
