@@ -191,10 +191,9 @@ int64_t sisl_autotune_scratch_bytes(const SislGrid *g, int32_t fp32);
 int sisl_autotune_operator(const SislOperator *op, int32_t fp32, void *x, void *y, void *scratch,
                            int32_t *chunk, int32_t *unroll, void *stream);
 /* Pin the operator launch shape for grid g and element type (chunk >= 2
- * levels per thread, unroll 1 or 2, or 3: two adjacent columns per thread
- * in the forward operator, which needs an even nx and otherwise runs
- * unroll 1, as the transpose always does) of the operator and of its
- * transpose, with and without the fused basis normalisation, e.g. to
+ * levels per thread, unroll 1 or 2, or 3: two adjacent columns per thread,
+ * which needs an even nx and otherwise runs unroll 1) of the operator and of
+ * its transpose, with and without the fused basis normalisation, e.g. to
  * reproduce a tuned choice.  Every shape gives the same bits. */
 int sisl_set_operator_shape(const SislGrid *g, int32_t fp32, int32_t chunk, int32_t unroll);
 /* y = A^T x (the transpose used by the GMRES adjoint solve,

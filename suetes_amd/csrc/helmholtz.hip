@@ -869,9 +869,370 @@ __global__ void __launch_bounds__(256) helmholtz_T_kernel(OpT<T> o, const T *__r
   helmholtz_T_body<T, UNR, NB, TC>(o, y, x, kchunk, blockIdx.x, gridDim.x, inv, vout, vs, res);
 }
 
+// The transpose with two adjacent columns (i, i+1) per thread, i even: the
+// left-cell quantities of column i+1 (its intermediate u_m cotangent, the
+// Exner-flux term, the eta_dot cotangent and k_x at the w levels, the face
+// between the two cells) are column i's own, and the pair's loads of mass,
+// w-level and v-face arrays are two-element vector loads.  Each output is the
+// same sequence of operations on the same operands as in helmholtz_T_body.
+template <typename T, int NB = 0, typename TC = T>
+__device__ __forceinline__ void helmholtz_T2_body(const OpT<T> &o, const T *__restrict__ y,
+                                                  T *__restrict__ x, int kchunk, int blk, int nblk,
+                                                  const double *inv = nullptr, void *vout = nullptr,
+                                                  double vs = 1.0, const T *__restrict__ res = nullptr) {
+  const Stag &s = o.s;
+  const int nx = s.nx, ny = s.ny, nz = s.nz;
+  const int nbx = (nx + kTX2 - 1) / kTX2, nby = (ny + kTY - 1) / kTY;
+  const int tile = xcd_tile(blk, nblk);
+  const int bx = tile % nbx, by = (tile / nbx) % nby, bz = tile / (nbx * nby);
+  const int i = bx * kTX2 + 2 * threadIdx.x, j = by * kTY + threadIdx.y;
+  if (i >= nx || j >= ny) return;
+  const int k0 = bz * kchunk, k1 = min(k0 + kchunk, nz + 1);
+  const T *YU = y + s.ou, *YV = y + s.ov, *YW = y + s.ow, *YP = y + s.om, *YE = y + s.oe;
+  T *XU = x + s.ou, *XV = x + s.ov, *XW = x + s.ow, *XP = x + s.om, *XE = x + s.oe;
+  FUSED_NORMALISATION_SETUP
+  auto put = [&](T *X_, uint32_t q, T v) {
+    if constexpr (NB == 3) v = ldg(res + (X_ - x), q) - v;
+    stg(X_, q, so(v));
+  };
+  auto put2 = [&](T *X_, uint32_t q, T va, T vb) {
+    if constexpr (NB == 3) {
+      T ra, rb;
+      ldg2(res + (X_ - x), q, ra, rb);
+      va = ra - va;
+      vb = rb - vb;
+    }
+    stg2(X_, q, so(va), so(vb));
+  };
+  const bool px = s.px, py = s.py;
+  const bool hl0 = i >= 1, hd = j >= 1;  // column i+1 always has a left cell (column i)
+  const T half = (T)0.5, mhalf = (T)-0.5, alpha = o.alpha, ha = half * o.alpha;
+  const TC *__restrict__ c_cua = reinterpret_cast<const TC *>(o.cua);
+  const TC *__restrict__ c_zxu = reinterpret_cast<const TC *>(o.zxu);
+  const TC *__restrict__ c_cx = reinterpret_cast<const TC *>(o.cx);
+  const TC *__restrict__ c_cva = reinterpret_cast<const TC *>(o.cva);
+  const TC *__restrict__ c_zev = reinterpret_cast<const TC *>(o.zev);
+  const TC *__restrict__ c_cy = reinterpret_cast<const TC *>(o.cy);
+  const TC *__restrict__ c_dw = reinterpret_cast<const TC *>(o.dw);
+  const TC *__restrict__ c_cw = reinterpret_cast<const TC *>(o.cw);
+  const TC *__restrict__ c_cz = reinterpret_cast<const TC *>(o.cz);
+  const TC *__restrict__ c_kx = reinterpret_cast<const TC *>(o.kx);
+  const TC *__restrict__ c_ky = reinterpret_cast<const TC *>(o.ky);
+  const TC *__restrict__ c_rdzw = reinterpret_cast<const TC *>(o.rdzw);
+  const TC *__restrict__ c_cpm = reinterpret_cast<const TC *>(o.cpm);
+  const uint32_t pm = (uint32_t)nx * ny, pu = (uint32_t)(nx + 1) * ny, pv = (uint32_t)nx * (ny + 1);
+  const uint32_t mc = (uint32_t)j * nx + i;
+  const uint32_t uc = (uint32_t)j * (nx + 1) + i, vc = (uint32_t)j * nx + i;
+  const uint32_t u0r = (uint32_t)j * (nx + 1);
+  // folded face after each cell (periodic: face nx is face 0); column i's is i+1
+  const int fr0 = i + 1, fr1 = (px && i + 2 == nx) ? 0 : i + 2;
+  const int gr = (py && j + 1 == ny) ? 0 : j + 1;
+  const T mudx_a = ldg(o.mudx, (uint32_t)s.u2(i, j)), mudx_b = ldg(o.mudx, (uint32_t)s.u2(fr0, j));
+  const T mudx_c = ldg(o.mudx, (uint32_t)s.u2(fr1, j));
+  const T afu_a = ldg(o.afu, (uint32_t)s.u2(i, j)), afu_b = ldg(o.afu, (uint32_t)s.u2(fr0, j));
+  const T afu_c = ldg(o.afu, (uint32_t)s.u2(fr1, j));
+  const T mudx_i[2] = {mudx_a, mudx_b}, mudx_r[2] = {mudx_b, mudx_c};
+  const T afu_i[2] = {afu_a, afu_b}, afu_r[2] = {afu_b, afu_c};
+  T mvdy_j[2], mvdy_r[2], afv_j[2], afv_r[2];
+  ldg2(o.mvdy, (uint32_t)s.v2(i, j), mvdy_j[0], mvdy_j[1]);
+  ldg2(o.mvdy, (uint32_t)s.v2(i, gr), mvdy_r[0], mvdy_r[1]);
+  ldg2(o.afv, (uint32_t)s.v2(i, j), afv_j[0], afv_j[1]);
+  ldg2(o.afv, (uint32_t)s.v2(i, gr), afv_r[0], afv_r[1]);
+  const T afv_lj0 = hl0 ? ldg(o.afv, (uint32_t)s.v2(i - 1, j)) : (T)0;
+  const T afv_lr0 = hl0 ? ldg(o.afv, (uint32_t)s.v2(i - 1, gr)) : (T)0;
+  T afu_dj[2] = {0, 0}, afu_dr[2] = {0, 0};
+  if (hd) {
+    const T da = ldg(o.afu, (uint32_t)s.u2(i, j - 1)), db = ldg(o.afu, (uint32_t)s.u2(fr0, j - 1));
+    const T dc = ldg(o.afu, (uint32_t)s.u2(fr1, j - 1));
+    afu_dj[0] = da; afu_dr[0] = db; afu_dj[1] = db; afu_dr[1] = dc;
+  }
+  T mmx_c[2], mmy_c[2], mmy_d[2] = {0, 0};
+  ldg2(o.mmx, (uint32_t)s.m2(i, j), mmx_c[0], mmx_c[1]);
+  ldg2(o.mmy, (uint32_t)s.m2(i, j), mmy_c[0], mmy_c[1]);
+  const T mmx_l0 = hl0 ? ldg(o.mmx, (uint32_t)s.m2(i - 1, j)) : (T)0;
+  if (hd) ldg2(o.mmy, (uint32_t)s.m2(i, j - 1), mmy_d[0], mmy_d[1]);
+  const T mmx_l[2] = {mmx_l0, mmx_c[0]};
+  const int frs[2] = {fr0, fr1};
+  auto ut = [&](int f, uint32_t row, uint32_t q) -> T {
+    if (f == 0 || f == nx) return px ? ldg(YU, row + q) + ldg(YU, row + nx + q) : (T)0;
+    return ldg(YU, row + f + q);
+  };
+  auto vt = [&](int ii, int f, uint32_t q) -> T {
+    if (f == 0 || f == ny) return py ? ldg(YV, (uint32_t)ii + q) + ldg(YV, (uint32_t)ny * nx + ii + q) : (T)0;
+    return ldg(YV, (uint32_t)f * nx + ii + q);
+  };
+  // the pair (ii = i, i+1) of v-face cotangents of face row f
+  auto vt2 = [&](int f, uint32_t q, T &a, T &b) {
+    if (f == 0 || f == ny) {
+      if (py) {
+        T a0, b0, a1, b1;
+        ldg2(YV, (uint32_t)i + q, a0, b0);
+        ldg2(YV, (uint32_t)ny * nx + i + q, a1, b1);
+        a = a0 + a1;
+        b = b0 + b1;
+      } else {
+        a = (T)0;
+        b = (T)0;
+      }
+      return;
+    }
+    ldg2(YV, (uint32_t)f * nx + i + q, a, b);
+  };
+  auto gzmb = [&](int c, T uti, T utr, T vtj, T vtr, int k) -> T {
+    const uint32_t qu = (uint32_t)k * pu, qv = (uint32_t)k * pv;
+    const uint32_t al = uc + c + qu, ar = u0r + frs[c] + qu, bl = vc + c + qv, br = (uint32_t)gr * nx + i + c + qv;
+    T g = mhalf * (uti * ldc<T>(c_cua, al) * ldc<T>(c_zxu, al) + utr * ldc<T>(c_cua, ar) * ldc<T>(c_zxu, ar));
+    g += mhalf * (vtj * ldc<T>(c_cva, bl) * ldc<T>(c_zev, bl) + vtr * ldc<T>(c_cva, br) * ldc<T>(c_zev, br));
+    return g;
+  };
+  T ut_i[2] = {0, 0}, ut_r[2] = {0, 0}, vt_j[2] = {0, 0}, vt_r[2] = {0, 0};
+  T gm1[2] = {0, 0}, g0[2] = {0, 0}, cpmP_m1[2] = {0, 0};
+  T e0[2] = {0, 0}, e0l0 = 0, e0d[2] = {0, 0};
+  T kx0[2] = {0, 0}, kx0l0 = 0, ky0[2] = {0, 0}, ky0d[2] = {0, 0};
+  T cwW0[2] = {0, 0};
+  auto load_faces = [&](uint32_t qu, uint32_t qv, T (&uti)[2], T (&utr)[2], T (&vtj)[2], T (&vtr)[2]) {
+    const T fa = ut(i, u0r, qu), fb = ut(i + 1, u0r, qu), fc = ut(i + 2, u0r, qu);
+    uti[0] = fa; utr[0] = fb; uti[1] = fb; utr[1] = fc;
+    vt2(j, qv, vtj[0], vtj[1]);
+    vt2(j + 1, qv, vtr[0], vtr[1]);
+  };
+  if (k0 < nz) {
+    load_faces((uint32_t)k0 * pu, (uint32_t)k0 * pv, ut_i, ut_r, vt_j, vt_r);
+    for (int c = 0; c < 2; ++c) g0[c] = gzmb(c, ut_i[c], ut_r[c], vt_j[c], vt_r[c], k0);
+  }
+  if (k0 >= 1) {
+    const int km = k0 - 1;
+    T a[2], b[2], cc[2], d[2];
+    load_faces((uint32_t)km * pu, (uint32_t)km * pv, a, b, cc, d);
+    for (int c = 0; c < 2; ++c) gm1[c] = gzmb(c, a[c], b[c], cc[c], d[c], km);
+    const uint32_t m = mc + (uint32_t)km * pm;
+    T cp[2], yp[2];
+    ldc2<T>(c_cpm, m, cp[0], cp[1]);
+    ldg2(YP, m, yp[0], yp[1]);
+    cpmP_m1[0] = cp[0] * yp[0];
+    cpmP_m1[1] = cp[1] * yp[1];
+  }
+  if (k0 >= 1 && k0 <= nz - 1) {
+    const uint32_t w = mc + (uint32_t)k0 * pm;
+    ldg2(YE, w, e0[0], e0[1]);
+    if (hl0) e0l0 = ldg(YE, w - 1);
+    if (hd) ldg2(YE, w - nx, e0d[0], e0d[1]);
+    T cw[2], yw[2];
+    ldc2<T>(c_cw, w, cw[0], cw[1]);
+    ldg2(YW, w, yw[0], yw[1]);
+    cwW0[0] = cw[0] * yw[0];
+    cwW0[1] = cw[1] * yw[1];
+  }
+  if (k0 <= nz) {
+    const uint32_t w = mc + (uint32_t)k0 * pm;
+    ldc2<T>(c_kx, w, kx0[0], kx0[1]);
+    ldc2<T>(c_ky, w, ky0[0], ky0[1]);
+    if (hl0) kx0l0 = ldc<T>(c_kx, w - 1);
+    if (hd) ldc2<T>(c_ky, w - nx, ky0d[0], ky0d[1]);
+  }
+  for (int k = k0; k < k1; ++k) {
+    const uint32_t qm = (uint32_t)k * pm, qu = (uint32_t)k * pu, qv = (uint32_t)k * pv;
+    const uint32_t wk = mc + qm;
+    if (k == nz) {  // top w level: identity rows
+      T yw[2], ye[2], r[2];
+      ldg2(YW, wk, yw[0], yw[1]);
+      ldg2(YE, wk, ye[0], ye[1]);
+      ldc2<T>(c_rdzw, wk, r[0], r[1]);
+      put2(XW, wk, yw[0], yw[1]);
+      put2(XE, wk, ye[0] * r[0], ye[1] * r[1]);
+      bput(BW, wk, yw[0]); bput(BW, wk + 1, yw[1]);
+      bput(BE, wk, ye[0]); bput(BE, wk + 1, ye[1]);
+      break;
+    }
+    // ---- level k+1 data ----
+    T ut_i1[2] = {0, 0}, ut_r1[2] = {0, 0}, vt_j1[2] = {0, 0}, vt_r1[2] = {0, 0}, g1[2] = {0, 0};
+    T e1[2] = {0, 0}, e1l0 = 0, e1d[2] = {0, 0}, cwW1[2] = {0, 0};
+    const uint32_t w1 = wk + pm;
+    if (k + 1 < nz) {
+      load_faces(qu + pu, qv + pv, ut_i1, ut_r1, vt_j1, vt_r1);
+      for (int c = 0; c < 2; ++c) g1[c] = gzmb(c, ut_i1[c], ut_r1[c], vt_j1[c], vt_r1[c], k + 1);
+      ldg2(YE, w1, e1[0], e1[1]);
+      if (hl0) e1l0 = ldg(YE, w1 - 1);
+      if (hd) ldg2(YE, w1 - nx, e1d[0], e1d[1]);
+      T cw[2], yw[2];
+      ldc2<T>(c_cw, w1, cw[0], cw[1]);
+      ldg2(YW, w1, yw[0], yw[1]);
+      cwW1[0] = cw[0] * yw[0];
+      cwW1[1] = cw[1] * yw[1];
+    }
+    T kx1[2], ky1[2], ky1d[2] = {0, 0};
+    ldc2<T>(c_kx, w1, kx1[0], kx1[1]);
+    ldc2<T>(c_ky, w1, ky1[0], ky1[1]);
+    const T kx1l0 = hl0 ? ldc<T>(c_kx, w1 - 1) : (T)0;
+    if (hd) ldc2<T>(c_ky, w1 - nx, ky1d[0], ky1d[1]);
+    // ---- mass-point intermediates at level k ----
+    T umb[2], vmb[2], vmb_d[2] = {0, 0};
+    for (int c = 0; c < 2; ++c) {
+      umb[c] = half * (vt_j[c] * afv_j[c] + vt_r[c] * afv_r[c]) + ha * (e0[c] * kx0[c] + e1[c] * kx1[c]);
+      vmb[c] = mhalf * (ut_i[c] * afu_i[c] + ut_r[c] * afu_r[c]) + ha * (e0[c] * ky0[c] + e1[c] * ky1[c]);
+    }
+    T umb_l0 = 0;
+    if (hl0) umb_l0 = half * (vt(i - 1, j, qv) * afv_lj0 + vt(i - 1, j + 1, qv) * afv_lr0) + ha * (e0l0 * kx0l0 + e1l0 * kx1l0);
+    if (hd) {
+      const uint32_t row = u0r - (nx + 1);
+      const T da = ut(i, row, qu), db = ut(i + 1, row, qu), dc = ut(i + 2, row, qu);
+      vmb_d[0] = mhalf * (da * afu_dj[0] + db * afu_dr[0]) + ha * (e0d[0] * ky0d[0] + e1d[0] * ky1d[0]);
+      vmb_d[1] = mhalf * (db * afu_dj[1] + dc * afu_dr[1]) + ha * (e0d[1] * ky0d[1] + e1d[1] * ky1d[1]);
+    }
+    const T umb_l[2] = {umb_l0, umb[0]};
+    T yp[2], cpm[2], cpmP[2], cpmPd[2] = {0, 0};
+    ldg2(YP, wk, yp[0], yp[1]);
+    ldc2<T>(c_cpm, wk, cpm[0], cpm[1]);
+    cpmP[0] = cpm[0] * yp[0];
+    cpmP[1] = cpm[1] * yp[1];
+    const T cpmPl0 = hl0 ? ldc<T>(c_cpm, wk - 1) * ldg(YP, wk - 1) : (T)0;
+    if (hd) {
+      T cd[2], pd[2];
+      ldc2<T>(c_cpm, wk - nx, cd[0], cd[1]);
+      ldg2(YP, wk - nx, pd[0], pd[1]);
+      cpmPd[0] = cd[0] * pd[0];
+      cpmPd[1] = cd[1] * pd[1];
+    }
+    const T cpmPl[2] = {cpmPl0, cpmP[0]};
+    // ---- u faces i, i+1 (and the east face) ----
+    for (int c = 0; c < 2; ++c) {
+      const bool hl = c == 1 || hl0;
+      const uint32_t au = uc + c + qu;
+      const T yu = ldg(YU, au);
+      bput(BU, au, yu);
+      T idp = yu;
+      if (px && i + c == 0) idp = yu + ldg(YU, au + nx);
+      T fl = 0;
+      if (hl) fl += cpmPl[c] * mmx_l[c];
+      fl -= cpmP[c] * mmx_c[c];
+      T out = idp + ldc<T>(c_cx, au) * fl;
+      T ub = 0;
+      if (hl) ub += umb_l[c];
+      ub += umb[c];
+      put(XU, au, out + half * ub);
+      if (c == 1 && i + 1 == nx - 1) {
+        const T yn = ldg(YU, au + 1);
+        bput(BU, au + 1, yn);
+        const T idn = px ? (T)0 : yn;
+        put(XU, au + 1, idn + ldc<T>(c_cx, au + 1) * (cpmP[1] * mmx_c[1]) + half * umb[1]);
+      }
+    }
+    // ---- v face j of both columns (and the north face) ----
+    {
+      const uint32_t av = vc + qv;
+      T yv[2];
+      ldg2(YV, av, yv[0], yv[1]);
+      bput(BV, av, yv[0]); bput(BV, av + 1, yv[1]);
+      T idp[2] = {yv[0], yv[1]};
+      if (py && j == 0) {
+        T pa, pb;
+        ldg2(YV, av + (uint32_t)ny * nx, pa, pb);
+        idp[0] = yv[0] + pa;
+        idp[1] = yv[1] + pb;
+      }
+      T cyv[2];
+      ldc2<T>(c_cy, av, cyv[0], cyv[1]);
+      T o2[2];
+      for (int c = 0; c < 2; ++c) {
+        T fl = 0;
+        if (hd) fl += cpmPd[c] * mmy_d[c];
+        fl -= cpmP[c] * mmy_c[c];
+        T out = idp[c] + cyv[c] * fl;
+        T vb = 0;
+        if (hd) vb += vmb_d[c];
+        vb += vmb[c];
+        o2[c] = out + half * vb;
+      }
+      put2(XV, av, o2[0], o2[1]);
+      if (j == ny - 1) {
+        T yn[2], cyn[2];
+        ldg2(YV, av + nx, yn[0], yn[1]);
+        bput(BV, av + nx, yn[0]); bput(BV, av + nx + 1, yn[1]);
+        ldc2<T>(c_cy, av + nx, cyn[0], cyn[1]);
+        T on[2];
+        for (int c = 0; c < 2; ++c) {
+          const T idn = py ? (T)0 : yn[c];
+          on[c] = idn + cyn[c] * (cpmP[c] * mmy_c[c]) + half * vmb[c];
+        }
+        put2(XV, av + nx, on[0], on[1]);
+      }
+    }
+    // ---- pi ----
+    {
+      T r0[2], rn[2], r1[2], rk[2];
+      ldc2<T>(c_rdzw, wk, rk[0], rk[1]);
+      ldc2<T>(c_rdzw, mc, r0[0], r0[1]);
+      ldc2<T>(c_rdzw, mc + (uint32_t)nz * pm, rn[0], rn[1]);
+      ldc2<T>(c_rdzw, w1, r1[0], r1[1]);
+      T outp[2];
+      for (int c = 0; c < 2; ++c) {
+        const uint32_t al = uc + c + qu, ar = u0r + frs[c] + qu, bl = vc + c + qv, br = (uint32_t)gr * nx + i + c + qv;
+        T out = yp[c];
+        out += ut_i[c] * ldc<T>(c_cua, al) * mudx_i[c];
+        out -= ut_r[c] * ldc<T>(c_cua, ar) * mudx_r[c];
+        out += vt_j[c] * ldc<T>(c_cva, bl) * mvdy_j[c];
+        out -= vt_r[c] * ldc<T>(c_cva, br) * mvdy_r[c];
+        if (k >= 1) out += cwW0[c];
+        if (k + 1 <= nz - 1) out -= cwW1[c];
+        if (k >= 1) out += half * (gm1[c] + g0[c]) * rk[c];
+        if (k == 1) out += half * gm1[c] * r0[c];
+        if (k == nz - 1) out += half * g0[c] * rn[c];
+        if (k + 1 <= nz - 1) out -= half * (g0[c] + g1[c]) * r1[c];
+        if (k == 0) out -= half * g0[c] * r0[c];
+        if (k == nz - 2) out -= half * g1[c] * rn[c];
+        outp[c] = out;
+      }
+      put2(XP, wk, outp[0], outp[1]);
+      bput(BP, wk, yp[0]); bput(BP, wk + 1, yp[1]);
+    }
+    // ---- w level k: w and eta_dot ----
+    {
+      T yw[2];
+      ldg2(YW, wk, yw[0], yw[1]);
+      bput(BW, wk, yw[0]); bput(BW, wk + 1, yw[1]);
+      if (k == 0) {
+        T ye[2], r[2];
+        ldg2(YE, wk, ye[0], ye[1]);
+        ldc2<T>(c_rdzw, wk, r[0], r[1]);
+        bput(BE, wk, ye[0]); bput(BE, wk + 1, ye[1]);
+        put2(XW, wk, yw[0], yw[1]);
+        put2(XE, wk, ye[0] * r[0], ye[1] * r[1]);
+      } else {
+        T dw[2], cz[2];
+        ldc2<T>(c_dw, wk, dw[0], dw[1]);
+        ldc2<T>(c_cz, wk, cz[0], cz[1]);
+        bput(BE, wk, e0[0]); bput(BE, wk + 1, e0[1]);
+        put2(XW, wk, dw[0] * yw[0] - alpha * e0[0], dw[1] * yw[1] - alpha * e0[1]);
+        put2(XE, wk, alpha * e0[0] + cz[0] * (cpmP_m1[0] - cpmP[0]), alpha * e0[1] + cz[1] * (cpmP_m1[1] - cpmP[1]));
+      }
+    }
+    // k+1 becomes k
+    for (int c = 0; c < 2; ++c) {
+      ut_i[c] = ut_i1[c]; ut_r[c] = ut_r1[c]; vt_j[c] = vt_j1[c]; vt_r[c] = vt_r1[c];
+      gm1[c] = g0[c]; g0[c] = g1[c];
+      cpmP_m1[c] = cpmP[c];
+      e0[c] = e1[c]; e0d[c] = e1d[c];
+      kx0[c] = kx1[c]; ky0[c] = ky1[c]; ky0d[c] = ky1d[c];
+      cwW0[c] = cwW1[c];
+    }
+    e0l0 = e1l0;
+    kx0l0 = kx1l0;
+  }
+}
+
+template <typename T, int NB, typename TC>
+__global__ void __launch_bounds__(256) helmholtz_T2_kernel(OpT<T> o, const T *__restrict__ y,
+                                                           T *__restrict__ x, int kchunk,
+                                                           const int *__restrict__ stop,
+                                                           const double *__restrict__ inv, void *vout,
+                                                           double vs, const T *__restrict__ res) {
+  if (stop && *stop) return;
+  helmholtz_T2_body<T, NB, TC>(o, y, x, kchunk, blockIdx.x, gridDim.x, inv, vout, vs, res);
+}
+
 // Launch shape of the operator and of its transpose: levels per thread
-// (chunk) and the unroll of the level loop (1, 2), or 3: the forward
-// operator with two columns per thread (its transpose then runs unroll 1).  Every shape computes
+// (chunk) and the unroll of the level loop (1, 2), or 3: two adjacent
+// columns per thread (even nx; otherwise unroll 1).  Every shape computes
 // bit-identical results; the autotuner below picks one per grid, element type
 // and direction by timing, the default is 8 chunks per column without
 // unrolling.
@@ -1031,10 +1392,56 @@ int launch_A(const OpT<T> &o, const T *x, T *y, hipStream_t st, const int *stop,
   return SISL_OK;
 }
 
+template <typename T, int NB, typename TC>
+static void goT2(dim3 g, hipStream_t st, const OpT<T> &o, const T *y, T *x, int kc, const int *stop,
+                 const Fuse &f) {
+  hipLaunchKernelGGL((helmholtz_T2_kernel<T, NB, TC>), g, dim3(kTX, kTY), 0, st, o, y, x, kc, stop, f.inv,
+                     f.vout, f.vs, static_cast<const T *>(f.res));
+}
+template <typename T, typename TC>
+static int launchT2_tc(dim3 g, const OpT<T> &o, const T *y, T *x, hipStream_t st, const int *stop, int kc,
+                       const Fuse &f) {
+  if (f.res) {
+    if (f.nb) return SISL_ERR_ARG;
+    if constexpr (sizeof(T) == 8) {
+      goT2<T, 3, TC>(g, st, o, y, x, kc, stop, f);
+      return SISL_OK;
+    }
+    return SISL_ERR_ARG;
+  }
+  if (f.nb == 0) {
+    goT2<T, 0, TC>(g, st, o, y, x, kc, stop, f);
+    return SISL_OK;
+  }
+  if constexpr (sizeof(T) == 4) {
+    if (f.nb == 1) {
+      goT2<T, 1, TC>(g, st, o, y, x, kc, stop, f);
+      return SISL_OK;
+    }
+    if (f.nb == 2) {
+      goT2<T, 2, TC>(g, st, o, y, x, kc, stop, f);
+      return SISL_OK;
+    }
+  }
+  return SISL_ERR_ARG;
+}
+
 template <typename T>
 static int launch_AT_shape(const OpT<T> &o, const T *y, T *x, hipStream_t st, const int *stop, AShape sh,
                            const Fuse &f = Fuse{}) {
-  if (sh.unr == 3) sh.unr = 1;  // the two-column form exists for the forward operator only
+  if (sh.unr == 3) {
+    if (two_column_ok(o.s)) {
+      const int kc = sh.kc;
+      const dim3 g((unsigned)(((o.s.nx + kTX2 - 1) / kTX2) * ((o.s.ny + kTY - 1) / kTY) *
+                              ((o.s.nz + 1 + kc - 1) / kc)));
+      if (f.cb) {
+        if constexpr (sizeof(T) == 4) return launchT2_tc<T, bf16_t>(g, o, y, x, st, stop, kc, f);
+        return SISL_ERR_ARG;
+      }
+      return launchT2_tc<T, T>(g, o, y, x, st, stop, kc, f);
+    }
+    sh.unr = 1;  // odd nx: one column per thread
+  }
   return launch_shape<ATKern, T>(o, y, x, st, stop, sh, f);
 }
 
@@ -1108,7 +1515,6 @@ static int autotune_A(const OpT<T> &o, const T *x, T *y, void *scratch, hipStrea
     const Fuse f = var >= 2 ? fz : Fuse{};
     float best_ms = 1e30f;
     for (const AShape &c : cand) {
-      if (tr && c.unr == 3) continue;  // forward only
       auto run = [&]() {
         if (tr) launch_AT_shape(o, x, y, st, nullptr, c, f);
         else launch_A_shape(o, x, y, st, nullptr, c, f);

@@ -325,15 +325,16 @@ def test_mixed_solver_reaches_tolerance(basis, coefs, gs):
         assert float((x - x64).norm() / x64.norm()) < 1e-8, transpose
 
 
+@pytest.mark.parametrize("cfgf", [small_lam, small_bubble])
 @pytest.mark.parametrize("transpose", [False, True])
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
-def test_operator_launch_shapes_bit_identical_and_autotune(dtype, transpose):
+def test_operator_launch_shapes_bit_identical_and_autotune(dtype, transpose, cfgf):
     """Every launch shape of the operator apply (and of its transpose) gives
-    the same bits; the autotuner returns a valid shape and leaves results
-    unchanged."""
+    the same bits, on a limited-area grid and on a doubly periodic one; the
+    autotuner returns a valid shape and leaves results unchanged."""
     import ctypes as C
     from suetes_amd import _lib
-    case, go, A_o, M_o, sol = _solver_pair(small_lam())
+    case, go, A_o, M_o, sol = _solver_pair(cfgf())
     ops = sol.op64
     if dtype == torch.float32:
         from suetes_amd.steppers import _OperatorSet
@@ -358,7 +359,7 @@ def test_operator_launch_shapes_bit_identical_and_autotune(dtype, transpose):
 
     outs, shapes = [], []
     for chunk in (2, 3, 5, 13, 81):
-        for unr in (1, 2, 3):  # 3: two columns per thread (forward operator)
+        for unr in (1, 2, 3):  # 3: two columns per thread
             _lib.check(L.sisl_set_operator_shape(C.byref(g), fp32, chunk, unr), "shape")
             outs.append(apply())
             shapes.append((chunk, unr))
