@@ -444,8 +444,10 @@ int sisl_ffsl_1d_f64_noalloc(const double *mass, const double *cfl, double *out,
 
 /* on = 1 (default): the mixed solver captures each inner Arnoldi cycle into
  * a hipGraph once (per workspace, operator, restart length, direction and
- * coefficient storage) and replays it for every later cycle; on = 0: eager
- * launches (bit-identical results).  No counterpart in the reference. */
+ * coefficient storage) and replays it for every later cycle, and so does the
+ * batched solver (per workspace layout, direction and Gram-Schmidt form);
+ * on = 0: eager launches (bit-identical results).  No counterpart in the
+ * reference. */
 int sisl_set_graphs(int32_t on);
 
 /* ---- kernel timing (measurement runs) ------------------------------------ */

@@ -133,7 +133,9 @@ __device__ inline T wave_sum(T v) {
 namespace sisl {
 // One member's operator application in a batched launch (device table entry):
 // input x + j * xj, output y, transpose scratch scr (2 mass arrays), skipped
-// when stop is set and stop[j * sj] != 0.
+// when stop is set and stop[j * sj] != 0.  Operator launches with the fused
+// basis normalisation (FP32 operator, FP16 basis) also read the scale *inv
+// and store the basis column vout + j * vld (vs: the basis storage scale).
 template <typename T>
 struct BOp {
   OpT<T> o;
@@ -143,13 +145,18 @@ struct BOp {
   T *scr;
   const int *stop;
   int sj;
+  const double *inv;
+  void *vout;
+  int64_t vld;
+  double vs;
 };
 // Batched launchers: member m = blockIdx.y over M members; `tab` in device
-// memory; `maxb` = the largest per-member block count (host-computed).
+// memory; `maxb` = the largest per-member block count (host-computed);
+// nb = 1: the operator launch does the fused basis normalisation.
 template <typename T>
-int launch_A_batched(const BOp<T> *tab, int M, int maxb, int j, hipStream_t s);
+int launch_A_batched(const BOp<T> *tab, int M, int maxb, int j, hipStream_t s, int nb = 0);
 template <typename T>
-int launch_AT_batched(const BOp<T> *tab, int M, int maxb, int j, hipStream_t s);
+int launch_AT_batched(const BOp<T> *tab, int M, int maxb, int j, hipStream_t s, int nb = 0);
 template <typename T>
 int launch_M_batched(const BOp<T> *tab, int M, int maxb, int L, int S, int j, hipStream_t s);
 template <typename T>

@@ -1549,40 +1549,64 @@ int blocks_A(const Stag &s) {
   const int kc = (s.nz + 1 + 7) / 8;
   return ((s.nx + kTX - 1) / kTX) * ((s.ny + kTY - 1) / kTY) * ((s.nz + 1 + kc - 1) / kc);
 }
-template <typename T>
+// NB = 1: fused basis normalisation (scale *b.inv, FP16 basis column j).
+// One column per thread: the members' grids are small, and the two-column
+// form's halved thread count ran the 32-member ensemble's operator slower
+// (1.19 vs 0.99 ms per launch).
+template <typename T, int NB>
 __global__ void __launch_bounds__(256) helmholtz_batched_kernel(const BOp<T> *__restrict__ tab, int j) {
   const BOp<T> &b = tab[blockIdx.y];
   if (b.stop && b.stop[j * b.sj]) return;
   const int nb = blocks_A_dev(b.o.s);
   if ((int)blockIdx.x >= nb) return;
-  helmholtz_body<T, 1, 0, T>(b.o, b.x + j * b.xj, b.y, (b.o.s.nz + 1 + 7) / 8, blockIdx.x, nb);
+  void *vout = nullptr;
+  if constexpr (NB == 1) vout = static_cast<_Float16 *>(b.vout) + j * b.vld;
+  helmholtz_body<T, 1, NB, T>(b.o, b.x + j * b.xj, b.y, (b.o.s.nz + 1 + 7) / 8, blockIdx.x, nb, b.inv, vout,
+                              b.vs);
 }
 
-template <typename T>
+template <typename T, int NB>
 __global__ void __launch_bounds__(256) helmholtz_T_batched_kernel(const BOp<T> *__restrict__ tab, int j) {
   const BOp<T> &b = tab[blockIdx.y];
   if (b.stop && b.stop[j * b.sj]) return;
   const int nb = blocks_A_dev(b.o.s);
   if ((int)blockIdx.x >= nb) return;
-  helmholtz_T_body<T, 1, 0, T>(b.o, b.x + j * b.xj, b.y, (b.o.s.nz + 1 + 7) / 8, blockIdx.x, nb);
+  void *vout = nullptr;
+  if constexpr (NB == 1) vout = static_cast<_Float16 *>(b.vout) + j * b.vld;
+  helmholtz_T_body<T, 1, NB, T>(b.o, b.x + j * b.xj, b.y, (b.o.s.nz + 1 + 7) / 8, blockIdx.x, nb, b.inv, vout,
+                                b.vs);
 }
 
 template <typename T>
-int launch_A_batched(const BOp<T> *tab, int M, int maxb, int j, hipStream_t st) {
-  hipLaunchKernelGGL(helmholtz_batched_kernel<T>, dim3(maxb, M), dim3(kTX, kTY), 0, st, tab, j);
+int launch_A_batched(const BOp<T> *tab, int M, int maxb, int j, hipStream_t st, int nb) {
+  if (nb) {
+    if constexpr (sizeof(T) == 4)
+      hipLaunchKernelGGL((helmholtz_batched_kernel<T, 1>), dim3(maxb, M), dim3(kTX, kTY), 0, st, tab, j);
+    else
+      return SISL_ERR_ARG;
+  } else {
+    hipLaunchKernelGGL((helmholtz_batched_kernel<T, 0>), dim3(maxb, M), dim3(kTX, kTY), 0, st, tab, j);
+  }
   SISL_CHECK_LAUNCH();
   return SISL_OK;
 }
 template <typename T>
-int launch_AT_batched(const BOp<T> *tab, int M, int maxb, int j, hipStream_t st) {
-  hipLaunchKernelGGL(helmholtz_T_batched_kernel<T>, dim3(maxb, M), dim3(kTX, kTY), 0, st, tab, j);
+int launch_AT_batched(const BOp<T> *tab, int M, int maxb, int j, hipStream_t st, int nb) {
+  if (nb) {
+    if constexpr (sizeof(T) == 4)
+      hipLaunchKernelGGL((helmholtz_T_batched_kernel<T, 1>), dim3(maxb, M), dim3(kTX, kTY), 0, st, tab, j);
+    else
+      return SISL_ERR_ARG;
+  } else {
+    hipLaunchKernelGGL((helmholtz_T_batched_kernel<T, 0>), dim3(maxb, M), dim3(kTX, kTY), 0, st, tab, j);
+  }
   SISL_CHECK_LAUNCH();
   return SISL_OK;
 }
-template int launch_A_batched<double>(const BOp<double> *, int, int, int, hipStream_t);
-template int launch_A_batched<float>(const BOp<float> *, int, int, int, hipStream_t);
-template int launch_AT_batched<double>(const BOp<double> *, int, int, int, hipStream_t);
-template int launch_AT_batched<float>(const BOp<float> *, int, int, int, hipStream_t);
+template int launch_A_batched<double>(const BOp<double> *, int, int, int, hipStream_t, int);
+template int launch_A_batched<float>(const BOp<float> *, int, int, int, hipStream_t, int);
+template int launch_AT_batched<double>(const BOp<double> *, int, int, int, hipStream_t, int);
+template int launch_AT_batched<float>(const BOp<float> *, int, int, int, hipStream_t, int);
 
 template int launch_A<double>(const OpT<double> &, const double *, double *, hipStream_t, const int *,
                               const Fuse *);

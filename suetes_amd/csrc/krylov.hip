@@ -617,13 +617,12 @@ __global__ void unit_kernel(const TS *__restrict__ src, TB *__restrict__ out, TC
 // the unnormalised start vector of an inner cycle, which the cycle's first
 // operator launch normalises (one read of w instead of a norm pass and a
 // separate normalisation pass).
-__global__ void __launch_bounds__(256) norm2_to_f32_kernel(const double *__restrict__ w,
-                                                           float *__restrict__ out, int64_t n,
-                                                           double *__restrict__ partials) {
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+__device__ __forceinline__ void norm2_to_f32_body(const double *__restrict__ w, float *__restrict__ out,
+                                                  int64_t n, double *__restrict__ partials, int blk, int nblk) {
+  const int64_t stride = (int64_t)nblk * blockDim.x;
   double acc = 0;
   const int64_t nq = n / 2;
-  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += stride) {
+  for (int64_t q = (int64_t)blk * blockDim.x + threadIdx.x; q < nq; q += stride) {
     double e[2];
     vload<2>(w, q, e);
     acc += e[0] * e[0] + e[1] * e[1];
@@ -632,12 +631,17 @@ __global__ void __launch_bounds__(256) norm2_to_f32_kernel(const double *__restr
     f.y = (float)e[1];
     reinterpret_cast<float2 *>(out)[q] = f;
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0 && (n & 1)) {
+  if (blk == 0 && threadIdx.x == 0 && (n & 1)) {
     const double v = w[n - 1];
     acc += v * v;
     out[n - 1] = (float)v;
   }
-  block_store_slot(acc, 0, partials, blockIdx.x);
+  block_store_slot(acc, 0, partials, blk);
+}
+__global__ void __launch_bounds__(256) norm2_to_f32_kernel(const double *__restrict__ w,
+                                                           float *__restrict__ out, int64_t n,
+                                                           double *__restrict__ partials) {
+  norm2_to_f32_body(w, out, n, partials, blockIdx.x, gridDim.x);
 }
 
 // x += sum_v y[v] vinv V[v]   (x is FP64)
@@ -1335,24 +1339,27 @@ __device__ __forceinline__ double *small_slot(Small *sm, int sel) {
 }
 
 // Arnoldi-step gating of member m at step j: stop = flags[j]; `gated`
-// additionally requires the second Gram-Schmidt pass.
+// additionally requires the second Gram-Schmidt pass.  The Arnoldi vector of
+// step j lives in w (even j) or cur (odd j): the operator launch of step j
+// reads the previous, unnormalised one and normalises it on the fly.
+__device__ __forceinline__ float *kb_arnoldi_vec(const KMem &k, int j) { return (j & 1) ? k.cur : k.w; }
 template <int G>
 __global__ void __launch_bounds__(256) kb_dots1_kernel(const KMem *__restrict__ km, int nv, int j,
                                                       int gated) {
   const KMem &k = km[blockIdx.y];
-  dots1_body<BT, float, G>(k.V, k.ldv, nv, k.w, k.n, k.partials, k.vinv, k.flags + j,
+  dots1_body<BT, float, G>(k.V, k.ldv, nv, kb_arnoldi_vec(k, j), k.n, k.partials, k.vinv, k.flags + j,
                            gated ? &k.sm->gate2 : nullptr, blockIdx.x, gridDim.x);
 }
 __global__ void __launch_bounds__(256) kb_dots_kernel(const KMem *__restrict__ km, int nv, int j,
                                                      int gated) {
   const KMem &k = km[blockIdx.y];
-  dots_body<BT, float>(k.V, k.ldv, nv, k.w, k.n, k.partials, k.vinv, k.flags + j,
+  dots_body<BT, float>(k.V, k.ldv, nv, kb_arnoldi_vec(k, j), k.n, k.partials, k.vinv, k.flags + j,
                        gated ? &k.sm->gate2 : nullptr, blockIdx.x, gridDim.x);
 }
 __global__ void __launch_bounds__(256) kb_update_kernel(const KMem *__restrict__ km, int nv, int j,
                                                        int sel_h, int gated) {
   const KMem &k = km[blockIdx.y];
-  update_body<BT, float>(k.V, k.ldv, nv, small_slot(k.sm, sel_h), k.w, k.n, k.partials, k.vinv,
+  update_body<BT, float>(k.V, k.ldv, nv, small_slot(k.sm, sel_h), kb_arnoldi_vec(k, j), k.n, k.partials, k.vinv,
                          k.flags + j, gated ? &k.sm->gate2 : nullptr, blockIdx.x, gridDim.x);
 }
 // out[v] = sum over blocks of partials[v][b]; grid (nslots, M).  mode 0:
@@ -1386,11 +1393,6 @@ __global__ void kb_decide_kernel(const KMem *__restrict__ km, int nv) {
 __global__ void kb_finalize_kernel(const KMem *__restrict__ km, int j, int R) {
   if (threadIdx.x == 0) finalize_body(km[blockIdx.x].sm, j, R, 1);
 }
-__global__ void __launch_bounds__(256) kb_normalize_kernel(const KMem *__restrict__ km, int j) {
-  const KMem &k = km[blockIdx.y];
-  normalize_gated_body<BT, float>(k.w, k.V + (int64_t)(j + 1) * k.ldv, k.cur, k.n, k.vscale, k.sm,
-                                  k.flags + j, blockIdx.x, gridDim.x);
-}
 // ||vec||^2 into scal[slot]: vec 0 = b64, 1 = w64 (outer loop, skipped when done)
 __global__ void __launch_bounds__(256) kb_norm2_kernel(const KMem *__restrict__ km, int which) {
   const KMem &k = km[blockIdx.y];
@@ -1405,17 +1407,12 @@ __global__ void kb_residual_kernel(const KMem *__restrict__ km) {  // t64 = b64 
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < k.n; e += stride)
     k.t64[e] = k.b64[e] - k.t64[e];
 }
-__global__ void kb_unit_kernel(const KMem *__restrict__ km) {  // V[0] = vscale w64/|w64|, cur
+// ||w64||^2 partials and the FP32 copy cur = w64, the next cycle's
+// unnormalised start vector (its first operator launch normalises it).
+__global__ void __launch_bounds__(256) kb_norm2_f32_kernel(const KMem *__restrict__ km) {
   const KMem &k = km[blockIdx.y];
   if (k.sm->done) return;
-  const double nr = sqrt(k.sm->scal[2]);
-  const double inv = nr > 2.220446049250313e-16 ? 1.0 / nr : 0.0;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < k.n; e += stride) {
-    const double u = inv != 0.0 ? k.w64[e] * inv : 0.0;
-    k.V[e] = (BT)(u * k.vscale);
-    k.cur[e] = (float)u;
-  }
+  norm2_to_f32_body(k.w64, k.cur, k.n, k.partials, blockIdx.x, gridDim.x);
 }
 // First kernel of a solve: clear the previous solve's done flag.
 __global__ void kb_begin_kernel(const KMem *__restrict__ km) {
@@ -1460,7 +1457,7 @@ __global__ void kb_reset_kernel(const KMem *__restrict__ km, int R, double inner
     sm->broken = fin;
     sm->steps = 0;
     sm->gate2 = 0;
-    sm->inv = 0.0;
+    sm->inv = (!fin && sm->rnorm > 0.0) ? 1.0 / sm->rnorm : 0.0;  // normalises the start vector
     for (int v = 0; v <= R; ++v) sm->g[v] = 0.0;
     sm->g[0] = sm->rnorm;
     sm->target = fmax(inner_rtol * sm->rnorm, 0.5 * sm->ptol);
@@ -1499,8 +1496,12 @@ struct EnsPlan {
   int64_t maxN;
   std::vector<int64_t> n;
   KMem *km;                  // device table
-  BOp<float> *A32, *M32;     // inner: cur -> w (stop flags[j]); in-place w
-  BOp<float> *AT32, *MT32;   // inner transpose: cur -> t; t -> w
+  // inner cycle, step parity q = j % 2 (stop flags[j]): the operator launch
+  // reads the previous unnormalised Arnoldi vector src_q (cur for even
+  // steps, w for odd ones), normalises it into basis column j and writes
+  // dst_q (the other buffer); the preconditioner runs in place on dst_q
+  BOp<float> *A32[2], *M32[2];    // src_q -> dst_q; in place dst_q
+  BOp<float> *AT32[2], *MT32[2];  // transpose: src_q -> t; t -> dst_q
   BOp<double> *A64, *M64;    // outer: x64 -> t64 (skip when done); t64 -> w64
   BOp<double> *AT64, *MT64;
   BOp<double> *Mb64, *MTb64; // start: b64 -> w64 (t64 for the transpose)
@@ -1509,8 +1510,9 @@ struct EnsPlan {
 static std::mutex g_ens_mu;
 static std::map<const void *, EnsPlan> g_ens;
 
+constexpr int kEnsTables = 14;  // 8 FP32 inner tables (2 parities x A, M, AT, MT), 6 FP64 outer ones
 static int64_t ens_head_bytes(int M) {
-  return align256(sizeof(KMem) * M) + 10 * align256(sizeof(BOp<double>) * M);
+  return align256(sizeof(KMem) * M) + kEnsTables * align256(sizeof(BOp<double>) * M);
 }
 static int64_t ens_member_bytes(const Stag &s, int R) {
   const Layout L = layout(s, R, true);
@@ -1526,7 +1528,13 @@ static int ens_setup(const SislOperator *ops64, const SislOperator *ops32, int M
   P.maxA = P.maxM = P.maxCols = P.maxFaces = 1;
   P.maxN = 1;
   std::vector<KMem> km(M);
-  std::vector<BOp<float>> a32(M), m32(M), at32(M), mt32(M);
+  std::vector<BOp<float>> a32[2], m32[2], at32[2], mt32[2];
+  for (int q = 0; q < 2; ++q) {
+    a32[q].resize(M);
+    m32[q].resize(M);
+    at32[q].resize(M);
+    mt32[q].resize(M);
+  }
   std::vector<BOp<double>> a64(M), m64(M), at64(M), mt64(M), mb64(M), mtb64(M);
   int64_t off = ens_head_bytes(M);
   int S = 1;
@@ -1561,16 +1569,20 @@ static int ens_setup(const SislOperator *ops64, const SislOperator *ops32, int M
     k.vinv = 1.0 / k.vscale;
     P.sm.push_back(k.sm);
     P.n.push_back(s.n);
-    a32[m] = BOp<float>{o32, k.cur, 0, k.w, k.scr, k.flags, 1};
-    m32[m] = BOp<float>{o32, k.w, 0, k.w, k.scr, k.flags, 1};
-    at32[m] = BOp<float>{o32, k.cur, 0, k.t, k.scr, k.flags, 1};
-    mt32[m] = BOp<float>{o32, k.t, 0, k.w, k.scr, k.flags, 1};
-    a64[m] = BOp<double>{o64, k.x64, 0, k.t64, k.scr64, &k.sm->done, 0};
-    m64[m] = BOp<double>{o64, k.t64, 0, k.w64, k.scr64, &k.sm->done, 0};
-    at64[m] = BOp<double>{o64, k.x64, 0, k.t64, k.scr64, &k.sm->done, 0};
-    mt64[m] = BOp<double>{o64, k.t64, 0, k.w64, k.scr64, &k.sm->done, 0};
-    mb64[m] = BOp<double>{o64, k.b64, 0, k.w64, k.scr64, nullptr, 0};
-    mtb64[m] = BOp<double>{o64, k.b64, 0, k.w64, k.scr64, nullptr, 0};
+    for (int q = 0; q < 2; ++q) {
+      float *src = q == 0 ? k.cur : k.w, *dst = q == 0 ? k.w : k.cur;
+      a32[q][m] = BOp<float>{o32, src, 0, dst, k.scr, k.flags, 1, &k.sm->inv, k.V, k.ldv, k.vscale};
+      m32[q][m] = BOp<float>{o32, dst, 0, dst, k.scr, k.flags, 1, nullptr, nullptr, 0, 1.0};
+      at32[q][m] = BOp<float>{o32, src, 0, k.t, k.scr, k.flags, 1, &k.sm->inv, k.V, k.ldv, k.vscale};
+      mt32[q][m] = BOp<float>{o32, k.t, 0, dst, k.scr, k.flags, 1, nullptr, nullptr, 0, 1.0};
+    }
+    const int *dn = &k.sm->done;
+    a64[m] = BOp<double>{o64, k.x64, 0, k.t64, k.scr64, dn, 0, nullptr, nullptr, 0, 1.0};
+    m64[m] = BOp<double>{o64, k.t64, 0, k.w64, k.scr64, dn, 0, nullptr, nullptr, 0, 1.0};
+    at64[m] = BOp<double>{o64, k.x64, 0, k.t64, k.scr64, dn, 0, nullptr, nullptr, 0, 1.0};
+    mt64[m] = BOp<double>{o64, k.t64, 0, k.w64, k.scr64, dn, 0, nullptr, nullptr, 0, 1.0};
+    mb64[m] = BOp<double>{o64, k.b64, 0, k.w64, k.scr64, nullptr, 0, nullptr, nullptr, 0, 1.0};
+    mtb64[m] = BOp<double>{o64, k.b64, 0, k.w64, k.scr64, nullptr, 0, nullptr, nullptr, 0, 1.0};
     P.maxA = std::max(P.maxA, blocks_A(s));
     P.maxM = std::max(P.maxM, blocks_M(s));
     P.maxCols = std::max(P.maxCols, s.nx * s.ny);
@@ -1584,9 +1596,9 @@ static int ens_setup(const SislOperator *ops64, const SislOperator *ops32, int M
   char *h = ws;
   P.km = (KMem *)h;
   h += align256(sizeof(KMem) * M);
-  BOp<float> **f32[] = {&P.A32, &P.M32, &P.AT32, &P.MT32};
-  std::vector<BOp<float>> *v32[] = {&a32, &m32, &at32, &mt32};
-  for (int q = 0; q < 4; ++q) {
+  BOp<float> **f32[] = {&P.A32[0], &P.A32[1], &P.M32[0], &P.M32[1], &P.AT32[0], &P.AT32[1], &P.MT32[0], &P.MT32[1]};
+  std::vector<BOp<float>> *v32[] = {&a32[0], &a32[1], &m32[0], &m32[1], &at32[0], &at32[1], &mt32[0], &mt32[1]};
+  for (int q = 0; q < 8; ++q) {
     *f32[q] = (BOp<float> *)h;
     h += align256(sizeof(BOp<double>) * M);
   }
@@ -1598,7 +1610,7 @@ static int ens_setup(const SislOperator *ops64, const SislOperator *ops32, int M
   }
   if (hipMemcpyAsync(P.km, km.data(), sizeof(KMem) * M, hipMemcpyHostToDevice, st) != hipSuccess)
     return SISL_ERR_LAUNCH;
-  for (int q = 0; q < 4; ++q)
+  for (int q = 0; q < 8; ++q)
     if (hipMemcpyAsync(*f32[q], v32[q]->data(), sizeof(BOp<float>) * M, hipMemcpyHostToDevice, st) != hipSuccess)
       return SISL_ERR_LAUNCH;
   for (int q = 0; q < 6; ++q)
@@ -1622,13 +1634,14 @@ static void kb_dots_dispatch(const EnsPlan &P, int nv, int j, int gated, hipStre
 static int ens_arnoldi(const EnsPlan &P, int j, int transpose, int single_cgs, hipStream_t st) {
   int rc;
   const int M = P.M, R = P.R, nv = j + 1;
+  const int q = j & 1;
   hipLaunchKernelGGL(kb_snapshot_kernel, dim3(M), dim3(64), 0, st, (const KMem *)P.km, j);
   if (!transpose) {
-    if ((rc = launch_A_batched<float>(P.A32, M, P.maxA, j, st))) return rc;
-    if ((rc = launch_M_batched<float>(P.M32, M, P.maxM, P.L, P.S, j, st))) return rc;
+    if ((rc = launch_A_batched<float>(P.A32[q], M, P.maxA, j, st, 1))) return rc;
+    if ((rc = launch_M_batched<float>(P.M32[q], M, P.maxM, P.L, P.S, j, st))) return rc;
   } else {
-    if ((rc = launch_AT_batched<float>(P.AT32, M, P.maxA, j, st))) return rc;
-    if ((rc = launch_MT_batched<float>(P.MT32, M, P.maxM, P.L, P.S, P.maxFaces, j, st))) return rc;
+    if ((rc = launch_AT_batched<float>(P.AT32[q], M, P.maxA, j, st, 1))) return rc;
+    if ((rc = launch_MT_batched<float>(P.MT32[q], M, P.maxM, P.L, P.S, P.maxFaces, j, st))) return rc;
   }
   kb_dots_dispatch(P, nv, j, 0, st);
   hipLaunchKernelGGL(kb_reduce_kernel, dim3(nv + 1, M), dim3(kBlock), 0, st, (const KMem *)P.km, j, kSelH1, 0, 0);
@@ -1642,8 +1655,95 @@ static int ens_arnoldi(const EnsPlan &P, int j, int transpose, int single_cgs, h
     hipLaunchKernelGGL(kb_reduce_kernel, dim3(1, M), dim3(kBlock), 0, st, (const KMem *)P.km, j, kSelNn1b, 1, 0);
   }
   hipLaunchKernelGGL(kb_finalize_kernel, dim3(M), dim3(64), 0, st, (const KMem *)P.km, j, R);
-  hipLaunchKernelGGL(kb_normalize_kernel, dim3(kRedGrid, M), dim3(kBlock), 0, st, (const KMem *)P.km, j);
   SISL_CHECK_LAUNCH();
+  return SISL_OK;
+}
+
+// One inner cycle of the batched solver: R Arnoldi steps, the members' least
+// squares and the update of x.  Its launches have fixed arguments (the plan's
+// tables in the workspace, the step indices, the launch geometry of the
+// member grids) and are gated on the device by each member's stop and done
+// flags, so the cycle is captured once per workspace layout, direction and
+// Gram-Schmidt form (on the private capture stream, as the single-member
+// cycle) and replayed on the caller's stream for every later cycle.
+static std::map<std::string, CachedGraph> g_ens_graphs;  // guarded by g_graph_mu
+constexpr size_t kMaxEnsGraphs = 16;
+
+static int ens_cycle_eager(const EnsPlan &P, int transpose, int single_cgs, hipStream_t st) {
+  int rc;
+  for (int j = 0; j < P.R; ++j)
+    if ((rc = ens_arnoldi(P, j, transpose, single_cgs, st))) return rc;
+  hipLaunchKernelGGL(kb_lstsq_kernel, dim3(P.M), dim3(kLsqThreads), 0, st, (const KMem *)P.km, P.R);
+  hipLaunchKernelGGL(kb_combine_kernel, dim3(kRedGrid, P.M), dim3(kBlock), 0, st, (const KMem *)P.km);
+  SISL_CHECK_LAUNCH();
+  return SISL_OK;
+}
+
+static int ens_cycle(const EnsPlan &P, int transpose, int single_cgs, hipStream_t st) {
+  bool use_graph;
+  {
+    std::lock_guard<std::mutex> g(g_graph_mu);
+    use_graph = g_graphs_on;
+  }
+  if (!use_graph || timer_on()) return ens_cycle_eager(P, transpose, single_cgs, st);
+  std::string key;
+  int dev = 0;
+  hipGetDevice(&dev);
+  key_add(key, dev);
+  key_add(key, (const void *)P.km);
+  const int geo[11] = {P.M, P.R, P.L, P.S, P.maxA, P.maxM, P.maxCols, P.maxFaces, transpose, single_cgs, 0};
+  key_add(key, geo);
+  key_add(key, P.maxN);
+  bool cached;
+  {
+    std::lock_guard<std::mutex> g(g_graph_mu);
+    if (g_ens_graphs.size() >= kMaxEnsGraphs && !g_ens_graphs.count(key)) {
+      for (auto &e : g_ens_graphs) {  // bound the cache: retire all once their replays finished
+        hipEventSynchronize(e.second.done);
+        hipGraphExecDestroy(e.second.exec);
+        hipEventDestroy(e.second.done);
+      }
+      g_ens_graphs.clear();
+    }
+    cached = g_ens_graphs.count(key) != 0;
+  }
+  if (!cached) {
+    hipStream_t cs;
+    {
+      std::lock_guard<std::mutex> g(g_graph_mu);
+      cs = capture_stream();
+    }
+    if (!cs) return SISL_ERR_LAUNCH;
+    if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) return SISL_ERR_LAUNCH;
+    const int rc = ens_cycle_eager(P, transpose, single_cgs, cs);
+    hipGraph_t graph = nullptr;
+    const hipError_t ec = hipStreamEndCapture(cs, &graph);
+    if (rc || ec != hipSuccess || !graph) {
+      if (graph) hipGraphDestroy(graph);
+      return rc ? rc : SISL_ERR_LAUNCH;
+    }
+    hipGraphExec_t exec = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    if (ei != hipSuccess) return SISL_ERR_LAUNCH;
+    hipEvent_t done = nullptr;
+    if (hipEventCreateWithFlags(&done, hipEventDisableTiming) != hipSuccess) {
+      hipGraphExecDestroy(exec);
+      return SISL_ERR_LAUNCH;
+    }
+    std::lock_guard<std::mutex> g(g_graph_mu);
+    if (g_ens_graphs.count(key)) {
+      hipGraphExecDestroy(exec);
+      hipEventDestroy(done);
+    } else {
+      g_ens_graphs[key] = CachedGraph{exec, done};
+    }
+  }
+  std::lock_guard<std::mutex> g(g_graph_mu);
+  auto it = g_ens_graphs.find(key);
+  if (it == g_ens_graphs.end()) return ens_cycle_eager(P, transpose, single_cgs, st);
+  if (hipGraphLaunch(it->second.exec, st) != hipSuccess) return SISL_ERR_LAUNCH;
+  if (hipEventRecord(it->second.done, st) != hipSuccess) return SISL_ERR_LAUNCH;
   return SISL_OK;
 }
 
@@ -1703,9 +1803,8 @@ static int ens_solve(const void *ws, const SislGmresParams &p, const double *con
     } else {
       if ((rc = launch_MT_batched<double>(P.MT64, M, P.maxM, P.L, P.S, P.maxFaces, 0, st))) return rc;
     }
-    hipLaunchKernelGGL(kb_norm2_kernel, dim3(kRedGrid, M), dim3(kBlock), 0, st, (const KMem *)P.km, 1);
+    hipLaunchKernelGGL(kb_norm2_f32_kernel, dim3(kRedGrid, M), dim3(kBlock), 0, st, (const KMem *)P.km);
     hipLaunchKernelGGL(kb_reduce_kernel, dim3(1, M), dim3(kBlock), 0, st, (const KMem *)P.km, 0, kSelScal0 + 2, 0, 1);
-    hipLaunchKernelGGL(kb_unit_kernel, dim3(kRedGrid, M), dim3(kBlock), 0, st, (const KMem *)P.km);
     hipLaunchKernelGGL(kb_reset_kernel, dim3(M), dim3(kBlock), 0, st, (const KMem *)P.km, R, p.inner_rtol,
                        max_outer);
     // one read of the members' done flags per outer iteration
@@ -1716,10 +1815,7 @@ static int ens_solve(const void *ws, const SislGmresParams &p, const double *con
     if (hipStreamSynchronize(st) != hipSuccess) return SISL_ERR_LAUNCH;
     for (int m = 0; m < M; ++m) all = all && done[m];
     if (all || outer > max_outer) break;
-    for (int j = 0; j < R; ++j)
-      if ((rc = ens_arnoldi(P, j, p.transpose, p.inner_single_cgs, st))) return rc;
-    hipLaunchKernelGGL(kb_lstsq_kernel, dim3(M), dim3(kLsqThreads), 0, st, (const KMem *)P.km, R);
-    hipLaunchKernelGGL(kb_combine_kernel, dim3(kRedGrid, M), dim3(kBlock), 0, st, (const KMem *)P.km);
+    if ((rc = ens_cycle(P, p.transpose, p.inner_single_cgs, st))) return rc;
   }
   for (int m = 0; m < M; ++m) {
     tout.src[m] = km[m].x64;
