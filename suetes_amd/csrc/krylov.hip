@@ -1465,6 +1465,25 @@ __global__ void kb_reset_kernel(const KMem *__restrict__ km, int R, double inner
 }
 // Least squares of the cycle (lstsq_block) on the device, one workgroup per
 // member; y into sm->y.
+// One member's solve outcome (a copy of its Small record's outer-loop fields).
+struct EnsOutcome {
+  int done, outer, steps_total, pad;
+  double b_norm, Mb_norm, ptol, rnorm;
+};
+// *all_done = 1 when every member's refinement has finished (one read per
+// outer iteration instead of one per member).
+__global__ void kb_all_done_kernel(const KMem *__restrict__ km, int M, int *__restrict__ all_done) {
+  if (threadIdx.x != 0) return;
+  int all = 1;
+  for (int m = 0; m < M; ++m) all = all && km[m].sm->done;
+  *all_done = all;
+}
+__global__ void kb_outcome_kernel(const KMem *__restrict__ km, int M, EnsOutcome *__restrict__ out) {
+  for (int m = threadIdx.x; m < M; m += blockDim.x) {
+    const Small *sm = km[m].sm;
+    out[m] = EnsOutcome{sm->done, sm->outer, sm->steps_total, 0, sm->b_norm, sm->Mb_norm, sm->ptol, sm->rnorm};
+  }
+}
 __global__ void __launch_bounds__(kLsqThreads) kb_lstsq_kernel(const KMem *__restrict__ km, int R) {
   Small *sm = km[blockIdx.x].sm;
   if (sm->done) return;  // uniform over the block
@@ -1505,14 +1524,18 @@ struct EnsPlan {
   BOp<double> *A64, *M64;    // outer: x64 -> t64 (skip when done); t64 -> w64
   BOp<double> *AT64, *MT64;
   BOp<double> *Mb64, *MTb64; // start: b64 -> w64 (t64 for the transpose)
+  int *all_done;             // 1 when every member's refinement has finished
+  EnsOutcome *outcome;       // the members' outcomes, gathered after a solve
   std::vector<Small *> sm;
+  std::vector<KMem> kmh;     // host copy of the device table
 };
 static std::mutex g_ens_mu;
 static std::map<const void *, EnsPlan> g_ens;
 
 constexpr int kEnsTables = 14;  // 8 FP32 inner tables (2 parities x A, M, AT, MT), 6 FP64 outer ones
 static int64_t ens_head_bytes(int M) {
-  return align256(sizeof(KMem) * M) + kEnsTables * align256(sizeof(BOp<double>) * M);
+  return align256(sizeof(KMem) * M) + kEnsTables * align256(sizeof(BOp<double>) * M) + align256(sizeof(int)) +
+         align256(sizeof(EnsOutcome) * M);
 }
 static int64_t ens_member_bytes(const Stag &s, int R) {
   const Layout L = layout(s, R, true);
@@ -1608,6 +1631,11 @@ static int ens_setup(const SislOperator *ops64, const SislOperator *ops32, int M
     *f64[q] = (BOp<double> *)h;
     h += align256(sizeof(BOp<double>) * M);
   }
+  P.kmh = km;
+  P.all_done = (int *)h;
+  h += align256(sizeof(int));
+  P.outcome = (EnsOutcome *)h;
+  h += align256(sizeof(EnsOutcome) * M);
   if (hipMemcpyAsync(P.km, km.data(), sizeof(KMem) * M, hipMemcpyHostToDevice, st) != hipSuccess)
     return SISL_ERR_LAUNCH;
   for (int q = 0; q < 8; ++q)
@@ -1760,10 +1788,7 @@ static int ens_solve(const void *ws, const SislGmresParams &p, const double *con
   const int M = P.M, R = P.R;
   int rc;
   PtrTable tin{}, tout{};
-  std::vector<KMem> km(M);
-  if (hipMemcpyAsync(km.data(), P.km, sizeof(KMem) * M, hipMemcpyDeviceToHost, st) != hipSuccess ||
-      hipStreamSynchronize(st) != hipSuccess)
-    return SISL_ERR_LAUNCH;
+  const std::vector<KMem> &km = P.kmh;
   for (int m = 0; m < M; ++m) {
     if (!b[m] || !x[m]) return SISL_ERR_ARG;
     tin.src[m] = b[m];
@@ -1789,7 +1814,6 @@ static int ens_solve(const void *ws, const SislGmresParams &p, const double *con
   hipLaunchKernelGGL(kb_reduce_kernel, dim3(1, M), dim3(kBlock), 0, st, (const KMem *)P.km, 0, kSelScal0 + 1, 0, 1);
   hipLaunchKernelGGL(kb_init_kernel, dim3(M), dim3(64), 0, st, (const KMem *)P.km, p.tol, p.atol);
   const int max_outer = p.max_outer > 0 ? p.max_outer : p.maxiter;
-  std::vector<int> done(M);
   for (int outer = 0;; ++outer) {
     // residual M (b - A x) in FP64, its norm, the first basis vector
     if (!p.transpose) {
@@ -1807,13 +1831,12 @@ static int ens_solve(const void *ws, const SislGmresParams &p, const double *con
     hipLaunchKernelGGL(kb_reduce_kernel, dim3(1, M), dim3(kBlock), 0, st, (const KMem *)P.km, 0, kSelScal0 + 2, 0, 1);
     hipLaunchKernelGGL(kb_reset_kernel, dim3(M), dim3(kBlock), 0, st, (const KMem *)P.km, R, p.inner_rtol,
                        max_outer);
-    // one read of the members' done flags per outer iteration
-    bool all = true;
-    for (int m = 0; m < M; ++m)
-      if (hipMemcpyAsync(&done[m], &P.sm[m]->done, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess)
-        return SISL_ERR_LAUNCH;
-    if (hipStreamSynchronize(st) != hipSuccess) return SISL_ERR_LAUNCH;
-    for (int m = 0; m < M; ++m) all = all && done[m];
+    // one read per outer iteration: whether every member has finished
+    hipLaunchKernelGGL(kb_all_done_kernel, dim3(1), dim3(64), 0, st, (const KMem *)P.km, M, P.all_done);
+    int all = 0;
+    if (hipMemcpyAsync(&all, P.all_done, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+      return SISL_ERR_LAUNCH;
     if (all || outer > max_outer) break;
     if ((rc = ens_cycle(P, p.transpose, p.inner_single_cgs, st))) return rc;
   }
@@ -1823,14 +1846,15 @@ static int ens_solve(const void *ws, const SislGmresParams &p, const double *con
   }
   hipLaunchKernelGGL(kb_copy_kernel, cg, dim3(kBlock), 0, st, tout);
   SISL_CHECK_LAUNCH();
-  // per-member outcome (one small copy per member after the solve)
+  // the members' outcomes, gathered on the device and read in one copy
+  hipLaunchKernelGGL(kb_outcome_kernel, dim3(1), dim3(64), 0, st, (const KMem *)P.km, M, P.outcome);
+  std::vector<EnsOutcome> oc(M);
+  if (hipMemcpyAsync(oc.data(), P.outcome, sizeof(EnsOutcome) * M, hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
+    return SISL_ERR_LAUNCH;
   bool finite = true;
   for (int m = 0; m < M; ++m) {
-    Small hs;
-    if (hipMemcpyAsync(&hs.done, &P.sm[m]->done, sizeof(int) * 4 + sizeof(double) * 4,
-                       hipMemcpyDeviceToHost, st) != hipSuccess ||
-        hipStreamSynchronize(st) != hipSuccess)
-      return SISL_ERR_LAUNCH;
+    const EnsOutcome &hs = oc[m];
     finite = finite && std::isfinite(hs.rnorm);
     if (stats) {
       stats[m].cycles = hs.outer;
