@@ -149,6 +149,59 @@ def _roofline(table):
     return roof
 
 
+def attainable(case, reps=20):
+    """Attainable HBM rates on this device, timed with HIP events on the
+    launch stream: a 1 GiB copy and a 1 GiB read with 16-byte accesses, and
+    the inner operator launch's access mix alone (its 18 input and 10 output
+    arrays, each touched once at the thread's own column, in its tiles and
+    level chunks, no stencil or arithmetic: csrc/probe.hip) against the
+    algorithmic bytes of that launch."""
+    import ctypes as C
+    import torch
+    from suetes_amd import _lib
+    L = _lib.lib()
+    vp = C.c_void_p
+    L.sisl_probe_stream.argtypes = [vp, vp, C.c_int64, C.c_int32, vp]
+    L.sisl_probe_operator_streams_f32.argtypes = [vp, vp, vp, vp, C.c_int32, vp]
+    s = _lib.stream_ptr()
+
+    def timed(fn):
+        for _ in range(3):
+            _lib.check(fn(), "probe")
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) * 1e-3 / reps
+    nb = 1 << 30
+    a = torch.ones(nb // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    t_copy = timed(lambda: L.sisl_probe_stream(a.data_ptr(), b.data_ptr(), nb, 0, s))
+    t_read = timed(lambda: L.sisl_probe_stream(a.data_ptr(), b.data_ptr(), nb, 1, s))
+    del a, b
+    sol = case.stepper.implicit_solver
+    ops = sol.op_inner
+    g = ops.struct.g
+    n = int(L.sisl_krylov_size(C.byref(g)))
+    x = torch.randn(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    bas = torch.empty(n, dtype=torch.float16, device="cuda")
+    nu, nv = (g.nx + 1) * g.ny * g.nz, g.nx * (g.ny + 1) * g.nz
+    nw, nm = g.nx * g.ny * (g.nz + 1), g.nx * g.ny * g.nz
+    alg = 4 * 2 * n + 2 * (3 * nu + 3 * nv + 6 * nw + nm) + 2 * n
+    best = None
+    for kc in (3, 6, 11):
+        t = timed(lambda: L.sisl_probe_operator_streams_f32(C.byref(ops.struct), x.data_ptr(), y.data_ptr(),
+                                                            bas.data_ptr(), kc, s))
+        best = t if best is None else min(best, t)
+    return {"copy_GBs": 2 * nb / t_copy / 1e9, "read_GBs": nb / t_read / 1e9,
+            "operator_access_mix_GBs": alg / best / 1e9, "operator_access_mix_us": best * 1e6,
+            "method": "HIP events around 20 launches of each probe (csrc/probe.hip); the access-mix "
+                      "rate is the operator launch's algorithmic bytes over the probe's time"}
+
+
 def copy_reference(nbytes=1 << 29, reps=5):
     """A plain device copy of known size (reads and writes `nbytes` each):
     the scale for the FETCH_SIZE / WRITE_SIZE hardware counters."""
@@ -569,7 +622,11 @@ def main():
                                   for k in ("u", "v", "w", "pi", "th_v", "rho")}}
         log(f"mixed vs fp64 after {args.err_steps} steps: {err_long['max_rel_l2']:.2e}")
     roof, table = instrumented_step(cmx, s_mx)
-    log(f"dominant kernel {roof['kernel']}: {roof['achieved']:.0f} GB/s")
+    roof["attainable"] = attainable(cmx)
+    if roof["kernel"].startswith("helmholtz_kernel") and roof["kernel"].endswith("[fp32]"):
+        roof["frac_of_access_mix"] = roof["achieved"] / roof["attainable"]["operator_access_mix_GBs"]
+    log(f"dominant kernel {roof['kernel']}: {roof['achieved']:.0f} GB/s; attainable: copy "
+        f"{roof['attainable']['copy_GBs']:.0f}, its access mix {roof['attainable']['operator_access_mix_GBs']:.0f}")
     adj = None
     if not args.no_adjoint:
         try:

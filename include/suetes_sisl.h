@@ -104,6 +104,22 @@ int sisl_tricubic_f64(int32_t fnx, int32_t fny, int32_t fnz, const double *field
                       const double *coords, double *out, int32_t use_limiter, void *stream);
 int sisl_tricubic_f32(int32_t fnx, int32_t fny, int32_t fnz, const float *field,
                       const float *coords, float *out, int32_t use_limiter, void *stream);
+/* Several fields on one set of departure points (the step's pairs that
+ * share one: pi' and theta' at mass points, the kinematic residual and w at
+ * w levels; steppers.py:853-875): offsets, their split and the weights are
+ * formed once per point.  Same bits as one tensor_product_interp_3d per
+ * field.  _mixed: FP64 fields and outputs, FP32 offsets and arithmetic (each
+ * field value rounded to FP32 as it is read, each result widened on the
+ * store: the same bits as converting the field, gathering in FP32 and
+ * widening the result). */
+int sisl_tricubic2_f64(int32_t fnx, int32_t fny, int32_t fnz, const double *f0, const double *f1,
+                       const double *coords, double *out0, double *out1, int32_t use_limiter, void *stream);
+int sisl_tricubic2_f32(int32_t fnx, int32_t fny, int32_t fnz, const float *f0, const float *f1,
+                       const float *coords, float *out0, float *out1, int32_t use_limiter, void *stream);
+int sisl_tricubic_mixed(int32_t fnx, int32_t fny, int32_t fnz, const double *field, const float *coords,
+                        double *out, int32_t use_limiter, void *stream);
+int sisl_tricubic2_mixed(int32_t fnx, int32_t fny, int32_t fnz, const double *f0, const double *f1,
+                         const float *coords, double *out0, double *out1, int32_t use_limiter, void *stream);
 
 /* Workspace of the deterministic tricubic adjoint, int32 device arrays:
  * key[n], perm[n], start[n+1], count[n], block_sums[sisl_scan_blocks(n)]. */
@@ -203,6 +219,16 @@ int sisl_helmholtz_apply_T_f64(const SislOperator *op, const double *x, double *
                                double *scratch, void *stream);
 int sisl_helmholtz_apply_T_f32(const SislOperator *op, const float *x, float *y,
                                float *scratch, void *stream);
+/* The inner FP32 solver's operator launch with the fused basis
+ * normalisation (no reference counterpart of its own: it is
+ * Euler3D.linear_operator, euler.py:374-439, applied to the Arnoldi vector
+ * of the mixed GMRES): y = A (x s) or A^T (x s) with s = *inv, and
+ * basis = FP16(vs * s * x) stored from the elements each thread owns.
+ * coef_bf16 = 1: the op's 3-D coefficient arrays are bfloat16.  Honours the
+ * pinned / tuned launch shapes (all give the same bits). */
+int sisl_helmholtz_apply_fused_f32(const SislOperator *op, const float *x, float *y, const double *inv,
+                                   void *basis, double vs, int32_t coef_bf16, int32_t transpose,
+                                   void *stream);
 
 /* ---- (c) column preconditioner ----------------------------------------- */
 
@@ -470,6 +496,18 @@ int sisl_timer_enable(int32_t on);
  * work (not retired by a device stop flag), algorithmic bytes of those. */
 int sisl_timer_read(int32_t kernel, int32_t fp32, double *total_ms, int64_t *launches,
                     int64_t *working_launches, double *bytes);
+
+/* Bandwidth probes for the roofline (measurement only, no reference
+ * counterpart).  sisl_probe_stream: mode 0 copies `bytes` (a multiple of 16)
+ * from in to out with 16-byte accesses, mode 1 only reads them (out: 4
+ * bytes).  sisl_probe_operator_streams_f32: the memory traffic of the inner
+ * solver's operator launch (the FP32 operator with bfloat16 3-D
+ * coefficients and the fused FP16 basis column) without its stencil or
+ * arithmetic: each input array read once at the thread's own column, the
+ * same outputs, tiles (64 x 4), level chunk and tile order. */
+int sisl_probe_stream(const void *in, void *out, int64_t bytes, int32_t mode, void *stream);
+int sisl_probe_operator_streams_f32(const SislOperator *op, const float *x, float *y, void *basis,
+                                    int32_t chunk, void *stream);
 
 /* ---- batched ensemble (one launch per kernel over members) --------------- */
 

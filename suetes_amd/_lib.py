@@ -35,6 +35,8 @@ EXPORTED = [
     "sisl_acoustic_workspace_f64", "sisl_acoustic_loop_f64", "sisl_slow_tendencies_f64",
     "sisl_tricubic_limited_adj_split_f64", "sisl_tricubic_limited_adj_split_f32",
     "sisl_tricubic_limited_adj_corners_f64", "sisl_tricubic_limited_adj_corners_f32",
+    "sisl_helmholtz_apply_fused_f32", "sisl_probe_stream", "sisl_probe_operator_streams_f32",
+    "sisl_tricubic2_f64", "sisl_tricubic2_f32", "sisl_tricubic_mixed", "sisl_tricubic2_mixed",
 ]
 
 # timed kernels (enum SislTimedKernel)
@@ -130,6 +132,12 @@ def lib():
             getattr(L, f"sisl_ffsl_1d_{p}").argtypes = [vp, vp, vp, vp, i64, i32, i64, i32, vp]
             getattr(L, f"sisl_ffsl_1d_T_{p}").argtypes = [vp, vp, vp, vp, vp, vp, i64, i32, i64, i32, vp]
         L.sisl_ffsl_1d_f64_noalloc.argtypes = [vp, vp, vp, i64, i32, i64, i32, vp]
+        for name in ("sisl_tricubic2_f64", "sisl_tricubic2_f32", "sisl_tricubic2_mixed"):
+            getattr(L, name).argtypes = [i32, i32, i32, vp, vp, vp, vp, vp, i32, vp]
+        L.sisl_tricubic_mixed.argtypes = [i32, i32, i32, vp, vp, vp, i32, vp]
+        L.sisl_helmholtz_apply_fused_f32.argtypes = [C.POINTER(SislOperator), vp, vp, vp, vp, d, i32, i32, vp]
+        L.sisl_probe_stream.argtypes = [vp, vp, i64, i32, vp]
+        L.sisl_probe_operator_streams_f32.argtypes = [C.POINTER(SislOperator), vp, vp, vp, i32, vp]
         L.sisl_set_graphs.argtypes = [i32]
         L.sisl_timer_enable.argtypes = [i32]
         L.sisl_timer_read.argtypes = [i32, i32, C.POINTER(d), C.POINTER(i64), C.POINTER(i64),

@@ -1659,6 +1659,20 @@ int sisl_autotune_operator(const SislOperator *op, int32_t fp32, void *x, void *
   if (!o.complete()) return SISL_ERR_ARG;
   return sisl::autotune_A(o, (const double *)x, (double *)y, scratch, sisl::as_stream(s), chunk, unroll);
 }
+int sisl_helmholtz_apply_fused_f32(const SislOperator *op, const float *x, float *y, const double *inv,
+                                   void *basis, double vs, int32_t coef_bf16, int32_t transpose, void *s) {
+  if (!op || !x || !y || !inv || !basis || x == y) return SISL_ERR_ARG;
+  auto o = sisl::OpT<float>::make(*op);
+  if (!o.complete()) return SISL_ERR_ARG;
+  sisl::Fuse f;
+  f.inv = inv;
+  f.vout = basis;
+  f.nb = 1;
+  f.vs = vs;
+  f.cb = coef_bf16 ? 1 : 0;
+  if (transpose) return sisl::launch_AT(o, x, y, (float *)nullptr, sisl::as_stream(s), nullptr, &f);
+  return sisl::launch_A(o, x, y, sisl::as_stream(s), nullptr, &f);
+}
 int sisl_helmholtz_apply_T_f64(const SislOperator *op, const double *x, double *y, double *scr,
                                void *s) {
   if (!op || !x || !y || x == y) return SISL_ERR_ARG;

@@ -52,11 +52,7 @@ __device__ __forceinline__ int offset_split(int X, T d, int n, T &t) {
 // reference's index clamping (or reads a staged copy that holds clamped
 // values).  Cascade z -> y -> x as the reference does.
 template <typename T, typename Fetch>
-__device__ __forceinline__ T tricubic_point(T tx, T ty, T tz, int lim, Fetch F) {
-  T wx[4], wy[4], wz[4];
-  cr_weights(tx, wx);
-  cr_weights(ty, wy);
-  cr_weights(tz, wz);
+__device__ __forceinline__ T tricubic_apply(const T wx[4], const T wy[4], const T wz[4], int lim, Fetch F) {
   T acc_x = 0;
 #pragma unroll
   for (int a = 0; a < 4; ++a) {
@@ -84,6 +80,62 @@ __device__ __forceinline__ T tricubic_point(T tx, T ty, T tz, int lim, Fetch F) 
   return acc_x;
 }
 
+// tricubic_apply reading the taps from global memory at clamped indices
+// (ix: columns, iy: row offsets y * nx, iz: levels).  The z sums are formed
+// one stencil plane at a time (the loop is kept rolled, so only one plane's
+// 16 taps are in flight), each in the order of tricubic_apply: the same bits.
+template <typename T, typename TF>
+__device__ __forceinline__ T tricubic_global(const TF *__restrict__ f, const int ix[4], const int iy[4],
+                                             const int iz[4], uint32_t plane, const T wx[4], const T wy[4],
+                                             const T wz[4], int lim) {
+  T az[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) az[a][b] = 0;
+#pragma unroll 1
+  for (int c = 0; c < 4; ++c) {
+    const uint32_t zo = (uint32_t)iz[c] * plane;
+    const T w = wz[c];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) az[a][b] += w * (T)ldg(f, zo + (uint32_t)iy[b] + (uint32_t)ix[a]);
+  }
+  T acc_x = 0;
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    T acc_y = 0;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc_y += wy[b] * az[a][b];
+    acc_x += wx[a] * acc_y;
+  }
+  if (lim) {
+    auto F = [&](int a, int b, int c) {
+      return (T)ldg(f, (uint32_t)iz[c] * plane + (uint32_t)iy[b] + (uint32_t)ix[a]);
+    };
+    T lo = F(1, 1, 1), hi = lo;
+#pragma unroll
+    for (int q = 1; q < 8; ++q) {
+      const T nb = F(1 + (q & 1), 1 + ((q >> 1) & 1), 1 + ((q >> 2) & 1));
+      lo = nb < lo ? nb : lo;
+      hi = nb > hi ? nb : hi;
+    }
+    acc_x = acc_x < lo ? lo : acc_x;
+    acc_x = acc_x > hi ? hi : acc_x;
+  }
+  return acc_x;
+}
+
+template <typename T, typename Fetch>
+__device__ __forceinline__ T tricubic_point(T tx, T ty, T tz, int lim, Fetch F) {
+  T wx[4], wy[4], wz[4];
+  cr_weights(tx, wx);
+  cr_weights(ty, wy);
+  cr_weights(tz, wz);
+  return tricubic_apply(wx, wy, wz, lim, F);
+}
+
 // Staged tricubic gather.  A block owns a 64 x 4 tile of (x, y) columns and
 // marches a chunk of levels.  A ring of 6 LDS planes holds levels k-2 .. k+3
 // of the field over the tile plus halo ([x0-2, x0+70) x [y0-2, y0+7)), each
@@ -92,13 +144,27 @@ __device__ __forceinline__ T tricubic_point(T tx, T ty, T tz, int lim, Fetch F) 
 // One new plane is staged per level (coalesced along x).  Points whose
 // stencil leaves the box (displacement beyond ~1 cell) gather from global
 // memory; both paths perform the same arithmetic in the same order.
+//
+// NF fields that share one set of departure points are gathered together:
+// the offsets, their split and the weights are formed once per point, each
+// field has its own ring.  TF is the storage type of the fields and outputs,
+// T the arithmetic (and offset) type: FP64 fields gathered in FP32 are
+// rounded to FP32 as they are staged (as a separate conversion would) and
+// the FP32 results widened on the store, so every variant gives the same
+// bits as one field at a time in T.
 constexpr int kIX = 64, kIY = 4, kSX = kIX + 8, kSY = kIY + 5, kRing = 6;
 
-template <typename T>
-__device__ __forceinline__ void tricubic_body(const Dims &d, const T *__restrict__ f,
-                                              const T *__restrict__ co, T *__restrict__ out, int lim,
-                                              int kchunk, int blk, int nblk) {
-  __shared__ T ring[kRing][kSY][kSX];
+template <typename TF, int NF>
+struct FieldSet {
+  const TF *f[NF];
+  TF *o[NF];
+};
+
+template <typename T, typename TF, int NF>
+__device__ __forceinline__ void tricubic_multi_body(const Dims &d, const FieldSet<TF, NF> &fs,
+                                                    const T *__restrict__ co, int lim, int kchunk, int blk,
+                                                    int nblk) {
+  __shared__ T ring[NF][kRing][kSY][kSX];
   const int nx = d.nx, ny = d.ny, nz = d.nz;
   const int nbx = (nx + kIX - 1) / kIX, nby = (ny + kIY - 1) / kIY;
   const int tile = xcd_tile(blk, nblk);
@@ -116,7 +182,8 @@ __device__ __forceinline__ void tricubic_body(const Dims &d, const T *__restrict
     for (int q = tid; q < kSY * kSX; q += kIX * kIY) {
       const int ly = q / kSX, lx = q - ly * kSX;
       const int gx = min(max(xs + lx, 0), nx - 1), gy = min(max(ys + ly, 0), ny - 1);
-      ring[slot][ly][lx] = ldg(f, zoff + (uint32_t)gy * nx + gx);
+#pragma unroll
+      for (int fi = 0; fi < NF; ++fi) ring[fi][slot][ly][lx] = (T)ldg(fs.f[fi], zoff + (uint32_t)gy * nx + gx);
     }
   };
   for (int kk = k0 - 2; kk <= k0 + 2; ++kk) stage(kk);
@@ -132,18 +199,31 @@ __device__ __forceinline__ void tricubic_body(const Dims &d, const T *__restrict
     const int z0 = offset_split(k, ldg(co, 2 * n + p), nz, tz);
     const bool staged = x0 - 1 >= xs && x0 + 2 < xs + kSX && y0 - 1 >= ys && y0 + 2 < ys + kSY &&
                         z0 - 1 >= k - 2 && z0 + 2 <= k + 3;
-    T r;
+    T wx[4], wy[4], wz[4];
+    cr_weights(tx, wx);
+    cr_weights(ty, wy);
+    cr_weights(tz, wz);
+    T r[NF];
     if (staged) {
       // one LDS base per stencil plane; taps are constant offsets from it
-      const T *zb[4];
+      int sl[4];
       int slot = (z0 - 1) % kRing;
       if (slot < 0) slot += kRing;
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
-        zb[c] = &ring[slot][y0 - 1 - ys][x0 - 1 - xs];
+        sl[c] = slot;
         slot = slot + 1 == kRing ? 0 : slot + 1;
       }
-      r = tricubic_point(tx, ty, tz, lim, [&](int a, int b, int c) { return zb[c][b * kSX + a]; });
+#pragma unroll
+      for (int fi = 0; fi < NF; ++fi) {
+        // one field at a time: without the scheduling barrier the compiler
+        // hoists every field's taps into registers at once (occupancy 2)
+        if (fi) __builtin_amdgcn_sched_barrier(0);
+        const T *zb[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) zb[c] = &ring[fi][sl[c]][y0 - 1 - ys][x0 - 1 - xs];
+        r[fi] = tricubic_apply(wx, wy, wz, lim, [&](int a, int b, int c) { return zb[c][b * kSX + a]; });
+      }
     } else {
       int ix[4], iy[4], iz[4];
 #pragma unroll
@@ -152,12 +232,28 @@ __device__ __forceinline__ void tricubic_body(const Dims &d, const T *__restrict
         iy[q] = min(max(y0 - 1 + q, 0), ny - 1) * nx;
         iz[q] = min(max(z0 - 1 + q, 0), nz - 1);
       }
-      r = tricubic_point(tx, ty, tz, lim, [&](int a, int b, int c) {
-        return ldg(f, (uint32_t)iz[c] * plane + (uint32_t)iy[b] + (uint32_t)ix[a]);
-      });
+#pragma unroll
+      for (int fi = 0; fi < NF; ++fi) r[fi] = tricubic_global(fs.f[fi], ix, iy, iz, plane, wx, wy, wz, lim);
     }
-    stg(out, p, r);
+#pragma unroll
+    for (int fi = 0; fi < NF; ++fi) stg(fs.o[fi], p, (TF)r[fi]);
   }
+}
+
+template <typename T>
+__device__ __forceinline__ void tricubic_body(const Dims &d, const T *__restrict__ f,
+                                              const T *__restrict__ co, T *__restrict__ out, int lim,
+                                              int kchunk, int blk, int nblk) {
+  FieldSet<T, 1> fs;
+  fs.f[0] = f;
+  fs.o[0] = out;
+  tricubic_multi_body<T, T, 1>(d, fs, co, lim, kchunk, blk, nblk);
+}
+
+template <typename T, typename TF, int NF>
+__global__ void __launch_bounds__(256) tricubic_multi_kernel(Dims d, FieldSet<TF, NF> fs,
+                                                             const T *__restrict__ co, int lim, int kchunk) {
+  tricubic_multi_body<T, TF, NF>(d, fs, co, lim, kchunk, blockIdx.x, gridDim.x);
 }
 
 template <typename T>
@@ -546,6 +642,31 @@ int tricubic(int fnx, int fny, int fnz, const T *f, const T *co, T *out, int lim
   const int tid = timer_begin(SISL_T_TRICUBIC, sizeof(T) == 4, (double)sizeof(T) * 5.0 * d.n(),
                               nullptr, nullptr, as_stream(st));
   hipLaunchKernelGGL(tricubic_kernel<T>, dim3(nb), dim3(kIX, kIY), 0, as_stream(st), d, f, co, out,
+                     lim, kc);
+  timer_end(tid, as_stream(st));
+  SISL_CHECK_LAUNCH();
+  return SISL_OK;
+}
+
+// NF fields on one set of departure points; TF storage, T arithmetic.
+template <typename T, typename TF, int NF>
+int tricubic_multi(int fnx, int fny, int fnz, const TF *const *f, const T *co, TF *const *out, int lim,
+                   void *st) {
+  if (!f || !co || !out || fnx < 1 || fny < 1 || fnz < 1) return SISL_ERR_ARG;
+  FieldSet<TF, NF> fs;
+  for (int q = 0; q < NF; ++q) {
+    if (!f[q] || !out[q]) return SISL_ERR_ARG;
+    fs.f[q] = f[q];
+    fs.o[q] = out[q];
+  }
+  Dims d{fnx, fny, fnz};
+  if ((uint64_t)d.n() * 3 * sizeof(T) >= (1ull << 32) || (uint64_t)d.n() * sizeof(TF) >= (1ull << 32))
+    return SISL_ERR_ARG;  // 32-bit offsets
+  const int kc = tricubic_chunk(fnz);
+  const int nb = tricubic_blocks(fnx, fny, fnz);
+  const double bytes = ((double)NF * 2.0 * sizeof(TF) + 3.0 * sizeof(T)) * d.n();
+  const int tid = timer_begin(SISL_T_TRICUBIC, sizeof(T) == 4, bytes, nullptr, nullptr, as_stream(st));
+  hipLaunchKernelGGL((tricubic_multi_kernel<T, TF, NF>), dim3(nb), dim3(kIX, kIY), 0, as_stream(st), d, fs, co,
                      lim, kc);
   timer_end(tid, as_stream(st));
   SISL_CHECK_LAUNCH();
@@ -1083,6 +1204,30 @@ int sisl_tricubic_f64(int32_t a, int32_t b, int32_t c, const double *f, const do
 int sisl_tricubic_f32(int32_t a, int32_t b, int32_t c, const float *f, const float *co,
                       float *out, int32_t lim, void *s) {
   return sisl::tricubic<float>(a, b, c, f, co, out, lim, s);
+}
+int sisl_tricubic2_f64(int32_t a, int32_t b, int32_t c, const double *f0, const double *f1, const double *co,
+                       double *o0, double *o1, int32_t lim, void *s) {
+  const double *f[2] = {f0, f1};
+  double *o[2] = {o0, o1};
+  return sisl::tricubic_multi<double, double, 2>(a, b, c, f, co, o, lim, s);
+}
+int sisl_tricubic2_f32(int32_t a, int32_t b, int32_t c, const float *f0, const float *f1, const float *co,
+                       float *o0, float *o1, int32_t lim, void *s) {
+  const float *f[2] = {f0, f1};
+  float *o[2] = {o0, o1};
+  return sisl::tricubic_multi<float, float, 2>(a, b, c, f, co, o, lim, s);
+}
+int sisl_tricubic_mixed(int32_t a, int32_t b, int32_t c, const double *f0, const float *co, double *o0,
+                        int32_t lim, void *s) {
+  const double *f[1] = {f0};
+  double *o[1] = {o0};
+  return sisl::tricubic_multi<float, double, 1>(a, b, c, f, co, o, lim, s);
+}
+int sisl_tricubic2_mixed(int32_t a, int32_t b, int32_t c, const double *f0, const double *f1, const float *co,
+                         double *o0, double *o1, int32_t lim, void *s) {
+  const double *f[2] = {f0, f1};
+  double *o[2] = {o0, o1};
+  return sisl::tricubic_multi<float, double, 2>(a, b, c, f, co, o, lim, s);
 }
 int sisl_tricubic_adj_plan_f64(int32_t a, int32_t b, int32_t c, const double *co, int32_t *key,
                                int32_t *perm, int32_t *start, int32_t *count, int32_t *sums,

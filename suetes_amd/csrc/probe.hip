@@ -1,0 +1,124 @@
+// Bandwidth probes for the roofline of the operator kernels (measurement only;
+// no solver path calls them).
+//
+// * stream_copy / stream_read: a plain copy and a plain read of `bytes` with
+//   16-byte vector accesses, one element per thread -- the attainable HBM rate
+//   of a well-formed streaming kernel on this device.
+// * operator_streams: the memory traffic of the FP32 operator launch with the
+//   fused FP16 basis normalisation (helmholtz_kernel<float, U, 1, bf16>) and
+//   nothing else: the same 5 packed input fields and 13 bfloat16 coefficient
+//   arrays, read once each at the thread's own column, the same 5 outputs and
+//   5 FP16 basis outputs, the same 64 x 4 tiles, level chunk and XCD tile
+//   order.  Its rate is the ceiling for the operator's access mix (as many
+//   streams, as many bytes, the same store granularity), with no neighbour
+//   loads and no arithmetic.
+#include "common.h"
+
+namespace sisl {
+
+__global__ void __launch_bounds__(256) stream_copy_kernel(const uint4 *__restrict__ in, uint4 *__restrict__ out,
+                                                         int64_t n) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) out[q] = in[q];
+}
+
+__global__ void __launch_bounds__(256) stream_read_kernel(const uint4 *__restrict__ in, uint32_t *__restrict__ out,
+                                                         int64_t n) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t acc = 0;
+  if (q < n) {
+    const uint4 v = in[q];
+    acc = v.x ^ v.y ^ v.z ^ v.w;
+  }
+  // a store that never happens for real data keeps the loads alive
+  if (acc == 0x9e3779b9u && q == 0) out[0] = acc;
+}
+
+constexpr int kPX = 64, kPY = 4;
+
+__device__ __forceinline__ float bfl(const bf16_t *b, uint32_t q) { return __uint_as_float((uint32_t)b[q] << 16); }
+
+__global__ void __launch_bounds__(256) operator_streams_kernel(OpT<float> o, const float *__restrict__ x,
+                                                              float *__restrict__ y, _Float16 *__restrict__ bas,
+                                                              int kchunk) {
+  const Stag &s = o.s;
+  const int nx = s.nx, ny = s.ny, nz = s.nz;
+  const int nbx = (nx + kPX - 1) / kPX, nby = (ny + kPY - 1) / kPY;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int bx = tile % nbx, by = (tile / nbx) % nby, bz = tile / (nbx * nby);
+  const int i = bx * kPX + threadIdx.x, j = by * kPY + threadIdx.y;
+  if (i >= nx || j >= ny) return;
+  const int k0 = bz * kchunk, k1 = min(k0 + kchunk, nz + 1);
+  const float *U = x + s.ou, *V = x + s.ov, *W = x + s.ow, *P = x + s.om, *E = x + s.oe;
+  float *YU = y + s.ou, *YV = y + s.ov, *YW = y + s.ow, *YP = y + s.om, *YE = y + s.oe;
+  _Float16 *BU = bas + s.ou, *BV = bas + s.ov, *BW = bas + s.ow, *BP = bas + s.om, *BE = bas + s.oe;
+  const bf16_t *c[13] = {reinterpret_cast<const bf16_t *>(o.cua), reinterpret_cast<const bf16_t *>(o.zxu),
+                         reinterpret_cast<const bf16_t *>(o.cx),  reinterpret_cast<const bf16_t *>(o.cva),
+                         reinterpret_cast<const bf16_t *>(o.zev), reinterpret_cast<const bf16_t *>(o.cy),
+                         reinterpret_cast<const bf16_t *>(o.dw),  reinterpret_cast<const bf16_t *>(o.cw),
+                         reinterpret_cast<const bf16_t *>(o.cz),  reinterpret_cast<const bf16_t *>(o.kx),
+                         reinterpret_cast<const bf16_t *>(o.ky),  reinterpret_cast<const bf16_t *>(o.rdzw),
+                         reinterpret_cast<const bf16_t *>(o.cpm)};
+  const uint32_t pm = (uint32_t)nx * ny, pu = (uint32_t)(nx + 1) * ny, pv = (uint32_t)nx * (ny + 1);
+  const uint32_t mc = (uint32_t)j * nx + i, uc = (uint32_t)j * (nx + 1) + i, vc = (uint32_t)j * nx + i;
+  for (int k = k0; k < k1; ++k) {
+    const uint32_t wk = mc + (uint32_t)k * pm;
+    float ww = ldg(W, wk) + ldg(E, wk);
+#pragma unroll
+    for (int q = 6; q < 12; ++q) ww += bfl(c[q], wk);
+    stg(YW, wk, ww);
+    stg(YE, wk, ww * 0.5f);
+    stg(BW, wk, (_Float16)ww);
+    stg(BE, wk, (_Float16)(ww * 0.5f));
+    if (k == nz) break;
+    const uint32_t au = uc + (uint32_t)k * pu, av = vc + (uint32_t)k * pv;
+    const float uu = ldg(U, au) + bfl(c[0], au) + bfl(c[1], au) + bfl(c[2], au);
+    const float vv = ldg(V, av) + bfl(c[3], av) + bfl(c[4], av) + bfl(c[5], av);
+    const float pp = ldg(P, wk) + bfl(c[12], wk);
+    stg(YU, au, uu);
+    stg(YV, av, vv);
+    stg(YP, wk, pp);
+    stg(BU, au, (_Float16)uu);
+    stg(BV, av, (_Float16)vv);
+    stg(BP, wk, (_Float16)pp);
+  }
+}
+
+}  // namespace sisl
+
+extern "C" {
+
+// mode 0: copy `bytes` from in to out; mode 1: read `bytes` of in (out: 4
+// bytes, never written for real data).  bytes: a multiple of 16.
+int sisl_probe_stream(const void *in, void *out, int64_t bytes, int32_t mode, void *s) {
+  if (!in || !out || bytes <= 0 || bytes % 16 || (mode != 0 && mode != 1)) return SISL_ERR_ARG;
+  const int64_t n = bytes / 16;
+  const dim3 g((unsigned)((n + 255) / 256));
+  if (mode == 0)
+    hipLaunchKernelGGL(sisl::stream_copy_kernel, g, dim3(256), 0, sisl::as_stream(s),
+                       static_cast<const uint4 *>(in), static_cast<uint4 *>(out), n);
+  else
+    hipLaunchKernelGGL(sisl::stream_read_kernel, g, dim3(256), 0, sisl::as_stream(s),
+                       static_cast<const uint4 *>(in), static_cast<uint32_t *>(out), n);
+  SISL_CHECK_LAUNCH();
+  return SISL_OK;
+}
+
+// The operator's access mix without its arithmetic (see the file comment).
+// op: the FP32 operator with bfloat16 3-D coefficients; x, y: packed FP32
+// vectors; basis: an FP16 packed vector; chunk: levels per thread (>= 1).
+int sisl_probe_operator_streams_f32(const SislOperator *op, const float *x, float *y, void *basis,
+                                    int32_t chunk, void *s) {
+  if (!op || !x || !y || !basis || chunk < 1) return SISL_ERR_ARG;
+  auto o = sisl::OpT<float>::make(*op);
+  if (!o.complete()) return SISL_ERR_ARG;
+  const sisl::Stag &st = o.s;
+  const dim3 g((unsigned)(((st.nx + sisl::kPX - 1) / sisl::kPX) * ((st.ny + sisl::kPY - 1) / sisl::kPY) *
+                          ((st.nz + 1 + chunk - 1) / chunk)));
+  hipLaunchKernelGGL(sisl::operator_streams_kernel, g, dim3(sisl::kPX, sisl::kPY), 0, sisl::as_stream(s), o, x,
+                     y, static_cast<_Float16 *>(basis), (int)chunk);
+  SISL_CHECK_LAUNCH();
+  return SISL_OK;
+}
+
+}  // extern "C"

@@ -319,6 +319,86 @@ class _Tricubic(torch.autograd.Function):
         return _tricubic_adjoint(g, data, ctx.share, ctx.tok), None, None
 
 
+def _tricubic_multi_fwd(fields, data, use_limiter):
+    """One launch over the fields sharing the offsets ``data``: FP64 fields
+    with FP32 offsets gather in FP32 and return FP64 (mixed precision)."""
+    _lib.require_device(data, *fields)
+    f0 = fields[0]
+    mixed = data.dtype == torch.float32 and f0.dtype == torch.float64
+    for f in fields:
+        if f.dtype != f0.dtype or tuple(data.shape) != (3,) + tuple(f.shape):
+            raise ValueError("fields must share dtype and shape, coords must be (3,)+field.shape")
+    if not mixed and data.dtype != f0.dtype:
+        raise ValueError("coords must have the fields' dtype (or FP32 offsets with FP64 fields)")
+    fz, fy, fx = f0.shape
+    outs = [torch.empty_like(f) for f in fields]
+    L = _lib.lib()
+    sfx = "mixed" if mixed else ("f64" if f0.dtype == torch.float64 else "f32")
+    s = _lib.stream_ptr(f0.device)
+    lim = int(bool(use_limiter))
+    if len(fields) == 1:
+        fn = L.sisl_tricubic_mixed if mixed else getattr(L, f"sisl_tricubic_{sfx}")
+        _lib.check(fn(fx, fy, fz, f0.data_ptr(), data.data_ptr(), outs[0].data_ptr(), lim, s), "tricubic")
+    elif len(fields) == 2:
+        _lib.check(getattr(L, f"sisl_tricubic2_{sfx}")(fx, fy, fz, fields[0].data_ptr(), fields[1].data_ptr(),
+                                                        data.data_ptr(), outs[0].data_ptr(), outs[1].data_ptr(),
+                                                        lim, s), "tricubic (two fields)")
+    else:
+        raise ValueError("one or two fields per launch")
+    return outs
+
+
+class _TricubicN(torch.autograd.Function):
+    """One or two fields gathered at one set of departure points in one
+    launch (FP64 fields with FP32 offsets: the mixed path, gathered in FP32).
+    Each field's adjoint is the deterministic transpose of its own gather
+    over the shared plan (FP32 in the mixed path, as the separate cast,
+    gather and widening would be differentiated)."""
+
+    @staticmethod
+    def forward(ctx, coords_obj, use_limiter, *fields):
+        fields = [f.contiguous() for f in fields]
+        data = coords_obj.data
+        ctx.set_materialize_grads(False)
+        ctx.share = coords_obj._share
+        ctx.toks = [ctx.share.register() for _ in fields]
+        ctx.use_limiter = use_limiter
+        if use_limiter:
+            ctx.save_for_backward(data, *fields)
+        else:
+            ctx.save_for_backward(data)
+        return tuple(_tricubic_multi_fwd(fields, data, use_limiter))
+
+    @staticmethod
+    def backward(ctx, *gs):
+        saved = ctx.saved_tensors
+        data, fields = saved[0], saved[1:]
+        out = []
+        for q, g in enumerate(gs):
+            tok = ctx.toks[q]
+            if g is None:
+                ctx.share.done(tok)
+                out.append(None)
+                continue
+            mixed = data.dtype == torch.float32 and g.dtype == torch.float64
+            gg = g.float() if mixed else g
+            if ctx.use_limiter:
+                f = fields[q].float() if mixed else fields[q]
+                gf = _limited_adjoint(gg, f, data, ctx.share, tok)
+            else:
+                gf = _tricubic_adjoint(gg, data, ctx.share, tok)
+            out.append(gf.double() if mixed else gf)
+        return (None, None) + tuple(out)
+
+
+def tensor_product_interp_pair(f0, f1, coords, use_limiter=False):
+    """``tensor_product_interp_3d`` of two fields on the same departure
+    points in one launch: the same bits as two separate calls."""
+    if not isinstance(coords, Coords):
+        coords = Coords(coords)
+    return _TricubicN.apply(coords, use_limiter, f0, f1)
+
+
 def tensor_product_interp_3d(field, coords, use_limiter=False):
     """Tricubic Catmull-Rom gather at departure points (operators.py:146-209).
 
@@ -329,7 +409,7 @@ def tensor_product_interp_3d(field, coords, use_limiter=False):
     if not isinstance(coords, Coords):
         coords = Coords(coords)
     if coords.data.dtype == torch.float32 and field.dtype == torch.float64:
-        return _Tricubic.apply(field.float(), coords, use_limiter).double()
+        return _TricubicN.apply(coords, use_limiter, field)[0]
     return _Tricubic.apply(field, coords, use_limiter)
 
 

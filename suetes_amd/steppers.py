@@ -22,7 +22,7 @@ import torch
 from . import _lib
 from .euler import Euler3D
 from .geometry import ref_view, to_internal
-from .operators import Coords, tensor_product_interp_3d
+from .operators import Coords, tensor_product_interp_3d, tensor_product_interp_pair
 
 KEYS = ("u", "v", "w", "pi", "eta_dot")
 LOC_ID = {"m": 0, "u": 1, "v": 2, "w": 3}
@@ -161,6 +161,15 @@ class SemiLagrangianAdvector3D:
 
     def advect_cubic(self, field, coords, use_limiter=False):
         return tensor_product_interp_3d(field, coords, use_limiter)
+
+    def advect_cubic_pair(self, f0, f1, coords, use_limiter=False):
+        """Two fields on the same departure points: one launch (the same
+        bits as two ``advect_cubic`` calls), or two calls of an
+        ``advect_cubic`` that a caller has put on this instance (the
+        ensemble routes them into its batched launches)."""
+        if "advect_cubic" in self.__dict__:
+            return self.advect_cubic(f0, coords, use_limiter), self.advect_cubic(f1, coords, use_limiter)
+        return tensor_product_interp_pair(f0, f1, coords, use_limiter)
 
 
 # ---------------------------------------------------------------------------
@@ -927,12 +936,10 @@ class SISLStepper3D:
         glue = self._fused(bg)
         u_in, v_in, w_in, pi_in, th_in, res_n, tth, x = ExplicitFn.apply(
             glue, state["u"], state["v"], state["w"], state["pi"], state["th_v"], state["eta_dot"])
-        lim, cub = self.use_limiter, self.advector.advect_cubic
-        reta = cub(res_n, co["w"], lim)
+        lim, pair = self.use_limiter, self.advector.advect_cubic_pair
+        reta, rhs_w = pair(res_n, w_in, co["w"], lim)
         rhs_u, rhs_v = self._advected_winds(u_in, v_in, co, state)
-        rhs_w = cub(w_in, co["w"], lim)
-        rhs_pi = cub(pi_in, co["m"], lim)
-        th_adv = cub(th_in, co["m"], lim)
+        rhs_pi, th_adv = pair(pi_in, th_in, co["m"], lim)
         if self._rho_transport_observed(bc_fn, state):
             rho_next = self.ffsl_advector.advect_3d_split(state["rho"], state, bg)
         else:
@@ -1009,7 +1016,8 @@ class SISLStepper3D:
         res_n = bg["dz_w_full"] * state["eta_dot"] + uw * g.i_z_xi_w + vw * g.i_z_eta_w - state["w"]
         lim = self.use_limiter
         cub = adv.advect_cubic
-        R_eta = -(1.0 - a) * cub(res_n, co["w"], lim)
+        res_adv, rhs_w = adv.advect_cubic_pair(res_n, w_in, co["w"], lim)
+        R_eta = -(1.0 - a) * res_adv
         rhs_u = cub(u_in, co["u"], lim)
         rhs_v = cub(v_in, co["v"], lim)
         if 0 in op.periodic_axes:
@@ -1020,9 +1028,7 @@ class SISLStepper3D:
             rhs_v = _set_slice(rhs_v, 1, -1, rhs_v[:, :1])
         else:
             rhs_v = _set_slice(_set_slice(rhs_v, 1, 0, state["v"][:, :1]), 1, -1, state["v"][:, -1:])
-        rhs_w = cub(w_in, co["w"], lim)
-        rhs_pi = cub(pi_in, co["m"], lim)
-        th_adv = cub(th_in, co["m"], lim)
+        rhs_pi, th_adv = adv.advect_cubic_pair(pi_in, th_in, co["m"], lim)
         th_next = th_adv + a * dt * tn["th_v"] + th_bg
         if self._rho_transport_observed(bc_fn, state):
             rho_next = self.ffsl_advector.advect_3d_split(state["rho"], state, bg)

@@ -92,6 +92,60 @@ def test_tricubic_staged_and_far_departures_match_oracle(lim):
     assert rel_l2(from_int(out32), ref) < 2e-6
 
 
+@pytest.mark.parametrize("lim", [False, True])
+@pytest.mark.parametrize("prec", ["f64", "f32", "mixed"])
+def test_tricubic_pair_and_mixed_storage_bit_identical(prec, lim):
+    """Two fields gathered in one launch on shared departure points give the
+    same bits as two single gathers, forward and adjoint; the mixed gather
+    (FP64 fields, FP32 offsets and arithmetic) gives the same bits as an FP64
+    -> FP32 cast, an FP32 gather and the widening of its result.  On a grid
+    of several tiles mixing staged and far departures; the FP64 pair also
+    against the oracle."""
+    from suetes_amd import operators as ops
+    rng = np.random.default_rng(71 + lim)
+    shape = (70, 9, 17)
+    base = index_grid(shape)
+    near = rng.uniform(-0.9, 0.9, size=(3,) + shape)
+    far = rng.uniform(-6.0, 6.0, size=(3,) + shape)
+    pts = base + np.where((rng.random(shape) < 0.2)[None], far, near)
+    fa, fb = rng.standard_normal(shape), rng.standard_normal(shape)
+    ga, gb = to_int(rng.standard_normal(shape)), to_int(rng.standard_normal(shape))
+    fdt = torch.float32 if prec == "f32" else torch.float64
+    cdt = torch.float64 if prec == "f64" else torch.float32
+    co = offsets_int(pts, cdt)
+
+    def single(f, g):
+        x = to_int(f, fdt).requires_grad_(True)
+        if prec == "mixed":
+            y = ops.tensor_product_interp_3d(x.float(), ops.Coords(co), lim).double()
+        else:
+            y = ops.tensor_product_interp_3d(x, ops.Coords(co), lim)
+        (gx,) = torch.autograd.grad(y, x, g.to(fdt))
+        return y.detach(), gx
+
+    ya, gxa = single(fa, ga)
+    yb, gxb = single(fb, gb)
+    xa, xb = to_int(fa, fdt).requires_grad_(True), to_int(fb, fdt).requires_grad_(True)
+    pa, pb = ops.tensor_product_interp_pair(xa, xb, ops.Coords(co), lim)
+    assert torch.equal(pa, ya) and torch.equal(pb, yb)
+    ha, hb = torch.autograd.grad((pa, pb), (xa, xb), (ga.to(fdt), gb.to(fdt)))
+    assert torch.equal(ha, gxa) and torch.equal(hb, gxb)
+    # one output unused: its field gets no cotangent, the other the single one
+    xa2 = to_int(fa, fdt).requires_grad_(True)
+    qa, _ = ops.tensor_product_interp_pair(xa2, to_int(fb, fdt), ops.Coords(co), lim)
+    (ka,) = torch.autograd.grad(qa, xa2, ga.to(fdt))
+    assert torch.equal(ka, gxa)
+    if prec == "mixed":  # the one-field mixed launch
+        xm = to_int(fa).requires_grad_(True)
+        ym = ops.tensor_product_interp_3d(xm, ops.Coords(co), lim)
+        assert ym.dtype == torch.float64 and torch.equal(ym, ya)
+        (gm,) = torch.autograd.grad(ym, xm, ga)
+        assert torch.equal(gm, gxa)
+    if prec == "f64":
+        assert rel_l2(from_int(pa), O.tricubic(fa, pts, lim)) < 1e-13
+        assert rel_l2(from_int(pb), O.tricubic(fb, pts, lim)) < 1e-13
+
+
 @pytest.mark.parametrize("checkpointed", [False, True])
 def test_tricubic_adjoint_plan_shared_then_released(monkeypatch, checkpointed):
     """Fields gathered at one coordinate set share one adjoint plan, built
