@@ -543,6 +543,14 @@ int sisl_gmres_batched_setup(const SislOperator *ops64, const SislOperator *ops3
                              int32_t restart, void *workspace, void *stream);
 int sisl_gmres_batched(int32_t nmembers, const SislGmresParams *p, const double *const *b,
                        double *const *x, void *workspace, SislGmresStats *stats, void *stream);
+/* Under a stream capture of the caller (a graph-captured time step)
+ * sisl_gmres_batched reads nothing back: it records every outer iteration
+ * the refinement could run (max_outer + 1), of which those after a member
+ * has converged do nothing on the device, so a replay gives the same bits
+ * as the eager solve; its stats are then set to cycles = -1.  This reads
+ * the members' outcomes of the last solve on the workspace (e.g. after a
+ * replay).  No counterpart in the reference. */
+int sisl_gmres_batched_outcome(int32_t nmembers, void *workspace, SislGmresStats *stats, void *stream);
 
 /* Packed-vector helpers used by the host layer. out = sum over d of x*y (FP64). */
 int sisl_dot_f64(const double *x, const double *y, int64_t n, double *out_dev, double *partials,

@@ -37,6 +37,7 @@ EXPORTED = [
     "sisl_tricubic_limited_adj_corners_f64", "sisl_tricubic_limited_adj_corners_f32",
     "sisl_helmholtz_apply_fused_f32", "sisl_probe_stream", "sisl_probe_operator_streams_f32",
     "sisl_tricubic2_f64", "sisl_tricubic2_f32", "sisl_tricubic_mixed", "sisl_tricubic2_mixed",
+    "sisl_gmres_batched_outcome",
 ]
 
 # timed kernels (enum SislTimedKernel)
@@ -125,6 +126,7 @@ def lib():
         L.sisl_gmres_batched_workspace_bytes.argtypes = [C.POINTER(SislGrid), i32, i32]
         L.sisl_gmres_batched_setup.argtypes = [C.POINTER(SislOperator), C.POINTER(SislOperator), i32, i32, vp, vp]
         L.sisl_gmres_batched.argtypes = [i32, C.POINTER(SislGmresParams), pp, pp, vp, C.POINTER(SislGmresStats), vp]
+        L.sisl_gmres_batched_outcome.argtypes = [i32, vp, C.POINTER(SislGmresStats), vp]
         for p in ("f64", "f32"):
             getattr(L, f"sisl_stagger_{p}").argtypes = [vp, vp, i64, i32, i64, i32, i32, d, vp, i64, vp]
             getattr(L, f"sisl_stagger_T_{p}").argtypes = [vp, vp, i64, i32, i64, i32, i32, i32, d, vp, i64, vp]

@@ -1707,13 +1707,20 @@ static int ens_cycle_eager(const EnsPlan &P, int transpose, int single_cgs, hipS
   return SISL_OK;
 }
 
+// Whether work issued to stream st is being captured into a graph.
+static bool stream_capturing(hipStream_t st) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  return hipStreamIsCapturing(st, &cs) == hipSuccess && cs == hipStreamCaptureStatusActive;
+}
+
 static int ens_cycle(const EnsPlan &P, int transpose, int single_cgs, hipStream_t st) {
   bool use_graph;
   {
     std::lock_guard<std::mutex> g(g_graph_mu);
     use_graph = g_graphs_on;
   }
-  if (!use_graph || timer_on()) return ens_cycle_eager(P, transpose, single_cgs, st);
+  // inside a caller's capture the cycle's launches become part of its graph
+  if (!use_graph || timer_on() || stream_capturing(st)) return ens_cycle_eager(P, transpose, single_cgs, st);
   std::string key;
   int dev = 0;
   hipGetDevice(&dev);
@@ -1775,6 +1782,8 @@ static int ens_cycle(const EnsPlan &P, int transpose, int single_cgs, hipStream_
   return SISL_OK;
 }
 
+static int ens_outcome(const EnsPlan &P, SislGmresStats *stats, hipStream_t st);
+
 static int ens_solve(const void *ws, const SislGmresParams &p, const double *const *b, double *const *x,
                      SislGmresStats *stats, hipStream_t st) {
   EnsPlan P;
@@ -1814,6 +1823,13 @@ static int ens_solve(const void *ws, const SislGmresParams &p, const double *con
   hipLaunchKernelGGL(kb_reduce_kernel, dim3(1, M), dim3(kBlock), 0, st, (const KMem *)P.km, 0, kSelScal0 + 1, 0, 1);
   hipLaunchKernelGGL(kb_init_kernel, dim3(M), dim3(64), 0, st, (const KMem *)P.km, p.tol, p.atol);
   const int max_outer = p.max_outer > 0 ? p.max_outer : p.maxiter;
+  // Under a caller's stream capture (a graph-captured time step) nothing is
+  // read back: every outer iteration the eager loop could run is recorded,
+  // and those after a member has finished do nothing on the device (every
+  // launch of the outer loop and of the cycle is gated by the member's done
+  // and stop flags), so a replay gives the eager solve's bits.  The members'
+  // outcomes stay on the device (sisl_gmres_batched_outcome).
+  const bool capturing = stream_capturing(st);
   for (int outer = 0;; ++outer) {
     // residual M (b - A x) in FP64, its norm, the first basis vector
     if (!p.transpose) {
@@ -1831,13 +1847,17 @@ static int ens_solve(const void *ws, const SislGmresParams &p, const double *con
     hipLaunchKernelGGL(kb_reduce_kernel, dim3(1, M), dim3(kBlock), 0, st, (const KMem *)P.km, 0, kSelScal0 + 2, 0, 1);
     hipLaunchKernelGGL(kb_reset_kernel, dim3(M), dim3(kBlock), 0, st, (const KMem *)P.km, R, p.inner_rtol,
                        max_outer);
-    // one read per outer iteration: whether every member has finished
-    hipLaunchKernelGGL(kb_all_done_kernel, dim3(1), dim3(64), 0, st, (const KMem *)P.km, M, P.all_done);
-    int all = 0;
-    if (hipMemcpyAsync(&all, P.all_done, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
-        hipStreamSynchronize(st) != hipSuccess)
-      return SISL_ERR_LAUNCH;
-    if (all || outer > max_outer) break;
+    if (capturing) {
+      if (outer > max_outer) break;
+    } else {
+      // one read per outer iteration: whether every member has finished
+      hipLaunchKernelGGL(kb_all_done_kernel, dim3(1), dim3(64), 0, st, (const KMem *)P.km, M, P.all_done);
+      int all = 0;
+      if (hipMemcpyAsync(&all, P.all_done, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
+          hipStreamSynchronize(st) != hipSuccess)
+        return SISL_ERR_LAUNCH;
+      if (all || outer > max_outer) break;
+    }
     if ((rc = ens_cycle(P, p.transpose, p.inner_single_cgs, st))) return rc;
   }
   for (int m = 0; m < M; ++m) {
@@ -1846,7 +1866,18 @@ static int ens_solve(const void *ws, const SislGmresParams &p, const double *con
   }
   hipLaunchKernelGGL(kb_copy_kernel, cg, dim3(kBlock), 0, st, tout);
   SISL_CHECK_LAUNCH();
-  // the members' outcomes, gathered on the device and read in one copy
+  if (capturing) {
+    if (stats)
+      for (int m = 0; m < M; ++m) stats[m] = SislGmresStats{-1, -1, 0.0, 0.0, 0.0};
+    return SISL_OK;
+  }
+  return ens_outcome(P, stats, st);
+}
+
+// The members' outcomes of the last solve on workspace P, gathered on the
+// device and read in one copy.
+static int ens_outcome(const EnsPlan &P, SislGmresStats *stats, hipStream_t st) {
+  const int M = P.M;
   hipLaunchKernelGGL(kb_outcome_kernel, dim3(1), dim3(64), 0, st, (const KMem *)P.km, M, P.outcome);
   std::vector<EnsOutcome> oc(M);
   if (hipMemcpyAsync(oc.data(), P.outcome, sizeof(EnsOutcome) * M, hipMemcpyDeviceToHost, st) != hipSuccess ||
@@ -1928,6 +1959,18 @@ int sisl_gmres_batched(int32_t M, const SislGmresParams *p, const double *const 
                        void *ws, SislGmresStats *stats, void *stream) {
   if (!p || !b || !x || !ws || M < 1 || M > sisl::kMaxMembers) return SISL_ERR_ARG;
   return sisl::ens_solve(ws, *p, b, x, stats, sisl::as_stream(stream));
+}
+
+int sisl_gmres_batched_outcome(int32_t M, void *ws, SislGmresStats *stats, void *stream) {
+  if (!ws || !stats || M < 1 || M > sisl::kMaxMembers) return SISL_ERR_ARG;
+  sisl::EnsPlan P;
+  {
+    std::lock_guard<std::mutex> g(sisl::g_ens_mu);
+    auto it = sisl::g_ens.find(ws);
+    if (it == sisl::g_ens.end() || it->second.M != M) return SISL_ERR_ARG;
+    P = it->second;
+  }
+  return sisl::ens_outcome(P, stats, sisl::as_stream(stream));
 }
 
 int sisl_dot_f64(const double *x, const double *y, int64_t n, double *out, double *partials,

@@ -241,6 +241,20 @@ class Ensemble:
         self._ws_key = key
         self._R = R
 
+    def solver_outcome(self):
+        """Per-member outcome (cycles, Arnoldi steps, residual, threshold) of
+        the last batched solve, read from the device."""
+        st = (_lib.SislGmresStats * self.M)()
+        _lib.check(_lib.lib().sisl_gmres_batched_outcome(self.M, self._workspace.data_ptr(), st,
+                                                         _lib.stream_ptr(self._workspace.device)),
+                   "batched solver outcome")
+        return [{"cycles": s.cycles, "arnoldi_steps": s.arnoldi_steps, "residual": s.residual, "ptol": s.ptol}
+                for s in st]
+
+    def capture_step(self, states, t, forcings, bc_fns):
+        """Record one lockstep step as a hipGraph (:class:`CapturedStep`)."""
+        return CapturedStep(self, states, t, forcings, bc_fns)
+
     def solve_flat(self, bs, x0s, transpose=False):
         sols = [s.implicit_solver for s in self.steppers]
         xs = [x.detach().clone().contiguous() for x in x0s]
@@ -407,6 +421,65 @@ class Ensemble:
             else:
                 sts = self.step_internal(sts, t, forcings, bc_fns)
         return [state_to_reference(s) for s in sts]
+
+
+class CapturedStep:
+    """One lockstep step of an :class:`Ensemble` recorded once as a hipGraph
+    and replayed: every member's departure points, gathers, fused glue,
+    boundaries and the batched solves of the step, with no host work between
+    the launches.  The batched solver records every refinement iteration it
+    could take (gated on the device once a member has converged), so each
+    replay gives the bits of an eager lockstep step.
+
+    ``states``: internal member dicts of a step that is not a first step
+    (history keys present); they are copied into static buffers, which
+    :meth:`advance` replaces with each step's result.  Recording needs the
+    ensemble to have taken an eager step with the same members, forcings and
+    boundary functions (workspaces, plans and tuned launch shapes exist).
+    No gradients: the recorded step runs without autograd."""
+
+    def __init__(self, ens, states, t, forcings, bc_fns):
+        for s in states:
+            if float(s.get("is_first_step", 1.0)) == 1.0:
+                raise ValueError("record a step after the first one (history keys in place)")
+        self.ens = ens
+        self.static = [{k: (v.detach().clone() if torch.is_tensor(v) else v) for k, v in s.items()}
+                       for s in states]
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.no_grad():
+            with torch.cuda.graph(self.graph):
+                self.out = ens.step_internal(self.static, t, forcings, bc_fns)
+        self._pairs = []
+        for st, out in zip(self.static, self.out):
+            for k, v in out.items():
+                if torch.is_tensor(v) and torch.is_tensor(st.get(k)):
+                    if st[k].shape != v.shape or st[k].dtype != v.dtype:
+                        raise ValueError(f"step output {k!r} does not match its input")
+                    self._pairs.append((st[k], v))
+        ptrs = {d.data_ptr() for d, _ in self._pairs}
+        self._alias = [src.data_ptr() in ptrs for _, src in self._pairs]
+
+    def replay(self):
+        """Run the recorded step on the static states; the result (internal
+        dicts, valid until the next replay) is returned."""
+        self.graph.replay()
+        return self.out
+
+    def advance(self):
+        """Replay, then make the result the static states of the next step
+        (outputs that are inputs of this step, the history keys, are copied
+        first)."""
+        self.graph.replay()
+        with torch.no_grad():
+            srcs = [src.clone() if a else src for (_, src), a in zip(self._pairs, self._alias)]
+            for (dst, _), src in zip(self._pairs, srcs):
+                dst.copy_(src)
+        return self.static
+
+    def stats(self):
+        """The batched solver's per-member outcome of the last solve of the
+        last replay (host read)."""
+        return self.ens.solver_outcome()
 
 
 def _flatten(states):
