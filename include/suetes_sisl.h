@@ -198,7 +198,9 @@ int sisl_helmholtz_apply_f32(const SislOperator *op, const float *x, float *y, v
 /* Autotuner of the operator apply: times every launch shape (levels per
  * thread, loop unroll) on the caller's packed buffers x (input, any finite
  * values) and y (scratch output) and keeps the fastest for this grid and
- * element type (fp32 = 1: the _f32 entry points) for the rest of the
+ * element type (fp32 = 1: the _f32 entry points; fp32 = 2: FP32 with the
+ * op's 3-D coefficient arrays in bfloat16, the mixed solver's inner
+ * operator, whose fused launches are tuned on their own) for the rest of the
  * process.  All shapes give bit-identical results.  `scratch`: caller-owned
  * device memory of sisl_autotune_scratch_bytes(g, fp32) bytes (NULL when
  * that is 0), used by the fused-normalisation variants.  Optional outputs:

@@ -5,6 +5,7 @@
 // u | v | w | pi*1000 | eta_dot*dz_w.  Every coefficient that the reference
 // recomputes per call (background averages, metrics, map factors) is
 // precomputed once per stepper by the host layer.
+#include <algorithm>
 #include <map>
 #include <mutex>
 #include <tuple>
@@ -1250,11 +1251,17 @@ uint64_t shape_generation() {
   return g_tune_generation;
 }
 
-// variant = transpose + 2 * (fused basis normalisation)
+// variant = transpose + 2 * (fused basis normalisation) + 4 * (bfloat16
+// coefficients); a bfloat16 variant not tuned on its own uses the FP32
+// coefficients' shape.
 static AShape a_shape(const Stag &s, int bytes, int variant = 0) {
   std::lock_guard<std::mutex> g(g_tune_mu);
   auto it = g_tune.find({s.nx, s.ny, s.nz, bytes, variant});
   if (it != g_tune.end()) return it->second;
+  if (variant >= 4) {
+    it = g_tune.find({s.nx, s.ny, s.nz, bytes, variant - 4});
+    if (it != g_tune.end()) return it->second;
+  }
   return AShape{(s.nz + 1 + 7) / 8, 1};
 }
 
@@ -1378,7 +1385,7 @@ template <typename T>
 int launch_A(const OpT<T> &o, const T *x, T *y, hipStream_t st, const int *stop, const Fuse *fuse) {
   if ((uint64_t)o.s.n * sizeof(T) >= (1ull << 32)) return SISL_ERR_ARG;  // 32-bit offsets
   const Fuse f = fuse ? *fuse : Fuse{};
-  const AShape sh = a_shape(o.s, (int)sizeof(T), f.nb ? 2 : 0);
+  const AShape sh = a_shape(o.s, (int)sizeof(T), (f.nb ? 2 : 0) + (f.cb ? 4 : 0));
   const Stag &s = o.s;
   const double sc = f.cb ? 2.0 : (double)sizeof(T);  // bytes per coefficient element
   const double bytes = (double)sizeof(T) * 2.0 * s.n + sc * (3.0 * s.u.n() + 3.0 * s.v.n() + 6.0 * s.w.n() + s.m.n()) +
@@ -1450,7 +1457,7 @@ int launch_AT(const OpT<T> &o, const T *y, T *x, T * /*scratch: unused*/, hipStr
               const Fuse *fuse) {
   if ((uint64_t)o.s.n * sizeof(T) >= (1ull << 32)) return SISL_ERR_ARG;  // 32-bit offsets
   const Fuse f = fuse ? *fuse : Fuse{};
-  const AShape sh = a_shape(o.s, (int)sizeof(T), f.nb ? 3 : 1);
+  const AShape sh = a_shape(o.s, (int)sizeof(T), (f.nb ? 3 : 1) + (f.cb ? 4 : 0));
   const Stag &s = o.s;
   // same arrays as the forward operator: y in, x out, 13 coefficient arrays (+ the basis column)
   const double sc = f.cb ? 2.0 : (double)sizeof(T);  // bytes per coefficient element
@@ -1479,7 +1486,7 @@ int64_t autotune_scratch_bytes(const Stag &s, bool fp32) { return fp32 ? 256 + 2
 
 template <typename T>
 static int autotune_A(const OpT<T> &o, const T *x, T *y, void *scratch, hipStream_t st, int32_t *best_kc,
-                      int32_t *best_unr) {
+                      int32_t *best_unr, bool cb = false) {
   if ((uint64_t)o.s.n * sizeof(T) >= (1ull << 32)) return SISL_ERR_ARG;
   const int levels = o.s.nz + 1;
   std::vector<AShape> cand;
@@ -1490,8 +1497,12 @@ static int autotune_A(const OpT<T> &o, const T *x, T *y, void *scratch, hipStrea
       if (kc >= 2) cand.push_back({kc, unr});
     }
   const int nvar = sizeof(T) == 4 ? 4 : 2;
+  // bfloat16 coefficients (cb): the fused variants only (the inner solver's launches)
+  const int var0 = cb ? 2 : 0;
+  if (cb && sizeof(T) != 4) return SISL_ERR_ARG;
   // fused variants: scale 1 from device memory, basis column into the caller's scratch
   Fuse fz;
+  fz.cb = cb ? 1 : 0;
   if (nvar == 4) {
     if (!scratch) return SISL_ERR_ARG;
     hipLaunchKernelGGL(set_one_kernel, dim3(1), dim3(64), 0, st, static_cast<double *>(scratch));
@@ -1510,24 +1521,32 @@ static int autotune_A(const OpT<T> &o, const T *x, T *y, void *scratch, hipStrea
   AShape best[4];
   for (int v = 0; v < 4; ++v) best[v] = {(levels + 7) / 8, 1};
   int rc = SISL_OK;
-  for (int var = 0; var < nvar && rc == SISL_OK; ++var) {
+  for (int var = var0; var < nvar && rc == SISL_OK; ++var) {
     const int tr = var & 1;
     const Fuse f = var >= 2 ? fz : Fuse{};
+    // two timing rounds over all candidates, the faster of each candidate's
+    // two times counts (one round alone let clock noise pick a shape ~20%
+    // slower than the best)
+    std::vector<float> t(cand.size(), 1e30f);
+    for (int round = 0; round < 2 && rc == SISL_OK; ++round)
+      for (size_t ci = 0; ci < cand.size(); ++ci) {
+        const AShape &c = cand[ci];
+        auto run = [&]() {
+          if (tr) launch_AT_shape(o, x, y, st, nullptr, c, f);
+          else launch_A_shape(o, x, y, st, nullptr, c, f);
+        };
+        for (int r = 0; r < 2; ++r) run();
+        hipEventRecord(e0, st);
+        for (int r = 0; r < 5; ++r) run();
+        hipEventRecord(e1, st);
+        if (hipEventSynchronize(e1) != hipSuccess) { rc = SISL_ERR_LAUNCH; break; }
+        float ms = 0;
+        hipEventElapsedTime(&ms, e0, e1);
+        t[ci] = std::min(t[ci], ms);
+      }
     float best_ms = 1e30f;
-    for (const AShape &c : cand) {
-      auto run = [&]() {
-        if (tr) launch_AT_shape(o, x, y, st, nullptr, c, f);
-        else launch_A_shape(o, x, y, st, nullptr, c, f);
-      };
-      for (int r = 0; r < 2; ++r) run();
-      hipEventRecord(e0, st);
-      for (int r = 0; r < 5; ++r) run();
-      hipEventRecord(e1, st);
-      if (hipEventSynchronize(e1) != hipSuccess) { rc = SISL_ERR_LAUNCH; break; }
-      float ms = 0;
-      hipEventElapsedTime(&ms, e0, e1);
-      if (ms < best_ms) { best_ms = ms; best[var] = c; }
-    }
+    for (size_t ci = 0; ci < cand.size(); ++ci)
+      if (t[ci] < best_ms) { best_ms = t[ci]; best[var] = cand[ci]; }
   }
   hipEventDestroy(e0);
   hipEventDestroy(e1);
@@ -1535,11 +1554,12 @@ static int autotune_A(const OpT<T> &o, const T *x, T *y, void *scratch, hipStrea
   SISL_CHECK_LAUNCH();
   {
     std::lock_guard<std::mutex> g(g_tune_mu);
-    for (int var = 0; var < nvar; ++var) g_tune[{o.s.nx, o.s.ny, o.s.nz, (int)sizeof(T), var}] = best[var];
+    for (int var = var0; var < nvar; ++var)
+      g_tune[{o.s.nx, o.s.ny, o.s.nz, (int)sizeof(T), var + (cb ? 4 : 0)}] = best[var];
     ++g_tune_generation;
   }
-  if (best_kc) *best_kc = best[0].kc;
-  if (best_unr) *best_unr = best[0].unr;
+  if (best_kc) *best_kc = best[var0].kc;
+  if (best_unr) *best_unr = best[var0].unr;
   return SISL_OK;
 }
 
@@ -1638,7 +1658,7 @@ int sisl_helmholtz_apply_f32(const SislOperator *op, const float *x, float *y, v
 int sisl_set_operator_shape(const SislGrid *g, int32_t fp32, int32_t chunk, int32_t unroll) {
   if (!g || chunk < 2 || unroll < 1 || unroll > 3) return SISL_ERR_ARG;
   std::lock_guard<std::mutex> lk(sisl::g_tune_mu);
-  for (int var = 0; var < 4; ++var) sisl::g_tune[{g->nx, g->ny, g->nz, fp32 ? 4 : 8, var}] = sisl::AShape{chunk, unroll};
+  for (int var = 0; var < 8; ++var) sisl::g_tune[{g->nx, g->ny, g->nz, fp32 ? 4 : 8, var}] = sisl::AShape{chunk, unroll};
   ++sisl::g_tune_generation;
   return SISL_OK;
 }
@@ -1650,10 +1670,12 @@ int64_t sisl_autotune_scratch_bytes(const SislGrid *g, int32_t fp32) {
 int sisl_autotune_operator(const SislOperator *op, int32_t fp32, void *x, void *y, void *scratch,
                            int32_t *chunk, int32_t *unroll, void *s) {
   if (!op || !x || !y || x == y) return SISL_ERR_ARG;
+  if (fp32 < 0 || fp32 > 2) return SISL_ERR_ARG;
   if (fp32) {
     auto o = sisl::OpT<float>::make(*op);
     if (!o.complete()) return SISL_ERR_ARG;
-    return sisl::autotune_A(o, (const float *)x, (float *)y, scratch, sisl::as_stream(s), chunk, unroll);
+    return sisl::autotune_A(o, (const float *)x, (float *)y, scratch, sisl::as_stream(s), chunk, unroll,
+                            fp32 == 2);
   }
   auto o = sisl::OpT<double>::make(*op);
   if (!o.complete()) return SISL_ERR_ARG;

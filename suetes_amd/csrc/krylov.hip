@@ -831,6 +831,8 @@ static void update(const TB *V, int64_t ldv, int nv, const double *h, TV *w, int
   else if (nv == 3) SISL_UPDX(3);
   else if (nv == 4) SISL_UPDX(4);
   else if (nv == 5) SISL_UPDX(5);
+  else if (nv == 9) SISL_UPDX(9);    // the two-group form would load 16 vectors
+  else if (nv == 10) SISL_UPDX(10);
   else
     hipLaunchKernelGGL((update_kernel<TB, TV>), dim3(kRedGrid), dim3(kBlock), 0, s, V, ldv, nv, h, w,
                        n, partials, vinv, stop, gate);

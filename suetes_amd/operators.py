@@ -376,7 +376,7 @@ class _TricubicN(torch.autograd.Function):
         out = []
         for q, g in enumerate(gs):
             tok = ctx.toks[q]
-            if g is None:
+            if g is None or not ctx.needs_input_grad[2 + q]:
                 ctx.share.done(tok)
                 out.append(None)
                 continue

@@ -302,6 +302,7 @@ class _OperatorSet:
         g = SislGrid(grid.nx, grid.ny, grid.nz, int(0 in periodic_axes), int(1 in periodic_axes))
         self.struct = SislOperator(g, float(alpha), *[self.tensors[k].data_ptr() for k in OPERATOR_FIELDS])
         self.dtype = dtype
+        self.coef_bf16 = c3 == torch.bfloat16
 
 
 _TUNED = set()
@@ -311,7 +312,7 @@ def _autotune(ops):
     """Let the library time the launch shapes of the operator apply once per
     grid and element type (the choice is kept for the process)."""
     g = ops.struct.g
-    key = (g.nx, g.ny, g.nz, ops.dtype)
+    key = (g.nx, g.ny, g.nz, ops.dtype, ops.coef_bf16)
     if key in _TUNED:
         return
     L = _lib.lib()
@@ -321,6 +322,8 @@ def _autotune(ops):
     y = torch.empty_like(x)
     fp32 = int(ops.dtype == torch.float32)
     nb = int(L.sisl_autotune_scratch_bytes(C.byref(g), fp32))
+    if ops.coef_bf16:
+        fp32 = 2  # FP32 arithmetic, bfloat16 3-D coefficients: the fused launches tuned on their own
     scr = torch.empty(max(nb, 1), dtype=torch.uint8, device=dev)
     kc, un = C.c_int32(), C.c_int32()
     _lib.check(L.sisl_autotune_operator(C.byref(ops.struct), fp32, x.data_ptr(), y.data_ptr(),
@@ -502,7 +505,7 @@ class SemiImplicitSolver3D:
             raise ValueError("restart above 64 is not supported by the HIP GMRES")
         nbytes = (L.sisl_gmres_workspace_bytes_mixed if self.precision == "mixed"
                   else L.sisl_gmres_workspace_bytes_f64)(C.byref(self.op64.struct.g), R)
-        for ops in (self.op64, self.op32):
+        for ops in (self.op64, self.op32, self.op_inner):
             if ops is not None:
                 _autotune(ops)
         self._ws_bytes = int(nbytes)
@@ -542,7 +545,8 @@ class SemiImplicitSolver3D:
         """bfloat16 coefficients for the forward solve only: the transposed
         kernels gather more neighbour loads per element and run slower on
         16-bit loads (measured), so the transposed solve keeps FP32."""
-        return self.precision == "mixed" and self.inner_coefficients == "bf16" and not transpose
+        return (self.precision == "mixed" and self.inner_coefficients == "bf16"
+                and (not transpose or getattr(self, "bf16_transposed", False)))
 
     def solve_flat(self, b, x0, transpose=False):
         """Solve on packed scaled vectors; returns a new tensor."""

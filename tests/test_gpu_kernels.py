@@ -135,6 +135,17 @@ def test_tricubic_pair_and_mixed_storage_bit_identical(prec, lim):
     qa, _ = ops.tensor_product_interp_pair(xa2, to_int(fb, fdt), ops.Coords(co), lim)
     (ka,) = torch.autograd.grad(qa, xa2, ga.to(fdt))
     assert torch.equal(ka, gxa)
+    # one field without a gradient: only the other's transpose runs
+    from suetes_amd import _lib
+    xa3 = to_int(fa, fdt).requires_grad_(True)
+    ra, rb = ops.tensor_product_interp_pair(xa3, to_int(fb, fdt), ops.Coords(co), lim)
+    _lib.timer_enable(True)
+    (la,) = torch.autograd.grad((ra, rb), xa3, (ga.to(fdt), gb.to(fdt)), allow_unused=True)
+    torch.cuda.synchronize()
+    _lib.timer_enable(False)
+    assert torch.equal(la, gxa)
+    if not lim:
+        assert _lib.timer_read("tricubic_adj", fdt == torch.float32 or prec == "mixed")[1] == 1
     if prec == "mixed":  # the one-field mixed launch
         xm = to_int(fa).requires_grad_(True)
         ym = ops.tensor_product_interp_3d(xm, ops.Coords(co), lim)
@@ -463,6 +474,38 @@ def test_mixed_solve_independent_of_launch_shapes(transpose):
     ref = outs[0][1]
     for tag, x in outs[1:]:
         assert torch.equal(x, ref), (tag, (x - ref).abs().max().item())
+
+
+def test_bfloat16_inner_operator_autotune_keeps_the_bits():
+    """The inner operator's bfloat16 launches are tuned on their own (fp32 =
+    2: FP32 arithmetic, bfloat16 3-D coefficients); re-tuning returns a valid
+    shape and leaves the mixed solve's bits unchanged."""
+    import ctypes as C
+    from suetes_amd import _lib
+    from suetes_amd.steppers import SemiImplicitSolver3D
+    case, go, A_o, M_o, sol = _solver_pair(small_lam())
+    rng = np.random.default_rng(31)
+    b = pack_ref(A_o(_rand_krylov(go, rng)))
+    ms = SemiImplicitSolver3D(sol.physics, sol.dt, sol.alpha, solver_tol=1e-10, solver_maxiter=30,
+                              solver_restart=20, precision="mixed")
+    ms.prepare(case.stepper.bg_precomputed())
+    assert ms.op_inner.coef_bf16
+    x1 = ms.solve_flat(b, torch.zeros_like(b))
+    L = _lib.lib()
+    g = ms.op_inner.struct.g
+    n = int(L.sisl_krylov_size(C.byref(g)))
+    x = torch.randn(n, dtype=torch.float32, device=DEV)
+    y = torch.empty_like(x)
+    nb = int(L.sisl_autotune_scratch_bytes(C.byref(g), 1))
+    scr = torch.empty(nb, dtype=torch.uint8, device=DEV)
+    kc, un = C.c_int32(), C.c_int32()
+    _lib.check(L.sisl_autotune_operator(C.byref(ms.op_inner.struct), 2, x.data_ptr(), y.data_ptr(),
+                                        scr.data_ptr(), C.byref(kc), C.byref(un), _lib.stream_ptr()),
+               "bf16 autotune")
+    assert kc.value >= 2 and un.value in (1, 2, 3)
+    x2 = ms.solve_flat(b, torch.zeros_like(b))
+    torch.cuda.synchronize()
+    assert torch.equal(x1, x2)
 
 
 def test_inner_cycle_graph_replays_match_eager_launches():
