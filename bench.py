@@ -202,6 +202,24 @@ def attainable(case, reps=20):
                       "rate is the operator launch's algorithmic bytes over the probe's time"}
 
 
+def operator_shapes(case):
+    """The operator launch shapes in force (levels per thread, unroll; unroll
+    3 = two columns per thread) for the launches of the mixed and FP64 steps."""
+    import ctypes as C
+    from suetes_amd import _lib
+    L = _lib.lib()
+    g = case.stepper.implicit_solver.op64.struct.g
+    out = {}
+    for name, fp32, var in (("fp32 A, fused normalisation, bf16 coefficients (inner forward)", 1, 6),
+                            ("fp32 A^T, fused normalisation (inner transposed)", 1, 3),
+                            ("fp64 A (FP64 solver; refinement residual)", 0, 0),
+                            ("fp64 A^T", 0, 1)):
+        kc, un = C.c_int32(), C.c_int32()
+        _lib.check(L.sisl_get_operator_shape(C.byref(g), fp32, var, C.byref(kc), C.byref(un)), "shape")
+        out[name] = [kc.value, un.value]
+    return out
+
+
 def copy_reference(nbytes=1 << 29, reps=5):
     """A plain device copy of known size (reads and writes `nbytes` each):
     the scale for the FETCH_SIZE / WRITE_SIZE hardware counters."""
@@ -690,9 +708,7 @@ def main():
         "roofline": roof, "kernels_mixed_step": table, "cpu_baseline": cpu,
         "literal_config": literal,
         "split_explicit": split,
-        "autotuned_operator_shape": {
-            "fp32": getattr(cmx.stepper.implicit_solver.op32, "tuned", None),
-            "fp64": getattr(c64.stepper.implicit_solver.op64, "tuned", None)},
+        "operator_launch_shapes": operator_shapes(cmx),
     }
     print(json.dumps(out), flush=True)
 

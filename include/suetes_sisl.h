@@ -202,9 +202,12 @@ int sisl_helmholtz_apply_f32(const SislOperator *op, const float *x, float *y, v
  * op's 3-D coefficient arrays in bfloat16, the mixed solver's inner
  * operator, whose fused launches are tuned on their own) for the rest of the
  * process.  All shapes give bit-identical results.  `scratch`: caller-owned
- * device memory of sisl_autotune_scratch_bytes(g, fp32) bytes (NULL when
- * that is 0), used by the fused-normalisation variants.  Optional outputs:
- * the chosen chunk and unroll.  Synchronises on its own timing events. */
+ * device memory of sisl_autotune_scratch_bytes(g, fp32) bytes: the
+ * fused-normalisation variants' scale and basis column, and a 512 MiB region
+ * copied before every timed launch so that no candidate runs from the
+ * last-level cache.  Optional outputs: the chosen chunk and unroll (of the
+ * plain operator; of its fused launch for fp32 = 2).  Synchronises on its own
+ * timing events. */
 int64_t sisl_autotune_scratch_bytes(const SislGrid *g, int32_t fp32);
 int sisl_autotune_operator(const SislOperator *op, int32_t fp32, void *x, void *y, void *scratch,
                            int32_t *chunk, int32_t *unroll, void *stream);
@@ -214,6 +217,9 @@ int sisl_autotune_operator(const SislOperator *op, int32_t fp32, void *x, void *
  * its transpose, with and without the fused basis normalisation, e.g. to
  * reproduce a tuned choice.  Every shape gives the same bits. */
 int sisl_set_operator_shape(const SislGrid *g, int32_t fp32, int32_t chunk, int32_t unroll);
+/* The launch shape in force for grid g, element type and variant
+ * (transpose + 2 * fused normalisation + 4 * bfloat16 coefficients). */
+int sisl_get_operator_shape(const SislGrid *g, int32_t fp32, int32_t variant, int32_t *chunk, int32_t *unroll);
 /* y = A^T x (the transpose used by the GMRES adjoint solve,
  * steppers.py:473-481).  scratch: not used (may be NULL); kept so that the
  * transposes of A and M share one signature shape. */

@@ -37,7 +37,7 @@ EXPORTED = [
     "sisl_tricubic_limited_adj_corners_f64", "sisl_tricubic_limited_adj_corners_f32",
     "sisl_helmholtz_apply_fused_f32", "sisl_probe_stream", "sisl_probe_operator_streams_f32",
     "sisl_tricubic2_f64", "sisl_tricubic2_f32", "sisl_tricubic_mixed", "sisl_tricubic2_mixed",
-    "sisl_gmres_batched_outcome",
+    "sisl_gmres_batched_outcome", "sisl_get_operator_shape",
 ]
 
 # timed kernels (enum SislTimedKernel)
@@ -117,6 +117,7 @@ def lib():
         L.sisl_autotune_scratch_bytes.restype = i64
         L.sisl_autotune_scratch_bytes.argtypes = [C.POINTER(SislGrid), i32]
         L.sisl_set_operator_shape.argtypes = [C.POINTER(SislGrid), i32, i32, i32]
+        L.sisl_get_operator_shape.argtypes = [C.POINTER(SislGrid), i32, i32, C.POINTER(i32), C.POINTER(i32)]
         pp = C.POINTER(vp)
         for p in ("f64", "f32"):
             getattr(L, f"sisl_tricubic_batched_{p}").argtypes = [i32, C.POINTER(i32), pp, pp, pp, i32, vp]
