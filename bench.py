@@ -579,6 +579,51 @@ def split_explicit_leg(args, steps=3, warmup=1):
             "max_abs_w_at_end": float(s["w"].abs().max())}
 
 
+def trace_summary(path, step=2):
+    """Per-kernel breakdown of one step from a rocprofv3 kernel-trace CSV of
+    `bench.py --steps 2 --warmup 1 ...` (steps are delimited by the first of
+    the four departure-point launches of each step; step 0 is the mixed
+    path's warm-up with the autotuning, steps 1-2 its timed steps).  Prints a
+    markdown table."""
+    import csv
+    import re
+    rows = []
+    with open(path, newline="") as f:
+        for r in csv.DictReader(f):
+            rows.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), r["Kernel_Name"]))
+    rows.sort()
+
+    codes = {"DF16_": "half", "f": "float", "d": "double", "t": "bf16"}
+
+    def short(name):
+        n = re.sub(r"\(.*", "", name).replace("void ", "").replace("unsigned short", "bf16")
+        m = re.match(r"_ZN4sisl\d+(\w+?)I((?:DF16_|[fdt]|Li\d+E)+)E", n)
+        if m:  # a mangled instantiation: decode its template arguments
+            toks = re.findall(r"DF16_|Li\d+E|[fdt]", m.group(2))
+            return "sisl::%s<%s>" % (m.group(1), ", ".join(
+                codes.get(t, t[2:-1]) for t in toks))
+        if n.startswith("sisl::"):
+            return n
+        if "at::native" in name or "copyBuffer" in name:
+            return "torch elementwise / copy"
+        return n[:40]
+    starts = [i for i, r in enumerate(rows) if "index_velocity_kernel" in r[2]][::4]
+    a, b = starts[step], starts[step + 1]
+    win = rows[a:b]
+    span = (win[-1][1] - win[0][0]) / 1e6
+    tot = {}
+    for s0, s1, n in win:
+        k = short(n)
+        c, t = tot.get(k, (0, 0.0))
+        tot[k] = (c + 1, t + (s1 - s0) / 1e6)
+    busy = sum(t for _, t in tot.values())
+    print(f"step {step}: {len(win)} launches, span {span:.1f} ms, kernels busy {busy:.1f} ms")
+    print("| kernel | launches | total ms | avg us |")
+    print("|---|---:|---:|---:|")
+    for k, (c, t) in sorted(tot.items(), key=lambda kv: -kv[1][1]):
+        print(f"| `{k}` | {c} | {t:.2f} | {1e3 * t / c:.1f} |")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -609,7 +654,13 @@ def main():
     ap.add_argument("--kernels-only", action="store_true",
                     help="only launch the operator kernel and a plain copy of known size "
                          "(the run that hardware-counter passes profile)")
+    ap.add_argument("--trace-summary", default=None,
+                    help="print the per-kernel breakdown of one mixed step from a rocprofv3 kernel-trace CSV "
+                         "(of a --steps 2 --warmup 1 run) and exit")
     args = ap.parse_args()
+    if args.trace_summary:
+        trace_summary(args.trace_summary)
+        return
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1",
               file=sys.stderr)
