@@ -237,6 +237,14 @@ int sisl_helmholtz_apply_T_f32(const SislOperator *op, const float *x, float *y,
 int sisl_helmholtz_apply_fused_f32(const SislOperator *op, const float *x, float *y, const double *inv,
                                    void *basis, double vs, int32_t coef_bf16, int32_t transpose,
                                    void *stream);
+/* The inner FP32 solver's operator launch on a stored FP16 basis column
+ * (inner_basis_operand; no reference counterpart of its own: it is
+ * Euler3D.linear_operator, euler.py:374-439, applied to the Arnoldi vector
+ * of the mixed GMRES): y = A (x16 s) or A^T (x16 s) with s = *inv, x16 the
+ * packed vector in FP16, converted to FP32 on load; y = the plain FP32
+ * launch on the converted values, times s.  coef_bf16 as above. */
+int sisl_helmholtz_apply_h16_f32(const SislOperator *op, const void *x16, float *y, const double *inv,
+                                 int32_t coef_bf16, int32_t transpose, void *stream);
 
 /* ---- (c) column preconditioner ----------------------------------------- */
 
@@ -272,6 +280,12 @@ typedef struct SislGmresParams {
                               inner Arnoldi step (0: iterated CGS as in the
                               FP64 solver, a second pass when the first
                               removed most of the vector)                     */
+  int32_t inner_basis_operand; /* mixed solver with basis_fp16 and
+                              inner_single_cgs: the inner operator reads the
+                              stored FP16 basis column and each update pass
+                              stores the next column (one scale per column)
+                              instead of writing an FP32 Arnoldi vector back
+                              (0: FP32 Arnoldi vectors between the passes)   */
 } SislGmresParams;
 
 typedef struct SislGmresStats {

@@ -35,7 +35,7 @@ EXPORTED = [
     "sisl_acoustic_workspace_f64", "sisl_acoustic_loop_f64", "sisl_slow_tendencies_f64",
     "sisl_tricubic_limited_adj_split_f64", "sisl_tricubic_limited_adj_split_f32",
     "sisl_tricubic_limited_adj_corners_f64", "sisl_tricubic_limited_adj_corners_f32",
-    "sisl_helmholtz_apply_fused_f32", "sisl_probe_stream", "sisl_probe_operator_streams_f32",
+    "sisl_helmholtz_apply_fused_f32", "sisl_helmholtz_apply_h16_f32", "sisl_probe_stream", "sisl_probe_operator_streams_f32",
     "sisl_tricubic2_f64", "sisl_tricubic2_f32", "sisl_tricubic_mixed", "sisl_tricubic2_mixed",
     "sisl_gmres_batched_outcome", "sisl_get_operator_shape",
 ]
@@ -63,7 +63,8 @@ class SislGmresParams(C.Structure):
     _fields_ = [("tol", C.c_double), ("atol", C.c_double), ("restart", C.c_int32),
                 ("maxiter", C.c_int32), ("transpose", C.c_int32), ("early_exit", C.c_int32),
                 ("inner_rtol", C.c_double), ("max_outer", C.c_int32), ("basis_fp16", C.c_int32),
-                ("coef_bf16", C.c_int32), ("inner_single_cgs", C.c_int32)]
+                ("coef_bf16", C.c_int32), ("inner_single_cgs", C.c_int32),
+                ("inner_basis_operand", C.c_int32)]
 
 
 class SislGmresStats(C.Structure):
@@ -139,6 +140,7 @@ def lib():
             getattr(L, name).argtypes = [i32, i32, i32, vp, vp, vp, vp, vp, i32, vp]
         L.sisl_tricubic_mixed.argtypes = [i32, i32, i32, vp, vp, vp, i32, vp]
         L.sisl_helmholtz_apply_fused_f32.argtypes = [C.POINTER(SislOperator), vp, vp, vp, vp, d, i32, i32, vp]
+        L.sisl_helmholtz_apply_h16_f32.argtypes = [C.POINTER(SislOperator), vp, vp, vp, i32, i32, vp]
         L.sisl_probe_stream.argtypes = [vp, vp, i64, i32, vp]
         L.sisl_probe_operator_streams_f32.argtypes = [C.POINTER(SislOperator), vp, vp, vp, i32, vp]
         L.sisl_set_graphs.argtypes = [i32]

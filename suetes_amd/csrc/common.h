@@ -182,7 +182,9 @@ void timer_end(int id, hipStream_t s);
 // Fused basis normalisation of an operator launch (FP32 operator only): the
 // input is the unnormalised Arnoldi vector, the outputs are scaled by *inv
 // and the launch stores the basis column vs * x * (*inv) into vout
-// (nb = 1: FP16, nb = 2: FP32).  nb = 0: a plain apply.
+// (nb = 1: FP16, nb = 2: FP32).  nb = 4 (FP32 operator): the input pointer
+// holds FP16 values (a stored basis column), the outputs are scaled by
+// *inv and nothing else is stored.  nb = 0: a plain apply.
 struct Fuse {
   const double *inv = nullptr;
   void *vout = nullptr;

@@ -39,20 +39,33 @@ __device__ __forceinline__ int blocks_A_dev(const Stag &s) {
 // every output is stored as res - (A x), res the packed right-hand side.
 template <typename T, int NB> struct BasisT { using type = T; };
 template <typename T> struct BasisT<T, 1> { using type = _Float16; };
+// NB = 4 (FP32 only): the input is a basis column stored in FP16 (read and
+// converted to T), the outputs are scaled by *inv, and nothing else is
+// stored -- the operator launch of an Arnoldi step whose vector the previous
+// step's update pass already wrote to the basis.
+template <typename T, int NB> struct InT { using type = T; };
+template <typename T> struct InT<T, 4> { using type = _Float16; };
+template <typename T, typename TI>
+__device__ __forceinline__ T ldx(const TI *base, uint32_t idx) {
+  return (T)ldg(base, idx);
+}
 #define FUSED_NORMALISATION_SETUP                                                   \
   using BT = typename BasisT<T, NB>::type;                                          \
   constexpr bool kNorm = NB == 1 || NB == 2;                                        \
+  constexpr bool kScale = kNorm || NB == 4;                                         \
   T osc = 1, bsc = 1;                                                               \
   BT *BU = nullptr, *BV = nullptr, *BW = nullptr, *BP = nullptr, *BE = nullptr;     \
-  if constexpr (kNorm) {                                                            \
+  if constexpr (kScale) {                                                           \
     const double iv = *inv;                                                         \
     osc = (T)iv;                                                                    \
     bsc = (T)(iv * vs);                                                             \
+  }                                                                                 \
+  if constexpr (kNorm) {                                                            \
     BT *b_ = static_cast<BT *>(vout);                                               \
     BU = b_ + s.ou; BV = b_ + s.ov; BW = b_ + s.ow; BP = b_ + s.om; BE = b_ + s.oe; \
   }                                                                                 \
   auto so = [&](T v) -> T {                                                         \
-    if constexpr (kNorm) return v * osc;                                            \
+    if constexpr (kScale) return v * osc;                                           \
     else return v;                                                                  \
   };                                                                                \
   auto bput = [&](BT *b, uint32_t q, T v) {                                         \
@@ -75,7 +88,9 @@ __device__ __forceinline__ void helmholtz_body(const OpT<T> &o, const T *__restr
   if (i >= nx || j >= ny) return;
   const int k0 = bz * kchunk, k1 = min(k0 + kchunk, nz + 1);
   const int il = i == 0 ? nx - 1 : i - 1, jd = j == 0 ? ny - 1 : j - 1;
-  const T *U = x + s.ou, *V = x + s.ov, *W = x + s.ow, *P = x + s.om, *E = x + s.oe;
+  using TI = typename InT<T, NB>::type;  // input element type
+  const TI *xi = reinterpret_cast<const TI *>(x);
+  const TI *U = xi + s.ou, *V = xi + s.ov, *W = xi + s.ow, *P = xi + s.om, *E = xi + s.oe;
   T *YU = y + s.ou, *YV = y + s.ov, *YW = y + s.ow, *YP = y + s.om, *YE = y + s.oe;
   FUSED_NORMALISATION_SETUP
   // output store (NB = 3: the residual res - A x)
@@ -117,32 +132,32 @@ __device__ __forceinline__ void helmholtz_body(const OpT<T> &o, const T *__restr
   T umc_p = 0, vmc_p = 0;
   if (k0 < nz) {
     const uint32_t q = (uint32_t)k0 * pm;
-    pc0 = ldg(P, mc + q); pl0 = ldg(P, ml + q); pd0 = ldg(P, md + q);
+    pc0 = ldx<T>(P, mc + q); pl0 = ldx<T>(P, ml + q); pd0 = ldx<T>(P, md + q);
   }
   if (k0 >= 1) {
     const uint32_t q = (uint32_t)(k0 - 1) * pm;
-    pcm = ldg(P, mc + q); plm = ldg(P, ml + q); pdm = ldg(P, md + q);
+    pcm = ldx<T>(P, mc + q); plm = ldx<T>(P, ml + q); pdm = ldx<T>(P, md + q);
     const uint32_t qu = (uint32_t)(k0 - 1) * pu, qv = (uint32_t)(k0 - 1) * pv;
-    umc_p = half * (ldg(U, uc + qu) + ldg(U, uc + qu + 1));
-    vmc_p = half * (ldg(V, vc + qv) + ldg(V, vc + qv + nx));
+    umc_p = half * (ldx<T>(U, uc + qu) + ldx<T>(U, uc + qu + 1));
+    vmc_p = half * (ldx<T>(V, vc + qv) + ldx<T>(V, vc + qv + nx));
   }
   if (k0 == 0) {
-    gc0 = (ldg(P, mc + pm) - pc0) * ldc<T>(c_rdzw, mc);
-    gl0 = (ldg(P, ml + pm) - pl0) * ldc<T>(c_rdzw, ml);
-    gd0 = (ldg(P, md + pm) - pd0) * ldc<T>(c_rdzw, md);
+    gc0 = (ldx<T>(P, mc + pm) - pc0) * ldc<T>(c_rdzw, mc);
+    gl0 = (ldx<T>(P, ml + pm) - pl0) * ldc<T>(c_rdzw, ml);
+    gd0 = (ldx<T>(P, md + pm) - pd0) * ldc<T>(c_rdzw, md);
   } else if (k0 < nz) {
     const uint32_t q = (uint32_t)k0 * pm;
     gc0 = (pc0 - pcm) * ldc<T>(c_rdzw, mc + q);
     gl0 = (pl0 - plm) * ldc<T>(c_rdzw, ml + q);
     gd0 = (pd0 - pdm) * ldc<T>(c_rdzw, md + q);
   }
-  T e0 = ldg(E, mc + (uint32_t)k0 * pm);
+  T e0 = ldx<T>(E, mc + (uint32_t)k0 * pm);
   T fz0 = (k0 >= 1 && k0 < nz) ? e0 * ldc<T>(c_cz, mc + (uint32_t)k0 * pm) : (T)0;
 #pragma unroll UNR
   for (int k = k0; k < k1; ++k) {
     const uint32_t qm = (uint32_t)k * pm, qu = (uint32_t)k * pu, qv = (uint32_t)k * pv;
     const uint32_t wk = mc + qm;  // own column at w level k (same plane layout)
-    const T wv = ldg(W, wk);
+    const T wv = ldx<T>(W, wk);
     if (k == nz) {  // top w level: both rows are identities
       put(YW, wk, wv);
       put(YE, wk, e0 * ldc<T>(c_rdzw, wk));
@@ -154,7 +169,7 @@ __device__ __forceinline__ void helmholtz_body(const OpT<T> &o, const T *__restr
     T gc1, gl1, gd1, pc1 = 0, pl1 = 0, pd1 = 0;
     if (k + 1 < nz) {
       const uint32_t q1 = qm + pm;
-      pc1 = ldg(P, mc + q1); pl1 = ldg(P, ml + q1); pd1 = ldg(P, md + q1);
+      pc1 = ldx<T>(P, mc + q1); pl1 = ldx<T>(P, ml + q1); pd1 = ldx<T>(P, md + q1);
       gc1 = (pc1 - pc0) * ldc<T>(c_rdzw, mc + q1);
       gl1 = (pl1 - pl0) * ldc<T>(c_rdzw, ml + q1);
       gd1 = (pd1 - pd0) * ldc<T>(c_rdzw, md + q1);
@@ -166,14 +181,14 @@ __device__ __forceinline__ void helmholtz_body(const OpT<T> &o, const T *__restr
     }
     const T gzc = half * (gc0 + gc1), gzl = half * (gl0 + gl1), gzd = half * (gd0 + gd1);
     const uint32_t au = uc + qu, av = vc + qv;
-    const T u0 = ldg(U, au), u1 = ldg(U, au + 1), v0 = ldg(V, av), v1 = ldg(V, av + nx);
+    const T u0 = ldx<T>(U, au), u1 = ldx<T>(U, au + 1), v0 = ldx<T>(V, av), v1 = ldx<T>(V, av + nx);
     const T um_k = half * (u0 + u1), vm_k = half * (v0 + v1);
     // ---- L_u at face i (euler.py:398, 411-415) ----
     {
       T out = u0;
       if (uform) {
         const uint32_t al = vl + qv;
-        const T vml = half * (ldg(V, al) + ldg(V, al + nx));
+        const T vml = half * (ldx<T>(V, al) + ldx<T>(V, al + nx));
         out = u0 + ldc<T>(c_cua, au) * (mudx * (pc0 - pl0) - ldc<T>(c_zxu, au) * (half * (gzl + gzc))) -
               afu * (half * (vml + vm_k));
       }
@@ -185,7 +200,7 @@ __device__ __forceinline__ void helmholtz_body(const OpT<T> &o, const T *__restr
       }
       if (i == 0 && s.px) {                            // periodic: face nx repeats face 0
         put(YU, ux + qu, out);
-        if constexpr (kNorm) bput(BU, ux + qu, ldg(U, ux + qu));
+        if constexpr (kNorm) bput(BU, ux + qu, ldx<T>(U, ux + qu));
       }
     }
     // ---- L_v at face j (euler.py:399, 416-420) ----
@@ -193,7 +208,7 @@ __device__ __forceinline__ void helmholtz_body(const OpT<T> &o, const T *__restr
       T out = v0;
       if (vform) {
         const uint32_t ad = ud + qu;
-        const T umd = half * (ldg(U, ad) + ldg(U, ad + 1));
+        const T umd = half * (ldx<T>(U, ad) + ldx<T>(U, ad + 1));
         out = v0 + ldc<T>(c_cva, av) * (mvdy * (pc0 - pd0) - ldc<T>(c_zev, av) * (half * (gzd + gzc))) +
               afv * (half * (umd + um_k));
       }
@@ -205,12 +220,12 @@ __device__ __forceinline__ void helmholtz_body(const OpT<T> &o, const T *__restr
       }
       if (j == 0 && s.py) {
         put(YV, vy + qv, out);
-        if constexpr (kNorm) bput(BV, vy + qv, ldg(V, vy + qv));
+        if constexpr (kNorm) bput(BV, vy + qv, ldx<T>(V, vy + qv));
       }
     }
     // ---- L_pi (euler.py:298-318, 401) ----
     const uint32_t w1 = wk + pm;
-    const T e1 = ldg(E, w1);
+    const T e1 = ldx<T>(E, w1);
     const T fz1 = (k + 1 < nz) ? e1 * ldc<T>(c_cz, w1) : (T)0;
     {
       const T fx = u1 * ldc<T>(c_cx, au + 1) - u0 * ldc<T>(c_cx, au);
@@ -267,6 +282,18 @@ __device__ __forceinline__ void ldg2(const T *base, uint32_t idx, T &a, T &b) {
   a = v.x;
   b = v.y;
 }
+// a pair of input elements, converted to T (FP16 input: one 32-bit load)
+template <typename T, typename TI>
+__device__ __forceinline__ void ldx2(const TI *base, uint32_t idx, T &a, T &b) {
+  if constexpr (sizeof(TI) == 2) {
+    typedef _Float16 h2_t __attribute__((ext_vector_type(2)));
+    const h2_t v = *reinterpret_cast<const h2_t *>(reinterpret_cast<const char *>(base) + idx * 2u);
+    a = (T)v.x;
+    b = (T)v.y;
+  } else {
+    ldg2<T>(base, idx, a, b);
+  }
+}
 template <typename T>
 __device__ __forceinline__ void stg2(T *base, uint32_t idx, T a, T b) {
   using V = typename Vec2T<T>::type;
@@ -318,7 +345,9 @@ __device__ __forceinline__ void helmholtz2_body(const OpT<T> &o, const T *__rest
   if (i >= nx || j >= ny) return;
   const int k0 = bz * kchunk, k1 = min(k0 + kchunk, nz + 1);
   const int il = i == 0 ? nx - 1 : i - 1, jd = j == 0 ? ny - 1 : j - 1;
-  const T *U = x + s.ou, *V = x + s.ov, *W = x + s.ow, *P = x + s.om, *E = x + s.oe;
+  using TI = typename InT<T, NB>::type;  // input element type
+  const TI *xi = reinterpret_cast<const TI *>(x);
+  const TI *U = xi + s.ou, *V = xi + s.ov, *W = xi + s.ow, *P = xi + s.om, *E = xi + s.oe;
   T *YU = y + s.ou, *YV = y + s.ov, *YW = y + s.ow, *YP = y + s.om, *YE = y + s.oe;
   FUSED_NORMALISATION_SETUP
   auto put = [&](T *Y_, uint32_t q, T v) {
@@ -372,36 +401,36 @@ __device__ __forceinline__ void helmholtz2_body(const OpT<T> &o, const T *__rest
   T umc_p[2] = {0, 0}, vmc_p[2] = {0, 0};
   if (k0 < nz) {
     const uint32_t q = (uint32_t)k0 * pm;
-    ldg2(P, mc + q, pc0[0], pc0[1]);
-    pl0 = ldg(P, ml + q);
-    ldg2(P, md + q, pd0[0], pd0[1]);
+    ldx2<T>(P, mc + q, pc0[0], pc0[1]);
+    pl0 = ldx<T>(P, ml + q);
+    ldx2<T>(P, md + q, pd0[0], pd0[1]);
   }
   if (k0 >= 1) {
     const uint32_t q = (uint32_t)(k0 - 1) * pm;
-    ldg2(P, mc + q, pcm[0], pcm[1]);
-    plm = ldg(P, ml + q);
-    ldg2(P, md + q, pdm[0], pdm[1]);
+    ldx2<T>(P, mc + q, pcm[0], pcm[1]);
+    plm = ldx<T>(P, ml + q);
+    ldx2<T>(P, md + q, pdm[0], pdm[1]);
     const uint32_t qu = (uint32_t)(k0 - 1) * pu, qv = (uint32_t)(k0 - 1) * pv;
-    const T ua = ldg(U, uc + qu), ub = ldg(U, uc + qu + 1), ue = ldg(U, uc + qu + 2);
+    const T ua = ldx<T>(U, uc + qu), ub = ldx<T>(U, uc + qu + 1), ue = ldx<T>(U, uc + qu + 2);
     umc_p[0] = half * (ua + ub);
     umc_p[1] = half * (ub + ue);
     T va[2], vb[2];
-    ldg2(V, vc + qv, va[0], va[1]);
-    ldg2(V, vc + qv + nx, vb[0], vb[1]);
+    ldx2<T>(V, vc + qv, va[0], va[1]);
+    ldx2<T>(V, vc + qv + nx, vb[0], vb[1]);
     vmc_p[0] = half * (va[0] + vb[0]);
     vmc_p[1] = half * (va[1] + vb[1]);
   }
   if (k0 == 0) {
     T p1[2], d1[2], r[2], rd[2];
-    ldg2(P, mc + pm, p1[0], p1[1]);
-    ldg2(P, md + pm, d1[0], d1[1]);
+    ldx2<T>(P, mc + pm, p1[0], p1[1]);
+    ldx2<T>(P, md + pm, d1[0], d1[1]);
     ldc2<T>(c_rdzw, mc, r[0], r[1]);
     ldc2<T>(c_rdzw, md, rd[0], rd[1]);
     for (int c = 0; c < 2; ++c) {
       gc0[c] = (p1[c] - pc0[c]) * r[c];
       gd0[c] = (d1[c] - pd0[c]) * rd[c];
     }
-    gl0 = (ldg(P, ml + pm) - pl0) * ldc<T>(c_rdzw, ml);
+    gl0 = (ldx<T>(P, ml + pm) - pl0) * ldc<T>(c_rdzw, ml);
   } else if (k0 < nz) {
     const uint32_t q = (uint32_t)k0 * pm;
     T r[2], rd[2];
@@ -414,7 +443,7 @@ __device__ __forceinline__ void helmholtz2_body(const OpT<T> &o, const T *__rest
     gl0 = (pl0 - plm) * ldc<T>(c_rdzw, ml + q);
   }
   T e0[2], fz0[2] = {0, 0};
-  ldg2(E, mc + (uint32_t)k0 * pm, e0[0], e0[1]);
+  ldx2<T>(E, mc + (uint32_t)k0 * pm, e0[0], e0[1]);
   if (k0 >= 1 && k0 < nz) {
     T cz[2];
     ldc2<T>(c_cz, mc + (uint32_t)k0 * pm, cz[0], cz[1]);
@@ -425,7 +454,7 @@ __device__ __forceinline__ void helmholtz2_body(const OpT<T> &o, const T *__rest
     const uint32_t qm = (uint32_t)k * pm, qu = (uint32_t)k * pu, qv = (uint32_t)k * pv;
     const uint32_t wk = mc + qm;
     T wv[2];
-    ldg2(W, wk, wv[0], wv[1]);
+    ldx2<T>(W, wk, wv[0], wv[1]);
     if (k == nz) {
       T r[2];
       ldc2<T>(c_rdzw, wk, r[0], r[1]);
@@ -444,9 +473,9 @@ __device__ __forceinline__ void helmholtz2_body(const OpT<T> &o, const T *__rest
       ldc2<T>(c_rdzw, md + q1, rd[0], rd[1]);
       const T rl = ldc<T>(c_rdzw, ml + q1);
       if (k + 1 < nz) {
-        ldg2(P, mc + q1, pc1[0], pc1[1]);
-        pl1 = ldg(P, ml + q1);
-        ldg2(P, md + q1, pd1[0], pd1[1]);
+        ldx2<T>(P, mc + q1, pc1[0], pc1[1]);
+        pl1 = ldx<T>(P, ml + q1);
+        ldx2<T>(P, md + q1, pd1[0], pd1[1]);
         for (int c = 0; c < 2; ++c) {
           gc1[c] = (pc1[c] - pc0[c]) * r[c];
           gd1[c] = (pd1[c] - pd0[c]) * rd[c];
@@ -464,10 +493,10 @@ __device__ __forceinline__ void helmholtz2_body(const OpT<T> &o, const T *__rest
     const T gzl = half * (gl0 + gl1);
     const T gzd0 = half * (gd0[0] + gd1[0]), gzd1 = half * (gd0[1] + gd1[1]);
     const uint32_t au = uc + qu, av = vc + qv;
-    const T ua = ldg(U, au), ub = ldg(U, au + 1), ue = ldg(U, au + 2);  // faces i, i+1, i+2
+    const T ua = ldx<T>(U, au), ub = ldx<T>(U, au + 1), ue = ldx<T>(U, au + 2);  // faces i, i+1, i+2
     T v0[2], v1[2];
-    ldg2(V, av, v0[0], v0[1]);
-    ldg2(V, av + nx, v1[0], v1[1]);
+    ldx2<T>(V, av, v0[0], v0[1]);
+    ldx2<T>(V, av + nx, v1[0], v1[1]);
     const T um0 = half * (ua + ub), um1 = half * (ub + ue);
     const T vm0 = half * (v0[0] + v1[0]), vm1 = half * (v0[1] + v1[1]);
     // ---- L_u at faces i and i+1 ----
@@ -475,7 +504,7 @@ __device__ __forceinline__ void helmholtz2_body(const OpT<T> &o, const T *__rest
       T out0 = ua;
       if (uform0) {
         const uint32_t al = vl + qv;
-        const T vml = half * (ldg(V, al) + ldg(V, al + nx));
+        const T vml = half * (ldx<T>(V, al) + ldx<T>(V, al + nx));
         out0 = ua + ldc<T>(c_cua, au) * (mudx0 * (pc0[0] - pl0) - ldc<T>(c_zxu, au) * (half * (gzl + gzc0))) -
                afu0 * (half * (vml + vm0));
       }
@@ -494,7 +523,7 @@ __device__ __forceinline__ void helmholtz2_body(const OpT<T> &o, const T *__rest
       }
       if (i == 0 && s.px) {  // periodic: face nx repeats face 0
         put(YU, ux + qu, out0);
-        if constexpr (kNorm) stg(BU, ux + qu, (BT)(ldg(U, ux + qu) * bsc));
+        if constexpr (kNorm) stg(BU, ux + qu, (BT)(ldx<T>(U, ux + qu) * bsc));
       }
     }
     // ---- L_v at face j of both columns ----
@@ -502,7 +531,7 @@ __device__ __forceinline__ void helmholtz2_body(const OpT<T> &o, const T *__rest
       T out[2] = {v0[0], v0[1]};
       if (vform) {
         const uint32_t ad = ud + qu;
-        const T da = ldg(U, ad), db = ldg(U, ad + 1), de = ldg(U, ad + 2);
+        const T da = ldx<T>(U, ad), db = ldx<T>(U, ad + 1), de = ldx<T>(U, ad + 2);
         const T umd[2] = {half * (da + db), half * (db + de)};
         const T umk[2] = {um0, um1};
         const T gzc[2] = {gzc0, gzc1}, gzd[2] = {gzd0, gzd1};
@@ -523,7 +552,7 @@ __device__ __forceinline__ void helmholtz2_body(const OpT<T> &o, const T *__rest
         put2(YV, vy + qv, out[0], out[1]);
         if constexpr (kNorm) {
           T na, nb_;
-          ldg2(V, vy + qv, na, nb_);
+          ldx2<T>(V, vy + qv, na, nb_);
           bput2(BV, vy + qv, na, nb_);
         }
       }
@@ -531,7 +560,7 @@ __device__ __forceinline__ void helmholtz2_body(const OpT<T> &o, const T *__rest
     // ---- L_pi ----
     const uint32_t w1 = wk + pm;
     T e1[2], fz1[2] = {0, 0};
-    ldg2(E, w1, e1[0], e1[1]);
+    ldx2<T>(E, w1, e1[0], e1[1]);
     if (k + 1 < nz) {
       T cz[2];
       ldc2<T>(c_cz, w1, cz[0], cz[1]);
@@ -637,7 +666,9 @@ __device__ __forceinline__ void helmholtz_T_body(const OpT<T> &o, const T *__res
   const int i = bx * kTX + threadIdx.x, j = by * kTY + threadIdx.y;
   if (i >= nx || j >= ny) return;
   const int k0 = bz * kchunk, k1 = min(k0 + kchunk, nz + 1);
-  const T *YU = y + s.ou, *YV = y + s.ov, *YW = y + s.ow, *YP = y + s.om, *YE = y + s.oe;
+  using TI = typename InT<T, NB>::type;  // input element type
+  const TI *yi = reinterpret_cast<const TI *>(y);
+  const TI *YU = yi + s.ou, *YV = yi + s.ov, *YW = yi + s.ow, *YP = yi + s.om, *YE = yi + s.oe;
   T *XU = x + s.ou, *XV = x + s.ov, *XW = x + s.ow, *XP = x + s.om, *XE = x + s.oe;
   FUSED_NORMALISATION_SETUP
   // output store (NB = 3: the residual res - A^T y)
@@ -684,12 +715,12 @@ __device__ __forceinline__ void helmholtz_T_body(const OpT<T> &o, const T *__res
   // Formula cotangent of u face f of row `row` (level offset q): faces 0 / nx are
   // identities unless periodic, where both carry the merged cotangent.
   auto ut = [&](int f, uint32_t row, uint32_t q) -> T {
-    if (f == 0 || f == nx) return px ? ldg(YU, row + q) + ldg(YU, row + nx + q) : (T)0;
-    return ldg(YU, row + f + q);
+    if (f == 0 || f == nx) return px ? ldx<T>(YU, row + q) + ldx<T>(YU, row + nx + q) : (T)0;
+    return ldx<T>(YU, row + f + q);
   };
   auto vt = [&](int ii, int f, uint32_t q) -> T {
-    if (f == 0 || f == ny) return py ? ldg(YV, (uint32_t)ii + q) + ldg(YV, (uint32_t)ny * nx + ii + q) : (T)0;
-    return ldg(YV, (uint32_t)f * nx + ii + q);
+    if (f == 0 || f == ny) return py ? ldx<T>(YV, (uint32_t)ii + q) + ldx<T>(YV, (uint32_t)ny * nx + ii + q) : (T)0;
+    return ldx<T>(YV, (uint32_t)f * nx + ii + q);
   };
   // gzm_b of the own cell at mass level k from the four face cotangents
   auto gzmb = [&](T uti, T utr, T vtj, T vtr, int k) -> T {
@@ -717,14 +748,14 @@ __device__ __forceinline__ void helmholtz_T_body(const OpT<T> &o, const T *__res
     const uint32_t qu = (uint32_t)km * pu, qv = (uint32_t)km * pv;
     gm1 = gzmb(ut(i, u0r, qu), ut(i + 1, u0r, qu), vt(i, j, qv), vt(i, j + 1, qv), km);
     const uint32_t m = mc + (uint32_t)km * pm;
-    cpmP_m1 = ldc<T>(c_cpm, m) * ldg(YP, m);
+    cpmP_m1 = ldc<T>(c_cpm, m) * ldx<T>(YP, m);
   }
   if (k0 >= 1 && k0 <= nz - 1) {
     const uint32_t w = mc + (uint32_t)k0 * pm;
-    e0 = ldg(YE, w);
-    if (hl) e0l = ldg(YE, w - 1);
-    if (hd) e0d = ldg(YE, w - nx);
-    cwW0 = ldc<T>(c_cw, w) * ldg(YW, w);
+    e0 = ldx<T>(YE, w);
+    if (hl) e0l = ldx<T>(YE, w - 1);
+    if (hd) e0d = ldx<T>(YE, w - nx);
+    cwW0 = ldc<T>(c_cw, w) * ldx<T>(YW, w);
   }
   if (k0 <= nz) {
     const uint32_t w = mc + (uint32_t)k0 * pm;
@@ -737,7 +768,7 @@ __device__ __forceinline__ void helmholtz_T_body(const OpT<T> &o, const T *__res
     const uint32_t qm = (uint32_t)k * pm, qu = (uint32_t)k * pu, qv = (uint32_t)k * pv;
     const uint32_t wk = mc + qm;  // own column at w level k and at mass level k
     if (k == nz) {  // top w level: identity rows
-      const T yw = ldg(YW, wk), ye = ldg(YE, wk);
+      const T yw = ldx<T>(YW, wk), ye = ldx<T>(YE, wk);
       put(XW, wk, yw);
       put(XE, wk, ye * ldc<T>(c_rdzw, wk));
       bput(BW, wk, yw);
@@ -753,10 +784,10 @@ __device__ __forceinline__ void helmholtz_T_body(const OpT<T> &o, const T *__res
       ut_i1 = ut(i, u0r, qu1); ut_r1 = ut(i + 1, u0r, qu1);
       vt_j1 = vt(i, j, qv1); vt_r1 = vt(i, j + 1, qv1);
       g1 = gzmb(ut_i1, ut_r1, vt_j1, vt_r1, k + 1);
-      e1 = ldg(YE, w1);
-      if (hl) e1l = ldg(YE, w1 - 1);
-      if (hd) e1d = ldg(YE, w1 - nx);
-      cwW1 = ldc<T>(c_cw, w1) * ldg(YW, w1);
+      e1 = ldx<T>(YE, w1);
+      if (hl) e1l = ldx<T>(YE, w1 - 1);
+      if (hd) e1d = ldx<T>(YE, w1 - nx);
+      cwW1 = ldc<T>(c_cw, w1) * ldx<T>(YW, w1);
     }
     const T kx1 = ldc<T>(c_kx, w1), ky1 = ldc<T>(c_ky, w1);
     const T kx1l = hl ? ldc<T>(c_kx, w1 - 1) : (T)0, ky1d = hd ? ldc<T>(c_ky, w1 - nx) : (T)0;
@@ -767,17 +798,17 @@ __device__ __forceinline__ void helmholtz_T_body(const OpT<T> &o, const T *__res
     if (hl) umb_l = half * (vt(i - 1, j, qv) * afv_lj + vt(i - 1, j + 1, qv) * afv_lr) + ha * (e0l * kx0l + e1l * kx1l);
     if (hd) vmb_d = mhalf * (ut(i, u0r - (nx + 1), qu) * afu_dj + ut(i + 1, u0r - (nx + 1), qu) * afu_dr) +
                     ha * (e0d * ky0d + e1d * ky1d);
-    const T yp = ldg(YP, wk);
+    const T yp = ldx<T>(YP, wk);
     const T cpmP = ldc<T>(c_cpm, wk) * yp;
-    const T cpmPl = hl ? ldc<T>(c_cpm, wk - 1) * ldg(YP, wk - 1) : (T)0;
-    const T cpmPd = hd ? ldc<T>(c_cpm, wk - nx) * ldg(YP, wk - nx) : (T)0;
+    const T cpmPl = hl ? ldc<T>(c_cpm, wk - 1) * ldx<T>(YP, wk - 1) : (T)0;
+    const T cpmPd = hd ? ldc<T>(c_cpm, wk - nx) * ldx<T>(YP, wk - nx) : (T)0;
     // ---- u face i (and the east face) ----
     {
       const uint32_t au = uc + qu;
-      const T yu = ldg(YU, au);
+      const T yu = ldx<T>(YU, au);
       bput(BU, au, yu);
       T idp = yu;
-      if (px && i == 0) idp = yu + ldg(YU, au + nx);
+      if (px && i == 0) idp = yu + ldx<T>(YU, au + nx);
       T fl = 0;
       if (hl) fl += cpmPl * mmx_l;
       fl -= cpmP * mmx_c;
@@ -787,7 +818,7 @@ __device__ __forceinline__ void helmholtz_T_body(const OpT<T> &o, const T *__res
       ub += umb;
       put(XU, au, out + half * ub);
       if (i == nx - 1) {
-        const T yn = ldg(YU, au + 1);
+        const T yn = ldx<T>(YU, au + 1);
         bput(BU, au + 1, yn);
         const T idn = px ? (T)0 : yn;
         put(XU, au + 1, idn + ldc<T>(c_cx, au + 1) * (cpmP * mmx_c) + half * umb);
@@ -796,10 +827,10 @@ __device__ __forceinline__ void helmholtz_T_body(const OpT<T> &o, const T *__res
     // ---- v face j (and the north face) ----
     {
       const uint32_t av = vc + qv;
-      const T yv = ldg(YV, av);
+      const T yv = ldx<T>(YV, av);
       bput(BV, av, yv);
       T idp = yv;
-      if (py && j == 0) idp = yv + ldg(YV, av + (uint32_t)ny * nx);
+      if (py && j == 0) idp = yv + ldx<T>(YV, av + (uint32_t)ny * nx);
       T fl = 0;
       if (hd) fl += cpmPd * mmy_d;
       fl -= cpmP * mmy_c;
@@ -809,7 +840,7 @@ __device__ __forceinline__ void helmholtz_T_body(const OpT<T> &o, const T *__res
       vb += vmb;
       put(XV, av, out + half * vb);
       if (j == ny - 1) {
-        const T yn = ldg(YV, av + nx);
+        const T yn = ldx<T>(YV, av + nx);
         bput(BV, av + nx, yn);
         const T idn = py ? (T)0 : yn;
         put(XV, av + nx, idn + ldc<T>(c_cy, av + nx) * (cpmP * mmy_c) + half * vmb);
@@ -837,10 +868,10 @@ __device__ __forceinline__ void helmholtz_T_body(const OpT<T> &o, const T *__res
     }
     // ---- w level k: w and eta_dot ----
     {
-      const T yw = ldg(YW, wk);
+      const T yw = ldx<T>(YW, wk);
       bput(BW, wk, yw);
       if (k == 0) {
-        const T ye = ldg(YE, wk);
+        const T ye = ldx<T>(YE, wk);
         bput(BE, wk, ye);
         put(XW, wk, yw);
         put(XE, wk, ye * ldc<T>(c_rdzw, wk));
@@ -889,7 +920,9 @@ __device__ __forceinline__ void helmholtz_T2_body(const OpT<T> &o, const T *__re
   const int i = bx * kTX2 + 2 * threadIdx.x, j = by * kTY + threadIdx.y;
   if (i >= nx || j >= ny) return;
   const int k0 = bz * kchunk, k1 = min(k0 + kchunk, nz + 1);
-  const T *YU = y + s.ou, *YV = y + s.ov, *YW = y + s.ow, *YP = y + s.om, *YE = y + s.oe;
+  using TI = typename InT<T, NB>::type;  // input element type
+  const TI *yi = reinterpret_cast<const TI *>(y);
+  const TI *YU = yi + s.ou, *YV = yi + s.ov, *YW = yi + s.ow, *YP = yi + s.om, *YE = yi + s.oe;
   T *XU = x + s.ou, *XV = x + s.ov, *XW = x + s.ow, *XP = x + s.om, *XE = x + s.oe;
   FUSED_NORMALISATION_SETUP
   auto put = [&](T *X_, uint32_t q, T v) {
@@ -955,20 +988,20 @@ __device__ __forceinline__ void helmholtz_T2_body(const OpT<T> &o, const T *__re
   const T mmx_l[2] = {mmx_l0, mmx_c[0]};
   const int frs[2] = {fr0, fr1};
   auto ut = [&](int f, uint32_t row, uint32_t q) -> T {
-    if (f == 0 || f == nx) return px ? ldg(YU, row + q) + ldg(YU, row + nx + q) : (T)0;
-    return ldg(YU, row + f + q);
+    if (f == 0 || f == nx) return px ? ldx<T>(YU, row + q) + ldx<T>(YU, row + nx + q) : (T)0;
+    return ldx<T>(YU, row + f + q);
   };
   auto vt = [&](int ii, int f, uint32_t q) -> T {
-    if (f == 0 || f == ny) return py ? ldg(YV, (uint32_t)ii + q) + ldg(YV, (uint32_t)ny * nx + ii + q) : (T)0;
-    return ldg(YV, (uint32_t)f * nx + ii + q);
+    if (f == 0 || f == ny) return py ? ldx<T>(YV, (uint32_t)ii + q) + ldx<T>(YV, (uint32_t)ny * nx + ii + q) : (T)0;
+    return ldx<T>(YV, (uint32_t)f * nx + ii + q);
   };
   // the pair (ii = i, i+1) of v-face cotangents of face row f
   auto vt2 = [&](int f, uint32_t q, T &a, T &b) {
     if (f == 0 || f == ny) {
       if (py) {
         T a0, b0, a1, b1;
-        ldg2(YV, (uint32_t)i + q, a0, b0);
-        ldg2(YV, (uint32_t)ny * nx + i + q, a1, b1);
+        ldx2<T>(YV, (uint32_t)i + q, a0, b0);
+        ldx2<T>(YV, (uint32_t)ny * nx + i + q, a1, b1);
         a = a0 + a1;
         b = b0 + b1;
       } else {
@@ -977,7 +1010,7 @@ __device__ __forceinline__ void helmholtz_T2_body(const OpT<T> &o, const T *__re
       }
       return;
     }
-    ldg2(YV, (uint32_t)f * nx + i + q, a, b);
+    ldx2<T>(YV, (uint32_t)f * nx + i + q, a, b);
   };
   auto gzmb = [&](int c, T uti, T utr, T vtj, T vtr, int k) -> T {
     const uint32_t qu = (uint32_t)k * pu, qv = (uint32_t)k * pv;
@@ -1009,18 +1042,18 @@ __device__ __forceinline__ void helmholtz_T2_body(const OpT<T> &o, const T *__re
     const uint32_t m = mc + (uint32_t)km * pm;
     T cp[2], yp[2];
     ldc2<T>(c_cpm, m, cp[0], cp[1]);
-    ldg2(YP, m, yp[0], yp[1]);
+    ldx2<T>(YP, m, yp[0], yp[1]);
     cpmP_m1[0] = cp[0] * yp[0];
     cpmP_m1[1] = cp[1] * yp[1];
   }
   if (k0 >= 1 && k0 <= nz - 1) {
     const uint32_t w = mc + (uint32_t)k0 * pm;
-    ldg2(YE, w, e0[0], e0[1]);
-    if (hl0) e0l0 = ldg(YE, w - 1);
-    if (hd) ldg2(YE, w - nx, e0d[0], e0d[1]);
+    ldx2<T>(YE, w, e0[0], e0[1]);
+    if (hl0) e0l0 = ldx<T>(YE, w - 1);
+    if (hd) ldx2<T>(YE, w - nx, e0d[0], e0d[1]);
     T cw[2], yw[2];
     ldc2<T>(c_cw, w, cw[0], cw[1]);
-    ldg2(YW, w, yw[0], yw[1]);
+    ldx2<T>(YW, w, yw[0], yw[1]);
     cwW0[0] = cw[0] * yw[0];
     cwW0[1] = cw[1] * yw[1];
   }
@@ -1036,8 +1069,8 @@ __device__ __forceinline__ void helmholtz_T2_body(const OpT<T> &o, const T *__re
     const uint32_t wk = mc + qm;
     if (k == nz) {  // top w level: identity rows
       T yw[2], ye[2], r[2];
-      ldg2(YW, wk, yw[0], yw[1]);
-      ldg2(YE, wk, ye[0], ye[1]);
+      ldx2<T>(YW, wk, yw[0], yw[1]);
+      ldx2<T>(YE, wk, ye[0], ye[1]);
       ldc2<T>(c_rdzw, wk, r[0], r[1]);
       put2(XW, wk, yw[0], yw[1]);
       put2(XE, wk, ye[0] * r[0], ye[1] * r[1]);
@@ -1052,12 +1085,12 @@ __device__ __forceinline__ void helmholtz_T2_body(const OpT<T> &o, const T *__re
     if (k + 1 < nz) {
       load_faces(qu + pu, qv + pv, ut_i1, ut_r1, vt_j1, vt_r1);
       for (int c = 0; c < 2; ++c) g1[c] = gzmb(c, ut_i1[c], ut_r1[c], vt_j1[c], vt_r1[c], k + 1);
-      ldg2(YE, w1, e1[0], e1[1]);
-      if (hl0) e1l0 = ldg(YE, w1 - 1);
-      if (hd) ldg2(YE, w1 - nx, e1d[0], e1d[1]);
+      ldx2<T>(YE, w1, e1[0], e1[1]);
+      if (hl0) e1l0 = ldx<T>(YE, w1 - 1);
+      if (hd) ldx2<T>(YE, w1 - nx, e1d[0], e1d[1]);
       T cw[2], yw[2];
       ldc2<T>(c_cw, w1, cw[0], cw[1]);
-      ldg2(YW, w1, yw[0], yw[1]);
+      ldx2<T>(YW, w1, yw[0], yw[1]);
       cwW1[0] = cw[0] * yw[0];
       cwW1[1] = cw[1] * yw[1];
     }
@@ -1082,15 +1115,15 @@ __device__ __forceinline__ void helmholtz_T2_body(const OpT<T> &o, const T *__re
     }
     const T umb_l[2] = {umb_l0, umb[0]};
     T yp[2], cpm[2], cpmP[2], cpmPd[2] = {0, 0};
-    ldg2(YP, wk, yp[0], yp[1]);
+    ldx2<T>(YP, wk, yp[0], yp[1]);
     ldc2<T>(c_cpm, wk, cpm[0], cpm[1]);
     cpmP[0] = cpm[0] * yp[0];
     cpmP[1] = cpm[1] * yp[1];
-    const T cpmPl0 = hl0 ? ldc<T>(c_cpm, wk - 1) * ldg(YP, wk - 1) : (T)0;
+    const T cpmPl0 = hl0 ? ldc<T>(c_cpm, wk - 1) * ldx<T>(YP, wk - 1) : (T)0;
     if (hd) {
       T cd[2], pd[2];
       ldc2<T>(c_cpm, wk - nx, cd[0], cd[1]);
-      ldg2(YP, wk - nx, pd[0], pd[1]);
+      ldx2<T>(YP, wk - nx, pd[0], pd[1]);
       cpmPd[0] = cd[0] * pd[0];
       cpmPd[1] = cd[1] * pd[1];
     }
@@ -1099,10 +1132,10 @@ __device__ __forceinline__ void helmholtz_T2_body(const OpT<T> &o, const T *__re
     for (int c = 0; c < 2; ++c) {
       const bool hl = c == 1 || hl0;
       const uint32_t au = uc + c + qu;
-      const T yu = ldg(YU, au);
+      const T yu = ldx<T>(YU, au);
       bput(BU, au, yu);
       T idp = yu;
-      if (px && i + c == 0) idp = yu + ldg(YU, au + nx);
+      if (px && i + c == 0) idp = yu + ldx<T>(YU, au + nx);
       T fl = 0;
       if (hl) fl += cpmPl[c] * mmx_l[c];
       fl -= cpmP[c] * mmx_c[c];
@@ -1112,7 +1145,7 @@ __device__ __forceinline__ void helmholtz_T2_body(const OpT<T> &o, const T *__re
       ub += umb[c];
       put(XU, au, out + half * ub);
       if (c == 1 && i + 1 == nx - 1) {
-        const T yn = ldg(YU, au + 1);
+        const T yn = ldx<T>(YU, au + 1);
         bput(BU, au + 1, yn);
         const T idn = px ? (T)0 : yn;
         put(XU, au + 1, idn + ldc<T>(c_cx, au + 1) * (cpmP[1] * mmx_c[1]) + half * umb[1]);
@@ -1122,12 +1155,12 @@ __device__ __forceinline__ void helmholtz_T2_body(const OpT<T> &o, const T *__re
     {
       const uint32_t av = vc + qv;
       T yv[2];
-      ldg2(YV, av, yv[0], yv[1]);
+      ldx2<T>(YV, av, yv[0], yv[1]);
       bput(BV, av, yv[0]); bput(BV, av + 1, yv[1]);
       T idp[2] = {yv[0], yv[1]};
       if (py && j == 0) {
         T pa, pb;
-        ldg2(YV, av + (uint32_t)ny * nx, pa, pb);
+        ldx2<T>(YV, av + (uint32_t)ny * nx, pa, pb);
         idp[0] = yv[0] + pa;
         idp[1] = yv[1] + pb;
       }
@@ -1147,7 +1180,7 @@ __device__ __forceinline__ void helmholtz_T2_body(const OpT<T> &o, const T *__re
       put2(XV, av, o2[0], o2[1]);
       if (j == ny - 1) {
         T yn[2], cyn[2];
-        ldg2(YV, av + nx, yn[0], yn[1]);
+        ldx2<T>(YV, av + nx, yn[0], yn[1]);
         bput(BV, av + nx, yn[0]); bput(BV, av + nx + 1, yn[1]);
         ldc2<T>(c_cy, av + nx, cyn[0], cyn[1]);
         T on[2];
@@ -1189,11 +1222,11 @@ __device__ __forceinline__ void helmholtz_T2_body(const OpT<T> &o, const T *__re
     // ---- w level k: w and eta_dot ----
     {
       T yw[2];
-      ldg2(YW, wk, yw[0], yw[1]);
+      ldx2<T>(YW, wk, yw[0], yw[1]);
       bput(BW, wk, yw[0]); bput(BW, wk + 1, yw[1]);
       if (k == 0) {
         T ye[2], r[2];
-        ldg2(YE, wk, ye[0], ye[1]);
+        ldx2<T>(YE, wk, ye[0], ye[1]);
         ldc2<T>(c_rdzw, wk, r[0], r[1]);
         bput(BE, wk, ye[0]); bput(BE, wk + 1, ye[1]);
         put2(XW, wk, yw[0], yw[1]);
@@ -1252,22 +1285,31 @@ uint64_t shape_generation() {
 }
 
 // variant = transpose + 2 * (fused basis normalisation) + 4 * (bfloat16
-// coefficients); a bfloat16 variant not tuned on its own uses the FP32
-// coefficients' shape.
+// coefficients) + 8 * (FP16 basis column as the input, in place of the fused
+// normalisation); a variant not tuned on its own uses the fused-normalisation
+// shape (FP16 input), then the FP32 coefficients' shape (bfloat16).
 static AShape a_shape(const Stag &s, int bytes, int variant = 0) {
   std::lock_guard<std::mutex> g(g_tune_mu);
-  auto it = g_tune.find({s.nx, s.ny, s.nz, bytes, variant});
-  if (it != g_tune.end()) return it->second;
-  if (variant >= 4) {
-    it = g_tune.find({s.nx, s.ny, s.nz, bytes, variant - 4});
-    if (it != g_tune.end()) return it->second;
+  auto find = [&](int v, AShape &out) {
+    auto it = g_tune.find({s.nx, s.ny, s.nz, bytes, v});
+    if (it == g_tune.end()) return false;
+    out = it->second;
+    return true;
+  };
+  AShape sh{(s.nz + 1 + 7) / 8, 1};
+  if (find(variant, sh)) return sh;
+  if (variant & 8) {
+    variant = (variant & 7) | 2;
+    if (find(variant, sh)) return sh;
   }
-  return AShape{(s.nz + 1 + 7) / 8, 1};
+  if (variant & 4) find(variant - 4, sh);
+  return sh;
 }
 
 // Launch of kernel K<T, UNR, NB> (operator or transpose) in a given shape;
-// NB = 1, 2 (fused basis normalisation) exist for the FP32 operator only, NB =
-// 3 (residual epilogue) for the FP64 one.
+// NB = 1, 2 (fused basis normalisation) and 4 (FP16 basis column as the
+// input) exist for the FP32 operator only, NB = 3 (residual epilogue) for the
+// FP64 one.
 template <typename T, int UNR, int NB, typename TC>
 struct AKern {
   static void go(dim3 g, hipStream_t st, const OpT<T> &o, const T *x, T *y, int kc, const int *stop,
@@ -1310,6 +1352,11 @@ static int launch_shape_tc(dim3 g, const OpT<T> &o, const T *x, T *y, hipStream_
     if (f.nb == 2) {
       if (unr == 2) K<T, 2, 2, TC>::go(g, st, o, x, y, kc, stop, f);
       else K<T, 1, 2, TC>::go(g, st, o, x, y, kc, stop, f);
+      return SISL_OK;
+    }
+    if (f.nb == 4) {
+      if (unr == 2) K<T, 2, 4, TC>::go(g, st, o, x, y, kc, stop, f);
+      else K<T, 1, 4, TC>::go(g, st, o, x, y, kc, stop, f);
       return SISL_OK;
     }
   }
@@ -1358,13 +1405,26 @@ static int launch2_tc(dim3 g, const OpT<T> &o, const T *x, T *y, hipStream_t st,
       go2<T, 2, TC>(g, st, o, x, y, kc, stop, f);
       return SISL_OK;
     }
+    if (f.nb == 4) {
+      go2<T, 4, TC>(g, st, o, x, y, kc, stop, f);
+      return SISL_OK;
+    }
   }
   return SISL_ERR_ARG;
+}
+
+// The scaled variants read *inv on the device, the fused ones also store
+// into vout: refused on the host when those pointers are missing.
+static bool fuse_args_ok(const Fuse &f) {
+  if ((f.nb == 1 || f.nb == 2 || f.nb == 4) && !f.inv) return false;
+  if ((f.nb == 1 || f.nb == 2) && !f.vout) return false;
+  return f.nb == 0 || f.nb == 1 || f.nb == 2 || f.nb == 4;
 }
 
 template <typename T>
 static int launch_A_shape(const OpT<T> &o, const T *x, T *y, hipStream_t st, const int *stop, AShape sh,
                           const Fuse &f = Fuse{}) {
+  if (!fuse_args_ok(f)) return SISL_ERR_ARG;
   if (sh.unr == 3) {
     if (two_column_ok(o.s)) {
       const int kc = sh.kc;
@@ -1385,10 +1445,11 @@ template <typename T>
 int launch_A(const OpT<T> &o, const T *x, T *y, hipStream_t st, const int *stop, const Fuse *fuse) {
   if ((uint64_t)o.s.n * sizeof(T) >= (1ull << 32)) return SISL_ERR_ARG;  // 32-bit offsets
   const Fuse f = fuse ? *fuse : Fuse{};
-  const AShape sh = a_shape(o.s, (int)sizeof(T), (f.nb ? 2 : 0) + (f.cb ? 4 : 0));
+  const AShape sh = a_shape(o.s, (int)sizeof(T), (f.nb == 4 ? 8 : f.nb ? 2 : 0) + (f.cb ? 4 : 0));
   const Stag &s = o.s;
   const double sc = f.cb ? 2.0 : (double)sizeof(T);  // bytes per coefficient element
-  const double bytes = (double)sizeof(T) * 2.0 * s.n + sc * (3.0 * s.u.n() + 3.0 * s.v.n() + 6.0 * s.w.n() + s.m.n()) +
+  const double si = f.nb == 4 ? 2.0 : (double)sizeof(T);  // bytes per input element
+  const double bytes = ((double)sizeof(T) + si) * s.n + sc * (3.0 * s.u.n() + 3.0 * s.v.n() + 6.0 * s.w.n() + s.m.n()) +
                        (f.nb == 1 ? 2.0 : f.nb == 2 ? (double)sizeof(T) : 0.0) * s.n +
                        (f.res ? (double)sizeof(T) * s.n : 0.0);
   const int tid = timer_begin(SISL_T_HELMHOLTZ, sizeof(T) == 4, bytes, stop, nullptr, st);
@@ -1429,6 +1490,10 @@ static int launchT2_tc(dim3 g, const OpT<T> &o, const T *y, T *x, hipStream_t st
       goT2<T, 2, TC>(g, st, o, y, x, kc, stop, f);
       return SISL_OK;
     }
+    if (f.nb == 4) {
+      goT2<T, 4, TC>(g, st, o, y, x, kc, stop, f);
+      return SISL_OK;
+    }
   }
   return SISL_ERR_ARG;
 }
@@ -1436,6 +1501,7 @@ static int launchT2_tc(dim3 g, const OpT<T> &o, const T *y, T *x, hipStream_t st
 template <typename T>
 static int launch_AT_shape(const OpT<T> &o, const T *y, T *x, hipStream_t st, const int *stop, AShape sh,
                            const Fuse &f = Fuse{}) {
+  if (!fuse_args_ok(f)) return SISL_ERR_ARG;
   if (sh.unr == 3) {
     if (two_column_ok(o.s)) {
       const int kc = sh.kc;
@@ -1457,11 +1523,12 @@ int launch_AT(const OpT<T> &o, const T *y, T *x, T * /*scratch: unused*/, hipStr
               const Fuse *fuse) {
   if ((uint64_t)o.s.n * sizeof(T) >= (1ull << 32)) return SISL_ERR_ARG;  // 32-bit offsets
   const Fuse f = fuse ? *fuse : Fuse{};
-  const AShape sh = a_shape(o.s, (int)sizeof(T), (f.nb ? 3 : 1) + (f.cb ? 4 : 0));
+  const AShape sh = a_shape(o.s, (int)sizeof(T), (f.nb == 4 ? 9 : f.nb ? 3 : 1) + (f.cb ? 4 : 0));
   const Stag &s = o.s;
   // same arrays as the forward operator: y in, x out, 13 coefficient arrays (+ the basis column)
   const double sc = f.cb ? 2.0 : (double)sizeof(T);  // bytes per coefficient element
-  const double bytes = (double)sizeof(T) * 2.0 * s.n + sc * (3.0 * s.u.n() + 3.0 * s.v.n() + 6.0 * s.w.n() + s.m.n()) +
+  const double si = f.nb == 4 ? 2.0 : (double)sizeof(T);  // bytes per input element
+  const double bytes = ((double)sizeof(T) + si) * s.n + sc * (3.0 * s.u.n() + 3.0 * s.v.n() + 6.0 * s.w.n() + s.m.n()) +
                        (f.nb == 1 ? 2.0 : f.nb == 2 ? (double)sizeof(T) : 0.0) * s.n +
                        (f.res ? (double)sizeof(T) * s.n : 0.0);
   const int tid = timer_begin(SISL_T_HELMHOLTZ_T, sizeof(T) == 4, bytes, stop, nullptr, st);
@@ -1473,8 +1540,9 @@ int launch_AT(const OpT<T> &o, const T *y, T *x, T * /*scratch: unused*/, hipStr
 }
 
 // Time every launch shape of the operator and of its transpose (FP32 also
-// with the fused FP16 basis normalisation) on the caller's buffers and keep
-// the fastest of each for this grid and element type.
+// with the fused FP16 basis normalisation and with an FP16 basis column as
+// the input) on the caller's buffers and keep the fastest of each for this
+// grid and element type.
 __global__ void set_one_kernel(double *p) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *p = 1.0;
 }
@@ -1511,14 +1579,16 @@ static int autotune_A(const OpT<T> &o, const T *x, T *y, void *scratch, hipStrea
       if (unr == 3 && !two_column_ok(o.s)) continue;
       if (kc >= 2) cand.push_back({kc, unr});
     }
-  const int nvar = sizeof(T) == 4 ? 4 : 2;
-  // bfloat16 coefficients (cb): the fused variants only (the inner solver's launches)
+  // FP32: 0, 1 plain, 2, 3 fused normalisation, 4, 5 FP16 basis input
+  // (operator, transpose); FP64: 0, 1
+  const int nvar = sizeof(T) == 4 ? 6 : 2;
+  // bfloat16 coefficients (cb): the inner solver's variants only
   const int var0 = cb ? 2 : 0;
   if (cb && sizeof(T) != 4) return SISL_ERR_ARG;
   // fused variants: scale 1 from device memory, basis column into the caller's scratch
   Fuse fz;
   fz.cb = cb ? 1 : 0;
-  if (nvar == 4) {
+  if (sizeof(T) == 4) {
     if (!scratch) return SISL_ERR_ARG;
     hipLaunchKernelGGL(set_one_kernel, dim3(1), dim3(64), 0, st, static_cast<double *>(scratch));
     SISL_CHECK_LAUNCH();
@@ -1543,12 +1613,15 @@ static int autotune_A(const OpT<T> &o, const T *x, T *y, void *scratch, hipStrea
     for (int q = 0; q < made; ++q) hipEventDestroy(ev[q]);
     return SISL_ERR_LAUNCH;
   }
-  AShape best[4];
-  for (int v = 0; v < 4; ++v) best[v] = {(levels + 7) / 8, 1};
+  // the FP16-input variants read the caller's x as FP16 (its first half)
+  Fuse fz16 = fz;
+  fz16.nb = 4;
+  AShape best[6];
+  for (int v = 0; v < 6; ++v) best[v] = {(levels + 7) / 8, 1};
   int rc = SISL_OK;
   for (int var = var0; var < nvar && rc == SISL_OK; ++var) {
     const int tr = var & 1;
-    const Fuse f = var >= 2 ? fz : Fuse{};
+    const Fuse f = var >= 4 ? fz16 : var >= 2 ? fz : Fuse{};
     // two timing rounds over all candidates, the faster of each candidate's
     // two times counts (one round alone let clock noise pick a shape ~20%
     // slower than the best)
@@ -1557,10 +1630,9 @@ static int autotune_A(const OpT<T> &o, const T *x, T *y, void *scratch, hipStrea
       for (size_t ci = 0; ci < cand.size(); ++ci) {
         const AShape &c = cand[ci];
         auto run = [&]() {
-          if (tr) launch_AT_shape(o, x, y, st, nullptr, c, f);
-          else launch_A_shape(o, x, y, st, nullptr, c, f);
+          return tr ? launch_AT_shape(o, x, y, st, nullptr, c, f) : launch_A_shape(o, x, y, st, nullptr, c, f);
         };
-        run();  // warm-up (code objects, first-touch)
+        if ((rc = run())) break;  // warm-up (code objects, first-touch)
         for (int r = 0; r < kReps; ++r) {
           flush_cache();
           hipEventRecord(ev[2 * r], st);
@@ -1586,7 +1658,7 @@ static int autotune_A(const OpT<T> &o, const T *x, T *y, void *scratch, hipStrea
   {
     std::lock_guard<std::mutex> g(g_tune_mu);
     for (int var = var0; var < nvar; ++var)
-      g_tune[{o.s.nx, o.s.ny, o.s.nz, (int)sizeof(T), var + (cb ? 4 : 0)}] = best[var];
+      g_tune[{o.s.nx, o.s.ny, o.s.nz, (int)sizeof(T), (var >= 4 ? var + 4 : var) + (cb ? 4 : 0)}] = best[var];
     ++g_tune_generation;
   }
   if (best_kc) *best_kc = best[var0].kc;
@@ -1689,13 +1761,13 @@ int sisl_helmholtz_apply_f32(const SislOperator *op, const float *x, float *y, v
 int sisl_set_operator_shape(const SislGrid *g, int32_t fp32, int32_t chunk, int32_t unroll) {
   if (!g || chunk < 2 || unroll < 1 || unroll > 3) return SISL_ERR_ARG;
   std::lock_guard<std::mutex> lk(sisl::g_tune_mu);
-  for (int var = 0; var < 8; ++var) sisl::g_tune[{g->nx, g->ny, g->nz, fp32 ? 4 : 8, var}] = sisl::AShape{chunk, unroll};
+  for (int var = 0; var < 16; ++var) sisl::g_tune[{g->nx, g->ny, g->nz, fp32 ? 4 : 8, var}] = sisl::AShape{chunk, unroll};
   ++sisl::g_tune_generation;
   return SISL_OK;
 }
 
 int sisl_get_operator_shape(const SislGrid *g, int32_t fp32, int32_t variant, int32_t *chunk, int32_t *unroll) {
-  if (!g || !chunk || !unroll || variant < 0 || variant > 7) return SISL_ERR_ARG;
+  if (!g || !chunk || !unroll || variant < 0 || variant > 15) return SISL_ERR_ARG;
   const sisl::AShape sh = sisl::a_shape(sisl::Stag::make(*g), fp32 ? 4 : 8, variant);
   *chunk = sh.kc;
   *unroll = sh.unr;
@@ -1730,6 +1802,19 @@ int sisl_helmholtz_apply_fused_f32(const SislOperator *op, const float *x, float
   f.nb = 1;
   f.vs = vs;
   f.cb = coef_bf16 ? 1 : 0;
+  if (transpose) return sisl::launch_AT(o, x, y, (float *)nullptr, sisl::as_stream(s), nullptr, &f);
+  return sisl::launch_A(o, x, y, sisl::as_stream(s), nullptr, &f);
+}
+int sisl_helmholtz_apply_h16_f32(const SislOperator *op, const void *x16, float *y, const double *inv,
+                                 int32_t coef_bf16, int32_t transpose, void *s) {
+  if (!op || !x16 || !y || !inv || x16 == (const void *)y) return SISL_ERR_ARG;
+  auto o = sisl::OpT<float>::make(*op);
+  if (!o.complete()) return SISL_ERR_ARG;
+  sisl::Fuse f;
+  f.inv = inv;
+  f.nb = 4;
+  f.cb = coef_bf16 ? 1 : 0;
+  const float *x = static_cast<const float *>(x16);  // read as FP16 by the NB = 4 kernels
   if (transpose) return sisl::launch_AT(o, x, y, (float *)nullptr, sisl::as_stream(s), nullptr, &f);
   return sisl::launch_A(o, x, y, sisl::as_stream(s), nullptr, &f);
 }

@@ -50,10 +50,13 @@ struct Small {
   // device-controlled outer loop (batched solver)
   int done, outer, steps_total, pad2;
   double b_norm, Mb_norm, ptol, rnorm;
+  // per-vector basis scales (inner FP16 steps whose operator reads the
+  // basis): unit basis vector v = vsc[v] * V[v]
+  double vsc[kMaxRestart + 2];
 };
 
 __device__ void decide_body(Small *sm, int nv);
-__device__ void finalize_body(Small *sm, int j, int R, int early_exit);
+__device__ void finalize_body(Small *sm, int j, int R, int early_exit, double vs = 0.0);
 
 // ---- block reduction helpers -------------------------------------------------
 // One value per thread -> partials[slot][block] (fixed-order, deterministic).
@@ -130,6 +133,7 @@ struct Tail {
   Small *sm = nullptr;
   int nv = 0, j = 0, R = 0, early_exit = 0;
   int *next_flag = nullptr;
+  double vs = 0.0;  // > 0: the update pass stored the next basis column (finalize_body)
 };
 
 __global__ void __launch_bounds__(kTailThreads) tail_kernel(const double *__restrict__ partials, Tail t,
@@ -151,7 +155,7 @@ __global__ void __launch_bounds__(kTailThreads) tail_kernel(const double *__rest
   if (t.mode == 1) {
     decide_body(t.sm, t.nv);
   } else {
-    finalize_body(t.sm, t.j, t.R, t.early_exit);
+    finalize_body(t.sm, t.j, t.R, t.early_exit, t.vs);
     *t.next_flag = t.sm->broken;
   }
 }
@@ -205,6 +209,18 @@ __device__ __forceinline__ bool pass_skipped(const int *stop, const int *gate) {
   return (stop && *stop) || (gate && !*gate);
 }
 
+// Inner FP16 Arnoldi steps whose operator reads the stored basis column:
+// `scale[v]` replaces the common basis scale vinv for basis vector v, and
+// the update pass stores the orthogonalised vector, pre-scaled by
+// vs / sqrt(*wn2) (wn2: its squared norm before the projection), into the
+// basis column `next` instead of writing it back to w.
+template <typename TB> struct BasisOut {
+  const double *scale = nullptr;
+  TB *next = nullptr;
+  const double *wn2 = nullptr;
+  double vs = 1.0;
+};
+
 // partial[v][block] = vinv * sum over the block's elements of V[v] . w for
 // every v in [v0, v0+8) of every group, plus (slot nv) w . w.  FP64
 // accumulation; vinv undoes the basis storage scale.
@@ -213,7 +229,8 @@ __device__ __forceinline__ void dots_body(const TB *__restrict__ V, int64_t ldv,
                                           const TV *__restrict__ w, int64_t n,
                                           double *__restrict__ partials, double vinv,
                                           const int *__restrict__ stop,
-                                          const int *__restrict__ gate, int blk, int nblk) {
+                                          const int *__restrict__ gate, int blk, int nblk,
+                                          const double *__restrict__ vsc = nullptr) {
   if (pass_skipped(stop, gate)) return;
   constexpr int N = Vec<TV>::N;
   const int64_t nq = n / N;
@@ -257,7 +274,7 @@ __device__ __forceinline__ void dots_body(const TB *__restrict__ V, int64_t ldv,
     }
     const int cnt = min(kGrp, nv - g * kGrp);
 #pragma unroll
-    for (int r = 0; r < kGrp; ++r) acc[r] *= vinv;
+    for (int r = 0; r < kGrp; ++r) acc[r] *= vsc ? vsc[min(g * kGrp + r, nv - 1)] : vinv;
     block_store_group(acc, g * kGrp, cnt, partials, blk);
   }
 }
@@ -267,8 +284,9 @@ __global__ void __launch_bounds__(256) dots_kernel(const TB *__restrict__ V, int
                                                    const TV *__restrict__ w, int64_t n,
                                                    double *__restrict__ partials, double vinv,
                                                    const int *__restrict__ stop,
-                                                   const int *__restrict__ gate) {
-  dots_body(V, ldv, nv, w, n, partials, vinv, stop, gate, blockIdx.x, gridDim.x);
+                                                   const int *__restrict__ gate,
+                                                   const double *__restrict__ vsc) {
+  dots_body(V, ldv, nv, w, n, partials, vinv, stop, gate, blockIdx.x, gridDim.x, vsc);
 }
 
 // Single-sweep variant for nv <= 8*G: w is read once, all G groups of
@@ -279,7 +297,8 @@ __device__ __forceinline__ void dots1_body(const TB *__restrict__ V, int64_t ldv
                                            const TV *__restrict__ w, int64_t n,
                                            double *__restrict__ partials, double vinv,
                                            const int *__restrict__ stop,
-                                           const int *__restrict__ gate, int blk, int nblk) {
+                                           const int *__restrict__ gate, int blk, int nblk,
+                                           const double *__restrict__ vsc = nullptr) {
   if (pass_skipped(stop, gate)) return;
   constexpr int N = Vec<TV>::N;
   const int64_t nq = n / N;
@@ -321,7 +340,7 @@ __device__ __forceinline__ void dots1_body(const TB *__restrict__ V, int64_t ldv
   for (int g = 0; g < G; ++g) {
     if (g * kGrp >= nv) break;
 #pragma unroll
-    for (int r = 0; r < kGrp; ++r) acc[g][r] *= vinv;
+    for (int r = 0; r < kGrp; ++r) acc[g][r] *= vsc ? vsc[min(g * kGrp + r, nv - 1)] : vinv;
     block_store_group(acc[g], g * kGrp, min(kGrp, nv - g * kGrp), partials, blk);
   }
 }
@@ -331,8 +350,9 @@ __global__ void __launch_bounds__(256) dots1_kernel(const TB *__restrict__ V, in
                                                     const TV *__restrict__ w, int64_t n,
                                                     double *__restrict__ partials, double vinv,
                                                     const int *__restrict__ stop,
-                                                    const int *__restrict__ gate) {
-  dots1_body<TB, TV, G>(V, ldv, nv, w, n, partials, vinv, stop, gate, blockIdx.x, gridDim.x);
+                                                    const int *__restrict__ gate,
+                                                    const double *__restrict__ vsc) {
+  dots1_body<TB, TV, G>(V, ldv, nv, w, n, partials, vinv, stop, gate, blockIdx.x, gridDim.x, vsc);
 }
 
 // Exactly NV basis vectors (no clamped group members): the same sums in the
@@ -343,7 +363,8 @@ __global__ void __launch_bounds__(256) dotsx_kernel(const TB *__restrict__ V, in
                                                     const TV *__restrict__ w, int64_t n,
                                                     double *__restrict__ partials, double vinv,
                                                     const int *__restrict__ stop,
-                                                    const int *__restrict__ gate) {
+                                                    const int *__restrict__ gate,
+                                                    const double *__restrict__ vsc) {
   if (pass_skipped(stop, gate)) return;
   constexpr int N = Vec<TV>::N;
   const int64_t nq = n / N;
@@ -380,25 +401,72 @@ __global__ void __launch_bounds__(256) dotsx_kernel(const TB *__restrict__ V, in
 #pragma unroll
     for (int r = 0; r < kGrp; ++r) {
       const int v = g * kGrp + r;
-      a[r] = v < NV ? acc[v < NV ? v : 0] * vinv : 0.0;
+      const int vc = v < NV ? v : 0;
+      a[r] = v < NV ? acc[vc] * (vsc ? vsc[vc] : vinv) : 0.0;
     }
     block_store_group(a, g * kGrp, min(kGrp, NV - g * kGrp), partials, blockIdx.x);
   }
 }
 
+// Store of the orthogonalised elements: back into w (slot 0 <- w . w), or
+// with `bo.next` pre-scaled into the next basis column (slot 0 <- the
+// squared norm of the stored values).
+template <int N, typename TB, typename TV>
+__device__ __forceinline__ void update_store(TV *__restrict__ w, int64_t q, const double (&s)[N],
+                                             const BasisOut<TB> &bo, double pre, double &ww) {
+  if (bo.next) {
+    TB sb[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) sb[k] = (TB)(float)(s[k] * pre);
+    vstore<N>(bo.next, q, sb);
+#pragma unroll
+    for (int k = 0; k < N; ++k) ww += (double)sb[k] * (double)sb[k];
+  } else {
+    TV sn[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) sn[k] = (TV)s[k];
+    vstore<N>(w, q, sn);
+#pragma unroll
+    for (int k = 0; k < N; ++k) ww += (double)sn[k] * (double)sn[k];
+  }
+}
+template <typename TB, typename TV>
+__device__ __forceinline__ void update_store1(TV *__restrict__ w, int64_t e, double sd, const BasisOut<TB> &bo,
+                                              double pre, double &ww) {
+  if (bo.next) {
+    const TB sb = (TB)(float)(sd * pre);
+    bo.next[e] = sb;
+    ww += (double)sb * (double)sb;
+  } else {
+    const TV sn = (TV)sd;
+    w[e] = sn;
+    ww += (double)sn * (double)sn;
+  }
+}
+// pre-scale of the next basis column: vs / ||w|| before the projection
+template <typename TB>
+__device__ __forceinline__ double next_prescale(const BasisOut<TB> &bo) {
+  if (!bo.next) return 0.0;
+  const double w2 = *bo.wn2;
+  return w2 > 0.0 ? bo.vs / sqrt(w2) : 0.0;
+}
+
 // w <- w - sum_v h[v] vinv V[v] (h read from device memory), slot 0 <- w . w
+// (or, with bo.next, the result stored into the next basis column)
 template <typename TB, typename TV>
 __device__ __forceinline__ void update_body(const TB *__restrict__ V, int64_t ldv, int nv,
                                             const double *__restrict__ h, TV *__restrict__ w,
                                             int64_t n, double *__restrict__ partials, double vinv,
                                             const int *__restrict__ stop,
-                                            const int *__restrict__ gate, int blk, int nblk) {
+                                            const int *__restrict__ gate, int blk, int nblk,
+                                            BasisOut<TB> bo = BasisOut<TB>{}) {
   if (pass_skipped(stop, gate)) return;
   constexpr int N = Vec<TV>::N;
   __shared__ double hs[kMaxRestart + 8];
   const int nvr = ((nv + kGrp - 1) / kGrp) * kGrp;
-  for (int v = threadIdx.x; v < nvr; v += blockDim.x) hs[v] = v < nv ? h[v] * vinv : 0.0;
+  for (int v = threadIdx.x; v < nvr; v += blockDim.x) hs[v] = v < nv ? h[v] * (bo.scale ? bo.scale[v] : vinv) : 0.0;
   __syncthreads();
+  const double pre = next_prescale(bo);
   const int64_t nq = n / N;
   const int64_t stride = (int64_t)nblk * blockDim.x;
   const int64_t t0 = (int64_t)blk * blockDim.x + threadIdx.x;
@@ -421,20 +489,13 @@ __device__ __forceinline__ void update_body(const TB *__restrict__ V, int64_t ld
         for (int k = 0; k < N; ++k) s[k] -= hv * (double)x[r][k];
       }
     }
-    TV sn[N];
-#pragma unroll
-    for (int k = 0; k < N; ++k) sn[k] = (TV)s[k];
-    vstore<N>(w, q, sn);
-#pragma unroll
-    for (int k = 0; k < N; ++k) ww += (double)sn[k] * (double)sn[k];
+    update_store<N>(w, q, s, bo, pre, ww);
   }
   if (t0 < n - nq * N) {  // scalar tail
     const int64_t e = nq * N + t0;
     double sd = (double)w[e];
     for (int v = 0; v < nv; ++v) sd -= hs[v] * (double)V[(int64_t)v * ldv + e];
-    const TV sn = (TV)sd;
-    w[e] = sn;
-    ww += (double)sn * (double)sn;
+    update_store1(w, e, sd, bo, pre, ww);
   }
   block_store_slot(ww, 0, partials, blk);
 }
@@ -445,8 +506,8 @@ __global__ void __launch_bounds__(256) update_kernel(const TB *__restrict__ V, i
                                                      TV *__restrict__ w, int64_t n,
                                                      double *__restrict__ partials, double vinv,
                                                      const int *__restrict__ stop,
-                                                     const int *__restrict__ gate) {
-  update_body(V, ldv, nv, h, w, n, partials, vinv, stop, gate, blockIdx.x, gridDim.x);
+                                                     const int *__restrict__ gate, BasisOut<TB> bo) {
+  update_body(V, ldv, nv, h, w, n, partials, vinv, stop, gate, blockIdx.x, gridDim.x, bo);
 }
 
 // update_kernel for exactly NV basis vectors (same bits, no clamped loads).
@@ -456,12 +517,13 @@ __global__ void __launch_bounds__(256) updatex_kernel(const TB *__restrict__ V, 
                                                       TV *__restrict__ w, int64_t n,
                                                       double *__restrict__ partials, double vinv,
                                                       const int *__restrict__ stop,
-                                                      const int *__restrict__ gate) {
+                                                      const int *__restrict__ gate, BasisOut<TB> bo) {
   if (pass_skipped(stop, gate)) return;
   constexpr int N = Vec<TV>::N;
   __shared__ double hs[NV];
-  for (int v = threadIdx.x; v < NV; v += blockDim.x) hs[v] = h[v] * vinv;
+  for (int v = threadIdx.x; v < NV; v += blockDim.x) hs[v] = h[v] * (bo.scale ? bo.scale[v] : vinv);
   __syncthreads();
+  const double pre = next_prescale(bo);
   const int64_t nq = n / N;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -481,20 +543,13 @@ __global__ void __launch_bounds__(256) updatex_kernel(const TB *__restrict__ V, 
 #pragma unroll
       for (int k = 0; k < N; ++k) sd[k] -= hv * (double)x[r][k];
     }
-    TV sn[N];
-#pragma unroll
-    for (int k = 0; k < N; ++k) sn[k] = (TV)sd[k];
-    vstore<N>(w, q, sn);
-#pragma unroll
-    for (int k = 0; k < N; ++k) ww += (double)sn[k] * (double)sn[k];
+    update_store<N>(w, q, sd, bo, pre, ww);
   }
   if (t0 < n - nq * N) {
     const int64_t e = nq * N + t0;
     double sd = (double)w[e];
     for (int v = 0; v < NV; ++v) sd -= hs[v] * (double)V[(int64_t)v * ldv + e];
-    const TV sn = (TV)sd;
-    w[e] = sn;
-    ww += (double)sn * (double)sn;
+    update_store1(w, e, sd, bo, pre, ww);
   }
   block_store_slot(ww, 0, partials, blockIdx.x);
 }
@@ -517,7 +572,12 @@ __device__ void decide_body(Small *sm, int nv) {
 
 // Arnoldi bookkeeping for step j (one thread): JAX's H row, breakdown test
 // (v_norm_1 <= eps * v_norm_0), Givens residual estimate for early exit.
-__device__ void finalize_body(Small *sm, int j, int R, int early_exit) {
+//
+// vs > 0: the update pass stored the orthogonalised vector pre-scaled by
+// vs / vn0 as basis column j + 1 and nn1a holds the squared norm q^2 of the
+// stored values, so that ||w|| = q vn0 / vs, the unit vector is V[j+1] / q
+// (scale 1 / q, which the next operator launch applies to its input).
+__device__ void finalize_body(Small *sm, int j, int R, int early_exit, double vs) {
   if (sm->broken) {
     sm->inv = 0.0;
     return;
@@ -526,13 +586,19 @@ __device__ void finalize_body(Small *sm, int j, int R, int early_exit) {
   double h[kMaxRestart + 2];
   for (int v = 0; v <= j; ++v) h[v] = sm->h1[v] + (two ? sm->h2[v] : 0.0);
   const double vn0 = sqrt(sm->h1[j + 1]);  // ||w||^2 before projection sits in slot nv
-  double vn1 = sqrt(two ? sm->nn1b[0] : sm->nn1a[0]);
+  const double q = vs > 0.0 ? sqrt(sm->nn1a[0]) : 0.0;
+  double vn1 = vs > 0.0 ? q * vn0 / vs : sqrt(two ? sm->nn1b[0] : sm->nn1a[0]);
   const double thresh = 2.220446049250313e-16 * vn0;
   if (!(vn1 > thresh)) vn1 = 0.0;
   h[j + 1] = vn1;
   double *row = sm->H + (int64_t)j * (R + 1);
   for (int v = 0; v <= R; ++v) row[v] = v <= j + 1 ? h[v] : 0.0;
-  sm->inv = vn1 > 0.0 ? 1.0 / vn1 : 0.0;
+  if (vs > 0.0) {
+    sm->vsc[j + 1] = vn1 > 0.0 ? 1.0 / q : 0.0;
+    sm->inv = sm->vsc[j + 1];
+  } else {
+    sm->inv = vn1 > 0.0 ? 1.0 / vn1 : 0.0;
+  }
   sm->steps = j + 1;
   if (vn1 == 0.0) sm->broken = 1;
   // Givens rotations on a copy of the new column
@@ -648,11 +714,12 @@ __global__ void __launch_bounds__(256) norm2_to_f32_kernel(const double *__restr
 template <typename TV>
 __device__ __forceinline__ void combine_body(const TV *__restrict__ V, int64_t ldv, int nv,
                                              const double *__restrict__ y, double *__restrict__ x,
-                                             int64_t n, double vinv, int blk, int nblk) {
+                                             int64_t n, double vinv, int blk, int nblk,
+                                             const double *__restrict__ vsc = nullptr) {
   constexpr int N = Vec<TV>::N;
   __shared__ double ys[kMaxRestart + 8];
   const int nvr = ((nv + kGrp - 1) / kGrp) * kGrp;
-  for (int v = threadIdx.x; v < nvr; v += blockDim.x) ys[v] = v < nv ? y[v] * vinv : 0.0;
+  for (int v = threadIdx.x; v < nvr; v += blockDim.x) ys[v] = v < nv ? y[v] * (vsc ? vsc[v] : vinv) : 0.0;
   __syncthreads();
   const int64_t nq = n / N;
   const int64_t stride = (int64_t)nblk * blockDim.x;
@@ -687,8 +754,9 @@ template <typename TV>
 __global__ void __launch_bounds__(256) combine_kernel(const TV *__restrict__ V, int64_t ldv, int nv,
                                                       const double *__restrict__ y,
                                                       double *__restrict__ x, int64_t n,
-                                                      double vinv, const int *__restrict__ nvp) {
-  combine_body(V, ldv, nvp ? *nvp : nv, y, x, n, vinv, blockIdx.x, gridDim.x);
+                                                      double vinv, const int *__restrict__ nvp,
+                                                      const double *__restrict__ vsc) {
+  combine_body(V, ldv, nvp ? *nvp : nv, y, x, n, vinv, blockIdx.x, gridDim.x, vsc);
 }
 
 // Least squares of one cycle, min || beta e1 - H^T y || over the `steps`
@@ -768,7 +836,9 @@ __global__ void __launch_bounds__(kLsqThreads) lstsq_kernel(Small *sm, int R, do
 
 __global__ void zero_steps_total_kernel(Small *sm) { sm->steps_total = 0; }
 
-__global__ void reset_cycle_kernel(Small *sm, int R, double g0, double target, int *flags) {
+// vinv0: the scale of basis vector 0 (stored by the cycle's first operator
+// launch as vs * unit vector)
+__global__ void reset_cycle_kernel(Small *sm, int R, double g0, double target, int *flags, double vinv0) {
   const int t = threadIdx.x;
   for (int q = t; q < R * (R + 1); q += blockDim.x) {
     const int r = q / (R + 1), c = q % (R + 1);
@@ -782,6 +852,7 @@ __global__ void reset_cycle_kernel(Small *sm, int R, double g0, double target, i
     for (int v = 0; v <= R; ++v) sm->g[v] = 0.0;
     sm->g[0] = g0;
     sm->target = target;
+    sm->vsc[0] = vinv0;
     flags[0] = 0;  // stop flag of step 0
   }
 }
@@ -791,15 +862,15 @@ __global__ void reset_cycle_kernel(Small *sm, int R, double g0, double target, i
 template <typename TB, typename TV>
 static void dots(const TB *V, int64_t ldv, int nv, const TV *w, int64_t n, double *partials,
                  hipStream_t s, double vinv = 1.0, const int *stop = nullptr,
-                 const int *gate = nullptr, const Tail *tl = nullptr) {
+                 const int *gate = nullptr, const Tail *tl = nullptr, const double *vsc = nullptr) {
   const double bytes = (double)n * ((double)nv * sizeof(TB) + sizeof(TV));
   const int tid = timer_begin(SISL_T_KRYLOV_DOTS, sizeof(TV) == 4, bytes, stop, gate, s);
   const dim3 gr(kRedGrid), bl(kBlock);
   // exact-count variants where they measured faster (short prefixes, and
   // 9 / 10 vectors, which the two-group variant would load as 16)
-#define SISL_DOTSX(K) hipLaunchKernelGGL((dotsx_kernel<TB, TV, K>), gr, bl, 0, s, V, ldv, w, n, partials, vinv, stop, gate)
+#define SISL_DOTSX(K) hipLaunchKernelGGL((dotsx_kernel<TB, TV, K>), gr, bl, 0, s, V, ldv, w, n, partials, vinv, stop, gate, vsc)
   if (nv <= 0)
-    hipLaunchKernelGGL((dots_kernel<TB, TV>), gr, bl, 0, s, V, ldv, nv, w, n, partials, vinv, stop, gate);
+    hipLaunchKernelGGL((dots_kernel<TB, TV>), gr, bl, 0, s, V, ldv, nv, w, n, partials, vinv, stop, gate, vsc);
   else if (nv == 1) SISL_DOTSX(1);
   else if (nv == 2) SISL_DOTSX(2);
   else if (nv == 3) SISL_DOTSX(3);
@@ -807,25 +878,25 @@ static void dots(const TB *V, int64_t ldv, int nv, const TV *w, int64_t n, doubl
   else if (nv == 9) SISL_DOTSX(9);
   else if (nv == 10) SISL_DOTSX(10);
   else if (nv <= kGrp)
-    hipLaunchKernelGGL((dots1_kernel<TB, TV, 1>), gr, bl, 0, s, V, ldv, nv, w, n, partials, vinv, stop, gate);
+    hipLaunchKernelGGL((dots1_kernel<TB, TV, 1>), gr, bl, 0, s, V, ldv, nv, w, n, partials, vinv, stop, gate, vsc);
   else if (nv <= 2 * kGrp)
-    hipLaunchKernelGGL((dots1_kernel<TB, TV, 2>), gr, bl, 0, s, V, ldv, nv, w, n, partials, vinv, stop, gate);
+    hipLaunchKernelGGL((dots1_kernel<TB, TV, 2>), gr, bl, 0, s, V, ldv, nv, w, n, partials, vinv, stop, gate, vsc);
   else if (nv <= 3 * kGrp)
-    hipLaunchKernelGGL((dots1_kernel<TB, TV, 3>), gr, bl, 0, s, V, ldv, nv, w, n, partials, vinv, stop, gate);
+    hipLaunchKernelGGL((dots1_kernel<TB, TV, 3>), gr, bl, 0, s, V, ldv, nv, w, n, partials, vinv, stop, gate, vsc);
   else if (nv <= 4 * kGrp)
-    hipLaunchKernelGGL((dots1_kernel<TB, TV, 4>), gr, bl, 0, s, V, ldv, nv, w, n, partials, vinv, stop, gate);
+    hipLaunchKernelGGL((dots1_kernel<TB, TV, 4>), gr, bl, 0, s, V, ldv, nv, w, n, partials, vinv, stop, gate, vsc);
   else
-    hipLaunchKernelGGL((dots_kernel<TB, TV>), gr, bl, 0, s, V, ldv, nv, w, n, partials, vinv, stop, gate);
+    hipLaunchKernelGGL((dots_kernel<TB, TV>), gr, bl, 0, s, V, ldv, nv, w, n, partials, vinv, stop, gate, vsc);
   timer_end(tid, s);
   if (tl) hipLaunchKernelGGL(tail_kernel, dim3(1), dim3(kTailThreads), 0, s, (const double *)partials, *tl, stop, gate);
 }
 template <typename TB, typename TV>
 static void update(const TB *V, int64_t ldv, int nv, const double *h, TV *w, int64_t n,
                    double *partials, hipStream_t s, double vinv = 1.0, const int *stop = nullptr,
-                   const int *gate = nullptr, const Tail *tl = nullptr) {
-  const double bytes = (double)n * ((double)nv * sizeof(TB) + 2.0 * sizeof(TV));
+                   const int *gate = nullptr, const Tail *tl = nullptr, BasisOut<TB> bo = BasisOut<TB>{}) {
+  const double bytes = (double)n * ((double)nv * sizeof(TB) + sizeof(TV) + (bo.next ? sizeof(TB) : sizeof(TV)));
   const int tid = timer_begin(SISL_T_KRYLOV_UPDATE, sizeof(TV) == 4, bytes, stop, gate, s);
-#define SISL_UPDX(K) hipLaunchKernelGGL((updatex_kernel<TB, TV, K>), dim3(kRedGrid), dim3(kBlock), 0, s, V, ldv, h, w, n, partials, vinv, stop, gate)
+#define SISL_UPDX(K) hipLaunchKernelGGL((updatex_kernel<TB, TV, K>), dim3(kRedGrid), dim3(kBlock), 0, s, V, ldv, h, w, n, partials, vinv, stop, gate, bo)
   if (nv == 1) SISL_UPDX(1);
   else if (nv == 2) SISL_UPDX(2);
   else if (nv == 3) SISL_UPDX(3);
@@ -835,15 +906,15 @@ static void update(const TB *V, int64_t ldv, int nv, const double *h, TV *w, int
   else if (nv == 10) SISL_UPDX(10);
   else
     hipLaunchKernelGGL((update_kernel<TB, TV>), dim3(kRedGrid), dim3(kBlock), 0, s, V, ldv, nv, h, w,
-                       n, partials, vinv, stop, gate);
+                       n, partials, vinv, stop, gate, bo);
   timer_end(tid, s);
   if (tl) hipLaunchKernelGGL(tail_kernel, dim3(1), dim3(kTailThreads), 0, s, (const double *)partials, *tl, stop, gate);
 }
 template <typename TB>
 static void combine(const TB *V, int64_t ldv, int nv, const double *y, double *x, int64_t n,
-                    double vinv, hipStream_t s, const int *nvp = nullptr) {
+                    double vinv, hipStream_t s, const int *nvp = nullptr, const double *vsc = nullptr) {
   hipLaunchKernelGGL(combine_kernel<TB>, dim3(kRedGrid), dim3(kBlock), 0, s, V, ldv, nv, y, x, n,
-                     vinv, nvp);
+                     vinv, nvp, vsc);
 }
 static void reduce(const double *partials, int nslots, double *out, hipStream_t s,
                    const int *stop = nullptr, const int *gate = nullptr) {
@@ -902,6 +973,10 @@ struct Krylov {
   TV *cur = nullptr;
   int cb = 0;  // 1: the operator's 3-D coefficient arrays are bfloat16
   int single_cgs = 0;  // 1: one Gram-Schmidt pass per step (no second pass)
+  // 1 (FP16 basis, one Gram-Schmidt pass): steps after the first apply the
+  // operator to the stored FP16 basis column, and each step's update pass
+  // stores the next column (with its own scale) instead of writing w back
+  int in16 = 0;
   int64_t ldv() const { return basis_ld(n); }
   double vscale() const { return basis_scale<TB>(n); }
   const TV *basis_in(int j) const {
@@ -929,8 +1004,12 @@ struct Krylov {
   // into the operator launch of step j (which reads the unnormalised w_{j-1},
   // scales its outputs and stores basis column j), and the Arnoldi vector
   // alternates between the buffers w (even j) and cur (odd j).  Step 0 reads
-  // the unit residual that starts the cycle.  FP64 solver: a separate
-  // normalisation pass after the step.
+  // the unnormalised residual that starts the cycle.  With in16, step j > 0
+  // reads basis column j itself (stored by step j-1's update pass, unit
+  // after the scale vsc[j]) and the Arnoldi vector always sits in w, so the
+  // operator applies to exactly the basis vector the projections and the
+  // solution update use.  FP64 solver: a separate normalisation pass after
+  // the step.
   int arnoldi(int j, int R, int early_exit) {
     int rc;
     const int64_t ld = ldv();
@@ -942,14 +1021,19 @@ struct Krylov {
       // step 0 reads the cycle's unnormalised start vector (cur), later
       // steps the previous unnormalised Arnoldi vector; every launch scales
       // by 1 / norm and stores basis column j
-      wj = (j % 2 == 0) ? w : cur;
-      const TV *src = j % 2 == 0 ? cur : w;
+      wj = (in16 || j % 2 == 0) ? w : cur;
       Fuse f;
       f.inv = &sm->inv;
-      f.vout = V + (int64_t)j * ld;
-      f.nb = sizeof(TB) == 2 ? 1 : 2;
-      f.vs = vs;
-      if ((rc = apply(src, wj, stop, &f))) return rc;
+      if (in16 && j > 0) {
+        f.nb = 4;  // input: basis column j in FP16, scaled by *inv = vsc[j]
+        if ((rc = apply(reinterpret_cast<const TV *>(V + (int64_t)j * ld), wj, stop, &f))) return rc;
+      } else {
+        const TV *src = j % 2 == 0 ? cur : w;
+        f.vout = V + (int64_t)j * ld;
+        f.nb = sizeof(TB) == 2 ? 1 : 2;
+        f.vs = vs;
+        if ((rc = apply(src, wj, stop, &f))) return rc;
+      }
     } else {
       if ((rc = apply(basis_in(j), w, stop))) return rc;
     }
@@ -957,11 +1041,20 @@ struct Krylov {
     const int *gate = &sm->gate2;
     // each pass is followed by one tail launch: its reduction and the bookkeeping after it
     Tail t{sm->h1, nv + 1, 0, sm, nv, j, R, early_exit, flags + j + 1};
-    dots<TB, TV>(V, ld, nv, wj, n, partials, s, vinv, stop, nullptr, &t);  // h1[0..nv-1], ||w||^2 in h1[nv]
+    const double *vsc = in16 ? sm->vsc : nullptr;
+    dots<TB, TV>(V, ld, nv, wj, n, partials, s, vinv, stop, nullptr, &t, vsc);  // h1[0..nv-1], ||w||^2 in h1[nv]
     if (single_cgs) {
       // one pass: gate2 stays 0 from the cycle reset, so the bookkeeping reads h1, nn1a
       t.out = sm->nn1a; t.nout = 1; t.mode = 2;
-      update<TB, TV>(V, ld, nv, sm->h1, wj, n, partials, s, vinv, stop, nullptr, &t);
+      BasisOut<TB> bo;
+      if (in16) {
+        bo.scale = sm->vsc;
+        bo.next = V + (int64_t)(j + 1) * ld;
+        bo.wn2 = &sm->h1[nv];
+        bo.vs = vs;
+        t.vs = vs;
+      }
+      update<TB, TV>(V, ld, nv, sm->h1, wj, n, partials, s, vinv, stop, nullptr, &t, bo);
     } else {
       t.out = sm->nn1a; t.nout = 1; t.mode = 1;                             // + second-pass decision
       update<TB, TV>(V, ld, nv, sm->h1, wj, n, partials, s, vinv, stop, nullptr, &t);
@@ -1051,7 +1144,7 @@ static int gmres64(const OpT<double> &op, const SislGmresParams &p, const double
   int k = 0;
   while (k < p.maxiter && rnorm > ptol) {
     hipLaunchKernelGGL(reset_cycle_kernel, dim3(1), dim3(kBlock), 0, s, sm, R, rnorm,
-                       p.early_exit ? ptol : 0.0, K.flags);
+                       p.early_exit ? ptol : 0.0, K.flags, 1.0);
     for (int j = 0; j < R; ++j)
       if ((rc = K.arnoldi(j, R, p.early_exit))) return rc;
     hipLaunchKernelGGL(lstsq_kernel, dim3(1), dim3(kLsqThreads), 0, s, sm, R, rnorm);
@@ -1135,6 +1228,7 @@ static std::string cycle_key(const Krylov<float, TB> &K, int R) {
   key_add(k, K.transpose);
   key_add(k, K.cb);
   key_add(k, K.single_cgs);
+  key_add(k, K.in16);
   key_add(k, K.n);
   const OpT<float> &o = K.op;
   const int dims[5] = {o.s.nx, o.s.ny, o.s.nz, o.s.px, o.s.py};
@@ -1249,6 +1343,7 @@ static int gmres_mixed(const OpT<double> &op64, const OpT<float> &op32, const Si
   K.cur = (float *)(ws + L.cur32);  // second Arnoldi buffer (and the FP16 path's unit residual)
   K.cb = p.coef_bf16 ? 1 : 0;
   K.single_cgs = p.inner_single_cgs ? 1 : 0;
+  K.in16 = (sizeof(TB) == 2 && K.single_cgs && p.inner_basis_operand) ? 1 : 0;
   int rc;
   double hs[4];
   auto M64 = [&](double *in, double *out) -> int {
@@ -1287,10 +1382,11 @@ static int gmres_mixed(const OpT<double> &op64, const OpT<float> &op32, const Si
     if (!(rnorm > 2.220446049250313e-16) && !std::isnan(rnorm)) rnorm = 0.0;
     if (!(rnorm > ptol) || outer >= max_outer) break;
     const double target = std::fmax(p.inner_rtol * rnorm, 0.5 * ptol);
-    hipLaunchKernelGGL(reset_cycle_kernel, dim3(1), dim3(kBlock), 0, s, sm, R, rnorm, target, K.flags);
+    hipLaunchKernelGGL(reset_cycle_kernel, dim3(1), dim3(kBlock), 0, s, sm, R, rnorm, target, K.flags,
+                       1.0 / K.vscale());
     if ((rc = run_inner_cycle(K, R))) return rc;
     hipLaunchKernelGGL(lstsq_kernel, dim3(1), dim3(kLsqThreads), 0, s, sm, R, rnorm);
-    combine<TB>(K.V, K.ldv(), R, sm->y, x, n, 1.0 / K.vscale(), s, &sm->steps);
+    combine<TB>(K.V, K.ldv(), R, sm->y, x, n, 1.0 / K.vscale(), s, &sm->steps, K.in16 ? sm->vsc : nullptr);
     ++outer;
   }
   if (st) {

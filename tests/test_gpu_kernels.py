@@ -369,10 +369,13 @@ def test_gmres_solution_matches_oracle():
     assert float(r.norm() / c.norm()) < 1e-9
 
 
+@pytest.mark.parametrize("operand", ["basis", "vector"])
 @pytest.mark.parametrize("gs", ["single", "iterated"])
 @pytest.mark.parametrize("basis", ["fp16", "fp32"])
 @pytest.mark.parametrize("coefs", ["bf16", "fp32"])
-def test_mixed_solver_reaches_tolerance(basis, coefs, gs):
+def test_mixed_solver_reaches_tolerance(basis, coefs, gs, operand):
+    if operand == "vector" and not (basis == "fp16" and gs == "single"):
+        pytest.skip("the operand choice exists for an FP16 basis with one Gram-Schmidt pass only")
     cfg = small_lam()
     case, go, A_o, M_o, sol = _solver_pair(cfg)
     rng = np.random.default_rng(22)
@@ -380,7 +383,7 @@ def test_mixed_solver_reaches_tolerance(basis, coefs, gs):
     from suetes_amd.steppers import SemiImplicitSolver3D
     ms = SemiImplicitSolver3D(sol.physics, sol.dt, sol.alpha, solver_tol=1e-10, solver_maxiter=30,
                               solver_restart=20, precision="mixed", inner_basis=basis,
-                              inner_coefficients=coefs, inner_gram_schmidt=gs)
+                              inner_coefficients=coefs, inner_gram_schmidt=gs, inner_operand=operand)
     ms.prepare(case.stepper.bg_precomputed())
     for transpose in (False, True):
         x = ms.solve_flat(b, torch.zeros_like(b), transpose=transpose)
@@ -443,12 +446,61 @@ def test_operator_launch_shapes_bit_identical_and_autotune(dtype, transpose, cfg
     assert torch.equal(apply(), outs[0])
 
 
+@pytest.mark.parametrize("coefs", ["fp32", "bf16"])
+@pytest.mark.parametrize("transpose", [False, True])
+def test_fp16_operand_launch_matches_fused_and_plain_launches(transpose, coefs):
+    """The operator launch on a stored FP16 basis column (input converted on
+    load, outputs scaled by *inv, nothing else stored) gives the bits of the
+    fused-normalisation launch on the same values in FP32, at every launch
+    shape, and (FP32 coefficients) of the plain launch times the scale."""
+    import ctypes as C
+    from suetes_amd import _lib
+    from suetes_amd.steppers import SemiImplicitSolver3D
+    case, go, A_o, M_o, sol = _solver_pair(small_lam())
+    ms = SemiImplicitSolver3D(sol.physics, sol.dt, sol.alpha, solver_tol=1e-10, solver_maxiter=30,
+                              solver_restart=20, precision="mixed", inner_coefficients=coefs)
+    ms.prepare(case.stepper.bg_precomputed())
+    ops = ms.op_inner
+    cb = int(coefs == "bf16")
+    L = _lib.lib()
+    g = ops.struct.g
+    n = int(L.sisl_krylov_size(C.byref(g)))
+    gen = torch.Generator(device=DEV).manual_seed(5)
+    x16 = (torch.randn(n, device=DEV, generator=gen) * 3.0).to(torch.float16)
+    x32 = x16.float()
+    inv = torch.tensor([0.3716], dtype=torch.float64, device=DEV)
+    basis = torch.empty(n, dtype=torch.float16, device=DEV)
+    st = _lib.stream_ptr()
+    outs = []
+    for chunk, unr in ((2, 1), (5, 2), (13, 1), (4, 3), (81, 3)):
+        _lib.check(L.sisl_set_operator_shape(C.byref(g), 1, chunk, unr), "shape")
+        y16 = torch.empty(n, dtype=torch.float32, device=DEV)
+        yf = torch.empty_like(y16)
+        _lib.check(L.sisl_helmholtz_apply_h16_f32(C.byref(ops.struct), x16.data_ptr(), y16.data_ptr(),
+                                                  inv.data_ptr(), cb, int(transpose), st), "h16")
+        _lib.check(L.sisl_helmholtz_apply_fused_f32(C.byref(ops.struct), x32.data_ptr(), yf.data_ptr(),
+                                                    inv.data_ptr(), basis.data_ptr(), 1.0, cb, int(transpose),
+                                                    st), "fused")
+        assert torch.equal(y16, yf), ((chunk, unr), (y16 - yf).abs().max().item())
+        if not cb:
+            yp = torch.empty_like(y16)
+            fn = L.sisl_helmholtz_apply_T_f32 if transpose else L.sisl_helmholtz_apply_f32
+            args = (C.byref(ops.struct), x32.data_ptr(), yp.data_ptr()) + ((None,) if transpose else ())
+            _lib.check(fn(*args, st), "plain")
+            assert torch.equal(y16, yp * torch.tensor(0.3716, dtype=torch.float32, device=DEV)), (chunk, unr)
+        outs.append(y16)
+    for y in outs[1:]:
+        assert torch.equal(y, outs[0])
+    assert bool(torch.isfinite(outs[0]).all()) and float(outs[0].abs().max()) > 0
+
+
 @pytest.mark.parametrize("transpose", [False, True])
 def test_mixed_solve_independent_of_launch_shapes(transpose):
     """The mixed solver (whose inner cycles use the operator kernels with the
-    fused basis normalisation, in FP32 and bfloat16 coefficient storage)
-    gives the same bits whatever launch shapes are pinned, with the inner
-    cycle replayed from a captured graph or launched eagerly."""
+    fused basis normalisation and on the stored FP16 basis column, in FP32
+    and bfloat16 coefficient storage) gives the same bits whatever launch
+    shapes are pinned, with the inner cycle replayed from a captured graph or
+    launched eagerly."""
     import ctypes as C
     from suetes_amd import _lib
     from suetes_amd.steppers import SemiImplicitSolver3D
