@@ -71,6 +71,8 @@ KERNEL_NAMES = {
     "helmholtz_T": "helmholtz_T_kernel (operator transpose A^T)",
     "precond_T": "precond_T_seg_kernel + faces (preconditioner transpose M^T)",
     "tricubic_adj": "tricubic_adj_kernel (deterministic gather transpose)",
+    "helmholtz_h16": "helmholtz_kernel on the FP16 basis column (operator apply A)",
+    "helmholtz_T_h16": "helmholtz_T_kernel on the FP16 basis column (operator transpose A^T)",
 }
 
 
@@ -126,13 +128,15 @@ def instrumented_step(case, state):
 
 # HBM bytes per launch of the dominant kernels from the two hardware-counter
 # passes (rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE, KiB per dispatch) on
-# this workload, scaled by the same counters of a plain 512 MiB device copy
-# (read factor 2.00, write factor 1.00); profiles/sisl_step_lam256x256x80.md.
-# FP32 operator: the launch with the fused basis normalisation and bfloat16
-# coefficient storage (helmholtz_kernel<float, 1, 1, bf16>, 6 levels per
-# thread), the bulk of the FP32 operator launches of a mixed step.
+# this workload, scaled by the same counters of a plain device copy of known
+# size (read factor 2.00, write factor 1.00); profiles/sisl_step_lam256x256x80.md.
+# FP32 operator on the FP16 basis column with bfloat16 coefficients
+# (helmholtz2_kernel<float, 4, bf16>, the inner steps after the first of each
+# cycle: the bulk of the operator launches of a mixed step), and the launch
+# with the fused basis normalisation (the first step of each cycle).
 MEASURED_TRAFFIC = {
-    "helmholtz_kernel (operator apply A) [fp32]": (124677 * 2.0 + 152270 * 1.0) * 1024,
+    "helmholtz_kernel on the FP16 basis column (operator apply A) [fp32]": (99172 * 2.0 + 103075 * 1.0) * 1024,
+    "helmholtz_kernel (operator apply A) [fp32]": (129684 * 2.0 + 154647 * 1.0) * 1024,
 }
 
 
@@ -152,10 +156,13 @@ def _roofline(table):
 def attainable(case, reps=20):
     """Attainable HBM rates on this device, timed with HIP events on the
     launch stream: a 1 GiB copy and a 1 GiB read with 16-byte accesses, and
-    the inner operator launch's access mix alone (its 18 input and 10 output
-    arrays, each touched once at the thread's own column, in its tiles and
+    the inner operator launches' access mixes alone (their input and output
+    arrays, each touched once at the thread's own column, in their tiles and
     level chunks, no stencil or arithmetic: csrc/probe.hip) against the
-    algorithmic bytes of that launch."""
+    algorithmic bytes of those launches: the launch on the stored FP16 basis
+    column (5 FP16 inputs, 13 bfloat16 coefficient arrays, 5 FP32 outputs)
+    and the launch with the fused basis normalisation (FP32 inputs and 5 FP16
+    basis outputs besides)."""
     import ctypes as C
     import torch
     from suetes_amd import _lib
@@ -163,6 +170,7 @@ def attainable(case, reps=20):
     vp = C.c_void_p
     L.sisl_probe_stream.argtypes = [vp, vp, C.c_int64, C.c_int32, vp]
     L.sisl_probe_operator_streams_f32.argtypes = [vp, vp, vp, vp, C.c_int32, vp]
+    L.sisl_probe_operator_streams_h16_f32.argtypes = [vp, vp, vp, C.c_int32, vp]
     s = _lib.stream_ptr()
 
     def timed(fn):
@@ -191,15 +199,23 @@ def attainable(case, reps=20):
     nu, nv = (g.nx + 1) * g.ny * g.nz, g.nx * (g.ny + 1) * g.nz
     nw, nm = g.nx * g.ny * (g.nz + 1), g.nx * g.ny * g.nz
     alg = 4 * 2 * n + 2 * (3 * nu + 3 * nv + 6 * nw + nm) + 2 * n
-    best = None
+    alg16 = (2 + 4) * n + 2 * (3 * nu + 3 * nv + 6 * nw + nm)
+    x16 = x.half()
+    best = best16 = None
     for kc in (3, 6, 11):
         t = timed(lambda: L.sisl_probe_operator_streams_f32(C.byref(ops.struct), x.data_ptr(), y.data_ptr(),
                                                             bas.data_ptr(), kc, s))
         best = t if best is None else min(best, t)
+        t = timed(lambda: L.sisl_probe_operator_streams_h16_f32(C.byref(ops.struct), x16.data_ptr(),
+                                                                y.data_ptr(), kc, s))
+        best16 = t if best16 is None else min(best16, t)
     return {"copy_GBs": 2 * nb / t_copy / 1e9, "read_GBs": nb / t_read / 1e9,
-            "operator_access_mix_GBs": alg / best / 1e9, "operator_access_mix_us": best * 1e6,
+            "operator_access_mix_GBs": alg16 / best16 / 1e9, "operator_access_mix_us": best16 * 1e6,
+            "operator_access_mix_fused_GBs": alg / best / 1e9, "operator_access_mix_fused_us": best * 1e6,
             "method": "HIP events around 20 launches of each probe (csrc/probe.hip); the access-mix "
-                      "rate is the operator launch's algorithmic bytes over the probe's time"}
+                      "rate is the operator launch's algorithmic bytes over the probe's time "
+                      "(operator_access_mix: the launch on the FP16 basis column; _fused: the launch "
+                      "with the fused basis normalisation)"}
 
 
 def operator_shapes(case):
@@ -210,8 +226,10 @@ def operator_shapes(case):
     L = _lib.lib()
     g = case.stepper.implicit_solver.op64.struct.g
     out = {}
-    for name, fp32, var in (("fp32 A, fused normalisation, bf16 coefficients (inner forward)", 1, 6),
-                            ("fp32 A^T, fused normalisation (inner transposed)", 1, 3),
+    for name, fp32, var in (("fp32 A on the FP16 basis column, bf16 coefficients (inner forward)", 1, 14),
+                            ("fp32 A^T on the FP16 basis column (inner transposed)", 1, 9),
+                            ("fp32 A, fused normalisation, bf16 coefficients (inner forward, first step)", 1, 6),
+                            ("fp32 A^T, fused normalisation (inner transposed, first step)", 1, 3),
                             ("fp64 A (FP64 solver; refinement residual)", 0, 0),
                             ("fp64 A^T", 0, 1)):
         kc, un = C.c_int32(), C.c_int32()
@@ -713,8 +731,10 @@ def main():
         log(f"mixed vs fp64 after {args.err_steps} steps: {err_long['max_rel_l2']:.2e}")
     roof, table = instrumented_step(cmx, s_mx)
     roof["attainable"] = attainable(cmx)
-    if roof["kernel"].startswith("helmholtz_kernel") and roof["kernel"].endswith("[fp32]"):
+    if roof["kernel"].startswith("helmholtz_kernel on the FP16") and roof["kernel"].endswith("[fp32]"):
         roof["frac_of_access_mix"] = roof["achieved"] / roof["attainable"]["operator_access_mix_GBs"]
+    elif roof["kernel"].startswith("helmholtz_kernel") and roof["kernel"].endswith("[fp32]"):
+        roof["frac_of_access_mix"] = roof["achieved"] / roof["attainable"]["operator_access_mix_fused_GBs"]
     log(f"dominant kernel {roof['kernel']}: {roof['achieved']:.0f} GB/s; attainable: copy "
         f"{roof['attainable']['copy_GBs']:.0f}, its access mix {roof['attainable']['operator_access_mix_GBs']:.0f}")
     adj = None

@@ -501,12 +501,13 @@ int sisl_set_graphs(int32_t on);
 /* ---- kernel timing (measurement runs) ------------------------------------ */
 
 /* Timed kernels: operator apply, column preconditioner, the Krylov
- * projection / update / normalisation passes, the tricubic gather, and the
- * transposes of the operator, the preconditioner and the gather. */
+ * projection / update / normalisation passes, the tricubic gather, the
+ * transposes of the operator, the preconditioner and the gather, and the
+ * operator and its transpose applied to a stored FP16 basis column. */
 enum SislTimedKernel {
   SISL_T_HELMHOLTZ = 0, SISL_T_PRECOND = 1, SISL_T_KRYLOV_DOTS = 2, SISL_T_KRYLOV_UPDATE = 3,
   SISL_T_KRYLOV_NORMALIZE = 4, SISL_T_TRICUBIC = 5, SISL_T_HELMHOLTZ_T = 6, SISL_T_PRECOND_T = 7,
-  SISL_T_TRICUBIC_ADJ = 8, SISL_T_COUNT = 9
+  SISL_T_TRICUBIC_ADJ = 8, SISL_T_HELMHOLTZ_H16 = 9, SISL_T_HELMHOLTZ_T_H16 = 10, SISL_T_COUNT = 11
 };
 /* on = 1: clear the record and bracket every later launch of a timed kernel
  * with HIP events on its stream; on = 0: stop recording (the record stays).
@@ -526,10 +527,14 @@ int sisl_timer_read(int32_t kernel, int32_t fp32, double *total_ms, int64_t *lau
  * solver's operator launch (the FP32 operator with bfloat16 3-D
  * coefficients and the fused FP16 basis column) without its stencil or
  * arithmetic: each input array read once at the thread's own column, the
- * same outputs, tiles (64 x 4), level chunk and tile order. */
+ * same outputs, tiles (64 x 4), level chunk and tile order.
+ * sisl_probe_operator_streams_h16_f32: the same for the launch on a stored
+ * FP16 basis column (x16: FP16 input arrays; no basis outputs). */
 int sisl_probe_stream(const void *in, void *out, int64_t bytes, int32_t mode, void *stream);
 int sisl_probe_operator_streams_f32(const SislOperator *op, const float *x, float *y, void *basis,
                                     int32_t chunk, void *stream);
+int sisl_probe_operator_streams_h16_f32(const SislOperator *op, const void *x16, float *y, int32_t chunk,
+                                        void *stream);
 
 /* ---- batched ensemble (one launch per kernel over members) --------------- */
 

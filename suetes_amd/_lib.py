@@ -37,12 +37,13 @@ EXPORTED = [
     "sisl_tricubic_limited_adj_corners_f64", "sisl_tricubic_limited_adj_corners_f32",
     "sisl_helmholtz_apply_fused_f32", "sisl_helmholtz_apply_h16_f32", "sisl_probe_stream", "sisl_probe_operator_streams_f32",
     "sisl_tricubic2_f64", "sisl_tricubic2_f32", "sisl_tricubic_mixed", "sisl_tricubic2_mixed",
-    "sisl_gmres_batched_outcome", "sisl_get_operator_shape",
+    "sisl_gmres_batched_outcome", "sisl_get_operator_shape", "sisl_probe_operator_streams_h16_f32",
 ]
 
 # timed kernels (enum SislTimedKernel)
 TIMED = {"helmholtz": 0, "precond": 1, "krylov_dots": 2, "krylov_update": 3,
-         "krylov_normalize": 4, "tricubic": 5, "helmholtz_T": 6, "precond_T": 7, "tricubic_adj": 8}
+         "krylov_normalize": 4, "tricubic": 5, "helmholtz_T": 6, "precond_T": 7, "tricubic_adj": 8,
+         "helmholtz_h16": 9, "helmholtz_T_h16": 10}
 
 
 class SislGrid(C.Structure):
@@ -143,6 +144,7 @@ def lib():
         L.sisl_helmholtz_apply_h16_f32.argtypes = [C.POINTER(SislOperator), vp, vp, vp, i32, i32, vp]
         L.sisl_probe_stream.argtypes = [vp, vp, i64, i32, vp]
         L.sisl_probe_operator_streams_f32.argtypes = [C.POINTER(SislOperator), vp, vp, vp, i32, vp]
+        L.sisl_probe_operator_streams_h16_f32.argtypes = [C.POINTER(SislOperator), vp, vp, i32, vp]
         L.sisl_set_graphs.argtypes = [i32]
         L.sisl_timer_enable.argtypes = [i32]
         L.sisl_timer_read.argtypes = [i32, i32, C.POINTER(d), C.POINTER(i64), C.POINTER(i64),

@@ -1452,7 +1452,8 @@ int launch_A(const OpT<T> &o, const T *x, T *y, hipStream_t st, const int *stop,
   const double bytes = ((double)sizeof(T) + si) * s.n + sc * (3.0 * s.u.n() + 3.0 * s.v.n() + 6.0 * s.w.n() + s.m.n()) +
                        (f.nb == 1 ? 2.0 : f.nb == 2 ? (double)sizeof(T) : 0.0) * s.n +
                        (f.res ? (double)sizeof(T) * s.n : 0.0);
-  const int tid = timer_begin(SISL_T_HELMHOLTZ, sizeof(T) == 4, bytes, stop, nullptr, st);
+  const int tid = timer_begin(f.nb == 4 ? SISL_T_HELMHOLTZ_H16 : SISL_T_HELMHOLTZ, sizeof(T) == 4, bytes, stop,
+                              nullptr, st);
   const int rc = launch_A_shape(o, x, y, st, stop, sh, f);
   timer_end(tid, st);
   if (rc) return rc;
@@ -1531,7 +1532,8 @@ int launch_AT(const OpT<T> &o, const T *y, T *x, T * /*scratch: unused*/, hipStr
   const double bytes = ((double)sizeof(T) + si) * s.n + sc * (3.0 * s.u.n() + 3.0 * s.v.n() + 6.0 * s.w.n() + s.m.n()) +
                        (f.nb == 1 ? 2.0 : f.nb == 2 ? (double)sizeof(T) : 0.0) * s.n +
                        (f.res ? (double)sizeof(T) * s.n : 0.0);
-  const int tid = timer_begin(SISL_T_HELMHOLTZ_T, sizeof(T) == 4, bytes, stop, nullptr, st);
+  const int tid = timer_begin(f.nb == 4 ? SISL_T_HELMHOLTZ_T_H16 : SISL_T_HELMHOLTZ_T, sizeof(T) == 4, bytes, stop,
+                              nullptr, st);
   const int rc = launch_AT_shape(o, y, x, st, stop, sh, f);
   timer_end(tid, st);
   if (rc) return rc;

@@ -11,7 +11,9 @@
 //   5 FP16 basis outputs, the same 64 x 4 tiles, level chunk and XCD tile
 //   order.  Its rate is the ceiling for the operator's access mix (as many
 //   streams, as many bytes, the same store granularity), with no neighbour
-//   loads and no arithmetic.
+//   loads and no arithmetic.  With TI = FP16 and no basis outputs it is the
+//   mix of the launch on a stored FP16 basis column (helmholtz kernels with
+//   NB = 4).
 #include "common.h"
 
 namespace sisl {
@@ -38,7 +40,8 @@ constexpr int kPX = 64, kPY = 4;
 
 __device__ __forceinline__ float bfl(const bf16_t *b, uint32_t q) { return __uint_as_float((uint32_t)b[q] << 16); }
 
-__global__ void __launch_bounds__(256) operator_streams_kernel(OpT<float> o, const float *__restrict__ x,
+template <typename TI>
+__global__ void __launch_bounds__(256) operator_streams_kernel(OpT<float> o, const TI *__restrict__ x,
                                                               float *__restrict__ y, _Float16 *__restrict__ bas,
                                                               int kchunk) {
   const Stag &s = o.s;
@@ -49,9 +52,11 @@ __global__ void __launch_bounds__(256) operator_streams_kernel(OpT<float> o, con
   const int i = bx * kPX + threadIdx.x, j = by * kPY + threadIdx.y;
   if (i >= nx || j >= ny) return;
   const int k0 = bz * kchunk, k1 = min(k0 + kchunk, nz + 1);
-  const float *U = x + s.ou, *V = x + s.ov, *W = x + s.ow, *P = x + s.om, *E = x + s.oe;
+  const TI *U = x + s.ou, *V = x + s.ov, *W = x + s.ow, *P = x + s.om, *E = x + s.oe;
   float *YU = y + s.ou, *YV = y + s.ov, *YW = y + s.ow, *YP = y + s.om, *YE = y + s.oe;
-  _Float16 *BU = bas + s.ou, *BV = bas + s.ov, *BW = bas + s.ow, *BP = bas + s.om, *BE = bas + s.oe;
+  _Float16 *BU = nullptr, *BV = nullptr, *BW = nullptr, *BP = nullptr, *BE = nullptr;
+  if (bas) { BU = bas + s.ou; BV = bas + s.ov; BW = bas + s.ow; BP = bas + s.om; BE = bas + s.oe; }
+  auto in = [&](const TI *p, uint32_t q) -> float { return (float)ldg(p, q); };
   const bf16_t *c[13] = {reinterpret_cast<const bf16_t *>(o.cua), reinterpret_cast<const bf16_t *>(o.zxu),
                          reinterpret_cast<const bf16_t *>(o.cx),  reinterpret_cast<const bf16_t *>(o.cva),
                          reinterpret_cast<const bf16_t *>(o.zev), reinterpret_cast<const bf16_t *>(o.cy),
@@ -63,24 +68,28 @@ __global__ void __launch_bounds__(256) operator_streams_kernel(OpT<float> o, con
   const uint32_t mc = (uint32_t)j * nx + i, uc = (uint32_t)j * (nx + 1) + i, vc = (uint32_t)j * nx + i;
   for (int k = k0; k < k1; ++k) {
     const uint32_t wk = mc + (uint32_t)k * pm;
-    float ww = ldg(W, wk) + ldg(E, wk);
+    float ww = in(W, wk) + in(E, wk);
 #pragma unroll
     for (int q = 6; q < 12; ++q) ww += bfl(c[q], wk);
     stg(YW, wk, ww);
     stg(YE, wk, ww * 0.5f);
-    stg(BW, wk, (_Float16)ww);
-    stg(BE, wk, (_Float16)(ww * 0.5f));
+    if (bas) {
+      stg(BW, wk, (_Float16)ww);
+      stg(BE, wk, (_Float16)(ww * 0.5f));
+    }
     if (k == nz) break;
     const uint32_t au = uc + (uint32_t)k * pu, av = vc + (uint32_t)k * pv;
-    const float uu = ldg(U, au) + bfl(c[0], au) + bfl(c[1], au) + bfl(c[2], au);
-    const float vv = ldg(V, av) + bfl(c[3], av) + bfl(c[4], av) + bfl(c[5], av);
-    const float pp = ldg(P, wk) + bfl(c[12], wk);
+    const float uu = in(U, au) + bfl(c[0], au) + bfl(c[1], au) + bfl(c[2], au);
+    const float vv = in(V, av) + bfl(c[3], av) + bfl(c[4], av) + bfl(c[5], av);
+    const float pp = in(P, wk) + bfl(c[12], wk);
     stg(YU, au, uu);
     stg(YV, av, vv);
     stg(YP, wk, pp);
-    stg(BU, au, (_Float16)uu);
-    stg(BV, av, (_Float16)vv);
-    stg(BP, wk, (_Float16)pp);
+    if (bas) {
+      stg(BU, au, (_Float16)uu);
+      stg(BV, av, (_Float16)vv);
+      stg(BP, wk, (_Float16)pp);
+    }
   }
 }
 
@@ -115,8 +124,25 @@ int sisl_probe_operator_streams_f32(const SislOperator *op, const float *x, floa
   const sisl::Stag &st = o.s;
   const dim3 g((unsigned)(((st.nx + sisl::kPX - 1) / sisl::kPX) * ((st.ny + sisl::kPY - 1) / sisl::kPY) *
                           ((st.nz + 1 + chunk - 1) / chunk)));
-  hipLaunchKernelGGL(sisl::operator_streams_kernel, g, dim3(sisl::kPX, sisl::kPY), 0, sisl::as_stream(s), o, x,
-                     y, static_cast<_Float16 *>(basis), (int)chunk);
+  hipLaunchKernelGGL(sisl::operator_streams_kernel<float>, g, dim3(sisl::kPX, sisl::kPY), 0, sisl::as_stream(s), o,
+                     x, y, static_cast<_Float16 *>(basis), (int)chunk);
+  SISL_CHECK_LAUNCH();
+  return SISL_OK;
+}
+
+// The access mix of the operator launch on a stored FP16 basis column: x16
+// holds the FP16 input arrays; no basis outputs.
+int sisl_probe_operator_streams_h16_f32(const SislOperator *op, const void *x16, float *y, int32_t chunk,
+                                        void *s) {
+  if (!op || !x16 || !y || chunk < 1) return SISL_ERR_ARG;
+  auto o = sisl::OpT<float>::make(*op);
+  if (!o.complete()) return SISL_ERR_ARG;
+  const sisl::Stag &st = o.s;
+  const dim3 g((unsigned)(((st.nx + sisl::kPX - 1) / sisl::kPX) * ((st.ny + sisl::kPY - 1) / sisl::kPY) *
+                          ((st.nz + 1 + chunk - 1) / chunk)));
+  hipLaunchKernelGGL(sisl::operator_streams_kernel<_Float16>, g, dim3(sisl::kPX, sisl::kPY), 0,
+                     sisl::as_stream(s), o, static_cast<const _Float16 *>(x16), y, (_Float16 *)nullptr,
+                     (int)chunk);
   SISL_CHECK_LAUNCH();
   return SISL_OK;
 }
