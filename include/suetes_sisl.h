@@ -286,6 +286,13 @@ typedef struct SislGmresParams {
                               stores the next column (one scale per column)
                               instead of writing an FP32 Arnoldi vector back
                               (0: FP32 Arnoldi vectors between the passes)   */
+  const struct SislOperator *op_refine; /* mixed solver, optional (NULL: the
+                              refinement applies M in FP64): an operator with
+                              FP32 coefficient arrays; the refinement forms
+                              b - A x in FP64, stores it rounded to FP32 and
+                              applies M (M^T) of this operator in FP32; the
+                              result is the stopping test's preconditioned
+                              residual and the next cycle's start vector      */
 } SislGmresParams;
 
 typedef struct SislGmresStats {

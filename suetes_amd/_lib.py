@@ -65,7 +65,7 @@ class SislGmresParams(C.Structure):
                 ("maxiter", C.c_int32), ("transpose", C.c_int32), ("early_exit", C.c_int32),
                 ("inner_rtol", C.c_double), ("max_outer", C.c_int32), ("basis_fp16", C.c_int32),
                 ("coef_bf16", C.c_int32), ("inner_single_cgs", C.c_int32),
-                ("inner_basis_operand", C.c_int32)]
+                ("inner_basis_operand", C.c_int32), ("op_refine", C.c_void_p)]
 
 
 class SislGmresStats(C.Structure):

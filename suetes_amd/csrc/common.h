@@ -191,7 +191,7 @@ struct Fuse {
   int nb = 0;
   double vs = 1.0;
   int cb = 0;  // 1: the 3-D coefficient arrays are bfloat16 (FP32 operator only)
-  const void *res = nullptr;  // packed vector b: store b - A x (not with nb)
+  const void *res = nullptr;  // packed vector b: store b - A x (nb 0; nb 5: in FP32 into vout)
 };
 template <typename T>
 int launch_A(const OpT<T> &op, const T *x, T *y, hipStream_t s, const int *stop = nullptr,

@@ -37,6 +37,9 @@ __device__ __forceinline__ int blocks_A_dev(const Stag &s) {
 //
 // NB = 3 (FP64 only) is the residual epilogue of the FP64 refinement instead:
 // every output is stored as res - (A x), res the packed right-hand side.
+// NB = 5 (FP64 only): the same residual, formed in FP64 and stored rounded to
+// FP32 into vout (float, packed layout) instead of y -- the input of the
+// FP32 preconditioner that forms the next inner cycle's start vector.
 template <typename T, int NB> struct BasisT { using type = T; };
 template <typename T> struct BasisT<T, 1> { using type = _Float16; };
 // NB = 4 (FP32 only): the input is a basis column stored in FP16 (read and
@@ -95,8 +98,9 @@ __device__ __forceinline__ void helmholtz_body(const OpT<T> &o, const T *__restr
   FUSED_NORMALISATION_SETUP
   // output store (NB = 3: the residual res - A x)
   auto put = [&](T *Y_, uint32_t q, T v) {
-    if constexpr (NB == 3) v = ldg(res + (Y_ - y), q) - v;
-    stg(Y_, q, so(v));
+    if constexpr (NB == 3 || NB == 5) v = ldg(res + (Y_ - y), q) - v;
+    if constexpr (NB == 5) stg(static_cast<float *>(vout) + (Y_ - y), q, (float)v);
+    else stg(Y_, q, so(v));
   };
   const bool uform = s.px || i > 0, vform = s.py || j > 0;  // else face 0 is the identity
   const T mudx = ldg(o.mudx, (uint32_t)s.u2(i, j)), afu = ldg(o.afu, (uint32_t)s.u2(i, j));
@@ -351,17 +355,19 @@ __device__ __forceinline__ void helmholtz2_body(const OpT<T> &o, const T *__rest
   T *YU = y + s.ou, *YV = y + s.ov, *YW = y + s.ow, *YP = y + s.om, *YE = y + s.oe;
   FUSED_NORMALISATION_SETUP
   auto put = [&](T *Y_, uint32_t q, T v) {
-    if constexpr (NB == 3) v = ldg(res + (Y_ - y), q) - v;
-    stg(Y_, q, so(v));
+    if constexpr (NB == 3 || NB == 5) v = ldg(res + (Y_ - y), q) - v;
+    if constexpr (NB == 5) stg(static_cast<float *>(vout) + (Y_ - y), q, (float)v);
+    else stg(Y_, q, so(v));
   };
   auto put2 = [&](T *Y_, uint32_t q, T va, T vb) {
-    if constexpr (NB == 3) {
+    if constexpr (NB == 3 || NB == 5) {
       T ra, rb;
       ldg2(res + (Y_ - y), q, ra, rb);
       va = ra - va;
       vb = rb - vb;
     }
-    stg2(Y_, q, so(va), so(vb));
+    if constexpr (NB == 5) stg2<float>(static_cast<float *>(vout) + (Y_ - y), q, (float)va, (float)vb);
+    else stg2(Y_, q, so(va), so(vb));
   };
   auto bput2 = [&](BT *b, uint32_t q, T va, T vb) {
     if constexpr (kNorm) stb2<BT>(b, q, (BT)(va * bsc), (BT)(vb * bsc));
@@ -673,8 +679,9 @@ __device__ __forceinline__ void helmholtz_T_body(const OpT<T> &o, const T *__res
   FUSED_NORMALISATION_SETUP
   // output store (NB = 3: the residual res - A^T y)
   auto put = [&](T *X_, uint32_t q, T v) {
-    if constexpr (NB == 3) v = ldg(res + (X_ - x), q) - v;
-    stg(X_, q, so(v));
+    if constexpr (NB == 3 || NB == 5) v = ldg(res + (X_ - x), q) - v;
+    if constexpr (NB == 5) stg(static_cast<float *>(vout) + (X_ - x), q, (float)v);
+    else stg(X_, q, so(v));
   };
   const bool px = s.px, py = s.py;
   const bool hl = i >= 1, hd = j >= 1;  // the cell to the left / below exists
@@ -926,17 +933,19 @@ __device__ __forceinline__ void helmholtz_T2_body(const OpT<T> &o, const T *__re
   T *XU = x + s.ou, *XV = x + s.ov, *XW = x + s.ow, *XP = x + s.om, *XE = x + s.oe;
   FUSED_NORMALISATION_SETUP
   auto put = [&](T *X_, uint32_t q, T v) {
-    if constexpr (NB == 3) v = ldg(res + (X_ - x), q) - v;
-    stg(X_, q, so(v));
+    if constexpr (NB == 3 || NB == 5) v = ldg(res + (X_ - x), q) - v;
+    if constexpr (NB == 5) stg(static_cast<float *>(vout) + (X_ - x), q, (float)v);
+    else stg(X_, q, so(v));
   };
   auto put2 = [&](T *X_, uint32_t q, T va, T vb) {
-    if constexpr (NB == 3) {
+    if constexpr (NB == 3 || NB == 5) {
       T ra, rb;
       ldg2(res + (X_ - x), q, ra, rb);
       va = ra - va;
       vb = rb - vb;
     }
-    stg2(X_, q, so(va), so(vb));
+    if constexpr (NB == 5) stg2<float>(static_cast<float *>(vout) + (X_ - x), q, (float)va, (float)vb);
+    else stg2(X_, q, so(va), so(vb));
   };
   const bool px = s.px, py = s.py;
   const bool hl0 = i >= 1, hd = j >= 1;  // column i+1 always has a left cell (column i)
@@ -1330,8 +1339,13 @@ template <template <typename, int, int, typename> class K, typename T, typename 
 static int launch_shape_tc(dim3 g, const OpT<T> &o, const T *x, T *y, hipStream_t st, const int *stop, int kc,
                            int unr, const Fuse &f) {
   if (f.res) {
-    if (f.nb) return SISL_ERR_ARG;
     if constexpr (sizeof(T) == 8) {
+      if (f.nb == 5) {
+        if (unr == 2) K<T, 2, 5, TC>::go(g, st, o, x, y, kc, stop, f);
+        else K<T, 1, 5, TC>::go(g, st, o, x, y, kc, stop, f);
+        return SISL_OK;
+      }
+      if (f.nb) return SISL_ERR_ARG;
       if (unr == 2) K<T, 2, 3, TC>::go(g, st, o, x, y, kc, stop, f);
       else K<T, 1, 3, TC>::go(g, st, o, x, y, kc, stop, f);
       return SISL_OK;
@@ -1385,8 +1399,12 @@ template <typename T, typename TC>
 static int launch2_tc(dim3 g, const OpT<T> &o, const T *x, T *y, hipStream_t st, const int *stop, int kc,
                       const Fuse &f) {
   if (f.res) {
-    if (f.nb) return SISL_ERR_ARG;
     if constexpr (sizeof(T) == 8) {
+      if (f.nb == 5) {
+        go2<T, 5, TC>(g, st, o, x, y, kc, stop, f);
+        return SISL_OK;
+      }
+      if (f.nb) return SISL_ERR_ARG;
       go2<T, 3, TC>(g, st, o, x, y, kc, stop, f);
       return SISL_OK;
     }
@@ -1418,6 +1436,7 @@ static int launch2_tc(dim3 g, const OpT<T> &o, const T *x, T *y, hipStream_t st,
 static bool fuse_args_ok(const Fuse &f) {
   if ((f.nb == 1 || f.nb == 2 || f.nb == 4) && !f.inv) return false;
   if ((f.nb == 1 || f.nb == 2) && !f.vout) return false;
+  if (f.nb == 5) return f.res && f.vout;  // residual stored in FP32
   return f.nb == 0 || f.nb == 1 || f.nb == 2 || f.nb == 4;
 }
 
@@ -1445,11 +1464,12 @@ template <typename T>
 int launch_A(const OpT<T> &o, const T *x, T *y, hipStream_t st, const int *stop, const Fuse *fuse) {
   if ((uint64_t)o.s.n * sizeof(T) >= (1ull << 32)) return SISL_ERR_ARG;  // 32-bit offsets
   const Fuse f = fuse ? *fuse : Fuse{};
-  const AShape sh = a_shape(o.s, (int)sizeof(T), (f.nb == 4 ? 8 : f.nb ? 2 : 0) + (f.cb ? 4 : 0));
+  const AShape sh = a_shape(o.s, (int)sizeof(T), (f.nb == 4 ? 8 : f.nb && f.nb != 5 ? 2 : 0) + (f.cb ? 4 : 0));
   const Stag &s = o.s;
   const double sc = f.cb ? 2.0 : (double)sizeof(T);  // bytes per coefficient element
   const double si = f.nb == 4 ? 2.0 : (double)sizeof(T);  // bytes per input element
-  const double bytes = ((double)sizeof(T) + si) * s.n + sc * (3.0 * s.u.n() + 3.0 * s.v.n() + 6.0 * s.w.n() + s.m.n()) +
+  const double so = f.nb == 5 ? 4.0 : (double)sizeof(T);  // bytes per output element
+  const double bytes = (so + si) * s.n + sc * (3.0 * s.u.n() + 3.0 * s.v.n() + 6.0 * s.w.n() + s.m.n()) +
                        (f.nb == 1 ? 2.0 : f.nb == 2 ? (double)sizeof(T) : 0.0) * s.n +
                        (f.res ? (double)sizeof(T) * s.n : 0.0);
   const int tid = timer_begin(f.nb == 4 ? SISL_T_HELMHOLTZ_H16 : SISL_T_HELMHOLTZ, sizeof(T) == 4, bytes, stop,
@@ -1471,8 +1491,12 @@ template <typename T, typename TC>
 static int launchT2_tc(dim3 g, const OpT<T> &o, const T *y, T *x, hipStream_t st, const int *stop, int kc,
                        const Fuse &f) {
   if (f.res) {
-    if (f.nb) return SISL_ERR_ARG;
     if constexpr (sizeof(T) == 8) {
+      if (f.nb == 5) {
+        goT2<T, 5, TC>(g, st, o, y, x, kc, stop, f);
+        return SISL_OK;
+      }
+      if (f.nb) return SISL_ERR_ARG;
       goT2<T, 3, TC>(g, st, o, y, x, kc, stop, f);
       return SISL_OK;
     }
@@ -1524,12 +1548,13 @@ int launch_AT(const OpT<T> &o, const T *y, T *x, T * /*scratch: unused*/, hipStr
               const Fuse *fuse) {
   if ((uint64_t)o.s.n * sizeof(T) >= (1ull << 32)) return SISL_ERR_ARG;  // 32-bit offsets
   const Fuse f = fuse ? *fuse : Fuse{};
-  const AShape sh = a_shape(o.s, (int)sizeof(T), (f.nb == 4 ? 9 : f.nb ? 3 : 1) + (f.cb ? 4 : 0));
+  const AShape sh = a_shape(o.s, (int)sizeof(T), (f.nb == 4 ? 9 : f.nb && f.nb != 5 ? 3 : 1) + (f.cb ? 4 : 0));
   const Stag &s = o.s;
   // same arrays as the forward operator: y in, x out, 13 coefficient arrays (+ the basis column)
   const double sc = f.cb ? 2.0 : (double)sizeof(T);  // bytes per coefficient element
   const double si = f.nb == 4 ? 2.0 : (double)sizeof(T);  // bytes per input element
-  const double bytes = ((double)sizeof(T) + si) * s.n + sc * (3.0 * s.u.n() + 3.0 * s.v.n() + 6.0 * s.w.n() + s.m.n()) +
+  const double so = f.nb == 5 ? 4.0 : (double)sizeof(T);  // bytes per output element
+  const double bytes = (so + si) * s.n + sc * (3.0 * s.u.n() + 3.0 * s.v.n() + 6.0 * s.w.n() + s.m.n()) +
                        (f.nb == 1 ? 2.0 : f.nb == 2 ? (double)sizeof(T) : 0.0) * s.n +
                        (f.res ? (double)sizeof(T) * s.n : 0.0);
   const int tid = timer_begin(f.nb == 4 ? SISL_T_HELMHOLTZ_T_H16 : SISL_T_HELMHOLTZ_T, sizeof(T) == 4, bytes, stop,

@@ -153,6 +153,13 @@ __device__ __forceinline__ T tricubic_point(T tx, T ty, T tz, int lim, Fetch F) 
 // the FP32 results widened on the store, so every variant gives the same
 // bits as one field at a time in T.
 constexpr int kIX = 64, kIY = 4, kSX = kIX + 8, kSY = kIY + 5, kRing = 6;
+// Elements of one staged plane, padded to a multiple of 64 dwords: the lanes
+// of a wave read the same tap at their own columns, and lanes whose stencils
+// start on different planes (vertical departures of either sign) then hit
+// the banks of their columns instead of colliding at a shifted bank.
+constexpr int kPlane = ((kSY * kSX + 63) / 64) * 64;
+// Staged elements per thread and plane.
+constexpr int kStageN = (kSY * kSX + kIX * kIY - 1) / (kIX * kIY);
 
 template <typename TF, int NF>
 struct FieldSet {
@@ -164,7 +171,7 @@ template <typename T, typename TF, int NF>
 __device__ __forceinline__ void tricubic_multi_body(const Dims &d, const FieldSet<TF, NF> &fs,
                                                     const T *__restrict__ co, int lim, int kchunk, int blk,
                                                     int nblk) {
-  __shared__ T ring[NF][kRing][kSY][kSX];
+  __shared__ T ring[NF][kRing * kPlane];
   const int nx = d.nx, ny = d.ny, nz = d.nz;
   const int nbx = (nx + kIX - 1) / kIX, nby = (ny + kIY - 1) / kIY;
   const int tile = xcd_tile(blk, nblk);
@@ -176,15 +183,26 @@ __device__ __forceinline__ void tricubic_multi_body(const Dims &d, const FieldSe
   const int k0 = bz * kchunk, k1 = min(k0 + kchunk, nz);
   const uint32_t plane = (uint32_t)nx * ny;
   const uint32_t n = plane * (uint32_t)nz;
+  // the clamped source offset within a level of each element this thread
+  // stages (the same for every level), -1 past the plane
+  int goff[kStageN];
+#pragma unroll
+  for (int r = 0; r < kStageN; ++r) {
+    const int q = tid + r * kIX * kIY;
+    const int ly = q / kSX, lx = q - ly * kSX;
+    const int gx = min(max(xs + lx, 0), nx - 1), gy = min(max(ys + ly, 0), ny - 1);
+    goff[r] = q < kSY * kSX ? gy * nx + gx : -1;
+  }
   auto stage = [&](int kk) {
     const int slot = ((kk % kRing) + kRing) % kRing;
     const uint32_t zoff = (uint32_t)min(max(kk, 0), nz - 1) * plane;
-    for (int q = tid; q < kSY * kSX; q += kIX * kIY) {
-      const int ly = q / kSX, lx = q - ly * kSX;
-      const int gx = min(max(xs + lx, 0), nx - 1), gy = min(max(ys + ly, 0), ny - 1);
 #pragma unroll
-      for (int fi = 0; fi < NF; ++fi) ring[fi][slot][ly][lx] = (T)ldg(fs.f[fi], zoff + (uint32_t)gy * nx + gx);
-    }
+    for (int r = 0; r < kStageN; ++r)
+      if (goff[r] >= 0) {
+#pragma unroll
+        for (int fi = 0; fi < NF; ++fi)
+          ring[fi][slot * kPlane + tid + r * kIX * kIY] = (T)ldg(fs.f[fi], zoff + (uint32_t)goff[r]);
+      }
   };
   for (int kk = k0 - 2; kk <= k0 + 2; ++kk) stage(kk);
   for (int k = k0; k < k1; ++k) {
@@ -221,7 +239,7 @@ __device__ __forceinline__ void tricubic_multi_body(const Dims &d, const FieldSe
         if (fi) __builtin_amdgcn_sched_barrier(0);
         const T *zb[4];
 #pragma unroll
-        for (int c = 0; c < 4; ++c) zb[c] = &ring[fi][sl[c]][y0 - 1 - ys][x0 - 1 - xs];
+        for (int c = 0; c < 4; ++c) zb[c] = &ring[fi][sl[c] * kPlane + (y0 - 1 - ys) * kSX + (x0 - 1 - xs)];
         r[fi] = tricubic_apply(wx, wy, wz, lim, [&](int a, int b, int c) { return zb[c][b * kSX + a]; });
       }
     } else {

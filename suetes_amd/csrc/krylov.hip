@@ -1328,9 +1328,16 @@ static int run_inner_cycle(Krylov<float, TB> &K, int R) {
 
 // Mixed: FP64 residual refinement around FP32 GMRES cycles with early exit;
 // TB is the inner basis storage type (float, or FP16 scaled by sqrt(n)).
+//
+// opr (optional, FP32 coefficients): the refinement stores its FP64 residual
+// b - A x rounded to FP32 (the operator launch's FP32 residual epilogue) and
+// applies the preconditioner in FP32 to it; z = M r then serves the stopping
+// test (its norm, relative error ~1e-7) and is the next cycle's start vector
+// as before.  Without it the refinement applies M in FP64.
 template <typename TB>
 static int gmres_mixed(const OpT<double> &op64, const OpT<float> &op32, const SislGmresParams &p,
-                       const double *b, double *x, char *ws, SislGmresStats *st, hipStream_t s) {
+                       const double *b, double *x, char *ws, SislGmresStats *st, hipStream_t s,
+                       const OpT<float> *opr = nullptr) {
   const int R = p.restart;
   const Layout L = layout(op64.s, R, true);
   const int64_t n = op64.s.n;
@@ -1362,21 +1369,37 @@ static int gmres_mixed(const OpT<double> &op64, const OpT<float> &op32, const Si
   const int max_outer = p.max_outer > 0 ? p.max_outer : p.maxiter;
   Fuse rb;  // the FP64 operator launch stores b - A x
   rb.res = b;
+  Fuse rb32 = rb;  // ... rounded to FP32 into K.t (free outside the inner cycles)
+  rb32.nb = 5;
+  rb32.vout = K.t;
   hipLaunchKernelGGL(zero_steps_total_kernel, dim3(1), dim3(1), 0, s, sm);
   int outer = 0;
   double rnorm = 0;
   for (;;) {
-    // r = b - A x (FP64, stored by the operator launch), z = M r (FP64)
-    if (!p.transpose) {
-      if ((rc = launch_A<double>(op64, x, t64, s, nullptr, &rb))) return rc;
+    if (opr) {
+      // r = b - A x (formed in FP64, stored in FP32), z = M r (FP32) into the
+      // next cycle's start vector, ||z||^2
+      if (!p.transpose) {
+        if ((rc = launch_A<double>(op64, x, t64, s, nullptr, &rb32))) return rc;
+        if ((rc = launch_M<float>(*opr, K.t, K.cur, s))) return rc;
+      } else {
+        if ((rc = launch_AT<double>(op64, x, t64, scr64, s, nullptr, &rb32))) return rc;
+        if ((rc = launch_MT<float>(*opr, K.t, K.cur, K.scr, s))) return rc;
+      }
+      norm2<float>(K.cur, n, partials, &sm->scal[2], s);
     } else {
-      if ((rc = launch_AT<double>(op64, x, t64, scr64, s, nullptr, &rb))) return rc;
+      // r = b - A x (FP64, stored by the operator launch), z = M r (FP64)
+      if (!p.transpose) {
+        if ((rc = launch_A<double>(op64, x, t64, s, nullptr, &rb))) return rc;
+      } else {
+        if ((rc = launch_AT<double>(op64, x, t64, scr64, s, nullptr, &rb))) return rc;
+      }
+      if ((rc = M64(t64, w64))) return rc;
+      // ||z||^2 and the FP32 start vector of the next cycle in one pass over z
+      hipLaunchKernelGGL(norm2_to_f32_kernel, dim3(kRedGrid), dim3(kBlock), 0, s, (const double *)w64, K.cur, n,
+                         partials);
+      reduce(partials, 1, &sm->scal[2], s);
     }
-    if ((rc = M64(t64, w64))) return rc;
-    // ||z||^2 and the FP32 start vector of the next cycle in one pass over z
-    hipLaunchKernelGGL(norm2_to_f32_kernel, dim3(kRedGrid), dim3(kBlock), 0, s, (const double *)w64, K.cur, n,
-                       partials);
-    reduce(partials, 1, &sm->scal[2], s);
     if ((rc = sync_copy(hs, sm->scal, sizeof(double) * 3, s))) return rc;
     rnorm = std::sqrt(hs[2]);
     if (!(rnorm > 2.220446049250313e-16) && !std::isnan(rnorm)) rnorm = 0.0;
@@ -2033,9 +2056,15 @@ int sisl_gmres_mixed(const SislOperator *op64, const SislOperator *op32, const S
   auto o64 = sisl::OpT<double>::make(*op64);
   auto o32 = sisl::OpT<float>::make(*op32);
   if (!o64.complete() || !o32.complete()) return SISL_ERR_ARG;
+  sisl::OpT<float> orf{};
+  if (p->op_refine) {
+    orf = sisl::OpT<float>::make(*p->op_refine);
+    if (!orf.complete() || orf.s.n != o64.s.n) return SISL_ERR_ARG;
+  }
+  const sisl::OpT<float> *opr = p->op_refine ? &orf : nullptr;
   if (p->basis_fp16)
-    return sisl::gmres_mixed<_Float16>(o64, o32, *p, b, x, (char *)ws, st, sisl::as_stream(stream));
-  return sisl::gmres_mixed<float>(o64, o32, *p, b, x, (char *)ws, st, sisl::as_stream(stream));
+    return sisl::gmres_mixed<_Float16>(o64, o32, *p, b, x, (char *)ws, st, sisl::as_stream(stream), opr);
+  return sisl::gmres_mixed<float>(o64, o32, *p, b, x, (char *)ws, st, sisl::as_stream(stream), opr);
 }
 
 int64_t sisl_gmres_batched_workspace_bytes(const SislGrid *grids, int32_t M, int32_t restart) {

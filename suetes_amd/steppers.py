@@ -67,8 +67,9 @@ def build_dynamical_core(core_type, grid, operators, constants, initial_state, p
     'fp32': storage of the FP32 inner Krylov basis) and
     ``inner_coefficients`` ('bf16' or 'fp32': storage of the inner
     operator's 3-D coefficient arrays), ``inner_gram_schmidt`` ('single'
-    or 'iterated') and ``inner_operand`` ('basis' or 'vector') of the mixed
-    solver, ``solver_early_exit``."""
+    or 'iterated'), ``inner_operand`` ('basis' or 'vector') and
+    ``refine_preconditioner`` ('fp32' or 'fp64': arithmetic of the
+    refinement's preconditioner) of the mixed solver, ``solver_early_exit``."""
     core = core_type.lower()
     if core == "split-explicit":
         return _build_split_explicit(grid, operators, constants, initial_state, physics_suite, interior_mask,
@@ -81,7 +82,7 @@ def build_dynamical_core(core_type, grid, operators, constants, initial_state, p
          "solver_restart": 100, "background_state": None, "precision": "fp64",
          "inner_rtol": 1e-4, "inner_restart": None, "inner_basis": "fp16",
          "inner_coefficients": "bf16", "inner_gram_schmidt": "single", "inner_operand": "basis",
-         "solver_early_exit": False}
+         "refine_preconditioner": "fp32", "solver_early_exit": False}
     p.update(kwargs)
     ref = p["background_state"] if p["background_state"] is not None else initial_state
     physics = Euler3D(grid, operators, constants, dt=p["dt"], initial_era5_state=ref,
@@ -106,6 +107,7 @@ def build_dynamical_core(core_type, grid, operators, constants, initial_state, p
                             inner_coefficients=p["inner_coefficients"],
                             inner_gram_schmidt=p["inner_gram_schmidt"],
                             inner_operand=p["inner_operand"],
+                            refine_preconditioner=p["refine_preconditioner"],
                             solver_early_exit=p["solver_early_exit"])
     return stepper, p["dt"]
 
@@ -474,7 +476,7 @@ class SemiImplicitSolver3D:
     def __init__(self, physics, dt, alpha=0.55, solver_tol=1e-4, solver_maxiter=20, solver_restart=20,
                  precision="fp64", inner_rtol=1e-4, early_exit=False, inner_restart=None,
                  inner_basis="fp16", inner_coefficients="bf16", inner_gram_schmidt="single",
-                 inner_operand="basis"):
+                 inner_operand="basis", refine_preconditioner="fp32"):
         self.physics, self.dt, self.alpha = physics, dt, alpha
         self.pi_scale = 1000.0
         self.solver_tol, self.solver_maxiter, self.solver_restart = solver_tol, solver_maxiter, solver_restart
@@ -495,6 +497,9 @@ class SemiImplicitSolver3D:
         if inner_operand not in ("basis", "vector"):
             raise ValueError("inner_operand must be 'basis' or 'vector'")
         self.inner_operand = inner_operand
+        if refine_preconditioner not in ("fp32", "fp64"):
+            raise ValueError("refine_preconditioner must be 'fp32' or 'fp64'")
+        self.refine_preconditioner = refine_preconditioner
         self.early_exit = early_exit
         self._bg_id = None
         self.last_stats = []
@@ -553,7 +558,15 @@ class SemiImplicitSolver3D:
                                     float(self.inner_rtol), self._max_outer(), int(self.inner_basis == "fp16"),
                                     int(self._bf16_inner(transpose)),
                                     int(self.precision == "mixed" and self.inner_gram_schmidt == "single"),
-                                    int(self.precision == "mixed" and self.inner_operand == "basis"))
+                                    int(self.precision == "mixed" and self.inner_operand == "basis"),
+                                    C.addressof(self.op32.struct) if self._fp32_refinement() else None)
+
+    def _fp32_refinement(self):
+        """The refinement applies the preconditioner in FP32 (FP32 coefficients)
+        to its FP64 residual rounded to FP32: the result only enters the
+        stopping test's norm and the next inner cycle's FP32 start vector, so
+        FP32 (relative error ~1e-7) replaces the FP64 pass and its conversion."""
+        return self.precision == "mixed" and self.refine_preconditioner == "fp32"
 
     def _bf16_inner(self, transpose):
         """bfloat16 coefficients for the forward solve only: the transposed
@@ -680,7 +693,8 @@ class SISLStepper3D:
     def __init__(self, physics, dt, alpha=0.55, use_limiter=False, use_checkpointing=False,
                  solver_tol=1e-4, solver_maxiter=20, solver_restart=20, precision="fp64",
                  inner_rtol=1e-4, solver_early_exit=False, inner_restart=None, inner_basis="fp16",
-                 inner_coefficients="bf16", inner_gram_schmidt="single", inner_operand="basis"):
+                 inner_coefficients="bf16", inner_gram_schmidt="single", inner_operand="basis",
+                 refine_preconditioner="fp32"):
         self.physics, self.dt, self.alpha = physics, dt, alpha
         self.use_limiter = use_limiter
         self.precision = precision
@@ -695,7 +709,8 @@ class SISLStepper3D:
                                                     inner_basis=inner_basis,
                                                     inner_coefficients=inner_coefficients,
                                                     inner_gram_schmidt=inner_gram_schmidt,
-                                                    inner_operand=inner_operand)
+                                                    inner_operand=inner_operand,
+                                                    refine_preconditioner=refine_preconditioner)
         self.use_checkpointing = use_checkpointing
         self.tracer_keys = physics.physics_suite.tracer_keys if physics.physics_suite is not None else []
         self._bg = None

@@ -230,7 +230,7 @@ def test_tricubic_plan_holder_releases_after_last_pending_adjoint():
 def test_mixed_solver_options_are_validated():
     from suetes_amd.steppers import SemiImplicitSolver3D
     for kw in ({"inner_gram_schmidt": "twice"}, {"inner_coefficients": "fp16"}, {"inner_basis": "bf16"},
-               {"inner_operand": "fp16"}):
+               {"inner_operand": "fp16"}, {"refine_preconditioner": "bf16"}):
         with pytest.raises(ValueError):
             SemiImplicitSolver3D(None, 1.0, precision="mixed", **kw)
 
