@@ -391,6 +391,21 @@ int sisl_explicit_f64(const SislTendency *t, const double *u, const double *v, c
                       const double *pi, const double *th, const double *eta, const double *lap2,
                       double *u_in, double *v_in, double *w_in, double *pi_in, double *th_in,
                       double *res, double *tth, double *x0, double *tend, void *stream);
+/* Reverse mode of the explicit stage above including its hyperdiffusion
+ * (the vector-Jacobian product that the reference's autodiff forms through
+ * euler.py:162-372 and diffusion.py:93-172, with the stop-gradient on the
+ * edge values the staggered operators repeat, operators.py:37-63): from the
+ * cotangents of u_in, v_in, w_in, pi_in, th_in, res and tth (any may be NULL
+ * for zero) at the saved state (u .. eta), the cotangents gu .. geta of the
+ * state.  Gather form, deterministic.  work:
+ * sisl_explicit_adj_workspace_f64(g) elements. */
+int64_t sisl_explicit_adj_workspace_f64(const SislGrid *g);
+int sisl_explicit_adj_f64(const SislTendency *t, const double *u, const double *v, const double *w,
+                          const double *pi, const double *th, const double *eta, const double *gu_in,
+                          const double *gv_in, const double *gw_in, const double *gpi_in,
+                          const double *gth_in, const double *gres, const double *gtth, double *gu,
+                          double *gv, double *gw, double *gpi, double *gth, double *geta, double *work,
+                          void *stream);
 /* One predictor-corrector pass (steppers.py:911-968): theta at n+1 from the
  * advected theta', its tendency and the guess's w; then the packed
  * right-hand side b = u | v | w | pi 1000 | eta of the implicit solve from

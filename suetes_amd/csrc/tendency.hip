@@ -671,6 +671,348 @@ static bool make_ctx(const SislTendency *d, TendCtx &t) {
 
 static unsigned nblocks(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
+// ---- reverse mode of the explicit stage (vector-Jacobian products) -----------------
+//
+// Cotangents of (u, v, w, pi, theta, eta_dot) from those of the stage's outputs
+// (u_in, v_in, w_in, pi_in, th_in, the kinematic residual, theta's tendency),
+// following the reference's autodiff of the same array expressions: the values
+// that the staggered operators repeat at closed edges (m -> u, v faces 0 and n,
+// m -> w levels 0 and nz) are stop-gradient copies (operators.py:37-63), so a
+// cotangent arriving there is dropped; on a periodic axis faces 0 and n both
+// pass theirs to cells n - 1 and 0.  Every kernel is in gather form: a thread
+// sums the contributions to its own elements in a fixed order, so gradients
+// are reproducible.  Five passes (plus four for the hyperdiffusion):
+//   adjA (faces, w levels): the cotangents of the face / w-level intermediates
+//                           (theta' averages, pressure gradients, dpi/dz
+//                           averages, u_m / v_m averages, curvature, del^4);
+//   adjB (mass points):     their transposes to mass-point intermediates and
+//                           the Exner-divergence coefficients;
+//   lapT faces / mass (x2): the transposed terrain-following Laplacian;
+//   adjD (faces, w levels): cotangents of u, v, w, eta_dot;
+//   adjE (mass points):     cotangents of pi and theta.
+
+// Face f of n is a stop-gradient copy (closed axis, f == 0 or f == n).
+__device__ __forceinline__ bool face_frozen(int f, int n, int per) { return !per && (f == 0 || f == n); }
+
+// Transpose of a face quantity X built from the mass pair face_pair(f) = (a, b):
+// the sum over the live faces of wa X(f) where a == c and wb X(f) where b == c.
+template <class FX>
+__device__ __forceinline__ double face_gather(int c, int n, int per, double wa, double wb, FX X) {
+  double s = 0.0;
+  if (!face_frozen(c, n, per)) s += wb * X(c);          // face c: (c - 1, c), or periodic face 0
+  if (!face_frozen(c + 1, n, per)) s += wa * X(c + 1);  // face c + 1: (c, c + 1), or periodic face n
+  if (per && c == 0) s += wb * X(n);                    // periodic face n: (n - 1, 0)
+  if (per && c == n - 1) s += wa * X(0);                // periodic face 0: (n - 1, 0)
+  return s;
+}
+// The same along z for m -> w quantities (w levels 0 and nz are frozen).
+template <class FX>
+__device__ __forceinline__ double level_gather(int c, int nz, double wa, double wb, FX X) {
+  double s = 0.0;
+  if (c >= 1) s += wb * X(c);
+  if (c + 1 <= nz - 1) s += wa * X(c + 1);
+  return s;
+}
+__device__ __forceinline__ double ldz(const double *p, int q) { return p ? p[q] : 0.0; }
+
+struct ExplicitAdjIO {
+  const double *u, *v, *w, *pi, *th, *eta;                        // saved inputs
+  const double *gu_in, *gv_in, *gw_in, *gpi_in, *gth_in, *gres, *gtth;  // output cotangents (null: zero)
+  int hyper;                                                       // the stage included -nu_h del^4
+  double *cu, *cv, *cw;  // adjA: 6 u-face, 6 v-face, 5 w-level arrays
+  double *cm;            // adjB: gpp, ggzm, gum, gvm, gwm, 4 del^4 cotangents, cx, cy, cz, gthp
+  const double *fb;      // del^4 transposed: cotangents of the four base fields (null: none)
+  double *tu, *tv, *tw;  // adjD: theta' flux cotangents at u, v faces and w levels
+  double *gu, *gv, *gw, *gpi, *gth, *geta;  // input cotangents
+};
+
+__global__ void __launch_bounds__(256) adjA_kernel(TendCtx t, ExplicitAdjIO io) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int sx = t.nx + 1, sy = t.ny + 1;
+  if (p >= sx * sy * (t.nz + 1)) return;
+  const int i = p % sx, j = (p / sx) % sy, k = p / (sx * sy);
+  const int nx = t.nx, ny = t.ny, nz = t.nz;
+  const double dt = t.dt, oma = 1.0 - t.alpha;
+  const double *U = io.u, *V = io.v, *PI = io.pi, *TH = io.th;
+  const int Nu = t.nu(), Nv = t.nv(), Nw = t.nw();
+  if (j < ny && k < nz) {  // u face (i, j, k)
+    const int qu = t.u(i, j, k);
+    double c[6] = {0, 0, 0, 0, 0, 0};
+    const double gin = ldz(io.gu_in, qu);
+    if (!face_frozen(i, nx, t.px) && gin != 0.0) {
+      int ia, ib;
+      face_pair(i, nx, t.px, ia, ib);
+      const double gx = ((pip(t, PI, t.m(ib, j, k)) - pip(t, PI, t.m(ia, j, k))) * t.rdx) * t.mu[t.u2(i, j)];
+      const double gzu = 0.5 * (gzm_pi(t, PI, ia, j, k) + gzm_pi(t, PI, ib, j, k));
+      const double thu = t.thu_bg[qu] + 0.5 * (thp(t, TH, t.m(ia, j, k)) + thp(t, TH, t.m(ib, j, k)));
+      const double vau = 0.5 * (vm_at(t, V, ia, j, k) + vm_at(t, V, ib, j, k));
+      const double cvu = 0.5 * (curv_m(t, U, V, ia, j, k) + curv_m(t, U, V, ib, j, k));
+      const double g = gin * (dt * oma);
+      const double gc = g * ((-t.cp) * thu);
+      c[0] = g * ((-t.cp) * (gx - t.zxu[qu] * gzu));  // -> theta' (average)
+      c[1] = gc * (t.rdx * t.mu[t.u2(i, j)]);           // -> pi' (difference)
+      c[2] = gc * (-t.zxu[qu]);                         // -> dpi/dz at mass points (average)
+      c[3] = g * (t.fu[t.u2(i, j)] + cvu);              // -> v_m (average)
+      c[4] = g * vau;                                   // -> curvature term (average)
+      c[5] = io.hyper ? gin * dt : 0.0;                 // -> -nu_h del^4 u_m (average)
+    }
+#pragma unroll
+    for (int r = 0; r < 6; ++r) io.cu[(size_t)r * Nu + qu] = c[r];
+  }
+  if (i < nx && k < nz) {  // v face (i, j, k)
+    const int qv = t.v(i, j, k);
+    double c[6] = {0, 0, 0, 0, 0, 0};
+    const double gin = ldz(io.gv_in, qv);
+    if (!face_frozen(j, ny, t.py) && gin != 0.0) {
+      int ja, jb;
+      face_pair(j, ny, t.py, ja, jb);
+      const double gy = ((pip(t, PI, t.m(i, jb, k)) - pip(t, PI, t.m(i, ja, k))) * t.rdy) * t.mv[t.v2(i, j)];
+      const double gzv = 0.5 * (gzm_pi(t, PI, i, ja, k) + gzm_pi(t, PI, i, jb, k));
+      const double thv = t.thv_bg[qv] + 0.5 * (thp(t, TH, t.m(i, ja, k)) + thp(t, TH, t.m(i, jb, k)));
+      const double uav = 0.5 * (um_at(t, U, i, ja, k) + um_at(t, U, i, jb, k));
+      const double cvv = 0.5 * (curv_m(t, U, V, i, ja, k) + curv_m(t, U, V, i, jb, k));
+      const double g = gin * (dt * oma);
+      const double gc = g * ((-t.cp) * thv);
+      c[0] = g * ((-t.cp) * (gy - t.zev[qv] * gzv));
+      c[1] = gc * (t.rdy * t.mv[t.v2(i, j)]);
+      c[2] = gc * (-t.zev[qv]);
+      c[3] = g * (-t.fv[t.v2(i, j)] - cvv);  // -> u_m (average)
+      c[4] = g * (-uav);
+      c[5] = io.hyper ? gin * dt : 0.0;
+    }
+#pragma unroll
+    for (int r = 0; r < 6; ++r) io.cv[(size_t)r * Nv + qv] = c[r];
+  }
+  if (i < nx && j < ny) {  // w level (i, j, k), k = 0..nz
+    const int qw = t.w(i, j, k);
+    double c[5] = {0, 0, 0, 0, 0};
+    if (k >= 1 && k <= nz - 1) {
+      const double gin = ldz(io.gw_in, qw), gr = ldz(io.gres, qw);
+      const double thpw = 0.5 * (thp(t, TH, t.m(i, j, k - 1)) + thp(t, TH, t.m(i, j, k)));
+      const double thw = t.thw_bg[qw] + thpw;
+      const double gzw = gzw_pi(t, PI, i, j, k);
+      const double g = gin * (dt * oma);
+      c[0] = g * ((-t.cp) * gzw + t.grav / t.thw_bg[qw]);         // -> theta' (average)
+      c[1] = g * ((-t.cp) * thw) * (t.rdz * t.rz[qw]);            // -> pi' (difference)
+      c[2] = io.hyper ? gin * dt : 0.0;                            // -> -nu_h del^4 w_m (average)
+      c[3] = gr * t.zxiw[qw];                                      // -> u_m (average)
+      c[4] = gr * t.zetaw[qw];                                     // -> v_m (average)
+    }
+#pragma unroll
+    for (int r = 0; r < 5; ++r) io.cw[(size_t)r * Nw + qw] = c[r];
+  }
+}
+
+__global__ void __launch_bounds__(256) adjB_kernel(TendCtx t, ExplicitAdjIO io) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int N = t.nm();
+  if (p >= N) return;
+  const int i = p % t.nx, j = (p / t.nx) % t.ny, k = p / (t.nx * t.ny);
+  const int nx = t.nx, ny = t.ny, nz = t.nz;
+  const int Nu = t.nu(), Nv = t.nv(), Nw = t.nw();
+  const double dt = t.dt, oma = 1.0 - t.alpha;
+  auto CU = [&](int r) { return [&, r](int f) { return io.cu[(size_t)r * Nu + t.u(f, j, k)]; }; };
+  auto CV = [&](int r) { return [&, r](int f) { return io.cv[(size_t)r * Nv + t.v(i, f, k)]; }; };
+  auto CW = [&](int r) { return [&, r](int kk) { return io.cw[(size_t)r * Nw + t.w(i, j, kk)]; }; };
+  const double gpin = ldz(io.gpi_in, p), gthin = ldz(io.gth_in, p);
+  const double gpp = gpin + face_gather(i, nx, t.px, -1.0, 1.0, CU(1)) + face_gather(j, ny, t.py, -1.0, 1.0, CV(1)) +
+                     level_gather(k, nz, -1.0, 1.0, CW(1));
+  const double ggzm = face_gather(i, nx, t.px, 0.5, 0.5, CU(2)) + face_gather(j, ny, t.py, 0.5, 0.5, CV(2));
+  const double gcv = face_gather(i, nx, t.px, 0.5, 0.5, CU(4)) + face_gather(j, ny, t.py, 0.5, 0.5, CV(4));
+  const double gvm = face_gather(i, nx, t.px, 0.5, 0.5, CU(3)) + gcv * t.dmdx[t.m2(i, j)] +
+                     level_gather(k, nz, 0.5, 0.5, CW(4));
+  const double gum = face_gather(j, ny, t.py, 0.5, 0.5, CV(3)) - gcv * t.dmdy[t.m2(i, j)] +
+                     level_gather(k, nz, 0.5, 0.5, CW(3));
+  const double gt = ldz(io.gtth, p) + gthin * (oma * dt);
+  const double gwm = (-gt) * t.dthm[p];
+  const double gthp = gthin + face_gather(i, nx, t.px, 0.5, 0.5, CU(0)) + face_gather(j, ny, t.py, 0.5, 0.5, CV(0)) +
+                      level_gather(k, nz, 0.5, 0.5, CW(0));
+  const double mmv = t.mm[t.m2(i, j)], dzm = t.dzm[p];
+  const double g = gpin * (oma * dt) * (-t.cpi[p]);
+  double *cm = io.cm;
+  cm[p] = gpp;
+  cm[(size_t)N + p] = ggzm;
+  cm[2 * (size_t)N + p] = gum;
+  cm[3 * (size_t)N + p] = gvm;
+  cm[4 * (size_t)N + p] = gwm;
+  if (io.hyper) {  // cotangents of del^4 of the four base fields (hyp = -nu_h del^4)
+    cm[5 * (size_t)N + p] = (-t.nu_h) * face_gather(i, nx, t.px, 0.5, 0.5, CU(5));
+    cm[6 * (size_t)N + p] = (-t.nu_h) * face_gather(j, ny, t.py, 0.5, 0.5, CV(5));
+    cm[7 * (size_t)N + p] = (-t.nu_h) * level_gather(k, nz, 0.5, 0.5, CW(2));
+    cm[8 * (size_t)N + p] = (-t.nu_h) * gt;
+  }
+  cm[9 * (size_t)N + p] = ((g * mmv) * (mmv * t.rdx)) / dzm;  // Exner-divergence coefficients
+  cm[10 * (size_t)N + p] = ((g * mmv) * (mmv * t.rdy)) / dzm;
+  cm[11 * (size_t)N + p] = (g * (t.rdz * t.dz)) / dzm;
+  cm[12 * (size_t)N + p] = gthp;
+}
+
+// Transposed terrain-following Laplacian (the adjoint of hyper_faces then
+// hyper_mass, with the stop-gradient edges): lapT_faces takes the mass-point
+// cotangents LB of the Laplacian to the cotangents GX, GY of the face fields
+// Gx, Gy; lapT_mass takes those to the cotangents FB of the input field.
+__global__ void __launch_bounds__(256) lapT_faces_kernel(TendCtx t, int nf, const double *__restrict__ LB,
+                                                         double *__restrict__ GX, double *__restrict__ GY) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int sx = t.nx + 1, sy = t.ny + 1;
+  if (p >= sx * sy * t.nz) return;
+  const int i = p % sx, j = (p / sx) % sy, k = p / (sx * sy);
+  const int nx = t.nx, ny = t.ny, nz = t.nz, N = t.nm(), Nu = t.nu(), Nv = t.nv();
+  for (int f = 0; f < nf; ++f) {
+    const double *L = LB + (size_t)f * N;
+    // cotangent of the vertical difference EX at w level kk of column (ci, cj),
+    // from dGz = avg_z(EX) weighted by -z_x (-z_y) at the two masses around kk
+    auto ebar = [&](int ci, int cj, int kk, const double *zm) {
+      if (kk < 1 || kk > nz - 1) return 0.0;
+      const int q0 = t.m(ci, cj, kk - 1), q1 = t.m(ci, cj, kk);
+      return 0.5 * ((-zm[q0]) * L[q0]) + 0.5 * ((-zm[q1]) * L[q1]);
+    };
+    // cotangent of the mass-point average Gm of column (ci, cj) at level k
+    auto gmbar = [&](int ci, int cj, const double *zm) {
+      return t.rdz * (ebar(ci, cj, k, zm) * t.rz[t.w(ci, cj, k)] - ebar(ci, cj, k + 1, zm) * t.rz[t.w(ci, cj, k + 1)]);
+    };
+    if (j < ny) {  // u face (i, j, k)
+      double s = 0.0;
+      if (i >= 1) s += L[t.m(i - 1, j, k)] * (t.rdx * t.mm[t.m2(i - 1, j)]);
+      if (i <= nx - 1) s -= L[t.m(i, j, k)] * (t.rdx * t.mm[t.m2(i, j)]);
+      if (i >= 1) s += 0.5 * gmbar(i - 1, j, t.zxm);
+      if (i <= nx - 1) s += 0.5 * gmbar(i, j, t.zxm);
+      GX[(size_t)f * Nu + t.u(i, j, k)] = s;
+    }
+    if (i < nx) {  // v face (i, j, k)
+      double s = 0.0;
+      if (j >= 1) s += L[t.m(i, j - 1, k)] * (t.rdy * t.mm[t.m2(i, j - 1)]);
+      if (j <= ny - 1) s -= L[t.m(i, j, k)] * (t.rdy * t.mm[t.m2(i, j)]);
+      if (j >= 1) s += 0.5 * gmbar(i, j - 1, t.zem);
+      if (j <= ny - 1) s += 0.5 * gmbar(i, j, t.zem);
+      GY[(size_t)f * Nv + t.v(i, j, k)] = s;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) lapT_mass_kernel(TendCtx t, int nf, const double *__restrict__ GX,
+                                                        const double *__restrict__ GY, double *__restrict__ FB) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int N = t.nm();
+  if (p >= N) return;
+  const int i = p % t.nx, j = (p / t.nx) % t.ny, k = p / (t.nx * t.ny);
+  const int nx = t.nx, ny = t.ny, nz = t.nz, Nu = t.nu(), Nv = t.nv();
+  for (int f = 0; f < nf; ++f) {
+    const double *Gx = GX + (size_t)f * Nu, *Gy = GY + (size_t)f * Nv;
+    double s = face_gather(i, nx, t.px, -1.0, 1.0,
+                           [&](int ff) { return Gx[t.u(ff, j, k)] * (t.rdx * t.mu[t.u2(ff, j)]); }) +
+               face_gather(j, ny, t.py, -1.0, 1.0,
+                           [&](int ff) { return Gy[t.v(i, ff, k)] * (t.rdy * t.mv[t.v2(i, ff)]); });
+    // cotangent of the mass-point vertical derivative dfz_m at level kk of this column
+    auto dfzbar = [&](int kk) {
+      return face_gather(i, nx, t.px, 0.5, 0.5, [&](int ff) { return (-t.zxu[t.u(ff, j, kk)]) * Gx[t.u(ff, j, kk)]; }) +
+             face_gather(j, ny, t.py, 0.5, 0.5, [&](int ff) { return (-t.zev[t.v(i, ff, kk)]) * Gy[t.v(i, ff, kk)]; });
+    };
+    const double dc = dfzbar(k);
+    const double dlo = k >= 1 ? dfzbar(k - 1) : 0.0, dhi = k + 1 <= nz - 1 ? dfzbar(k + 1) : 0.0;
+    // D at w level kk (interior) carries 0.5 (dfz_m(kk - 1) + dfz_m(kk)); F(k) enters D(k) with +, D(k + 1) with -
+    if (k >= 1) s += t.rdz * ((0.5 * (dlo + dc)) * t.rz[t.w(i, j, k)]);
+    if (k + 1 <= nz - 1) s -= t.rdz * ((0.5 * (dc + dhi)) * t.rz[t.w(i, j, k + 1)]);
+    FB[(size_t)f * N + p] = s;
+  }
+}
+
+__global__ void __launch_bounds__(256) adjD_kernel(TendCtx t, ExplicitAdjIO io) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int sx = t.nx + 1, sy = t.ny + 1;
+  if (p >= sx * sy * (t.nz + 1)) return;
+  const int i = p % sx, j = (p / sx) % sy, k = p / (sx * sy);
+  const int nx = t.nx, ny = t.ny, nz = t.nz, N = t.nm();
+  const double *U = io.u, *V = io.v, *E = io.eta, *TH = io.th;
+  const double *cm = io.cm, *fb = io.fb;
+  auto mget = [&](int r, int q) { return cm[(size_t)r * N + q]; };
+  if (j < ny && k < nz) {  // u face (i, j, k)
+    const int qu = t.u(i, j, k);
+    auto gumT = [&](int c) {
+      const int q = t.m(c, j, k);
+      return mget(2, q) + (fb ? fb[q] : 0.0);
+    };
+    double s = ldz(io.gu_in, qu);
+    double um = 0.0;
+    if (i >= 1) um += gumT(i - 1);
+    if (i <= nx - 1) um += gumT(i);
+    s += 0.5 * um;
+    const double cf = (i >= 1 ? mget(9, t.m(i - 1, j, k)) : 0.0) - (i <= nx - 1 ? mget(9, t.m(i, j, k)) : 0.0);
+    int ia, ib;
+    face_pair(i, nx, t.px, ia, ib);
+    const double thpu = 0.5 * (thp(t, TH, t.m(ia, j, k)) + thp(t, TH, t.m(ib, j, k)));
+    s += cf * (t.cx[qu] + thpu * t.cxp[qu]);
+    io.gu[qu] = s;
+    io.tu[qu] = face_frozen(i, nx, t.px) ? 0.0 : cf * (U[qu] * t.cxp[qu]);
+  }
+  if (i < nx && k < nz) {  // v face (i, j, k)
+    const int qv = t.v(i, j, k);
+    auto gvmT = [&](int c) {
+      const int q = t.m(i, c, k);
+      return mget(3, q) + (fb ? fb[(size_t)N + q] : 0.0);
+    };
+    double s = ldz(io.gv_in, qv);
+    double vm = 0.0;
+    if (j >= 1) vm += gvmT(j - 1);
+    if (j <= ny - 1) vm += gvmT(j);
+    s += 0.5 * vm;
+    const double cf = (j >= 1 ? mget(10, t.m(i, j - 1, k)) : 0.0) - (j <= ny - 1 ? mget(10, t.m(i, j, k)) : 0.0);
+    int ja, jb;
+    face_pair(j, ny, t.py, ja, jb);
+    const double thpv = 0.5 * (thp(t, TH, t.m(i, ja, k)) + thp(t, TH, t.m(i, jb, k)));
+    s += cf * (t.cy[qv] + thpv * t.cyp[qv]);
+    io.gv[qv] = s;
+    io.tv[qv] = face_frozen(j, ny, t.py) ? 0.0 : cf * (V[qv] * t.cyp[qv]);
+  }
+  if (i < nx && j < ny) {  // w level (i, j, k), k = 0..nz
+    const int qw = t.w(i, j, k);
+    auto gwmT = [&](int c) {
+      const int q = t.m(i, j, c);
+      return mget(4, q) + (fb ? fb[2 * (size_t)N + q] : 0.0);
+    };
+    const double gr = ldz(io.gres, qw);
+    double s = ldz(io.gw_in, qw) - gr;
+    double wm = 0.0;
+    if (k >= 1) wm += gwmT(k - 1);
+    if (k <= nz - 1) wm += gwmT(k);
+    io.gw[qw] = s + 0.5 * wm;
+    double e = gr * t.dzw[qw];
+    double tz = 0.0;
+    if (k >= 1 && k <= nz - 1) {
+      const double cf = mget(11, t.m(i, j, k - 1)) - mget(11, t.m(i, j, k));
+      const double thpw = 0.5 * (thp(t, TH, t.m(i, j, k - 1)) + thp(t, TH, t.m(i, j, k)));
+      e += cf * (t.czs[qw] + thpw * t.czp[qw]);
+      tz = cf * (E[qw] * t.czp[qw]);
+    }
+    io.geta[qw] = e;
+    io.tw[qw] = tz;
+  }
+}
+
+__global__ void __launch_bounds__(256) adjE_kernel(TendCtx t, ExplicitAdjIO io) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int N = t.nm();
+  if (p >= N) return;
+  const int i = p % t.nx, j = (p / t.nx) % t.ny, k = p / (t.nx * t.ny);
+  const int nx = t.nx, ny = t.ny, nz = t.nz;
+  const double *cm = io.cm;
+  // cotangent of d pi'/dz at w level kk (interior) from the mass-point
+  // averages at kk - 1 and kk, times the difference's factor rdz rz(kk)
+  auto gw = [&](int kk) {
+    const double avg = 0.5 * (cm[(size_t)N + t.m(i, j, kk - 1)] + cm[(size_t)N + t.m(i, j, kk)]);
+    return avg * (t.rdz * t.rz[t.w(i, j, kk)]);
+  };
+  double s = cm[p];
+  if (k >= 1) s += gw(k);
+  if (k + 1 <= nz - 1) s -= gw(k + 1);
+  io.gpi[p] = s;
+  double th = cm[12 * (size_t)N + p] + (io.fb ? io.fb[3 * (size_t)N + p] : 0.0);
+  th += face_gather(i, nx, t.px, 0.5, 0.5, [&](int f) { return io.tu[t.u(f, j, k)]; });
+  th += face_gather(j, ny, t.py, 0.5, 0.5, [&](int f) { return io.tv[t.v(i, f, k)]; });
+  th += level_gather(k, nz, 0.5, 0.5, [&](int kk) { return io.tw[t.w(i, j, kk)]; });
+  io.gth[p] = th;
+}
+
 }  // namespace sisl
 
 extern "C" {
@@ -772,4 +1114,54 @@ int sisl_final_f64(const SislTendency *d, const double *x, double *u, double *v,
   return SISL_OK;
 }
 
+int64_t sisl_explicit_adj_workspace_f64(const SislGrid *g) {
+  if (!g) return -1;
+  const int64_t N = (int64_t)g->nx * g->ny * g->nz;
+  const int64_t Nu = (int64_t)(g->nx + 1) * g->ny * g->nz, Nv = (int64_t)g->nx * (g->ny + 1) * g->nz;
+  const int64_t Nw = (int64_t)g->nx * g->ny * (g->nz + 1);
+  // face / w-level intermediates (also the transposed Laplacian's face fields),
+  // 13 mass arrays, two 4-field mass arrays of the transposed del^4, theta' fluxes
+  return 6 * Nu + 6 * Nv + 5 * Nw + 13 * N + 8 * N + Nu + Nv + Nw;
+}
+
+int sisl_explicit_adj_f64(const SislTendency *d, const double *u, const double *v, const double *w,
+                          const double *pi, const double *th, const double *eta, const double *gu_in,
+                          const double *gv_in, const double *gw_in, const double *gpi_in, const double *gth_in,
+                          const double *gres, const double *gtth, double *gu, double *gv, double *gw, double *gpi,
+                          double *gth, double *geta, double *work, void *stream) {
+  sisl::TendCtx t;
+  if (!sisl::make_ctx(d, t) || !u || !v || !w || !pi || !th || !eta || !gu || !gv || !gw || !gpi || !gth ||
+      !geta || !work)
+    return SISL_ERR_ARG;
+  hipStream_t s = sisl::as_stream(stream);
+  const int64_t N = (int64_t)t.nx * t.ny * t.nz;
+  const int64_t Nu = (int64_t)(t.nx + 1) * t.ny * t.nz, Nv = (int64_t)t.nx * (t.ny + 1) * t.nz;
+  const int64_t Nw = (int64_t)t.nx * t.ny * (t.nz + 1);
+  double *cu = work, *cv = cu + 6 * Nu, *cw = cv + 6 * Nv, *cm = cw + 5 * Nw;
+  double *l1 = cm + 13 * N, *fb = l1 + 4 * N, *tu = fb + 4 * N, *tv = tu + Nu, *tw = tv + Nv;
+  const int hyper = t.nu_h != 0.0;
+  sisl::ExplicitAdjIO io{u, v, w, pi, th, eta, gu_in, gv_in, gw_in, gpi_in, gth_in, gres, gtth, hyper,
+                         cu, cv, cw, cm, hyper ? fb : nullptr, tu, tv, tw, gu, gv, gw, gpi, gth, geta};
+  const int64_t span = (int64_t)(t.nx + 1) * (t.ny + 1) * (t.nz + 1);
+  const unsigned bs = sisl::nblocks(span), bm = sisl::nblocks(N);
+  const unsigned bf = sisl::nblocks((int64_t)(t.nx + 1) * (t.ny + 1) * t.nz);
+  hipLaunchKernelGGL(sisl::adjA_kernel, dim3(bs), dim3(sisl::kBlock), 0, s, t, io);
+  hipLaunchKernelGGL(sisl::adjB_kernel, dim3(bm), dim3(sisl::kBlock), 0, s, t, io);
+  if (hyper) {  // del^4 transposed: the face arrays of the first two passes are free again
+    const double *lb = cm + 5 * N;
+    hipLaunchKernelGGL(sisl::lapT_faces_kernel, dim3(bf), dim3(sisl::kBlock), 0, s, t, 4, lb, cu, cv);
+    hipLaunchKernelGGL(sisl::lapT_mass_kernel, dim3(bm), dim3(sisl::kBlock), 0, s, t, 4, (const double *)cu,
+                       (const double *)cv, l1);
+    hipLaunchKernelGGL(sisl::lapT_faces_kernel, dim3(bf), dim3(sisl::kBlock), 0, s, t, 4, (const double *)l1, cu,
+                       cv);
+    hipLaunchKernelGGL(sisl::lapT_mass_kernel, dim3(bm), dim3(sisl::kBlock), 0, s, t, 4, (const double *)cu,
+                       (const double *)cv, fb);
+  }
+  hipLaunchKernelGGL(sisl::adjD_kernel, dim3(bs), dim3(sisl::kBlock), 0, s, t, io);
+  hipLaunchKernelGGL(sisl::adjE_kernel, dim3(bm), dim3(sisl::kBlock), 0, s, t, io);
+  SISL_CHECK_LAUNCH();
+  return SISL_OK;
+}
+
 }  // extern "C"
+

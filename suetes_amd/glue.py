@@ -14,10 +14,14 @@ runs the same arithmetic in five FP64 kernels:
   solve (two launches per predictor-corrector pass);
 * ``final``: divergence damping and unscaling of the solution (one launch).
 
-Reverse mode: each group is one autograd node whose backward re-evaluates
-the torch restatement of the same group on the saved inputs and
-differentiates it, so cotangents follow the reference's own rules (the
-stop-gradient on repeated edge values, the ties of the clamps) exactly.
+Reverse mode: each group is one autograd node.  The explicit stage's
+backward is a hand-written gather-form kernel set
+(``sisl_explicit_adj_f64``); the predictor-corrector and final groups
+re-evaluate the torch restatement of the same group on the saved inputs and
+differentiate it.  Both follow the reference's own rules (the stop-gradient
+on repeated edge values) exactly; with the stop-gradient switched off
+(``use_stop_grad = False`` on the operators) the explicit stage is
+differentiated through its torch restatement as well.
 """
 from __future__ import annotations
 
@@ -50,6 +54,9 @@ def _bind(L):
     L.sisl_explicit_f64.argtypes = [P] + [vp] * 16 + [vp]
     L.sisl_predictor_f64.argtypes = [P] + [vp] * 17 + [vp]
     L.sisl_final_f64.argtypes = [P, vp, vp, vp, vp, vp, vp]
+    L.sisl_explicit_adj_workspace_f64.restype = i64
+    L.sisl_explicit_adj_workspace_f64.argtypes = [C.POINTER(_lib.SislGrid)]
+    L.sisl_explicit_adj_f64.argtypes = [P] + [vp] * 20 + [vp]
     L._glue_bound = True
     return L
 
@@ -123,6 +130,27 @@ class FusedGlue:
                                        tend.data_ptr() if tend is not None else None, s), "explicit tendencies")
         return tuple(outs) + (x0,)
 
+    def explicit_adj(self, saved, gouts):
+        """Cotangents of (u, v, w, pi, th, eta) from those of the explicit
+        stage's outputs (u_in, v_in, w_in, pi_in, th_in, res, tth; None for
+        zero), at the saved state."""
+        u, v, w, pi, th, eta = [t.contiguous() for t in saved]
+        _lib.require_device(u, v, w, pi, th, eta)
+        gs = [g.contiguous() if g is not None else None for g in gouts[:7]]
+        for g in gs:
+            if g is not None:
+                _lib.require_device(g)
+        L = self.L
+        n = int(L.sisl_explicit_adj_workspace_f64(C.byref(self.struct.g)))
+        work = torch.empty(n, dtype=torch.float64, device=u.device)
+        outs = [torch.empty_like(u), torch.empty_like(v), torch.empty_like(w), torch.empty_like(pi),
+                torch.empty_like(th), torch.empty_like(eta)]
+        _lib.check(L.sisl_explicit_adj_f64(C.byref(self.struct), *[t.data_ptr() for t in (u, v, w, pi, th, eta)],
+                                           *[g.data_ptr() if g is not None else None for g in gs],
+                                           *[o.data_ptr() for o in outs], work.data_ptr(),
+                                           _lib.stream_ptr(u.device)), "explicit stage adjoint")
+        return outs
+
     def predictor(self, rhs_u, rhs_v, rhs_w, rhs_pi, th_adv, tth, w_n, reta, g_pi, g_u, g_v, g_w, g_eta,
                   m_u, m_v):
         """(b, th_next): the packed right-hand side of one predictor-corrector
@@ -171,19 +199,35 @@ def _recompute_grads(fn, saved, need, gouts, nout):
 class ExplicitFn(torch.autograd.Function):
     """Fused explicit stage; outputs (u_in, v_in, w_in, pi_in, th_in, res, tth, x0)."""
 
+    # the state fields (u, v, w, pi, th, eta) each output depends on
+    # (euler.py:162-372: u_in, v_in, w_in, pi_in, th_in, res, tth); an output
+    # none of whose inputs is differentiated is marked non-differentiable, so
+    # that the gathers downstream of it skip their adjoints
+    DEPS = ((0, 1, 3, 4), (0, 1, 3, 4), (2, 3, 4), (0, 1, 3, 4, 5), (2, 4), (0, 1, 2, 5), (2, 4))
+
     @staticmethod
     def forward(ctx, glue, u, v, w, pi, th, eta):
         outs = glue.explicit(u, v, w, pi, th, eta)
         ctx.glue = glue
         ctx.save_for_backward(u, v, w, pi, th, eta)
-        ctx.mark_non_differentiable(outs[-1])
+        need = ctx.needs_input_grad[1:]
+        ctx.frozen = [not any(need[q] for q in d) for d in ExplicitFn.DEPS]
+        ctx.mark_non_differentiable(outs[-1], *[o for o, f in zip(outs, ctx.frozen) if f])
         return outs
 
     @staticmethod
     def backward(ctx, *gouts):
-        g = _recompute_grads(ctx.glue.stepper._explicit_stage_torch, ctx.saved_tensors, ctx.needs_input_grad[1:],
-                             gouts, 7)
-        return (None,) + tuple(g)
+        need = ctx.needs_input_grad[1:]
+        if not any(need):
+            return (None,) * 7
+        if not getattr(ctx.glue.stepper.physics.op, "use_stop_grad", True):
+            # the exact derivative (edge copies pass their cotangents on) is
+            # only in the torch restatement
+            g = _recompute_grads(ctx.glue.stepper._explicit_stage_torch, ctx.saved_tensors, need, gouts, 7)
+            return (None,) + tuple(g)
+        gouts = [None if f else g for g, f in zip(gouts[:7], ctx.frozen)]
+        g = ctx.glue.explicit_adj(ctx.saved_tensors, gouts)
+        return (None,) + tuple(gi if n else None for gi, n in zip(g, need))
 
 
 class PredictorFn(torch.autograd.Function):

@@ -840,10 +840,12 @@ class SISLStepper3D:
         with one), with FP64 device state and with a background and grid
         that carry no gradient (terrain controls run the torch ops, whose
         reverse mode reaches the metrics).  A step whose state is being
-        differentiated also runs the torch ops: their autograd graph keeps
-        the intermediates, which costs less than re-evaluating the torch
-        restatement of each fused group in the backward pass (value and
-        gradient at 256x256x80: 235 vs 240 ms mixed, 700 vs 707 ms FP64)."""
+        differentiated runs the explicit stage on its fused kernels with the
+        hand-written reverse mode and the predictor-corrector and final
+        groups as torch ops (their autograd graph keeps the intermediates,
+        which costs less than re-evaluating their torch restatement in the
+        backward pass: value and gradient at 256x256x80 235 vs 240 ms mixed,
+        700 vs 707 ms FP64, measured with all three groups re-evaluated)."""
         if not self.fused_glue or self.physics.physics_suite is not None:
             return False
         u = state["u"]
@@ -851,10 +853,11 @@ class SISLStepper3D:
             return False
         if any(k in state for k in self.tracer_keys):
             return False
-        if torch.is_grad_enabled() and any(torch.is_tensor(v) and v.requires_grad for v in state.values()):
-            return False
         self.implicit_solver.prepare(bg)
         return not _operator_inputs(self.implicit_solver)
+
+    def _differentiated(self, state):
+        return torch.is_grad_enabled() and any(torch.is_tensor(v) and v.requires_grad for v in state.values())
 
     def _fused(self, bg):
         from .glue import FusedGlue
@@ -968,6 +971,14 @@ class SISLStepper3D:
         from .glue import ExplicitFn, FinalFn, PredictorFn
         ph, c = self.physics, self.physics.c
         glue = self._fused(bg)
+        if self._differentiated(state):
+            predictor, final = self._predictor_torch, self._final_torch
+        else:
+            def predictor(*a):
+                return PredictorFn.apply(glue, *a)
+
+            def final(x_):
+                return FinalFn.apply(glue, x_)
         u_in, v_in, w_in, pi_in, th_in, res_n, tth, x = ExplicitFn.apply(
             glue, state["u"], state["v"], state["w"], state["pi"], state["th_v"], state["eta_dot"])
         lim, pair = self.use_limiter, self.advector.advect_cubic_pair
@@ -984,14 +995,14 @@ class SISLStepper3D:
         solver, like = self.implicit_solver, self._like()
         dzw = bg["dz_w_full"]
         for _ in range(2):
-            b, th_next = PredictorFn.apply(glue, rhs_u, rhs_v, rhs_w, rhs_pi, th_adv, tth, state["w"], reta,
-                                           g_pi, g_u, g_v, g_w, g_eta, m_u, m_v)
+            b, th_next = predictor(rhs_u, rhs_v, rhs_w, rhs_pi, th_adv, tth, state["w"], reta, g_pi, g_u, g_v,
+                                   g_w, g_eta, m_u, m_v)
             x = solver.solve_packed(b, x, bg)
             d = unpack(x, like)
             g_u, g_v, g_w = d["u"], d["v"], d["w"]
             g_pi, g_eta = d["pi"] / solver.pi_scale, d["eta_dot"] / dzw
             m_u, m_v = g_u, g_v
-        u_o, v_o, pi_o, eta_o = FinalFn.apply(glue, x)
+        u_o, v_o, pi_o, eta_o = final(x)
         out = {"u": u_o, "v": v_o, "w": g_w, "pi": pi_o, "rho": rho_next, "th_v": th_next, "eta_dot": eta_o}
         out = self._call_bc(bc_fn, out, forcing)
         out["rho"] = c["p0"] / (c["Rd"] * out["th_v"]) * (out["pi"] ** (c["cvd"] / c["Rd"]))
