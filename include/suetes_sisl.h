@@ -422,6 +422,26 @@ int sisl_predictor_f64(const SislTendency *t, const double *rhs_u, const double 
  * damping, pi = x_pi / 1000 + pi_bg and eta = x_eta / dz_w. */
 int sisl_final_f64(const SislTendency *t, const double *x, double *u, double *v, double *pi, double *eta,
                    void *stream);
+/* Reverse mode of sisl_predictor_f64 (steppers.py:911-968 through the
+ * reference's autodiff, stop-gradient edge values as in
+ * sisl_explicit_adj_f64; the kinematic bottom row passes nothing back): from
+ * the cotangents gb of b and gth of th_next (NULL for zero) at the saved
+ * guess, metric winds and th_next, the cotangents of the fifteen inputs in
+ * sisl_predictor_f64's order.  work: sisl_predictor_adj_workspace_f64(g)
+ * elements. */
+int64_t sisl_predictor_adj_workspace_f64(const SislGrid *g);
+int sisl_predictor_adj_f64(const SislTendency *t, const double *g_pi, const double *g_u,
+                           const double *g_v, const double *g_eta, const double *m_u, const double *m_v,
+                           const double *th_next, const double *gb, const double *gth, double *grhs_u,
+                           double *grhs_v, double *grhs_w, double *grhs_pi, double *gth_adv,
+                           double *gtth, double *gw_n, double *greta, double *gg_pi, double *gg_u,
+                           double *gg_v, double *gg_w, double *gg_eta, double *gm_u, double *gm_v,
+                           double *work, void *stream);
+/* Reverse mode of sisl_final_f64: the cotangent gx of the packed solution
+ * from those of u, v, pi and eta (NULL for zero; the solution's w is not an
+ * output of the group, its slot is zero).  work: N elements. */
+int sisl_final_adj_f64(const SislTendency *t, const double *gu, const double *gv, const double *gpi,
+                       const double *geta, double *gx, double *work, void *stream);
 
 /* SplitExplicitStepper3D._compute_slow_tendencies (suetes/regional3d/
  * steppers.py:1358-1470) with _upwind_flux_3rd_order (steppers.py:1310-1356)

@@ -38,7 +38,8 @@ EXPORTED = [
     "sisl_helmholtz_apply_fused_f32", "sisl_helmholtz_apply_h16_f32", "sisl_probe_stream", "sisl_probe_operator_streams_f32",
     "sisl_tricubic2_f64", "sisl_tricubic2_f32", "sisl_tricubic_mixed", "sisl_tricubic2_mixed",
     "sisl_gmres_batched_outcome", "sisl_get_operator_shape", "sisl_probe_operator_streams_h16_f32",
-    "sisl_explicit_adj_workspace_f64", "sisl_explicit_adj_f64",
+    "sisl_explicit_adj_workspace_f64", "sisl_explicit_adj_f64", "sisl_predictor_adj_workspace_f64",
+    "sisl_predictor_adj_f64", "sisl_final_adj_f64",
 ]
 
 # timed kernels (enum SislTimedKernel)

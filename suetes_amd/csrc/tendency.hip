@@ -1013,6 +1013,273 @@ __global__ void __launch_bounds__(256) adjE_kernel(TendCtx t, ExplicitAdjIO io) 
   io.gth[p] = th;
 }
 
+
+// ---- reverse mode of one predictor-corrector pass and of the final group ----------
+//
+// Predictor (steppers.py:911-968 through the reference's autodiff): from the
+// cotangents of the packed right-hand side b and of theta at n+1, the
+// cotangents of the fifteen inputs of sisl_predictor_f64.  The kinematic
+// bottom row of b is built from stop-gradient edge values and the top row is
+// zero, so neither passes anything back.  Passes: predA (faces, w levels),
+// predB (mass), predD (faces, w levels), predE (mass), predF (w levels).
+struct PredictorAdjIO {
+  const double *g_pi, *g_u, *g_v, *g_eta, *m_u, *m_v;  // saved inputs the derivatives need
+  const double *th_next;                               // saved output
+  const double *gb, *gth;                              // cotangents of b, th_next (null: zero)
+  double *cu, *cv, *cw;  // predA: 5 u-face, 5 v-face, 2 w-level arrays
+  double *cm;            // predB: tp, gp, ggzm, gum, gvm, cx, cy, cz
+  double *tu, *tv, *tw;  // predD: theta' flux cotangents
+  // input cotangents (sisl_predictor_f64 order)
+  double *grhs_u, *grhs_v, *grhs_w, *grhs_pi, *gth_adv, *gtth, *gw_n, *greta, *gg_pi, *gg_u, *gg_v, *gg_w,
+      *gg_eta, *gm_u, *gm_v;
+};
+
+__global__ void __launch_bounds__(256) predA_kernel(TendCtx t, PredictorAdjIO io) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int sx = t.nx + 1, sy = t.ny + 1;
+  if (p >= sx * sy * (t.nz + 1)) return;
+  const int i = p % sx, j = (p / sx) % sy, k = p / (sx * sy);
+  const int nx = t.nx, ny = t.ny, nz = t.nz;
+  const int Nu = t.nu(), Nv = t.nv(), Nw = t.nw();
+  const double adt = t.alpha * t.dt, cad = (t.cp * t.alpha) * t.dt;
+  const double *TN = io.th_next;
+  auto tp1 = [&](int q) { return TN[q] - t.th_bg[q]; };
+  if (j < ny && k < nz) {  // u face
+    const int qu = t.u(i, j, k);
+    double c[5] = {0, 0, 0, 0, 0};
+    const double gb = ldz(io.gb, qu);
+    io.grhs_u[qu] = gb;
+    if (!face_frozen(i, nx, t.px) && gb != 0.0) {
+      int ia, ib;
+      face_pair(i, nx, t.px, ia, ib);
+      const double thu1 = 0.5 * (tp1(t.m(ia, j, k)) + tp1(t.m(ib, j, k)));
+      const double gx = ((io.g_pi[t.m(ib, j, k)] - io.g_pi[t.m(ia, j, k)]) * t.rdx) * t.mu[t.u2(i, j)];
+      const double gzu = 0.5 * (gzm_p(t, io.g_pi, ia, j, k) + gzm_p(t, io.g_pi, ib, j, k));
+      const double vau = 0.5 * (vm_at(t, io.m_v, ia, j, k) + vm_at(t, io.m_v, ib, j, k));
+      const double cvu = 0.5 * (curv_m(t, io.m_u, io.m_v, ia, j, k) + curv_m(t, io.m_u, io.m_v, ib, j, k));
+      const double gc = gb * ((-cad) * thu1);
+      c[0] = gb * ((-cad) * (gx - t.zxu[qu] * gzu));  // -> theta' at n+1 (average)
+      c[1] = gc * (t.rdx * t.mu[t.u2(i, j)]);         // -> guess pi' (difference)
+      c[2] = gc * (-t.zxu[qu]);                       // -> dpi/dz at mass points (average)
+      c[3] = (gb * adt) * cvu;                        // -> v_m of the metric winds (average)
+      c[4] = (gb * adt) * vau;                        // -> curvature term (average)
+    }
+#pragma unroll
+    for (int r = 0; r < 5; ++r) io.cu[(size_t)r * Nu + qu] = c[r];
+  }
+  if (i < nx && k < nz) {  // v face
+    const int qv = t.v(i, j, k);
+    double c[5] = {0, 0, 0, 0, 0};
+    const double gb = ldz(io.gb, Nu + qv);
+    io.grhs_v[qv] = gb;
+    if (!face_frozen(j, ny, t.py) && gb != 0.0) {
+      int ja, jb;
+      face_pair(j, ny, t.py, ja, jb);
+      const double thv1 = 0.5 * (tp1(t.m(i, ja, k)) + tp1(t.m(i, jb, k)));
+      const double gy = ((io.g_pi[t.m(i, jb, k)] - io.g_pi[t.m(i, ja, k)]) * t.rdy) * t.mv[t.v2(i, j)];
+      const double gzv = 0.5 * (gzm_p(t, io.g_pi, i, ja, k) + gzm_p(t, io.g_pi, i, jb, k));
+      const double uav = 0.5 * (um_at(t, io.m_u, i, ja, k) + um_at(t, io.m_u, i, jb, k));
+      const double cvv = 0.5 * (curv_m(t, io.m_u, io.m_v, i, ja, k) + curv_m(t, io.m_u, io.m_v, i, jb, k));
+      const double gc = gb * ((-cad) * thv1);
+      c[0] = gb * ((-cad) * (gy - t.zev[qv] * gzv));
+      c[1] = gc * (t.rdy * t.mv[t.v2(i, j)]);
+      c[2] = gc * (-t.zev[qv]);
+      c[3] = (gb * adt) * (-cvv);  // -> u_m (average)
+      c[4] = (gb * adt) * (-uav);
+    }
+#pragma unroll
+    for (int r = 0; r < 5; ++r) io.cv[(size_t)r * Nv + qv] = c[r];
+  }
+  if (i < nx && j < ny) {  // w level
+    const int qw = t.w(i, j, k);
+    double c[2] = {0, 0};
+    const bool in = k >= 1 && k <= nz - 1;
+    const double gb = in ? ldz(io.gb, Nu + Nv + qw) : 0.0;
+    io.grhs_w[qw] = gb;
+    io.greta[qw] = in ? (-(1.0 - t.alpha)) * ldz(io.gb, Nu + Nv + Nw + t.nm() + qw) : 0.0;
+    if (in && gb != 0.0) {
+      const double thw1 = 0.5 * (tp1(t.m(i, j, k - 1)) + tp1(t.m(i, j, k)));
+      const double gzw = gzw_p(t, io.g_pi, i, j, k);
+      c[0] = gb * (adt * (t.grav / t.thw_bg[qw]) + (-cad) * gzw);  // -> theta' (average)
+      c[1] = gb * ((-cad) * thw1) * (t.rdz * t.rz[qw]);             // -> guess pi' (difference)
+    }
+    io.cw[qw] = c[0];
+    io.cw[(size_t)Nw + qw] = c[1];
+  }
+}
+
+__global__ void __launch_bounds__(256) predB_kernel(TendCtx t, PredictorAdjIO io) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int N = t.nm();
+  if (p >= N) return;
+  const int i = p % t.nx, j = (p / t.nx) % t.ny, k = p / (t.nx * t.ny);
+  const int nx = t.nx, ny = t.ny, nz = t.nz;
+  const int Nu = t.nu(), Nv = t.nv(), Nw = t.nw();
+  const double adt = t.alpha * t.dt;
+  auto CU = [&](int r) { return [&, r](int f) { return io.cu[(size_t)r * Nu + t.u(f, j, k)]; }; };
+  auto CV = [&](int r) { return [&, r](int f) { return io.cv[(size_t)r * Nv + t.v(i, f, k)]; }; };
+  auto CW = [&](int r) { return [&, r](int kk) { return io.cw[(size_t)r * Nw + t.w(i, j, kk)]; }; };
+  double *cm = io.cm;
+  cm[p] = ldz(io.gth, p) + face_gather(i, nx, t.px, 0.5, 0.5, CU(0)) + face_gather(j, ny, t.py, 0.5, 0.5, CV(0)) +
+          level_gather(k, nz, 0.5, 0.5, CW(0));
+  cm[(size_t)N + p] = face_gather(i, nx, t.px, -1.0, 1.0, CU(1)) + face_gather(j, ny, t.py, -1.0, 1.0, CV(1)) +
+                      level_gather(k, nz, -1.0, 1.0, CW(1));
+  cm[2 * (size_t)N + p] = face_gather(i, nx, t.px, 0.5, 0.5, CU(2)) + face_gather(j, ny, t.py, 0.5, 0.5, CV(2));
+  const double gcv = face_gather(i, nx, t.px, 0.5, 0.5, CU(4)) + face_gather(j, ny, t.py, 0.5, 0.5, CV(4));
+  cm[3 * (size_t)N + p] = face_gather(j, ny, t.py, 0.5, 0.5, CV(3)) - gcv * t.dmdy[t.m2(i, j)];
+  cm[4 * (size_t)N + p] = face_gather(i, nx, t.px, 0.5, 0.5, CU(3)) + gcv * t.dmdx[t.m2(i, j)];
+  const double gpb = ldz(io.gb, Nu + Nv + Nw + p);
+  io.grhs_pi[p] = gpb * 1000.0;
+  const double mmv = t.mm[t.m2(i, j)], dzm = t.dzm[p];
+  const double g = ((gpb * 1000.0) * adt) * (-t.cpi[p]);
+  cm[5 * (size_t)N + p] = ((g * mmv) * (mmv * t.rdx)) / dzm;
+  cm[6 * (size_t)N + p] = ((g * mmv) * (mmv * t.rdy)) / dzm;
+  cm[7 * (size_t)N + p] = (g * (t.rdz * t.dz)) / dzm;
+}
+
+__global__ void __launch_bounds__(256) predD_kernel(TendCtx t, PredictorAdjIO io) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int sx = t.nx + 1, sy = t.ny + 1;
+  if (p >= sx * sy * (t.nz + 1)) return;
+  const int i = p % sx, j = (p / sx) % sy, k = p / (sx * sy);
+  const int nx = t.nx, ny = t.ny, nz = t.nz, N = t.nm();
+  const double *cm = io.cm, *TN = io.th_next;
+  auto tp1 = [&](int q) { return TN[q] - t.th_bg[q]; };
+  auto mget = [&](int r, int q) { return cm[(size_t)r * N + q]; };
+  if (j < ny && k < nz) {  // u face
+    const int qu = t.u(i, j, k);
+    double um = 0.0;
+    if (i >= 1) um += mget(3, t.m(i - 1, j, k));
+    if (i <= nx - 1) um += mget(3, t.m(i, j, k));
+    io.gm_u[qu] = 0.5 * um;
+    const double cf = (i >= 1 ? mget(5, t.m(i - 1, j, k)) : 0.0) - (i <= nx - 1 ? mget(5, t.m(i, j, k)) : 0.0);
+    int ia, ib;
+    face_pair(i, nx, t.px, ia, ib);
+    const double thu1 = 0.5 * (tp1(t.m(ia, j, k)) + tp1(t.m(ib, j, k)));
+    io.gg_u[qu] = cf * (thu1 * t.cxp[qu]);
+    io.tu[qu] = face_frozen(i, nx, t.px) ? 0.0 : cf * (io.g_u[qu] * t.cxp[qu]);
+  }
+  if (i < nx && k < nz) {  // v face
+    const int qv = t.v(i, j, k);
+    double vm = 0.0;
+    if (j >= 1) vm += mget(4, t.m(i, j - 1, k));
+    if (j <= ny - 1) vm += mget(4, t.m(i, j, k));
+    io.gm_v[qv] = 0.5 * vm;
+    const double cf = (j >= 1 ? mget(6, t.m(i, j - 1, k)) : 0.0) - (j <= ny - 1 ? mget(6, t.m(i, j, k)) : 0.0);
+    int ja, jb;
+    face_pair(j, ny, t.py, ja, jb);
+    const double thv1 = 0.5 * (tp1(t.m(i, ja, k)) + tp1(t.m(i, jb, k)));
+    io.gg_v[qv] = cf * (thv1 * t.cyp[qv]);
+    io.tv[qv] = face_frozen(j, ny, t.py) ? 0.0 : cf * (io.g_v[qv] * t.cyp[qv]);
+  }
+  if (i < nx && j < ny) {  // w level
+    const int qw = t.w(i, j, k);
+    double ge = 0.0, tz = 0.0;
+    if (k >= 1 && k <= nz - 1) {
+      const double cf = mget(7, t.m(i, j, k - 1)) - mget(7, t.m(i, j, k));
+      const double thw1 = 0.5 * (tp1(t.m(i, j, k - 1)) + tp1(t.m(i, j, k)));
+      ge = cf * (thw1 * t.czp[qw]);
+      tz = cf * (io.g_eta[qw] * t.czp[qw]);
+    }
+    io.gg_eta[qw] = ge;
+    io.tw[qw] = tz;
+  }
+}
+
+__global__ void __launch_bounds__(256) predE_kernel(TendCtx t, PredictorAdjIO io) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int N = t.nm();
+  if (p >= N) return;
+  const int i = p % t.nx, j = (p / t.nx) % t.ny, k = p / (t.nx * t.ny);
+  const int nx = t.nx, ny = t.ny, nz = t.nz;
+  const double *cm = io.cm;
+  double th = cm[p];
+  th += face_gather(i, nx, t.px, 0.5, 0.5, [&](int f) { return io.tu[t.u(f, j, k)]; });
+  th += face_gather(j, ny, t.py, 0.5, 0.5, [&](int f) { return io.tv[t.v(i, f, k)]; });
+  th += level_gather(k, nz, 0.5, 0.5, [&](int kk) { return io.tw[t.w(i, j, kk)]; });
+  io.gth_adv[p] = th;
+  io.gtth[p] = (t.alpha * t.dt) * th;
+  auto gw = [&](int kk) {
+    const double avg = 0.5 * (cm[2 * (size_t)N + t.m(i, j, kk - 1)] + cm[2 * (size_t)N + t.m(i, j, kk)]);
+    return avg * (t.rdz * t.rz[t.w(i, j, kk)]);
+  };
+  double s = cm[(size_t)N + p];
+  if (k >= 1) s += gw(k);
+  if (k + 1 <= nz - 1) s -= gw(k + 1);
+  io.gg_pi[p] = s;
+}
+
+// w_n and the guess's w enter theta at n+1 through their mass-point averages
+__global__ void __launch_bounds__(256) predF_kernel(TendCtx t, PredictorAdjIO io) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= t.nw()) return;
+  const int i = p % t.nx, j = (p / t.nx) % t.ny, k = p / (t.nx * t.ny);
+  const double adt = t.alpha * t.dt;
+  auto T = [&](int c) {
+    const int q = t.m(i, j, c);
+    return (adt * io.gth_adv[q]) * t.dthm[q];
+  };
+  double s = 0.0;
+  if (k >= 1) s += T(k - 1);
+  if (k <= t.nz - 1) s += T(k);
+  io.gw_n[p] = 0.5 * s;
+  io.gg_w[p] = -(0.5 * s);
+}
+
+// Final group (steppers.py:1003-1035): from the cotangents of u, v (after
+// divergence damping), pi and eta, the cotangent of the packed solution x.
+struct FinalAdjIO {
+  const double *gu, *gv, *gpi, *geta;  // null: zero
+  double *gdiv;                        // mass scratch
+  double *gx;                          // packed cotangent
+};
+
+__global__ void __launch_bounds__(256) finA_kernel(TendCtx t, FinalAdjIO io) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int N = t.nm();
+  if (p >= N) return;
+  const int i = p % t.nx, j = (p / t.nx) % t.ny, k = p / (t.nx * t.ny);
+  const double kd = t.nu_div * t.dt;
+  const double s =
+      face_gather(i, t.nx, t.px, -1.0, 1.0,
+                  [&](int f) { return (kd * ldz(io.gu, t.u(f, j, k))) * (t.rdx * t.mu[t.u2(f, j)]); }) +
+      face_gather(j, t.ny, t.py, -1.0, 1.0,
+                  [&](int f) { return (kd * ldz(io.gv, t.v(i, f, k))) * (t.rdy * t.mv[t.v2(i, f)]); });
+  io.gdiv[p] = s;
+}
+
+__global__ void __launch_bounds__(256) finB_kernel(TendCtx t, FinalAdjIO io) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int sx = t.nx + 1, sy = t.ny + 1;
+  if (p >= sx * sy * (t.nz + 1)) return;
+  const int i = p % sx, j = (p / sx) % sy, k = p / (sx * sy);
+  const int nx = t.nx, ny = t.ny, nz = t.nz;
+  const int Nu = t.nu(), Nv = t.nv(), Nw = t.nw(), N = t.nm();
+  if (j < ny && k < nz) {
+    const int qu = t.u(i, j, k);
+    double s = ldz(io.gu, qu);
+    if (i >= 1) s += io.gdiv[t.m(i - 1, j, k)] * (t.rdx * t.mm[t.m2(i - 1, j)]);
+    if (i <= nx - 1) s -= io.gdiv[t.m(i, j, k)] * (t.rdx * t.mm[t.m2(i, j)]);
+    io.gx[qu] = s;
+  }
+  if (i < nx && k < nz) {
+    const int qv = t.v(i, j, k);
+    double s = ldz(io.gv, qv);
+    if (j >= 1) s += io.gdiv[t.m(i, j - 1, k)] * (t.rdy * t.mm[t.m2(i, j - 1)]);
+    if (j <= ny - 1) s -= io.gdiv[t.m(i, j, k)] * (t.rdy * t.mm[t.m2(i, j)]);
+    io.gx[Nu + qv] = s;
+  }
+  if (i < nx && j < ny) {
+    const int qw = t.w(i, j, k);
+    io.gx[Nu + Nv + qw] = 0.0;  // w of the solution leaves through its own path
+    io.gx[Nu + Nv + Nw + N + qw] = ldz(io.geta, qw) / t.dzw[qw];
+    if (k < nz) {
+      const int q = t.m(i, j, k);
+      io.gx[Nu + Nv + Nw + q] = ldz(io.gpi, q) / 1000.0;
+    }
+  }
+}
+
 }  // namespace sisl
 
 extern "C" {
@@ -1163,5 +1430,60 @@ int sisl_explicit_adj_f64(const SislTendency *d, const double *u, const double *
   return SISL_OK;
 }
 
+int64_t sisl_predictor_adj_workspace_f64(const SislGrid *g) {
+  if (!g) return -1;
+  const int64_t N = (int64_t)g->nx * g->ny * g->nz;
+  const int64_t Nu = (int64_t)(g->nx + 1) * g->ny * g->nz, Nv = (int64_t)g->nx * (g->ny + 1) * g->nz;
+  const int64_t Nw = (int64_t)g->nx * g->ny * (g->nz + 1);
+  return 5 * Nu + 5 * Nv + 2 * Nw + 8 * N + Nu + Nv + Nw;
+}
+
+int sisl_predictor_adj_f64(const SislTendency *d, const double *g_pi, const double *g_u, const double *g_v,
+                           const double *g_eta, const double *m_u, const double *m_v, const double *th_next,
+                           const double *gb, const double *gth, double *grhs_u, double *grhs_v, double *grhs_w,
+                           double *grhs_pi, double *gth_adv, double *gtth, double *gw_n, double *greta,
+                           double *gg_pi, double *gg_u, double *gg_v, double *gg_w, double *gg_eta, double *gm_u,
+                           double *gm_v, double *work, void *stream) {
+  sisl::TendCtx t;
+  if (!sisl::make_ctx(d, t)) return SISL_ERR_ARG;
+  const void *req[] = {g_pi, g_u, g_v, g_eta, m_u, m_v, th_next, grhs_u, grhs_v, grhs_w, grhs_pi, gth_adv,
+                       gtth, gw_n, greta, gg_pi, gg_u, gg_v, gg_w, gg_eta, gm_u, gm_v, work};
+  for (const void *q : req)
+    if (!q) return SISL_ERR_ARG;
+  hipStream_t s = sisl::as_stream(stream);
+  const int64_t N = (int64_t)t.nx * t.ny * t.nz;
+  const int64_t Nu = (int64_t)(t.nx + 1) * t.ny * t.nz, Nv = (int64_t)t.nx * (t.ny + 1) * t.nz;
+  const int64_t Nw = (int64_t)t.nx * t.ny * (t.nz + 1);
+  double *cu = work, *cv = cu + 5 * Nu, *cw = cv + 5 * Nv, *cm = cw + 2 * Nw;
+  double *tu = cm + 8 * N, *tv = tu + Nu, *tw = tv + Nv;
+  sisl::PredictorAdjIO io{g_pi, g_u, g_v, g_eta, m_u, m_v, th_next, gb, gth, cu, cv, cw, cm, tu, tv, tw,
+                          grhs_u, grhs_v, grhs_w, grhs_pi, gth_adv, gtth, gw_n, greta, gg_pi, gg_u, gg_v, gg_w,
+                          gg_eta, gm_u, gm_v};
+  const int64_t span = (int64_t)(t.nx + 1) * (t.ny + 1) * (t.nz + 1);
+  const unsigned bs = sisl::nblocks(span), bm = sisl::nblocks(N), bw = sisl::nblocks(Nw);
+  hipLaunchKernelGGL(sisl::predA_kernel, dim3(bs), dim3(sisl::kBlock), 0, s, t, io);
+  hipLaunchKernelGGL(sisl::predB_kernel, dim3(bm), dim3(sisl::kBlock), 0, s, t, io);
+  hipLaunchKernelGGL(sisl::predD_kernel, dim3(bs), dim3(sisl::kBlock), 0, s, t, io);
+  hipLaunchKernelGGL(sisl::predE_kernel, dim3(bm), dim3(sisl::kBlock), 0, s, t, io);
+  hipLaunchKernelGGL(sisl::predF_kernel, dim3(bw), dim3(sisl::kBlock), 0, s, t, io);
+  SISL_CHECK_LAUNCH();
+  return SISL_OK;
+}
+
+int sisl_final_adj_f64(const SislTendency *d, const double *gu, const double *gv, const double *gpi,
+                       const double *geta, double *gx, double *work, void *stream) {
+  sisl::TendCtx t;
+  if (!sisl::make_ctx(d, t) || !gx || !work) return SISL_ERR_ARG;
+  hipStream_t s = sisl::as_stream(stream);
+  const int64_t N = (int64_t)t.nx * t.ny * t.nz;
+  const int64_t span = (int64_t)(t.nx + 1) * (t.ny + 1) * (t.nz + 1);
+  sisl::FinalAdjIO io{gu, gv, gpi, geta, work, gx};
+  hipLaunchKernelGGL(sisl::finA_kernel, dim3(sisl::nblocks(N)), dim3(sisl::kBlock), 0, s, t, io);
+  hipLaunchKernelGGL(sisl::finB_kernel, dim3(sisl::nblocks(span)), dim3(sisl::kBlock), 0, s, t, io);
+  SISL_CHECK_LAUNCH();
+  return SISL_OK;
+}
+
 }  // extern "C"
+
 

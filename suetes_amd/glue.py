@@ -14,14 +14,13 @@ runs the same arithmetic in five FP64 kernels:
   solve (two launches per predictor-corrector pass);
 * ``final``: divergence damping and unscaling of the solution (one launch).
 
-Reverse mode: each group is one autograd node.  The explicit stage's
-backward is a hand-written gather-form kernel set
-(``sisl_explicit_adj_f64``); the predictor-corrector and final groups
-re-evaluate the torch restatement of the same group on the saved inputs and
-differentiate it.  Both follow the reference's own rules (the stop-gradient
-on repeated edge values) exactly; with the stop-gradient switched off
-(``use_stop_grad = False`` on the operators) the explicit stage is
-differentiated through its torch restatement as well.
+Reverse mode: each group is one autograd node whose backward is a
+hand-written gather-form kernel set (``sisl_explicit_adj_f64``,
+``sisl_predictor_adj_f64``, ``sisl_final_adj_f64``) following the
+reference's own rules (the stop-gradient on repeated edge values) exactly.
+With the stop-gradient switched off (``use_stop_grad = False`` on the
+operators: the exact derivative) a group's backward re-evaluates its torch
+restatement on the saved inputs and differentiates it.
 """
 from __future__ import annotations
 
@@ -57,6 +56,10 @@ def _bind(L):
     L.sisl_explicit_adj_workspace_f64.restype = i64
     L.sisl_explicit_adj_workspace_f64.argtypes = [C.POINTER(_lib.SislGrid)]
     L.sisl_explicit_adj_f64.argtypes = [P] + [vp] * 20 + [vp]
+    L.sisl_predictor_adj_workspace_f64.restype = i64
+    L.sisl_predictor_adj_workspace_f64.argtypes = [C.POINTER(_lib.SislGrid)]
+    L.sisl_predictor_adj_f64.argtypes = [P] + [vp] * 25 + [vp]
+    L.sisl_final_adj_f64.argtypes = [P] + [vp] * 6 + [vp]
     L._glue_bound = True
     return L
 
@@ -165,6 +168,41 @@ class FusedGlue:
                                              b.data_ptr(), _lib.stream_ptr(rhs_u.device)), "predictor")
         return b, th_next
 
+    def predictor_adj(self, saved, th_next, gb, gth):
+        """Cotangents of the fifteen predictor inputs from those of (b,
+        th_next) (None for zero), at the saved inputs and th_next."""
+        ins = [t.contiguous() for t in saved]
+        g_pi, g_u, g_v, g_eta, m_u, m_v = ins[8], ins[9], ins[10], ins[12], ins[13], ins[14]
+        th_next = th_next.contiguous()
+        gb = gb.contiguous() if gb is not None else None
+        gth = gth.contiguous() if gth is not None else None
+        _lib.require_device(g_pi, g_u, g_v, g_eta, m_u, m_v, th_next, *[g for g in (gb, gth) if g is not None])
+        L = self.L
+        n = int(L.sisl_predictor_adj_workspace_f64(C.byref(self.struct.g)))
+        work = torch.empty(n, dtype=torch.float64, device=th_next.device)
+        outs = [torch.empty_like(t) for t in ins]
+        _lib.check(L.sisl_predictor_adj_f64(C.byref(self.struct), *[t.data_ptr() for t in
+                                                                    (g_pi, g_u, g_v, g_eta, m_u, m_v, th_next)],
+                                            gb.data_ptr() if gb is not None else None,
+                                            gth.data_ptr() if gth is not None else None,
+                                            *[o.data_ptr() for o in outs], work.data_ptr(),
+                                            _lib.stream_ptr(th_next.device)), "predictor adjoint")
+        return outs
+
+    def final_adj(self, gouts, like):
+        """Cotangent of the packed solution from those of (u, v, pi, eta)."""
+        gs = [g.contiguous() if g is not None else None for g in gouts]
+        for g in gs:
+            if g is not None:
+                _lib.require_device(g)
+        gx = torch.empty_like(like)
+        work = torch.empty(self.nx * self.ny * self.nz, dtype=torch.float64, device=like.device)
+        _lib.check(self.L.sisl_final_adj_f64(C.byref(self.struct), *[g.data_ptr() if g is not None else None
+                                                                     for g in gs],
+                                             gx.data_ptr(), work.data_ptr(), _lib.stream_ptr(like.device)),
+                   "final adjoint")
+        return gx
+
     def final(self, x):
         """(u, v, pi, eta) of the step's result from the packed solution x."""
         nx, ny, nz = self.nx, self.ny, self.nz
@@ -237,14 +275,22 @@ class PredictorFn(torch.autograd.Function):
     def forward(ctx, glue, *ins):
         b, th_next = glue.predictor(*ins)
         ctx.glue = glue
-        ctx.save_for_backward(*ins)
+        ctx.set_materialize_grads(False)
+        ctx.save_for_backward(*ins, th_next)
         return b, th_next
 
     @staticmethod
     def backward(ctx, *gouts):
-        g = _recompute_grads(ctx.glue.stepper._predictor_torch, ctx.saved_tensors, ctx.needs_input_grad[1:],
-                             gouts, 2)
-        return (None,) + tuple(g)
+        need = ctx.needs_input_grad[1:]
+        saved = ctx.saved_tensors
+        ins, th_next = saved[:-1], saved[-1]
+        if not any(need) or all(g is None for g in gouts):
+            return (None,) * 16
+        if not getattr(ctx.glue.stepper.physics.op, "use_stop_grad", True):
+            g = _recompute_grads(ctx.glue.stepper._predictor_torch, ins, need, gouts, 2)
+            return (None,) + tuple(g)
+        g = ctx.glue.predictor_adj(ins, th_next, gouts[0], gouts[1])
+        return (None,) + tuple(gi if n else None for gi, n in zip(g, need))
 
 
 class FinalFn(torch.autograd.Function):
@@ -254,10 +300,17 @@ class FinalFn(torch.autograd.Function):
     def forward(ctx, glue, x):
         outs = glue.final(x)
         ctx.glue = glue
+        ctx.set_materialize_grads(False)
         ctx.save_for_backward(x)
         return outs
 
     @staticmethod
     def backward(ctx, *gouts):
-        g = _recompute_grads(ctx.glue.stepper._final_torch, ctx.saved_tensors, ctx.needs_input_grad[1:], gouts, 4)
-        return (None,) + tuple(g)
+        if not ctx.needs_input_grad[1] or all(g is None for g in gouts):
+            return None, None
+        (x,) = ctx.saved_tensors
+        if not getattr(ctx.glue.stepper.physics.op, "use_stop_grad", True):
+            g = _recompute_grads(ctx.glue.stepper._final_torch, ctx.saved_tensors, ctx.needs_input_grad[1:],
+                                 gouts, 4)
+            return (None,) + tuple(g)
+        return None, ctx.glue.final_adj(gouts, x)

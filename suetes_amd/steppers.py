@@ -840,12 +840,8 @@ class SISLStepper3D:
         with one), with FP64 device state and with a background and grid
         that carry no gradient (terrain controls run the torch ops, whose
         reverse mode reaches the metrics).  A step whose state is being
-        differentiated runs the explicit stage on its fused kernels with the
-        hand-written reverse mode and the predictor-corrector and final
-        groups as torch ops (their autograd graph keeps the intermediates,
-        which costs less than re-evaluating their torch restatement in the
-        backward pass: value and gradient at 256x256x80 235 vs 240 ms mixed,
-        700 vs 707 ms FP64, measured with all three groups re-evaluated)."""
+        differentiated runs the same kernels; their reverse mode is
+        hand-written as well (csrc/tendency.hip)."""
         if not self.fused_glue or self.physics.physics_suite is not None:
             return False
         u = state["u"]
@@ -855,9 +851,6 @@ class SISLStepper3D:
             return False
         self.implicit_solver.prepare(bg)
         return not _operator_inputs(self.implicit_solver)
-
-    def _differentiated(self, state):
-        return torch.is_grad_enabled() and any(torch.is_tensor(v) and v.requires_grad for v in state.values())
 
     def _fused(self, bg):
         from .glue import FusedGlue
@@ -971,14 +964,12 @@ class SISLStepper3D:
         from .glue import ExplicitFn, FinalFn, PredictorFn
         ph, c = self.physics, self.physics.c
         glue = self._fused(bg)
-        if self._differentiated(state):
-            predictor, final = self._predictor_torch, self._final_torch
-        else:
-            def predictor(*a):
-                return PredictorFn.apply(glue, *a)
 
-            def final(x_):
-                return FinalFn.apply(glue, x_)
+        def predictor(*a):
+            return PredictorFn.apply(glue, *a)
+
+        def final(x_):
+            return FinalFn.apply(glue, x_)
         u_in, v_in, w_in, pi_in, th_in, res_n, tth, x = ExplicitFn.apply(
             glue, state["u"], state["v"], state["w"], state["pi"], state["th_v"], state["eta_dot"])
         lim, pair = self.use_limiter, self.advector.advect_cubic_pair

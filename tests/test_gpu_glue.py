@@ -238,3 +238,49 @@ def test_explicit_stage_adjoint_matches_autodiff_of_the_restatement(name, cfg):
             e = _rel(h, r)
             assert e <= 1e-12, (sel, name_i, e)
     print("  all cotangent selections within 1e-12", flush=True)
+
+
+@pytest.mark.parametrize("name,cfg", _explicit_vjp_cases(), ids=lambda v: v if isinstance(v, str) else "")
+def test_predictor_and_final_adjoints_match_autodiff_of_the_restatement(name, cfg):
+    """The hand-written reverse modes of a predictor-corrector pass and of the
+    final group (sisl_predictor_adj_f64, sisl_final_adj_f64) against torch
+    autograd through their torch restatements (_predictor_torch,
+    _final_torch; the reference's stop-gradient on repeated edge values):
+    every input cotangent rel-L2 <= 1e-12."""
+    from suetes_amd.cases import build_case
+    from suetes_amd.steppers import pack
+    print(f"[case] predictor / final adjoints vs autodiff, {name}", flush=True)
+    case = build_case(cfg, device=DEV)
+    stp = case.stepper
+    glue = stp._fused(stp.bg_precomputed())
+    s = _perturbed_state(case, 4)
+    g = torch.Generator(device="cpu").manual_seed(8)
+
+    def r(t, sc):
+        return (sc * torch.randn(t.shape, generator=g, dtype=torch.float64)).to(DEV)
+    pins = (s["u"], s["v"], s["w"], r(s["pi"], 1e-4), r(s["th_v"], 0.3), r(s["th_v"], 1e-3), s["w"],
+            r(s["w"], 1e-3), r(s["pi"], 1e-4), s["u"] + r(s["u"], 0.5), s["v"] + r(s["v"], 0.5),
+            s["w"] + r(s["w"], 0.05), r(s["w"], 1e-4), s["u"] + r(s["u"], 0.5), s["v"] + r(s["v"], 0.5))
+    leaves = [t.detach().clone().requires_grad_(True) for t in pins]
+    b, th_next = stp._predictor_torch(*leaves)
+    yb, yt = r(b, 1.0), r(th_next, 1.0)
+    for gb, gt in ((yb, yt), (yb, None), (None, yt)):
+        pairs = [(o, y) for o, y in ((b, gb), (th_next, gt)) if y is not None]
+        ref = torch.autograd.grad([o for o, _ in pairs], leaves, [y for _, y in pairs], retain_graph=True,
+                                  allow_unused=True)
+        got = glue.predictor_adj(pins, th_next.detach(), gb, gt)
+        for q, (rq, h) in enumerate(zip(ref, got)):
+            rq = torch.zeros_like(h) if rq is None else rq
+            if float(rq.norm()) == 0.0:
+                assert float(h.norm()) == 0.0, (q, gb is None, gt is None)
+                continue
+            e = _rel(h, rq)
+            assert e <= 1e-12, (q, gb is None, gt is None, e)
+    x = pack({"u": s["u"], "v": s["v"], "w": s["w"], "pi": r(s["pi"], 0.1), "eta_dot": r(s["w"], 1e-2)})
+    xl = x.detach().clone().requires_grad_(True)
+    outs = stp._final_torch(xl)
+    ys = [r(o, 1.0) for o in outs]
+    (ref,) = torch.autograd.grad(outs, [xl], ys)
+    got = glue.final_adj(ys, x)
+    assert _rel(got, ref) <= 1e-12, _rel(got, ref)
+    print("  predictor and final within 1e-12", flush=True)
