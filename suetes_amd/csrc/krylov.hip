@@ -57,6 +57,12 @@ struct Small {
 
 __device__ void decide_body(Small *sm, int nv);
 __device__ void finalize_body(Small *sm, int j, int R, int early_exit, double vs = 0.0);
+// finalize_body for restarts up to kSmallRestart with the Hessenberg column
+// and the rotations held in registers (fully unrolled, predicated on j):
+// the same arithmetic in the same order, without the private-memory round
+// trips of a dynamically indexed array
+constexpr int kSmallRestart = 16;
+__device__ void finalize_small(Small *sm, int j, int R, int early_exit, double vs);
 
 // ---- block reduction helpers -------------------------------------------------
 // One value per thread -> partials[slot][block] (fixed-order, deterministic).
@@ -136,6 +142,9 @@ struct Tail {
   double vs = 0.0;  // > 0: the update pass stored the next basis column (finalize_body)
 };
 
+// SMALL: the pass's restart is at most kSmallRestart (finalize_small; no
+// private memory in the kernel).
+template <bool SMALL>
 __global__ void __launch_bounds__(kTailThreads) tail_kernel(const double *__restrict__ partials, Tail t,
                                                             const int *__restrict__ stop,
                                                             const int *__restrict__ gate) {
@@ -155,7 +164,8 @@ __global__ void __launch_bounds__(kTailThreads) tail_kernel(const double *__rest
   if (t.mode == 1) {
     decide_body(t.sm, t.nv);
   } else {
-    finalize_body(t.sm, t.j, t.R, t.early_exit, t.vs);
+    if constexpr (SMALL) finalize_small(t.sm, t.j, t.R, t.early_exit, t.vs);
+    else finalize_body(t.sm, t.j, t.R, t.early_exit, t.vs);
     *t.next_flag = t.sm->broken;
   }
 }
@@ -617,6 +627,68 @@ __device__ void finalize_body(Small *sm, int j, int R, int early_exit, double vs
   if (early_exit && fabs(sm->g[j + 1]) <= sm->target) sm->broken = 1;
 }
 
+__device__ void finalize_small(Small *sm, int j, int R, int early_exit, double vs) {
+  constexpr int HM = kSmallRestart + 2;
+  if (sm->broken) {
+    sm->inv = 0.0;
+    return;
+  }
+  const int two = sm->gate2;
+  double h[HM], cs[HM], sn[HM];
+#pragma unroll
+  for (int v = 0; v < HM; ++v) {
+    h[v] = v <= j ? sm->h1[v] + (two ? sm->h2[v] : 0.0) : 0.0;
+    cs[v] = v < j ? sm->cs[v] : 0.0;
+    sn[v] = v < j ? sm->sn[v] : 0.0;
+  }
+  const double vn0 = sqrt(sm->h1[j + 1]);
+  const double q = vs > 0.0 ? sqrt(sm->nn1a[0]) : 0.0;
+  double vn1 = vs > 0.0 ? q * vn0 / vs : sqrt(two ? sm->nn1b[0] : sm->nn1a[0]);
+  const double thresh = 2.220446049250313e-16 * vn0;
+  if (!(vn1 > thresh)) vn1 = 0.0;
+#pragma unroll
+  for (int v = 0; v < HM; ++v)
+    if (v == j + 1) h[v] = vn1;
+  double *row = sm->H + (int64_t)j * (R + 1);
+#pragma unroll
+  for (int v = 0; v < HM; ++v)
+    if (v <= R) row[v] = v <= j + 1 ? h[v] : 0.0;
+  if (vs > 0.0) {
+    const double sc = vn1 > 0.0 ? 1.0 / q : 0.0;
+    sm->vsc[j + 1] = sc;
+    sm->inv = sc;
+  } else {
+    sm->inv = vn1 > 0.0 ? 1.0 / vn1 : 0.0;
+  }
+  sm->steps = j + 1;
+  int broken = vn1 == 0.0 ? 1 : 0;
+  // Givens rotations on a copy of the new column
+#pragma unroll
+  for (int v = 0; v < HM - 1; ++v)
+    if (v < j) {
+      const double a = cs[v] * h[v] + sn[v] * h[v + 1];
+      const double b = -sn[v] * h[v] + cs[v] * h[v + 1];
+      h[v] = a;
+      h[v + 1] = b;
+    }
+  double hj = 0.0, hj1 = 0.0;
+#pragma unroll
+  for (int v = 0; v < HM; ++v) {
+    if (v == j) hj = h[v];
+    if (v == j + 1) hj1 = h[v];
+  }
+  const double r = hypot(hj, hj1);
+  const double c = r > 0.0 ? hj / r : 1.0, s = r > 0.0 ? hj1 / r : 0.0;
+  sm->cs[j] = c;
+  sm->sn[j] = s;
+  const double gj = sm->g[j];
+  const double g1 = -s * gj;
+  sm->g[j + 1] = g1;
+  sm->g[j] = c * gj;
+  if (early_exit && fabs(g1) <= sm->target) broken = 1;
+  if (broken) sm->broken = 1;
+}
+
 // V[j+1] = vscale * w * inv (basis storage) and, when `cur` is given, the
 // unscaled working copy cur = w * inv that the next operator apply reads;
 // nothing once the cycle has stopped (those columns are outside the least
@@ -859,6 +931,14 @@ __global__ void reset_cycle_kernel(Small *sm, int R, double g0, double target, i
 
 // ---- host-side dispatch -------------------------------------------------------
 
+static void launch_tail(const double *partials, const Tail *tl, const int *stop, const int *gate, hipStream_t s) {
+  if (!tl) return;
+  if (tl->R <= kSmallRestart)
+    hipLaunchKernelGGL(tail_kernel<true>, dim3(1), dim3(kTailThreads), 0, s, partials, *tl, stop, gate);
+  else
+    hipLaunchKernelGGL(tail_kernel<false>, dim3(1), dim3(kTailThreads), 0, s, partials, *tl, stop, gate);
+}
+
 template <typename TB, typename TV>
 static void dots(const TB *V, int64_t ldv, int nv, const TV *w, int64_t n, double *partials,
                  hipStream_t s, double vinv = 1.0, const int *stop = nullptr,
@@ -888,7 +968,7 @@ static void dots(const TB *V, int64_t ldv, int nv, const TV *w, int64_t n, doubl
   else
     hipLaunchKernelGGL((dots_kernel<TB, TV>), gr, bl, 0, s, V, ldv, nv, w, n, partials, vinv, stop, gate, vsc);
   timer_end(tid, s);
-  if (tl) hipLaunchKernelGGL(tail_kernel, dim3(1), dim3(kTailThreads), 0, s, (const double *)partials, *tl, stop, gate);
+  launch_tail(partials, tl, stop, gate, s);
 }
 template <typename TB, typename TV>
 static void update(const TB *V, int64_t ldv, int nv, const double *h, TV *w, int64_t n,
@@ -908,7 +988,7 @@ static void update(const TB *V, int64_t ldv, int nv, const double *h, TV *w, int
     hipLaunchKernelGGL((update_kernel<TB, TV>), dim3(kRedGrid), dim3(kBlock), 0, s, V, ldv, nv, h, w,
                        n, partials, vinv, stop, gate, bo);
   timer_end(tid, s);
-  if (tl) hipLaunchKernelGGL(tail_kernel, dim3(1), dim3(kTailThreads), 0, s, (const double *)partials, *tl, stop, gate);
+  launch_tail(partials, tl, stop, gate, s);
 }
 template <typename TB>
 static void combine(const TB *V, int64_t ldv, int nv, const double *y, double *x, int64_t n,
