@@ -955,6 +955,8 @@ static void dots(const TB *V, int64_t ldv, int nv, const TV *w, int64_t n, doubl
   else if (nv == 2) SISL_DOTSX(2);
   else if (nv == 3) SISL_DOTSX(3);
   else if (nv == 4) SISL_DOTSX(4);
+  else if (nv == 5) SISL_DOTSX(5);  // 64 / 75 us against 82 us of the group of 8
+  else if (nv == 6) SISL_DOTSX(6);  // (7 and 8 exact: 87 / 96 us, slower)
   else if (nv == 9) SISL_DOTSX(9);
   else if (nv == 10) SISL_DOTSX(10);
   else if (nv <= kGrp)
@@ -981,7 +983,7 @@ static void update(const TB *V, int64_t ldv, int nv, const double *h, TV *w, int
   else if (nv == 2) SISL_UPDX(2);
   else if (nv == 3) SISL_UPDX(3);
   else if (nv == 4) SISL_UPDX(4);
-  else if (nv == 5) SISL_UPDX(5);
+  else if (nv == 5) SISL_UPDX(5);   // (6 to 8 exact: 88 / 100 / 105 us, slower than the group form)
   else if (nv == 9) SISL_UPDX(9);    // the two-group form would load 16 vectors
   else if (nv == 10) SISL_UPDX(10);
   else
