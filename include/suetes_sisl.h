@@ -292,7 +292,9 @@ typedef struct SislGmresParams {
                               b - A x in FP64, stores it rounded to FP32 and
                               applies M (M^T) of this operator in FP32; the
                               result is the stopping test's preconditioned
-                              residual and the next cycle's start vector      */
+                              residual and the next cycle's start vector.
+                              Batched solver: non-NULL selects the same FP32
+                              refinement with each member's FP32 operator    */
 } SislGmresParams;
 
 typedef struct SislGmresStats {

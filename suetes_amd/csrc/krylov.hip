@@ -1610,6 +1610,20 @@ __global__ void kb_residual_kernel(const KMem *__restrict__ km) {  // t64 = b64 
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < k.n; e += stride)
     k.t64[e] = k.b64[e] - k.t64[e];
 }
+// t = b64 - t64 rounded to FP32 (the FP32 refinement's residual)
+__global__ void kb_residual32_kernel(const KMem *__restrict__ km) {
+  const KMem &k = km[blockIdx.y];
+  if (k.sm->done) return;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < k.n; e += stride)
+    k.t[e] = (float)(k.b64[e] - k.t64[e]);
+}
+// ||cur||^2 partials (the FP32 refinement's preconditioned residual)
+__global__ void __launch_bounds__(256) kb_norm2_cur_kernel(const KMem *__restrict__ km) {
+  const KMem &k = km[blockIdx.y];
+  if (k.sm->done) return;
+  dots_body<float, float>(k.cur, k.n, 0, k.cur, k.n, k.partials, 1.0, nullptr, nullptr, blockIdx.x, gridDim.x);
+}
 // ||w64||^2 partials and the FP32 copy cur = w64, the next cycle's
 // unnormalised start vector (its first operator launch normalises it).
 __global__ void __launch_bounds__(256) kb_norm2_f32_kernel(const KMem *__restrict__ km) {
@@ -1727,6 +1741,7 @@ struct EnsPlan {
   BOp<double> *A64, *M64;    // outer: x64 -> t64 (skip when done); t64 -> w64
   BOp<double> *AT64, *MT64;
   BOp<double> *Mb64, *MTb64; // start: b64 -> w64 (t64 for the transpose)
+  BOp<float> *Mr32, *MTr32;  // refinement in FP32: t (FP32 residual) -> cur (skip when done)
   int *all_done;             // 1 when every member's refinement has finished
   EnsOutcome *outcome;       // the members' outcomes, gathered after a solve
   std::vector<Small *> sm;
@@ -1735,7 +1750,7 @@ struct EnsPlan {
 static std::mutex g_ens_mu;
 static std::map<const void *, EnsPlan> g_ens;
 
-constexpr int kEnsTables = 14;  // 8 FP32 inner tables (2 parities x A, M, AT, MT), 6 FP64 outer ones
+constexpr int kEnsTables = 16;  // 8 FP32 inner tables (2 parities x A, M, AT, MT), 6 FP64 outer ones, 2 FP32 refinement ones
 static int64_t ens_head_bytes(int M) {
   return align256(sizeof(KMem) * M) + kEnsTables * align256(sizeof(BOp<double>) * M) + align256(sizeof(int)) +
          align256(sizeof(EnsOutcome) * M);
@@ -1762,6 +1777,7 @@ static int ens_setup(const SislOperator *ops64, const SislOperator *ops32, int M
     mt32[q].resize(M);
   }
   std::vector<BOp<double>> a64(M), m64(M), at64(M), mt64(M), mb64(M), mtb64(M);
+  std::vector<BOp<float>> mr32(M), mtr32(M);
   int64_t off = ens_head_bytes(M);
   int S = 1;
   for (int m = 0; m < M; ++m) {
@@ -1808,6 +1824,8 @@ static int ens_setup(const SislOperator *ops64, const SislOperator *ops32, int M
     at64[m] = BOp<double>{o64, k.x64, 0, k.t64, k.scr64, dn, 0, nullptr, nullptr, 0, 1.0};
     mt64[m] = BOp<double>{o64, k.t64, 0, k.w64, k.scr64, dn, 0, nullptr, nullptr, 0, 1.0};
     mb64[m] = BOp<double>{o64, k.b64, 0, k.w64, k.scr64, nullptr, 0, nullptr, nullptr, 0, 1.0};
+    mr32[m] = BOp<float>{o32, k.t, 0, k.cur, k.scr, dn, 0, nullptr, nullptr, 0, 1.0};
+    mtr32[m] = BOp<float>{o32, k.t, 0, k.cur, k.scr, dn, 0, nullptr, nullptr, 0, 1.0};
     mtb64[m] = BOp<double>{o64, k.b64, 0, k.w64, k.scr64, nullptr, 0, nullptr, nullptr, 0, 1.0};
     P.maxA = std::max(P.maxA, blocks_A(s));
     P.maxM = std::max(P.maxM, blocks_M(s));
@@ -1822,9 +1840,11 @@ static int ens_setup(const SislOperator *ops64, const SislOperator *ops32, int M
   char *h = ws;
   P.km = (KMem *)h;
   h += align256(sizeof(KMem) * M);
-  BOp<float> **f32[] = {&P.A32[0], &P.A32[1], &P.M32[0], &P.M32[1], &P.AT32[0], &P.AT32[1], &P.MT32[0], &P.MT32[1]};
-  std::vector<BOp<float>> *v32[] = {&a32[0], &a32[1], &m32[0], &m32[1], &at32[0], &at32[1], &mt32[0], &mt32[1]};
-  for (int q = 0; q < 8; ++q) {
+  BOp<float> **f32[] = {&P.A32[0], &P.A32[1], &P.M32[0], &P.M32[1], &P.AT32[0], &P.AT32[1], &P.MT32[0], &P.MT32[1],
+                        &P.Mr32, &P.MTr32};
+  std::vector<BOp<float>> *v32[] = {&a32[0], &a32[1], &m32[0], &m32[1], &at32[0], &at32[1], &mt32[0], &mt32[1],
+                                    &mr32, &mtr32};
+  for (int q = 0; q < 10; ++q) {
     *f32[q] = (BOp<float> *)h;
     h += align256(sizeof(BOp<double>) * M);
   }
@@ -1841,7 +1861,7 @@ static int ens_setup(const SislOperator *ops64, const SislOperator *ops32, int M
   h += align256(sizeof(EnsOutcome) * M);
   if (hipMemcpyAsync(P.km, km.data(), sizeof(KMem) * M, hipMemcpyHostToDevice, st) != hipSuccess)
     return SISL_ERR_LAUNCH;
-  for (int q = 0; q < 8; ++q)
+  for (int q = 0; q < 10; ++q)
     if (hipMemcpyAsync(*f32[q], v32[q]->data(), sizeof(BOp<float>) * M, hipMemcpyHostToDevice, st) != hipSuccess)
       return SISL_ERR_LAUNCH;
   for (int q = 0; q < 6; ++q)
@@ -2040,13 +2060,25 @@ static int ens_solve(const void *ws, const SislGmresParams &p, const double *con
     } else {
       if ((rc = launch_AT_batched<double>(P.AT64, M, P.maxA, 0, st))) return rc;
     }
-    hipLaunchKernelGGL(kb_residual_kernel, dim3(kRedGrid, M), dim3(kBlock), 0, st, (const KMem *)P.km);
-    if (!p.transpose) {
-      if ((rc = launch_M_batched<double>(P.M64, M, P.maxM, P.L, P.S, 0, st))) return rc;
+    if (p.op_refine) {
+      // as the single-member solver: the residual stored in FP32, the
+      // preconditioner in FP32 into the next cycle's start vector
+      hipLaunchKernelGGL(kb_residual32_kernel, dim3(kRedGrid, M), dim3(kBlock), 0, st, (const KMem *)P.km);
+      if (!p.transpose) {
+        if ((rc = launch_M_batched<float>(P.Mr32, M, P.maxM, P.L, P.S, 0, st))) return rc;
+      } else {
+        if ((rc = launch_MT_batched<float>(P.MTr32, M, P.maxM, P.L, P.S, P.maxFaces, 0, st))) return rc;
+      }
+      hipLaunchKernelGGL(kb_norm2_cur_kernel, dim3(kRedGrid, M), dim3(kBlock), 0, st, (const KMem *)P.km);
     } else {
-      if ((rc = launch_MT_batched<double>(P.MT64, M, P.maxM, P.L, P.S, P.maxFaces, 0, st))) return rc;
+      hipLaunchKernelGGL(kb_residual_kernel, dim3(kRedGrid, M), dim3(kBlock), 0, st, (const KMem *)P.km);
+      if (!p.transpose) {
+        if ((rc = launch_M_batched<double>(P.M64, M, P.maxM, P.L, P.S, 0, st))) return rc;
+      } else {
+        if ((rc = launch_MT_batched<double>(P.MT64, M, P.maxM, P.L, P.S, P.maxFaces, 0, st))) return rc;
+      }
+      hipLaunchKernelGGL(kb_norm2_f32_kernel, dim3(kRedGrid, M), dim3(kBlock), 0, st, (const KMem *)P.km);
     }
-    hipLaunchKernelGGL(kb_norm2_f32_kernel, dim3(kRedGrid, M), dim3(kBlock), 0, st, (const KMem *)P.km);
     hipLaunchKernelGGL(kb_reduce_kernel, dim3(1, M), dim3(kBlock), 0, st, (const KMem *)P.km, 0, kSelScal0 + 2, 0, 1);
     hipLaunchKernelGGL(kb_reset_kernel, dim3(M), dim3(kBlock), 0, st, (const KMem *)P.km, R, p.inner_rtol,
                        max_outer);
